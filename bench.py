@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X IFE-PIC hot path on one GPU.
+
+Workload (default): 512 x 512 Cartesian mesh, three immersed conductors at -1, 0.5, 0,
+16,777,216 electrons + 16,777,216 Ar+ ions (64 per cell per species), FP32 gather /
+push / MCC with fixed-point (int64) deposition, FP32-inner / FP64-refined Jacobi-PCG to
+1e-8, PPR recovery, null-collision MCC against 300 K argon at 3.2e20 m^-3, periodic
+cell sort. A step is one full explicit PIC step (MoveAll x2, GetParChg_2D,
+IFE_Solve_2D, GetEField_SIDG_PPR, MCC, sort every 20 steps) with every input resident in
+HBM. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-pushes/s + PCG ms/solve @1e-8 rel residual, learned vs plain, 1 GPU"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(name: str):
+    """HBM bytes per launch of the push kernel from the committed rocprofv3 PMC summary."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(pkg, pl, w, sample_particles: int) -> dict:
+    """One full explicit step of the CPU oracle (scalar C, 1 core) on a bounded sample of the
+    same workload state: MoveAll + lookup + deposition for `sample_particles` particles per
+    species, and the field half of the step (RHS, MY_PCG to 1e-8 in FP64, PPR) on the full
+    512^2 mesh. Scaled to particle-pushes/s of the whole workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    dom = w["domain"]
+    hp = pl.setup_array("HP")
+    xs = hp[0::2][:: dom.nny].copy()
+    ys = hp[1::2][: dom.nny].copy()
+    ei, ni = pl.setup_array("ELEMENT_INDEX"), pl.setup_array("NODE_INDEX")
+    efx, efy = pl.field("EFX"), pl.field("EFY")
+    coef = pl.field("COEF_PPR").reshape(-1, 6)
+    norm = dom.normalisation()
+    n_total = 0
+    t_part = 0.0
+    for isp, sp in enumerate((pkg.ELECTRON, pkg.ARGON_ION)):
+        p = pl.particles(isp)
+        n_total += len(p["x"])
+        k = min(sample_particles, len(p["x"]))
+        qm = (sp["charge_C"] / pkg.ELECTRON_CHARGE) / (sp["mass_kg"] / pkg.ELECTRON_MASS)
+        t0 = time.perf_counter()
+        x, y, vx, vy, vz, alive, cnt, err = orc.move_all(
+            64, xs, ys, ei, ni, dom.object_array(), efx, efy, coef[:, 1:].ravel(), qm, dom.dt, norm["v_ref"],
+            sp["mass_kg"], 2.0 * pkg.EVTOK, 1, 0, orc.tag(isp, orc.RNG_EMIT),
+            p["x"][:k], p["y"][:k], p["vx"][:k], p["vy"][:k], p["vz"][:k], p["loc"][:k] - 1)
+        keep = alive.astype(bool)
+        loc, _ = orc.locate(xs, ys, x[keep], y[keep])
+        orc.deposit(64, xs, ys, 0, x[keep], y[keep], vx[keep], vy[keep], vz[keep], loc, sp["mass_kg"], norm["v_ref"], True)
+        t_part += (time.perf_counter() - t0) * (len(p["x"]) / k)
+    # field half on the full mesh
+    nt = pl.setup_array("NODE_TYPE").reshape(-1, 2)[:, 1]
+    rho = pl.field("RHO")
+    phi = pl.field("PHI")
+    rp, col, val, diag = (pl.setup_array(n) for n in ("A_ROWPTR", "A_COL", "A_VAL", "A_DIAG"))
+    un = pl.setup_array("UNKNOWN_NODE") - 1
+    t0 = time.perf_counter()
+    b = orc.rhs(xs, ys, 0, nt, pl.setup_array("DIRICHLET_PHI"), rho, len(diag))
+    xsol, _, it, _ = orc.pcg(rp, col, val, diag, b, phi[un], 1e-8, 50000)
+    phi2 = phi.copy()
+    phi2[un] = xsol
+    orc.ppr(xs, ys, pl.setup_array("POINT_PATCH"), phi2)
+    t_field = time.perf_counter() - t0
+    t_step = t_part + t_field
+    return {"value": n_total / t_step, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
+            "sample": f"oracle.c scalar FP64 step: mover+lookup+deposit timed on {sample_particles} particles per "
+                      f"species and scaled to {n_total}; RHS + MY_PCG (1e-8, {it} it) + PPR on the full mesh; "
+                      f"{t_step:.2f} s per step"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); the sharded design is "
+              "described in DESIGN.md", file=sys.stderr)
+        return 2
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    pl = pkg.PicLibrary(0)
+    log(f"bench: setting up {args.workload}")
+    t0 = time.perf_counter()
+    w = pkg.build_workload(pl, args.workload, seed=2024)
+    pl.initial_field()
+    log(f"bench: set-up {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps")
+    pl.step(args.warmup)
+    n0 = pl.counts(0)["nlive"] + pl.counts(1)["nlive"]
+    log(f"bench: timing {args.steps} steps, {n0} particles")
+    t0 = time.perf_counter()
+    pl.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    phase = pl.phase_times()          # device ms summed over the timed steps
+    n1 = pl.counts(0)["nlive"] + pl.counts(1)["nlive"]
+    pushes = 0.5 * (n0 + n1) * args.steps
+    value = pushes / elapsed
+    solve = pl.last_solve()
+    # plain PCG ms/solve at 1e-8 on the step's RHS (device time of the PCG phase)
+    pcg_ms = phase[3] / args.steps
+    # roofline of the dominant kernel: fused gather/push/BC/lookup/deposit, 2 launches per step
+    nsp = 2
+    push_ms = phase[0] / (args.steps * nsp)
+    bytes_per_particle = 44.0     # read x,y,vx,vy,vz,cell (24 B) + write x,y,vx,vy,cell_next (20 B), FP32
+    alg_bytes = bytes_per_particle * (0.5 * (n0 + n1)) / nsp
+    achieved = alg_bytes / (push_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles("traffic_push_cfg3.md")
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-pushes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: 512x512 Cartesian, 3 conductors, 2x16.8M particles, "
+                               "FP32 particles + FP32/FP64 PCG to 1e-8, MCC, sort/20",
+                   "mesh": [w["domain"].nnx - 1, w["domain"].nny - 1], "particles_per_species": int(n0 // 2),
+                   "parallelism": "single GPU"},
+        "pcg_ms_per_solve": pcg_ms, "pcg_iterations": solve["iterations"], "pcg_outer": solve["outer"],
+        "pcg_relres": solve["relres"],
+        "phase_ms_per_step": {k: phase[i] / args.steps for i, k in enumerate(
+            ["push_lookup_deposit", "deposit_finalize", "rhs", "pcg", "ppr", "mcc", "sort", "total"])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_push_deposit<float,1>", "bytes_per_launch": alg_bytes, "ms_per_launch": push_ms},
+    }
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle, 1 core)")
+        try:
+            out["cpu_baseline"] = cpu_baseline(pkg, pl, w, args.cpu_sample)
+        except Exception as e:  # the baseline is reported, not the product
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

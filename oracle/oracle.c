@@ -1,0 +1,657 @@
+/*
+ * oracle.c -- CPU restatement of the PIC-IFE_GEC per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This file is the checker the GPU library is compared
+ * against; it is loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+ * leg of bench.py, and never by the library itself.
+ *
+ * Each function restates one reference routine (paths relative to the reference
+ * checkout) in plain C, in the reference's operation order:
+ *   or_philox / or_stream      Philox4x32-10 (Salmon et al. SC'11), pinned by the
+ *                              published known-answer vectors
+ *   or_move_all_f64 / _f32     MoveAll + MoveOne + Get_ParticleEField_PPR +
+ *                              AdjustParticleBoundary (MCCInterface.f90:217-640,
+ *                              IMPIC/MoveOne.f90, PPR/Get_ParticleEField_PPR.f90)
+ *   or_locate                  GetParChg_2D.f90:137-163 + InjectPositioning.f90 +
+ *                              ParticlePositioning.f90 (unrefined mesh)
+ *   or_deposit                 GetParChg_2D.f90:122-911, two accumulations: the
+ *                              reference's FP64 sequential sums, and the int64 fixed
+ *                              point the GPU uses (bit-identical target)
+ *   or_ppr                     GetEfield_SIDG_PPR.f90:153-304 + BRINV.f90 (per-node
+ *                              least squares redone from the point patches)
+ *   or_pcg                     MY_PCG.f90 + MATVEC_1.f90 + MSOLVE_1.f90
+ *   or_assemble                FE stiffness / RHS by 2x2 Gauss quadrature
+ *                              (Gauss_integration_local_stiffness, FE_RHS_HW_2D) on
+ *                              the uniform Q1 mesh
+ *   or_mcc_*                   MCCPublic.F90, TypeMCC.f90, MCCElectron.F90, MCCIon.F90,
+ *                              MCCChi.f90 with the counter-based selection of the build
+ * Parity status of each function is listed in DESIGN.md.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ Philox */
+typedef struct { uint32_t v[4]; } or_p4;
+
+or_p4 or_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    or_p4 o = {{c0, c1, c2, c3}};
+    return o;
+}
+
+void or_philox_block(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    or_p4 o = or_philox(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    memcpy(out, o.v, 16);
+}
+
+typedef struct { uint32_t k0, k1, c0, c1, c2, blk; uint32_t buf[4]; int used; } or_stream;
+
+static void or_stream_init(or_stream* s, uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag) {
+    s->k0 = (uint32_t)seed; s->k1 = (uint32_t)(seed >> 32);
+    s->c0 = slot; s->c1 = step; s->c2 = tag; s->blk = 0; s->used = 4;
+}
+static uint32_t or_u32(or_stream* s) {
+    if (s->used == 4) {
+        or_p4 o = or_philox(s->c0, s->c1, s->c2, s->blk, s->k0, s->k1);
+        memcpy(s->buf, o.v, 16);
+        s->blk++; s->used = 0;
+    }
+    return s->buf[s->used++];
+}
+/* 52 random bits centred in their bin: a double in the open interval (0,1) */
+static double or_u01(or_stream* s) {
+    uint32_t a = or_u32(s), b = or_u32(s);
+    uint64_t m = ((uint64_t)a << 20) | (uint64_t)(b >> 12);
+    return ((double)m + 0.5) * (1.0 / 4503599627370496.0);
+}
+static uint32_t or_tag(int species, int purpose) { return ((uint32_t)purpose << 16) | (uint32_t)(species & 0xFFFF); }
+enum { OR_RNG_LOAD = 1, OR_RNG_EMIT = 2, OR_RNG_SELECT = 3, OR_RNG_COLLIDE = 4 };
+
+/* ------------------------------------------------------------------ mesh */
+typedef struct { int shape; double x0, y0, x1, y1, r2; } or_object;
+
+typedef struct {
+    int nnx, nny, nex, ney, nnodes, nelem;
+    double xmin, xmax, ymin, ymax, hx, hy;
+    const double* xs; const double* ys;   /* node coordinate lines (Cubic_Partition_2D x(i), y(j)) */
+    const int* element_index;             /* nelem */
+    const int* node_index;                /* nnodes */
+    int n_objects;
+    or_object obj[8];
+} or_mesh;
+
+/* builds an or_mesh from flat arguments (ctypes friendly) */
+static void mk_mesh(or_mesh* m, int nnx, int nny, const double* xs, const double* ys,
+                    const int* element_index, const int* node_index, int n_objects, const double* objects /* 6 per object */) {
+    memset(m, 0, sizeof(*m));
+    m->nnx = nnx; m->nny = nny; m->nex = nnx - 1; m->ney = nny - 1;
+    m->nnodes = nnx * nny; m->nelem = (nnx - 1) * (nny - 1);
+    m->xs = xs; m->ys = ys;
+    m->xmin = xs[0]; m->xmax = xs[nnx - 1]; m->ymin = ys[0]; m->ymax = ys[nny - 1];
+    m->hx = (m->xmax - m->xmin) / (double)(nnx - 1);
+    m->hy = (m->ymax - m->ymin) / (double)(nny - 1);
+    m->element_index = element_index; m->node_index = node_index;
+    m->n_objects = n_objects;
+    for (int k = 0; k < n_objects && k < 8; ++k) {
+        const double* o = objects + 6 * k;
+        m->obj[k].shape = (int)o[0];
+        m->obj[k].x0 = o[1]; m->obj[k].y0 = o[2]; m->obj[k].x1 = o[3]; m->obj[k].y1 = o[4]; m->obj[k].r2 = o[5];
+    }
+}
+
+/* ------------------------------------------------------------------ MoveAll */
+#define REAL double
+#define FN(name) or_##name##_f64
+#include "oracle_push.inc"
+#undef REAL
+#undef FN
+#define REAL float
+#define FN(name) or_##name##_f32
+#include "oracle_push.inc"
+#undef REAL
+#undef FN
+
+int or_move_all(int bits, int nnx, int nny, const double* xs, const double* ys, const int* element_index,
+                const int* node_index, int n_objects, const double* objects,
+                const void* efx, const void* efy, const void* coef5, double qm, double dt,
+                double v_ref, double mass, double init_temp_K, uint64_t seed, uint32_t step, uint32_t tag_emit,
+                int64_t n, void* x, void* y, void* vx, void* vy, void* vz, const int* cell, int* alive,
+                int64_t* counters) {
+    or_mesh m;
+    mk_mesh(&m, nnx, nny, xs, ys, element_index, node_index, n_objects, objects);
+    int err = 0;
+    if (bits == 64) {
+        or_move_all_f64(&m, (const double*)efx, (const double*)efy, (const double*)coef5, xs, ys, qm, dt, v_ref, mass,
+                        init_temp_K, seed, step, tag_emit, n, (double*)x, (double*)y, (double*)vx, (double*)vy,
+                        (double*)vz, cell, alive, counters, &err);
+    } else {
+        float* xsf = (float*)malloc(sizeof(float) * nnx);
+        float* ysf = (float*)malloc(sizeof(float) * nny);
+        for (int i = 0; i < nnx; ++i) xsf[i] = (float)xs[i];
+        for (int j = 0; j < nny; ++j) ysf[j] = (float)ys[j];
+        or_move_all_f32(&m, (const float*)efx, (const float*)efy, (const float*)coef5, xsf, ysf, (float)qm, (float)dt,
+                        v_ref, mass, init_temp_K, seed, step, tag_emit, n, (float*)x, (float*)y, (float*)vx,
+                        (float*)vy, (float*)vz, cell, alive, counters, &err);
+        free(xsf); free(ysf);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------ lookup */
+/* GetParChg_2D.f90:137-163 with Vert_o = min - h (Input_2D.f90:76-78), hxi = 1/hx;
+ * MOD(rxp, Mapxmin) as the exact remainder; InjectPositioning.f90:13-22 (division);
+ * ParticlePositioning on the unrefined mesh returns the estimate. 0-based result. */
+static int locate1(const or_mesh* m, double X, double Y, int* bad) {
+    double hxi = 1.0 / m->hx, hyi = 1.0 / m->hy;
+    double vox = m->xmin - m->hx, voy = m->ymin - m->hy;
+    double rxp = (X - vox) * hxi;
+    int i = (int)rxp;
+    double ryp = (Y - voy) * hyi;
+    int j = (int)ryp;
+    if (j == m->nny) j = j - 1;
+    if (i == m->nnx) i = i - 1;
+    double mapx = m->hx / 1.0, mapy = m->hy / 1.0;     /* hx(1) / 2**repeat_refinement */
+    if (fmod(rxp, mapx) > 1e-10 && fmod(ryp, mapy) > 1e-10) {
+        double rx = (X - m->xmin) / mapx;
+        i = (int)rx + 1;
+        double ry = (Y - m->ymin) / mapy;
+        j = (int)ry + 1;
+    }
+    if (i < 1 || i > m->nex || j < 1 || j > m->ney) {
+        *bad = 1;
+        i = i < 1 ? 1 : (i > m->nex ? m->nex : i);
+        j = j < 1 ? 1 : (j > m->ney ? m->ney : j);
+    }
+    return (j - 1) + (i - 1) * m->ney;
+}
+
+int or_locate(int nnx, int nny, const double* xs, const double* ys, int64_t n, const double* x, const double* y, int* out) {
+    or_mesh m;
+    mk_mesh(&m, nnx, nny, xs, ys, NULL, NULL, 0, NULL);
+    int bad = 0;
+    for (int64_t p = 0; p < n; ++p) out[p] = locate1(&m, x[p], y[p], &bad);
+    return bad;
+}
+
+/* ------------------------------------------------------------------ deposition */
+/* GetParChg_2D.f90:122-911 for one species. Weights of :788-821, moments W and W*Energy
+ * (Energy = 0.5*Mass*|v|^2*v_ref^2/JtoeV, ParticleOne.f90:333-341, :128) accumulated to
+ * element corners (:823-843), corners summed to nodes in element order (:849-866).
+ * fix != 0: every term quantised to round(term*2^40) and summed as int64 (the build's
+ * accumulation, order independent); fix == 0: the reference's FP64 sums in particle
+ * order. Output node sums w_node, e_node (FP64), before normalisation. */
+int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, int delta, int64_t n,
+               const void* x, const void* y, const void* vx, const void* vy, const void* vz, const int* cell,
+               double mass, double v_ref, int fix, double* w_node, double* e_node) {
+    const double JtoeV = 1.6022e-19;
+    int nex = nnx - 1, ney = nny - 1, nelem = nex * ney, nnodes = nnx * nny;
+    double ymin = ys[0];
+    double* rs = (double*)calloc((size_t)4 * nelem, sizeof(double));
+    double* ek = (double*)calloc((size_t)4 * nelem, sizeof(double));
+    long long* qs = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
+    long long* qe = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
+    for (int64_t p = 0; p < n; ++p) {
+        int e = cell[p];
+        if (e < 0) continue;
+        double X, Y, VX, VY, VZ;
+        if (bits == 64) {
+            X = ((const double*)x)[p]; Y = ((const double*)y)[p];
+            VX = ((const double*)vx)[p]; VY = ((const double*)vy)[p]; VZ = ((const double*)vz)[p];
+        } else {
+            X = ((const float*)x)[p]; Y = ((const float*)y)[p];
+            VX = ((const float*)vx)[p]; VY = ((const float*)vy)[p]; VZ = ((const float*)vz)[p];
+        }
+        double Energy = 0.5 * mass * (VX * VX + VY * VY + VZ * VZ);
+        Energy = Energy * v_ref * v_ref;
+        Energy = Energy / JtoeV;
+        int ci = e / ney, cj = e % ney;
+        double hxp = xs[ci + 1] - xs[ci], hyp = ys[cj + 1] - ys[cj];
+        double dx = (X - xs[ci]) / hxp, dy = (Y - ys[cj]) / hyp;
+        double xm = 1.0 - dx, ym = 1.0 - dy;
+        double W[4];
+        if (delta == 0) { W[0] = xm * ym; W[1] = dx * ym; W[2] = dx * dy; W[3] = xm * dy; }
+        else {
+            double BETA = (fabs(ys[cj] - ymin) < 1e-12) ? 0.75 : 1.0;
+            double R1 = ymin + ys[cj], R2 = ymin + ys[cj + 1], R = Y, den = R2 * R2 - R1 * R1;
+            W[0] = xm * (R2 * R2 - R * R) / den * BETA; W[1] = dx * (R2 * R2 - R * R) / den * BETA;
+            W[2] = dx * (R * R - R1 * R1) / den; W[3] = xm * (R * R - R1 * R1) / den;
+        }
+        for (int k = 0; k < 4; ++k) {
+            if (fix) {
+                qs[4 * (size_t)e + k] += llrint(W[k] * 1099511627776.0);
+                qe[4 * (size_t)e + k] += llrint(W[k] * Energy * 1099511627776.0);
+            } else {
+                rs[4 * (size_t)e + k] = rs[4 * (size_t)e + k] + W[k];
+                ek[4 * (size_t)e + k] = ek[4 * (size_t)e + k] + W[k] * Energy;
+            }
+        }
+    }
+    if (fix) {
+        long long* nw = (long long*)calloc(nnodes, sizeof(long long));
+        long long* ne = (long long*)calloc(nnodes, sizeof(long long));
+        for (int e = 0; e < nelem; ++e) {
+            int ci = e / ney, cj = e % ney;
+            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+            for (int k = 0; k < 4; ++k) { nw[ht[k]] += qs[4 * (size_t)e + k]; ne[ht[k]] += qe[4 * (size_t)e + k]; }
+        }
+        for (int i = 0; i < nnodes; ++i) {
+            w_node[i] = (double)nw[i] * (1.0 / 1099511627776.0);
+            e_node[i] = (double)ne[i] * (1.0 / 1099511627776.0);
+        }
+        free(nw); free(ne);
+    } else {
+        for (int i = 0; i < nnodes; ++i) { w_node[i] = 0.0; e_node[i] = 0.0; }
+        for (int e = 0; e < nelem; ++e) {
+            int ci = e / ney, cj = e % ney;
+            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+            for (int k = 0; k < 4; ++k) {
+                w_node[ht[k]] = w_node[ht[k]] + rs[4 * (size_t)e + k];
+                e_node[ht[k]] = e_node[ht[k]] + ek[4 * (size_t)e + k];
+            }
+        }
+    }
+    free(rs); free(ek); free(qs); free(qe);
+    return 0;
+}
+
+/* Normalisation of GetParChg_2D.f90:869-911: rho_s = Weight*qs*sum/L_ref/L_ref/n_ref/CellVol
+ * (Cartesian), Ek_tot = Weight*sum_E/..., rho += rho_s over species. */
+void or_normalize(int nnodes, int delta, double weight, double qs_norm, double L_ref, double n_ref,
+                  const double* vol, const double* w_node, const double* e_node, double* rho_s, double* ek, double* rho) {
+    for (int i = 0; i < nnodes; ++i) {
+        if (delta == 0) {
+            rho_s[i] = weight * qs_norm * w_node[i] / L_ref / L_ref / n_ref / vol[i];
+            ek[i] = weight * e_node[i] / L_ref / L_ref / n_ref / vol[i];
+        } else {
+            rho_s[i] = weight * qs_norm * w_node[i] / L_ref / L_ref / L_ref / n_ref / vol[i];
+            ek[i] = weight * e_node[i] / L_ref / L_ref / L_ref / n_ref / vol[i];
+        }
+        rho[i] = rho[i] + rho_s[i];
+    }
+}
+
+/* ------------------------------------------------------------------ PPR */
+/* BRINV (PPR/BRINV.f90:1-72): full-pivot Gauss-Jordan inverse in place. Returns 0 if
+ * singular (the reference prints ERR**NOT INV). */
+int or_brinv(double* a, int n) {
+    int is[16], js[16];
+    for (int k = 0; k < n; ++k) {
+        double d = 0.0;
+        for (int i = k; i < n; ++i)
+            for (int j = k; j < n; ++j) {
+                double p = fabs(a[i * n + j]);
+                if (p > d) { d = p; is[k] = i; js[k] = j; }
+            }
+        if (d + 1.0 == 1.0) return 0;
+        if (is[k] != k) for (int j = 0; j < n; ++j) { double t = a[k * n + j]; a[k * n + j] = a[is[k] * n + j]; a[is[k] * n + j] = t; }
+        if (js[k] != k) for (int i = 0; i < n; ++i) { double t = a[i * n + k]; a[i * n + k] = a[i * n + js[k]]; a[i * n + js[k]] = t; }
+        a[k * n + k] = 1.0 / a[k * n + k];
+        for (int j = 0; j < n; ++j) if (j != k) a[k * n + j] = a[k * n + j] * a[k * n + k];
+        for (int i = 0; i < n; ++i)
+            if (i != k)
+                for (int j = 0; j < n; ++j)
+                    if (j != k) a[i * n + j] = a[i * n + j] - a[i * n + k] * a[k * n + j];
+        for (int i = 0; i < n; ++i) if (i != k) a[i * n + k] = -a[i * n + k] * a[k * n + k];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        if (js[k] != k) for (int j = 0; j < n; ++j) { double t = a[k * n + j]; a[k * n + j] = a[js[k] * n + j]; a[js[k] * n + j] = t; }
+        if (is[k] != k) for (int i = 0; i < n; ++i) { double t = a[i * n + k]; a[i * n + k] = a[i * n + is[k]]; a[i * n + is[k]] = t; }
+    }
+    return 1;
+}
+
+/* GetEfield_SIDG_PPR.f90:153-241, 301-304. point_patch is the reference's
+ * point_patch(nnodes, 50) in Fortran order (count in column 1, 1-based node ids).
+ * coef (6 x nnodes), efx, efy out. Returns the number of singular patches. */
+int or_ppr(int nnx, int nny, const double* xs, const double* ys, const int* point_patch, const double* phi,
+           double* coef, double* efx, double* efy) {
+    int nnodes = nnx * nny, nsing = 0;
+    double pnts[2][64], AT[6][64], A_MUL[36], ATn[6][64];
+    for (int i = 0; i < nnodes; ++i) {
+        int cnt = point_patch[i];
+        for (int r = 0; r < 6; ++r) coef[6 * (size_t)i + r] = 0.0;
+        efx[i] = 0.0; efy[i] = 0.0;
+        if (cnt == 0) continue;
+        double hxi_ = xs[i / nny], hyi_ = ys[i % nny];
+        double h = 0, h_temp;
+        double phv[64];
+        for (int j = 0; j < cnt; ++j) {
+            int k = point_patch[(size_t)(j + 1) * nnodes + i] - 1;
+            pnts[0][j] = xs[k / nny]; pnts[1][j] = ys[k % nny];
+            phv[j] = phi[k];
+            h_temp = fmax(fabs(pnts[0][j] - hxi_), fabs(pnts[1][j] - hyi_));
+            if (h_temp > h) h = h_temp;
+        }
+        for (int j = 0; j < cnt; ++j) { pnts[0][j] = (pnts[0][j] - hxi_) / h; pnts[1][j] = (pnts[1][j] - hyi_) / h; }
+        for (int j = 0; j < cnt; ++j) {
+            AT[0][j] = 1.0; AT[1][j] = pnts[0][j]; AT[2][j] = pnts[1][j];
+            AT[3][j] = pnts[0][j] * pnts[0][j]; AT[4][j] = pnts[0][j] * pnts[1][j]; AT[5][j] = pnts[1][j] * pnts[1][j];
+        }
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) {
+                double s = 0.0;
+                for (int j = 0; j < cnt; ++j) s += AT[a][j] * AT[b][j];
+                A_MUL[a * 6 + b] = s;
+            }
+        if (!or_brinv(A_MUL, 6)) { nsing++; continue; }
+        for (int a = 0; a < 6; ++a)
+            for (int j = 0; j < cnt; ++j) {
+                double s = 0.0;
+                for (int b = 0; b < 6; ++b) s += A_MUL[a * 6 + b] * AT[b][j];
+                ATn[a][j] = s;
+            }
+        double t[6];
+        for (int a = 0; a < 6; ++a) {
+            double s = 0.0;
+            for (int j = 0; j < cnt; ++j) s += ATn[a][j] * phv[j];
+            t[a] = s;
+        }
+        coef[6 * (size_t)i + 0] = t[0];
+        coef[6 * (size_t)i + 1] = t[1] / h; coef[6 * (size_t)i + 2] = t[2] / h;
+        coef[6 * (size_t)i + 3] = t[3] / (h * h); coef[6 * (size_t)i + 4] = t[4] / (h * h); coef[6 * (size_t)i + 5] = t[5] / (h * h);
+        efx[i] = -coef[6 * (size_t)i + 1];
+        efy[i] = -coef[6 * (size_t)i + 2];
+    }
+    return nsing;
+}
+
+/* Get_ParticleEField_PPR.f90:1-50 (returns grad phi; callers negate). e 0-based. */
+void or_ppr_gather(int nnx, int nny, const double* xs, const double* ys, const int* element_index, const int* node_index,
+                   const double* coef, int64_t n, const double* x, const double* y, const int* ele, double* ex, double* ey) {
+    int ney = nny - 1;
+    for (int64_t p = 0; p < n; ++p) {
+        int e = ele[p];
+        int ci = e / ney, cj = e % ney;
+        int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+        double hx_[4] = {xs[ci], xs[ci + 1], xs[ci + 1], xs[ci]}, hy_[4] = {ys[cj], ys[cj], ys[cj + 1], ys[cj + 1]};
+        double Ex = 0, Ey = 0;
+        int count = 0, iface = element_index[e] > 0;
+        for (int i = 0; i < 4; ++i) {
+            if (iface && node_index[ht[i]] != -1) continue;
+            const double* c = coef + 6 * (size_t)ht[i];
+            count = count + 1;
+            double x_ref = x[p] - hx_[i], y_ref = y[p] - hy_[i];
+            Ex = Ex + c[1] + 2. * c[3] * x_ref + c[4] * y_ref;
+            Ey = Ey + c[2] + 2. * c[5] * y_ref + c[4] * x_ref;
+        }
+        if (iface) { Ex = Ex / count; Ey = Ey / count; } else { Ex = Ex / 4; Ey = Ey / 4; }
+        ex[p] = Ex; ey[p] = Ey;
+    }
+}
+
+/* ------------------------------------------------------------------ MY_PCG */
+/* MY_PCG.f90:49-141 with MATVEC_1 (CSR row sums) and MSOLVE_1 (X = Y / diag), 0-based CSR.
+ * b: RHS (in) / final residual (out); x: guess (in) / solution (out).
+ * Returns IERR (0 ok, 1 max iterations, 2 rho0 == 0, 3 p.Ap == 0); *iter = i-1. */
+int or_pcg(int n, const int* rowptr, const int* col, const double* val, const double* diag,
+           double* b, double* x, double reltol, int maxiter, int* iter) {
+    double *X0 = malloc(sizeof(double) * n), *R0 = malloc(sizeof(double) * n), *W = malloc(sizeof(double) * n);
+    double *P = malloc(sizeof(double) * n), *Q = malloc(sizeof(double) * n);
+    double alpha, beta, rho = 0, rho0 = 0, normR, normRHS = 0, RELTOL2 = reltol * reltol;
+    int i, j, ierr = 0;
+    for (j = 0; j < n; ++j) normRHS = normRHS + b[j] * b[j];
+    for (j = 0; j < n; ++j) X0[j] = x[j];
+    for (j = 0; j < n; ++j) { double s = 0; for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * X0[col[k]]; R0[j] = s; }
+    for (j = 0; j < n; ++j) R0[j] = b[j] - R0[j];
+    for (i = 1; i <= maxiter; ++i) {
+        for (j = 0; j < n; ++j) W[j] = R0[j] / diag[j];
+        rho = 0;
+        for (j = 0; j < n; ++j) rho = rho + R0[j] * W[j];
+        if (i == 1) { for (j = 0; j < n; ++j) P[j] = W[j]; }
+        else {
+            if (rho0 == 0) { ierr = 2; break; }
+            beta = rho / rho0;
+            for (j = 0; j < n; ++j) P[j] = W[j] + beta * P[j];
+        }
+        for (j = 0; j < n; ++j) { double s = 0; for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * P[col[k]]; Q[j] = s; }
+        alpha = 0;
+        for (j = 0; j < n; ++j) alpha = alpha + P[j] * Q[j];
+        if (alpha == 0) { ierr = 3; break; }
+        alpha = rho / alpha;
+        for (j = 0; j < n; ++j) { X0[j] = X0[j] + alpha * P[j]; R0[j] = R0[j] - alpha * Q[j]; }
+        rho0 = rho;
+        normR = 0;
+        for (j = 0; j < n; ++j) normR = normR + R0[j] * R0[j];
+        if (normR / normRHS <= RELTOL2) break;
+    }
+    if (i >= maxiter) ierr = 1;      /* MY_PCG.f90:129-133: ITER left unset */
+    else *iter = i - 1;
+    for (j = 0; j < n; ++j) { x[j] = X0[j]; b[j] = R0[j]; }
+    free(X0); free(R0); free(W); free(P); free(Q);
+    return ierr;
+}
+
+/* ------------------------------------------------------------------ assembly */
+/* Q1 element integrals by 2x2 Gauss (Generate_Gauss_reference with 4 points), the way
+ * Gauss_integration_local_stiffness and FE_RHS_HW_2D evaluate them: K_ab = int grad N_a.
+ * grad N_b, M_ab = int N_a N_b (times r when axisymmetric). Dense element matrices for an
+ * element with lower-left corner (x0,y0), sizes hx, hy; node order LL, LR, UR, UL. */
+void or_element_matrices(int delta, double x0, double y0, double hx, double hy, double* K, double* M) {
+    const double g = 1.0 / sqrt(3.0);
+    const double gp[2] = {-g, g};
+    for (int a = 0; a < 16; ++a) { K[a] = 0; M[a] = 0; }
+    for (int qi = 0; qi < 2; ++qi)
+        for (int qj = 0; qj < 2; ++qj) {
+            double xi = gp[qi], et = gp[qj];
+            double wq = (hx / 2.0) * (hy / 2.0);
+            double yq = y0 + (et + 1.0) * hy / 2.0;
+            if (delta == 1) wq *= yq;
+            double N[4] = {0.25 * (1 - xi) * (1 - et), 0.25 * (1 + xi) * (1 - et), 0.25 * (1 + xi) * (1 + et), 0.25 * (1 - xi) * (1 + et)};
+            double dNxi[4] = {-0.25 * (1 - et), 0.25 * (1 - et), 0.25 * (1 + et), -0.25 * (1 + et)};
+            double dNet[4] = {-0.25 * (1 - xi), -0.25 * (1 + xi), 0.25 * (1 + xi), 0.25 * (1 - xi)};
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) {
+                    double gx = (dNxi[a] * 2.0 / hx) * (dNxi[b] * 2.0 / hx);
+                    double gy = (dNet[a] * 2.0 / hy) * (dNet[b] * 2.0 / hy);
+                    K[a * 4 + b] += wq * (gx + gy);
+                    M[a * 4 + b] += wq * N[a] * N[b];
+                }
+        }
+    (void)x0;
+}
+
+/* rhs_i = sum_e int N_i rho_h - (Dirichlet lift) over unknown rows; node_type2: unknown id
+ * (1-based) or <= 0 for Dirichlet nodes with values dphi. Global_RHS_PDE_2D.f90:78-140 +
+ * Global_RHS_EBC_2D_bjw.f90:66-78 restated by element loops. */
+void or_rhs(int nnx, int nny, const double* xs, const double* ys, int delta, const int* node_type2,
+            const double* dphi, const double* rho, int nunk, double* rhs) {
+    int nex = nnx - 1, ney = nny - 1;
+    double K[16], M[16];
+    for (int u = 0; u < nunk; ++u) rhs[u] = 0.0;
+    double* ebc = calloc(nunk, sizeof(double));
+    for (int ci = 0; ci < nex; ++ci)
+        for (int cj = 0; cj < ney; ++cj) {
+            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+            or_element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj], K, M);
+            for (int a = 0; a < 4; ++a) {
+                int u = node_type2[ht[a]];
+                if (u <= 0) continue;
+                double s = 0.0;
+                for (int b = 0; b < 4; ++b) s += M[a * 4 + b] * rho[ht[b]];
+                rhs[u - 1] += s;
+                for (int b = 0; b < 4; ++b) if (node_type2[ht[b]] <= 0) ebc[u - 1] += K[a * 4 + b] * dphi[ht[b]];
+            }
+        }
+    for (int u = 0; u < nunk; ++u) rhs[u] = rhs[u] - ebc[u];
+    free(ebc);
+}
+
+/* y = A x over unknowns with the element-assembled stiffness (matrix free). */
+void or_stiffness_apply(int nnx, int nny, const double* xs, const double* ys, int delta, const int* node_type2,
+                        int nunk, const double* xu, double* yu) {
+    int nex = nnx - 1, ney = nny - 1;
+    double K[16], M[16];
+    for (int u = 0; u < nunk; ++u) yu[u] = 0.0;
+    for (int ci = 0; ci < nex; ++ci)
+        for (int cj = 0; cj < ney; ++cj) {
+            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+            or_element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj], K, M);
+            for (int a = 0; a < 4; ++a) {
+                int u = node_type2[ht[a]];
+                if (u <= 0) continue;
+                for (int b = 0; b < 4; ++b) {
+                    int v = node_type2[ht[b]];
+                    if (v > 0) yu[u - 1] += K[a * 4 + b] * xu[v - 1];
+                }
+            }
+        }
+}
+
+/* ------------------------------------------------------------------ MCC */
+#define OR_KB 1.3807e-23
+#define OR_EXP 2.71828
+#define OR_JTOEV 1.6022e-19
+#define OR_PI 3.141592653589793238
+#define OR_MINREAL 1.0e-15
+
+/* Selection: one Philox word of counter (slot, step, tag, 0) per live slot against
+ * floor(CollisionRatio * 2^32). Writes the selected slots in slot order. */
+int64_t or_mcc_select(int64_t n, const int* cell, uint32_t threshold, uint64_t seed, uint32_t step, uint32_t tag, int* out) {
+    int64_t k = 0;
+    for (int64_t s = 0; s < n; ++s) {
+        if (cell[s] < 0) continue;
+        or_p4 o = or_philox((uint32_t)s, step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        if (o.v[0] < threshold) out[k++] = (int)s;
+    }
+    return k;
+}
+
+typedef struct { double x, y, z; } or_v3;
+
+static or_v3 or_vel_maxwell(or_stream* s, double mass, double temp) {       /* ParticleOne.f90:135-152 */
+    double Beta = 1.0 / (OR_KB * temp), FuncA = 1.0, FuncB = 0.0, R = 0.5;
+    int it = 0;
+    while (FuncA > FuncB) {
+        R = or_u01(s); FuncA = R * R;
+        R = or_u01(s); FuncB = -OR_EXP * R * log(R);
+        if (++it > 1000) break;
+    }
+    double V = sqrt(-3.0 * log(R) / Beta / mass);
+    R = or_u01(s);                                                               /* :247-261 */
+    double CosTheta = 1.0 - 2.0 * R, SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+    R = or_u01(s);
+    double Fai = 2.0 * OR_PI * R;
+    or_v3 v = {V * CosTheta, V * SinTheta * cos(Fai), V * SinTheta * sin(Fai)};
+    return v;
+}
+
+typedef struct { double Gx, Gy, Gz, Gper, G, Energy, MassRatio; or_v3 v, vt; } or_mcpo;
+
+static void or_post(or_mcpo* M, or_stream* s, int iso) {                     /* TypeMCC.f90:101-133 */
+    if (fabs(M->Gper) < OR_MINREAL) {
+        double R = or_u01(s), CosTheta = 1.0 - 2.0 * R, SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+        R = or_u01(s);
+        double Fai = 2.0 * OR_PI * R;
+        M->v.x = M->G * CosTheta; M->v.y = M->G * SinTheta * cos(Fai); M->v.z = M->G * SinTheta * sin(Fai);
+        return;
+    }
+    double CosTheta;
+    if (!iso) {                                                                  /* MCCChi.f90:13-25 */
+        if (M->Energy < OR_MINREAL) CosTheta = 1.0;
+        else { double R = or_u01(s); CosTheta = (M->Energy + 2.0 - 2.0 * pow(M->Energy + 1.0, R)) / M->Energy; }
+    } else { double R = or_u01(s); CosTheta = 1.0 - 2.0 * R; }
+    double FcosTheta = 1.0 - CosTheta, SinTheta = sqrt(1.0 - CosTheta * CosTheta);
+    double R = or_u01(s), APhi = 2.0 * OR_PI * R, CosPhi = cos(APhi), SinPhi = sin(APhi);
+    double hx = M->Gper * CosPhi;
+    double hy = -(M->Gx * M->Gy * CosPhi + M->G * M->Gz * SinPhi) / M->Gper;
+    double hz = -(M->Gx * M->Gz * CosPhi - M->G * M->Gy * SinPhi) / M->Gper;
+    M->v.x = M->v.x - M->MassRatio * (M->Gx * FcosTheta + hx * SinTheta);
+    M->v.y = M->v.y - M->MassRatio * (M->Gy * FcosTheta + hy * SinTheta);
+    M->v.z = M->v.z - M->MassRatio * (M->Gz * FcosTheta + hz * SinTheta);
+}
+
+static void or_vupd(or_mcpo* M, double f) {                                   /* TypeMCC.f90:84-99 */
+    M->Gx *= f; M->Gy *= f; M->Gz *= f; M->Gper *= f; M->G *= f;
+    M->v.x *= f; M->v.y *= f; M->v.z *= f;
+}
+
+/* One collision (MCCPublic.F90:58-90) of selected particle `slot` (normalised velocity
+ * v in/out). prob: Probility(nreact, nsigma) column-major. Returns the reaction index
+ * (0 = null); for ionisation writes the new electron and ion velocities (normalised)
+ * into new_e[3], new_i[3]. */
+int or_mcc_collide(int electron, int slot, uint64_t seed, uint32_t step, uint32_t tag,
+                   double v_ref, double mass, double gas_mass, double gas_temp,
+                   int nreact, int nsigma, double emin, double eint, double emax,
+                   const double* prob, const int* rtype, const double* thr,
+                   double* v, double* new_e, double* new_i) {
+    or_stream s;
+    or_stream_init(&s, seed, (uint32_t)slot, step, tag);
+    or_mcpo M;
+    M.v.x = v[0] * v_ref; M.v.y = v[1] * v_ref; M.v.z = v[2] * v_ref;                  /* VelRes(VFactor) */
+    double Miu = mass * gas_mass / (mass + gas_mass);
+    M.MassRatio = gas_mass / (mass + gas_mass);
+    M.vt = or_vel_maxwell(&s, gas_mass, gas_temp);
+    M.Gx = M.v.x - M.vt.x; M.Gy = M.v.y - M.vt.y; M.Gz = M.v.z - M.vt.z;
+    double Gtemp = M.Gy * M.Gy + M.Gz * M.Gz;
+    M.Gper = sqrt(Gtemp);
+    double G2 = Gtemp + M.Gx * M.Gx;
+    M.G = sqrt(G2);
+    M.Energy = 0.5 * Miu * G2 / OR_JTOEV;
+    double P[4] = {0, 0, 0, 0};
+    double lE = log10(M.Energy);
+    int N = (int)ceil((lE - emin) / eint);
+    if (N < 1) for (int i = 0; i < nreact; ++i) P[i] = prob[i];
+    else if (N < nsigma) {
+        double S1 = (double)N - (lE - emin) / eint, S2 = 1.0 - S1;
+        for (int i = 0; i < nreact; ++i) P[i] = prob[(size_t)(N - 1) * nreact + i] * S1 + prob[(size_t)N * nreact + i] * S2;
+    } else for (int i = 0; i < nreact; ++i) P[i] = prob[(size_t)(nsigma - 1) * nreact + i] * M.Energy / emax;
+    for (int i = 0; i < nreact; ++i) if (M.Energy < thr[i]) P[i] = 0.0;
+    double R = or_u01(&s);
+    int index = 0;
+    for (int i = 0; i < nreact; ++i) if (R < P[i]) { index = i + 1; break; }
+    if (index > 0) {
+        int type = rtype[index - 1];
+        double th = thr[index - 1];
+        if (electron) {
+            if (type == 11) or_post(&M, &s, 0);
+            else if (type == 12) { double f = sqrt(1.0 - th / M.Energy); M.Energy = M.Energy - th; or_vupd(&M, f); or_post(&M, &s, 0); }
+            else if (type == 13) {                                                   /* MCCElectron.F90:48-80 */
+                or_v3 vsolid = M.v;
+                double Eorig = M.Energy, EnergyTemp = M.Energy - th;
+                double R1 = or_u01(&s);
+                double EnergyOld = 10.0 * tan(R1 * atan(EnergyTemp) / 20.0);
+                if (fabs(EnergyOld) < OR_MINREAL) EnergyOld = OR_MINREAL;
+                double f = sqrt(EnergyOld / M.Energy);
+                or_vupd(&M, f);
+                or_post(&M, &s, 0);
+                double EnergyNew = EnergyTemp - EnergyOld, f2 = sqrt(EnergyNew / Eorig);
+                new_e[0] = vsolid.x * f2 / v_ref; new_e[1] = vsolid.y * f2 / v_ref; new_e[2] = vsolid.z * f2 / v_ref;
+                new_i[0] = M.vt.x / v_ref; new_i[1] = M.vt.y / v_ref; new_i[2] = M.vt.z / v_ref;
+            }
+        } else {
+            if (type == 101) or_post(&M, &s, 1);
+            else if (type == 103 || type == 102) M.v = M.vt;
+        }
+    }
+    double vf = 1.0 / v_ref;
+    v[0] = M.v.x * vf; v[1] = M.v.y * vf; v[2] = M.v.z * vf;
+    return index;
+}
+
+/* Cumulative probability table of ProbilityNonReactive (MCCInit.F90:115-151) from
+ * sigma(nreact, nsigma) (m^2) on the 10^(emin + i*eint) grid. Returns SigmaMax. */
+double or_mcc_table(int nreact, int nsigma, double emin, double eint, double mass, double gas_mass, double density,
+                    const double* sigma, double* prob) {
+    double Miu = mass * gas_mass / (mass + gas_mass), mx = 0.0;
+    for (int i = 0; i < nsigma; ++i) {
+        double Energy = pow(10.0, emin + (double)i * eint);
+        double V = sqrt(2.0 * Energy * OR_JTOEV / Miu);
+        for (int j = 0; j < nreact; ++j) prob[(size_t)i * nreact + j] = sigma[(size_t)i * nreact + j] * V * density;
+    }
+    for (int i = 0; i < nsigma; ++i)
+        for (int j = 1; j < nreact; ++j) prob[(size_t)i * nreact + j] = prob[(size_t)i * nreact + j] + prob[(size_t)i * nreact + j - 1];
+    for (size_t k = 0; k < (size_t)nsigma * nreact; ++k) if (prob[k] > mx) mx = prob[k];
+    for (size_t k = 0; k < (size_t)nsigma * nreact; ++k) prob[k] = prob[k] / mx;
+    return mx;
+}

@@ -1,0 +1,216 @@
+"""ctypes front of the CPU oracle (liboracle.so built from oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+cpu_baseline leg of bench.py; the GPU library never uses it. See oracle.c for the
+reference routine each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+
+def build() -> None:
+    subprocess.run(["make", "-C", str(HERE)], check=True, stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        _lib.or_philox_block.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        _lib.or_move_all.restype = C.c_int
+        _lib.or_move_all.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                                     C.c_uint32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_locate.restype = C.c_int
+        _lib.or_locate.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+        _lib.or_deposit.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int64,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        _lib.or_normalize.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_brinv.restype = C.c_int
+        _lib.or_brinv.argtypes = [C.c_void_p, C.c_int]
+        _lib.or_ppr.restype = C.c_int
+        _lib.or_ppr.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]
+        _lib.or_ppr_gather.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_pcg.restype = C.c_int
+        _lib.or_pcg.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_double, C.c_int, C.POINTER(C.c_int)]
+        _lib.or_element_matrices.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                             C.c_void_p]
+        _lib.or_rhs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_int, C.c_void_p]
+        _lib.or_stiffness_apply.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                            C.c_void_p, C.c_void_p]
+        _lib.or_mcc_select.restype = C.c_int64
+        _lib.or_mcc_select.argtypes = [C.c_int64, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.c_void_p]
+        _lib.or_mcc_collide.restype = C.c_int
+        _lib.or_mcc_collide.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_mcc_table.restype = C.c_double
+        _lib.or_mcc_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_void_p, C.c_void_p]
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def philox(ctr, key) -> np.ndarray:
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox_block(c, k, o)
+    return np.array(list(o), dtype=np.uint32)
+
+
+def tag(species: int, purpose: int) -> int:
+    return (purpose << 16) | (species & 0xFFFF)
+
+
+RNG_EMIT, RNG_SELECT, RNG_COLLIDE = 2, 3, 4
+
+
+def move_all(bits, xs, ys, element_index, node_index, objects, efx, efy, coef5, qm, dt, v_ref, mass, init_temp_K,
+             seed, step, tag_emit, x, y, vx, vy, vz, cell):
+    """MoveAll on copies; returns (x, y, vx, vy, vz, alive, counters, err)."""
+    ft = np.float64 if bits == 64 else np.float32
+    st = [np.array(a, dtype=ft) for a in (x, y, vx, vy, vz)]
+    f = [np.ascontiguousarray(a, dtype=ft) for a in (efx, efy, coef5)]
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    alive = np.zeros(len(cell), dtype=np.int32)
+    cnt = np.zeros(8, dtype=np.int64)
+    objs = np.ascontiguousarray(objects, dtype=np.float64)
+    err = lib().or_move_all(bits, len(xs), len(ys), _p(xs), _p(ys), _p(element_index), _p(node_index),
+                            len(objects), _p(objs), _p(f[0]), _p(f[1]), _p(f[2]), qm, dt, v_ref, mass, init_temp_K,
+                            seed, step, tag_emit, len(cell), *[_p(a) for a in st], _p(cell), _p(alive), _p(cnt))
+    return (*st, alive, cnt, err)
+
+
+def locate(xs, ys, x, y) -> tuple[np.ndarray, int]:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(len(x), dtype=np.int32)
+    bad = lib().or_locate(len(xs), len(ys), _p(xs), _p(ys), len(x), _p(x), _p(y), _p(out))
+    return out, bad
+
+
+def deposit(bits, xs, ys, delta, x, y, vx, vy, vz, cell, mass, v_ref, fix):
+    ft = np.float64 if bits == 64 else np.float32
+    arrs = [np.ascontiguousarray(a, dtype=ft) for a in (x, y, vx, vy, vz)]
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    nn = len(xs) * len(ys)
+    w, e = np.zeros(nn), np.zeros(nn)
+    lib().or_deposit(bits, len(xs), len(ys), _p(xs), _p(ys), delta, len(cell), *[_p(a) for a in arrs], _p(cell),
+                     mass, v_ref, 1 if fix else 0, _p(w), _p(e))
+    return w, e
+
+
+def normalize(delta, weight, qs_norm, L_ref, n_ref, vol, w, e, rho):
+    rs, ek = np.zeros_like(w), np.zeros_like(w)
+    lib().or_normalize(len(w), delta, weight, qs_norm, L_ref, n_ref, _p(np.ascontiguousarray(vol)), _p(w), _p(e),
+                       _p(rs), _p(ek), _p(rho))
+    return rs, ek
+
+
+def brinv(a: np.ndarray) -> tuple[np.ndarray, int]:
+    b = np.ascontiguousarray(a, dtype=np.float64).copy()
+    ok = lib().or_brinv(_p(b), b.shape[0])
+    return b, ok
+
+
+def ppr(xs, ys, point_patch, phi):
+    nn = len(xs) * len(ys)
+    coef, efx, efy = np.zeros(6 * nn), np.zeros(nn), np.zeros(nn)
+    pp = np.ascontiguousarray(point_patch, dtype=np.int32)
+    ns = lib().or_ppr(len(xs), len(ys), _p(xs), _p(ys), _p(pp), _p(np.ascontiguousarray(phi)), _p(coef), _p(efx),
+                      _p(efy))
+    return coef, efx, efy, ns
+
+
+def ppr_gather(xs, ys, element_index, node_index, coef, x, y, ele0):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    e = np.ascontiguousarray(ele0, dtype=np.int32)
+    ex, ey = np.zeros_like(x), np.zeros_like(x)
+    lib().or_ppr_gather(len(xs), len(ys), _p(xs), _p(ys), _p(element_index), _p(node_index), _p(coef), len(x), _p(x),
+                        _p(y), _p(e), _p(ex), _p(ey))
+    return ex, ey
+
+
+def pcg(rowptr, col, val, diag, b, x0, tol, maxit):
+    """MY_PCG restated: returns (x, r, iter, ierr)."""
+    b = np.array(b, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    it = C.c_int(-1)
+    ierr = lib().or_pcg(len(diag), _p(np.ascontiguousarray(rowptr, dtype=np.int32)),
+                        _p(np.ascontiguousarray(col, dtype=np.int32)), _p(np.ascontiguousarray(val)),
+                        _p(np.ascontiguousarray(diag)), _p(b), _p(x), tol, maxit, C.byref(it))
+    return x, b, it.value, ierr
+
+
+def element_matrices(delta, x0, y0, hx, hy):
+    K, M = np.zeros(16), np.zeros(16)
+    lib().or_element_matrices(delta, x0, y0, hx, hy, _p(K), _p(M))
+    return K.reshape(4, 4), M.reshape(4, 4)
+
+
+def rhs(xs, ys, delta, node_type2, dphi, rho, nunk):
+    out = np.zeros(nunk)
+    lib().or_rhs(len(xs), len(ys), _p(xs), _p(ys), delta, _p(np.ascontiguousarray(node_type2, dtype=np.int32)),
+                 _p(np.ascontiguousarray(dphi)), _p(np.ascontiguousarray(rho)), nunk, _p(out))
+    return out
+
+
+def stiffness_apply(xs, ys, delta, node_type2, xu):
+    xu = np.ascontiguousarray(xu, dtype=np.float64)
+    out = np.zeros_like(xu)
+    lib().or_stiffness_apply(len(xs), len(ys), _p(xs), _p(ys), delta,
+                             _p(np.ascontiguousarray(node_type2, dtype=np.int32)), len(xu), _p(xu), _p(out))
+    return out
+
+
+def mcc_select(cell, threshold, seed, step, tag_):
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    out = np.zeros(len(cell), dtype=np.int32)
+    k = lib().or_mcc_select(len(cell), _p(cell), threshold, seed, step, tag_, _p(out))
+    return out[:k]
+
+
+def mcc_collide(electron, slot, seed, step, tag_, v_ref, mass, gas_mass, gas_temp, nreact, nsigma, emin, eint, emax,
+                prob, rtype, thr, v):
+    v = np.array(v, dtype=np.float64)
+    ne, ni = np.zeros(3), np.zeros(3)
+    idx = lib().or_mcc_collide(electron, slot, seed, step, tag_, v_ref, mass, gas_mass, gas_temp, nreact, nsigma, emin,
+                               eint, emax, _p(np.ascontiguousarray(prob)),
+                               _p(np.ascontiguousarray(rtype, dtype=np.int32)), _p(np.ascontiguousarray(thr)), _p(v),
+                               _p(ne), _p(ni))
+    return idx, v, ne, ni
+
+
+def mcc_table(nreact, nsigma, emin, eint, mass, gas_mass, density, sigma):
+    prob = np.zeros(nreact * nsigma)
+    smax = lib().or_mcc_table(nreact, nsigma, emin, eint, mass, gas_mass, density,
+                              _p(np.ascontiguousarray(sigma, dtype=np.float64)), _p(prob))
+    return prob, smax

@@ -1,0 +1,811 @@
+// extern "C" entry points of include/pic_hip.h: library state, set-up upload, species,
+// the per-step drop-ins and the stand-alone drop-ins.
+#include "pic_device.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+PicState g;
+
+int pic_fail(int code, const std::string& msg) {
+    g.err = msg;
+    return code;
+}
+
+#define REQUIRE_INIT() do { if (!g.inited) return pic_fail(PIC_ERR_STATE, "pic_init not called"); } while (0)
+#define REQUIRE_DOMAIN() do { REQUIRE_INIT(); if (!g.have_domain) return pic_fail(PIC_ERR_STATE, "pic_setup_domain not called"); } while (0)
+
+template <typename T>
+static int dupload(T** d, const std::vector<T>& h) {
+    if (*d) { hipFree(*d); *d = nullptr; }
+    size_t n = std::max<size_t>(h.size(), 1);
+    HIPCHK(hipMalloc((void**)d, n * sizeof(T)));
+    if (!h.empty()) HIPCHK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PIC_OK;
+}
+template <typename T>
+static int dalloc(T** d, size_t n) {
+    if (*d) { hipFree(*d); *d = nullptr; }
+    HIPCHK(hipMalloc((void**)d, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemset(*d, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return PIC_OK;
+}
+#define TRY(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+static int check_err_flags() {
+    unsigned int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, g.d_err, sizeof(unsigned int), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (!e) return PIC_OK;
+    HIPCHK(hipMemset(g.d_err, 0, sizeof(unsigned int)));
+    if (e & PIC_EBIT_NONFINITE) return pic_fail(PIC_ERR_NONFINITE, "non-finite particle state");
+    if (e & PIC_EBIT_POSITION) return pic_fail(PIC_ERR_POSITION, "particle located outside the mesh");
+    if (e & (PIC_EBIT_BOUNDARY | PIC_EBIT_WRONGCROSS)) return pic_fail(PIC_ERR_BOUNDARY, "particle boundary loop did not settle");
+    if (e & PIC_EBIT_EKRHO) return pic_fail(PIC_ERR_POSITION, "Ek_tot /= 0 with rho == 0 (GetParChg_2D.f90:887-890)");
+    if (e & PIC_EBIT_MCCLOOP) return pic_fail(PIC_ERR_BOUNDARY, "MCC rejection loop did not end");
+    return pic_fail(PIC_ERR_HIP, "device error flag " + std::to_string(e));
+}
+
+template <typename R>
+__global__ void k_load(int64_t n, const double* x, const double* y, const double* vx, const double* vy, const double* vz,
+                       const int* loc, R* X, R* Y, R* VX, R* VY, R* VZ, int* cell, MeshParams m, unsigned int* err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    R xr = (R)x[i], yr = (R)y[i];
+    X[i] = xr; Y[i] = yr; VX[i] = (R)vx[i]; VY[i] = (R)vy[i]; VZ[i] = (R)vz[i];
+    if (loc) cell[i] = loc[i] - 1;
+    else {
+        int bad = 0;
+        int e = pic_locate_elem((double)xr, (double)yr, m, &bad);
+        if (bad) atomicOr(err, PIC_EBIT_POSITION);
+        cell[i] = e;
+    }
+}
+
+static MeshParams dev_mesh() {
+    MeshParams m = g.mesh;
+    m.xs = g.d_xs; m.ys = g.d_ys; m.xsf = g.d_xsf; m.ysf = g.d_ysf;
+    m.element_index = g.d_element_index; m.node_index = g.d_node_index;
+    return m;
+}
+
+template <typename R>
+static void to_host_double(const void* dsrc, int64_t n, std::vector<double>& out) {
+    std::vector<R> tmp(n);
+    hipMemcpy(tmp.data(), dsrc, n * sizeof(R), hipMemcpyDeviceToHost);
+    out.assign(tmp.begin(), tmp.end());
+}
+
+__global__ void k_count_live2(int64_t n, const int* cell, unsigned long long* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool l = i < n && cell[i] >= 0;
+    unsigned long long b = __ballot(l);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+}
+
+__global__ void k_impose_dirichlet(int n, const int* nt2, const double* dphi, double* phi) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && nt2[i] <= 0) phi[i] = dphi[i];
+}
+
+__global__ void k_coef_split(int nn, const double* coef6, double* coef5, float* coeff) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    for (int r = 0; r < 5; ++r) { double v = coef6[(size_t)n * 6 + r + 1]; coef5[(size_t)n * 5 + r] = v; coeff[(size_t)n * 5 + r] = (float)v; }
+}
+__global__ void k_d2f_n(int64_t n, const double* a, float* b) { int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = (float)a[i]; }
+
+extern "C" {
+
+const char* pic_last_error(void) { return g.err.c_str(); }
+
+int pic_device_count(int* n) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) { *n = 0; return pic_fail(PIC_ERR_NO_DEVICE, hipGetErrorString(e)); }
+    *n = c;
+    return PIC_OK;
+}
+
+int pic_init(int device) {
+    if (g.inited) return PIC_OK;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess || c == 0) return pic_fail(PIC_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= c) return pic_fail(PIC_ERR_ARG, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    g.device = device;
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIPCHK(hipHostMalloc((void**)&g.h_scal_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&g.d_scal, 64 * sizeof(double)));
+    HIPCHK(hipMemset(g.d_scal, 0, 64 * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&g.d_sync, 64 * sizeof(unsigned int)));
+    HIPCHK(hipMemset(g.d_sync, 0, 64 * sizeof(unsigned int)));
+    HIPCHK(hipMalloc((void**)&g.d_err, sizeof(unsigned int)));
+    HIPCHK(hipMemset(g.d_err, 0, sizeof(unsigned int)));
+    for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
+    g.inited = true;
+    return PIC_OK;
+}
+
+int pic_finalize(void) {
+    if (!g.inited) return PIC_OK;
+    hipDeviceSynchronize();
+    hipDeviceReset();
+    g = PicState();
+    return PIC_OK;
+}
+
+int pic_set_precision(int particle_bits, int solver_mode) {
+    if (particle_bits != 32 && particle_bits != 64) return pic_fail(PIC_ERR_ARG, "particle_bits must be 32 or 64");
+    if (solver_mode != 0 && solver_mode != 1) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0 or 1");
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s)
+        if (g.sp[s].defined && particle_bits != g.particle_bits) return pic_fail(PIC_ERR_STATE, "species already allocated");
+    g.particle_bits = particle_bits;
+    g.solver_mode = solver_mode;
+    return PIC_OK;
+}
+
+static void free_species(int s) {
+    SpeciesHost& S = g.sp[s];
+    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob, S.rtype,
+                    S.thr, S.mcc_events};
+    for (void* p : ptrs) if (p) hipFree(p);
+    S = SpeciesHost();
+}
+
+int pic_setup_domain(const pic_domain_config* cfg) {
+    REQUIRE_INIT();
+    if (!cfg) return pic_fail(PIC_ERR_ARG, "cfg is NULL");
+    // species and MCC tables belong to a domain (accumulators are sized by its nodes)
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
+    g.nspecies = 0;
+    g.mcc_ready = false;
+    g.have_domain = false;
+    TRY(setup_build_domain(cfg));
+    TRY(setup_build_ppr_classes());
+    const MeshParams& m = g.mesh;
+    TRY(dupload(&g.d_xs, g.h_xs)); TRY(dupload(&g.d_ys, g.h_ys));
+    std::vector<float> xf(g.h_xs.begin(), g.h_xs.end()), yf(g.h_ys.begin(), g.h_ys.end());
+    TRY(dupload(&g.d_xsf, xf)); TRY(dupload(&g.d_ysf, yf));
+    TRY(dupload(&g.d_element_index, g.h_element_index));
+    TRY(dupload(&g.d_node_index, g.h_node_index));
+    std::vector<double> vol(m.nnodes);
+    for (int n = 0; n < m.nnodes; ++n) vol[n] = g.h_cell_volume[2 * (size_t)n + 1];
+    TRY(dupload(&g.d_cell_volume, vol));
+    TRY(dupload(&g.d_dirichlet_phi, g.h_dirichlet_phi));
+    TRY(dupload(&g.d_node_type2, g.h_node_type2));
+    std::vector<int> un0(g.h_unknown_node.size());
+    for (size_t i = 0; i < un0.size(); ++i) un0[i] = g.h_unknown_node[i] - 1;
+    TRY(dupload(&g.d_unknown_node, un0));
+    // fields: phi starts at the Dirichlet values (InitialPhi_2D / SetObjects_2D), 0 elsewhere
+    TRY(dupload(&g.d_phi, g.h_dirichlet_phi));
+    TRY(dalloc(&g.d_efx, m.nnodes)); TRY(dalloc(&g.d_efy, m.nnodes));
+    TRY(dalloc(&g.d_efxf, m.nnodes)); TRY(dalloc(&g.d_efyf, m.nnodes));
+    TRY(dalloc(&g.d_coef, (size_t)m.nnodes * 6)); TRY(dalloc(&g.d_coef5, (size_t)m.nnodes * 5));
+    TRY(dalloc(&g.d_coeff, (size_t)m.nnodes * 5));
+    TRY(dalloc(&g.d_rho, m.nnodes));
+    TRY(dalloc(&g.d_rho_s, (size_t)m.nnodes * PIC_MAX_SPECIES));
+    TRY(dalloc(&g.d_rho_p, (size_t)m.nnodes * PIC_MAX_SPECIES));
+    TRY(dalloc(&g.d_ek, (size_t)m.nnodes * PIC_MAX_SPECIES));
+    // PPR
+    TRY(dupload(&g.d_ppr_class, g.h_ppr_class));
+    TRY(dupload(&g.d_ppr_classes, g.h_ppr_classes));
+    g.n_ppr_classes = (int)g.h_ppr_classes.size();
+    // solver
+    int nu = g.nunknowns;
+    TRY(dupload(&g.d_A_rowptr, g.h_A_rowptr)); TRY(dupload(&g.d_A_col, g.h_A_col));
+    TRY(dupload(&g.d_A_val, g.h_A_val)); TRY(dupload(&g.d_A_diag, g.h_A_diag));
+    std::vector<float> avf(g.h_A_val.begin(), g.h_A_val.end()), adf(g.h_A_diag.begin(), g.h_A_diag.end());
+    TRY(dupload(&g.d_A_valf, avf)); TRY(dupload(&g.d_A_diagf, adf));
+    TRY(dupload(&g.d_B_rowptr, g.h_B_rowptr)); TRY(dupload(&g.d_B_col, g.h_B_col));
+    TRY(dupload(&g.d_B_val, g.h_B_val)); TRY(dupload(&g.d_c, g.h_c));
+    TRY(dalloc(&g.d_b, nu)); TRY(dalloc(&g.d_x, nu)); TRY(dalloc(&g.d_r, nu)); TRY(dalloc(&g.d_z, nu));
+    TRY(dalloc(&g.d_p, nu)); TRY(dalloc(&g.d_p2, nu)); TRY(dalloc(&g.d_q, nu));
+    TRY(dalloc(&g.d_bf, nu)); TRY(dalloc(&g.d_xf, nu)); TRY(dalloc(&g.d_rf, nu)); TRY(dalloc(&g.d_zf, nu));
+    TRY(dalloc(&g.d_pf, nu)); TRY(dalloc(&g.d_p2f, nu)); TRY(dalloc(&g.d_qf, nu));
+    size_t nblk = (size_t)std::max(m.nnodes, nu) / 256 + 2;
+    TRY(dalloc(&g.d_partials, 4 * nblk + 64));
+    g.have_domain = true;
+    return PIC_OK;
+}
+
+int pic_get_sizes(int32_t* nnodes, int32_t* nelem, int32_t* nunknowns, int64_t* nnz_A, int64_t* nnz_B) {
+    REQUIRE_DOMAIN();
+    if (nnodes) *nnodes = g.mesh.nnodes;
+    if (nelem) *nelem = g.mesh.nelem;
+    if (nunknowns) *nunknowns = g.nunknowns;
+    if (nnz_A) *nnz_A = (int64_t)g.h_A_col.size();
+    if (nnz_B) *nnz_B = (int64_t)g.h_B_col.size();
+    return PIC_OK;
+}
+
+int pic_get_setup_array(int which, void* dst, int64_t n) {
+    REQUIRE_DOMAIN();
+    const MeshParams& m = g.mesh;
+    auto put = [&](const void* src, size_t cnt, size_t esz) -> int {
+        if ((int64_t)cnt > n) return pic_fail(PIC_ERR_ARG, "destination too small");
+        memcpy(dst, src, cnt * esz);
+        return PIC_OK;
+    };
+    switch (which) {
+    case PIC_SETUP_HP: {
+        std::vector<double> hp(2 * (size_t)m.nnodes);
+        for (int i = 0; i < m.nnx; ++i)
+            for (int j = 0; j < m.nny; ++j) { size_t k = (size_t)i * m.nny + j; hp[2 * k] = g.h_xs[i]; hp[2 * k + 1] = g.h_ys[j]; }
+        return put(hp.data(), hp.size(), 8);
+    }
+    case PIC_SETUP_HT: {
+        std::vector<int> ht(5 * (size_t)m.nelem, 0);
+        for (int e = 0; e < m.nelem; ++e) {
+            int ci = e / m.ney, cj = e % m.ney;
+            int n1 = ci * m.nny + cj;
+            ht[5 * e] = n1 + 1; ht[5 * e + 1] = n1 + m.nny + 1; ht[5 * e + 2] = n1 + m.nny + 2; ht[5 * e + 3] = n1 + 2;
+        }
+        return put(ht.data(), ht.size(), 4);
+    }
+    case PIC_SETUP_ELEMENT_INDEX: return put(g.h_element_index.data(), g.h_element_index.size(), 4);
+    case PIC_SETUP_NODE_INDEX: return put(g.h_node_index.data(), g.h_node_index.size(), 4);
+    case PIC_SETUP_NODE_TYPE: {
+        std::vector<int> nt(2 * (size_t)m.nnodes);
+        for (int k = 0; k < m.nnodes; ++k) { nt[2 * k] = g.h_node_type2[k] > 0 ? 0 : 1; nt[2 * k + 1] = g.h_node_type2[k]; }
+        return put(nt.data(), nt.size(), 4);
+    }
+    case PIC_SETUP_CELL_VOLUME: return put(g.h_cell_volume.data(), g.h_cell_volume.size(), 8);
+    case PIC_SETUP_A_ROWPTR: return put(g.h_A_rowptr.data(), g.h_A_rowptr.size(), 4);
+    case PIC_SETUP_A_COL: return put(g.h_A_col.data(), g.h_A_col.size(), 4);
+    case PIC_SETUP_A_VAL: return put(g.h_A_val.data(), g.h_A_val.size(), 8);
+    case PIC_SETUP_A_DIAG: return put(g.h_A_diag.data(), g.h_A_diag.size(), 8);
+    case PIC_SETUP_B_ROWPTR: return put(g.h_B_rowptr.data(), g.h_B_rowptr.size(), 4);
+    case PIC_SETUP_B_COL: return put(g.h_B_col.data(), g.h_B_col.size(), 4);
+    case PIC_SETUP_B_VAL: return put(g.h_B_val.data(), g.h_B_val.size(), 8);
+    case PIC_SETUP_RHS_CONST: return put(g.h_c.data(), g.h_c.size(), 8);
+    case PIC_SETUP_DIRICHLET_PHI: return put(g.h_dirichlet_phi.data(), g.h_dirichlet_phi.size(), 8);
+    case PIC_SETUP_POINT_PATCH: return put(g.h_point_patch.data(), g.h_point_patch.size(), 4);
+    case PIC_SETUP_UNKNOWN_NODE: return put(g.h_unknown_node.data(), g.h_unknown_node.size(), 4);
+    default: return pic_fail(PIC_ERR_ARG, "unknown set-up array");
+    }
+}
+
+// ----------------------------------------------------------------------- species
+int pic_set_species(int isp, const pic_species_params* p, int64_t capacity) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES) return pic_fail(PIC_ERR_ARG, "species index out of range");
+    if (!p || capacity <= 0 || capacity > (int64_t)INT32_MAX) return pic_fail(PIC_ERR_ARG, "bad species parameters");
+    SpeciesHost& S = g.sp[isp];
+    S.p = *p;
+    S.qs = p->charge_C / PIC_ELECTRON_CHARGE;
+    S.xm = p->mass_kg / PIC_ELECTRON_MASS;
+    S.qm = S.qs / S.xm;
+    S.capacity = capacity;
+    S.nslots = 0; S.nlive = 0;
+    size_t esz = g.particle_bits == 64 ? 8 : 4;
+    void** arr[5] = {&S.x, &S.y, &S.vx, &S.vy, &S.vz};
+    for (int a = 0; a < 5; ++a) {
+        if (*arr[a]) hipFree(*arr[a]);
+        *arr[a] = nullptr;
+        HIPCHK(hipMalloc(arr[a], (size_t)capacity * esz));
+    }
+    TRY(dalloc(&S.cell, (size_t)capacity)); TRY(dalloc(&S.cell_next, (size_t)capacity));
+    TRY(dalloc(&S.acc_w, (size_t)g.mesh.nnodes)); TRY(dalloc(&S.acc_e, (size_t)g.mesh.nnodes));
+    TRY(dalloc(&S.counters, 16)); TRY(dalloc(&S.mcc_events, 32));
+    S.acc_dirty = false; S.pending = false; S.defined = true;
+    g.nspecies = std::max(g.nspecies, isp + 1);
+    return PIC_OK;
+}
+
+
+int pic_load_particles(int isp, int64_t n, const double* x, const double* y, const double* vx, const double* vy,
+                       const double* vz, const int32_t* loc) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    SpeciesHost& S = g.sp[isp];
+    if (n < 0 || n > S.capacity) return pic_fail(PIC_ERR_CAPACITY, "more particles than capacity");
+    S.nslots = n; S.nlive = n; S.pending = false;
+    if (n == 0) return PIC_OK;
+    double* d = nullptr;
+    int* dl = nullptr;
+    HIPCHK(hipMalloc(&d, 5 * (size_t)n * sizeof(double)));
+    const double* src[5] = {x, y, vx, vy, vz};
+    for (int a = 0; a < 5; ++a) HIPCHK(hipMemcpy(d + (size_t)a * n, src[a], n * sizeof(double), hipMemcpyHostToDevice));
+    if (loc) { HIPCHK(hipMalloc(&dl, n * sizeof(int))); HIPCHK(hipMemcpy(dl, loc, n * sizeof(int), hipMemcpyHostToDevice)); }
+    int blocks = (int)((n + 255) / 256);
+    MeshParams m = dev_mesh();
+    if (g.particle_bits == 64)
+        hipLaunchKernelGGL(k_load<double>, dim3(blocks), dim3(256), 0, g.stream, n, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, dl,
+                           (double*)S.x, (double*)S.y, (double*)S.vx, (double*)S.vy, (double*)S.vz, S.cell, m, g.d_err);
+    else
+        hipLaunchKernelGGL(k_load<float>, dim3(blocks), dim3(256), 0, g.stream, n, d, d + n, d + 2 * n, d + 3 * n, d + 4 * n, dl,
+                           (float*)S.x, (float*)S.y, (float*)S.vx, (float*)S.vy, (float*)S.vz, S.cell, m, g.d_err);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    hipFree(d);
+    if (dl) hipFree(dl);
+    return check_err_flags();
+}
+
+int pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    if (ppc <= 0 || !(temp_eV >= 0)) return pic_fail(PIC_ERR_ARG, "bad ppc / temperature");
+    std::vector<double> x, y, vx, vy, vz;
+    std::vector<int> cell;
+    setup_maxwellian(isp, ppc, temp_eV, seed, x, y, vx, vy, vz, cell);
+    if ((int64_t)x.size() > g.sp[isp].capacity) return pic_fail(PIC_ERR_CAPACITY, "load exceeds capacity");
+    return pic_load_particles(isp, (int64_t)x.size(), x.data(), y.data(), vx.data(), vy.data(), vz.data(), nullptr);
+}
+
+
+int pic_get_particles(int isp, double* x, double* y, double* vx, double* vy, double* vz, int32_t* loc,
+                      int64_t capacity, int64_t* n) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    SpeciesHost& S = g.sp[isp];
+    HIPCHK(hipStreamSynchronize(g.stream));
+    int64_t ns = S.nslots;
+    std::vector<double> a[5];
+    void* src[5] = {S.x, S.y, S.vx, S.vy, S.vz};
+    for (int k = 0; k < 5; ++k) {
+        if (g.particle_bits == 64) to_host_double<double>(src[k], ns, a[k]);
+        else to_host_double<float>(src[k], ns, a[k]);
+    }
+    std::vector<int> c(ns);
+    if (ns) HIPCHK(hipMemcpy(c.data(), S.pending ? S.cell : S.cell, ns * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t k = 0;
+    for (int64_t i = 0; i < ns; ++i) {
+        if (c[i] < 0) continue;
+        if (k >= capacity) return pic_fail(PIC_ERR_CAPACITY, "destination too small");
+        if (x) x[k] = a[0][i];
+        if (y) y[k] = a[1][i];
+        if (vx) vx[k] = a[2][i];
+        if (vy) vy[k] = a[3][i];
+        if (vz) vz[k] = a[4][i];
+        if (loc) loc[k] = c[i] + 1;
+        ++k;
+    }
+    *n = k;
+    return PIC_OK;
+}
+
+int pic_load_bundle(int isp, const pic_particle_one* po, int64_t npar) {
+    if (npar < 0 || (npar > 0 && !po)) return pic_fail(PIC_ERR_ARG, "bad bundle");
+    std::vector<double> x(npar), y(npar), vx(npar), vy(npar), vz(npar);
+    std::vector<int> loc(npar);
+    for (int64_t i = 0; i < npar; ++i) {
+        x[i] = po[i].X; y[i] = po[i].Y; vx[i] = po[i].Vx; vy[i] = po[i].Vy; vz[i] = po[i].Vz; loc[i] = po[i].Location;
+    }
+    return pic_load_particles(isp, npar, x.data(), y.data(), vx.data(), vy.data(), vz.data(), loc.data());
+}
+
+int pic_store_bundle(int isp, pic_particle_one* po, int64_t capacity, int64_t* npar) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    int64_t cap = g.sp[isp].nslots;
+    std::vector<double> x(cap), y(cap), vx(cap), vy(cap), vz(cap);
+    std::vector<int> loc(cap);
+    int64_t n = 0;
+    TRY(pic_get_particles(isp, x.data(), y.data(), vx.data(), vy.data(), vz.data(), loc.data(), cap, &n));
+    if (n > capacity) return pic_fail(PIC_ERR_CAPACITY, "destination too small");
+    for (int64_t i = 0; i < n; ++i) {
+        pic_particle_one& o = po[i];
+        memset(&o, 0, sizeof(o));
+        o.X = x[i]; o.Y = y[i]; o.Vx = vx[i]; o.Vy = vy[i]; o.Vz = vz[i]; o.Location = loc[i];
+    }
+    *npar = n;
+    return PIC_OK;
+}
+
+
+int pic_get_counts(int isp, int64_t* nlive, int64_t* nslots, int64_t counters[8]) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    SpeciesHost& S = g.sp[isp];
+    unsigned long long h[16];
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(h, S.counters, sizeof(h), hipMemcpyDeviceToHost));
+    if (counters) for (int k = 0; k < 8; ++k) counters[k] = (int64_t)h[k];
+    if (nslots) *nslots = S.nslots;
+    if (nlive) {
+        unsigned long long* d = (unsigned long long*)g.d_partials;
+        HIPCHK(hipMemset(d, 0, 8));
+        const int* c = S.pending ? S.cell_next : S.cell;
+        if (S.nslots) hipLaunchKernelGGL(k_count_live2, dim3((int)((S.nslots + 255) / 256)), dim3(256), 0, g.stream, S.nslots, c, d);
+        unsigned long long v = 0;
+        HIPCHK(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        *nlive = (int64_t)v;
+    }
+    return PIC_OK;
+}
+
+int pic_get_mcc_events(int isp, int64_t* events, int nmax) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    unsigned long long h[32];
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(h, g.sp[isp].mcc_events, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nmax && k < 31; ++k) events[k] = (int64_t)h[k + 1];
+    return PIC_OK;
+}
+
+// ----------------------------------------------------------------------- per step
+int pic_MoveAll(int isp, int delta, int PushFlag) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    if (PushFlag != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "only the explicit mover (PushFlag = 0) is on this path");
+    if (delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "delta differs from the set-up domain");
+    if (delta == 1) return pic_fail(PIC_ERR_UNSUPPORTED, "axisymmetric mover not built");
+    SpeciesHost& S = g.sp[isp];
+    if (S.pending) S.acc_dirty = true;   // a repeated MoveAll discards the previous pending deposit
+    TRY(launch_push_deposit(isp, true));
+    S.pending = true;
+    return PIC_OK;
+}
+
+static int commit_locations() {
+    for (int s = 0; s < g.nspecies; ++s) {
+        SpeciesHost& S = g.sp[s];
+        if (!S.defined) continue;
+        if (!S.pending) TRY(launch_push_deposit(s, false));
+        std::swap(S.cell, S.cell_next);
+        S.pending = false;
+    }
+    return PIC_OK;
+}
+
+int pic_GetParChg_2D(int delta) {
+    REQUIRE_DOMAIN();
+    if (delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "delta differs from the set-up domain");
+    TRY(commit_locations());
+    TRY(launch_finalize_deposit());
+    return check_err_flags();
+}
+
+
+static int solve_now() {
+    TRY(launch_rhs());
+    if (g.solver_mode == 0) return run_pcg_fp64(true);
+    return run_pcg_mixed();
+}
+
+int pic_IFE_Solve_2D(const double* Rho, double* Phi, int nnx, int nny, int delta, double time) {
+    REQUIRE_DOMAIN();
+    (void)time;
+    if (nnx != g.mesh.nnx || nny != g.mesh.nny || delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "mesh mismatch");
+    int nn = g.mesh.nnodes;
+    if (Rho) HIPCHK(hipMemcpy(g.d_rho, Rho, nn * sizeof(double), hipMemcpyHostToDevice));
+    if (Phi) {
+        HIPCHK(hipMemcpy(g.d_phi, Phi, nn * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_impose_dirichlet, dim3((nn + 255) / 256), dim3(256), 0, g.stream, nn, g.d_node_type2, g.d_dirichlet_phi, g.d_phi);
+    }
+    int rc = solve_now();
+    if (Phi) {
+        HIPCHK(hipStreamSynchronize(g.stream));
+        HIPCHK(hipMemcpy(Phi, g.d_phi, nn * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return rc;
+}
+
+int pic_last_solve_info(int32_t* iterations, double* relres, int32_t* outer) {
+    if (iterations) *iterations = g.last_iter;
+    if (relres) *relres = g.last_relres;
+    if (outer) *outer = g.last_outer;
+    return PIC_OK;
+}
+
+int pic_set_pcg_params(double tol, int32_t maxit) {
+    if (!(tol > 0) || maxit <= 0) return pic_fail(PIC_ERR_ARG, "bad PCG parameters");
+    g.pcg_tol = tol; g.pcg_maxit = maxit;
+    return PIC_OK;
+}
+
+int pic_GetEField_SIDG_PPR(void) {
+    REQUIRE_DOMAIN();
+    TRY(launch_ppr());
+    return PIC_OK;
+}
+
+int pic_MCCBundleInit(const pic_gas_params* gas, int electron_species, int ion_species) {
+    REQUIRE_DOMAIN();
+    if (!gas) return pic_fail(PIC_ERR_ARG, "gas is NULL");
+    if (electron_species < 0 || electron_species >= PIC_MAX_SPECIES || !g.sp[electron_species].defined ||
+        ion_species < 0 || ion_species >= PIC_MAX_SPECIES || !g.sp[ion_species].defined || ion_species == electron_species)
+        return pic_fail(PIC_ERR_ARG, "species for MCC not defined");
+    g.gas = *gas;
+    g.e_species = electron_species; g.i_species = ion_species;
+    double dt_SI = g.cfg.dt * g.time_ref;
+    int sps[2] = {electron_species, ion_species};
+    for (int k = 0; k < 2; ++k) {
+        int s = sps[k];
+        SpeciesHost& S = g.sp[s];
+        setup_analytic_tables(s, k == 0, *gas, S.p.mass_kg, dt_SI);
+        S.product_species = (k == 0) ? ion_species : -1;
+        TRY(dupload(&S.prob, S.prob_host)); TRY(dupload(&S.rtype, S.rtype_host)); TRY(dupload(&S.thr, S.thr_host));
+        double thr = std::floor(S.collision_ratio * 4294967296.0);
+        S.select_threshold = thr >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)thr;
+        S.has_mcc = true;
+    }
+    g.mcc_ready = true;
+    return PIC_OK;
+}
+
+int pic_mcc_set_table(int isp, int nreact, int nsigma, double energy_min_log10, double energy_interval_log10,
+                      const double* prob, const int32_t* reaction_type, const double* threshold, double sigma_max,
+                      double collision_ratio, int product_species) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    if (nreact < 1 || nreact > 4 || nsigma < 2 || !prob || !reaction_type || !threshold)
+        return pic_fail(PIC_ERR_ARG, "bad table");
+    SpeciesHost& S = g.sp[isp];
+    S.nreact = nreact; S.nsigma = nsigma; S.emin = energy_min_log10; S.eint = energy_interval_log10;
+    S.emax = std::pow(10.0, energy_min_log10 + energy_interval_log10 * (nsigma - 1));
+    S.prob_host.assign(prob, prob + (size_t)nreact * nsigma);
+    S.rtype_host.assign(reaction_type, reaction_type + nreact);
+    S.thr_host.assign(threshold, threshold + nreact);
+    S.sigma_max = sigma_max; S.collision_ratio = collision_ratio; S.product_species = product_species;
+    TRY(dupload(&S.prob, S.prob_host)); TRY(dupload(&S.rtype, S.rtype_host)); TRY(dupload(&S.thr, S.thr_host));
+    double thr = std::floor(collision_ratio * 4294967296.0);
+    S.select_threshold = thr >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)thr;
+    S.has_mcc = true;
+    if (product_species >= 0) { g.e_species = isp; g.i_species = product_species; }
+    g.mcc_ready = true;
+    return PIC_OK;
+}
+
+int pic_mcc_get_table(int isp, double* prob, int64_t n, double* sigma_max, double* collision_ratio) {
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].has_mcc) return pic_fail(PIC_ERR_ARG, "species has no table");
+    SpeciesHost& S = g.sp[isp];
+    if (prob) {
+        if (n < (int64_t)S.prob_host.size()) return pic_fail(PIC_ERR_ARG, "destination too small");
+        memcpy(prob, S.prob_host.data(), S.prob_host.size() * sizeof(double));
+    }
+    if (sigma_max) *sigma_max = S.sigma_max;
+    if (collision_ratio) *collision_ratio = S.collision_ratio;
+    return PIC_OK;
+}
+
+int pic_MCC(void) {
+    REQUIRE_DOMAIN();
+    if (!g.mcc_ready) return pic_fail(PIC_ERR_STATE, "no MCC tables registered");
+    for (int s = 0; s < g.nspecies; ++s)
+        if (g.sp[s].pending) return pic_fail(PIC_ERR_STATE, "MCC between MoveAll and GetParChg_2D");
+    TRY(run_mcc());
+    return check_err_flags();
+}
+
+int pic_set_run_params(int32_t sort_every, uint64_t seed, int32_t mcc_enabled) {
+    if (sort_every < 0) return pic_fail(PIC_ERR_ARG, "sort_every < 0");
+    g.sort_every = sort_every; g.seed = seed; g.mcc_enabled = mcc_enabled;
+    return PIC_OK;
+}
+
+int pic_sort_particles(void) {
+    REQUIRE_DOMAIN();
+    for (int s = 0; s < g.nspecies; ++s)
+        if (g.sp[s].pending) return pic_fail(PIC_ERR_STATE, "sort between MoveAll and GetParChg_2D");
+    return run_sort_all();
+}
+
+int pic_get_step(int64_t* step) { *step = g.step; return PIC_OK; }
+int pic_set_step(int64_t step) { if (step < 0) return pic_fail(PIC_ERR_ARG, "step < 0"); g.step = step; return PIC_OK; }
+
+int pic_initial_field(void) {
+    REQUIRE_DOMAIN();
+    TRY(pic_GetParChg_2D(g.mesh.delta));
+    TRY(solve_now());
+    TRY(launch_ppr());
+    return check_err_flags();
+}
+
+int pic_step(int nsteps) {
+    REQUIRE_DOMAIN();
+    double acc[8] = {0};
+    for (int it = 0; it < nsteps; ++it) {
+        HIPCHK(hipEventRecord(g.ev[0], g.stream));
+        for (int s = 0; s < g.nspecies; ++s) {
+            if (!g.sp[s].defined) continue;
+            TRY(pic_MoveAll(s, g.mesh.delta, 0));
+        }
+        HIPCHK(hipEventRecord(g.ev[1], g.stream));
+        TRY(commit_locations());
+        TRY(launch_finalize_deposit());
+        HIPCHK(hipEventRecord(g.ev[2], g.stream));
+        TRY(launch_rhs());
+        HIPCHK(hipEventRecord(g.ev[3], g.stream));
+        TRY(g.solver_mode == 0 ? run_pcg_fp64(true) : run_pcg_mixed());
+        HIPCHK(hipEventRecord(g.ev[4], g.stream));
+        TRY(launch_ppr());
+        HIPCHK(hipEventRecord(g.ev[5], g.stream));
+        if (g.mcc_enabled && g.mcc_ready) TRY(run_mcc());
+        HIPCHK(hipEventRecord(g.ev[6], g.stream));
+        g.step++;
+        if (g.sort_every > 0 && g.step % g.sort_every == 0) TRY(run_sort_all());
+        HIPCHK(hipEventRecord(g.ev[7], g.stream));
+        HIPCHK(hipEventSynchronize(g.ev[7]));
+        float ms;
+        for (int k = 0; k < 7; ++k) { hipEventElapsedTime(&ms, g.ev[k], g.ev[k + 1]); acc[k] += ms; }
+        hipEventElapsedTime(&ms, g.ev[0], g.ev[7]); acc[7] += ms;
+        TRY(check_err_flags());
+    }
+    for (int k = 0; k < 8; ++k) g.phase_ms[k] = acc[k];
+    return PIC_OK;
+}
+
+int pic_get_phase_times(double* ms, int n) {
+    for (int k = 0; k < n && k < 8; ++k) ms[k] = g.phase_ms[k];
+    return PIC_OK;
+}
+
+// ----------------------------------------------------------------------- fields
+int pic_get_field(int which, double* dst, int64_t n) {
+    REQUIRE_DOMAIN();
+    int64_t nn = g.mesh.nnodes;
+    const double* src = nullptr;
+    int64_t cnt = 0;
+    switch (which) {
+    case PIC_F_RHO: src = g.d_rho; cnt = nn; break;
+    case PIC_F_PHI: src = g.d_phi; cnt = nn; break;
+    case PIC_F_EFX: src = g.d_efx; cnt = nn; break;
+    case PIC_F_EFY: src = g.d_efy; cnt = nn; break;
+    case PIC_F_COEF_PPR: src = g.d_coef; cnt = 6 * nn; break;
+    case PIC_F_RHO_S: src = g.d_rho_s; cnt = nn * g.nspecies; break;
+    case PIC_F_RHO_P: src = g.d_rho_p; cnt = nn * g.nspecies; break;
+    case PIC_F_EK_TOT: src = g.d_ek; cnt = nn * g.nspecies; break;
+    case PIC_F_RHS: src = g.d_b; cnt = g.nunknowns; break;
+    default: return pic_fail(PIC_ERR_ARG, "unknown field");
+    }
+    if (n < cnt) return pic_fail(PIC_ERR_ARG, "destination too small");
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(dst, src, cnt * sizeof(double), hipMemcpyDeviceToHost));
+    return PIC_OK;
+}
+
+
+int pic_set_field(int which, const double* src, int64_t n) {
+    REQUIRE_DOMAIN();
+    int64_t nn = g.mesh.nnodes;
+    switch (which) {
+    case PIC_F_RHO: if (n != nn) return pic_fail(PIC_ERR_ARG, "size"); HIPCHK(hipMemcpy(g.d_rho, src, nn * 8, hipMemcpyHostToDevice)); break;
+    case PIC_F_PHI: if (n != nn) return pic_fail(PIC_ERR_ARG, "size"); HIPCHK(hipMemcpy(g.d_phi, src, nn * 8, hipMemcpyHostToDevice)); break;
+    case PIC_F_EFX: if (n != nn) return pic_fail(PIC_ERR_ARG, "size"); HIPCHK(hipMemcpy(g.d_efx, src, nn * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_d2f_n, dim3((int)((nn + 255) / 256)), dim3(256), 0, g.stream, nn, g.d_efx, g.d_efxf); break;
+    case PIC_F_EFY: if (n != nn) return pic_fail(PIC_ERR_ARG, "size"); HIPCHK(hipMemcpy(g.d_efy, src, nn * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_d2f_n, dim3((int)((nn + 255) / 256)), dim3(256), 0, g.stream, nn, g.d_efy, g.d_efyf); break;
+    case PIC_F_COEF_PPR: if (n != 6 * nn) return pic_fail(PIC_ERR_ARG, "size");
+        HIPCHK(hipMemcpy(g.d_coef, src, 6 * nn * 8, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_coef_split, dim3((int)((nn + 255) / 256)), dim3(256), 0, g.stream, (int)nn, g.d_coef, g.d_coef5, g.d_coeff); break;
+    default: return pic_fail(PIC_ERR_ARG, "field not settable");
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return PIC_OK;
+}
+
+// ----------------------------------------------------------------------- drop-ins
+int pic_MY_JPCG_Solver(const pic_sparse* matrix, double* R, double* X, const double* M, int NZ, int N, int* ITER) {
+    REQUIRE_INIT();
+    if (!matrix || !R || !X || !M || N <= 0 || NZ < N + 1) return pic_fail(PIC_ERR_ARG, "bad MY_JPCG_Solver arguments");
+    std::vector<int> rowptr(N + 1);
+    for (int i = 0; i <= N; ++i) rowptr[i] = matrix[i].SROW - 1;
+    if (rowptr[0] != 0 || rowptr[N] > NZ) return pic_fail(PIC_ERR_ARG, "SROW row pointers inconsistent");
+    int nnz = rowptr[N];
+    std::vector<int> col(nnz);
+    std::vector<double> val(nnz);
+    for (int k = 0; k < nnz; ++k) {
+        col[k] = matrix[k].JCOL - 1;
+        val[k] = matrix[k].K;
+        if (col[k] < 0 || col[k] >= N) return pic_fail(PIC_ERR_ARG, "JCOL out of range");
+    }
+    int *drp = nullptr, *dcol = nullptr;
+    double *dval = nullptr, *ddiag = nullptr, *db = nullptr, *dx = nullptr, *dr = nullptr;
+    HIPCHK(hipMalloc(&drp, (N + 1) * sizeof(int))); HIPCHK(hipMalloc(&dcol, nnz * sizeof(int)));
+    HIPCHK(hipMalloc(&dval, nnz * sizeof(double))); HIPCHK(hipMalloc(&ddiag, N * sizeof(double)));
+    HIPCHK(hipMalloc(&db, N * sizeof(double))); HIPCHK(hipMalloc(&dx, N * sizeof(double))); HIPCHK(hipMalloc(&dr, N * sizeof(double)));
+    HIPCHK(hipMemcpy(drp, rowptr.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dcol, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dval, val.data(), nnz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddiag, M, N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(db, R, N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx, X, N * sizeof(double), hipMemcpyHostToDevice));
+    // scratch for partial sums sized to this system
+    double* save_partials = g.d_partials;
+    double* part = nullptr;
+    HIPCHK(hipMalloc(&part, (4 * ((size_t)N / 256 + 2) + 64) * sizeof(double)));
+    g.d_partials = part;
+    int iters = 0, ierr = 0;
+    int rc = pcg_generic_fp64(N, drp, dcol, dval, ddiag, db, dx, g.pcg_tol, g.pcg_maxit, &iters, &ierr, dr);
+    g.d_partials = save_partials;
+    if (rc == PIC_OK) {
+        HIPCHK(hipMemcpy(X, dx, N * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(R, dr, N * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    hipFree(part); hipFree(drp); hipFree(dcol); hipFree(dval); hipFree(ddiag); hipFree(db); hipFree(dx); hipFree(dr);
+    if (rc) return rc;
+    if (ierr == 1) return pic_fail(PIC_ERR_PCG_MAXITER, "MY_PCG IERR=1 (ITER left unset, as in the reference)");
+    *ITER = iters - 1;     // MY_PCG.f90:129-133: ITER = i-1
+    if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "MY_PCG IERR=2/3");
+    return PIC_OK;
+}
+
+int pic_Get_ParticleEField_PPR(int64_t n, const double* x, const double* y, const int32_t* EleIndex, double* Ex, double* Ey) {
+    REQUIRE_DOMAIN();
+    if (n <= 0) return PIC_OK;
+    double* d; int* de;
+    HIPCHK(hipMalloc(&d, 4 * (size_t)n * sizeof(double)));
+    HIPCHK(hipMalloc(&de, (size_t)n * sizeof(int)));
+    HIPCHK(hipMemcpy(d, x, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + n, y, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(de, EleIndex, n * 4, hipMemcpyHostToDevice));
+    TRY(launch_ppr_gather_points(n, d, d + n, de, d + 2 * n, d + 3 * n));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(Ex, d + 2 * n, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Ey, d + 3 * n, n * 8, hipMemcpyDeviceToHost));
+    hipFree(d); hipFree(de);
+    return PIC_OK;
+}
+
+static int locate_common(int64_t n, const double* x, const double* y, int32_t* out, int mode) {
+    REQUIRE_DOMAIN();
+    if (n <= 0) return PIC_OK;
+    double* d; int* de;
+    HIPCHK(hipMalloc(&d, 2 * (size_t)n * sizeof(double)));
+    HIPCHK(hipMalloc(&de, (size_t)n * sizeof(int)));
+    HIPCHK(hipMemcpy(d, x, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + n, y, n * 8, hipMemcpyHostToDevice));
+    TRY(launch_locate_points(n, d, d + n, de, mode));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(out, de, n * 4, hipMemcpyDeviceToHost));
+    hipFree(d); hipFree(de);
+    return check_err_flags();
+}
+
+int pic_locate(int64_t n, const double* x, const double* y, int32_t* n_element) { return locate_common(n, x, y, n_element, 0); }
+int pic_InjectPositioning(int64_t n, const double* x, const double* y, int32_t* n_element) { return locate_common(n, x, y, n_element, 1); }
+
+// ParticlePositioning on the unrefined mesh (ParticlePositioning.f90:240-363): the final
+// element is the given one; traverse_flag marks inner-edge points that are not corners.
+int pic_ParticlePositioning(int64_t n, const double* x, const double* y, int32_t* n_element, int32_t* traverse_flag) {
+    REQUIRE_DOMAIN();
+    const MeshParams& m = g.mesh;
+    const double sv = 1e-12;
+    for (int64_t k = 0; k < n; ++k) {
+        int e = n_element[k] - 1;
+        traverse_flag[k] = 0;
+        if (e < 0 || e >= m.nelem) return pic_fail(PIC_ERR_POSITION, "element out of range");
+        int ci = e / m.ney, cj = e % m.ney;
+        double l = x[k] - g.h_xs[ci], r = x[k] - g.h_xs[ci + 1], b = y[k] - g.h_ys[cj], t = y[k] - g.h_ys[cj + 1];
+        bool L = std::fabs(l) < sv, Rr = std::fabs(r) < sv, B = std::fabs(b) < sv, T = std::fabs(t) < sv;
+        if (!(L || Rr || B || T)) continue;
+        if ((L && T) || (Rr && T) || (L && B) || (Rr && B)) continue;
+        if (L) traverse_flag[k] = std::fabs(g.h_xs[ci] - m.xmin) < sv ? 0 : 1;
+        else if (Rr) traverse_flag[k] = std::fabs(g.h_xs[ci + 1] - m.xmax) < sv ? 0 : 1;
+        else if (B) traverse_flag[k] = std::fabs(g.h_ys[cj] - m.ymin) < sv ? 0 : 1;
+        else if (T) traverse_flag[k] = std::fabs(g.h_ys[cj + 1] - m.ymax) < sv ? 0 : 1;
+    }
+    return PIC_OK;
+}
+
+// generate_Elementmap.f90:19-57 on the unrefined mesh (repeat_refinement must be 0).
+int pic_generate_Elementmap(int32_t* Element_Map, int repeat_refinement) {
+    REQUIRE_DOMAIN();
+    if (repeat_refinement != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "refined meshes (repeat_refinement > 0) are not built");
+    const MeshParams& m = g.mesh;
+    double dx = m.hx / 1.0, dy = m.hy / 1.0;
+    for (int i = 1; i <= m.nex; ++i)
+        for (int j = 1; j <= m.ney; ++j) {
+            double x = m.xmin + (dx) * (2 * i - 1) / 2;
+            double y = m.ymin + (dy) * (2 * j - 1) / 2;
+            double rxp = (x - m.vox) / m.hx;
+            int k = (int)rxp;
+            double ryp = (y - m.voy) / m.hy;
+            int l = (int)ryp;
+            if (l == m.nny) l = l - 1;
+            if (k == m.nnx) k = k - 1;
+            int ne = l + (k - 1) * (m.nny - 1);
+            Element_Map[(size_t)(i - 1) + (size_t)(j - 1) * m.nex] = ne;
+        }
+    return PIC_OK;
+}
+
+}  // extern "C"

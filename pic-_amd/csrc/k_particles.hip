@@ -1,0 +1,455 @@
+// Particle kernels: the fused mover (MoveAll/MoveOne/Get_ParticleEField_PPR/
+// AdjustParticleBoundary) with the element lookup and deposition of GetParChg_2D, the
+// stand-alone lookup+deposition, the deposition finalisation (corner -> node sums and
+// normalisation) and the batched drop-ins (point lookup, PPR point gather).
+//
+// Particles are structure-of-arrays slots; one thread per slot, 256-thread blocks,
+// consecutive slots in consecutive lanes so every state stream is a coalesced HBM
+// stream. Slots are kept sorted by element (periodic cell sort), so a wave covers one
+// or two elements: the node moments of a wave are summed with a segmented wave
+// reduction keyed by element and only segment tails issue (int64) atomics.
+#include "pic_device.h"
+
+template <typename R> struct PushArgs {
+    R *x, *y, *vx, *vy, *vz;
+    const int* cell;
+    int* cell_next;
+    int64_t n;
+    MeshParams m;
+    const R* efx; const R* efy;
+    const R* coef5;           // c2..c6 per node, node-major (5 per node)
+    const R* xsR; const R* ysR;
+    R qm, dt;
+    R xmin, xmax, ymin, ymax, emit_eps;   // domain and the 10E-6 offset of the reference
+    double vref_inv;          // 1/v_ref  (VelRes(1.0/PB%VFactor))
+    double kappa_c;           // (Kappa-1.5)/beta, beta = m/(2 kB T)   (SI)
+    double energy_c;          // 0.5*Mass*v_ref^2/JtoeV
+    uint64_t seed; uint32_t step; uint32_t tag_emit;
+    long long* acc_w; long long* acc_e;
+    unsigned long long* counters;
+    unsigned int* err;
+};
+
+__device__ inline void wave_count(unsigned long long* ctr, bool pred) {
+    unsigned long long b = __ballot(pred);
+    if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)b) - 1)) atomicAdd(ctr, (unsigned long long)__popcll(b));
+}
+
+__device__ inline long long shfl_up_ll(long long v, int d) {
+    int lo = (int)(unsigned long long)v, hi = (int)((unsigned long long)v >> 32);
+    lo = __shfl_up(lo, d, 64); hi = __shfl_up(hi, d, 64);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Segmented inclusive sum over the wave of eight int64 values keyed by `key`; lanes at
+// the tail of a run of equal keys add the run total to the node accumulators.
+__device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8],
+                                    long long* acc_w, long long* acc_e) {
+    const int lane = threadIdx.x & 63;
+    int kprev = __shfl_up(key, 1, 64);
+    int knext = __shfl_down(key, 1, 64);
+    bool head = (lane == 0) || (kprev != key);
+    bool tail = (lane == 63) || (knext != key);
+    unsigned long long heads = __ballot(head);
+    unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    int seg_start = 63 - __clzll((long long)below);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        bool take = (lane - off) >= seg_start;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            long long u = shfl_up_ll(v[k], off);
+            if (take) v[k] += u;
+        }
+    }
+    if (tail && key >= 0) {
+        int ci = key / m.ney, cj = key - ci * m.ney;
+        int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+        atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
+        atomicAdd((unsigned long long*)&acc_w[n2], (unsigned long long)v[1]);
+        atomicAdd((unsigned long long*)&acc_w[n3], (unsigned long long)v[2]);
+        atomicAdd((unsigned long long*)&acc_w[n4], (unsigned long long)v[3]);
+        atomicAdd((unsigned long long*)&acc_e[n1], (unsigned long long)v[4]);
+        atomicAdd((unsigned long long*)&acc_e[n2], (unsigned long long)v[5]);
+        atomicAdd((unsigned long long*)&acc_e[n3], (unsigned long long)v[6]);
+        atomicAdd((unsigned long long*)&acc_e[n4], (unsigned long long)v[7]);
+    }
+}
+
+// Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64.
+__device__ inline void deposit_weights(double X, double Y, int e, const MeshParams& m, double W[4]) {
+    int ci = e / m.ney, cj = e - ci * m.ney;
+    double x1 = m.xs[ci], x2 = m.xs[ci + 1], y1 = m.ys[cj], y2 = m.ys[cj + 1];
+    double hxp = x2 - x1, hyp = y2 - y1;
+    double dx = (X - x1) / hxp;
+    double dy = (Y - y1) / hyp;
+    double xm = 1.0 - dx, ym = 1.0 - dy;
+    if (m.delta == 0) {
+        W[0] = xm * ym; W[1] = dx * ym; W[2] = dx * dy; W[3] = xm * dy;
+    } else {
+        double beta = (fabs(y1 - m.ymin) < 1e-12) ? 0.75 : 1.0;
+        double R1 = m.ymin + y1, R2 = m.ymin + y2, Rr = Y, den = R2 * R2 - R1 * R1;
+        W[0] = xm * (R2 * R2 - Rr * Rr) / den * beta;
+        W[1] = dx * (R2 * R2 - Rr * Rr) / den * beta;
+        W[2] = dx * (Rr * Rr - R1 * R1) / den;
+        W[3] = xm * (Rr * Rr - R1 * R1) / den;
+    }
+}
+
+template <typename R, int DO_PUSH>
+__global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const MeshParams& m = a.m;
+    bool live = false;
+    int e_new = -1;
+    R X = 0, Y = 0, VX = 0, VY = 0, VZ = 0;
+    bool emitted = false, absorbed_wall = false, absorbed_obj = false, reflected = false;
+    if (p < a.n) {
+        int e_old = a.cell[p];
+        if (e_old >= 0) {
+            live = true;
+            X = a.x[p]; Y = a.y[p]; VX = a.vx[p]; VY = a.vy[p]; VZ = a.vz[p];
+            if (DO_PUSH) {
+                R T = a.dt;
+                int ivel = 1, ipos = 1, it = 0;
+                int n = e_old;
+                pic_stream rs; bool rs_init = false;
+                while (ivel == 1 || ipos == 1) {
+                    if (++it > 64) { atomicOr(a.err, PIC_EBIT_BOUNDARY); break; }
+                    // ---------------- MoveOne: gather + push (MoveOne.f90:77-146, 213-219)
+                    if (ivel == 1) {
+                        int ci = n / m.ney, cj = n - ci * m.ney;
+                        R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
+                        R hxp = x2 - x1, hyp = y2 - y1;
+                        R dx = (X - x1) / hxp;
+                        R dy = (Y - y1) / hyp;
+                        R xm = (R)1 - dx, ym = (R)1 - dy;
+                        R W1, W2, W3, W4;
+                        if (m.delta == 0) {
+                            W1 = xm * ym; W2 = dx * ym; W3 = dx * dy; W4 = xm * dy;
+                        } else {
+                            R R1 = (R)m.ymin + y1, R2 = (R)m.ymin + y2, Rp = Y, den = R2 * R2 - R1 * R1;
+                            W1 = xm * (R2 * R2 - Rp * Rp) / den;
+                            W2 = dx * (R2 * R2 - Rp * Rp) / den;
+                            W3 = dx * (Rp * Rp - R1 * R1) / den;
+                            W4 = xm * (Rp * Rp - R1 * R1) / den;
+                        }
+                        int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+                        R Ex, Ey;
+                        if (m.element_index[n] > 0) {
+                            // Get_ParticleEField_PPR.f90:19-33: vacuum-side nodes only
+                            R ex = 0, ey = 0;
+                            int count = 0;
+                            int nd[4] = {n1, n2, n3, n4};
+                            R nx_[4] = {x1, x2, x2, x1}, ny_[4] = {y1, y1, y2, y2};
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                if (m.node_index[nd[k]] == -1) {
+                                    const R* c = a.coef5 + (size_t)nd[k] * 5;
+                                    count++;
+                                    R xr = X - nx_[k], yr = Y - ny_[k];
+                                    ex = ex + c[0] + (R)2 * c[2] * xr + c[3] * yr;
+                                    ey = ey + c[1] + (R)2 * c[4] * yr + c[3] * xr;
+                                }
+                            }
+                            ex = ex / (R)count;
+                            ey = ey / (R)count;
+                            Ex = -ex; Ey = -ey;
+                        } else {
+                            Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
+                            Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
+                        }
+                        if (m.delta == 0) {
+                            VX = VX + a.qm * Ex * T;
+                            VY = VY + a.qm * Ey * T;
+                        } else {
+                            // axisymmetric branch, MoveOne.f90:220-235 with Z carried as theta = 0
+                            VX = VX + a.qm * Ex * T;
+                            VY = VY + a.qm * Ey * T;
+                        }
+                        ivel = 0;
+                    }
+                    X = X + T * VX;
+                    Y = Y + T * VY;
+                    ipos = 0;
+                    // ---------------- AdjustParticleBoundary (MCCInterface.f90:361-499)
+                    if (X <= a.xmin) {
+                        ipos = 1;
+                        R TR = (X - a.xmin) / VX;
+                        T = TR;
+                        Y = Y - VY * TR;
+                        X = a.xmin + a.emit_eps;
+                        if (!rs_init) { pic_stream_init(&rs, a.seed, (uint32_t)p, a.step, a.tag_emit); rs_init = true; }
+                        double R1 = pic_stream_u01(&rs);
+                        // Kappa = 2: ranf**(-1/(Kappa-1)) == 1/ranf
+                        double vxs = sqrt(a.kappa_c * (1.0 / R1 - 1.0));
+                        VX = (R)(vxs * a.vref_inv);
+                        VY = (R)((double)VY * a.vref_inv);
+                        VZ = (R)((double)VZ * a.vref_inv);
+                        emitted = true;
+                    } else if (X > a.xmax) {
+                        X = (R)-2000;
+                        absorbed_wall = true;
+                    } else if (Y < a.ymin) {
+                        if (m.delta == 0) {
+                            ipos = 1;
+                            R TR = (Y - a.ymin) / VY;
+                            T = TR;
+                            X = X - VX * TR;
+                            VY = -VY;
+                            Y = a.ymin + a.emit_eps;
+                            reflected = true;
+                            if (Y > a.ymax) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                        } else {
+                            atomicOr(a.err, PIC_EBIT_WRONGCROSS);
+                        }
+                    } else if (Y > a.ymax) {
+                        if (m.delta == 0) {
+                            ipos = 1;
+                            R TR = (Y - a.ymax) / VY;
+                            T = TR;
+                            X = X - VX * TR;
+                            VY = -VY;
+                            Y = a.ymax - a.emit_eps;
+                            reflected = true;
+                            if (Y < a.ymin) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                        }
+                    }
+                    for (int k = 0; k < m.n_objects; ++k) {
+                        const DevObject& o = m.obj[k];
+                        double xd = (double)X, yd = (double)Y;
+                        bool in = false;
+                        if (o.shape == 3) in = xd > o.x0 && xd < o.x1 && yd > o.y0 && yd < o.y1;
+                        else if (o.shape == 1) in = (xd - o.x0) * (xd - o.x0) + (yd - o.y0) * (yd - o.y0) < o.r2;
+                        if (in) { X = (R)-2000; absorbed_obj = true; }
+                    }
+                }
+                if (X <= a.xmin) live = false;      // MoveAll deletion test, MCCInterface.f90:617-620
+                if (!(X == X) || !(Y == Y) || !(VX == VX) || !(VY == VY)) { atomicOr(a.err, PIC_EBIT_NONFINITE); live = false; }
+                a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
+                if (emitted) a.vz[p] = VZ;
+            }
+            if (live) {
+                int bad = 0;
+                e_new = pic_locate_elem((double)X, (double)Y, m, &bad);
+                if (bad) atomicOr(a.err, PIC_EBIT_POSITION);
+            }
+            a.cell_next[p] = live ? e_new : -1;
+        }
+    }
+    if (DO_PUSH) {
+        wave_count(&a.counters[PIC_C_EMIT], emitted);
+        wave_count(&a.counters[PIC_C_RIGHT], absorbed_wall && !live);
+        wave_count(&a.counters[PIC_C_OBJECT], absorbed_obj && !live);
+        wave_count(&a.counters[PIC_C_REFLECT], reflected);
+        wave_count(&a.counters[PIC_C_DELETED], p < a.n && a.cell[p] >= 0 && !live);
+    }
+    // ---------------- deposition (GetParChg_2D.f90:788-843), fixed point
+    long long v[8];
+    int key = -1;
+    if (live) {
+        double W[4];
+        double Xd = (double)X, Yd = (double)Y;
+        deposit_weights(Xd, Yd, e_new, m, W);
+        double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
+        double energy = a.energy_c * (vxd * vxd + vyd * vyd + vzd * vzd);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { v[k] = pic_fix(W[k]); v[4 + k] = pic_fix(W[k] * energy); }
+        key = e_new;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = 0;
+    }
+    wave_deposit(key, m, v, a.acc_w, a.acc_e);
+}
+
+template <typename R>
+static PushArgs<R> make_push_args(int isp) {
+    SpeciesHost& S = g.sp[isp];
+    PushArgs<R> a;
+    a.x = (R*)S.x; a.y = (R*)S.y; a.vx = (R*)S.vx; a.vy = (R*)S.vy; a.vz = (R*)S.vz;
+    a.cell = S.cell; a.cell_next = S.cell_next; a.n = S.nslots;
+    a.m = g.mesh;
+    a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
+    a.m.element_index = g.d_element_index; a.m.node_index = g.d_node_index;
+    if (sizeof(R) == 8) {
+        a.efx = (const R*)g.d_efx; a.efy = (const R*)g.d_efy; a.coef5 = (const R*)g.d_coef5;
+        a.xsR = (const R*)g.d_xs; a.ysR = (const R*)g.d_ys;
+    } else {
+        a.efx = (const R*)g.d_efxf; a.efy = (const R*)g.d_efyf; a.coef5 = (const R*)g.d_coeff;
+        a.xsR = (const R*)g.d_xsf; a.ysR = (const R*)g.d_ysf;
+    }
+    a.qm = (R)S.qm; a.dt = (R)g.cfg.dt;
+    a.xmin = (R)g.mesh.xmin; a.xmax = (R)g.mesh.xmax; a.ymin = (R)g.mesh.ymin; a.ymax = (R)g.mesh.ymax;
+    a.emit_eps = (R)(double)10.0e-6f;
+    a.vref_inv = 1.0 / g.v_ref;
+    double beta = S.p.mass_kg / (2.0 * PIC_KB * S.p.init_temperature_K);
+    a.kappa_c = (1.0 * 2.0 - 1.5) / beta;
+    a.energy_c = 0.5 * S.p.mass_kg * g.v_ref * g.v_ref / PIC_JTOEV;
+    a.seed = g.seed; a.step = (uint32_t)g.step; a.tag_emit = pic_stream_tag(isp, PIC_RNG_EMIT);
+    a.acc_w = S.acc_w; a.acc_e = S.acc_e;
+    a.counters = S.counters; a.err = g.d_err;
+    return a;
+}
+
+__global__ void k_zero_ll(long long* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int launch_zero_acc(int isp) {
+    SpeciesHost& S = g.sp[isp];
+    int64_t n = g.mesh.nnodes;
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_ll, dim3(blocks), dim3(256), 0, g.stream, S.acc_w, n);
+    hipLaunchKernelGGL(k_zero_ll, dim3(blocks), dim3(256), 0, g.stream, S.acc_e, n);
+    HIPCHK(hipGetLastError());
+    S.acc_dirty = false;
+    return PIC_OK;
+}
+
+int launch_push_deposit(int isp, bool do_push) {
+    SpeciesHost& S = g.sp[isp];
+    if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
+    if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
+    int blocks = (int)((S.nslots + 255) / 256);
+    if (g.particle_bits == 64) {
+        PushArgs<double> a = make_push_args<double>(isp);
+        if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(256), 0, g.stream, a);
+        else hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(256), 0, g.stream, a);
+    } else {
+        PushArgs<float> a = make_push_args<float>(isp);
+        if (do_push) hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(256), 0, g.stream, a);
+        else hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(256), 0, g.stream, a);
+    }
+    HIPCHK(hipGetLastError());
+    S.acc_dirty = true;
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------- finalisation
+// GetParChg_2D.f90:849-911: node sums are already formed (fixed point); convert, check
+// Ek != 0 with rho == 0 (the reference STOPs), normalise and total over species; the
+// accumulators are cleared for the next deposition.
+struct FinalArgs {
+    int nnodes, nspecies, delta;
+    long long* acc_w[PIC_MAX_SPECIES];
+    long long* acc_e[PIC_MAX_SPECIES];
+    double weight_qs[PIC_MAX_SPECIES];
+    double weight[PIC_MAX_SPECIES];
+    double L_ref, n_ref;
+    const double* vol;
+    double *rho, *rho_s, *rho_p, *ek;
+    unsigned int* err;
+};
+
+__global__ void k_finalize(FinalArgs a) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.nnodes) return;
+    double vol = a.vol[n];
+    double rho = 0.0;
+    for (int s = 0; s < a.nspecies; ++s) {
+        long long qw = a.acc_w[s][n], qe = a.acc_e[s][n];
+        a.acc_w[s][n] = 0; a.acc_e[s][n] = 0;
+        double w = (double)qw * PIC_FIX_INV;
+        double e = (double)qe * PIC_FIX_INV;
+        if (e != 0.0 && w == 0.0) atomicOr(a.err, PIC_EBIT_EKRHO);
+        double rs, ek;
+        if (a.delta == 0) {
+            rs = a.weight_qs[s] * w / a.L_ref / a.L_ref / a.n_ref / vol;
+            ek = a.weight[s] * e / a.L_ref / a.L_ref / a.n_ref / vol;
+        } else {
+            rs = a.weight_qs[s] * w / a.L_ref / a.L_ref / a.L_ref / a.n_ref / vol;
+            ek = a.weight[s] * e / a.L_ref / a.L_ref / a.L_ref / a.n_ref / vol;
+        }
+        size_t o = (size_t)s * a.nnodes + n;
+        a.rho_p[o] = w; a.rho_s[o] = rs; a.ek[o] = ek;
+        rho = rho + rs;
+    }
+    a.rho[n] = rho;
+}
+
+int launch_finalize_deposit() {
+    FinalArgs a;
+    a.nnodes = g.mesh.nnodes; a.nspecies = g.nspecies; a.delta = g.mesh.delta;
+    for (int s = 0; s < g.nspecies; ++s) {
+        SpeciesHost& S = g.sp[s];
+        if (S.acc_dirty == false) { int rc = launch_zero_acc(s); if (rc) return rc; }
+        a.acc_w[s] = S.acc_w; a.acc_e[s] = S.acc_e;
+        a.weight_qs[s] = S.p.weight * S.qs;     // Weight * qs(isp), evaluated left to right
+        a.weight[s] = S.p.weight;
+        S.acc_dirty = false;
+    }
+    a.L_ref = g.L_ref; a.n_ref = g.cfg.n_ref;
+    a.vol = g.d_cell_volume;
+    a.rho = g.d_rho; a.rho_s = g.d_rho_s; a.rho_p = g.d_rho_p; a.ek = g.d_ek;
+    a.err = g.d_err;
+    hipLaunchKernelGGL(k_finalize, dim3((a.nnodes + 255) / 256), dim3(256), 0, g.stream, a);
+    HIPCHK(hipGetLastError());
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------- batched drop-ins
+__global__ void k_locate_points(int64_t n, const double* x, const double* y, int* out, MeshParams m, int mode, unsigned int* err) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double X = x[i], Y = y[i];
+    if (mode == 0) {               // full GetParChg lookup
+        int bad = 0;
+        int e = pic_locate_elem(X, Y, m, &bad);
+        if (bad) atomicOr(err, PIC_EBIT_POSITION);
+        out[i] = e + 1;
+    } else if (mode == 1) {        // InjectPositioning only (no clamp, like the reference)
+        double rx = (X - m.xmin) / m.mapx;
+        int ii = (int)rx + 1;
+        double ry = (Y - m.ymin) / m.mapy;
+        int jj = (int)ry + 1;
+        if (ii < 1 || ii > m.nex || jj < 1 || jj > m.ney) { atomicOr(err, PIC_EBIT_POSITION); out[i] = 0; }
+        else out[i] = jj + (ii - 1) * m.ney;
+    }
+}
+
+int launch_locate_points(int64_t n, const double* dx, const double* dy, int* dout, int mode) {
+    MeshParams m = g.mesh;
+    m.xs = g.d_xs; m.ys = g.d_ys; m.element_index = g.d_element_index; m.node_index = g.d_node_index;
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_locate_points, dim3(blocks), dim3(256), 0, g.stream, n, dx, dy, dout, m, mode, g.d_err);
+    HIPCHK(hipGetLastError());
+    return PIC_OK;
+}
+
+// Get_ParticleEField_PPR for arbitrary points (FP64 coef_ppr).
+__global__ void k_ppr_gather_points(int64_t n, const double* x, const double* y, const int* ele,
+                                    double* ex_out, double* ey_out, MeshParams m, const double* coef6) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int e = ele[i] - 1;
+    if (e < 0 || e >= m.nelem) { ex_out[i] = 0; ey_out[i] = 0; return; }
+    int ci = e / m.ney, cj = e - ci * m.ney;
+    int nd[4] = {ci * m.nny + cj, (ci + 1) * m.nny + cj, (ci + 1) * m.nny + cj + 1, ci * m.nny + cj + 1};
+    double nx_[4] = {m.xs[ci], m.xs[ci + 1], m.xs[ci + 1], m.xs[ci]};
+    double ny_[4] = {m.ys[cj], m.ys[cj], m.ys[cj + 1], m.ys[cj + 1]};
+    double X = x[i], Y = y[i], Ex = 0.0, Ey = 0.0;
+    bool iface = m.element_index[e] > 0;
+    int count = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (iface && m.node_index[nd[k]] != -1) continue;
+        const double* c = coef6 + (size_t)nd[k] * 6;
+        count++;
+        double xr = X - nx_[k], yr = Y - ny_[k];
+        Ex = Ex + c[1] + 2.0 * c[3] * xr + c[4] * yr;
+        Ey = Ey + c[2] + 2.0 * c[5] * yr + c[4] * xr;
+    }
+    if (iface) { Ex = Ex / (double)count; Ey = Ey / (double)count; }
+    else { Ex = Ex / 4.0; Ey = Ey / 4.0; }
+    ex_out[i] = Ex; ey_out[i] = Ey;
+}
+
+int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, const int* ele, double* ex, double* ey) {
+    MeshParams m = g.mesh;
+    m.xs = g.d_xs; m.ys = g.d_ys; m.element_index = g.d_element_index; m.node_index = g.d_node_index;
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(k_ppr_gather_points, dim3(blocks), dim3(256), 0, g.stream, n, dx, dy, ele, ex, ey, m, g.d_coef);
+    HIPCHK(hipGetLastError());
+    return PIC_OK;
+}

@@ -1,0 +1,69 @@
+// Device helpers shared by the kernel translation units: element lookup, fixed-point
+// quantisation and wave-level segmented reduction.
+#pragma once
+#include "pic_state.h"
+#include "philox.h"
+
+// Element lookup of GetParChg_2D.f90:137-163 on the unrefined mesh, FP64 throughout:
+//   estimate i = INT((x - Vert_o) * hxi), j likewise, clamped at nx-1 / ny-1;
+//   if Mod(rxp, Mapxmin) > 1e-10 and Mod(ryp, Mapymin) > 1e-10 the InjectPositioning
+//   branch (InjectPositioning.f90:13-22: i = INT((x - dxmin)/dx) + 1, Element_Map is the
+//   identity without refinement); otherwise ParticlePositioning, which on an unrefined
+//   mesh without DG edges returns the estimate (ParticlePositioning.f90:240-363 with
+//   empty EdgeParent -> part_edge_count(1) == 0 -> Finalindex, GetParChg_2D.f90:776-778).
+// MOD of reals is taken as the exact remainder (fmod), which is what a == INT(a/p)*p
+// rounds to whenever p is a power of two. Returns the 0-based element, sets *bad when
+// the result falls outside the mesh.
+PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
+    double rxp = (X - m.vox) * m.hxi;
+    int i = (int)rxp;
+    double ryp = (Y - m.voy) * m.hyi;
+    int j = (int)ryp;
+    if (j == m.nny) j = j - 1;
+    if (i == m.nnx) i = i - 1;
+    double modx, mody;
+    if (m.pow2_map) {
+        modx = rxp - m.mapx * (double)(long long)(rxp / m.mapx);
+        mody = ryp - m.mapy * (double)(long long)(ryp / m.mapy);
+    } else {
+        modx = fmod(rxp, m.mapx);
+        mody = fmod(ryp, m.mapy);
+    }
+    if (modx > 1e-10 && mody > 1e-10) {
+        double rx = (X - m.xmin) / m.mapx;
+        i = (int)rx + 1;
+        double ry = (Y - m.ymin) / m.mapy;
+        j = (int)ry + 1;
+    }
+    if (i < 1 || i > m.nex || j < 1 || j > m.ney || !(X == X) || !(Y == Y)) {
+        *bad = 1;
+        i = i < 1 ? 1 : (i > m.nex ? m.nex : i);
+        j = j < 1 ? 1 : (j > m.ney ? m.ney : j);
+    }
+    return (j - 1) + (i - 1) * m.ney;
+}
+
+// Fixed-point quantum of deposition: value * 2^40 rounded to nearest (even), as int64.
+// Integer sums are exact and independent of order, so the node moments are
+// bit-identical whatever the particle order, atomic interleaving or kernel fusion.
+#if defined(__HIPCC__)
+__device__ inline long long pic_fix(double v) { return __double2ll_rn(v * PIC_FIX_SCALE); }
+#endif
+
+// Device error bits.
+enum {
+    PIC_EBIT_POSITION = 1u,
+    PIC_EBIT_BOUNDARY = 2u,
+    PIC_EBIT_NONFINITE = 4u,
+    PIC_EBIT_WRONGCROSS = 8u,
+    PIC_EBIT_EKRHO = 16u,
+    PIC_EBIT_OVERFLOW = 32u,
+    PIC_EBIT_CAPACITY = 64u,
+    PIC_EBIT_MCCLOOP = 128u
+};
+
+// Counter slots of SpeciesHost::counters.
+enum {
+    PIC_C_EMIT = 0, PIC_C_RIGHT = 1, PIC_C_OBJECT = 2, PIC_C_REFLECT = 3,
+    PIC_C_MCC_SELECTED = 4, PIC_C_MCC_REAL = 5, PIC_C_MCC_CREATED = 6, PIC_C_DELETED = 7
+};

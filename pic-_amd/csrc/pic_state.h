@@ -1,0 +1,224 @@
+// Internal state of the MI355X PIC hot-path library (not part of the C-ABI).
+//
+// Layout in HBM (one GPU):
+//   mesh      : node x / y coordinate lines (nnx, nny doubles + float copies),
+//               element_index (nelem int32), node_index (nnodes int32)
+//   fields    : phi (nnodes f64), efx/efy (nnodes f64 + f32 copies),
+//               coef (5 x nnodes f64 + f32: coef_ppr(2:6,:) node-major for the
+//               interface-element gather), rho / rho_s / rho_p / Ek_tot (f64)
+//   deposit   : per species two int64 fixed-point node accumulators (sum W, sum W*E)
+//   particles : per species SoA slots x, y, vx, vy, vz (f32 or f64), cell and
+//               cell_next (int32, 0-based element, -1 = dead slot)
+//   solver    : CSR of the stiffness on unknowns (f64 + f32 values), Jacobi diagonal,
+//               CSR of the RHS operator B (unknowns x nodes), constant c, PCG vectors
+//   PPR       : per node a recovery class id; per class the patch offsets and the
+//               6 x npatch least-squares rows (f64)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/pic_hip.h"
+
+#define PIC_MAX_OBJECTS 8
+#define PIC_MAX_SPECIES 4
+#define PIC_PATCH_MAX 50
+#define PIC_FIX_SHIFT 40                    // fixed-point scale 2^40 for deposition
+#define PIC_FIX_SCALE 1099511627776.0       // 2^40
+#define PIC_FIX_INV (1.0 / 1099511627776.0)
+
+// Physical constants of the reference (MCC_jw/code/base/Constants.f90:3-16).
+#define PIC_KB 1.3807e-23
+#define PIC_EXP_CONST 2.71828
+#define PIC_JTOEV 1.6022e-19
+#define PIC_EVTOK 11605.0
+#define PIC_PI 3.141592653589793238
+#define PIC_MINREAL 1.0e-15
+#define PIC_ELECTRON_MASS 9.1095e-31
+#define PIC_ELECTRON_CHARGE 1.6022e-19
+#define PIC_EPSILON0 8.8542e-12
+// Pi_LY of SetupGrids_2D_QLL / MoveOne (3.14159265358979D0).
+#define PIC_PI_LY 3.14159265358979
+
+struct DevObject {
+    int shape;      // 1 circle, 3 box
+    double x0, y0;  // circle centre or box lower-left
+    double x1, y1;  // box upper-right
+    double r2;      // circle radius squared
+};
+
+// Everything a particle kernel needs about the mesh, passed by value.
+struct MeshParams {
+    int nnx, nny;            // nodes (nx, ny in Domain_2D)
+    int nex, ney;            // elements per direction (nx-1, ny-1)
+    int nnodes, nelem;
+    int delta;
+    double xmin, xmax, ymin, ymax;     // dxmin..dymax
+    double hx, hy, hxi, hyi;            // hx(1:2), hxi
+    double vox, voy;                    // Vert_o(1:2) = min - h
+    double mapx, mapy;                  // hx / 2^r
+    int pow2_map;                       // 1 if mapx, mapy are powers of two (exact division)
+    const double* xs; const double* ys; // node coordinate lines
+    const float* xsf; const float* ysf;
+    const int* element_index;           // nelem
+    const int* node_index;              // nnodes
+    int n_objects;
+    DevObject obj[PIC_MAX_OBJECTS];
+};
+
+struct SpeciesHost {
+    bool defined = false;
+    pic_species_params p{};
+    double qs = 0, xm = 0, qm = 0;      // normalised (Input_2D.f90:479-483)
+    int64_t capacity = 0;
+    int64_t nslots = 0;                 // occupied slots (live + dead)
+    int64_t nlive = 0;
+    void* x = nullptr; void* y = nullptr;
+    void* vx = nullptr; void* vy = nullptr; void* vz = nullptr;
+    int* cell = nullptr; int* cell_next = nullptr;
+    long long* acc_w = nullptr;         // nnodes fixed-point sum W
+    long long* acc_e = nullptr;         // nnodes fixed-point sum W*E
+    bool acc_dirty = false;
+    bool pending = false;               // MoveAll has filled acc and cell_next
+    unsigned long long* counters = nullptr;  // device [16]
+    // MCC table
+    bool has_mcc = false;
+    int nreact = 0, nsigma = 0;
+    double emin = 0, eint = 0, emax = 0;
+    double sigma_max = 0, collision_ratio = 0;
+    uint32_t select_threshold = 0;      // floor(collision_ratio * 2^32)
+    int product_species = -1;           // ion species created by ionization
+    std::vector<double> prob_host;      // nreact x nsigma
+    std::vector<int> rtype_host;
+    std::vector<double> thr_host;
+    double* prob = nullptr;             // device, nreact x nsigma (column-major like Probility)
+    int* rtype = nullptr;
+    double* thr = nullptr;
+    unsigned long long* mcc_events = nullptr; // device [32]
+};
+
+struct PprClass {
+    int n;                        // patch size
+    int di[PIC_PATCH_MAX], dj[PIC_PATCH_MAX];
+    double R[6][PIC_PATCH_MAX];   // (A^T A)^{-1} A^T of the scaled patch
+    double h;                     // patch scale
+};
+
+struct PicState {
+    bool inited = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int particle_bits = 64;
+    int solver_mode = 0;
+    std::string err;
+
+    // domain
+    bool have_domain = false;
+    pic_domain_config cfg{};
+    MeshParams mesh{};
+    int nunknowns = 0;
+    double L_ref = 0, time_ref = 0, v_ref = 0, phi_ref = 0;
+
+    // host copies of set-up arrays (Fortran conventions where exported)
+    std::vector<double> h_xs, h_ys;
+    std::vector<int> h_element_index, h_node_index, h_node_type2, h_unknown_node;
+    std::vector<double> h_cell_volume, h_dirichlet_phi;
+    std::vector<int> h_A_rowptr, h_A_col, h_B_rowptr, h_B_col;
+    std::vector<double> h_A_val, h_A_diag, h_B_val, h_c;
+    std::vector<int> h_point_patch;   // nnodes x 50, Fortran order
+    std::vector<int> h_ppr_class;     // per node class id (-1 = no patch)
+    std::vector<PprClass> h_ppr_classes;
+    std::vector<unsigned char> h_coef_needed;
+
+    // device mesh arrays
+    double *d_xs = nullptr, *d_ys = nullptr;
+    float *d_xsf = nullptr, *d_ysf = nullptr;
+    int *d_element_index = nullptr, *d_node_index = nullptr;
+    double* d_cell_volume = nullptr;   // nnodes (row 2 of Cell_Volume_zwz)
+    double* d_dirichlet_phi = nullptr;
+    int* d_node_type2 = nullptr;
+    int* d_unknown_node = nullptr;     // 0-based node of each unknown
+
+    // fields
+    double *d_phi = nullptr, *d_efx = nullptr, *d_efy = nullptr, *d_coef = nullptr; // coef: 6 x nnodes
+    double* d_coef5 = nullptr;                                                         // c2..c6 node-major (5 x nnodes)
+    float *d_efxf = nullptr, *d_efyf = nullptr, *d_coeff = nullptr;                  // coeff: 5 x nnodes (c2..c6)
+    double *d_rho = nullptr, *d_rho_s = nullptr, *d_rho_p = nullptr, *d_ek = nullptr;
+
+    // PPR
+    int* d_ppr_class = nullptr;
+    PprClass* d_ppr_classes = nullptr;
+    int n_ppr_classes = 0;
+
+    // solver
+    int *d_A_rowptr = nullptr, *d_A_col = nullptr;
+    double* d_A_val = nullptr; float* d_A_valf = nullptr;
+    double* d_A_diag = nullptr; float* d_A_diagf = nullptr;
+    int *d_B_rowptr = nullptr, *d_B_col = nullptr;
+    double* d_B_val = nullptr; double* d_c = nullptr;
+    double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_p2 = nullptr, *d_q = nullptr;
+    float *d_bf = nullptr, *d_xf = nullptr, *d_rf = nullptr, *d_zf = nullptr, *d_pf = nullptr, *d_p2f = nullptr, *d_qf = nullptr;
+    double* d_scal = nullptr;          // PCG scalars (device)
+    double* d_partials = nullptr;      // block partial sums
+    unsigned int* d_sync = nullptr;    // last-block counters
+    double* h_scal_pinned = nullptr;
+    double pcg_tol = 1e-9;
+    int pcg_maxit = 50000;
+    int last_iter = 0, last_outer = 0;
+    double last_relres = 0;
+
+    // species
+    int nspecies = 0;
+    SpeciesHost sp[PIC_MAX_SPECIES];
+
+    // MCC / run
+    bool mcc_ready = false;
+    pic_gas_params gas{};
+    int e_species = 0, i_species = 1;
+    int sort_every = 20;
+    uint64_t seed = 12345;
+    int mcc_enabled = 1;
+    int64_t step = 0;
+    unsigned int* d_err = nullptr;     // device error flags
+    // scratch for selection / compaction
+    int* d_sel = nullptr; int64_t sel_cap = 0;
+    int* d_blockcount = nullptr; int64_t blockcount_cap = 0;
+    void* d_newbuf = nullptr; int64_t newbuf_cap = 0;
+    void* d_sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
+    void* d_sort_buf = nullptr; size_t sort_buf_bytes = 0;
+
+    hipEvent_t ev[16];
+    double phase_ms[8] = {0};
+};
+
+extern PicState g;
+
+// error helpers
+int pic_fail(int code, const std::string& msg);
+#define HIPCHK(call) do { hipError_t e_ = (call); if (e_ != hipSuccess) \
+    return pic_fail(PIC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); } while (0)
+
+// host-side setup (setup.cpp)
+int setup_build_domain(const pic_domain_config* cfg);
+int setup_build_ppr_classes();
+void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, double mass_kg, double dt_SI);
+void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
+                      std::vector<double>& x, std::vector<double>& y,
+                      std::vector<double>& vx, std::vector<double>& vy, std::vector<double>& vz,
+                      std::vector<int>& cell);
+
+// device launchers
+int launch_push_deposit(int isp, bool do_push);
+int launch_finalize_deposit();
+int launch_rhs();
+int launch_ppr();
+int run_pcg_fp64(bool reference_semantics);
+int run_pcg_mixed();
+int run_mcc();
+int run_sort_all();
+int launch_zero_acc(int isp);
+int launch_locate_points(int64_t n, const double* dx, const double* dy, int* dout, int mode);
+int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, const int* ele, double* ex, double* ey);
+int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,
+                     const double* b, double* x, double tol, int maxit, int* iters, int* ierr,
+                     double* r_out);

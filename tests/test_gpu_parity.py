@@ -1,0 +1,324 @@
+"""GPU parity: the HIP library through its C-ABI against the CPU oracle on identical inputs.
+
+Bars: element indices, counts, particle state of the mover and deposited node moments
+bit-exact; PPR coefficients, RHS and stiffness operators within 1e-12 relative; PCG
+solutions within 1e-10 relative of the FP64 oracle at FP64; MCC event counts exact and
+post-collision velocities within 1e-12 relative (transcendentals differ by ulps)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def mesh_of(pl, dom):
+    hp = pl.setup_array("HP")
+    xs = hp[0::2][:: dom.nny].copy()
+    ys = hp[1::2][: dom.nny].copy()
+    return xs, ys, pl.setup_array("ELEMENT_INDEX"), pl.setup_array("NODE_INDEX")
+
+
+@pytest.fixture(scope="module")
+def small(gpu_lib, pkg):
+    say("case: small24 setup")
+    w = pkg.build_workload(gpu_lib, "small24", seed=11, ppc=16, mcc=True)
+    gpu_lib.initial_field()
+    return w
+
+
+# ------------------------------------------------------------------ set-up arrays
+def test_setup_matches_restatements(gpu_lib, pkg, orc, small):
+    from patch_ref import element_and_node_classes, point_patches
+    pl, dom = gpu_lib, small["domain"]
+    say("case: setup arrays small24")
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    circles = [(o.x, o.y, o.r) for o in dom.objects]
+    ni_r, ei_r = element_and_node_classes(dom.nnx, dom.nny, xs, ys, circles)
+    assert np.array_equal(ni, ni_r) and np.array_equal(ei, ei_r)
+    pp_r, _ = point_patches(dom.nnx, dom.nny, ei_r)
+    assert np.array_equal(pl.setup_array("POINT_PATCH"), pp_r)
+    # stiffness on unknowns == oracle element-by-element operator
+    nt = pl.setup_array("NODE_TYPE").reshape(-1, 2)[:, 1]
+    rp, col, val = pl.setup_array("A_ROWPTR"), pl.setup_array("A_COL"), pl.setup_array("A_VAL")
+    nu = len(rp) - 1
+    rng = np.random.default_rng(0)
+    xu = rng.normal(size=nu)
+    y_lib = np.array([val[rp[i]:rp[i + 1]] @ xu[col[rp[i]:rp[i + 1]]] for i in range(nu)])
+    y_or = orc.stiffness_apply(xs, ys, 0, nt, xu)
+    np.testing.assert_allclose(y_lib, y_or, rtol=1e-12, atol=1e-12)
+    # rhs = B rho - c == oracle element loops with the Dirichlet lift
+    rho = rng.normal(size=dom.nnx * dom.nny)
+    brp, bcol, bval = pl.setup_array("B_ROWPTR"), pl.setup_array("B_COL"), pl.setup_array("B_VAL")
+    c = pl.setup_array("RHS_CONST")
+    b_lib = np.array([bval[brp[i]:brp[i + 1]] @ rho[bcol[brp[i]:brp[i + 1]]] for i in range(nu)]) - c
+    b_or = orc.rhs(xs, ys, 0, nt, pl.setup_array("DIRICHLET_PHI"), rho, nu)
+    np.testing.assert_allclose(b_lib, b_or, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------ mover
+@pytest.mark.parametrize("bits", [64, 32])
+def test_moveall_bit_exact(gpu_lib, pkg, orc, bits):
+    pl = gpu_lib
+    say(f"case: MoveAll bit-exact bits={bits}")
+    w = pkg.workload("small24")
+    dom = w["domain"]
+    pl.setup(dom, bits=bits, solver=0, tol=1e-12)
+    weight = pkg.species_weight(dom, 16)
+    nel = (dom.nnx - 1) * (dom.nny - 1)
+    pl.set_species(0, pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], weight, 2 * pkg.EVTOK, nel * 32)
+    pl.set_species(1, pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], weight, 0.026 * pkg.EVTOK, nel * 32)
+    pl.load_maxwellian(0, 16, 2.0, 5)
+    pl.load_maxwellian(1, 16, 0.026, 6)
+    pl.set_run_params(0, 77, 0)
+    pl.set_step(3)
+    pl.initial_field()
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    coef = pl.field("COEF_PPR").reshape(-1, 6)
+    efx, efy = pl.field("EFX"), pl.field("EFY")
+    # make the field strong enough to drive particles into walls and the object
+    efx *= 50.0
+    efy *= 50.0
+    pl.set_field("EFX", efx)
+    pl.set_field("EFY", efy)
+    coef[:, 1:] *= 50.0
+    pl.set_field("COEF_PPR", coef.ravel())
+    norm = dom.normalisation()
+    ft = np.float64 if bits == 64 else np.float32
+    for isp, (chg, mass, T) in enumerate([(pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], 2.0),
+                                          (pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], 0.026)]):
+        p0 = pl.particles(isp)
+        qm = (chg / pkg.ELECTRON_CHARGE) / (mass / pkg.ELECTRON_MASS)
+        res = orc.move_all(bits, xs, ys, ei, ni, dom.object_array(), efx.astype(ft), efy.astype(ft),
+                           coef[:, 1:].astype(ft).ravel(), qm, dom.dt, norm["v_ref"], mass, T * pkg.EVTOK, 77, 3,
+                           orc.tag(isp, orc.RNG_EMIT), p0["x"], p0["y"], p0["vx"], p0["vy"], p0["vz"], p0["loc"] - 1)
+        x, y, vx, vy, vz, alive, cnt, err = res
+        assert err == 0
+        pl.MoveAll(isp)
+        pl.GetParChg_2D()
+        p1 = pl.particles(isp)
+        keep = alive.astype(bool)
+        assert len(p1["x"]) == keep.sum()
+        for k, a in zip(("x", "y", "vx", "vy", "vz"), (x, y, vx, vy, vz)):
+            assert np.array_equal(p1[k], a[keep].astype(np.float64)), k
+        c = pl.counts(isp)
+        assert (c["emit"], c["right"], c["object"], c["reflect"], c["deleted"]) == \
+               tuple(int(v) for v in (cnt[0], cnt[1], cnt[2], cnt[3], cnt[7]))
+        loc, bad = orc.locate(xs, ys, x[keep].astype(np.float64), y[keep].astype(np.float64))
+        assert bad == 0 and np.array_equal(p1["loc"] - 1, loc)
+        say(f"  species {isp}: {keep.sum()} live, emitted {cnt[0]}, absorbed {cnt[1] + cnt[2]}, reflected {cnt[3]}")
+
+
+# ------------------------------------------------------------------ deposition
+@pytest.mark.parametrize("bits", [64, 32])
+def test_getparchg_moments(gpu_lib, pkg, orc, bits):
+    pl = gpu_lib
+    say(f"case: GetParChg moments bits={bits}")
+    w = pkg.build_workload(pl, "small24", seed=21, ppc=16, mcc=False)
+    dom = w["domain"]
+    pl.setup(dom, bits=bits, solver=0, tol=1e-12)
+    weight = pkg.species_weight(dom, 16)
+    nel = (dom.nnx - 1) * (dom.nny - 1)
+    pl.set_species(0, pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], weight, 2 * pkg.EVTOK, nel * 20)
+    pl.set_species(1, pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], weight, 0.026 * pkg.EVTOK, nel * 20)
+    pl.load_maxwellian(0, 16, 2.0, 21)
+    pl.load_maxwellian(1, 16, 0.026, 22)
+    pl.GetParChg_2D()
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    norm = dom.normalisation()
+    vol = pl.setup_array("CELL_VOLUME")[1::2]
+    nn = dom.nnx * dom.nny
+    rho_or = np.zeros(nn)
+    rho_s, rho_p, ek = pl.field("RHO_S"), pl.field("RHO_P"), pl.field("EK_TOT")
+    for isp, (chg, mass) in enumerate([(pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"]),
+                                       (pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"])]):
+        p = pl.particles(isp)
+        loc, bad = orc.locate(xs, ys, p["x"], p["y"])
+        assert bad == 0 and np.array_equal(loc, p["loc"] - 1)
+        ft = np.float64 if bits == 64 else np.float32
+        args = (bits, xs, ys, 0, p["x"].astype(ft), p["y"].astype(ft), p["vx"].astype(ft), p["vy"].astype(ft),
+                p["vz"].astype(ft), loc, mass, norm["v_ref"])
+        wq, eq = orc.deposit(*args, fix=True)
+        wf, ef = orc.deposit(*args, fix=False)
+        assert np.array_equal(rho_p[isp * nn:(isp + 1) * nn], wq)            # bit-exact
+        np.testing.assert_allclose(rho_p[isp * nn:(isp + 1) * nn], wf, rtol=1e-12, atol=1e-12)   # vs FP64 sums
+        rs, e = orc.normalize(0, weight, chg / pkg.ELECTRON_CHARGE, norm["L_ref"], dom.n_ref, vol, wq, eq, rho_or)
+        assert np.array_equal(rho_s[isp * nn:(isp + 1) * nn], rs)
+        assert np.array_equal(ek[isp * nn:(isp + 1) * nn], e)
+        assert wq.sum() == pytest.approx(len(p["x"]), rel=1e-12)
+    assert np.array_equal(pl.field("RHO"), rho_or)
+
+
+# ------------------------------------------------------------------ PPR
+def test_ppr_matches_oracle(gpu_lib, pkg, orc, small):
+    pl, dom = gpu_lib, small["domain"]
+    say("case: PPR small24")
+    pkg.build_workload(pl, "small24", seed=11, ppc=16, mcc=False)
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    rng = np.random.default_rng(8)
+    phi = rng.normal(size=dom.nnx * dom.nny)
+    pl.set_field("PHI", phi)
+    pl.GetEField_SIDG_PPR()
+    coef_or, efx_or, efy_or, ns = orc.ppr(xs, ys, pl.setup_array("POINT_PATCH"), phi)
+    assert ns == 0
+    np.testing.assert_allclose(pl.field("COEF_PPR"), coef_or, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(pl.field("EFX"), efx_or, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(pl.field("EFY"), efy_or, rtol=1e-12, atol=1e-12)
+    # Get_ParticleEField_PPR drop-in
+    e0 = rng.integers(0, (dom.nnx - 1) * (dom.nny - 1), 500)
+    ci, cj = np.divmod(e0, dom.nny - 1)
+    px, py = xs[ci] + rng.random(500), ys[cj] + rng.random(500)
+    ex, ey = pl.Get_ParticleEField_PPR(px, py, e0 + 1)
+    ex_o, ey_o = orc.ppr_gather(xs, ys, ei, ni, pl.field("COEF_PPR"), px, py, e0)
+    np.testing.assert_allclose(ex, ex_o, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(ey, ey_o, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------------ PCG
+def test_pcg_fp64_matches_my_pcg(gpu_lib, pkg, orc):
+    pl = gpu_lib
+    say("case: PCG fp64 small40")
+    w = pkg.workload("small40")
+    dom = w["domain"]
+    pl.setup(dom, bits=64, solver=0, tol=1e-12)
+    rng = np.random.default_rng(12)
+    nn = dom.nnx * dom.nny
+    rho = rng.normal(size=nn)
+    phi = pl.setup_array("DIRICHLET_PHI").copy()
+    pl.IFE_Solve_2D(Rho=rho, Phi=phi)
+    info = pl.last_solve()
+    rp, col, val = pl.setup_array("A_ROWPTR"), pl.setup_array("A_COL"), pl.setup_array("A_VAL")
+    diag = pl.setup_array("A_DIAG")
+    b = pl.field("RHS")
+    un = pl.setup_array("UNKNOWN_NODE") - 1
+    x_or, r_or, it_or, ierr = orc.pcg(rp, col, val, diag, b, np.zeros(len(b)), 1e-12, 50000)
+    assert ierr == 0
+    assert abs(info["iterations"] - (it_or + 1)) <= 1
+    np.testing.assert_allclose(phi[un], x_or, rtol=1e-10, atol=1e-10 * np.abs(x_or).max())
+    assert info["relres"] < 1e-11
+    # MY_JPCG_Solver drop-in on the reference's SPARSE layout
+    mat, nz = pkg.sparse_from_csr(rp, col, val)
+    R = b.copy()
+    X = np.zeros(len(b))
+    it = pl.MY_JPCG_Solver(mat, R, X, diag.copy(), nz, len(b))
+    assert abs(it - it_or) <= 1
+    np.testing.assert_allclose(X, x_or, rtol=1e-10, atol=1e-10 * np.abs(x_or).max())
+
+
+def test_pcg_mixed_precision_refinement(gpu_lib, pkg):
+    pl = gpu_lib
+    say("case: PCG mixed small64")
+    w = pkg.workload("small64")
+    dom = w["domain"]
+    nn = dom.nnx * dom.nny
+    rng = np.random.default_rng(3)
+    rho = rng.normal(size=nn)
+    pl.setup(dom, bits=32, solver=1, tol=1e-8)
+    phi32 = pl.setup_array("DIRICHLET_PHI").copy()
+    pl.IFE_Solve_2D(Rho=rho, Phi=phi32)
+    info = pl.last_solve()
+    assert info["relres"] <= 1e-8 and info["outer"] >= 1
+    pl.setup(dom, bits=64, solver=0, tol=1e-13)
+    phi64 = pl.setup_array("DIRICHLET_PHI").copy()
+    pl.IFE_Solve_2D(Rho=rho, Phi=phi64)
+    # stated bound for the mixed solve: ||phi32 - phi64|| / ||phi64|| <= 1e-6 at a 1e-8 residual
+    assert np.linalg.norm(phi32 - phi64) / np.linalg.norm(phi64) < 1e-6
+    say(f"  mixed: {info}")
+
+
+# ------------------------------------------------------------------ MCC
+def test_mcc_selection_and_events_match(gpu_lib, pkg, orc):
+    pl = gpu_lib
+    say("case: MCC small24 (dense gas)")
+    w = pkg.build_workload(pl, "small24", seed=31, ppc=16, mcc=True, gas_density=3.2e25)
+    pl.set_run_params(0, 31, 1)
+    pl.set_step(5)
+    pl.GetParChg_2D()
+    norm = w["domain"].normalisation()
+    before = [pl.particles(s) for s in (0, 1)]
+    tabs = []
+    for s, nr in ((0, 3), (1, 2)):
+        prob, smax, cr = pl.mcc_table(s, nr)
+        tabs.append((prob, cr))
+    pl.MCC()
+    after = [pl.particles(s) for s in (0, 1)]
+    masses = (pkg.ELECTRON["mass_kg"], pkg.ARGON_ION["mass_kg"])
+    specs = {0: dict(nr=3, emin=-3.0, rtype=[11, 12, 13], thr=[0.0, 11.55, 15.76]),
+             1: dict(nr=2, emin=-1.0, rtype=[101, 103], thr=[0.0, 0.0])}
+    new_e, new_i = [], []
+    for s in (0, 1):
+        prob, cr = tabs[s]
+        thr = np.uint32(min(np.floor(cr * 2 ** 32), 2 ** 32 - 1))
+        b = before[s]
+        sel = orc.mcc_select(b["loc"] - 1, int(thr), 31, 5, orc.tag(s, orc.RNG_SELECT))
+        assert pl.counts(s)["mcc_selected"] == len(sel) and len(sel) > 20
+        sp = specs[s]
+        ev = np.zeros(sp["nr"], dtype=np.int64)
+        v_exp = np.stack([b["vx"], b["vy"], b["vz"]], 1).copy()
+        for k in sel:
+            idx, v, ne, ni = orc.mcc_collide(1 if s == 0 else 0, int(k), 31, 5, orc.tag(s, orc.RNG_COLLIDE),
+                                             norm["v_ref"], masses[s], masses[1], 300.0, sp["nr"], 10001, sp["emin"],
+                                             (3.0 - sp["emin"]) / 10000, 1e3, prob, sp["rtype"], sp["thr"], v_exp[k])
+            v_exp[k] = v
+            if idx:
+                ev[idx - 1] += 1
+            if s == 0 and idx == 3:
+                new_e.append(ne)
+                new_i.append(ni)
+        assert np.array_equal(pl.mcc_events(s, sp["nr"]), ev), (s, ev)
+        a = after[s]
+        n0 = len(b["x"])
+        np.testing.assert_allclose(np.stack([a["vx"], a["vy"], a["vz"]], 1)[:n0], v_exp, rtol=1e-12, atol=1e-15)
+        say(f"  species {s}: selected {len(sel)}, events {ev.tolist()}")
+    ne_lib = np.stack([after[0]["vx"], after[0]["vy"], after[0]["vz"]], 1)[len(before[0]["x"]):]
+    ni_lib = np.stack([after[1]["vx"], after[1]["vy"], after[1]["vz"]], 1)[len(before[1]["x"]):]
+    assert len(ne_lib) == len(new_e) and len(ni_lib) == len(new_i)
+    if new_e:
+        np.testing.assert_allclose(ne_lib, np.array(new_e), rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(ni_lib, np.array(new_i), rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------------------ drop-ins
+def test_positioning_dropins(gpu_lib, pkg, orc):
+    pl = gpu_lib
+    say("case: positioning drop-ins")
+    w = pkg.workload("small24")
+    dom = w["domain"]
+    pl.setup(dom, bits=64, solver=0)
+    xs = np.arange(dom.nnx, dtype=float)
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.random(2000) * 24, np.arange(25.0), np.full(25, 7.0)])
+    y = np.concatenate([rng.random(2000) * 24, np.full(25, 3.0), np.arange(25.0)])
+    e_lib = pl.locate(x, y)
+    e_or, bad = orc.locate(xs, xs, x, y)
+    assert bad == 0 and np.array_equal(e_lib - 1, e_or)
+    inner = (x > 0.5) & (x < 23.5) & (y > 0.5) & (y < 23.5) & (np.mod(x, 1) > 1e-9) & (np.mod(y, 1) > 1e-9)
+    ei = pl.InjectPositioning(x[inner], y[inner])
+    assert np.array_equal(ei - 1, e_or[inner])
+    ne, tf = pl.ParticlePositioning(x, y, e_lib)
+    assert np.array_equal(ne, e_lib)
+    on_inner_edge = ((np.mod(x, 1) == 0) & (x > 0) & (x < 24)) | ((np.mod(y, 1) == 0) & (y > 0) & (y < 24))
+    corner = (np.mod(x, 1) == 0) & (np.mod(y, 1) == 0)
+    assert np.array_equal(tf.astype(bool), on_inner_edge & ~corner)
+    em = pl.generate_Elementmap(0)
+    assert np.array_equal(em, (np.arange(24)[None, :] + 1) + np.arange(24)[:, None] * 24)
+
+
+# ------------------------------------------------------------------ full step determinism
+def test_full_step_deterministic_and_conserving(gpu_lib, pkg):
+    pl = gpu_lib
+    say("case: full step determinism small32")
+    outs = []
+    for rep in range(2):
+        w = pkg.build_workload(pl, "small32", seed=41, ppc=16, mcc=True, gas_density=3.2e24)
+        pl.set_run_params(3, 41, 1)
+        pl.set_step(0)
+        pl.initial_field()
+        pl.step(7)
+        outs.append((pl.field("PHI"), pl.field("RHO"), pl.particles(0)["x"], pl.particles(1)["vx"]))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    c0 = pl.counts(0)
+    assert c0["nlive"] == len(outs[1][2])

@@ -1,0 +1,390 @@
+"""CPU tests: pin the oracle against published vectors, the reference's own fixture
+files (copied as data under tests/golden/) and what the mathematics fixes; check the
+host logic of the package and that the C-ABI library exports every declared symbol."""
+import ctypes
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+# ------------------------------------------------------------------ Philox
+def test_philox_known_answers(orc):
+    # Random123 kat_vectors, philox4x32_10
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, out in kat:
+        assert tuple(int(v) for v in orc.philox(ctr, key)) == out
+
+
+# ------------------------------------------------------------------ normalisation
+def test_normalisation_matches_reference_output(pkg):
+    """OUTPUT/physics_parameter.inp:40-50 of the reference (n_ref=1e21, T_ref=1 eV)."""
+    gold = json.loads((GOLD / "physics_parameter.json").read_text())
+    d = pkg.Domain(3, 3, n_ref=1e21, T_ref_eV=1.0)
+    n = d.normalisation()
+    for k in ("L_ref", "time_ref", "v_ref", "phi_ref"):
+        assert n[k] == pytest.approx(gold[k], rel=1e-14), k
+
+
+# ------------------------------------------------------------------ BRINV
+def test_brinv_matches_numpy_inverse(orc):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        a = rng.normal(size=(6, 6))
+        inv, ok = orc.brinv(a)
+        assert ok == 1
+        np.testing.assert_allclose(inv, np.linalg.inv(a), rtol=1e-10, atol=1e-12)
+    # a matrix that needs row and column pivoting (zero leading diagonal)
+    a = np.eye(6)[[3, 0, 5, 1, 4, 2]] * np.arange(1, 7)
+    inv, ok = orc.brinv(a)
+    np.testing.assert_allclose(inv @ a, np.eye(6), atol=1e-14)
+    # singular
+    _, ok = orc.brinv(np.ones((6, 6)))
+    assert ok == 0
+
+
+# ------------------------------------------------------------------ element matrices
+def test_element_matrices_closed_form(orc):
+    hx, hy = 1.0, 0.5
+    K, M = orc.element_matrices(0, 0.0, 0.0, hx, hy)
+    Mx = np.array([[2, -2, -1, 1], [-2, 2, 1, -1], [-1, 1, 2, -2], [1, -1, -2, 2]]) / 6.0
+    My = np.array([[2, 1, -1, -2], [1, 2, -2, -1], [-1, -2, 2, 1], [-2, -1, 1, 2]]) / 6.0
+    np.testing.assert_allclose(K, hy / hx * Mx + hx / hy * My, rtol=1e-14, atol=1e-15)
+    P = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) * hx * hy / 36.0
+    np.testing.assert_allclose(M, P, rtol=1e-14)
+    # rows of K sum to zero (constants are in the kernel); M sums to the area
+    assert np.abs(K.sum(1)).max() < 1e-14 and M.sum() == pytest.approx(hx * hy)
+    # axisymmetric: M integrates r over the element
+    K1, M1 = orc.element_matrices(1, 0.0, 2.0, 1.0, 1.0)
+    assert M1.sum() == pytest.approx(2.5)
+
+
+# ------------------------------------------------------------------ patches and PPR
+def _mesh(n, circles=()):
+    from patch_ref import element_and_node_classes, point_patches
+    xs = np.arange(n + 1, dtype=np.float64)
+    ys = np.arange(n + 1, dtype=np.float64)
+    ni, ei = element_and_node_classes(n + 1, n + 1, xs, ys, circles)
+    pp, ptype = point_patches(n + 1, n + 1, ei)
+    return xs, ys, ni, ei, pp, ptype
+
+
+def test_patch_sizes_hand_derived():
+    n = 6
+    xs, ys, ni, ei, pp, ptype = _mesh(n)
+    nn = (n + 1) ** 2
+    cnt = pp[:nn]
+    nid = lambda i, j: i * (n + 1) + j
+    assert cnt[nid(0, 0)] == 9            # corner: 3x3 block at the corner
+    assert cnt[nid(0, 3)] == 15           # left edge: 3 columns x 5 rows
+    assert cnt[nid(0, 1)] == 12           # next to a corner: 3 x 4
+    assert cnt[nid(3, 3)] == 9            # interior: 3x3
+    assert cnt[nid(3, n)] == 15           # top edge
+
+
+def test_ppr_reproduces_quadratics_exactly(orc):
+    """Polynomial preservation: for phi quadratic, every patch fit is exact, so coef_ppr holds
+    the Taylor coefficients at each node and Get_ParticleEField_PPR returns grad phi."""
+    n = 8
+    xs, ys, ni, ei, pp, ptype = _mesh(n, circles=[(4.0, 4.0, 1.6)])
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    a = [0.3, -1.2, 0.7, 0.25, -0.4, 0.15]
+    phi = (a[0] + a[1] * X + a[2] * Y + a[3] * X * X + a[4] * X * Y + a[5] * Y * Y).ravel()
+    coef, efx, efy, ns = orc.ppr(xs, ys, pp, phi)
+    assert ns == 0
+    coef = coef.reshape(-1, 6)
+    Xf, Yf = X.ravel(), Y.ravel()
+    has = pp[: len(Xf)] > 0
+    np.testing.assert_allclose(coef[has, 1], (a[1] + 2 * a[3] * Xf + a[4] * Yf)[has], atol=1e-11)
+    np.testing.assert_allclose(coef[has, 2], (a[2] + a[4] * Xf + 2 * a[5] * Yf)[has], atol=1e-11)
+    np.testing.assert_allclose(coef[has, 3], a[3], atol=1e-11)
+    np.testing.assert_allclose(coef[has, 4], a[4], atol=1e-11)
+    np.testing.assert_allclose(coef[has, 5], a[5], atol=1e-11)
+    np.testing.assert_allclose(efx[has], -coef[has, 1])
+    # point gather on vacuum elements returns grad phi exactly
+    rng = np.random.default_rng(3)
+    e0 = np.flatnonzero(ei == -1)[:50]
+    ci, cj = np.divmod(e0, n)
+    px, py = ci + rng.random(len(e0)), cj + rng.random(len(e0))
+    ex, ey = orc.ppr_gather(xs, ys, ei, ni, coef.ravel(), px, py, e0)
+    np.testing.assert_allclose(ex, a[1] + 2 * a[3] * px + a[4] * py, atol=1e-10)
+    np.testing.assert_allclose(ey, a[2] + a[4] * px + 2 * a[5] * py, atol=1e-10)
+
+
+def test_ppr_interior_closed_form(orc):
+    """On a uniform interior 3x3 patch the least-squares gradient is the averaged central
+    difference E_x = -(1/(6h)) sum_eta (phi[i+1,j+eta] - phi[i-1,j+eta])."""
+    n = 6
+    xs, ys, ni, ei, pp, _ = _mesh(n)
+    rng = np.random.default_rng(5)
+    phi = rng.normal(size=(n + 1) ** 2)
+    coef, efx, efy, _ = orc.ppr(xs, ys, pp, phi)
+    P = phi.reshape(n + 1, n + 1)
+    i, j = 3, 2
+    ex = -(P[i + 1, j - 1:j + 2] - P[i - 1, j - 1:j + 2]).sum() / 6.0
+    ey = -(P[i - 1:i + 2, j + 1] - P[i - 1:i + 2, j - 1]).sum() / 6.0
+    assert efx[i * (n + 1) + j] == pytest.approx(ex, rel=1e-12)
+    assert efy[i * (n + 1) + j] == pytest.approx(ey, rel=1e-12)
+
+
+# ------------------------------------------------------------------ PCG
+def _poisson_system(orc, n):
+    xs = np.arange(n + 1, dtype=float)
+    ys = xs.copy()
+    nnx = n + 1
+    nt2 = np.zeros(nnx * nnx, dtype=np.int32)
+    u = 0
+    for i in range(nnx):
+        for j in range(nnx):
+            if i in (0, n) or j in (0, n):
+                nt2[i * nnx + j] = -1
+            else:
+                u += 1
+                nt2[i * nnx + j] = u
+    # assemble A from the oracle's element-by-element operator (columns = unit vectors)
+    A = np.zeros((u, u))
+    for k in range(u):
+        e = np.zeros(u)
+        e[k] = 1.0
+        A[:, k] = orc.stiffness_apply(xs, ys, 0, nt2, e)
+    return xs, ys, nt2, A
+
+
+def test_pcg_matches_direct_solve(orc):
+    xs, ys, nt2, A = _poisson_system(orc, 12)
+    n = A.shape[0]
+    rows, cols = np.nonzero(np.abs(A) > 0)
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    rowptr = np.cumsum(rowptr).astype(np.int32)
+    val = A[rows, cols]
+    b = np.random.default_rng(2).normal(size=n)
+    x, r, it, ierr = orc.pcg(rowptr, cols, val, np.diag(A).copy(), b, np.zeros(n), 1e-13, 1000)
+    assert ierr == 0 and it > 0
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-11)
+    # the returned residual is the recursively updated one and is small
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-13
+    # IERR = 1 at MAXITER, ITER left unset
+    _, _, it2, ierr2 = orc.pcg(rowptr, cols, val, np.diag(A).copy(), b, np.zeros(n), 1e-30, 3)
+    assert ierr2 == 1 and it2 == -1
+
+
+def test_stiffness_interior_stencil(orc):
+    xs, ys, nt2, A = _poisson_system(orc, 6)
+    # Q1 Laplacian on squares: 8/3 on the diagonal, -1/3 to all eight neighbours
+    row = A[A.shape[0] // 2]
+    assert row.max() == pytest.approx(8.0 / 3.0)
+    assert sorted(np.round(row[row != 0] * 3).astype(int).tolist()) == [-1] * 8 + [8]
+
+
+def test_rhs_linear_and_mass_consistent(orc):
+    n = 8
+    xs = np.arange(n + 1, dtype=float)
+    nnx = n + 1
+    nt2 = np.array([(-1 if (i in (0, n) or j in (0, n)) else 1) for i in range(nnx) for j in range(nnx)], dtype=np.int32)
+    nt2[nt2 > 0] = np.arange(1, (nt2 > 0).sum() + 1)
+    nu = int(nt2.max())
+    rng = np.random.default_rng(4)
+    r1, r2 = rng.normal(size=nnx * nnx), rng.normal(size=nnx * nnx)
+    z = np.zeros(nnx * nnx)
+    b1 = orc.rhs(xs, xs, 0, nt2, z, r1, nu)
+    b2 = orc.rhs(xs, xs, 0, nt2, z, r2, nu)
+    b12 = orc.rhs(xs, xs, 0, nt2, z, 2.0 * r1 - r2, nu)
+    np.testing.assert_allclose(b12, 2.0 * b1 - b2, atol=1e-12)
+    # constant rho = 1: interior rows integrate N_i = h^2
+    b1 = orc.rhs(xs, xs, 0, nt2, z, np.ones(nnx * nnx), nu)
+    np.testing.assert_allclose(b1, 1.0, rtol=1e-14)
+
+
+# ------------------------------------------------------------------ push / boundaries
+def _uniform_field_case(orc, bits, n_part=200, E=(0.3, -0.2), seed=11):
+    n = 8
+    xs, ys, ni, ei, pp, _ = _mesh(n)
+    nn = (n + 1) ** 2
+    efx, efy = np.full(nn, E[0]), np.full(nn, E[1])
+    coef5 = np.zeros(nn * 5)
+    rng = np.random.default_rng(seed)
+    x = 2.0 + 4.0 * rng.random(n_part)
+    y = 2.0 + 4.0 * rng.random(n_part)
+    vx, vy, vz = rng.normal(size=(3, n_part)) * 0.5
+    cell, bad = orc.locate(xs, ys, x, y)
+    assert bad == 0
+    return xs, ys, ni, ei, efx, efy, coef5, x, y, vx, vy, vz, cell
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_push_uniform_field_closed_form(orc, bits):
+    xs, ys, ni, ei, efx, efy, coef5, x, y, vx, vy, vz, cell = _uniform_field_case(orc, bits)
+    qm, dt = -1.0, 0.05
+    ft = np.float64 if bits == 64 else np.float32
+    out = orc.move_all(bits, xs, ys, ei, ni, np.zeros((0, 6)), efx, efy, coef5, qm, dt, 4e5, 9.1e-31, 2 * 11605.0,
+                       1, 0, 0, x, y, vx, vy, vz, cell)
+    X, Y, VX, VY, VZ, alive, cnt, err = out
+    assert err == 0 and alive.all() and cnt.sum() == 0
+    # v += qm E dt and x += v dt in the working precision, with the weights summing to one
+    ev = ft(qm) * ft(0.3) * ft(dt)
+    np.testing.assert_allclose(VX, vx.astype(ft) + ev, rtol=0, atol=4 * np.finfo(ft).eps)
+    np.testing.assert_allclose(X, x.astype(ft) + ft(dt) * VX, rtol=0, atol=16 * np.finfo(ft).eps)
+    np.testing.assert_array_equal(VZ, vz.astype(ft))
+
+
+def test_push_walls(orc):
+    n = 8
+    xs, ys, ni, ei, pp, _ = _mesh(n)
+    nn = (n + 1) ** 2
+    z = np.zeros(nn)
+    # particles heading to the right wall, the bottom wall, the top wall and the left wall
+    x = np.array([7.9, 4.0, 4.0, 0.05])
+    y = np.array([4.0, 0.02, 7.97, 4.0])
+    vx = np.array([4.0, 0.0, 0.0, -2.0])
+    vy = np.array([0.0, -1.0, 1.0, 0.5])
+    vz = np.array([0.0, 0.0, 0.0, 0.3])
+    cell, _ = orc.locate(xs, ys, x, y)
+    v_ref = 419396.657327663
+    X, Y, VX, VY, VZ, alive, cnt, err = orc.move_all(64, xs, ys, ei, ni, np.zeros((0, 6)), z, z, np.zeros(5 * nn), -1.0,
+                                                     0.05, v_ref, 9.1095e-31, 2 * 11605.0, 3, 0, orc.tag(0, 2),
+                                                     x, y, vx, vy, vz, cell)
+    assert err == 0
+    assert alive.tolist() == [0, 1, 1, 1]                       # right wall absorbs (X = -2000, deleted)
+    assert X[0] == -2000.0 and cnt[1] == 1 and cnt[7] == 1
+    # bottom: specular reflection, y re-moved from dymin + 10E-6 with the remaining time
+    t_rem = (0.02 - 0.05 - 0.0) / -1.0
+    assert VY[1] == 1.0 and Y[1] == pytest.approx(float(np.float32(1e-5)) + t_rem * 1.0, abs=1e-15)
+    assert VY[2] == -1.0 and cnt[3] == 2
+    # left wall: kappa re-emission, Vy and Vz divided by v_ref again (VelRes of all three)
+    assert VX[3] > 0 and cnt[0] == 1
+    assert VY[3] == pytest.approx(0.5 / v_ref) and VZ[3] == pytest.approx(0.3 / v_ref)
+
+
+def test_push_object_absorbs(orc):
+    n = 8
+    xs, ys, ni, ei, pp, _ = _mesh(n, circles=[(4.0, 4.0, 1.5)])
+    nn = (n + 1) ** 2
+    z = np.zeros(nn)
+    x, y = np.array([2.0, 2.0]), np.array([4.0, 7.0])
+    vx, vy, vz = np.array([30.0, 30.0]), np.zeros(2), np.zeros(2)
+    cell, _ = orc.locate(xs, ys, x, y)
+    obj = np.array([[1, 4.0, 4.0, 0, 0, 2.25]])
+    out = orc.move_all(64, xs, ys, ei, ni, obj, z, z, np.zeros(5 * nn), -1.0, 0.05, 4e5, 9.1e-31, 2 * 11605.0,
+                       1, 0, 0, x, y, vx, vy, vz, cell)
+    assert out[5].tolist() == [0, 1] and out[6][2] == 1
+
+
+# ------------------------------------------------------------------ lookup
+def test_locate_brute_force(orc):
+    n = 5
+    xs = np.arange(n + 1, dtype=float) * 2.0     # hx = 2: power of two
+    ys = np.arange(n + 1, dtype=float) * 2.0
+    rng = np.random.default_rng(9)
+    x = rng.random(500) * 10.0
+    y = rng.random(500) * 10.0
+    # exact grid points and the far walls
+    x = np.concatenate([x, [0.0, 2.0, 4.0, 10.0, 10.0, 3.0]])
+    y = np.concatenate([y, [0.0, 2.0, 5.0, 10.0, 3.0, 10.0]])
+    e, bad = orc.locate(xs, ys, x, y)
+    assert bad == 0
+    i = np.minimum(np.floor(x / 2.0).astype(int), n - 1)
+    j = np.minimum(np.floor(y / 2.0).astype(int), n - 1)
+    np.testing.assert_array_equal(e, j + i * n)
+
+
+# ------------------------------------------------------------------ deposition
+@pytest.mark.parametrize("bits", [64, 32])
+def test_deposit_conservation_and_fixed_point(orc, bits):
+    n = 6
+    xs = np.arange(n + 1, dtype=float)
+    rng = np.random.default_rng(4)
+    m = 3000
+    x, y = rng.random(m) * n, rng.random(m) * n
+    v = rng.normal(size=(3, m))
+    cell, _ = orc.locate(xs, xs, x, y)
+    wq, eq = orc.deposit(bits, xs, xs, 0, x, y, *v, cell, 9.1095e-31, 4.19e5, fix=True)
+    wf, ef = orc.deposit(bits, xs, xs, 0, x, y, *v, cell, 9.1095e-31, 4.19e5, fix=False)
+    assert wq.sum() == pytest.approx(m, rel=1e-12)                    # weights sum to one per particle
+    np.testing.assert_allclose(wq, wf, rtol=1e-12, atol=1e-11)       # fixed point == FP64 sums
+    np.testing.assert_allclose(eq, ef, rtol=1e-11, atol=1e-10)
+    # order independence of the fixed-point sums
+    perm = rng.permutation(m)
+    wp, ep = orc.deposit(bits, xs, xs, 0, x[perm], y[perm], *(v[:, perm]), cell[perm], 9.1095e-31, 4.19e5, fix=True)
+    assert np.array_equal(wp, wq) and np.array_equal(ep, eq)
+
+
+def test_deposit_particle_on_node(orc):
+    xs = np.arange(5, dtype=float)
+    w, e = orc.deposit(64, xs, xs, 0, np.array([2.0]), np.array([3.0]), np.zeros(1), np.zeros(1), np.zeros(1),
+                       np.array([2 * 4 + 3 - 0]), 1.0, 1.0, fix=True)
+    # the particle sits on the lower-left node of element (i=3, j=4) in 1-based terms
+    assert w[2 * 5 + 3] == 1.0 and w.sum() == 1.0
+
+
+# ------------------------------------------------------------------ MCC
+def test_mcc_table_cumulative(orc):
+    ns, nr = 101, 3
+    sig = np.abs(np.random.default_rng(0).normal(size=(ns, nr))) * 1e-20
+    prob, smax = orc.mcc_table(nr, ns, -3.0, 0.06, 9.1e-31, 6.6e-26, 3e20, sig.ravel())
+    P = prob.reshape(ns, nr)
+    assert P.max() == pytest.approx(1.0) and np.all(np.diff(P, axis=1) >= 0)
+    E = 10 ** (-3.0 + 0.06 * np.arange(ns))
+    miu = 9.1e-31 * 6.6e-26 / (9.1e-31 + 6.6e-26)
+    nu = (sig.sum(1) * np.sqrt(2 * E * 1.6022e-19 / miu) * 3e20)
+    assert smax == pytest.approx(nu.max(), rel=1e-12)
+    np.testing.assert_allclose(P[:, -1], nu / smax, rtol=1e-12)
+
+
+def test_mcc_charge_exchange_and_excitation_kinematics(orc):
+    # one-reaction tables forcing each event
+    ns = 11
+    prob = np.ones(ns)
+    v = np.array([1e-3, -2e-3, 0.5e-3])
+    # charge exchange: the ion takes the gas target velocity
+    idx, vout, _, _ = orc.mcc_collide(0, 5, 9, 1, orc.tag(1, 4), 4e5, 6.6e-26, 6.6e-26, 300.0, 1, ns, -1.0, 0.4, 1e3,
+                                      prob, [103], [0.0], v)
+    assert idx == 1 and np.linalg.norm(vout * 4e5) < 2000.0   # thermal speed of 300 K argon
+    # excitation of a fast electron: |v| drops by the threshold energy in the centre of mass
+    ve = np.array([8.0, 0.0, 0.0])   # normalised (~32 eV)
+    idx, vout, _, _ = orc.mcc_collide(1, 7, 9, 1, orc.tag(0, 4), 4.19396657e5, 9.1095e-31, 6.69e-26, 300.0, 1, ns,
+                                      -3.0, 0.6, 1e3, prob, [12], [11.55], ve)
+    e_in = 0.5 * 9.1095e-31 * (8.0 * 4.19396657e5) ** 2 / 1.6022e-19
+    e_out = 0.5 * 9.1095e-31 * np.sum((vout * 4.19396657e5) ** 2) / 1.6022e-19
+    assert e_in - e_out == pytest.approx(11.55, abs=0.05)
+    # ionisation creates a secondary electron whose energy completes the budget
+    idx, vout, ne, ni = orc.mcc_collide(1, 8, 9, 1, orc.tag(0, 4), 4.19396657e5, 9.1095e-31, 6.69e-26, 300.0, 1, ns,
+                                        -3.0, 0.6, 1e3, prob, [13], [15.76], ve)
+    e_out = 0.5 * 9.1095e-31 * np.sum((vout * 4.19396657e5) ** 2) / 1.6022e-19
+    e_sec = 0.5 * 9.1095e-31 * np.sum((ne * 4.19396657e5) ** 2) / 1.6022e-19
+    assert e_in - e_out - e_sec == pytest.approx(15.76, abs=0.05)
+    assert np.linalg.norm(ni * 4.19396657e5) < 2000.0
+
+
+def test_mcc_selection_rate(orc):
+    n = 200000
+    cell = np.zeros(n, dtype=np.int32)
+    thr = int(0.01 * 2 ** 32)
+    sel = orc.mcc_select(cell, thr, 5, 3, orc.tag(0, 3))
+    assert abs(len(sel) - 2000) < 5 * np.sqrt(2000)
+    assert np.all(np.diff(sel) > 0)
+
+
+# ------------------------------------------------------------------ library exports
+def test_library_exports_every_declared_symbol(pkg):
+    lib = ctypes.CDLL(str(pkg.LIB_PATH))
+    syms = pkg.header_symbols()
+    assert len(syms) > 30
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python prototypes cover the header exactly
+    assert sorted(pkg._PROTOS) == syms
+
+
+def test_workload_definitions(pkg):
+    w = pkg.workload("cfg3")
+    assert w["domain"].nnx == 513 and len(w["domain"].objects) == 3 and w["ppc"] == 64
+    assert (w["domain"].nnx - 1) ** 2 * w["ppc"] == 16777216
+    w1 = pkg.workload("cfg1")
+    assert (w1["domain"].nnx - 1) ** 2 * w1["ppc"] == 409600
