@@ -112,14 +112,18 @@ static void mk_mesh(or_mesh* m, int nnx, int nny, const double* xs, const double
 
 /* ------------------------------------------------------------------ MoveAll */
 #define REAL double
+#define FMOD fmod
 #define FN(name) or_##name##_f64
 #include "oracle_push.inc"
 #undef REAL
+#undef FMOD
 #undef FN
 #define REAL float
+#define FMOD fmodf
 #define FN(name) or_##name##_f32
 #include "oracle_push.inc"
 #undef REAL
+#undef FMOD
 #undef FN
 
 int or_move_all(int bits, int nnx, int nny, const double* xs, const double* ys, const int* element_index,

@@ -38,10 +38,21 @@ static int check_err_flags() {
     HIPCHK(hipMemcpyAsync(&e, g.d_err, sizeof(unsigned int), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     if (!e) return PIC_OK;
+    PicErrInfo ei;
+    HIPCHK(hipMemcpy(&ei, g.d_errinfo, sizeof(ei), hipMemcpyDeviceToHost));
     HIPCHK(hipMemset(g.d_err, 0, sizeof(unsigned int)));
-    if (e & PIC_EBIT_NONFINITE) return pic_fail(PIC_ERR_NONFINITE, "non-finite particle state");
-    if (e & PIC_EBIT_POSITION) return pic_fail(PIC_ERR_POSITION, "particle located outside the mesh");
-    if (e & (PIC_EBIT_BOUNDARY | PIC_EBIT_WRONGCROSS)) return pic_fail(PIC_ERR_BOUNDARY, "particle boundary loop did not settle");
+    HIPCHK(hipMemset(g.d_errinfo, 0, sizeof(PicErrInfo)));
+    std::string where;
+    if (ei.taken) {
+        char buf[320];
+        snprintf(buf, sizeof buf, " [step %lld, species %d, slot %lld: x=%.17g y=%.17g vx=%.17g vy=%.17g iterations=%g "
+                 "element %g -> %g]", (long long)g.step, (int)ei.v[7], ei.slot, ei.v[0], ei.v[1], ei.v[2], ei.v[3],
+                 ei.v[4], ei.v[5], ei.v[6]);
+        where = buf;
+    }
+    if (e & PIC_EBIT_NONFINITE) return pic_fail(PIC_ERR_NONFINITE, "non-finite particle state" + where);
+    if (e & (PIC_EBIT_BOUNDARY | PIC_EBIT_WRONGCROSS)) return pic_fail(PIC_ERR_BOUNDARY, "particle boundary loop did not settle" + where);
+    if (e & PIC_EBIT_POSITION) return pic_fail(PIC_ERR_POSITION, "particle located outside the mesh" + where);
     if (e & PIC_EBIT_EKRHO) return pic_fail(PIC_ERR_POSITION, "Ek_tot /= 0 with rho == 0 (GetParChg_2D.f90:887-890)");
     if (e & PIC_EBIT_MCCLOOP) return pic_fail(PIC_ERR_BOUNDARY, "MCC rejection loop did not end");
     return pic_fail(PIC_ERR_HIP, "device error flag " + std::to_string(e));
@@ -77,11 +88,43 @@ static void to_host_double(const void* dsrc, int64_t n, std::vector<double>& out
     out.assign(tmp.begin(), tmp.end());
 }
 
-__global__ void k_count_live2(int64_t n, const int* cell, unsigned long long* out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool l = i < n && cell[i] >= 0;
-    unsigned long long b = __ballot(l);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
+// live-slot count: per-block counts, then a one-block sum (no same-address atomics)
+__global__ __launch_bounds__(256) void k_count_live_blocks(int64_t n, const int* cell, int* blockcount) {
+    __shared__ int sh[4];
+    int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    int c = 0;
+    for (int k = 0; k < 4; ++k) { int64_t i = base + k * 256; if (i < n && cell[i] >= 0) c++; }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) blockcount[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+__global__ __launch_bounds__(1024) void k_sum_counts(int nb, const int* blockcount, long long* out) {
+    __shared__ long long sh[1024];
+    long long s = 0;
+    for (int i = threadIdx.x; i < nb; i += 1024) s += blockcount[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) { if (threadIdx.x < (unsigned)off) sh[threadIdx.x] += sh[threadIdx.x + off]; __syncthreads(); }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// number of slots with cell >= 0 among the first n
+int count_live_slots(int64_t n, const int* cell, int64_t* out) {
+    if (n == 0) { *out = 0; return PIC_OK; }
+    int nb = (int)((n + 1023) / 1024);
+    int* bc = nullptr;
+    HIPCHK(hipMalloc(&bc, (size_t)nb * sizeof(int) + 16));
+    long long* d = (long long*)g.d_partials;
+    hipLaunchKernelGGL(k_count_live_blocks, dim3(nb), dim3(256), 0, g.stream, n, cell, bc);
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, g.stream, nb, bc, d);
+    HIPCHK(hipGetLastError());
+    long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    (void)hipFree(bc);
+    *out = (int64_t)v;
+    return PIC_OK;
 }
 
 __global__ void k_impose_dirichlet(int n, const int* nt2, const double* dphi, double* phi) {
@@ -95,6 +138,14 @@ __global__ void k_coef_split(int nn, const double* coef6, double* coef5, float* 
     for (int r = 0; r < 5; ++r) { double v = coef6[(size_t)n * 6 + r + 1]; coef5[(size_t)n * 5 + r] = v; coeff[(size_t)n * 5 + r] = (float)v; }
 }
 __global__ void k_d2f_n(int64_t n, const double* a, float* b) { int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = (float)a[i]; }
+
+static void free_species(int s) {
+    SpeciesHost& S = g.sp[s];
+    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob, S.rtype,
+                    S.thr, S.mcc_events};
+    for (void* p : ptrs) if (p) hipFree(p);
+    S = SpeciesHost();
+}
 
 extern "C" {
 
@@ -123,6 +174,8 @@ int pic_init(int device) {
     HIPCHK(hipMemset(g.d_sync, 0, 64 * sizeof(unsigned int)));
     HIPCHK(hipMalloc((void**)&g.d_err, sizeof(unsigned int)));
     HIPCHK(hipMemset(g.d_err, 0, sizeof(unsigned int)));
+    HIPCHK(hipMalloc(&g.d_errinfo, sizeof(PicErrInfo)));
+    HIPCHK(hipMemset(g.d_errinfo, 0, sizeof(PicErrInfo)));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
     g.inited = true;
     return PIC_OK;
@@ -139,20 +192,18 @@ int pic_finalize(void) {
 int pic_set_precision(int particle_bits, int solver_mode) {
     if (particle_bits != 32 && particle_bits != 64) return pic_fail(PIC_ERR_ARG, "particle_bits must be 32 or 64");
     if (solver_mode != 0 && solver_mode != 1) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0 or 1");
-    for (int s = 0; s < PIC_MAX_SPECIES; ++s)
-        if (g.sp[s].defined && particle_bits != g.particle_bits) return pic_fail(PIC_ERR_STATE, "species already allocated");
+    if (particle_bits != g.particle_bits) {
+        // particle storage changes width: existing species are discarded
+        if (g.inited) HIPCHK(hipStreamSynchronize(g.stream));
+        for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
+        g.nspecies = 0;
+        g.mcc_ready = false;
+    }
     g.particle_bits = particle_bits;
     g.solver_mode = solver_mode;
     return PIC_OK;
 }
 
-static void free_species(int s) {
-    SpeciesHost& S = g.sp[s];
-    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob, S.rtype,
-                    S.thr, S.mcc_events};
-    for (void* p : ptrs) if (p) hipFree(p);
-    S = SpeciesHost();
-}
 
 int pic_setup_domain(const pic_domain_config* cfg) {
     REQUIRE_INIT();
@@ -205,8 +256,41 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     TRY(dalloc(&g.d_p, nu)); TRY(dalloc(&g.d_p2, nu)); TRY(dalloc(&g.d_q, nu));
     TRY(dalloc(&g.d_bf, nu)); TRY(dalloc(&g.d_xf, nu)); TRY(dalloc(&g.d_rf, nu)); TRY(dalloc(&g.d_zf, nu));
     TRY(dalloc(&g.d_pf, nu)); TRY(dalloc(&g.d_p2f, nu)); TRY(dalloc(&g.d_qf, nu));
+    // node-space PCG: the nine diagonals of the stiffness (node offsets), padded vectors
+    {
+        const int off[9] = {-m.nny - 1, -m.nny, -m.nny + 1, -1, 0, 1, m.nny - 1, m.nny, m.nny + 1};
+        std::vector<std::vector<double>> dia(9, std::vector<double>(m.nnodes, 0.0));
+        for (int u = 0; u < nu; ++u) {
+            int nu_node = g.h_unknown_node[u] - 1;
+            for (int k = g.h_A_rowptr[u]; k < g.h_A_rowptr[u + 1]; ++k) {
+                int nv = g.h_unknown_node[g.h_A_col[k]] - 1;
+                int d = nv - nu_node, kk = -1;
+                for (int q = 0; q < 9; ++q) if (off[q] == d) kk = q;
+                if (kk < 0) return pic_fail(PIC_ERR_UNSUPPORTED, "stiffness entry off the nine-point stencil");
+                dia[kk][nu_node] = g.h_A_val[k];
+            }
+        }
+        for (int q = 0; q < 9; ++q) {
+            g.dia_off[q] = off[q];
+            TRY(dupload(&g.d_dia64[q], dia[q]));
+            std::vector<float> f(dia[q].begin(), dia[q].end());
+            TRY(dupload(&g.d_dia32[q], f));
+        }
+        size_t pad = ((size_t)m.nny + 2 + 63) / 64 * 64;
+        size_t len = (size_t)m.nnodes + 2 * pad;
+        if (g.d_pcgvec) { (void)hipFree(g.d_pcgvec); g.d_pcgvec = nullptr; }
+        HIPCHK(hipMalloc(&g.d_pcgvec, len * 7 * (sizeof(double) + sizeof(float))));
+        HIPCHK(hipMemset(g.d_pcgvec, 0, len * 7 * (sizeof(double) + sizeof(float))));
+        double* d0 = (double*)g.d_pcgvec;
+        double** dv[7] = {&g.d_bn, &g.d_xn, &g.d_rn, &g.d_zn, &g.d_pn, &g.d_p2n, &g.d_qn};
+        for (int k = 0; k < 7; ++k) *dv[k] = d0 + k * len + pad;
+        float* f0 = (float*)(d0 + 7 * len);
+        float** fv[7] = {&g.d_bfn, &g.d_xfn, &g.d_rfn, &g.d_zfn, &g.d_pfn, &g.d_p2fn, &g.d_qfn};
+        for (int k = 0; k < 7; ++k) *fv[k] = f0 + k * len + pad;
+        g.pcg_pstride = (m.nnodes + 255) / 256 + 1;
+    }
     size_t nblk = (size_t)std::max(m.nnodes, nu) / 256 + 2;
-    TRY(dalloc(&g.d_partials, 4 * nblk + 64));
+    TRY(dalloc(&g.d_partials, std::max<size_t>(4 * nblk + 64, (size_t)4 * g.pcg_pstride + 64)));
     g.have_domain = true;
     return PIC_OK;
 }
@@ -407,14 +491,8 @@ int pic_get_counts(int isp, int64_t* nlive, int64_t* nslots, int64_t counters[8]
     if (counters) for (int k = 0; k < 8; ++k) counters[k] = (int64_t)h[k];
     if (nslots) *nslots = S.nslots;
     if (nlive) {
-        unsigned long long* d = (unsigned long long*)g.d_partials;
-        HIPCHK(hipMemset(d, 0, 8));
         const int* c = S.pending ? S.cell_next : S.cell;
-        if (S.nslots) hipLaunchKernelGGL(k_count_live2, dim3((int)((S.nslots + 255) / 256)), dim3(256), 0, g.stream, S.nslots, c, d);
-        unsigned long long v = 0;
-        HIPCHK(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, g.stream));
-        HIPCHK(hipStreamSynchronize(g.stream));
-        *nlive = (int64_t)v;
+        TRY(count_live_slots(S.nslots, c, nlive));
     }
     return PIC_OK;
 }
@@ -446,7 +524,7 @@ int pic_MoveAll(int isp, int delta, int PushFlag) {
 static int commit_locations() {
     for (int s = 0; s < g.nspecies; ++s) {
         SpeciesHost& S = g.sp[s];
-        if (!S.defined) continue;
+        if (!S.defined) return pic_fail(PIC_ERR_STATE, "species indices must be contiguous from 0");
         if (!S.pending) TRY(launch_push_deposit(s, false));
         std::swap(S.cell, S.cell_next);
         S.pending = false;
@@ -711,7 +789,7 @@ int pic_MY_JPCG_Solver(const pic_sparse* matrix, double* R, double* X, const dou
     // scratch for partial sums sized to this system
     double* save_partials = g.d_partials;
     double* part = nullptr;
-    HIPCHK(hipMalloc(&part, (4 * ((size_t)N / 256 + 2) + 64) * sizeof(double)));
+    HIPCHK(hipMalloc(&part, std::max<size_t>(4 * ((size_t)N / 256 + 2) + 64, 4096) * sizeof(double)));
     g.d_partials = part;
     int iters = 0, ierr = 0;
     int rc = pcg_generic_fp64(N, drp, dcol, dval, ddiag, db, dx, g.pcg_tol, g.pcg_maxit, &iters, &ierr, dr);

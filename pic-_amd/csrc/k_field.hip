@@ -2,32 +2,33 @@
 // Jacobi-preconditioned conjugate gradient of MY_PCG (FP64), with an FP32 inner solve
 // inside FP64 iterative refinement.
 //
-// PCG layout: CSR with int32 columns, one row per thread. Each iteration is two kernels:
+// PCG layout: the nine stiffness diagonals in node space (or CSR for arbitrary matrices),
+// one row per thread. Each iteration is two kernels:
 //   k_pcg_spmv : p <- z + beta p (recomputed on the fly for the 9-point neighbourhood),
 //                q <- A p, partial p.q
 //   k_pcg_update: x += alpha p, r -= alpha q, z = r / diag, partial r.z and r.r
-// Block partials are reduced by the last block to finish (fixed block order, so the
-// scalars are deterministic); that block also forms alpha / beta and the convergence
-// flag. Kernels of an iteration after convergence return at once, so the host checks
-// the flag only every few iterations.
+// Kernels of an iteration after convergence return at once, so the host checks the flag
+// only every few iterations.
 #include "pic_device.h"
 
-#define PCG_BLOCK 256
-
 // ---------------------------------------------------------------- RHS
-__global__ void k_rhs(int n, const int* rowptr, const int* col, const double* val,
-                      const double* rho, const double* c, double* b) {
+// rhs_u = (B rho)_u - c_u for every unknown u, stored at its node (node-space PCG) and
+// in unknown order (the IFE_Solve_2D rhs vector).
+__global__ void k_rhs(int n, const int* rowptr, const int* col, const double* val, const double* rho,
+                      const double* c, const int* unknown_node, double* b, double* bn) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s = 0.0;
     for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * rho[col[k]];
-    b[i] = s - c[i];
+    double v = s - c[i];
+    b[i] = v;
+    bn[unknown_node[i]] = v;
 }
 
 int launch_rhs() {
     int n = g.nunknowns;
     hipLaunchKernelGGL(k_rhs, dim3((n + 255) / 256), dim3(256), 0, g.stream, n, g.d_B_rowptr, g.d_B_col,
-                       g.d_B_val, g.d_rho, g.d_c, g.d_b);
+                       g.d_B_val, g.d_rho, g.d_c, g.d_unknown_node, g.d_b, g.d_bn);
     HIPCHK(hipGetLastError());
     return PIC_OK;
 }
@@ -85,11 +86,43 @@ int launch_ppr() {
 }
 
 // ---------------------------------------------------------------- PCG
+// No workgroup of a kernel waits for another: the block partial sums of one kernel are
+// reduced in the prologue of the next kernel by every block (same operands, same fixed
+// order, so every block and every run gets the same scalars). This avoids the device-scope
+// fences and same-address atomics that a last-block reduction costs on a multi-XCD GPU.
+// One row per thread, PCG_BLOCK-thread blocks; partial regions are g.pcg_pstride long.
+#define PCG_BLOCK 256
+
 // scalar slots in d_scal
-enum { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_BETA, S_NORMRHS, S_NORMR, S_DONE, S_ITER, S_IERR, S_TOL2, S_MAXIT, S_PQ, S_NSCAL };
+enum { S_RHO0 = 0, S_RHO1, S_ALPHA, S_BETA, S_NORMRHS, S_NORMR, S_DONE, S_ITER, S_IERR, S_TOL2, S_MAXIT, S_PQ, S_NSCAL };
+// partial-sum regions
+enum { P_RZ = 0, P_RR = 1, P_BB = 2, P_PQ = 3, P_NREG = 4 };
+
+// Operators: CSR (any matrix, the MY_JPCG_Solver drop-in) and the nine diagonals of the
+// Q1 stiffness on the structured mesh in node numbering (offsets -nny-1 ... nny+1). Both
+// sum a row's terms in ascending column order, so they round identically; the DIA
+// operator adds the zero terms of Dirichlet neighbours (their vector entries are zero).
+template <typename T> struct CsrOp {
+    const int* rowptr; const int* col; const T* val;
+    template <class F> __device__ inline T row(int i, F pj) const {
+        T s = 0;
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * pj(col[k]);
+        return s;
+    }
+};
+template <typename T> struct DiaOp {
+    const T* c[9];
+    int off[9];
+    template <class F> __device__ inline T row(int i, F pj) const {
+        T s = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s = s + c[k][i] * pj(i + off[k]);
+        return s;
+    }
+};
 
 template <int NV>
-__device__ inline void block_reduce_store(double v[NV], double* partials, int nblocks) {
+__device__ inline void block_partials(double v[NV], double* partials, int pstride, const int regs[NV]) {
     __shared__ double sh[NV][PCG_BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -104,196 +137,202 @@ __device__ inline void block_reduce_store(double v[NV], double* partials, int nb
         for (int k = 0; k < NV; ++k) {
             double s = 0.0;
             for (int w = 0; w < PCG_BLOCK / 64; ++w) s += sh[k][w];
-            partials[(size_t)k * nblocks + blockIdx.x] = s;
+            partials[(size_t)regs[k] * pstride + blockIdx.x] = s;
         }
     }
 }
 
-// Returns true in thread 0 of the last block to arrive (after all partials are visible).
-__device__ inline bool last_block(unsigned int* counter) {
-    __shared__ bool am_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = atomicAdd(counter, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    return am_last;
-}
-
-__device__ inline double sum_partials(const double* partials, int nblocks) {
+// Sum of the `nb` partials of region `reg` written by the previous kernel, in a fixed
+// order; every thread of every block gets the same value.
+__device__ inline double reduce_region(const double* partials, int pstride, int reg, int nb) {
+    __shared__ double red[PCG_BLOCK];
+    const double* pr = partials + (size_t)reg * pstride;
     double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += partials[b];
-    return s;
+    for (int b = threadIdx.x; b < nb; b += PCG_BLOCK) s += pr[b];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = PCG_BLOCK / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < (unsigned)off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    double t = red[0];
+    __syncthreads();
+    return t;
 }
 
-// r = b - A x, z = r / diag, partials of r.z and b.b
-template <typename T>
-__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_init(int n, const int* rowptr, const int* col, const T* val,
-                                                        const T* diag, const T* b, const T* x, T* r, T* z,
-                                                        double* scal, double* partials, unsigned int* counter) {
+// r = b - A x, z = r / diag (0 where diag == 0: Dirichlet rows of the DIA operator);
+// partials of r.z and b.b
+template <typename T, class Op>
+__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_init(int n, Op A, const T* diag, const T* b, const T* x, T* r, T* z,
+                                                        double* scal, double* partials, int pstride) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        T s = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
+        T s = A.row(i, [&](int j) { return x[j]; });
         T ri = b[i] - s;
-        T zi = ri / diag[i];
+        T d = diag[i];
+        T zi = d != (T)0 ? ri / d : (T)0;
         r[i] = ri; z[i] = zi;
         v[0] = (double)ri * (double)zi;
         v[1] = (double)b[i] * (double)b[i];
     }
-    block_reduce_store<2>(v, partials, gridDim.x);
-    if (last_block(counter) && threadIdx.x == 0) {
-        scal[S_RHO] = sum_partials(partials, gridDim.x);
-        scal[S_NORMRHS] = sum_partials(partials + gridDim.x, gridDim.x);
-        scal[S_BETA] = 0.0;
-        scal[S_DONE] = 0.0; scal[S_ITER] = 0.0; scal[S_IERR] = 0.0;
-        if (scal[S_NORMRHS] == 0.0) scal[S_DONE] = 1.0;   // zero RHS: x = 0 is exact
-        *counter = 0;
-    }
+    const int regs[2] = {P_RZ, P_BB};
+    block_partials<2>(v, partials, pstride, regs);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_DONE] = 0.0; scal[S_ITER] = 0.0; scal[S_IERR] = 0.0; }
 }
 
-template <typename T>
-__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_spmv(int n, const int* rowptr, const int* col, const T* val,
-                                                        const T* z, const T* p_old, T* p_new, T* q,
-                                                        double* scal, double* partials, unsigned int* counter) {
+// Iteration k (0-based) of MY_PCG.f90:73-127, first half: rho_k = r.z (previous kernel's
+// partials), convergence test of iteration k-1 (||r||^2/||b||^2 <= tol^2 with b the RHS),
+// p = z + (rho_k/rho_{k-1}) p (p = z when k == 0), q = A p, partials of p.q.
+template <typename T, class Op>
+__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_spmv(int n, int k, int nb, Op A, const T* z, const T* p_old,
+                                                        T* p_new, T* q, double* scal, double* partials, int pstride) {
     if (scal[S_DONE] != 0.0) return;
+    double rho = reduce_region(partials, pstride, P_RZ, nb);
+    double beta = 0.0;
+    if (k == 0) {
+        double bb = reduce_region(partials, pstride, P_BB, nb);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[S_NORMRHS] = bb;
+            if (bb == 0.0) scal[S_DONE] = 1.0;      // zero RHS: x = 0 is exact
+        }
+        if (bb == 0.0) return;
+    } else {
+        double normr = reduce_region(partials, pstride, P_RR, nb);
+        double rho_old = scal[(k - 1) & 1 ? S_RHO1 : S_RHO0];
+        bool conv = normr / scal[S_NORMRHS] <= scal[S_TOL2];
+        bool maxed = (double)k >= scal[S_MAXIT];
+        if (conv || maxed || rho_old == 0.0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                scal[S_DONE] = 1.0; scal[S_ITER] = (double)k; scal[S_NORMR] = normr;
+                scal[S_IERR] = maxed ? 1.0 : (conv ? 0.0 : 2.0);   // MY_PCG.f90:88-90, 129-131
+            }
+            return;
+        }
+        beta = rho / rho_old;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[k & 1 ? S_RHO1 : S_RHO0] = rho; scal[S_ITER] = (double)k; }
+    const T tb = (T)beta;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const T beta = (T)scal[S_BETA];
-    const bool first = scal[S_ITER] == 0.0;
     double v[1] = {0.0};
     if (i < n) {
-        T s = 0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            int j = col[k];
-            T pj = first ? z[j] : z[j] + beta * p_old[j];
-            s = s + val[k] * pj;
-        }
-        T pi = first ? z[i] : z[i] + beta * p_old[i];
+        T s;
+        if (k == 0) s = A.row(i, [&](int j) { return z[j]; });
+        else s = A.row(i, [&](int j) { return z[j] + tb * p_old[j]; });
+        T pi = (k == 0) ? z[i] : z[i] + tb * p_old[i];
         p_new[i] = pi; q[i] = s;
         v[0] = (double)pi * (double)s;
     }
-    block_reduce_store<1>(v, partials, gridDim.x);
-    if (last_block(counter) && threadIdx.x == 0) {
-        double pq = sum_partials(partials, gridDim.x);
-        scal[S_PQ] = pq;
-        if (pq == 0.0) { scal[S_IERR] = 3.0; scal[S_DONE] = 1.0; scal[S_ALPHA] = 0.0; }
-        else scal[S_ALPHA] = scal[S_RHO] / pq;
-        *counter = 0;
-    }
+    const int regs[1] = {P_PQ};
+    block_partials<1>(v, partials, pstride, regs);
 }
 
+// Second half: alpha = rho_k / p.q, x += alpha p, r -= alpha q, z = r / diag; partials of
+// r.z and r.r for the next iteration.
 template <typename T>
-__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_update(int n, const T* diag, const T* p, const T* q, T* x, T* r, T* z,
-                                                          double* scal, double* partials, unsigned int* counter) {
+__global__ __launch_bounds__(PCG_BLOCK) void k_pcg_update(int n, int k, int nb, const T* diag, const T* p, const T* q,
+                                                          T* x, T* r, T* z, double* scal, double* partials, int pstride) {
     if (scal[S_DONE] != 0.0) return;
+    double pq = reduce_region(partials, pstride, P_PQ, nb);
+    if (pq == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_DONE] = 1.0; scal[S_IERR] = 3.0; scal[S_ITER] = (double)k; }
+        return;
+    }
+    double alpha = scal[k & 1 ? S_RHO1 : S_RHO0] / pq;
+    const T ta = (T)alpha;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const T alpha = (T)scal[S_ALPHA];
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        T xi = x[i] + alpha * p[i];
-        T ri = r[i] - alpha * q[i];
-        T zi = ri / diag[i];
+        T xi = x[i] + ta * p[i];
+        T ri = r[i] - ta * q[i];
+        T d = diag[i];
+        T zi = d != (T)0 ? ri / d : (T)0;
         x[i] = xi; r[i] = ri; z[i] = zi;
         v[0] = (double)ri * (double)zi;
         v[1] = (double)ri * (double)ri;
     }
-    block_reduce_store<2>(v, partials, gridDim.x);
-    if (last_block(counter) && threadIdx.x == 0) {
-        double rho_new = sum_partials(partials, gridDim.x);
-        double normr = sum_partials(partials + gridDim.x, gridDim.x);
-        double it = scal[S_ITER] + 1.0;
-        scal[S_ITER] = it;
-        scal[S_NORMR] = normr;
-        double rho_old = scal[S_RHO];
-        if (normr / scal[S_NORMRHS] <= scal[S_TOL2]) scal[S_DONE] = 1.0;
-        else if (it >= scal[S_MAXIT]) { scal[S_DONE] = 1.0; scal[S_IERR] = 1.0; }
-        else if (rho_old == 0.0) { scal[S_DONE] = 1.0; scal[S_IERR] = 2.0; }
-        else { scal[S_BETA] = rho_new / rho_old; }
-        scal[S_RHO_OLD] = rho_old;
-        scal[S_RHO] = rho_new;
-        *counter = 0;
-    }
+    const int regs[2] = {P_RZ, P_RR};
+    block_partials<2>(v, partials, pstride, regs);
 }
 
-// true residual ||b - A x||^2 and ||b||^2 (FP64)
-__global__ __launch_bounds__(PCG_BLOCK) void k_resid64(int n, const int* rowptr, const int* col, const double* val,
-                                                       const double* b, const double* x, double* r,
-                                                       double* out2, double* partials, unsigned int* counter) {
+// true residual ||b - A x||^2 and ||b||^2 (FP64): partials, then one block sums them
+template <class Op>
+__global__ __launch_bounds__(PCG_BLOCK) void k_resid64(int n, Op A, const double* b, const double* x, double* r,
+                                                       double* partials, int pstride) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (i < n) {
-        double s = 0.0;
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) s = s + val[k] * x[col[k]];
+        double s = A.row(i, [&](int j) { return x[j]; });
         double ri = b[i] - s;
         if (r) r[i] = ri;
         v[0] = ri * ri; v[1] = b[i] * b[i];
     }
-    block_reduce_store<2>(v, partials, gridDim.x);
-    if (last_block(counter) && threadIdx.x == 0) {
-        out2[0] = sum_partials(partials, gridDim.x);
-        out2[1] = sum_partials(partials + gridDim.x, gridDim.x);
-        *counter = 0;
-    }
+    const int regs[2] = {P_RR, P_BB};
+    block_partials<2>(v, partials, pstride, regs);
+}
+__global__ __launch_bounds__(PCG_BLOCK) void k_sum2(int nb, const double* partials, int pstride, double* out2) {
+    double a0 = reduce_region(partials, pstride, P_RR, nb);
+    double a1 = reduce_region(partials, pstride, P_BB, nb);
+    if (threadIdx.x == 0) { out2[0] = a0; out2[1] = a1; }
 }
 
 __global__ void k_d2f(int n, const double* a, float* b) { int i = blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = (float)a[i]; }
 __global__ void k_axpy_f2d(int n, const float* a, double* b) { int i = blockIdx.x * blockDim.x + threadIdx.x; if (i < n) b[i] = b[i] + (double)a[i]; }
 __global__ void k_zero_f(int n, float* a) { int i = blockIdx.x * blockDim.x + threadIdx.x; if (i < n) a[i] = 0.0f; }
 
-// gather phi at unknowns / scatter back (IFE_Solve_2D.f90:80-84, 243-247, 514-516)
-__global__ void k_phi_to_x(int n, const int* unknown_node, const double* phi, double* x) {
+// phi at unknown nodes <-> node-space solution vector (IFE_Solve_2D.f90:80-84, 243-247, 514-516)
+__global__ void k_phi_to_xn(int n, const int* nt2, const double* phi, double* x) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = phi[unknown_node[i]];
+    if (i < n) x[i] = nt2[i] > 0 ? phi[i] : 0.0;
 }
-__global__ void k_x_to_phi(int n, const int* unknown_node, const double* x, double* phi) {
+__global__ void k_xn_to_phi(int n, const int* nt2, const double* x, double* phi) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) phi[unknown_node[i]] = x[i];
+    if (i < n && nt2[i] > 0) phi[i] = x[i];
 }
 
-// One Jacobi-PCG solve on device vectors (storage type T). Host loop in chunks.
-template <typename T>
-static int pcg_run(int n, const int* rowptr, const int* col, const T* val, const T* diag, const T* b, T* x,
-                   T* r, T* z, T* p, T* p2, T* q, double tol, int maxit, int* iters, int* ierr) {
-    int nb = (n + PCG_BLOCK - 1) / PCG_BLOCK;
+static inline int pcg_blocks(int n) { return (n + PCG_BLOCK - 1) / PCG_BLOCK; }
+
+// One Jacobi-PCG solve on device vectors (storage T, operator Op). The host launches the
+// iterations in chunks and reads the done flag between chunks.
+template <typename T, class Op>
+static int pcg_run(int n, const Op& A, const T* diag, const T* b, T* x, T* r, T* z, T* p, T* p2, T* q, double tol,
+                   int maxit, double* partials, int pstride, int* iters, int* ierr) {
+    int nb = pcg_blocks(n);
+    if (nb > pstride) return pic_fail(PIC_ERR_STATE, "PCG partial buffer too small");
     double* scal = g.d_scal;
     double h2[2] = {tol * tol, (double)maxit};
     HIPCHK(hipMemcpyAsync(scal + S_TOL2, h2, 2 * sizeof(double), hipMemcpyHostToDevice, g.stream));
-    hipLaunchKernelGGL((k_pcg_init<T>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, rowptr, col, val, diag, b, x, r, z,
-                       scal, g.d_partials, g.d_sync);
+    hipLaunchKernelGGL((k_pcg_init<T, Op>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, A, diag, b, x, r, z, scal,
+                       partials, pstride);
     HIPCHK(hipGetLastError());
     T* pa = p; T* pb = p2;
-    const int CHUNK = 16;
-    int done_iters = 0;
+    const int CHUNK = 32;
+    int k = 0;
     for (;;) {
-        for (int k = 0; k < CHUNK; ++k) {
-            hipLaunchKernelGGL((k_pcg_spmv<T>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, rowptr, col, val, z, pa, pb, q,
-                               scal, g.d_partials, g.d_sync);
-            hipLaunchKernelGGL((k_pcg_update<T>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, diag, pb, q, x, r, z,
-                               scal, g.d_partials, g.d_sync);
+        for (int c = 0; c < CHUNK; ++c, ++k) {
+            hipLaunchKernelGGL((k_pcg_spmv<T, Op>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, k, nb, A, z, pa, pb, q,
+                               scal, partials, pstride);
+            hipLaunchKernelGGL((k_pcg_update<T>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, k, nb, diag, pb, q, x, r, z,
+                               scal, partials, pstride);
             T* t = pa; pa = pb; pb = t;
         }
         HIPCHK(hipGetLastError());
-        done_iters += CHUNK;
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
         if (g.h_scal_pinned[S_DONE] != 0.0) break;
-        if (done_iters > maxit + CHUNK) break;
+        if (k > maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
     }
-    // the buffer holding the last p is irrelevant to callers
     *iters = (int)g.h_scal_pinned[S_ITER];
     *ierr = (int)g.h_scal_pinned[S_IERR];
     return PIC_OK;
 }
 
-static int resid64(int n, const int* rowptr, const int* col, const double* val, const double* b, const double* x,
-                   double* r, double* rr, double* bb) {
-    int nb = (n + PCG_BLOCK - 1) / PCG_BLOCK;
+template <class Op>
+static int resid64(int n, const Op& A, const double* b, const double* x, double* r, double* rr, double* bb) {
+    int nb = pcg_blocks(n);
     double* out = g.d_scal + S_NSCAL;
-    hipLaunchKernelGGL(k_resid64, dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, rowptr, col, val, b, x, r, out,
-                       g.d_partials, g.d_sync);
+    hipLaunchKernelGGL((k_resid64<Op>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, A, b, x, r, g.d_partials, g.pcg_pstride);
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(PCG_BLOCK), 0, g.stream, nb, g.d_partials, g.pcg_pstride, out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(g.h_scal_pinned + S_NSCAL, out, 2 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -302,19 +341,27 @@ static int resid64(int n, const int* rowptr, const int* col, const double* val, 
     return PIC_OK;
 }
 
-// Solve on the registered system: b = g.d_b, warm start from g.d_phi at unknowns.
+template <typename T>
+static DiaOp<T> dia_op(int prec64) {
+    DiaOp<T> A;
+    for (int k = 0; k < 9; ++k) { A.c[k] = (const T*)(prec64 ? (void*)g.d_dia64[k] : (void*)g.d_dia32[k]); A.off[k] = g.dia_off[k]; }
+    return A;
+}
+
+// Solve on the registered system in node space: b = g.d_bn, warm start from g.d_phi.
 int run_pcg_fp64(bool) {
-    int n = g.nunknowns;
+    int n = g.mesh.nnodes;
     int nb256 = (n + 255) / 256;
-    hipLaunchKernelGGL(k_phi_to_x, dim3(nb256), dim3(256), 0, g.stream, n, g.d_unknown_node, g.d_phi, g.d_x);
+    hipLaunchKernelGGL(k_phi_to_xn, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_phi, g.d_xn);
+    DiaOp<double> A = dia_op<double>(1);
     int iters = 0, ierr = 0;
-    int rc = pcg_run<double>(n, g.d_A_rowptr, g.d_A_col, g.d_A_val, g.d_A_diag, g.d_b, g.d_x, g.d_r, g.d_z, g.d_p,
-                             g.d_p2, g.d_q, g.pcg_tol, g.pcg_maxit, &iters, &ierr);
+    int rc = pcg_run<double>(n, A, g.d_dia64[4], g.d_bn, g.d_xn, g.d_rn, g.d_zn, g.d_pn, g.d_p2n, g.d_qn, g.pcg_tol,
+                             g.pcg_maxit, g.d_partials, g.pcg_pstride, &iters, &ierr);
     if (rc) return rc;
     double rr, bb;
-    rc = resid64(n, g.d_A_rowptr, g.d_A_col, g.d_A_val, g.d_b, g.d_x, nullptr, &rr, &bb);
+    rc = resid64(n, A, g.d_bn, g.d_xn, nullptr, &rr, &bb);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_x_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_unknown_node, g.d_x, g.d_phi);
+    hipLaunchKernelGGL(k_xn_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_xn, g.d_phi);
     HIPCHK(hipGetLastError());
     g.last_iter = iters; g.last_outer = 1;
     g.last_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
@@ -326,14 +373,16 @@ int run_pcg_fp64(bool) {
 // FP32 Jacobi-PCG on the correction inside FP64 iterative refinement; the outer residual
 // is recomputed in FP64 and decides convergence at pcg_tol.
 int run_pcg_mixed() {
-    int n = g.nunknowns;
+    int n = g.mesh.nnodes;
     int nb256 = (n + 255) / 256;
-    hipLaunchKernelGGL(k_phi_to_x, dim3(nb256), dim3(256), 0, g.stream, n, g.d_unknown_node, g.d_phi, g.d_x);
+    hipLaunchKernelGGL(k_phi_to_xn, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_phi, g.d_xn);
+    DiaOp<double> A64 = dia_op<double>(1);
+    DiaOp<float> A32 = dia_op<float>(0);
     int total = 0, outer = 0;
     double rr = 0, bb = 0;
     const double inner_tol = 1e-4;
     for (;;) {
-        int rc = resid64(n, g.d_A_rowptr, g.d_A_col, g.d_A_val, g.d_b, g.d_x, g.d_r, &rr, &bb);
+        int rc = resid64(n, A64, g.d_bn, g.d_xn, g.d_rn, &rr, &bb);
         if (rc) return rc;
         double rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
         if (rel <= g.pcg_tol || bb == 0.0) break;
@@ -341,17 +390,17 @@ int run_pcg_mixed() {
             g.last_iter = total; g.last_outer = outer; g.last_relres = rel;
             return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
         }
-        hipLaunchKernelGGL(k_d2f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_r, g.d_bf);
-        hipLaunchKernelGGL(k_zero_f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xf);
+        hipLaunchKernelGGL(k_d2f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_rn, g.d_bfn);
+        hipLaunchKernelGGL(k_zero_f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xfn);
         int it = 0, ierr = 0;
-        rc = pcg_run<float>(n, g.d_A_rowptr, g.d_A_col, g.d_A_valf, g.d_A_diagf, g.d_bf, g.d_xf, g.d_rf, g.d_zf,
-                            g.d_pf, g.d_p2f, g.d_qf, inner_tol, g.pcg_maxit - total, &it, &ierr);
+        rc = pcg_run<float>(n, A32, g.d_dia32[4], g.d_bfn, g.d_xfn, g.d_rfn, g.d_zfn, g.d_pfn, g.d_p2fn, g.d_qfn,
+                            inner_tol, g.pcg_maxit - total, g.d_partials, g.pcg_pstride, &it, &ierr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_axpy_f2d, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xf, g.d_x);
+        hipLaunchKernelGGL(k_axpy_f2d, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xfn, g.d_xn);
         total += it; outer++;
         if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "inner PCG breakdown");
     }
-    hipLaunchKernelGGL(k_x_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_unknown_node, g.d_x, g.d_phi);
+    hipLaunchKernelGGL(k_xn_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_xn, g.d_phi);
     HIPCHK(hipGetLastError());
     g.last_iter = total; g.last_outer = outer;
     g.last_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
@@ -361,13 +410,15 @@ int run_pcg_mixed() {
 // Stand-alone FP64 PCG on caller-provided device CSR (MY_JPCG_Solver drop-in).
 int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,
                      const double* b, double* x, double tol, int maxit, int* iters, int* ierr, double* r_out) {
-    double *r, *z, *p, *p2, *q;
+    double *r, *z, *p, *p2, *q, *part;
+    int pstride = pcg_blocks(n) + 1;
     HIPCHK(hipMalloc(&r, sizeof(double) * n)); HIPCHK(hipMalloc(&z, sizeof(double) * n));
     HIPCHK(hipMalloc(&p, sizeof(double) * n)); HIPCHK(hipMalloc(&p2, sizeof(double) * n));
-    HIPCHK(hipMalloc(&q, sizeof(double) * n));
-    int rc = pcg_run<double>(n, rowptr, col, val, diag, b, x, r, z, p, p2, q, tol, maxit, iters, ierr);
+    HIPCHK(hipMalloc(&q, sizeof(double) * n)); HIPCHK(hipMalloc(&part, sizeof(double) * P_NREG * pstride));
+    CsrOp<double> A; A.rowptr = rowptr; A.col = col; A.val = val;
+    int rc = pcg_run<double>(n, A, diag, b, x, r, z, p, p2, q, tol, maxit, part, pstride, iters, ierr);
     if (rc == PIC_OK && r_out) HIPCHK(hipMemcpyAsync(r_out, r, sizeof(double) * n, hipMemcpyDeviceToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
-    hipFree(r); hipFree(z); hipFree(p); hipFree(p2); hipFree(q);
+    (void)hipFree(r); (void)hipFree(z); (void)hipFree(p); (void)hipFree(p2); (void)hipFree(q); (void)hipFree(part);
     return rc;
 }

@@ -429,13 +429,14 @@ int run_mcc() {
 }
 
 // ---------------------------------------------------------------- cell sort
-// Stable radix sort of (element, slot) with dead slots keyed past the last element,
+// Stable radix sort of (tile-major element key, slot) with dead slots keyed past the end,
 // then a gather of every SoA array; occupied slots shrink to the live count.
-__global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsigned int* vals, unsigned int dead_key) {
+__global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsigned int* vals, unsigned int dead_key,
+                            int ney, int nty) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c = cell[i];
-    keys[i] = c >= 0 ? (unsigned int)c : dead_key;
+    keys[i] = c >= 0 ? pic_sort_key(c, ney, nty) : dead_key;
     vals[i] = (unsigned int)i;
 }
 template <typename T>
@@ -443,18 +444,14 @@ __global__ void k_gather(int64_t n, const unsigned int* perm, const T* src, T* d
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[perm[i]];
 }
-__global__ void k_count_live(int64_t n, const int* cell, unsigned long long* out) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool l = i < n && cell[i] >= 0;
-    unsigned long long b = __ballot(l);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(out, (unsigned long long)__popcll(b));
-}
+#define TRY_MCC(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
 
 static int sort_species(int isp) {
     SpeciesHost& S = g.sp[isp];
     int64_t n = S.nslots;
     if (n == 0) return PIC_OK;
-    unsigned int dead_key = (unsigned int)g.mesh.nelem;
+    int nty = (g.mesh.ney + SORT_TILE - 1) / SORT_TILE, ntx = (g.mesh.nex + SORT_TILE - 1) / SORT_TILE;
+    unsigned int dead_key = (unsigned int)(ntx * nty * SORT_TILE * SORT_TILE);
     int end_bit = 1;
     while ((1u << end_bit) <= dead_key) end_bit++;
     size_t esz = (g.particle_bits == 64) ? 8 : 4;
@@ -471,7 +468,7 @@ static int sort_species(int isp) {
     unsigned int* v1 = v0 + n;
     void* tmpdata = (void*)(v1 + n);
     int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, g.stream, n, S.cell, k0, v0, dead_key);
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, g.stream, n, S.cell, k0, v0, dead_key, g.mesh.ney, nty);
     size_t tb = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
     if (g.sort_tmp_bytes < tb) {
@@ -482,11 +479,8 @@ static int sort_species(int isp) {
     }
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(g.d_sort_tmp, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
     // live count
-    unsigned long long* dcnt = (unsigned long long*)g.d_partials;
-    HIPCHK(hipMemsetAsync(dcnt, 0, 8, g.stream));
-    hipLaunchKernelGGL(k_count_live, dim3(blocks), dim3(256), 0, g.stream, n, S.cell, dcnt);
-    unsigned long long live = 0;
-    HIPCHK(hipMemcpyAsync(&live, dcnt, 8, hipMemcpyDeviceToHost, g.stream));
+    int64_t live = 0;
+    TRY_MCC(count_live_slots(n, S.cell, &live));
     // gather arrays through the temporary
     void* arrays[5] = {S.x, S.y, S.vx, S.vy, S.vz};
     for (int a = 0; a < 5; ++a) {

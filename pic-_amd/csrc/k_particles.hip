@@ -10,6 +10,10 @@
 // reduction keyed by element and only segment tails issue (int64) atomics.
 #include "pic_device.h"
 
+#define PUSH_BLOCK 512
+#define PUSH_PPT 8
+#define WIN_NODES (3 * SORT_TILE + 1) * (3 * SORT_TILE + 1)
+
 template <typename R> struct PushArgs {
     R *x, *y, *vx, *vy, *vz;
     const int* cell;
@@ -23,16 +27,24 @@ template <typename R> struct PushArgs {
     R xmin, xmax, ymin, ymax, emit_eps;   // domain and the 10E-6 offset of the reference
     double vref_inv;          // 1/v_ref  (VelRes(1.0/PB%VFactor))
     double kappa_c;           // (Kappa-1.5)/beta, beta = m/(2 kB T)   (SI)
-    double energy_c;          // 0.5*Mass*v_ref^2/JtoeV
+    double mass, v_ref;       // species mass (kg) and v_ref for the energy moment
     uint64_t seed; uint32_t step; uint32_t tag_emit;
     long long* acc_w; long long* acc_e;
     unsigned long long* counters;
     unsigned int* err;
+    PicErrInfo* errinfo;
+    int species;
 };
 
 __device__ inline void wave_count(unsigned long long* ctr, bool pred) {
     unsigned long long b = __ballot(pred);
     if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)b) - 1)) atomicAdd(ctr, (unsigned long long)__popcll(b));
+}
+
+__device__ inline long long shfl_xor_ll(long long v, int d) {
+    int lo = (int)(unsigned long long)v, hi = (int)((unsigned long long)v >> 32);
+    lo = __shfl_xor(lo, d, 64); hi = __shfl_xor(hi, d, 64);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
 __device__ inline long long shfl_up_ll(long long v, int d) {
@@ -41,10 +53,14 @@ __device__ inline long long shfl_up_ll(long long v, int d) {
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// LDS deposition window of a block: node columns x0..x0+nx-1, rows y0..y0+ny-1.
+struct DepWindow { int x0, y0, nx, ny; };
+
 // Segmented inclusive sum over the wave of eight int64 values keyed by `key`; lanes at
-// the tail of a run of equal keys add the run total to the node accumulators.
-__device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8],
-                                    long long* acc_w, long long* acc_e) {
+// the tail of a run of equal keys add the run total to the block's LDS window when the
+// element's four nodes lie in it, to the global node accumulators otherwise.
+__device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8], long long* acc_w, long long* acc_e,
+                                    const DepWindow& w, long long* winW, long long* winE) {
     const int lane = threadIdx.x & 63;
     int kprev = __shfl_up(key, 1, 64);
     int knext = __shfl_down(key, 1, 64);
@@ -53,27 +69,63 @@ __device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8]
     unsigned long long heads = __ballot(head);
     unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
     int seg_start = 63 - __clzll((long long)below);
+    if (heads != 1ull) {      // more than one run in the wave: segmented scan
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        bool take = (lane - off) >= seg_start;
+        for (int off = 1; off < 64; off <<= 1) {
+            bool take = (lane - off) >= seg_start;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            long long u = shfl_up_ll(v[k], off);
-            if (take) v[k] += u;
+            for (int k = 0; k < 8; ++k) {
+                long long u = shfl_up_ll(v[k], off);
+                if (take) v[k] += u;
+            }
+        }
+    } else {                  // one run: butterfly reduction, total in every lane
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] += shfl_xor_ll(v[k], off);
         }
     }
     if (tail && key >= 0) {
         int ci = key / m.ney, cj = key - ci * m.ney;
-        int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-        atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
-        atomicAdd((unsigned long long*)&acc_w[n2], (unsigned long long)v[1]);
-        atomicAdd((unsigned long long*)&acc_w[n3], (unsigned long long)v[2]);
-        atomicAdd((unsigned long long*)&acc_w[n4], (unsigned long long)v[3]);
-        atomicAdd((unsigned long long*)&acc_e[n1], (unsigned long long)v[4]);
-        atomicAdd((unsigned long long*)&acc_e[n2], (unsigned long long)v[5]);
-        atomicAdd((unsigned long long*)&acc_e[n3], (unsigned long long)v[6]);
-        atomicAdd((unsigned long long*)&acc_e[n4], (unsigned long long)v[7]);
+        int li = ci - w.x0, lj = cj - w.y0;
+        if (li >= 0 && lj >= 0 && li + 1 < w.nx && lj + 1 < w.ny) {
+            int l1 = li * w.ny + lj, l2 = l1 + w.ny, l3 = l2 + 1, l4 = l1 + 1;
+            atomicAdd((unsigned long long*)&winW[l1], (unsigned long long)v[0]);
+            atomicAdd((unsigned long long*)&winW[l2], (unsigned long long)v[1]);
+            atomicAdd((unsigned long long*)&winW[l3], (unsigned long long)v[2]);
+            atomicAdd((unsigned long long*)&winW[l4], (unsigned long long)v[3]);
+            atomicAdd((unsigned long long*)&winE[l1], (unsigned long long)v[4]);
+            atomicAdd((unsigned long long*)&winE[l2], (unsigned long long)v[5]);
+            atomicAdd((unsigned long long*)&winE[l3], (unsigned long long)v[6]);
+            atomicAdd((unsigned long long*)&winE[l4], (unsigned long long)v[7]);
+        } else {
+            int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+            atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
+            atomicAdd((unsigned long long*)&acc_w[n2], (unsigned long long)v[1]);
+            atomicAdd((unsigned long long*)&acc_w[n3], (unsigned long long)v[2]);
+            atomicAdd((unsigned long long*)&acc_w[n4], (unsigned long long)v[3]);
+            atomicAdd((unsigned long long*)&acc_e[n1], (unsigned long long)v[4]);
+            atomicAdd((unsigned long long*)&acc_e[n2], (unsigned long long)v[5]);
+            atomicAdd((unsigned long long*)&acc_e[n3], (unsigned long long)v[6]);
+            atomicAdd((unsigned long long*)&acc_e[n4], (unsigned long long)v[7]);
+        }
     }
+}
+
+// Specular folding of y into [ymin, ymax] (reflections off both walls), Vy reversed.
+template <typename R>
+__device__ inline void fold_y(R& Y, R& VY, R ymin, R ymax) {
+    R L = ymax - ymin;
+    R d = Y - ymin;
+    if (!(d >= (R)0 && d <= L)) {
+        R per = (R)2 * L;
+        d = fmod(d, per);
+        if (d < (R)0) d = d + per;
+        if (d > L) d = per - d;
+        Y = ymin + d;
+    }
+    VY = -VY;
 }
 
 // Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64.
@@ -97,8 +149,8 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
 }
 
 template <typename R, int DO_PUSH>
-__global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void push_one(const PushArgs<R>& a, int64_t p, const DepWindow& w,
+                                         long long* winW, long long* winE) {
     const MeshParams& m = a.m;
     bool live = false;
     int e_new = -1;
@@ -115,7 +167,11 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
                 int n = e_old;
                 pic_stream rs; bool rs_init = false;
                 while (ivel == 1 || ipos == 1) {
-                    if (++it > 64) { atomicOr(a.err, PIC_EBIT_BOUNDARY); break; }
+                    if (++it > 64) {
+                        atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                        pic_record(a.errinfo, PIC_EBIT_BOUNDARY, p, X, Y, VX, VY, it, e_old, -1, a.species);
+                        break;
+                    }
                     // ---------------- MoveOne: gather + push (MoveOne.f90:77-146, 213-219)
                     if (ivel == 1) {
                         int ci = n / m.ney, cj = n - ci * m.ney;
@@ -173,11 +229,14 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
                     Y = Y + T * VY;
                     ipos = 0;
                     // ---------------- AdjustParticleBoundary (MCCInterface.f90:361-499)
+                    // Back-up times lie in [0, T] whenever the reference's arithmetic is
+                    // well posed; outside [-T/2, 2T] (possible only after the re-emission has
+                    // divided Vy by v_ref again) the position is folded into the domain instead.
                     if (X <= a.xmin) {
                         ipos = 1;
                         R TR = (X - a.xmin) / VX;
-                        T = TR;
-                        Y = Y - VY * TR;
+                        bool ok = TR >= (R)-0.5 * T && TR <= (R)2 * T;
+                        if (ok) { T = TR; Y = Y - VY * TR; }
                         X = a.xmin + a.emit_eps;
                         if (!rs_init) { pic_stream_init(&rs, a.seed, (uint32_t)p, a.step, a.tag_emit); rs_init = true; }
                         double R1 = pic_stream_u01(&rs);
@@ -187,6 +246,7 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
                         VY = (R)((double)VY * a.vref_inv);
                         VZ = (R)((double)VZ * a.vref_inv);
                         emitted = true;
+                        if (!ok) { T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax); }
                     } else if (X > a.xmax) {
                         X = (R)-2000;
                         absorbed_wall = true;
@@ -194,12 +254,16 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
                         if (m.delta == 0) {
                             ipos = 1;
                             R TR = (Y - a.ymin) / VY;
-                            T = TR;
-                            X = X - VX * TR;
-                            VY = -VY;
-                            Y = a.ymin + a.emit_eps;
                             reflected = true;
-                            if (Y > a.ymax) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                            if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
+                                T = TR;
+                                X = X - VX * TR;
+                                VY = -VY;
+                                Y = a.ymin + a.emit_eps;
+                                if (Y > a.ymax) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                            } else {
+                                T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
+                            }
                         } else {
                             atomicOr(a.err, PIC_EBIT_WRONGCROSS);
                         }
@@ -207,12 +271,16 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
                         if (m.delta == 0) {
                             ipos = 1;
                             R TR = (Y - a.ymax) / VY;
-                            T = TR;
-                            X = X - VX * TR;
-                            VY = -VY;
-                            Y = a.ymax - a.emit_eps;
                             reflected = true;
-                            if (Y < a.ymin) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                            if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
+                                T = TR;
+                                X = X - VX * TR;
+                                VY = -VY;
+                                Y = a.ymax - a.emit_eps;
+                                if (Y < a.ymin) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                            } else {
+                                T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
+                            }
                         }
                     }
                     for (int k = 0; k < m.n_objects; ++k) {
@@ -232,7 +300,10 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
             if (live) {
                 int bad = 0;
                 e_new = pic_locate_elem((double)X, (double)Y, m, &bad);
-                if (bad) atomicOr(a.err, PIC_EBIT_POSITION);
+                if (bad) {
+                    atomicOr(a.err, PIC_EBIT_POSITION);
+                    pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, e_new, a.species);
+                }
             }
             a.cell_next[p] = live ? e_new : -1;
         }
@@ -252,7 +323,10 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
         double Xd = (double)X, Yd = (double)Y;
         deposit_weights(Xd, Yd, e_new, m, W);
         double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
-        double energy = a.energy_c * (vxd * vxd + vyd * vyd + vzd * vzd);
+        // EnergyParticleOne (ParticleOne.f90:339-340) then /JtoeV (GetParChg_2D.f90:128)
+        double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
+        energy = energy * a.v_ref * a.v_ref;
+        energy = energy / PIC_JTOEV;
 #pragma unroll
         for (int k = 0; k < 4; ++k) { v[k] = pic_fix(W[k]); v[4 + k] = pic_fix(W[k] * energy); }
         key = e_new;
@@ -260,7 +334,63 @@ __global__ __launch_bounds__(256) void k_push_deposit(PushArgs<R> a) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) v[k] = 0;
     }
-    wave_deposit(key, m, v, a.acc_w, a.acc_e);
+    wave_deposit(key, m, v, a.acc_w, a.acc_e, w, winW, winE);
+}
+
+
+// Fused mover + lookup + deposition. A block of PUSH_BLOCK threads takes PUSH_PPT *
+// PUSH_BLOCK consecutive slots (slot = base + k*PUSH_BLOCK + tid, coalesced). Its node
+// moments go to an LDS window covering the 3x3 sort tiles around the tile of the block's
+// middle particle (slots are sorted tile-major, so almost every deposit lands there);
+// the window is flushed with one global atomic per touched node and moment. Deposits
+// outside the window go straight to global atomics. Integer sums: the result does not
+// depend on which path a deposit takes.
+template <typename R, int DO_PUSH>
+__global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
+    __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
+    __shared__ int s_center;
+    const int64_t base = (int64_t)blockIdx.x * (PUSH_BLOCK * PUSH_PPT);
+    const MeshParams& m = a.m;
+    if (threadIdx.x < 64) {
+        // first live particle at or after the block's middle slot (one wave looks)
+        int64_t mid = base + (PUSH_BLOCK * PUSH_PPT) / 2;
+        if (mid >= a.n) mid = base;
+        int64_t q = mid + threadIdx.x;
+        int c = (q < a.n) ? a.cell[q] : -1;
+        unsigned long long b = __ballot(c >= 0);
+        int first = b ? (__ffsll((long long)b) - 1) : -1;
+        int cc = __shfl(c, first < 0 ? 0 : first, 64);
+        if (threadIdx.x == 0) s_center = (first < 0) ? -1 : cc;
+    }
+    __syncthreads();
+    DepWindow w;
+    w.nx = 0; w.ny = 0; w.x0 = 0; w.y0 = 0;
+    if (s_center >= 0) {
+        int ci = s_center / m.ney, cj = s_center - ci * m.ney;
+        int ti = ci / SORT_TILE, tj = cj / SORT_TILE;
+        int x0 = (ti - 1) * SORT_TILE, y0 = (tj - 1) * SORT_TILE;
+        int x1 = (ti + 2) * SORT_TILE, y1 = (tj + 2) * SORT_TILE;     // last node index of the 3x3 tiles
+        if (x0 < 0) x0 = 0;
+        if (y0 < 0) y0 = 0;
+        if (x1 > m.nnx - 1) x1 = m.nnx - 1;
+        if (y1 > m.nny - 1) y1 = m.nny - 1;
+        w.x0 = x0; w.y0 = y0; w.nx = x1 - x0 + 1; w.ny = y1 - y0 + 1;
+    }
+    const int wn = w.nx * w.ny;
+    for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) { winW[k] = 0; winE[k] = 0; }
+    __syncthreads();
+    for (int k = 0; k < PUSH_PPT; ++k) {
+        int64_t p = base + (int64_t)k * PUSH_BLOCK + threadIdx.x;
+        push_one<R, DO_PUSH>(a, p, w, winW, winE);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) {
+        int i = k / w.ny, j = k - i * w.ny;
+        int node = (w.x0 + i) * m.nny + (w.y0 + j);
+        long long vw = winW[k], ve = winE[k];
+        if (vw != 0) atomicAdd((unsigned long long*)&a.acc_w[node], (unsigned long long)vw);
+        if (ve != 0) atomicAdd((unsigned long long*)&a.acc_e[node], (unsigned long long)ve);
+    }
 }
 
 template <typename R>
@@ -285,10 +415,11 @@ static PushArgs<R> make_push_args(int isp) {
     a.vref_inv = 1.0 / g.v_ref;
     double beta = S.p.mass_kg / (2.0 * PIC_KB * S.p.init_temperature_K);
     a.kappa_c = (1.0 * 2.0 - 1.5) / beta;
-    a.energy_c = 0.5 * S.p.mass_kg * g.v_ref * g.v_ref / PIC_JTOEV;
+    a.mass = S.p.mass_kg; a.v_ref = g.v_ref;
     a.seed = g.seed; a.step = (uint32_t)g.step; a.tag_emit = pic_stream_tag(isp, PIC_RNG_EMIT);
     a.acc_w = S.acc_w; a.acc_e = S.acc_e;
     a.counters = S.counters; a.err = g.d_err;
+    a.errinfo = (PicErrInfo*)g.d_errinfo; a.species = isp;
     return a;
 }
 
@@ -312,15 +443,15 @@ int launch_push_deposit(int isp, bool do_push) {
     SpeciesHost& S = g.sp[isp];
     if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
     if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
-    int blocks = (int)((S.nslots + 255) / 256);
+    int blocks = (int)((S.nslots + PUSH_BLOCK * PUSH_PPT - 1) / (PUSH_BLOCK * PUSH_PPT));
     if (g.particle_bits == 64) {
         PushArgs<double> a = make_push_args<double>(isp);
-        if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(256), 0, g.stream, a);
-        else hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(256), 0, g.stream, a);
+        if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        else hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
     } else {
         PushArgs<float> a = make_push_args<float>(isp);
-        if (do_push) hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(256), 0, g.stream, a);
-        else hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(256), 0, g.stream, a);
+        if (do_push) hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        else hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
     }
     HIPCHK(hipGetLastError());
     S.acc_dirty = true;
@@ -374,7 +505,7 @@ int launch_finalize_deposit() {
     a.nnodes = g.mesh.nnodes; a.nspecies = g.nspecies; a.delta = g.mesh.delta;
     for (int s = 0; s < g.nspecies; ++s) {
         SpeciesHost& S = g.sp[s];
-        if (S.acc_dirty == false) { int rc = launch_zero_acc(s); if (rc) return rc; }
+        if (!S.defined) return pic_fail(PIC_ERR_STATE, "species indices must be contiguous from 0");
         a.acc_w[s] = S.acc_w; a.acc_e[s] = S.acc_e;
         a.weight_qs[s] = S.p.weight * S.qs;     // Weight * qs(isp), evaluated left to right
         a.weight[s] = S.p.weight;

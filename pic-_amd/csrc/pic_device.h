@@ -43,6 +43,17 @@ PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
     return (j - 1) + (i - 1) * m.ney;
 }
 
+// Sort tiles: SORT_TILE x SORT_TILE elements, tiles visited column by column with the
+// direction alternating (serpentine), elements inside a tile column-major.
+#define SORT_TILE 16
+PIC_HD unsigned int pic_sort_key(int e, int ney, int nty) {
+    int ci = e / ney, cj = e - ci * ney;
+    int ti = ci / SORT_TILE, tj = cj / SORT_TILE;
+    int tpos = (ti & 1) ? (nty - 1 - tj) : tj;
+    unsigned int tile = (unsigned int)(ti * nty + tpos);
+    return tile * (SORT_TILE * SORT_TILE) + (unsigned int)((ci % SORT_TILE) * SORT_TILE + (cj % SORT_TILE));
+}
+
 // Fixed-point quantum of deposition: value * 2^40 rounded to nearest (even), as int64.
 // Integer sums are exact and independent of order, so the node moments are
 // bit-identical whatever the particle order, atomic interleaving or kernel fusion.
@@ -61,6 +72,24 @@ enum {
     PIC_EBIT_CAPACITY = 64u,
     PIC_EBIT_MCCLOOP = 128u
 };
+
+// First failing particle, recorded for the error message (first writer wins).
+struct PicErrInfo {
+    unsigned int taken;
+    unsigned int bit;
+    long long slot;
+    double v[8];     // x, y, vx, vy, boundary iterations, old element, new element, species
+};
+#if defined(__HIPCC__)
+__device__ inline void pic_record(PicErrInfo* ei, unsigned int bit, long long slot, double x, double y, double vx,
+                                  double vy, double it, double e_old, double e_new, double sp) {
+    if (ei && atomicCAS(&ei->taken, 0u, 1u) == 0u) {
+        ei->bit = bit; ei->slot = slot;
+        ei->v[0] = x; ei->v[1] = y; ei->v[2] = vx; ei->v[3] = vy; ei->v[4] = it; ei->v[5] = e_old;
+        ei->v[6] = e_new; ei->v[7] = sp;
+    }
+}
+#endif
 
 // Counter slots of SpeciesHost::counters.
 enum {

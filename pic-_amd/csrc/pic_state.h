@@ -158,6 +158,13 @@ struct PicState {
     double* d_B_val = nullptr; double* d_c = nullptr;
     double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_p2 = nullptr, *d_q = nullptr;
     float *d_bf = nullptr, *d_xf = nullptr, *d_rf = nullptr, *d_zf = nullptr, *d_pf = nullptr, *d_p2f = nullptr, *d_qf = nullptr;
+    // node-space PCG: nine stiffness diagonals (offsets dia_off) and padded vectors
+    double* d_dia64[9] = {nullptr}; float* d_dia32[9] = {nullptr};
+    int dia_off[9] = {0};
+    void* d_pcgvec = nullptr;          // backing store of the padded node-space vectors
+    double *d_bn = nullptr, *d_xn = nullptr, *d_rn = nullptr, *d_zn = nullptr, *d_pn = nullptr, *d_p2n = nullptr, *d_qn = nullptr;
+    float *d_bfn = nullptr, *d_xfn = nullptr, *d_rfn = nullptr, *d_zfn = nullptr, *d_pfn = nullptr, *d_p2fn = nullptr, *d_qfn = nullptr;
+    int pcg_pstride = 0;               // partial-sum region length
     double* d_scal = nullptr;          // PCG scalars (device)
     double* d_partials = nullptr;      // block partial sums
     unsigned int* d_sync = nullptr;    // last-block counters
@@ -180,6 +187,7 @@ struct PicState {
     int mcc_enabled = 1;
     int64_t step = 0;
     unsigned int* d_err = nullptr;     // device error flags
+    void* d_errinfo = nullptr;         // PicErrInfo of the first failing particle
     // scratch for selection / compaction
     int* d_sel = nullptr; int64_t sel_cap = 0;
     int* d_blockcount = nullptr; int64_t blockcount_cap = 0;
@@ -217,6 +225,7 @@ int run_pcg_mixed();
 int run_mcc();
 int run_sort_all();
 int launch_zero_acc(int isp);
+int count_live_slots(int64_t n, const int* cell, int64_t* out);
 int launch_locate_points(int64_t n, const double* dx, const double* dy, int* dout, int mode);
 int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, const int* ele, double* ex, double* ey);
 int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,

@@ -16,8 +16,7 @@
 //   PPR point patches        : Get_PointPatch.f90:83-590, least squares of
 //                              GetEfield_SIDG_PPR.f90:153-241 (geometry only -> rows)
 //   MCC tables               : MCCInit.F90:14-152 ProbilityNonReactive, MCCSigma.f90:9-120
-#include "pic_state.h"
-#include "philox.h"
+#include "pic_device.h"
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -459,7 +458,7 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 }
 
 // ------------------------------------------------------------------ particle loader
-// ppc particles per vacuum element, element order; uniform position inside the element,
+// ppc particles per vacuum element, elements in sort order; uniform position inside the element,
 // Maxwellian velocity (three Box-Muller normals) at temp_eV. Particles falling inside an
 // object are dropped (the reference's AdjustBoundary_2D pre-filter, Main_IFE_Test_2.f90:218).
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
@@ -473,7 +472,14 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
     size_t cap = (size_t)m.nelem * ppc;
     x.reserve(cap); y.reserve(cap); vx.reserve(cap); vy.reserve(cap); vz.reserve(cap); cell.reserve(cap);
     const uint32_t tag = pic_stream_tag(isp, PIC_RNG_LOAD);
-    for (int e = 0; e < m.nelem; ++e) {
+    // elements in the tile-major order of the cell sort, so the loaded slots start sorted
+    int nty = (m.ney + SORT_TILE - 1) / SORT_TILE;
+    std::vector<int> order(m.nelem);
+    for (int e = 0; e < m.nelem; ++e) order[e] = e;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return pic_sort_key(a, m.ney, nty) < pic_sort_key(b, m.ney, nty);
+    });
+    for (int e : order) {
         if (g.h_element_index[e] == -2) continue;
         int ci = e / m.ney, cj = e % m.ney;
         double x0 = g.h_xs[ci], x1 = g.h_xs[ci + 1], y0 = g.h_ys[cj], y1 = g.h_ys[cj + 1];

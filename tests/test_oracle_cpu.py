@@ -388,3 +388,52 @@ def test_workload_definitions(pkg):
     assert (w["domain"].nnx - 1) ** 2 * w["ppc"] == 16777216
     w1 = pkg.workload("cfg1")
     assert (w1["domain"].nnx - 1) ** 2 * w1["ppc"] == 409600
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_push_corner_crossing_settles(orc, bits):
+    """A particle crossing the left and the bottom wall in one move: the left-wall
+    re-emission divides Vy by v_ref (VelRes of all three components) and the bottom
+    back-up time then exceeds the move; the position is mirrored and the loop ends."""
+    n = 8
+    xs, ys, ni, ei, pp, _ = _mesh(n)
+    nn = (n + 1) ** 2
+    z = np.zeros(nn)
+    x, y = np.array([0.02, 0.03, 7.99]), np.array([0.01, 7.98, 0.02])
+    vx, vy, vz = np.array([-2.0, -1.5, 0.5]), np.array([-3.0, 2.5, -2.0]), np.zeros(3)
+    cell, _ = orc.locate(xs, ys, x, y)
+    X, Y, VX, VY, VZ, alive, cnt, err = orc.move_all(bits, xs, ys, ei, ni, np.zeros((0, 6)), z, z, np.zeros(5 * nn),
+                                                     -1.0, 0.05, 419396.657327663, 9.1095e-31, 2 * 11605.0, 3, 0,
+                                                     orc.tag(0, 2), x, y, vx, vy, vz, cell)
+    assert err == 0
+    live = alive.astype(bool)
+    assert np.all((X[live] > 0) & (X[live] <= n) & (Y[live] >= 0) & (Y[live] <= n))
+    assert cnt[0] >= 2 and cnt[3] >= 2
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_push_wall_fuzz_always_settles(orc, bits):
+    """Random particles near the walls and corners with large velocities, repeated steps:
+    the boundary loop always ends inside the domain (or the particle is absorbed)."""
+    n = 8
+    xs, ys, ni, ei, pp, _ = _mesh(n)
+    nn = (n + 1) ** 2
+    rng = np.random.default_rng(17)
+    m = 20000
+    x = np.where(rng.random(m) < 0.5, rng.random(m) * 0.3, n - rng.random(m) * 0.3)
+    y = np.where(rng.random(m) < 0.5, rng.random(m) * 0.3, n - rng.random(m) * 0.3)
+    x = np.clip(x, 1e-3, n - 1e-3)
+    v = rng.normal(size=(3, m)) * rng.choice([0.1, 2.0, 40.0], size=m)
+    ex, ey = rng.normal(size=(2, nn)) * 5.0
+    cell, _ = orc.locate(xs, ys, x, y)
+    state = [x, y, *v]
+    for step in range(4):
+        X, Y, VX, VY, VZ, alive, cnt, err = orc.move_all(bits, xs, ys, ei, ni, np.zeros((0, 6)), ex, ey,
+                                                         np.zeros(5 * nn), -1.0, 0.05, 419396.657327663, 9.1095e-31,
+                                                         2 * 11605.0, 5, step, orc.tag(0, 2), *state, cell)
+        assert err == 0
+        live = alive.astype(bool)
+        assert np.all((X[live] > 0) & (X[live] <= n) & (Y[live] >= 0) & (Y[live] <= n))
+        state = [a[live].astype(np.float64) for a in (X, Y, VX, VY, VZ)]
+        cell, bad = orc.locate(xs, ys, state[0], state[1])
+        assert bad == 0
