@@ -226,6 +226,10 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     for (int n = 0; n < m.nnodes; ++n) vol[n] = g.h_cell_volume[2 * (size_t)n + 1];
     TRY(dupload(&g.d_cell_volume, vol));
     TRY(dupload(&g.d_dirichlet_phi, g.h_dirichlet_phi));
+    {
+        std::vector<DevObject> ob(g.mesh.obj, g.mesh.obj + PIC_MAX_OBJECTS);
+        TRY(dupload(&g.d_objects, ob));
+    }
     TRY(dupload(&g.d_node_type2, g.h_node_type2));
     std::vector<int> un0(g.h_unknown_node.size());
     for (size_t i = 0; i < un0.size(); ++i) un0[i] = g.h_unknown_node[i] - 1;

@@ -3,15 +3,19 @@
 // stand-alone lookup+deposition, the deposition finalisation (corner -> node sums and
 // normalisation) and the batched drop-ins (point lookup, PPR point gather).
 //
-// Particles are structure-of-arrays slots; one thread per slot, 256-thread blocks,
-// consecutive slots in consecutive lanes so every state stream is a coalesced HBM
-// stream. Slots are kept sorted by element (periodic cell sort), so a wave covers one
-// or two elements: the node moments of a wave are summed with a segmented wave
-// reduction keyed by element and only segment tails issue (int64) atomics.
+// Particles are structure-of-arrays slots kept in tile-major element order (periodic
+// cell sort). A thread owns PUSH_PPT consecutive slots and moves them with vector loads
+// and stores (a wave reads and writes contiguous 2 KiB runs of every array). Its
+// deposits are summed in registers while consecutive particles share an element; the
+// per-thread runs are merged by one segmented wave reduction keyed by element, and run
+// totals go to an LDS node window of the block (flushed once with global atomics) or,
+// outside the window, straight to global atomics. All sums are int64 fixed point, so
+// the node moments are identical whatever path or order a deposit takes.
 #include "pic_device.h"
 
-#define PUSH_BLOCK 512
+#define PUSH_BLOCK 256
 #define PUSH_PPT 8
+#define PUSH_SLOTS (PUSH_BLOCK * PUSH_PPT)
 #define WIN_NODES (3 * SORT_TILE + 1) * (3 * SORT_TILE + 1)
 
 template <typename R> struct PushArgs {
@@ -20,11 +24,13 @@ template <typename R> struct PushArgs {
     int* cell_next;
     int64_t n;
     MeshParams m;
+    const DevObject* objs;    // device copy of the objects (keeps them out of SGPRs)
     const R* efx; const R* efy;
     const R* coef5;           // c2..c6 per node, node-major (5 per node)
     const R* xsR; const R* ysR;
     R qm, dt;
     R xmin, xmax, ymin, ymax, emit_eps;   // domain and the 10E-6 offset of the reference
+    R hxR, hxiR, hyR, hyiR;               // mesh spacing and reciprocal in R (exact if pow2)
     double vref_inv;          // 1/v_ref  (VelRes(1.0/PB%VFactor))
     double kappa_c;           // (Kappa-1.5)/beta, beta = m/(2 kB T)   (SI)
     double mass, v_ref;       // species mass (kg) and v_ref for the energy moment
@@ -35,11 +41,6 @@ template <typename R> struct PushArgs {
     PicErrInfo* errinfo;
     int species;
 };
-
-__device__ inline void wave_count(unsigned long long* ctr, bool pred) {
-    unsigned long long b = __ballot(pred);
-    if (b && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)b) - 1)) atomicAdd(ctr, (unsigned long long)__popcll(b));
-}
 
 __device__ inline long long shfl_xor_ll(long long v, int d) {
     int lo = (int)(unsigned long long)v, hi = (int)((unsigned long long)v >> 32);
@@ -53,12 +54,47 @@ __device__ inline long long shfl_up_ll(long long v, int d) {
     return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+// Wave total of a per-lane count, added to a global counter by one lane (rare events).
+__device__ inline void wave_add(unsigned long long* ctr, int c) {
+    if (__ballot(c != 0) == 0ull) return;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(ctr, (unsigned long long)c);
+}
+
 // LDS deposition window of a block: node columns x0..x0+nx-1, rows y0..y0+ny-1.
 struct DepWindow { int x0, y0, nx, ny; };
 
-// Segmented inclusive sum over the wave of eight int64 values keyed by `key`; lanes at
-// the tail of a run of equal keys add the run total to the block's LDS window when the
-// element's four nodes lie in it, to the global node accumulators otherwise.
+// Add the eight moments of element `key` to the window (when its four nodes lie in it)
+// or to the global node accumulators.
+__device__ inline void add_element(int key, const long long v[8], const MeshParams& m, long long* acc_w,
+                                   long long* acc_e, const DepWindow& w, long long* winW, long long* winE) {
+    int ci = key / m.ney, cj = key - ci * m.ney;
+    int li = ci - w.x0, lj = cj - w.y0;
+    if (li >= 0 && lj >= 0 && li + 1 < w.nx && lj + 1 < w.ny) {
+        int l1 = li * w.ny + lj, l2 = l1 + w.ny, l3 = l2 + 1, l4 = l1 + 1;
+        atomicAdd((unsigned long long*)&winW[l1], (unsigned long long)v[0]);
+        atomicAdd((unsigned long long*)&winW[l2], (unsigned long long)v[1]);
+        atomicAdd((unsigned long long*)&winW[l3], (unsigned long long)v[2]);
+        atomicAdd((unsigned long long*)&winW[l4], (unsigned long long)v[3]);
+        atomicAdd((unsigned long long*)&winE[l1], (unsigned long long)v[4]);
+        atomicAdd((unsigned long long*)&winE[l2], (unsigned long long)v[5]);
+        atomicAdd((unsigned long long*)&winE[l3], (unsigned long long)v[6]);
+        atomicAdd((unsigned long long*)&winE[l4], (unsigned long long)v[7]);
+    } else {
+        int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+        atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
+        atomicAdd((unsigned long long*)&acc_w[n2], (unsigned long long)v[1]);
+        atomicAdd((unsigned long long*)&acc_w[n3], (unsigned long long)v[2]);
+        atomicAdd((unsigned long long*)&acc_w[n4], (unsigned long long)v[3]);
+        atomicAdd((unsigned long long*)&acc_e[n1], (unsigned long long)v[4]);
+        atomicAdd((unsigned long long*)&acc_e[n2], (unsigned long long)v[5]);
+        atomicAdd((unsigned long long*)&acc_e[n3], (unsigned long long)v[6]);
+        atomicAdd((unsigned long long*)&acc_e[n4], (unsigned long long)v[7]);
+    }
+}
+
+// Segmented inclusive sum over the wave of eight int64 values keyed by `key` (runs of
+// equal keys in lane order); the lane at the tail of each run adds the run total.
 __device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8], long long* acc_w, long long* acc_e,
                                     const DepWindow& w, long long* winW, long long* winE) {
     const int lane = threadIdx.x & 63;
@@ -69,7 +105,7 @@ __device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8]
     unsigned long long heads = __ballot(head);
     unsigned long long below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
     int seg_start = 63 - __clzll((long long)below);
-    if (heads != 1ull) {      // more than one run in the wave: segmented scan
+    if (heads != 1ull) {
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             bool take = (lane - off) >= seg_start;
@@ -79,38 +115,14 @@ __device__ inline void wave_deposit(int key, const MeshParams& m, long long v[8]
                 if (take) v[k] += u;
             }
         }
-    } else {                  // one run: butterfly reduction, total in every lane
+    } else {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) v[k] += shfl_xor_ll(v[k], off);
         }
     }
-    if (tail && key >= 0) {
-        int ci = key / m.ney, cj = key - ci * m.ney;
-        int li = ci - w.x0, lj = cj - w.y0;
-        if (li >= 0 && lj >= 0 && li + 1 < w.nx && lj + 1 < w.ny) {
-            int l1 = li * w.ny + lj, l2 = l1 + w.ny, l3 = l2 + 1, l4 = l1 + 1;
-            atomicAdd((unsigned long long*)&winW[l1], (unsigned long long)v[0]);
-            atomicAdd((unsigned long long*)&winW[l2], (unsigned long long)v[1]);
-            atomicAdd((unsigned long long*)&winW[l3], (unsigned long long)v[2]);
-            atomicAdd((unsigned long long*)&winW[l4], (unsigned long long)v[3]);
-            atomicAdd((unsigned long long*)&winE[l1], (unsigned long long)v[4]);
-            atomicAdd((unsigned long long*)&winE[l2], (unsigned long long)v[5]);
-            atomicAdd((unsigned long long*)&winE[l3], (unsigned long long)v[6]);
-            atomicAdd((unsigned long long*)&winE[l4], (unsigned long long)v[7]);
-        } else {
-            int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-            atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
-            atomicAdd((unsigned long long*)&acc_w[n2], (unsigned long long)v[1]);
-            atomicAdd((unsigned long long*)&acc_w[n3], (unsigned long long)v[2]);
-            atomicAdd((unsigned long long*)&acc_w[n4], (unsigned long long)v[3]);
-            atomicAdd((unsigned long long*)&acc_e[n1], (unsigned long long)v[4]);
-            atomicAdd((unsigned long long*)&acc_e[n2], (unsigned long long)v[5]);
-            atomicAdd((unsigned long long*)&acc_e[n3], (unsigned long long)v[6]);
-            atomicAdd((unsigned long long*)&acc_e[n4], (unsigned long long)v[7]);
-        }
-    }
+    if (tail && key >= 0) add_element(key, v, m, acc_w, acc_e, w, winW, winE);
 }
 
 // Specular folding of y into [ymin, ymax] (reflections off both walls), Vy reversed.
@@ -128,13 +140,14 @@ __device__ inline void fold_y(R& Y, R& VY, R ymin, R ymax) {
     VY = -VY;
 }
 
-// Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64.
+// Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64. With a
+// power-of-two spacing the division is the (bit-identical) multiplication by 1/h.
 __device__ inline void deposit_weights(double X, double Y, int e, const MeshParams& m, double W[4]) {
     int ci = e / m.ney, cj = e - ci * m.ney;
     double x1 = m.xs[ci], x2 = m.xs[ci + 1], y1 = m.ys[cj], y2 = m.ys[cj + 1];
     double hxp = x2 - x1, hyp = y2 - y1;
-    double dx = (X - x1) / hxp;
-    double dy = (Y - y1) / hyp;
+    double dx = (m.pow2_map && hxp == m.hx) ? (X - x1) * m.hxi : (X - x1) / hxp;
+    double dy = (m.pow2_map && hyp == m.hy) ? (Y - y1) * m.hyi : (Y - y1) / hyp;
     double xm = 1.0 - dx, ym = 1.0 - dy;
     if (m.delta == 0) {
         W[0] = xm * ym; W[1] = dx * ym; W[2] = dx * dy; W[3] = xm * dy;
@@ -148,212 +161,179 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
     }
 }
 
-template <typename R, int DO_PUSH>
-__device__ __forceinline__ void push_one(const PushArgs<R>& a, int64_t p, const DepWindow& w,
-                                         long long* winW, long long* winE) {
+struct PushEvents { int emit, right, object, reflect, deleted; };
+
+// MoveAll for one particle (MCCInterface.f90:591-615 with MoveOne + AdjustParticleBoundary):
+// state in registers, returns false when the particle is deleted.
+template <typename R>
+__device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X, R& Y, R& VX, R& VY, R& VZ,
+                                bool& vz_changed, PushEvents& ev) {
     const MeshParams& m = a.m;
-    bool live = false;
-    int e_new = -1;
-    R X = 0, Y = 0, VX = 0, VY = 0, VZ = 0;
+    R T = a.dt;
+    int ivel = 1, ipos = 1, it = 0;
+    pic_stream rs; bool rs_init = false;
     bool emitted = false, absorbed_wall = false, absorbed_obj = false, reflected = false;
-    if (p < a.n) {
-        int e_old = a.cell[p];
-        if (e_old >= 0) {
-            live = true;
-            X = a.x[p]; Y = a.y[p]; VX = a.vx[p]; VY = a.vy[p]; VZ = a.vz[p];
-            if (DO_PUSH) {
-                R T = a.dt;
-                int ivel = 1, ipos = 1, it = 0;
-                int n = e_old;
-                pic_stream rs; bool rs_init = false;
-                while (ivel == 1 || ipos == 1) {
-                    if (++it > 64) {
-                        atomicOr(a.err, PIC_EBIT_BOUNDARY);
-                        pic_record(a.errinfo, PIC_EBIT_BOUNDARY, p, X, Y, VX, VY, it, e_old, -1, a.species);
-                        break;
-                    }
-                    // ---------------- MoveOne: gather + push (MoveOne.f90:77-146, 213-219)
-                    if (ivel == 1) {
-                        int ci = n / m.ney, cj = n - ci * m.ney;
-                        R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
-                        R hxp = x2 - x1, hyp = y2 - y1;
-                        R dx = (X - x1) / hxp;
-                        R dy = (Y - y1) / hyp;
-                        R xm = (R)1 - dx, ym = (R)1 - dy;
-                        R W1, W2, W3, W4;
-                        if (m.delta == 0) {
-                            W1 = xm * ym; W2 = dx * ym; W3 = dx * dy; W4 = xm * dy;
-                        } else {
-                            R R1 = (R)m.ymin + y1, R2 = (R)m.ymin + y2, Rp = Y, den = R2 * R2 - R1 * R1;
-                            W1 = xm * (R2 * R2 - Rp * Rp) / den;
-                            W2 = dx * (R2 * R2 - Rp * Rp) / den;
-                            W3 = dx * (Rp * Rp - R1 * R1) / den;
-                            W4 = xm * (Rp * Rp - R1 * R1) / den;
-                        }
-                        int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-                        R Ex, Ey;
-                        if (m.element_index[n] > 0) {
-                            // Get_ParticleEField_PPR.f90:19-33: vacuum-side nodes only
-                            R ex = 0, ey = 0;
-                            int count = 0;
-                            int nd[4] = {n1, n2, n3, n4};
-                            R nx_[4] = {x1, x2, x2, x1}, ny_[4] = {y1, y1, y2, y2};
+    while (ivel == 1 || ipos == 1) {
+        if (++it > 64) {
+            atomicOr(a.err, PIC_EBIT_BOUNDARY);
+            pic_record(a.errinfo, PIC_EBIT_BOUNDARY, p, X, Y, VX, VY, it, e_old, -1, a.species);
+            break;
+        }
+        // ---------------- MoveOne: gather + push (MoveOne.f90:77-146, 213-219)
+        if (ivel == 1) {
+            int n = e_old;
+            int ci = n / m.ney, cj = n - ci * m.ney;
+            R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
+            R hxp = x2 - x1, hyp = y2 - y1;
+            R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
+            R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
+            R xm = (R)1 - dx, ym = (R)1 - dy;
+            R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
+            int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+            R Ex, Ey;
+            if (m.element_index[n] > 0) {
+                // Get_ParticleEField_PPR.f90:19-33: vacuum-side nodes only
+                R ex = 0, ey = 0;
+                int count = 0;
+                const int nd[4] = {n1, n2, n3, n4};
+                const R nx_[4] = {x1, x2, x2, x1}, ny_[4] = {y1, y1, y2, y2};
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                if (m.node_index[nd[k]] == -1) {
-                                    const R* c = a.coef5 + (size_t)nd[k] * 5;
-                                    count++;
-                                    R xr = X - nx_[k], yr = Y - ny_[k];
-                                    ex = ex + c[0] + (R)2 * c[2] * xr + c[3] * yr;
-                                    ey = ey + c[1] + (R)2 * c[4] * yr + c[3] * xr;
-                                }
-                            }
-                            ex = ex / (R)count;
-                            ey = ey / (R)count;
-                            Ex = -ex; Ey = -ey;
-                        } else {
-                            Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
-                            Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
-                        }
-                        if (m.delta == 0) {
-                            VX = VX + a.qm * Ex * T;
-                            VY = VY + a.qm * Ey * T;
-                        } else {
-                            // axisymmetric branch, MoveOne.f90:220-235 with Z carried as theta = 0
-                            VX = VX + a.qm * Ex * T;
-                            VY = VY + a.qm * Ey * T;
-                        }
-                        ivel = 0;
-                    }
-                    X = X + T * VX;
-                    Y = Y + T * VY;
-                    ipos = 0;
-                    // ---------------- AdjustParticleBoundary (MCCInterface.f90:361-499)
-                    // Back-up times lie in [0, T] whenever the reference's arithmetic is
-                    // well posed; outside [-T/2, 2T] (possible only after the re-emission has
-                    // divided Vy by v_ref again) the position is folded into the domain instead.
-                    if (X <= a.xmin) {
-                        ipos = 1;
-                        R TR = (X - a.xmin) / VX;
-                        bool ok = TR >= (R)-0.5 * T && TR <= (R)2 * T;
-                        if (ok) { T = TR; Y = Y - VY * TR; }
-                        X = a.xmin + a.emit_eps;
-                        if (!rs_init) { pic_stream_init(&rs, a.seed, (uint32_t)p, a.step, a.tag_emit); rs_init = true; }
-                        double R1 = pic_stream_u01(&rs);
-                        // Kappa = 2: ranf**(-1/(Kappa-1)) == 1/ranf
-                        double vxs = sqrt(a.kappa_c * (1.0 / R1 - 1.0));
-                        VX = (R)(vxs * a.vref_inv);
-                        VY = (R)((double)VY * a.vref_inv);
-                        VZ = (R)((double)VZ * a.vref_inv);
-                        emitted = true;
-                        if (!ok) { T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax); }
-                    } else if (X > a.xmax) {
-                        X = (R)-2000;
-                        absorbed_wall = true;
-                    } else if (Y < a.ymin) {
-                        if (m.delta == 0) {
-                            ipos = 1;
-                            R TR = (Y - a.ymin) / VY;
-                            reflected = true;
-                            if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
-                                T = TR;
-                                X = X - VX * TR;
-                                VY = -VY;
-                                Y = a.ymin + a.emit_eps;
-                                if (Y > a.ymax) atomicOr(a.err, PIC_EBIT_BOUNDARY);
-                            } else {
-                                T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
-                            }
-                        } else {
-                            atomicOr(a.err, PIC_EBIT_WRONGCROSS);
-                        }
-                    } else if (Y > a.ymax) {
-                        if (m.delta == 0) {
-                            ipos = 1;
-                            R TR = (Y - a.ymax) / VY;
-                            reflected = true;
-                            if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
-                                T = TR;
-                                X = X - VX * TR;
-                                VY = -VY;
-                                Y = a.ymax - a.emit_eps;
-                                if (Y < a.ymin) atomicOr(a.err, PIC_EBIT_BOUNDARY);
-                            } else {
-                                T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
-                            }
-                        }
-                    }
-                    for (int k = 0; k < m.n_objects; ++k) {
-                        const DevObject& o = m.obj[k];
-                        double xd = (double)X, yd = (double)Y;
-                        bool in = false;
-                        if (o.shape == 3) in = xd > o.x0 && xd < o.x1 && yd > o.y0 && yd < o.y1;
-                        else if (o.shape == 1) in = (xd - o.x0) * (xd - o.x0) + (yd - o.y0) * (yd - o.y0) < o.r2;
-                        if (in) { X = (R)-2000; absorbed_obj = true; }
+                for (int k = 0; k < 4; ++k) {
+                    if (m.node_index[nd[k]] == -1) {
+                        const R* c = a.coef5 + (size_t)nd[k] * 5;
+                        count++;
+                        R xr = X - nx_[k], yr = Y - ny_[k];
+                        ex = ex + c[0] + (R)2 * c[2] * xr + c[3] * yr;
+                        ey = ey + c[1] + (R)2 * c[4] * yr + c[3] * xr;
                     }
                 }
-                if (X <= a.xmin) live = false;      // MoveAll deletion test, MCCInterface.f90:617-620
-                if (!(X == X) || !(Y == Y) || !(VX == VX) || !(VY == VY)) { atomicOr(a.err, PIC_EBIT_NONFINITE); live = false; }
-                a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
-                if (emitted) a.vz[p] = VZ;
+                ex = ex / (R)count;
+                ey = ey / (R)count;
+                Ex = -ex; Ey = -ey;
+            } else {
+                Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
+                Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
             }
-            if (live) {
-                int bad = 0;
-                e_new = pic_locate_elem((double)X, (double)Y, m, &bad);
-                if (bad) {
-                    atomicOr(a.err, PIC_EBIT_POSITION);
-                    pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, e_new, a.species);
+            VX = VX + a.qm * Ex * T;
+            VY = VY + a.qm * Ey * T;
+            ivel = 0;
+        }
+        X = X + T * VX;
+        Y = Y + T * VY;
+        ipos = 0;
+        // ---------------- AdjustParticleBoundary (MCCInterface.f90:361-499)
+        // Back-up times lie in [0, T] whenever the reference's arithmetic is well posed;
+        // outside [-T/2, 2T] (possible only after the re-emission has divided Vy by v_ref
+        // again) the position is folded into the domain instead.
+        if (X <= a.xmin) {
+            ipos = 1;
+            R TR = (X - a.xmin) / VX;
+            bool ok = TR >= (R)-0.5 * T && TR <= (R)2 * T;
+            if (ok) { T = TR; Y = Y - VY * TR; }
+            X = a.xmin + a.emit_eps;
+            if (!rs_init) { pic_stream_init(&rs, a.seed, (uint32_t)p, a.step, a.tag_emit); rs_init = true; }
+            double R1 = pic_stream_u01(&rs);
+            // Kappa = 2: ranf**(-1/(Kappa-1)) == 1/ranf
+            double vxs = sqrt(a.kappa_c * (1.0 / R1 - 1.0));
+            VX = (R)(vxs * a.vref_inv);
+            VY = (R)((double)VY * a.vref_inv);
+            VZ = (R)((double)VZ * a.vref_inv);
+            emitted = true;
+            vz_changed = true;
+            if (!ok) { T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax); }
+        } else if (X > a.xmax) {
+            X = (R)-2000;
+            absorbed_wall = true;
+        } else if (Y < a.ymin) {
+            if (m.delta == 0) {
+                ipos = 1;
+                R TR = (Y - a.ymin) / VY;
+                reflected = true;
+                if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
+                    T = TR;
+                    X = X - VX * TR;
+                    VY = -VY;
+                    Y = a.ymin + a.emit_eps;
+                    if (Y > a.ymax) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                } else {
+                    T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
+                }
+            } else {
+                atomicOr(a.err, PIC_EBIT_WRONGCROSS);
+            }
+        } else if (Y > a.ymax) {
+            if (m.delta == 0) {
+                ipos = 1;
+                R TR = (Y - a.ymax) / VY;
+                reflected = true;
+                if (TR >= (R)-0.5 * T && TR <= (R)2 * T) {
+                    T = TR;
+                    X = X - VX * TR;
+                    VY = -VY;
+                    Y = a.ymax - a.emit_eps;
+                    if (Y < a.ymin) atomicOr(a.err, PIC_EBIT_BOUNDARY);
+                } else {
+                    T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
                 }
             }
-            a.cell_next[p] = live ? e_new : -1;
+        }
+        for (int k = 0; k < m.n_objects; ++k) {
+            const DevObject o = a.objs[k];
+            double xd = (double)X, yd = (double)Y;
+            bool in = false;
+            if (o.shape == 3) in = xd > o.x0 && xd < o.x1 && yd > o.y0 && yd < o.y1;
+            else if (o.shape == 1) in = (xd - o.x0) * (xd - o.x0) + (yd - o.y0) * (yd - o.y0) < o.r2;
+            if (in) { X = (R)-2000; absorbed_obj = true; }
         }
     }
-    if (DO_PUSH) {
-        wave_count(&a.counters[PIC_C_EMIT], emitted);
-        wave_count(&a.counters[PIC_C_RIGHT], absorbed_wall && !live);
-        wave_count(&a.counters[PIC_C_OBJECT], absorbed_obj && !live);
-        wave_count(&a.counters[PIC_C_REFLECT], reflected);
-        wave_count(&a.counters[PIC_C_DELETED], p < a.n && a.cell[p] >= 0 && !live);
-    }
-    // ---------------- deposition (GetParChg_2D.f90:788-843), fixed point
-    long long v[8];
-    int key = -1;
-    if (live) {
-        double W[4];
-        double Xd = (double)X, Yd = (double)Y;
-        deposit_weights(Xd, Yd, e_new, m, W);
-        double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
-        // EnergyParticleOne (ParticleOne.f90:339-340) then /JtoeV (GetParChg_2D.f90:128)
-        double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
-        energy = energy * a.v_ref * a.v_ref;
-        energy = energy / PIC_JTOEV;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { v[k] = pic_fix(W[k]); v[4 + k] = pic_fix(W[k] * energy); }
-        key = e_new;
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = 0;
-    }
-    wave_deposit(key, m, v, a.acc_w, a.acc_e, w, winW, winE);
+    bool live = !(X <= a.xmin);          // MoveAll deletion test, MCCInterface.f90:617-620
+    if (!(X == X) || !(Y == Y) || !(VX == VX) || !(VY == VY)) { atomicOr(a.err, PIC_EBIT_NONFINITE); live = false; }
+    ev.emit += emitted; ev.reflect += reflected;
+    ev.right += (absorbed_wall && !live); ev.object += (absorbed_obj && !live); ev.deleted += !live;
+    return live;
 }
 
+template <typename R> struct Vec8;
+template <> struct Vec8<float> {
+    static __device__ inline void load(const float* p, float v[8]) {
+        float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    }
+    static __device__ inline void store(float* p, const float v[8]) {
+        *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
+        *(float4*)(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    }
+};
+template <> struct Vec8<double> {
+    static __device__ inline void load(const double* p, double v[8]) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { double2 a = *(const double2*)(p + 2 * k); v[2 * k] = a.x; v[2 * k + 1] = a.y; }
+    }
+    static __device__ inline void store(double* p, const double v[8]) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) *(double2*)(p + 2 * k) = make_double2(v[2 * k], v[2 * k + 1]);
+    }
+};
+__device__ inline void load8i(const int* p, int v[8]) {
+    int4 a = *(const int4*)p, b = *(const int4*)(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+}
+__device__ inline void store8i(int* p, const int v[8]) {
+    *(int4*)p = make_int4(v[0], v[1], v[2], v[3]);
+    *(int4*)(p + 4) = make_int4(v[4], v[5], v[6], v[7]);
+}
 
-// Fused mover + lookup + deposition. A block of PUSH_BLOCK threads takes PUSH_PPT *
-// PUSH_BLOCK consecutive slots (slot = base + k*PUSH_BLOCK + tid, coalesced). Its node
-// moments go to an LDS window covering the 3x3 sort tiles around the tile of the block's
-// middle particle (slots are sorted tile-major, so almost every deposit lands there);
-// the window is flushed with one global atomic per touched node and moment. Deposits
-// outside the window go straight to global atomics. Integer sums: the result does not
-// depend on which path a deposit takes.
+// Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
 template <typename R, int DO_PUSH>
 __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
     __shared__ int s_center;
-    const int64_t base = (int64_t)blockIdx.x * (PUSH_BLOCK * PUSH_PPT);
     const MeshParams& m = a.m;
+    const int64_t base = (int64_t)blockIdx.x * PUSH_SLOTS;
+    // window: the 3x3 sort tiles around the tile of the first live particle at or after
+    // the block's middle slot
     if (threadIdx.x < 64) {
-        // first live particle at or after the block's middle slot (one wave looks)
-        int64_t mid = base + (PUSH_BLOCK * PUSH_PPT) / 2;
+        int64_t mid = base + PUSH_SLOTS / 2;
         if (mid >= a.n) mid = base;
         int64_t q = mid + threadIdx.x;
         int c = (q < a.n) ? a.cell[q] : -1;
@@ -369,7 +349,7 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
         int ci = s_center / m.ney, cj = s_center - ci * m.ney;
         int ti = ci / SORT_TILE, tj = cj / SORT_TILE;
         int x0 = (ti - 1) * SORT_TILE, y0 = (tj - 1) * SORT_TILE;
-        int x1 = (ti + 2) * SORT_TILE, y1 = (tj + 2) * SORT_TILE;     // last node index of the 3x3 tiles
+        int x1 = (ti + 2) * SORT_TILE, y1 = (tj + 2) * SORT_TILE;
         if (x0 < 0) x0 = 0;
         if (y0 < 0) y0 = 0;
         if (x1 > m.nnx - 1) x1 = m.nnx - 1;
@@ -378,16 +358,111 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     }
     const int wn = w.nx * w.ny;
     for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) { winW[k] = 0; winE[k] = 0; }
-    __syncthreads();
-    for (int k = 0; k < PUSH_PPT; ++k) {
-        int64_t p = base + (int64_t)k * PUSH_BLOCK + threadIdx.x;
-        push_one<R, DO_PUSH>(a, p, w, winW, winE);
+
+    // ---------------- load this thread's PUSH_PPT consecutive slots
+    const int64_t p0 = base + (int64_t)threadIdx.x * PUSH_PPT;
+    const bool full = p0 + PUSH_PPT <= a.n;
+    R X[PUSH_PPT], Y[PUSH_PPT], VX[PUSH_PPT], VY[PUSH_PPT], VZ[PUSH_PPT];
+    int cel[PUSH_PPT];
+    if (full) {
+        load8i(a.cell + p0, cel);
+        Vec8<R>::load(a.x + p0, X); Vec8<R>::load(a.y + p0, Y);
+        Vec8<R>::load(a.vx + p0, VX); Vec8<R>::load(a.vy + p0, VY); Vec8<R>::load(a.vz + p0, VZ);
+    } else {
+#pragma unroll
+        for (int k = 0; k < PUSH_PPT; ++k) {
+            int64_t p = p0 + k;
+            bool in = p < a.n;
+            cel[k] = in ? a.cell[p] : -1;
+            X[k] = in ? a.x[p] : (R)0; Y[k] = in ? a.y[p] : (R)0;
+            VX[k] = in ? a.vx[p] : (R)0; VY[k] = in ? a.vy[p] : (R)0; VZ[k] = in ? a.vz[p] : (R)0;
+        }
     }
+    __syncthreads();     // window zeroed before any deposit
+
+    // ---------------- move, locate, deposit (run accumulation in registers)
+    PushEvents ev = {0, 0, 0, 0, 0};
+    int next[PUSH_PPT];
+    int vzmask = 0;
+    int rkey = -1;
+    long long acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0;
+    for (int k = 0; k < PUSH_PPT; ++k) {
+        int e_old = cel[k];
+        next[k] = -1;
+        if (e_old < 0) continue;
+        int64_t p = p0 + k;
+        bool live = true;
+        if (DO_PUSH) {
+            bool vzc = false;
+            live = move_one<R>(a, p, e_old, X[k], Y[k], VX[k], VY[k], VZ[k], vzc, ev);
+            if (vzc) vzmask |= 1 << k;
+        }
+        if (!live) continue;
+        int bad = 0;
+        double Xd = (double)X[k], Yd = (double)Y[k];
+        int e_new = pic_locate_elem(Xd, Yd, m, &bad);
+        if (bad) {
+            atomicOr(a.err, PIC_EBIT_POSITION);
+            pic_record(a.errinfo, PIC_EBIT_POSITION, p, X[k], Y[k], VX[k], VY[k], 0, e_old, e_new, a.species);
+        }
+        next[k] = e_new;
+        // GetParChg_2D.f90:788-843: weights and kinetic energy (ParticleOne.f90:339-340, :128)
+        double W[4];
+        deposit_weights(Xd, Yd, e_new, m, W);
+        double vxd = (double)VX[k], vyd = (double)VY[k], vzd = (double)VZ[k];
+        double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
+        energy = energy * a.v_ref * a.v_ref;
+        energy = energy / PIC_JTOEV;
+        long long qv[8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { qv[q] = pic_fix(W[q]); qv[4 + q] = pic_fix(W[q] * energy); }
+        if (e_new != rkey) {
+            if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
+            rkey = e_new;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] = qv[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc[q] += qv[q];
+        }
+    }
+    wave_deposit(rkey, m, acc, a.acc_w, a.acc_e, w, winW, winE);
+
+    // ---------------- write back
+    if (full) {
+        store8i(a.cell_next + p0, next);
+        if (DO_PUSH) {
+            Vec8<R>::store(a.x + p0, X); Vec8<R>::store(a.y + p0, Y);
+            Vec8<R>::store(a.vx + p0, VX); Vec8<R>::store(a.vy + p0, VY);
+        }
+    } else {
+        for (int k = 0; k < PUSH_PPT; ++k) {
+            int64_t p = p0 + k;
+            if (p >= a.n) break;
+            a.cell_next[p] = next[k];
+            if (DO_PUSH) { a.x[p] = X[k]; a.y[p] = Y[k]; a.vx[p] = VX[k]; a.vy[p] = VY[k]; }
+        }
+    }
+    if (DO_PUSH && vzmask) {
+        for (int k = 0; k < PUSH_PPT; ++k) if (vzmask & (1 << k)) a.vz[p0 + k] = VZ[k];
+    }
+    if (DO_PUSH) {
+        wave_add(&a.counters[PIC_C_EMIT], ev.emit);
+        wave_add(&a.counters[PIC_C_RIGHT], ev.right);
+        wave_add(&a.counters[PIC_C_OBJECT], ev.object);
+        wave_add(&a.counters[PIC_C_REFLECT], ev.reflect);
+        wave_add(&a.counters[PIC_C_DELETED], ev.deleted);
+    }
+
+    // ---------------- flush the window
     __syncthreads();
     for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) {
+        long long vw = winW[k], ve = winE[k];
+        if (vw == 0 && ve == 0) continue;
         int i = k / w.ny, j = k - i * w.ny;
         int node = (w.x0 + i) * m.nny + (w.y0 + j);
-        long long vw = winW[k], ve = winE[k];
         if (vw != 0) atomicAdd((unsigned long long*)&a.acc_w[node], (unsigned long long)vw);
         if (ve != 0) atomicAdd((unsigned long long*)&a.acc_e[node], (unsigned long long)ve);
     }
@@ -402,6 +477,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.m = g.mesh;
     a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
     a.m.element_index = g.d_element_index; a.m.node_index = g.d_node_index;
+    a.objs = g.d_objects;
     if (sizeof(R) == 8) {
         a.efx = (const R*)g.d_efx; a.efy = (const R*)g.d_efy; a.coef5 = (const R*)g.d_coef5;
         a.xsR = (const R*)g.d_xs; a.ysR = (const R*)g.d_ys;
@@ -412,6 +488,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.qm = (R)S.qm; a.dt = (R)g.cfg.dt;
     a.xmin = (R)g.mesh.xmin; a.xmax = (R)g.mesh.xmax; a.ymin = (R)g.mesh.ymin; a.ymax = (R)g.mesh.ymax;
     a.emit_eps = (R)(double)10.0e-6f;
+    a.hxR = (R)g.mesh.hx; a.hxiR = (R)g.mesh.hxi; a.hyR = (R)g.mesh.hy; a.hyiR = (R)g.mesh.hyi;
     a.vref_inv = 1.0 / g.v_ref;
     double beta = S.p.mass_kg / (2.0 * PIC_KB * S.p.init_temperature_K);
     a.kappa_c = (1.0 * 2.0 - 1.5) / beta;
@@ -443,7 +520,7 @@ int launch_push_deposit(int isp, bool do_push) {
     SpeciesHost& S = g.sp[isp];
     if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
     if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
-    int blocks = (int)((S.nslots + PUSH_BLOCK * PUSH_PPT - 1) / (PUSH_BLOCK * PUSH_PPT));
+    int blocks = (int)((S.nslots + PUSH_SLOTS - 1) / PUSH_SLOTS);
     if (g.particle_bits == 64) {
         PushArgs<double> a = make_push_args<double>(isp);
         if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);

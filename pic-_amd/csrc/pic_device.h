@@ -23,16 +23,17 @@ PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
     if (i == m.nnx) i = i - 1;
     double modx, mody;
     if (m.pow2_map) {
-        modx = rxp - m.mapx * (double)(long long)(rxp / m.mapx);
-        mody = ryp - m.mapy * (double)(long long)(ryp / m.mapy);
+        // exact: a power-of-two divisor (1/mapx is exact and x*(1/mapx) == x/mapx)
+        modx = rxp - m.mapx * (double)(long long)(rxp * m.imapx);
+        mody = ryp - m.mapy * (double)(long long)(ryp * m.imapy);
     } else {
         modx = fmod(rxp, m.mapx);
         mody = fmod(ryp, m.mapy);
     }
     if (modx > 1e-10 && mody > 1e-10) {
-        double rx = (X - m.xmin) / m.mapx;
+        double rx = m.pow2_map ? (X - m.xmin) * m.imapx : (X - m.xmin) / m.mapx;
         i = (int)rx + 1;
-        double ry = (Y - m.ymin) / m.mapy;
+        double ry = m.pow2_map ? (Y - m.ymin) * m.imapy : (Y - m.ymin) / m.mapy;
         j = (int)ry + 1;
     }
     if (i < 1 || i > m.nex || j < 1 || j > m.ney || !(X == X) || !(Y == Y)) {

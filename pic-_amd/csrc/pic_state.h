@@ -57,6 +57,7 @@ struct MeshParams {
     double hx, hy, hxi, hyi;            // hx(1:2), hxi
     double vox, voy;                    // Vert_o(1:2) = min - h
     double mapx, mapy;                  // hx / 2^r
+    double imapx, imapy;                // 1 / mapx, 1 / mapy
     int pow2_map;                       // 1 if mapx, mapy are powers of two (exact division)
     const double* xs; const double* ys; // node coordinate lines
     const float* xsf; const float* ysf;
@@ -135,6 +136,7 @@ struct PicState {
     float *d_xsf = nullptr, *d_ysf = nullptr;
     int *d_element_index = nullptr, *d_node_index = nullptr;
     double* d_cell_volume = nullptr;   // nnodes (row 2 of Cell_Volume_zwz)
+    DevObject* d_objects = nullptr;    // objects for the particle boundary test
     double* d_dirichlet_phi = nullptr;
     int* d_node_type2 = nullptr;
     int* d_unknown_node = nullptr;     // 0-based node of each unknown

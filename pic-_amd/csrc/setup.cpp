@@ -111,6 +111,7 @@ int setup_build_domain(const pic_domain_config* c) {
     m.hx = hx; m.hy = hy; m.hxi = 1.0 / hx; m.hyi = 1.0 / hy;
     m.vox = c->xmin - hx; m.voy = c->ymin - hy;                 // Vert_o = min - h
     m.mapx = hx; m.mapy = hy;                                    // hx / 2^0
+    m.imapx = 1.0 / hx; m.imapy = 1.0 / hy;
     int ex_;
     m.pow2_map = (std::frexp(hx, &ex_) == 0.5 && std::frexp(hy, &ex_) == 0.5) ? 1 : 0;
     m.n_objects = c->n_objects;
