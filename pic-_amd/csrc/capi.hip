@@ -292,6 +292,31 @@ int pic_setup_domain(const pic_domain_config* cfg) {
         float** fv[7] = {&g.d_bfn, &g.d_xfn, &g.d_rfn, &g.d_zfn, &g.d_pfn, &g.d_p2fn, &g.d_qfn};
         for (int k = 0; k < 7; ++k) *fv[k] = f0 + k * len + pad;
         g.pcg_pstride = (m.nnodes + 255) / 256 + 1;
+        // Jacobi-scaled off-diagonals c_k[i] / sqrt(d_i d_j) for the FP32 CG (0 where d == 0)
+        std::vector<double> sinv(m.nnodes, 0.0);
+        for (int i = 0; i < m.nnodes; ++i) sinv[i] = dia[4][i] > 0.0 ? 1.0 / std::sqrt(dia[4][i]) : 0.0;
+        for (int q = 0, kk = 0; q < 9; ++q) {
+            if (q == 4) continue;
+            std::vector<float> f(m.nnodes, 0.0f);
+            for (int i = 0; i < m.nnodes; ++i) {
+                int j = i + off[q];
+                if (dia[q][i] != 0.0 && j >= 0 && j < m.nnodes) f[i] = (float)(dia[q][i] * sinv[i] * sinv[j]);
+            }
+            g.cg_off[kk] = off[q];
+            TRY(dupload(&g.d_cg_off[kk], f));
+            kk++;
+        }
+        // padded: eight FP32 vectors, then D^-1/2 (FP64; read at the neighbours of every row)
+        if (g.d_cgvec) { (void)hipFree(g.d_cgvec); g.d_cgvec = nullptr; }
+        size_t cg_bytes = len * 8 * sizeof(float) + len * sizeof(double);
+        HIPCHK(hipMalloc(&g.d_cgvec, cg_bytes));
+        HIPCHK(hipMemset(g.d_cgvec, 0, cg_bytes));
+        float* c0 = (float*)g.d_cgvec;
+        float** cv[8] = {&g.d_cg_r[0], &g.d_cg_r[1], &g.d_cg_w[0], &g.d_cg_w[1], &g.d_cg_s[0], &g.d_cg_s[1], &g.d_cg_p, &g.d_cg_x};
+        for (int k = 0; k < 8; ++k) *cv[k] = c0 + k * len + pad;
+        g.d_sinv = (double*)(c0 + 8 * len) + pad;
+        HIPCHK(hipMemcpy(g.d_sinv, sinv.data(), sizeof(double) * m.nnodes, hipMemcpyHostToDevice));
+        if (const char* e = std::getenv("PIC_CG_RPT")) { int v = std::atoi(e); if (v == 1 || v == 2 || v == 4) g.cg_rpt = v; }
     }
     size_t nblk = (size_t)std::max(m.nnodes, nu) / 256 + 2;
     TRY(dalloc(&g.d_partials, std::max<size_t>(4 * nblk + 64, (size_t)4 * g.pcg_pstride + 64)));

@@ -94,7 +94,8 @@ int launch_ppr() {
 #define PCG_BLOCK 256
 
 // scalar slots in d_scal
-enum { S_RHO0 = 0, S_RHO1, S_ALPHA, S_BETA, S_NORMRHS, S_NORMR, S_DONE, S_ITER, S_IERR, S_TOL2, S_MAXIT, S_PQ, S_NSCAL };
+enum { S_RHO0 = 0, S_RHO1, S_ALPHA, S_BETA, S_NORMRHS, S_NORMR, S_DONE, S_ITER, S_IERR, S_TOL2, S_MAXIT, S_PQ,
+       S_GAM0, S_GAM1, S_ALP0, S_ALP1, S_NSCAL };
 // partial-sum regions
 enum { P_RZ = 0, P_RR = 1, P_BB = 2, P_PQ = 3, P_NREG = 4 };
 
@@ -290,6 +291,183 @@ __global__ void k_xn_to_phi(int n, const int* nt2, const double* x, double* phi)
     if (i < n && nt2[i] > 0) phi[i] = x[i];
 }
 
+// ---------------------------------------------------------------- one-kernel CG (FP32 inner solve)
+// Jacobi-preconditioned CG on the symmetrically scaled system D^-1/2 A D^-1/2 y = D^-1/2 b
+// (the same iterates as Jacobi-PCG on A x = b, with a unit diagonal, so the preconditioned
+// residual is the residual itself), in the Chronopoulos-Gear arrangement: one reduction
+// per iteration (gamma = r.r, delta = w.r with w = A r). Every iteration is ONE kernel:
+//   beta = gamma/gamma_old, alpha = gamma/(delta - beta*gamma/alpha_old)   (prologue)
+//   s = w + beta s, p = r + beta p, x += alpha p, r' = r - alpha s, w' = A r'
+// where w' needs r' at the eight neighbours; it recomputes them from the previous
+// iteration's r, w, s (double-buffered), with the same expression the owning row uses, so
+// the values are identical. Partials of gamma' and delta' go to the other partial buffer.
+#define CG_BLOCK 256
+enum { CG_PG = 0, CG_PD = 1 };   // partial regions (x2 parities)
+
+struct Dia8 { const float* c[8]; int off[8]; };
+
+// Sum of two partial regions over nb blocks, fixed order, identical in every block.
+__device__ inline void cg_reduce2(const double* pa, const double* pb, int nb, double& a, double& b) {
+    __shared__ double sh[2][CG_BLOCK / 64];
+    double sa = 0.0, sb = 0.0;
+    for (int k = threadIdx.x; k < nb; k += CG_BLOCK) { sa += pa[k]; sb += pb[k]; }
+    for (int off = 32; off > 0; off >>= 1) { sa += __shfl_xor(sa, off, 64); sb += __shfl_xor(sb, off, 64); }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][wid] = sa; sh[1][wid] = sb; }
+    __syncthreads();
+    a = 0.0; b = 0.0;
+#pragma unroll
+    for (int w = 0; w < CG_BLOCK / 64; ++w) { a += sh[0][w]; b += sh[1][w]; }
+    __syncthreads();
+}
+
+__device__ inline void cg_partials2(double a, double b, double* pa, double* pb) {
+    __shared__ double sh[2][CG_BLOCK / 64];
+    for (int off = 32; off > 0; off >>= 1) { a += __shfl_xor(a, off, 64); b += __shfl_xor(b, off, 64); }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][wid] = a; sh[1][wid] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ta = 0.0, tb = 0.0;
+#pragma unroll
+        for (int w = 0; w < CG_BLOCK / 64; ++w) { ta += sh[0][w]; tb += sh[1][w]; }
+        pa[blockIdx.x] = ta; pb[blockIdx.x] = tb;
+    }
+}
+
+// r0 = D^-1/2 b64 (x0 = 0), w0 = A' r0; partials of r0.r0 and w0.r0 (parity 0).
+template <int RPT>
+__global__ __launch_bounds__(CG_BLOCK) void k_cg_init(int n, Dia8 A, const double* b64, const double* sinv, float* r,
+                                                      float* w, float* x, double* scal, double* part, int pstride) {
+    double g = 0.0, d = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int i = blockIdx.x * (CG_BLOCK * RPT) + q * CG_BLOCK + threadIdx.x;
+        if (i < n) {
+            float ri = (float)(sinv[i] * b64[i]);
+            float wi = ri;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                int j = i + A.off[k];
+                float rj = (float)(sinv[j] * b64[j]);
+                wi = wi + A.c[k][i] * rj;
+            }
+            r[i] = ri; w[i] = wi; x[i] = 0.0f;
+            g += (double)ri * (double)ri;
+            d += (double)wi * (double)ri;
+        }
+    }
+    cg_partials2(g, d, part + (size_t)(2 * 0 + CG_PG) * pstride, part + (size_t)(2 * 0 + CG_PD) * pstride);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_DONE] = 0.0; scal[S_ITER] = 0.0; scal[S_IERR] = 0.0; }
+}
+
+template <int RPT>
+__global__ __launch_bounds__(CG_BLOCK) void k_cg_iter(int n, int k, int nb, Dia8 A, const float* r_o, const float* w_o,
+                                                      const float* s_o, float* r_n, float* w_n, float* s_n, float* p,
+                                                      float* x, double* scal, double* part, int pstride) {
+    if (scal[S_DONE] != 0.0) return;
+    const int par = k & 1;
+    double gam, del;
+    cg_reduce2(part + (size_t)(2 * par + CG_PG) * pstride, part + (size_t)(2 * par + CG_PD) * pstride, nb, gam, del);
+    double alpha, beta = 0.0;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (k == 0) {
+        if (lead) scal[S_NORMRHS] = gam;
+        if (gam == 0.0) { if (lead) { scal[S_DONE] = 1.0; scal[S_ITER] = 0.0; } return; }
+        alpha = gam / del;
+    } else {
+        bool conv = gam / scal[S_NORMRHS] <= scal[S_TOL2];
+        bool maxed = (double)k >= scal[S_MAXIT];
+        if (conv || maxed) {
+            if (lead) { scal[S_DONE] = 1.0; scal[S_ITER] = (double)k; scal[S_NORMR] = gam; scal[S_IERR] = maxed && !conv ? 1.0 : 0.0; }
+            return;
+        }
+        double gam_old = scal[par ? S_GAM0 : S_GAM1], alp_old = scal[par ? S_ALP0 : S_ALP1];
+        beta = gam / gam_old;
+        double den = del - beta * gam / alp_old;
+        alpha = gam / den;
+    }
+    if (!(alpha > 0.0) || !(alpha < 1e300)) {      // breakdown (A not SPD in FP32, or NaN)
+        if (lead) { scal[S_DONE] = 1.0; scal[S_IERR] = 2.0; scal[S_ITER] = (double)k; }
+        return;
+    }
+    if (lead) { scal[par ? S_GAM1 : S_GAM0] = gam; scal[par ? S_ALP1 : S_ALP0] = alpha; scal[S_ITER] = (double)k; }
+    const float fa = (float)alpha, fb = (float)beta;
+    double g = 0.0, d = 0.0;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int i = blockIdx.x * (CG_BLOCK * RPT) + q * CG_BLOCK + threadIdx.x;
+        if (i < n) {
+            float si = (k == 0) ? w_o[i] : w_o[i] + fb * s_o[i];
+            float pi = (k == 0) ? r_o[i] : r_o[i] + fb * p[i];
+            float ri = r_o[i] - fa * si;
+            float wi = ri;
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk) {
+                int j = i + A.off[kk];
+                float sj = (k == 0) ? w_o[j] : w_o[j] + fb * s_o[j];
+                float rj = r_o[j] - fa * sj;
+                wi = wi + A.c[kk][i] * rj;
+            }
+            x[i] = x[i] + fa * pi;
+            p[i] = pi; s_n[i] = si; r_n[i] = ri; w_n[i] = wi;
+            g += (double)ri * (double)ri;
+            d += (double)wi * (double)ri;
+        }
+    }
+    const int np = par ^ 1;
+    cg_partials2(g, d, part + (size_t)(2 * np + CG_PG) * pstride, part + (size_t)(2 * np + CG_PD) * pstride);
+}
+
+// x64 += D^-1/2 y
+__global__ void k_cg_apply(int n, const double* sinv, const float* y, double* x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + sinv[i] * (double)y[i];
+}
+
+template <int RPT>
+static int cg1_run_t(int n, const Dia8& A, const double* b64, double tol, int maxit, int* iters, int* ierr) {
+    const int rows = CG_BLOCK * RPT;
+    int nb = (n + rows - 1) / rows;
+    if (nb > g.pcg_pstride) return pic_fail(PIC_ERR_STATE, "CG partial buffer too small");
+    double* scal = g.d_scal;
+    double* part = g.d_partials;
+    int ps = g.pcg_pstride;
+    double h2[2] = {tol * tol, (double)maxit};
+    HIPCHK(hipMemcpyAsync(scal + S_TOL2, h2, 2 * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    float *r[2] = {g.d_cg_r[0], g.d_cg_r[1]}, *w[2] = {g.d_cg_w[0], g.d_cg_w[1]}, *s[2] = {g.d_cg_s[0], g.d_cg_s[1]};
+    hipLaunchKernelGGL((k_cg_init<RPT>), dim3(nb), dim3(CG_BLOCK), 0, g.stream, n, A, b64, g.d_sinv, r[0], w[0],
+                       g.d_cg_x, scal, part, ps);
+    HIPCHK(hipGetLastError());
+    const int CHUNK = 32;
+    int k = 0;
+    for (;;) {
+        for (int c = 0; c < CHUNK; ++c, ++k) {
+            int o = k & 1, nw = o ^ 1;
+            hipLaunchKernelGGL((k_cg_iter<RPT>), dim3(nb), dim3(CG_BLOCK), 0, g.stream, n, k, nb, A, r[o], w[o], s[o],
+                               r[nw], w[nw], s[nw], g.d_cg_p, g.d_cg_x, scal, part, ps);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        if (g.h_scal_pinned[S_DONE] != 0.0) break;
+        if (k > maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
+    }
+    *iters = (int)g.h_scal_pinned[S_ITER];
+    *ierr = (int)g.h_scal_pinned[S_IERR];
+    return PIC_OK;
+}
+
+static int cg1_run(int n, const double* b64, double tol, int maxit, int* iters, int* ierr) {
+    Dia8 A;
+    for (int k = 0; k < 8; ++k) { A.c[k] = g.d_cg_off[k]; A.off[k] = g.cg_off[k]; }
+    switch (g.cg_rpt) {
+    case 1: return cg1_run_t<1>(n, A, b64, tol, maxit, iters, ierr);
+    case 2: return cg1_run_t<2>(n, A, b64, tol, maxit, iters, ierr);
+    default: return cg1_run_t<4>(n, A, b64, tol, maxit, iters, ierr);
+    }
+}
+
 static inline int pcg_blocks(int n) { return (n + PCG_BLOCK - 1) / PCG_BLOCK; }
 
 // One Jacobi-PCG solve on device vectors (storage T, operator Op). The host launches the
@@ -370,14 +548,14 @@ int run_pcg_fp64(bool) {
     return PIC_OK;
 }
 
-// FP32 Jacobi-PCG on the correction inside FP64 iterative refinement; the outer residual
-// is recomputed in FP64 and decides convergence at pcg_tol.
+// FP32 CG on the Jacobi-scaled correction equation inside FP64 iterative refinement; the
+// outer residual is recomputed in FP64 with the unscaled operator and decides convergence
+// at pcg_tol.
 int run_pcg_mixed() {
     int n = g.mesh.nnodes;
     int nb256 = (n + 255) / 256;
     hipLaunchKernelGGL(k_phi_to_xn, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_phi, g.d_xn);
     DiaOp<double> A64 = dia_op<double>(1);
-    DiaOp<float> A32 = dia_op<float>(0);
     int total = 0, outer = 0;
     double rr = 0, bb = 0;
     const double inner_tol = 1e-4;
@@ -390,13 +568,10 @@ int run_pcg_mixed() {
             g.last_iter = total; g.last_outer = outer; g.last_relres = rel;
             return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
         }
-        hipLaunchKernelGGL(k_d2f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_rn, g.d_bfn);
-        hipLaunchKernelGGL(k_zero_f, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xfn);
         int it = 0, ierr = 0;
-        rc = pcg_run<float>(n, A32, g.d_dia32[4], g.d_bfn, g.d_xfn, g.d_rfn, g.d_zfn, g.d_pfn, g.d_p2fn, g.d_qfn,
-                            inner_tol, g.pcg_maxit - total, g.d_partials, g.pcg_pstride, &it, &ierr);
+        rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_axpy_f2d, dim3(nb256), dim3(256), 0, g.stream, n, g.d_xfn, g.d_xn);
+        hipLaunchKernelGGL(k_cg_apply, dim3(nb256), dim3(256), 0, g.stream, n, g.d_sinv, g.d_cg_x, g.d_xn);
         total += it; outer++;
         if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "inner PCG breakdown");
     }

@@ -167,6 +167,14 @@ struct PicState {
     double *d_bn = nullptr, *d_xn = nullptr, *d_rn = nullptr, *d_zn = nullptr, *d_pn = nullptr, *d_p2n = nullptr, *d_qn = nullptr;
     float *d_bfn = nullptr, *d_xfn = nullptr, *d_rfn = nullptr, *d_zfn = nullptr, *d_pfn = nullptr, *d_p2fn = nullptr, *d_qfn = nullptr;
     int pcg_pstride = 0;               // partial-sum region length
+    // one-kernel CG on the Jacobi-scaled system: eight scaled off-diagonals, D^-1/2, vectors
+    float* d_cg_off[8] = {nullptr};
+    int cg_off[8] = {0};
+    double* d_sinv = nullptr;
+    void* d_cgvec = nullptr;
+    float *d_cg_r[2] = {nullptr, nullptr}, *d_cg_w[2] = {nullptr, nullptr}, *d_cg_s[2] = {nullptr, nullptr};
+    float *d_cg_p = nullptr, *d_cg_x = nullptr;
+    int cg_rpt = 1;                    // rows per thread of the CG kernels
     double* d_scal = nullptr;          // PCG scalars (device)
     double* d_partials = nullptr;      // block partial sums
     unsigned int* d_sync = nullptr;    // last-block counters
