@@ -32,13 +32,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(name: str):
-    """HBM bytes per launch of the push kernel from the committed rocprofv3 PMC summary."""
+def traffic_from_profiles(name: str) -> dict:
+    """HBM bytes per launch per kernel from the committed rocprofv3 PMC summary: lines
+    `traffic_bytes_per_launch <kernel>: <bytes>`."""
     p = ROOT / "profiles" / name
     if not p.exists():
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
-    return float(m.group(1)) if m else None
+        return {}
+    return {m.group(1): float(m.group(2)) for m in
+            re.finditer(r"traffic_bytes_per_launch\s+(\S+):\s*([0-9.eE+]+)", p.read_text())}
 
 
 def cpu_baseline(pkg, pl, w, sample_particles: int) -> dict:
@@ -100,7 +101,18 @@ def main() -> int:
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": x.numel() * 4, "launches": 5}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); the sharded design is "
               "described in DESIGN.md", file=sys.stderr)
@@ -127,13 +139,28 @@ def main() -> int:
     solve = pl.last_solve()
     # plain PCG ms/solve at 1e-8 on the step's RHS (device time of the PCG phase)
     pcg_ms = phase[3] / args.steps
-    # roofline of the dominant kernel: fused gather/push/BC/lookup/deposit, 2 launches per step
-    nsp = 2
-    push_ms = phase[0] / (args.steps * nsp)
-    bytes_per_particle = 44.0     # read x,y,vx,vy,vz,cell (24 B) + write x,y,vx,vy,cell_next (20 B), FP32
-    alg_bytes = bytes_per_particle * (0.5 * (n0 + n1)) / nsp
-    achieved = alg_bytes / (push_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles("traffic_push_cfg3.md")
+    ks = pl.kernel_stats()
+    nnodes = (w["domain"].nnx) * (w["domain"].nny)
+    # Rooflines of the two kernels that take the step's time, from HIP events in the timed
+    # region. Algorithmic bytes per launch:
+    #  push: 44 B per particle (read x, y, vx, vy, vz, cell; write x, y, vx, vy, cell_next, FP32)
+    #  CG iteration: 72 B per node (read r, w, s, p, x and 8 scaled off-diagonals; write r, w, s, p, x)
+    push_ms = ks["push_ms"] / max(ks["push_launches"], 1)
+    push_bytes = 44.0 * (0.5 * (n0 + n1)) / 2
+    cg_ms = ks["cg_ms"] / max(ks["cg_launches"], 1)
+    cg_bytes = 72.0 * nnodes
+    traffic = traffic_from_profiles("traffic_cfg3.md")
+    roof_push = {"bound": "hbm", "achieved": push_bytes / (push_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s", "traffic": traffic.get("k_push_deposit"), "kernel": "k_push_deposit<float,1>",
+                 "bytes_per_launch": push_bytes, "ms_per_launch": push_ms, "launches": ks["push_launches"],
+                 "share_of_step": ks["push_ms"] / phase[7]}
+    roof_cg = {"bound": "hbm", "achieved": cg_bytes / (cg_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "traffic": traffic.get("k_cg_iter"), "kernel": "k_cg_iter<1>",
+               "bytes_per_launch": cg_bytes, "ms_per_launch": cg_ms, "launches": ks["cg_launches"],
+               "share_of_step": ks["cg_ms"] / phase[7]}
+    for r in (roof_push, roof_cg):
+        r["frac"] = r["achieved"] / r["peak"]
+    dominant, other = (roof_cg, roof_push) if ks["cg_ms"] >= ks["push_ms"] else (roof_push, roof_cg)
     out = {
         "metric": METRIC, "value": value, "unit": "particle-pushes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps, "higher_is_better": True,
@@ -146,9 +173,8 @@ def main() -> int:
         "pcg_relres": solve["relres"],
         "phase_ms_per_step": {k: phase[i] / args.steps for i, k in enumerate(
             ["push_lookup_deposit", "deposit_finalize", "rhs", "pcg", "ppr", "mcc", "sort", "total"])},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_push_deposit<float,1>", "bytes_per_launch": alg_bytes, "ms_per_launch": push_ms},
+        "roofline": dominant,
+        "roofline_other": other,
     }
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle, 1 core)")

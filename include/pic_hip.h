@@ -257,7 +257,10 @@ int  pic_get_field(int which, double* dst, int64_t n);
 int  pic_set_field(int which, const double* src, int64_t n);
 
 /* Per-phase device time of the last pic_step call, ms: [0] push+locate+deposit,
- * [1] deposit finalise, [2] rhs, [3] PCG, [4] PPR, [5] MCC, [6] sort, [7] total. */
+ * [1] deposit finalise, [2] rhs, [3] PCG, [4] PPR, [5] MCC, [6] sort, [7] total; with
+ * n >= 12 also kernel statistics: [8] ms in the CG iteration kernel, [9] its launches,
+ * [10] ms in the fused push kernel, [11] its launches (HIP events on the library's
+ * stream). */
 int  pic_get_phase_times(double* ms, int n);
 
 /* --------------------------------------------------- stand-alone drop-ins (host I/O)

@@ -407,6 +407,13 @@ class PicLibrary:
         check(lib().pic_get_phase_times(_d(a), 8))
         return a
 
+    def kernel_stats(self) -> dict:
+        """HIP-event times of the CG iteration kernel and the fused push kernel over the
+        last step() call."""
+        a = np.zeros(12)
+        check(lib().pic_get_phase_times(_d(a), 12))
+        return {"cg_ms": a[8], "cg_launches": int(a[9]), "push_ms": a[10], "push_launches": int(a[11])}
+
     def Get_ParticleEField_PPR(self, x, y, EleIndex):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)

@@ -177,6 +177,7 @@ int pic_init(int device) {
     HIPCHK(hipMalloc(&g.d_errinfo, sizeof(PicErrInfo)));
     HIPCHK(hipMemset(g.d_errinfo, 0, sizeof(PicErrInfo)));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) for (int i = 0; i < 2; ++i) HIPCHK(hipEventCreate(&g.ev_push[s][i]));
     g.inited = true;
     return PIC_OK;
 }
@@ -708,6 +709,8 @@ int pic_initial_field(void) {
 int pic_step(int nsteps) {
     REQUIRE_DOMAIN();
     double acc[8] = {0};
+    for (int k = 0; k < 4; ++k) g.kstat[k] = 0.0;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.ev_push_pending[s] = false;
     for (int it = 0; it < nsteps; ++it) {
         HIPCHK(hipEventRecord(g.ev[0], g.stream));
         for (int s = 0; s < g.nspecies; ++s) {
@@ -733,6 +736,12 @@ int pic_step(int nsteps) {
         float ms;
         for (int k = 0; k < 7; ++k) { hipEventElapsedTime(&ms, g.ev[k], g.ev[k + 1]); acc[k] += ms; }
         hipEventElapsedTime(&ms, g.ev[0], g.ev[7]); acc[7] += ms;
+        for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
+            if (!g.ev_push_pending[s]) continue;
+            hipEventElapsedTime(&ms, g.ev_push[s][0], g.ev_push[s][1]);
+            g.kstat[2] += ms; g.kstat[3] += 1.0;
+            g.ev_push_pending[s] = false;
+        }
         TRY(check_err_flags());
     }
     for (int k = 0; k < 8; ++k) g.phase_ms[k] = acc[k];
@@ -741,6 +750,7 @@ int pic_step(int nsteps) {
 
 int pic_get_phase_times(double* ms, int n) {
     for (int k = 0; k < n && k < 8; ++k) ms[k] = g.phase_ms[k];
+    for (int k = 8; k < n && k < 12; ++k) ms[k] = g.kstat[k - 8];
     return PIC_OK;
 }
 

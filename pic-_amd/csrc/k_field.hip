@@ -365,10 +365,40 @@ template <int RPT>
 __global__ __launch_bounds__(CG_BLOCK) void k_cg_iter(int n, int k, int nb, Dia8 A, const float* r_o, const float* w_o,
                                                       const float* s_o, float* r_n, float* w_n, float* s_n, float* p,
                                                       float* x, double* scal, double* part, int pstride) {
-    if (scal[S_DONE] != 0.0) return;
+    __shared__ double sh[2][CG_BLOCK / 64];
+    __shared__ double sh_done;
     const int par = k & 1;
-    double gam, del;
-    cg_reduce2(part + (size_t)(2 * par + CG_PG) * pstride, part + (size_t)(2 * par + CG_PD) * pstride, nb, gam, del);
+    // Every load that does not depend on the scalars is issued first, so the latency of the
+    // partial sums (written by all blocks of the previous kernel) overlaps the vector reads.
+    const double* pa = part + (size_t)(2 * par + CG_PG) * pstride;
+    const double* pb = part + (size_t)(2 * par + CG_PD) * pstride;
+    double sa = 0.0, sb = 0.0;
+    for (int q = threadIdx.x; q < nb; q += CG_BLOCK) { sa += pa[q]; sb += pb[q]; }
+    if (threadIdx.x == 0) sh_done = scal[S_DONE];
+    const double gam_old = scal[par ? S_GAM0 : S_GAM1], alp_old = scal[par ? S_ALP0 : S_ALP1];
+    const double normrhs = scal[S_NORMRHS], tol2 = scal[S_TOL2], maxit = scal[S_MAXIT];
+    float ro[RPT], wo[RPT], so[RPT], po[RPT], xo[RPT], cc[RPT][8], rj[RPT][8], wj[RPT][8], sj[RPT][8];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        int i = blockIdx.x * (CG_BLOCK * RPT) + q * CG_BLOCK + threadIdx.x;
+        int ii = i < n ? i : n - 1;
+        ro[q] = r_o[ii]; wo[q] = w_o[ii]; so[q] = s_o[ii]; po[q] = p[ii]; xo[q] = x[ii];
+#pragma unroll
+        for (int kk = 0; kk < 8; ++kk) {
+            int j = ii + A.off[kk];
+            cc[q][kk] = A.c[kk][ii]; rj[q][kk] = r_o[j]; wj[q][kk] = w_o[j]; sj[q][kk] = s_o[j];
+        }
+    }
+    // block-uniform reduction of gamma = r.r and delta = w.r of the previous kernel
+    for (int off = 32; off > 0; off >>= 1) { sa += __shfl_xor(sa, off, 64); sb += __shfl_xor(sb, off, 64); }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][wid] = sa; sh[1][wid] = sb; }
+    __syncthreads();
+    double gam = 0.0, del = 0.0;
+#pragma unroll
+    for (int w = 0; w < CG_BLOCK / 64; ++w) { gam += sh[0][w]; del += sh[1][w]; }
+    if (sh_done != 0.0) return;
+    __syncthreads();                               // sh is reused by the partials below
     double alpha, beta = 0.0;
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (k == 0) {
@@ -376,13 +406,12 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_iter(int n, int k, int nb, Dia8
         if (gam == 0.0) { if (lead) { scal[S_DONE] = 1.0; scal[S_ITER] = 0.0; } return; }
         alpha = gam / del;
     } else {
-        bool conv = gam / scal[S_NORMRHS] <= scal[S_TOL2];
-        bool maxed = (double)k >= scal[S_MAXIT];
+        bool conv = gam / normrhs <= tol2;
+        bool maxed = (double)k >= maxit;
         if (conv || maxed) {
             if (lead) { scal[S_DONE] = 1.0; scal[S_ITER] = (double)k; scal[S_NORMR] = gam; scal[S_IERR] = maxed && !conv ? 1.0 : 0.0; }
             return;
         }
-        double gam_old = scal[par ? S_GAM0 : S_GAM1], alp_old = scal[par ? S_ALP0 : S_ALP1];
         beta = gam / gam_old;
         double den = del - beta * gam / alp_old;
         alpha = gam / den;
@@ -397,19 +426,19 @@ __global__ __launch_bounds__(CG_BLOCK) void k_cg_iter(int n, int k, int nb, Dia8
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         int i = blockIdx.x * (CG_BLOCK * RPT) + q * CG_BLOCK + threadIdx.x;
-        if (i < n) {
-            float si = (k == 0) ? w_o[i] : w_o[i] + fb * s_o[i];
-            float pi = (k == 0) ? r_o[i] : r_o[i] + fb * p[i];
-            float ri = r_o[i] - fa * si;
-            float wi = ri;
+        float si = (k == 0) ? wo[q] : wo[q] + fb * so[q];
+        float pi = (k == 0) ? ro[q] : ro[q] + fb * po[q];
+        float ri = ro[q] - fa * si;
+        float wi = ri;
 #pragma unroll
-            for (int kk = 0; kk < 8; ++kk) {
-                int j = i + A.off[kk];
-                float sj = (k == 0) ? w_o[j] : w_o[j] + fb * s_o[j];
-                float rj = r_o[j] - fa * sj;
-                wi = wi + A.c[kk][i] * rj;
-            }
-            x[i] = x[i] + fa * pi;
+        for (int kk = 0; kk < 8; ++kk) {
+            // the neighbour's own update, same expression and order as its row computes it
+            float sjj = (k == 0) ? wj[q][kk] : wj[q][kk] + fb * sj[q][kk];
+            float rjj = rj[q][kk] - fa * sjj;
+            wi = wi + cc[q][kk] * rjj;
+        }
+        if (i < n) {
+            x[i] = xo[q] + fa * pi;
             p[i] = pi; s_n[i] = si; r_n[i] = ri; w_n[i] = wi;
             g += (double)ri * (double)ri;
             d += (double)wi * (double)ri;
@@ -442,14 +471,19 @@ static int cg1_run_t(int n, const Dia8& A, const double* b64, double tol, int ma
     const int CHUNK = 32;
     int k = 0;
     for (;;) {
+        HIPCHK(hipEventRecord(g.ev[10], g.stream));
         for (int c = 0; c < CHUNK; ++c, ++k) {
             int o = k & 1, nw = o ^ 1;
             hipLaunchKernelGGL((k_cg_iter<RPT>), dim3(nb), dim3(CG_BLOCK), 0, g.stream, n, k, nb, A, r[o], w[o], s[o],
                                r[nw], w[nw], s[nw], g.d_cg_p, g.d_cg_x, scal, part, ps);
         }
         HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(g.ev[11], g.stream));
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, g.ev[10], g.ev[11]));
+        g.kstat[0] += ms; g.kstat[1] += CHUNK;
         if (g.h_scal_pinned[S_DONE] != 0.0) break;
         if (k > maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
     }

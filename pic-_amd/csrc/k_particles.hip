@@ -521,6 +521,7 @@ int launch_push_deposit(int isp, bool do_push) {
     if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
     if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
     int blocks = (int)((S.nslots + PUSH_SLOTS - 1) / PUSH_SLOTS);
+    if (do_push) HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
     if (g.particle_bits == 64) {
         PushArgs<double> a = make_push_args<double>(isp);
         if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
@@ -531,6 +532,7 @@ int launch_push_deposit(int isp, bool do_push) {
         else hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
     }
     HIPCHK(hipGetLastError());
+    if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream)); g.ev_push_pending[isp] = true; }
     S.acc_dirty = true;
     return PIC_OK;
 }

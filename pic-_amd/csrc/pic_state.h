@@ -175,6 +175,11 @@ struct PicState {
     float *d_cg_r[2] = {nullptr, nullptr}, *d_cg_w[2] = {nullptr, nullptr}, *d_cg_s[2] = {nullptr, nullptr};
     float *d_cg_p = nullptr, *d_cg_x = nullptr;
     int cg_rpt = 1;                    // rows per thread of the CG kernels
+    // kernel statistics of the last pic_step: [0] ms in CG iteration kernels (chunk events),
+    // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches
+    double kstat[4] = {0, 0, 0, 0};
+    hipEvent_t ev_push[PIC_MAX_SPECIES][2];
+    bool ev_push_pending[PIC_MAX_SPECIES] = {false};
     double* d_scal = nullptr;          // PCG scalars (device)
     double* d_partials = nullptr;      // block partial sums
     unsigned int* d_sync = nullptr;    // last-block counters
