@@ -105,7 +105,9 @@ int  pic_device_count(int* n);
 
 /* Precision of the particle state (32 or 64) and of the field solve
  * (0 = FP64 Jacobi-PCG, the reference semantics; 1 = FP32 inner PCG inside FP64
- * iterative refinement). Must precede pic_setup_domain / pic_set_species. */
+ * iterative refinement; 2 = as 1 with the learned flexible CG as the inner solver once
+ * pic_learn_setup has run, falling back to the plain inner CG on stagnation). Must precede
+ * pic_setup_domain / pic_set_species. */
 int  pic_set_precision(int particle_bits, int solver_mode);
 
 /* ------------------------------------------------------------- setup (host FP64)
@@ -293,6 +295,39 @@ int  pic_locate(int64_t n, const double* x, const double* y, int32_t* n_element)
 /* generate_Elementmap (generate_Elementmap.f90:1-60): Element_Map of size
  * (2^r (nx-1)) x (2^r (ny-1)), column-major, 1-based element ids. */
 int  pic_generate_Elementmap(int32_t* Element_Map, int repeat_refinement);
+
+/* ------------------------------------------- learned preconditioner (no reference counterpart)
+ * A linear convolutional encoder-decoder on the node grid (BF16 MFMA, FP32 accumulation),
+ * symmetric positive semi-definite by construction, used as the preconditioner of a
+ * flexible CG on the Jacobi-scaled system (solver_mode 2). It is trained on the GPU by the
+ * library's own forward and backward kernels on residual/error pairs (r = A' e) that the
+ * plain inner CG harvests: the solve's RHS with its solution correction, and the iterates
+ * every `every` chunks of 32 iterations. */
+/* Build the network for the current domain: levels down to ~3x3 nodes (at most max_levels),
+ * channels[l] in {16, 32, 64} (the last entry repeats), multilevel initialisation. */
+int  pic_learn_setup(int32_t max_levels, const int32_t* channels, int32_t nchannels, uint64_t seed);
+/* Harvest pairs from the plain inner solves into a device ring of `capacity` pairs. */
+int  pic_learn_harvest(int32_t enable, int32_t capacity, int32_t every);
+/* Adam on the energy-norm loss, minibatches drawn from the ring; loss of the first and
+ * the last minibatch (mean per pair). */
+int  pic_learn_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+/* Mean loss over pairs [0, npairs) of the ring, no update. */
+int  pic_learn_eval(int32_t npairs, double* mean_loss);
+/* Pairs in the ring, FCG iterations of the last learned solve, fallbacks to the plain CG so
+ * far, last training loss. */
+int  pic_learn_info(int32_t* dataset, int32_t* last_fcg_iterations, int64_t* fallbacks, double* last_loss);
+/* z = M r on host arrays of nnodes floats (node numbering). */
+int  pic_learn_apply(const float* r, float* z, int64_t n);
+int  pic_learn_get_params(float* dst, int64_t n, int64_t* nparams);
+int  pic_learn_set_params(const float* src, int64_t n);
+/* levels, per-level H, W, C (arrays of 10), parameter offsets (2*levels+1 entries:
+ * In, Down_1..Down_{L-1}, Q_0..Q_{L-1}, total). */
+int  pic_learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets);
+/* Replace the ring by npairs host pairs (r, e), each nnodes floats. */
+int  pic_learn_set_pairs(int32_t npairs, const float* r, const float* e);
+int  pic_learn_get_pair(int32_t k, float* r, float* e, double* ea);
+/* Summed loss and gradient (no update) over the ring pairs idx[0..npairs). */
+int  pic_learn_gradient(int32_t npairs, const int32_t* idx, double* loss, float* grad, int64_t n);
 
 #ifdef __cplusplus
 }

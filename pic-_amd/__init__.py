@@ -81,6 +81,7 @@ _P = C.POINTER
 _dp = _P(C.c_double)
 _ip = _P(C.c_int32)
 _i64p = _P(C.c_int64)
+_fp = _P(C.c_float)
 
 _PROTOS = {
     "pic_init": (C.c_int, [C.c_int]),
@@ -125,6 +126,18 @@ _PROTOS = {
     "pic_ParticlePositioning": (C.c_int, [C.c_int64, _dp, _dp, _ip, _ip]),
     "pic_locate": (C.c_int, [C.c_int64, _dp, _dp, _ip]),
     "pic_generate_Elementmap": (C.c_int, [_ip, C.c_int]),
+    "pic_learn_setup": (C.c_int, [C.c_int32, _ip, C.c_int32, C.c_uint64]),
+    "pic_learn_harvest": (C.c_int, [C.c_int32, C.c_int32, C.c_int32]),
+    "pic_learn_train": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, _dp, _dp]),
+    "pic_learn_eval": (C.c_int, [C.c_int32, _dp]),
+    "pic_learn_info": (C.c_int, [_ip, _ip, _i64p, _dp]),
+    "pic_learn_apply": (C.c_int, [_fp, _fp, C.c_int64]),
+    "pic_learn_get_params": (C.c_int, [_fp, C.c_int64, _i64p]),
+    "pic_learn_set_params": (C.c_int, [_fp, C.c_int64]),
+    "pic_learn_layout": (C.c_int, [_ip, _ip, _ip, _ip, _i64p]),
+    "pic_learn_set_pairs": (C.c_int, [C.c_int32, _fp, _fp]),
+    "pic_learn_get_pair": (C.c_int, [C.c_int32, _fp, _fp, _dp]),
+    "pic_learn_gradient": (C.c_int, [C.c_int32, _ip, _dp, _fp, C.c_int64]),
 }
 
 
@@ -168,6 +181,10 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_ip)
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
 
 
 # ----------------------------------------------------------------- configuration
@@ -274,6 +291,10 @@ class PicLibrary:
         check(lib().pic_set_pcg_params(tol, maxit))
         self.dom = dom
         self.bits = bits
+
+    def set_solver(self, solver: int) -> None:
+        """0 FP64 Jacobi-PCG, 1 mixed precision, 2 mixed precision with the learned FCG."""
+        check(lib().pic_set_precision(self.bits, solver))
 
     def sizes(self):
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
@@ -449,6 +470,79 @@ class PicLibrary:
         em = np.zeros((d.nnx - 1) * (d.nny - 1), dtype=np.int32)
         check(lib().pic_generate_Elementmap(_i(em), repeat_refinement))
         return em.reshape((d.nnx - 1, d.nny - 1), order="F")
+
+    # ------------------------------------------------------- learned preconditioner
+    def learn_setup(self, max_levels: int = 10, channels=(16, 16, 32), seed: int = 1) -> None:
+        ch = np.ascontiguousarray(channels, dtype=np.int32)
+        check(lib().pic_learn_setup(max_levels, _i(ch), len(ch), seed))
+
+    def learn_harvest(self, enable: bool = True, capacity: int = 256, every: int = 1) -> None:
+        check(lib().pic_learn_harvest(int(enable), capacity, every))
+
+    def learn_train(self, steps: int, batch: int = 4, lr: float = 1e-3, seed: int = 1) -> tuple:
+        a, b = C.c_double(), C.c_double()
+        check(lib().pic_learn_train(steps, batch, lr, seed, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def learn_eval(self, npairs: int) -> float:
+        v = C.c_double()
+        check(lib().pic_learn_eval(npairs, C.byref(v)))
+        return v.value
+
+    def learn_info(self) -> dict:
+        ds, it, fb, loss = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
+        check(lib().pic_learn_info(C.byref(ds), C.byref(it), C.byref(fb), C.byref(loss)))
+        return {"dataset": ds.value, "last_fcg_iterations": it.value, "fallbacks": fb.value, "last_loss": loss.value}
+
+    def learn_apply(self, r) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        z = np.zeros_like(r)
+        check(lib().pic_learn_apply(_f(r), _f(z), len(r)))
+        return z
+
+    def learn_params(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().pic_learn_get_params(None, 0, C.byref(n)))
+        p = np.zeros(n.value, dtype=np.float32)
+        check(lib().pic_learn_get_params(_f(p), len(p), C.byref(n)))
+        return p
+
+    def learn_set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().pic_learn_set_params(_f(p), len(p)))
+
+    def learn_layout(self) -> dict:
+        L = C.c_int32()
+        H = np.zeros(10, np.int32); W = np.zeros(10, np.int32); Cc = np.zeros(10, np.int32)
+        off = np.zeros(21, np.int64)
+        check(lib().pic_learn_layout(C.byref(L), _i(H), _i(W), _i(Cc), off.ctypes.data_as(_i64p)))
+        n = L.value
+        return {"levels": n, "H": H[:n].tolist(), "W": W[:n].tolist(), "C": Cc[:n].tolist(),
+                "off_in": int(off[0]), "off_down": [None] + [int(o) for o in off[1:n]],
+                "off_q": [int(o) for o in off[n:2 * n]], "nparams": int(off[2 * n])}
+
+    def learn_set_pairs(self, r, e) -> None:
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        check(lib().pic_learn_set_pairs(r.shape[0], _f(r), _f(e)))
+
+    def learn_get_pair(self, k: int, n: int) -> tuple:
+        r = np.zeros(n, np.float32); e = np.zeros(n, np.float32); ea = C.c_double()
+        check(lib().pic_learn_get_pair(k, _f(r), _f(e), C.byref(ea)))
+        return r, e, ea.value
+
+    def learn_gradient(self, idx, nparams: int) -> tuple:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.zeros(nparams, np.float32)
+        loss = C.c_double()
+        check(lib().pic_learn_gradient(len(idx), _i(idx), C.byref(loss), _f(g), nparams))
+        return loss.value, g
+
+    def kernel_stats_full(self) -> dict:
+        a = np.zeros(14)
+        check(lib().pic_get_phase_times(_d(a), 14))
+        return {"cg_ms": a[8], "cg_launches": int(a[9]), "push_ms": a[10], "push_launches": int(a[11]),
+                "fcg_ms": a[12], "fcg_iterations": int(a[13])}
 
     @staticmethod
     def MY_JPCG_Solver(matrix, R, X, M, NZ: int, N: int) -> int:

@@ -185,6 +185,7 @@ int pic_init(int device) {
 int pic_finalize(void) {
     if (!g.inited) return PIC_OK;
     hipDeviceSynchronize();
+    learn_free();
     hipDeviceReset();
     g = PicState();
     return PIC_OK;
@@ -192,7 +193,7 @@ int pic_finalize(void) {
 
 int pic_set_precision(int particle_bits, int solver_mode) {
     if (particle_bits != 32 && particle_bits != 64) return pic_fail(PIC_ERR_ARG, "particle_bits must be 32 or 64");
-    if (solver_mode != 0 && solver_mode != 1) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0 or 1");
+    if (solver_mode < 0 || solver_mode > 2) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0, 1 or 2");
     if (particle_bits != g.particle_bits) {
         // particle storage changes width: existing species are discarded
         if (g.inited) HIPCHK(hipStreamSynchronize(g.stream));
@@ -209,8 +210,9 @@ int pic_set_precision(int particle_bits, int solver_mode) {
 int pic_setup_domain(const pic_domain_config* cfg) {
     REQUIRE_INIT();
     if (!cfg) return pic_fail(PIC_ERR_ARG, "cfg is NULL");
-    // species and MCC tables belong to a domain (accumulators are sized by its nodes)
+    // species, MCC tables and the learned network belong to a domain (sized by its nodes)
     HIPCHK(hipStreamSynchronize(g.stream));
+    learn_free();
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
     g.nspecies = 0;
     g.mcc_ready = false;
@@ -709,7 +711,7 @@ int pic_initial_field(void) {
 int pic_step(int nsteps) {
     REQUIRE_DOMAIN();
     double acc[8] = {0};
-    for (int k = 0; k < 4; ++k) g.kstat[k] = 0.0;
+    for (int k = 0; k < 6; ++k) g.kstat[k] = 0.0;
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.ev_push_pending[s] = false;
     for (int it = 0; it < nsteps; ++it) {
         HIPCHK(hipEventRecord(g.ev[0], g.stream));
@@ -750,7 +752,7 @@ int pic_step(int nsteps) {
 
 int pic_get_phase_times(double* ms, int n) {
     for (int k = 0; k < n && k < 8; ++k) ms[k] = g.phase_ms[k];
-    for (int k = 8; k < n && k < 12; ++k) ms[k] = g.kstat[k - 8];
+    for (int k = 8; k < n && k < 14; ++k) ms[k] = g.kstat[k - 8];
     return PIC_OK;
 }
 
@@ -923,6 +925,61 @@ int pic_generate_Elementmap(int32_t* Element_Map, int repeat_refinement) {
             Element_Map[(size_t)(i - 1) + (size_t)(j - 1) * m.nex] = ne;
         }
     return PIC_OK;
+}
+
+// ----------------------------------------------------------------------- learned preconditioner
+int pic_learn_setup(int32_t max_levels, const int32_t* channels, int32_t nchannels, uint64_t seed) {
+    REQUIRE_DOMAIN();
+    return learn_setup(max_levels, channels, nchannels, seed);
+}
+int pic_learn_harvest(int32_t enable, int32_t capacity, int32_t every) {
+    REQUIRE_DOMAIN();
+    return learn_set_harvest(enable, capacity, every);
+}
+int pic_learn_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+    REQUIRE_DOMAIN();
+    return learn_train(steps, batch, lr, seed, loss_first, loss_last);
+}
+int pic_learn_eval(int32_t npairs, double* mean_loss) {
+    REQUIRE_DOMAIN();
+    if (!mean_loss) return pic_fail(PIC_ERR_ARG, "mean_loss is NULL");
+    return learn_eval(npairs, mean_loss);
+}
+int pic_learn_info(int32_t* dataset, int32_t* last_fcg_iterations, int64_t* fallbacks, double* last_loss) {
+    return learn_info(dataset, last_fcg_iterations, fallbacks, last_loss);
+}
+int pic_learn_apply(const float* r, float* z, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!r || !z) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return learn_apply_host(r, z, n);
+}
+int pic_learn_get_params(float* dst, int64_t n, int64_t* nparams) {
+    REQUIRE_DOMAIN();
+    return learn_params(dst, nullptr, n, nparams);
+}
+int pic_learn_set_params(const float* src, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!src) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return learn_params(nullptr, src, n, nullptr);
+}
+int pic_learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets) {
+    REQUIRE_DOMAIN();
+    if (!levels || !H || !W || !C || !offsets) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return learn_layout(levels, H, W, C, offsets);
+}
+int pic_learn_set_pairs(int32_t npairs, const float* r, const float* e) {
+    REQUIRE_DOMAIN();
+    if (!r || !e) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return learn_set_pairs(npairs, r, e);
+}
+int pic_learn_get_pair(int32_t k, float* r, float* e, double* ea) {
+    REQUIRE_DOMAIN();
+    return learn_get_pair(k, r, e, ea);
+}
+int pic_learn_gradient(int32_t npairs, const int32_t* idx, double* loss, float* grad, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!idx || !loss || !grad) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return learn_gradient(npairs, idx, loss, grad, n);
 }
 
 }  // extern "C"

@@ -304,8 +304,6 @@ __global__ void k_xn_to_phi(int n, const int* nt2, const double* x, double* phi)
 #define CG_BLOCK 256
 enum { CG_PG = 0, CG_PD = 1 };   // partial regions (x2 parities)
 
-struct Dia8 { const float* c[8]; int off[8]; };
-
 // Sum of two partial regions over nb blocks, fixed order, identical in every block.
 __device__ inline void cg_reduce2(const double* pa, const double* pb, int nb, double& a, double& b) {
     __shared__ double sh[2][CG_BLOCK / 64];
@@ -468,8 +466,10 @@ static int cg1_run_t(int n, const Dia8& A, const double* b64, double tol, int ma
     hipLaunchKernelGGL((k_cg_init<RPT>), dim3(nb), dim3(CG_BLOCK), 0, g.stream, n, A, b64, g.d_sinv, r[0], w[0],
                        g.d_cg_x, scal, part, ps);
     HIPCHK(hipGetLastError());
+    const bool harvest = learn_harvesting();
+    if (harvest) { int rc = learn_snapshot(g.d_cg_x, 0); if (rc) return rc; }
     const int CHUNK = 32;
-    int k = 0;
+    int k = 0, chunk = 0;
     for (;;) {
         HIPCHK(hipEventRecord(g.ev[10], g.stream));
         for (int c = 0; c < CHUNK; ++c, ++k) {
@@ -486,7 +486,10 @@ static int cg1_run_t(int n, const Dia8& A, const double* b64, double tol, int ma
         g.kstat[0] += ms; g.kstat[1] += CHUNK;
         if (g.h_scal_pinned[S_DONE] != 0.0) break;
         if (k > maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
+        ++chunk;
+        if (harvest) { int rc = learn_snapshot(g.d_cg_x, chunk); if (rc) return rc; }
     }
+    if (harvest) { int rc = learn_harvest_finish(g.d_cg_x); if (rc) return rc; }
     *iters = (int)g.h_scal_pinned[S_ITER];
     *ierr = (int)g.h_scal_pinned[S_IERR];
     return PIC_OK;
@@ -593,6 +596,10 @@ int run_pcg_mixed() {
     int total = 0, outer = 0;
     double rr = 0, bb = 0;
     const double inner_tol = 1e-4;
+    // solver_mode 2: the learned FCG is the inner solver until it stagnates; the plain CG
+    // then finishes this solve. The FP64 outer residual judges convergence either way.
+    bool use_learned = g.solver_mode == 2 && learn_ready();
+    g.last_learned_iters = 0; g.last_fallback = 0;
     for (;;) {
         int rc = resid64(n, A64, g.d_bn, g.d_xn, g.d_rn, &rr, &bb);
         if (rc) return rc;
@@ -603,8 +610,18 @@ int run_pcg_mixed() {
             return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
         }
         int it = 0, ierr = 0;
-        rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
-        if (rc) return rc;
+        if (use_learned) {
+            int stagnated = 0;
+            rc = learn_fcg_run(n, g.d_rn, inner_tol, std::min(g.learn_maxit, g.pcg_maxit - total), &it, &ierr, &stagnated);
+            if (rc) return rc;
+            g.last_learned_iters += it;
+            if (stagnated) { use_learned = false; g.last_fallback = 1; learn_count_fallback(); }
+            if (ierr == 2 || ierr == 3) ierr = 0;           // handed over to the plain CG
+            if (ierr == 1) { use_learned = false; ierr = 0; }
+        } else {
+            rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_cg_apply, dim3(nb256), dim3(256), 0, g.stream, n, g.d_sinv, g.d_cg_x, g.d_xn);
         total += it; outer++;
         if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "inner PCG breakdown");

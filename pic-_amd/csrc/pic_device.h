@@ -97,3 +97,7 @@ enum {
     PIC_C_EMIT = 0, PIC_C_RIGHT = 1, PIC_C_OBJECT = 2, PIC_C_REFLECT = 3,
     PIC_C_MCC_SELECTED = 4, PIC_C_MCC_REAL = 5, PIC_C_MCC_CREATED = 6, PIC_C_DELETED = 7
 };
+
+// Jacobi-scaled node-space operator: eight scaled off-diagonals (the unit diagonal is
+// implicit), offsets -nny-1 ... nny+1 without 0.
+struct Dia8 { const float* c[8]; int off[8]; };

@@ -175,9 +175,12 @@ struct PicState {
     float *d_cg_r[2] = {nullptr, nullptr}, *d_cg_w[2] = {nullptr, nullptr}, *d_cg_s[2] = {nullptr, nullptr};
     float *d_cg_p = nullptr, *d_cg_x = nullptr;
     int cg_rpt = 1;                    // rows per thread of the CG kernels
+    int learn_maxit = 400;             // FCG iterations per inner solve before handing over
+    int last_learned_iters = 0, last_fallback = 0;
     // kernel statistics of the last pic_step: [0] ms in CG iteration kernels (chunk events),
-    // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches
-    double kstat[4] = {0, 0, 0, 0};
+    // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches,
+    // [4] ms in learned-FCG iterations (chunk events), [5] FCG iterations launched
+    double kstat[6] = {0, 0, 0, 0, 0, 0};
     hipEvent_t ev_push[PIC_MAX_SPECIES][2];
     bool ev_push_pending[PIC_MAX_SPECIES] = {false};
     double* d_scal = nullptr;          // PCG scalars (device)
@@ -243,6 +246,25 @@ int launch_zero_acc(int isp);
 int count_live_slots(int64_t n, const int* cell, int64_t* out);
 int launch_locate_points(int64_t n, const double* dx, const double* dy, int* dout, int mode);
 int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, const int* ele, double* ex, double* ey);
+// learned preconditioner (k_learn.hip)
+int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed);
+int learn_set_harvest(int enable, int capacity, int every);
+bool learn_harvesting();
+int learn_snapshot(const float* x, int chunk);
+int learn_harvest_finish(const float* xfinal);
+int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+int learn_eval(int npairs, double* mean_loss);
+int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, int* ierr, int* stagnated);
+bool learn_ready();
+int learn_apply_host(const float* r, float* z, int64_t n);
+int learn_params(float* dst, const float* src, int64_t n, int64_t* nparams);
+int learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets);
+int learn_set_pairs(int npairs, const float* r, const float* e);
+int learn_get_pair(int k, float* r, float* e, double* ea);
+int learn_gradient(int np, const int32_t* idx, double* loss, float* grad, int64_t n);
+int learn_info(int32_t* dataset, int32_t* last_iters, int64_t* fallbacks, double* last_loss);
+void learn_count_fallback();
+void learn_free();
 int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,
                      const double* b, double* x, double tol, int maxit, int* iters, int* ierr,
                      double* r_out);
