@@ -93,6 +93,68 @@ def cpu_baseline(pkg, pl, w, sample_particles: int) -> dict:
                       f"{t_step:.2f} s per step"}
 
 
+def solve_compare(pl, reps: int) -> dict:
+    """Plain (mode 1) vs learned (mode 2) inner solver on the same RHS and warm start: the
+    charge density of the current state and the potential of the previous step."""
+    import time as _t
+    dom = pl.dom
+    rho = pl.field("RHO")
+    phi0 = pl.field("PHI")
+    res = {}
+    for name, mode in (("plain", 1), ("learned", 2)):
+        pl.set_solver(mode)
+        times, its = [], []
+        for _ in range(reps):
+            phi = phi0.copy()
+            t0 = _t.perf_counter()
+            pl.IFE_Solve_2D(Rho=rho, Phi=phi)
+            times.append(_t.perf_counter() - t0)
+            info = pl.last_solve()
+            its.append(info["iterations"])
+        res[name] = {"ms_per_solve": 1e3 * float(np.median(times)), "iterations": int(np.median(its)),
+                     "relres": info["relres"], "outer": info["outer"]}
+    pl.set_solver(1)
+    return res
+
+
+def learned_leg(pkg, pl, w, args) -> dict:
+    """Harvest pairs from the plain solver during untimed steps, train the network on the GPU,
+    then compare the plain and learned solves and time full steps with the learned solver."""
+    import time as _t
+    log("bench: learned preconditioner: harvest")
+    pl.learn_setup(12, (16, 16), seed=7)
+    pl.learn_harvest(True, 512, 1)
+    pl.step(args.harvest_steps)
+    pl.learn_harvest(False, 512, 1)
+    info = pl.learn_info()
+    log(f"bench: {info['dataset']} pairs; training {args.train_steps} steps")
+    t0 = _t.perf_counter()
+    l0, l1 = pl.learn_train(args.train_steps, batch=4, lr=2e-3, seed=11)
+    t_train = _t.perf_counter() - t0
+    log("bench: solve comparison")
+    cmp_ = solve_compare(pl, args.solve_reps)
+    log(f"bench: {cmp_}")
+    # whole steps with the learned solver (same step contract as the plain run)
+    pl.set_solver(2)
+    pl.step(2)
+    n0 = pl.counts(0)["nlive"] + pl.counts(1)["nlive"]
+    t0 = _t.perf_counter()
+    pl.step(args.steps)
+    el = _t.perf_counter() - t0
+    n1 = pl.counts(0)["nlive"] + pl.counts(1)["nlive"]
+    ph = pl.phase_times()
+    ks = pl.kernel_stats_full()
+    fb = pl.learn_info()["fallbacks"]
+    pl.set_solver(1)
+    return {"pairs": info["dataset"], "train_steps": args.train_steps, "train_s": t_train,
+            "loss_first_batch": l0, "loss_last_batch": l1, "solve": cmp_,
+            "iteration_ratio": cmp_["plain"]["iterations"] / max(cmp_["learned"]["iterations"], 1),
+            "speedup_per_solve": cmp_["plain"]["ms_per_solve"] / cmp_["learned"]["ms_per_solve"],
+            "step_value": 0.5 * (n0 + n1) * args.steps / el, "step_ms": 1e3 * el / args.steps,
+            "pcg_ms_per_solve": ph[3] / args.steps, "fcg_ms": ks["fcg_ms"], "fcg_iterations": ks["fcg_iterations"],
+            "fallbacks": fb}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +163,10 @@ def main() -> int:
     ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--harvest-steps", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--solve-reps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -176,6 +242,8 @@ def main() -> int:
         "roofline": dominant,
         "roofline_other": other,
     }
+    if not args.no_learned:
+        out["learned"] = learned_leg(pkg, pl, w, args)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle, 1 core)")
         try:

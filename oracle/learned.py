@@ -1,23 +1,35 @@
-"""CPU restatement of the learned preconditioner (numpy, FP64 with optional BF16 emulation).
+"""CPU restatement of the learned multigrid preconditioner (numpy/scipy, FP64 with optional
+BF16 emulation).
 
-TEST INFRASTRUCTURE ONLY: loaded by tests/ as the checker of the GPU network kernels
-(pic-_amd/csrc/k_learn.hip), never by the library. The network has no counterpart in the
-reference; this file restates the build's own definition:
+TEST INFRASTRUCTURE ONLY: loaded by tests/ as the checker of the GPU kernels in
+pic-_amd/csrc/k_learn.hip, never by the library. The network has no counterpart in the
+reference; this file restates the build's own definition.
 
-  E_0 = In r (3x3), E_l = Down_l E_{l-1} (3x3, stride 2: coarse node I <-> fine node 2I),
-  D_{L-1} = S_{L-1} E_{L-1}, D_l = S_l E_l + Down_{l+1}^T D_{l+1}, y = m * In^T D_0,
-  S_l = Q_l Q_l^T, loss = sum_pairs (y - e)^T A' (y - e) / (e^T A' e).
+Hierarchy: level 0 is the Jacobi-scaled node-space operator A' (unit diagonal on unknown
+nodes). Level l+1 has the nodes (I, J) <-> fine (2I, 2J); P_l is bilinear interpolation
+with the rows of fine Dirichlet nodes (zero diagonal) removed, R_l = P_l^T, and
+A_{l+1} = P_l^T A_l P_l (Galerkin). The coarsest level is solved exactly.
 
-Weights are stored per tap t = 3*(di+1) + (dj+1) as W[t][co][ci]; activations as
-(H, W, C) arrays (node (i, j) -> [i, j]). The gradient is computed by the adjoint
-recursions (G = encoder of g, H = decoder of G) and checked against central differences
-of the loss in tests/test_learned_cpu.py.
+Symmetric V(1,1)-cycle y = V_l(r):
+    x  = K_l r
+    rc = R_l (r - A_l x);  ec = V_{l+1}(rc)
+    x2 = x + P_l ec
+    y  = x2 + K_l (r - A_l x2)
+Smoothers: learned on levels l < Ls,  K_l = Dh In_l^T S_l In_l Dh  (Dh = diag(A_l)^-1/2,
+In_l a 3x3 conv 1 -> C_l channels, S_l = Q_l Q_l^T), damped Jacobi omega/diag elsewhere.
+Loss over pairs (r, e = A'^-1 r):  sum (y - e)^T A' (y - e) / (e^T A' e).
+For symmetric K and M, with g = dL/dy, the gradient of a learned smoother is
+    dL/dK = s2(g) (x) r_l(r) + r_l(g) (x) s2(r)
+where r_l(.) and s2(.) are the inputs of its pre- and post-smoothing in the V-cycle run on
+r or on g (see tests/test_learned_cpu.py for the central-difference check).
 """
 from __future__ import annotations
 
 import numpy as np
+import scipy.sparse as sp
 
 TAPS = [(t // 3 - 1, t % 3 - 1) for t in range(9)]
+OMEGA = 0.8
 
 
 def bf16(a):
@@ -27,7 +39,7 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def level_sizes(H0: int, W0: int, max_levels: int = 10):
+def level_sizes(H0: int, W0: int, max_levels: int = 12):
     H, W = [H0], [W0]
     while len(H) < max_levels and min(H[-1], W[-1]) > 3:
         H.append((H[-1] + 1) // 2)
@@ -35,34 +47,67 @@ def level_sizes(H0: int, W0: int, max_levels: int = 10):
     return H, W
 
 
-def unpack(params, lay):
-    """Views of the flat parameter vector in the library's layout."""
-    L, C = lay["levels"], lay["C"]
-    p = np.asarray(params, dtype=np.float64)
-    w_in = p[lay["off_in"]: lay["off_in"] + 9 * C[0] * 16].reshape(9, C[0], 16)
-    w_dn = [None] + [p[lay["off_down"][l]: lay["off_down"][l] + 9 * C[l] * C[l - 1]].reshape(9, C[l], C[l - 1])
-                     for l in range(1, L)]
-    q = [p[lay["off_q"][l]: lay["off_q"][l] + C[l] * C[l]].reshape(C[l], C[l]) for l in range(L)]
-    return w_in, w_dn, q
-
-
-def layout(H0: int, W0: int, channels, max_levels: int = 10) -> dict:
+def layout(H0: int, W0: int, channels, learned_levels: int = 2, max_levels: int = 12) -> dict:
     H, W = level_sizes(H0, W0, max_levels)
     L = len(H)
-    C = [channels[min(l, len(channels) - 1)] for l in range(L)]
-    off = 0
-    off_in = off
-    off += 9 * C[0] * 16
-    off_down = [None]
-    for l in range(1, L):
-        off_down.append(off)
-        off += 9 * C[l] * C[l - 1]
-    off_q = []
-    for l in range(L):
+    Ls = min(learned_levels, L - 1)
+    C = [channels[min(l, len(channels) - 1)] for l in range(Ls)]
+    off, off_in, off_q = 0, [], []
+    for l in range(Ls):
+        off_in.append(off)
+        off += 9 * C[l] * 16
+    for l in range(Ls):
         off_q.append(off)
         off += C[l] * C[l]
-    return {"levels": L, "H": H, "W": W, "C": C, "off_in": off_in, "off_down": off_down, "off_q": off_q,
-            "nparams": off}
+    return {"levels": L, "learned": Ls, "H": H, "W": W, "C": C, "off_in": off_in, "off_q": off_q, "nparams": off}
+
+
+def unpack(params, lay):
+    p = np.asarray(params, dtype=np.float64)
+    w_in = [p[o:o + 9 * c * 16].reshape(9, c, 16) for o, c in zip(lay["off_in"], lay["C"])]
+    q = [p[o:o + c * c].reshape(c, c) for o, c in zip(lay["off_q"], lay["C"])]
+    return w_in, q
+
+
+# ------------------------------------------------------------------ hierarchy
+def prolongation(Hf, Wf, Hc, Wc, maskf):
+    rows, cols, vals = [], [], []
+    for i in range(Hf):
+        ci = [(i // 2, 1.0)] if i % 2 == 0 else [((i - 1) // 2, 0.5), ((i + 1) // 2, 0.5)]
+        for j in range(Wf):
+            f = i * Wf + j
+            if maskf[f] == 0:
+                continue
+            cj = [(j // 2, 1.0)] if j % 2 == 0 else [((j - 1) // 2, 0.5), ((j + 1) // 2, 0.5)]
+            for I, wi in ci:
+                for J, wj in cj:
+                    if I < Hc and J < Wc:
+                        rows.append(f); cols.append(I * Wc + J); vals.append(wi * wj)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(Hf * Wf, Hc * Wc))
+
+
+def build_hierarchy(A0, H0, W0, max_levels: int = 12):
+    H, W = level_sizes(H0, W0, max_levels)
+    levels = []
+    A = sp.csr_matrix(A0)
+    for l in range(len(H)):
+        d = A.diagonal()
+        mask = (d != 0).astype(np.float64)
+        lev = {"H": H[l], "W": W[l], "A": A, "d": d, "mask": mask,
+               "dinv": np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0),
+               "dh": np.where(d > 0, 1.0 / np.sqrt(np.where(d > 0, d, 1.0)), 0.0)}
+        if l + 1 < len(H):
+            P = prolongation(H[l], W[l], H[l + 1], W[l + 1], mask)
+            lev["P"] = P
+            A = (P.T @ A @ P).tocsr()
+        levels.append(lev)
+    last = levels[-1]
+    idx = np.nonzero(last["mask"])[0]
+    inv = np.zeros((last["H"] * last["W"],) * 2)
+    if len(idx):
+        inv[np.ix_(idx, idx)] = np.linalg.inv(last["A"].toarray()[np.ix_(idx, idx)])
+    last["inv"] = inv
+    return levels
 
 
 # ------------------------------------------------------------------ convolutions
@@ -78,26 +123,6 @@ def conv_same(x, w, sign=1):
     return out
 
 
-def conv_down(x, w, Ho, Wo):
-    """out[I, J] = sum_t w[t] @ x[2I + di, 2J + dj] (zero outside)."""
-    H, W, _ = x.shape
-    xp = np.zeros((2 * Ho + 2, 2 * Wo + 2, x.shape[2]))
-    xp[1:H + 1, 1:W + 1] = x
-    out = np.zeros((Ho, Wo, w.shape[1]))
-    for t, (di, dj) in enumerate(TAPS):
-        out += xp[1 + di:1 + di + 2 * Ho:2, 1 + dj:1 + dj + 2 * Wo:2] @ w[t].T
-    return out
-
-
-def conv_up(y, w, H, W):
-    """Adjoint of conv_down with the same w: out[2I + di, 2J + dj] += w[t]^T @ y[I, J]."""
-    Ho, Wo, _ = y.shape
-    outp = np.zeros((2 * Ho + 2, 2 * Wo + 2, w.shape[2]))
-    for t, (di, dj) in enumerate(TAPS):
-        outp[1 + di:1 + di + 2 * Ho:2, 1 + dj:1 + dj + 2 * Wo:2] += y @ w[t]
-    return outp[1:H + 1, 1:W + 1]
-
-
 def corr_same(o, x):
     """dW[t][co][ci] = sum_p o[p, co] x[p + d_t, ci]."""
     H, W, _ = o.shape
@@ -106,95 +131,123 @@ def corr_same(o, x):
     return np.stack([np.einsum("pqo,pqi->oi", o, xp[1 + di:1 + di + H, 1 + dj:1 + dj + W]) for di, dj in TAPS])
 
 
-def corr_down(o, x):
-    """dW[t][co][ci] = sum_P o[P, co] x[2P + d_t, ci]."""
-    Ho, Wo, _ = o.shape
-    H, W, _ = x.shape
-    xp = np.zeros((2 * Ho + 2, 2 * Wo + 2, x.shape[2]))
-    xp[1:H + 1, 1:W + 1] = x
-    return np.stack([np.einsum("pqo,pqi->oi", o, xp[1 + di:1 + di + 2 * Ho:2, 1 + dj:1 + dj + 2 * Wo:2])
-                     for di, dj in TAPS])
+def _as_chan(v, H, W):
+    x = np.zeros((H, W, 16))
+    x[:, :, 0] = v.reshape(H, W)
+    return x
 
 
 # ------------------------------------------------------------------ network
-def _encode(x0, w_in, w_dn, lay, rnd):
-    H, W, C = lay["H"], lay["W"], lay["C"]
-    xin = np.zeros((H[0], W[0], 16))
-    xin[:, :, 0] = rnd(x0.reshape(H[0], W[0]))
-    E = [rnd(conv_same(xin, rnd(w_in)))]
-    for l in range(1, lay["levels"]):
-        E.append(rnd(conv_down(E[l - 1], rnd(w_dn[l]), H[l], W[l])))
-    return E
+class _Net:
+    def __init__(self, params, lay, levels, emulate_bf16=False):
+        self.lay, self.levels = lay, levels
+        self.rnd = bf16 if emulate_bf16 else (lambda a: a)
+        w_in, q = unpack(params, lay)
+        self.w_in, self.q = w_in, q
+        self.S = [qq @ qq.T for qq in q]
+        rnd = self.rnd
+        self.SW = [rnd(np.einsum("oc,tci->toi", S, w)) for S, w in zip(self.S, w_in)]   # S In, [9][C][16]
+        self.WT = [rnd(np.transpose(w, (0, 2, 1))) for w in w_in]                          # In^T, [9][16][C]
+
+    def smooth(self, l, v):
+        lev = self.levels[l]
+        if l < self.lay["learned"]:
+            H, W = lev["H"], lev["W"]
+            u = _as_chan(self.rnd(lev["dh"] * v), H, W)
+            E = self.rnd(conv_same(u, self.SW[l]))
+            return conv_same(E, self.WT[l], sign=-1)[:, :, 0].reshape(-1) * lev["dh"]
+        return OMEGA * lev["dinv"] * v
+
+    def vcycle(self, l, r, sites):
+        lev = self.levels[l]
+        if l == len(self.levels) - 1:
+            return lev["inv"] @ r
+        A, P = lev["A"], lev["P"]
+        x = self.smooth(l, r)
+        ec = self.vcycle(l + 1, P.T @ (r - A @ x), sites)
+        x2 = x + P @ ec
+        s2 = r - A @ x2
+        if l < self.lay["learned"]:
+            sites[l] = (r.copy(), s2.copy())
+        return x2 + self.smooth(l, s2)
 
 
-def _decode(E, w_dn, S, lay, rnd):
-    L = lay["levels"]
-    H, W = lay["H"], lay["W"]
-    D = [None] * L
-    D[L - 1] = rnd(E[L - 1] @ rnd(S[L - 1]).T)
-    for l in range(L - 2, -1, -1):
-        D[l] = rnd(conv_up(D[l + 1], rnd(w_dn[l + 1]), H[l], W[l]) + E[l] @ rnd(S[l]).T)
-    return D
+def forward(params, lay, levels, r, emulate_bf16: bool = False):
+    """y = M r on a flat node vector; returns y and the smoother inputs per learned level."""
+    net = _Net(params, lay, levels, emulate_bf16)
+    sites = {}
+    y = net.vcycle(0, np.asarray(r, np.float64), sites)
+    return y, sites
 
 
-def forward(params, lay, r, mask, emulate_bf16: bool = False):
-    """y = M r on a flat node vector r (length H0*W0); returns y and the tensors E, D."""
-    rnd = bf16 if emulate_bf16 else (lambda a: a)
-    w_in, w_dn, q = unpack(params, lay)
-    S = [qq @ qq.T for qq in q]
-    E = _encode(np.asarray(r, np.float64), w_in, w_dn, lay, rnd)
-    D = _decode(E, w_dn, S, lay, rnd)
-    y = conv_same(D[0], rnd(np.transpose(w_in, (0, 2, 1))), sign=-1)[:, :, 0].reshape(-1)
-    return y * (np.asarray(mask) > 0), E, D
-
-
-def loss_and_grad(params, lay, pairs, A, mask, emulate_bf16: bool = False):
-    """Summed loss over pairs [(r, e)] and its gradient (flat, library layout).
-    A: the scaled operator (scipy sparse or dense) on node vectors."""
-    rnd = bf16 if emulate_bf16 else (lambda a: a)
-    L, H, W, C = lay["levels"], lay["H"], lay["W"], lay["C"]
-    w_in, w_dn, q = unpack(params, lay)
-    S = [qq @ qq.T for qq in q]
-    m = (np.asarray(mask) > 0).astype(np.float64)
+def loss_and_grad(params, lay, levels, pairs, emulate_bf16: bool = False):
+    """Summed loss over pairs [(r, e)] and its gradient (flat, library layout)."""
+    net = _Net(params, lay, levels, emulate_bf16)
+    A = levels[0]["A"]
+    m = levels[0]["mask"]
     grad = np.zeros(lay["nparams"])
     total = 0.0
     for r, e in pairs:
         r = np.asarray(r, np.float64)
         e = np.asarray(e, np.float64)
-        y, E, D = forward(params, lay, r, mask, emulate_bf16)
+        sr = {}
+        y = net.vcycle(0, r, sr)
         ea = float(e @ (A @ e))
         res = y - e
         Ares = A @ res
         total += float(res @ Ares) / ea
         g = 2.0 * m * Ares / ea
-        G = _encode(g, w_in, w_dn, lay, rnd)
-        Hh = _decode(G, w_dn, S, lay, rnd)
-        rin = np.zeros((H[0], W[0], 16)); rin[:, :, 0] = rnd(r.reshape(H[0], W[0]))
-        gin = np.zeros((H[0], W[0], 16)); gin[:, :, 0] = rnd(g.reshape(H[0], W[0]))
-        d_in = corr_same(Hh[0], rin) + corr_same(D[0], gin)
-        grad[lay["off_in"]: lay["off_in"] + d_in.size] += d_in.reshape(-1)
-        for l in range(1, L):
-            d = corr_down(Hh[l], E[l - 1]) + corr_down(D[l], G[l - 1])
-            grad[lay["off_down"][l]: lay["off_down"][l] + d.size] += d.reshape(-1)
-        for l in range(L):
-            dS = np.einsum("pqo,pqi->oi", G[l], E[l])
-            dQ = (dS + dS.T) @ q[l]
+        sg = {}
+        net.vcycle(0, g, sg)
+        for l in range(lay["learned"]):
+            lev = levels[l]
+            H, W, dh = lev["H"], lev["W"], lev["dh"]
+            dS = np.zeros_like(net.S[l])
+            dW = np.zeros_like(net.w_in[l])
+            for u, v in ((sg[l][1], sr[l][0]), (sg[l][0], sr[l][1])):
+                uu = _as_chan(net.rnd(dh * u), H, W)
+                vv = _as_chan(net.rnd(dh * v), H, W)
+                U = net.rnd(conv_same(uu, net.rnd(net.w_in[l])))
+                V = net.rnd(conv_same(vv, net.rnd(net.w_in[l])))
+                SU = net.rnd(conv_same(uu, net.SW[l]))
+                SV = net.rnd(conv_same(vv, net.SW[l]))
+                dS += np.einsum("pqo,pqi->oi", U, V)
+                dW += corr_same(SV, uu) + corr_same(SU, vv)
+            dQ = (dS + dS.T) @ net.q[l]
+            grad[lay["off_in"][l]: lay["off_in"][l] + dW.size] += dW.reshape(-1)
             grad[lay["off_q"][l]: lay["off_q"][l] + dQ.size] += dQ.reshape(-1)
     return total, grad
 
 
-def loss_only(params, lay, pairs, A, mask):
+def loss_only(params, lay, levels, pairs):
+    A = levels[0]["A"]
     tot = 0.0
     for r, e in pairs:
-        y, _, _ = forward(params, lay, r, mask)
+        y, _ = forward(params, lay, levels, r)
         res = y - np.asarray(e, np.float64)
         tot += float(res @ (A @ res)) / float(np.asarray(e) @ (A @ np.asarray(e)))
     return tot
 
 
+def initial_params(lay, seed: int = 1, small: float = 0.01):
+    """The library's initialisation: channel 0 of In is the identity tap, S = omega e0 e0^T
+    (Q_00 = sqrt(omega)), so the untrained smoother is damped Jacobi."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(lay["nparams"])
+    for l in range(lay["learned"]):
+        C = lay["C"][l]
+        w = p[lay["off_in"][l]: lay["off_in"][l] + 9 * C * 16].reshape(9, C, 16)
+        w[:, :, 0] = small * rng.uniform(-1, 1, (9, C))
+        w[:, 0, 0] = 0.0
+        w[4, 0, 0] = 1.0
+        q = p[lay["off_q"][l]: lay["off_q"][l] + C * C].reshape(C, C)
+        q[:] = small * rng.uniform(-1, 1, (C, C))
+        q[0, 0] = np.sqrt(OMEGA)
+    return p
+
+
 def scaled_operator(rowptr, col, val, diag, unknown_node, nnodes):
     """A' = D^-1/2 A D^-1/2 in node numbering (scipy CSR), with the D^-1/2 vector."""
-    import scipy.sparse as sp
     un = np.asarray(unknown_node) - 1
     n = len(diag)
     rows = np.repeat(np.arange(n), np.diff(rowptr))

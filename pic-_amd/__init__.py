@@ -472,9 +472,11 @@ class PicLibrary:
         return em.reshape((d.nnx - 1, d.nny - 1), order="F")
 
     # ------------------------------------------------------- learned preconditioner
-    def learn_setup(self, max_levels: int = 10, channels=(16, 16, 32), seed: int = 1) -> None:
+    def learn_setup(self, max_levels: int = 12, channels=(16, 16), seed: int = 1) -> None:
+        """Multigrid network for the current domain; len(channels) finest levels get learned
+        smoothers with channels[l] channels."""
         ch = np.ascontiguousarray(channels, dtype=np.int32)
-        check(lib().pic_learn_setup(max_levels, _i(ch), len(ch), seed))
+        check(lib().pic_learn_setup(max_levels, _i(ch) if len(ch) else None, len(ch), seed))
 
     def learn_harvest(self, enable: bool = True, capacity: int = 256, every: int = 1) -> None:
         check(lib().pic_learn_harvest(int(enable), capacity, every))
@@ -512,14 +514,14 @@ class PicLibrary:
         check(lib().pic_learn_set_params(_f(p), len(p)))
 
     def learn_layout(self) -> dict:
-        L = C.c_int32()
-        H = np.zeros(10, np.int32); W = np.zeros(10, np.int32); Cc = np.zeros(10, np.int32)
-        off = np.zeros(21, np.int64)
-        check(lib().pic_learn_layout(C.byref(L), _i(H), _i(W), _i(Cc), off.ctypes.data_as(_i64p)))
-        n = L.value
-        return {"levels": n, "H": H[:n].tolist(), "W": W[:n].tolist(), "C": Cc[:n].tolist(),
-                "off_in": int(off[0]), "off_down": [None] + [int(o) for o in off[1:n]],
-                "off_q": [int(o) for o in off[n:2 * n]], "nparams": int(off[2 * n])}
+        L = np.zeros(2, np.int32)
+        H = np.zeros(12, np.int32); W = np.zeros(12, np.int32); Cc = np.zeros(12, np.int32)
+        off = np.zeros(16, np.int64)
+        check(lib().pic_learn_layout(_i(L), _i(H), _i(W), _i(Cc), off.ctypes.data_as(_i64p)))
+        n, ls = int(L[0]), int(L[1])
+        return {"levels": n, "learned": ls, "H": H[:n].tolist(), "W": W[:n].tolist(), "C": Cc[:ls].tolist(),
+                "off_in": [int(o) for o in off[:ls]], "off_q": [int(o) for o in off[ls:2 * ls]],
+                "nparams": int(off[2 * ls])}
 
     def learn_set_pairs(self, r, e) -> None:
         r = np.ascontiguousarray(r, dtype=np.float32)

@@ -1,33 +1,40 @@
-// Learned preconditioner for the field solve: a convolutional encoder-decoder on the node
-// grid, BF16 MFMA with FP32 accumulation, trained on the GPU with its own forward and
-// backward kernels on residual/error pairs harvested from the plain solver, and applied as
-// the preconditioner of a flexible CG (FCG(1)) on the Jacobi-scaled system.
+// Learned preconditioner for the field solve: a multigrid-structured convolutional
+// encoder-decoder on the node grid whose smoothers are small convolutional networks (BF16
+// MFMA, FP32 accumulation). It is trained on the GPU with its own forward and backward
+// kernels on residual/error pairs harvested from the plain solver, and applied as the
+// preconditioner of a flexible CG (FCG(1)) on the Jacobi-scaled system.
 //
-// Network (linear, tied weights, symmetric positive semi-definite by construction):
-//   E_0 = In r                      3x3 conv, 1 -> C_0 channels, node grid H_0 x W_0
-//   E_l = Down_l E_{l-1}            3x3 stride-2 conv (coarse node I <-> fine node 2I)
-//   D_{L-1} = S_{L-1} E_{L-1}
-//   D_l = S_l E_l + Down_{l+1}^T D_{l+1}
-//   M r = m . In^T D_0              m = 1 on unknown nodes, 0 elsewhere
-// with S_l = Q_l Q_l^T (1x1 channel mixing), i.e. M = sum_l P_l^T S_l P_l, P_l = Down_l..Down_1 In.
-// Initialised as a multilevel (BPX-type) preconditioner: channel 0 of In is the identity,
-// channel 0 of every Down is full weighting, S_l = s_l e_0 e_0^T; the other channels start
-// small and random. Training minimises the energy-norm error of M on pairs (r, e), r = A' e:
-//   loss = sum_pairs (M r - e)^T A' (M r - e) / (e^T A' e)
-// Gradients (dL/dy = g = 2 m . A'(y - e) / e^T A' e):
-//   G_0 = In g, G_l = Down_l G_{l-1};  H_l = S_l G_l + Down_{l+1}^T H_{l+1}
-//   dIn   = corr(H_0, r) + corr(D_0, g)
-//   dDown_l = corr(H_l, E_{l-1}) + corr(D_l, G_{l-1})
-//   dS_l = sum_p G_l E_l^T,  dQ_l = (dS_l + dS_l^T) Q_l
-// where corr(o, x)[t][co][ci] = sum_p o[co](p) x[ci](src(p, t)).
+// Hierarchy (set up once per domain, FP64 on the host, stored FP32): level 0 is the
+// Jacobi-scaled node operator A' (unit diagonal on unknowns); level l+1 holds the nodes
+// (I, J) <-> fine (2I, 2J); P_l is bilinear interpolation with the rows of fine Dirichlet
+// nodes removed, R_l = P_l^T, A_{l+1} = P_l^T A_l P_l (nine diagonals per level); the
+// coarsest level (<= 3 nodes across) is solved exactly with a dense inverse.
 //
-// Activations are NHWC bf16 (pixel-major, channels contiguous, C a multiple of 16), one
-// pixel per node (node index i*nny + j of the node numbering). MFMA 16x16x16 bf16: a wave
-// computes 16 output pixels x 16 output channels per instruction, K = 16 input channels of
-// one tap.
+// Network = symmetric V(1,1)-cycle (encoder: smooth, residual, restrict; decoder:
+// interpolate, residual, smooth):
+//   x  = K_l r;  rc = R_l (r - A_l x);  ec = V_{l+1}(rc);  x2 = x + P_l ec;
+//   y  = x2 + K_l (r - A_l x2)
+// Smoothers: on the finest levels l < Ls a learned convolution
+//   K_l = Dh In_l^T S_l In_l Dh    (Dh = diag(A_l)^-1/2, In_l: 3x3 conv 1 -> C_l channels,
+//                                   S_l = Q_l Q_l^T, so K_l is symmetric positive semi-definite)
+// initialised to damped Jacobi (In = identity tap on channel 0, S = omega e0 e0^T); damped
+// Jacobi omega/diag on the coarser levels. M is symmetric; the untrained M is the classical
+// Jacobi V-cycle.
+// Training minimises sum (M r - e)^T A' (M r - e) / (e^T A' e) over pairs r = A' e. With
+// g = dL/dy, the gradient of a learned smoother is dL/dK = s2(g) (x) r_l(r) + r_l(g) (x) s2(r)
+// (inputs of its pre-/post-smoothing in the V-cycles run on r and on g), which gives, with
+// U = In Dh u, V = In Dh v:
+//   dS += sum_p U V^T,  dIn += corr(S V, Dh u) + corr(S U, Dh v),  dQ = (dS + dS^T) Q.
+// corr(o, x)[t][co][ci] = sum_p o[co](p) x[ci](p + d_t).
+//
+// Activations of the learned smoothers are NHWC bf16 (pixel-major, C contiguous, C a
+// multiple of 16), one pixel per node. MFMA 16x16x16 bf16: a wave computes 16 pixels x 16
+// output channels per instruction with K = 16 input channels of one tap.
 #include "pic_device.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -35,7 +42,10 @@ typedef uint16_t bf16;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-#define LN_MAX_LEVELS 10
+#define LN_MAX_LEVELS 12
+#define LN_MAX_LEARNED 4
+#define LN_OMEGA 0.8f
+#define LN_MAX_COARSE 2048
 
 __host__ __device__ inline bf16 f2bf(float f) {
     // round to nearest even (finite inputs)
@@ -47,33 +57,26 @@ __host__ __device__ inline bf16 f2bf(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return (bf16)(u >> 16);
 }
-__device__ inline float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 
-enum { CM_SAME = 0, CM_SAME_T = 1, CM_DOWN = 2, CM_UP = 3, CM_NONE = 4, CM_CENTER = 5 };
+enum { CM_SAME = 0, CM_SAME_T = 1, CM_NONE = 4, CM_CENTER = 5 };
 
 // source pixel of tap (di, dj) for output pixel (pi, pj); false if it does not exist
 __device__ inline bool conv_src(int mode, int pi, int pj, int di, int dj, int Hin, int Win, int& si, int& sj) {
-    switch (mode) {
-    case CM_SAME: si = pi + di; sj = pj + dj; break;
-    case CM_SAME_T: si = pi - di; sj = pj - dj; break;
-    case CM_DOWN: si = 2 * pi + di; sj = 2 * pj + dj; break;
-    case CM_UP: {
-        int a = pi - di, b = pj - dj;
-        if ((a & 1) || (b & 1)) return false;
-        si = a >> 1; sj = b >> 1; break;
-    }
-    default: si = pi; sj = pj; break;
-    }
+    if (mode == CM_SAME_T) { si = pi - di; sj = pj - dj; }
+    else if (mode == CM_SAME) { si = pi + di; sj = pj + dj; }
+    else { si = pi; sj = pj; }
     return si >= 0 && si < Hin && sj >= 0 && sj < Win;
 }
 
 // ---------------------------------------------------------------- convolution (MFMA)
 struct ConvArgs {
     const bf16* in; const float* in_f32; int Cin, Hin, Win; int64_t in_bs;
+    const float* in_scale;                           // in_f32 only: multiplied by in_scale[src]
     const bf16* w;                                   // [9][Cout][Cin]
     const bf16* in2; int Cin2; int64_t in2_bs; const bf16* w2;   // 1x1 term at the output pixel: [Cout][Cin2]
     bf16* out; float* out_f32; int Cout, Hout, Wout; int64_t out_bs;
-    const double* mask;                              // out_f32: zero where mask[p] <= 0
+    const float* out_scale;                          // out_f32 only: multiplied by out_scale[p]
+    const float* add_f32;                            // out_f32 only: + add_f32[b][p] (after scaling)
     int mode;
     const double* done;                              // skip the launch when *done != 0
 };
@@ -105,7 +108,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
             for (int cb = 0; cb < a.Cin; cb += 16) {
                 bf16x4 av;
                 if (inf) {                             // one FP32 input channel (channel 0)
-                    float v = ok ? inf[sp] : 0.0f;
+                    float v = 0.0f;
+                    if (ok) { v = inf[sp]; if (a.in_scale) v = v * a.in_scale[sp]; }
                     av = zero4;
                     if (kq == 0 && cb == 0) av[0] = (short)f2bf(v);
                 } else {
@@ -139,7 +143,8 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
         if (a.out_f32) {
             if (row == 0) {
                 float v = acc[0][r];
-                if (a.mask && !(a.mask[p] > 0.0)) v = 0.0f;
+                if (a.out_scale) v = v * a.out_scale[p];
+                if (a.add_f32) v = a.add_f32[(int64_t)b * a.out_bs + p] + v;
                 a.out_f32[(int64_t)b * a.out_bs + p] = v;
             }
         } else {
@@ -150,11 +155,157 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- multigrid transfer kernels
+struct MgArgs {
+    int Hf, Wf, Hc, Wc;
+    const float* dia;            // fine operator, nine diagonals [9][Hf*Wf], tap t <-> (t/3-1, t%3-1)
+    const float* dinv;           // fine 1/diag (0 on Dirichlet nodes)
+    const float* r;              // fine residual (input of this level)
+    float* x;                    // fine pre-smoothed correction
+    float* rc;                   // coarse residual (down)
+    const float* ec;             // coarse correction (up)
+    float* y;                    // fine output (up)
+    float* s2;                   // fine post-smoothing residual (up, learned smoother)
+    int64_t fs, cs;              // batch strides of fine / coarse vectors (r, x, y, s2 / rc, ec)
+    int64_t r_bs, y_bs;          // batch strides of r and y (level 0 uses the caller's vectors)
+    float omega;
+    int jacobi;                  // 1: x = omega/diag r here (down) and the post-smoothing too (up)
+    const double* done;
+};
+
+__device__ inline float rweight(int a) { return a == 0 ? 1.0f : 0.5f; }
+
+// Down: one thread per coarse node (I, J): x on the owned fine nodes (Jacobi levels),
+// rc = R (r - A x) over the 3x3 fine nodes around (2I, 2J).
+__global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
+    if (a.done && *a.done != 0.0) return;
+    const int b = blockIdx.y;
+    const int nc = a.Hc * a.Wc, nf = a.Hf * a.Wf;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int I = c / a.Wc, J = c - I * a.Wc;
+    const int fi0 = 2 * I, fj0 = 2 * J;
+    const float* r = a.r + (int64_t)b * a.r_bs;
+    float* x = a.x + (int64_t)b * a.fs;
+    float xw[5][5];
+#pragma unroll
+    for (int u = 0; u < 5; ++u)
+#pragma unroll
+        for (int v = 0; v < 5; ++v) {
+            int fi = fi0 + u - 2, fj = fj0 + v - 2;
+            float val = 0.0f;
+            if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
+                int f = fi * a.Wf + fj;
+                val = a.jacobi ? a.omega * a.dinv[f] * r[f] : x[f];
+            }
+            xw[u][v] = val;
+        }
+    if (a.jacobi) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                int fi = fi0 + u, fj = fj0 + v;
+                if (fi < a.Hf && fj < a.Wf) x[fi * a.Wf + fj] = xw[u + 2][v + 2];
+            }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int u = -1; u <= 1; ++u)
+#pragma unroll
+        for (int v = -1; v <= 1; ++v) {
+            int fi = fi0 + u, fj = fj0 + v;
+            if (fi < 0 || fi >= a.Hf || fj < 0 || fj >= a.Wf) continue;
+            int f = fi * a.Wf + fj;
+            if (a.dia[4 * (int64_t)nf + f] == 0.0f) continue;          // P row of a Dirichlet node is zero
+            float s = r[f];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) s = s - a.dia[(int64_t)t * nf + f] * xw[u + 2 + t / 3 - 1][v + 2 + t % 3 - 1];
+            acc = acc + rweight(u) * rweight(v) * s;
+        }
+    a.rc[(int64_t)b * a.cs + c] = acc;
+}
+
+// x2 at fine node (gi, gj): x + P ec (zero on Dirichlet nodes)
+__device__ inline float mg_x2(const MgArgs& a, const float* x, const float* ec, int gi, int gj) {
+    if (gi < 0 || gi >= a.Hf || gj < 0 || gj >= a.Wf) return 0.0f;
+    const int nf = a.Hf * a.Wf;
+    const int g = gi * a.Wf + gj;
+    if (a.dia[4 * (int64_t)nf + g] == 0.0f) return 0.0f;
+    float pe = 0.0f;
+    const int i0 = gi >> 1, j0 = gj >> 1;
+    const bool oi = gi & 1, oj = gj & 1;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (u == 1 && !oi) break;
+        int I = i0 + u;
+        if (I >= a.Hc) continue;
+        float wi = oi ? 0.5f : 1.0f;
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            if (v == 1 && !oj) break;
+            int J = j0 + v;
+            if (J >= a.Wc) continue;
+            float wj = oj ? 0.5f : 1.0f;
+            pe = pe + wi * wj * ec[I * a.Wc + J];
+        }
+    }
+    return x[g] + pe;
+}
+
+// Up: one thread per fine node: x2 = x + P ec; Jacobi levels y = x2 + omega/diag (r - A x2),
+// learned levels y = x2 and s2 = r - A x2 (the learned post-smoothing adds K s2 to y).
+__global__ __launch_bounds__(256) void k_mg_up(MgArgs a) {
+    if (a.done && *a.done != 0.0) return;
+    const int b = blockIdx.y;
+    const int nf = a.Hf * a.Wf;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const int fi = f / a.Wf, fj = f - fi * a.Wf;
+    const float* r = a.r + (int64_t)b * a.r_bs;
+    const float* x = a.x + (int64_t)b * a.fs;
+    const float* ec = a.ec + (int64_t)b * a.cs;
+    float* y = a.y + (int64_t)b * a.y_bs;
+    if (a.dia[4 * (int64_t)nf + f] == 0.0f) {
+        y[f] = 0.0f;
+        if (!a.jacobi) a.s2[(int64_t)b * a.fs + f] = 0.0f;
+        return;
+    }
+    float s = r[f];
+    float xc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        float xv = mg_x2(a, x, ec, fi + t / 3 - 1, fj + t % 3 - 1);
+        if (t == 4) xc = xv;
+        s = s - a.dia[(int64_t)t * nf + f] * xv;
+    }
+    if (a.jacobi) {
+        y[f] = xc + a.omega * a.dinv[f] * s;
+    } else {
+        y[f] = xc;
+        a.s2[(int64_t)b * a.fs + f] = s;
+    }
+}
+
+// Coarsest level: y = inv r (dense, n <= LN_MAX_COARSE), one block per batch entry.
+__global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, const float* r, float* y, int64_t bs,
+                                                   const double* done) {
+    if (done && *done != 0.0) return;
+    const float* rb = r + (int64_t)blockIdx.y * bs;
+    float* yb = y + (int64_t)blockIdx.y * bs;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        float s = 0.0f;
+        for (int j = 0; j < n; ++j) s = s + inv[(int64_t)i * n + j] * rb[j];
+        yb[i] = s;
+    }
+}
+
 // ---------------------------------------------------------------- weight-gradient correlation (MFMA)
 // part[blk][t][co][ci] = sum_{b, p in block} og[b][p][co] * in[b][src(p, t)][ci]
 struct CorrArgs {
     const bf16* og; int Cout, Hout, Wout; int64_t og_bs;
     const bf16* in; const float* in_f32; int Cin, Hin, Win; int64_t in_bs;
+    const float* in_scale;                           // in_f32 only: multiplied by in_scale[src]
     int mode;                                        // CM_SAME, CM_DOWN or CM_CENTER (1 tap)
     int batch, ppb;                                  // batch size, pixels per block (multiple of 16)
     float* part;
@@ -210,7 +361,11 @@ __global__ __launch_bounds__(256) void k_corr(CorrArgs a) {
                     for (int j = 0; j < 4; ++j) {
                         short v = 0;
                         if (okp[j]) {
-                            if (inf) v = (ci == 0) ? (short)f2bf(inf[sp[j]]) : (short)0;
+                            if (inf) {
+                                float xv = inf[sp[j]];
+                                if (a.in_scale) xv = xv * a.in_scale[sp[j]];
+                                v = (ci == 0) ? (short)f2bf(xv) : (short)0;
+                            }
                             else v = (short)inb[sp[j] * a.Cin + ci];
                         }
                         bv[i][j] = v;
@@ -237,15 +392,14 @@ __global__ __launch_bounds__(256) void k_corr(CorrArgs a) {
     }
 }
 
-// grad[k] = sum over regions, over blocks, of part[region + blk*size + k] (fixed order)
-struct GradRegion { int64_t off; int nblk; };
-__global__ void k_reduce_grad(int64_t size, const float* part, GradRegion r0, GradRegion r1, int nreg, float* grad) {
+// grad[k] = sum over the regions, over their blocks, of part[off + blk*size + k] (fixed order)
+struct GradRegions { int64_t off[4]; int nblk[4]; int n; };
+__global__ void k_reduce_grad(int64_t size, const float* part, GradRegions rg, float* grad) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= size) return;
     float s = 0.0f;
-    for (int b = 0; b < r0.nblk; ++b) s += part[r0.off + (int64_t)b * size + k];
-    if (nreg > 1)
-        for (int b = 0; b < r1.nblk; ++b) s += part[r1.off + (int64_t)b * size + k];
+    for (int q = 0; q < rg.n; ++q)
+        for (int b = 0; b < rg.nblk[q]; ++b) s += part[rg.off[q] + (int64_t)b * size + k];
     grad[k] = s;
 }
 
@@ -270,23 +424,21 @@ __global__ void k_adam(int64_t n, float* p, const float* gr, float* m, float* v,
     p[k] = p[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
 }
 
-// packed bf16 weights: W [9][Co][Ci] -> Wb (same) and WbT [9][Ci][Co]
-__global__ void k_pack_conv(int Co, int Ci, const float* W, bf16* Wb, bf16* WbT) {
+// Packed bf16 weights of one learned smoother from W_in [9][C][16] (input channel 0) and
+// Q [C][C]: pin = W_in, pinT = W_in^T per tap [9][16][C], pSin = (Q Q^T) W_in per tap.
+__global__ void k_pack_smoother(int C, const float* W, const float* Q, bf16* pin, bf16* pinT, bf16* pSin) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 9 * Co * Ci) return;
-    int t = k / (Co * Ci), r = k - t * Co * Ci, co = r / Ci, ci = r - co * Ci;
-    bf16 h = f2bf(W[k]);
-    Wb[k] = h;
-    WbT[(int64_t)t * Co * Ci + (int64_t)ci * Co + co] = h;
-}
-// S = Q Q^T -> bf16 [C][C]
-__global__ void k_pack_s(int C, const float* Q, bf16* S) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= C * C) return;
-    int i = k / C, j = k - i * C;
+    if (k >= 9 * C * 16) return;
+    int t = k / (C * 16), r = k - t * C * 16, co = r / 16, ci = r - co * 16;
+    pin[k] = f2bf(W[k]);
+    pinT[(int64_t)t * 16 * C + (int64_t)ci * C + co] = f2bf(W[k]);
     float s = 0.0f;
-    for (int m = 0; m < C; ++m) s += Q[i * C + m] * Q[j * C + m];
-    S[k] = f2bf(s);
+    for (int c = 0; c < C; ++c) {
+        float sc = 0.0f;                                 // S[co][c] = sum_m Q[co][m] Q[c][m]
+        for (int m = 0; m < C; ++m) sc += Q[co * C + m] * Q[c * C + m];
+        s += sc * W[((int64_t)t * C + c) * 16 + ci];
+    }
+    pSin[k] = f2bf(s);
 }
 
 // The scaled operator A' (unit diagonal on unknown rows, eight scaled off-diagonals, Dia8)
@@ -294,7 +446,7 @@ __global__ void k_pack_s(int C, const float* Q, bf16* S) {
 
 // loss and its gradient: res = y - e, g = 2 m . A' res / ea, loss partial res.A'res / ea
 __global__ __launch_bounds__(256) void k_loss_grad(int n, int batch, Dia8 A, const double* sinv, const float* y,
-                                                   const float* e, const double* ea, const int* idx, float* g,
+                                                   const float* e, const double* ea, float* g,
                                                    double* lpart) {
     __shared__ double sh[4];
     double acc = 0.0;
@@ -302,7 +454,7 @@ __global__ __launch_bounds__(256) void k_loss_grad(int n, int batch, Dia8 A, con
         int i = blockIdx.x * blockDim.x + threadIdx.x;
         const float* yb = y + (int64_t)b * n;
         const float* eb = e + (int64_t)b * n;
-        double inv = 1.0 / ea[idx[b]];
+        double inv = 1.0 / ea[b];
         if (i < n) {
             float ri = yb[i] - eb[i];
             float ar = ri;
@@ -343,6 +495,51 @@ __global__ __launch_bounds__(256) void k_dot_f(int n, const float* a, const floa
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// ---------------------------------------------------------------- synthetic pairs
+// Rough training errors complement the harvested (mostly smooth) ones so that every
+// frequency is in the loss: e = mask * uniform(-sqrt 3, sqrt 3) from the counter-based stream
+// (seed; node, step, tag), then `s` damped-Jacobi sweeps e <- e - A'e/2 shift it to smoother modes.
+__global__ void k_rough_e(int n, uint64_t seed, uint32_t step, uint32_t tag, const double* sinv, float* e) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pic_stream rs;
+    pic_stream_init(&rs, seed, (uint32_t)i, step, tag);
+    double u = pic_stream_u01(&rs);
+    e[i] = sinv[i] > 0.0 ? (float)((2.0 * u - 1.0) * 1.7320508075688772) : 0.0f;
+}
+__device__ inline float apply_scaled(int n, const Dia8& A, const float* v, int i) {
+    float s = v[i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int j = i + A.off[k];
+        if (j >= 0 && j < n) s = s + A.c[k][i] * v[j];
+    }
+    return s;
+}
+__global__ void k_smooth_e(int n, Dia8 A, const float* e, float* e_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) e_out[i] = e[i] - 0.5f * apply_scaled(n, A, e, i);
+}
+// r = A' e, partial e.r
+__global__ __launch_bounds__(256) void k_pair_from_e(int n, Dia8 A, const float* e, float* r, double* part) {
+    __shared__ double sh[4];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (i < n) {
+        float ri = apply_scaled(n, A, e, i);
+        r[i] = ri;
+        acc = (double)e[i] * (double)ri;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+__global__ void k_gather_ea(int nb, const int* idx, const double* ds_ea, double* eab) {
+    int b = threadIdx.x;
+    if (b < nb) eab[b] = ds_ea[idx[b]];
 }
 
 // ---------------------------------------------------------------- harvest
@@ -495,27 +692,40 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, int kk,
 }
 
 // ================================================================ host side
+struct MgLevel {
+    int H = 0, W = 0; int64_t n = 0;
+    float* dia = nullptr;        // [9][n]
+    float* dinv = nullptr;       // [n]
+    float* dh = nullptr;         // [n]  diag^-1/2
+    // per run (0: input r, 1: loss gradient g) vectors of bcap samples
+    float *r[2] = {nullptr, nullptr}, *x[2] = {nullptr, nullptr}, *y[2] = {nullptr, nullptr}, *s2[2] = {nullptr, nullptr};
+};
+
 struct LearnNet {
     bool ready = false;
-    int L = 0;                                       // levels 0..L-1
-    int H[LN_MAX_LEVELS] = {0}, W[LN_MAX_LEVELS] = {0}, C[LN_MAX_LEVELS] = {0};
-    int64_t npix[LN_MAX_LEVELS] = {0};
-    // parameters (FP32, flat): In [9][C0][16]; Down_l [9][C_l][C_{l-1}] (l >= 1); Q_l [C_l][C_l]
-    int64_t off_in = 0, off_dn[LN_MAX_LEVELS] = {0}, off_q[LN_MAX_LEVELS] = {0}, nparams = 0;
+    int L = 0, Ls = 0;                               // levels, learned smoother levels
+    MgLevel lv[LN_MAX_LEVELS];
+    int C[LN_MAX_LEARNED] = {0};
+    float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator
+    void* d_ops = nullptr;                           // one allocation: dia, dinv, dh of all levels
+    // parameters (FP32, flat): per learned level W_in [9][C][16] then Q [C][C]
+    int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
     float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr, *d_dS = nullptr;
-    bf16 *d_pk = nullptr;                            // packed bf16 weights (one allocation)
-    bf16 *pin = nullptr, *pinT = nullptr, *pdn[LN_MAX_LEVELS] = {nullptr}, *pdnT[LN_MAX_LEVELS] = {nullptr},
-         *pS[LN_MAX_LEVELS] = {nullptr};
+    bf16* d_pk = nullptr;
+    bf16 *pin[LN_MAX_LEARNED] = {nullptr}, *pinT[LN_MAX_LEARNED] = {nullptr}, *pSin[LN_MAX_LEARNED] = {nullptr};
     int adam_t = 0;
     // activations for a batch of bcap samples
     int bcap = 0;
     void* d_act = nullptr;
-    bf16 *E[LN_MAX_LEVELS] = {nullptr}, *D[LN_MAX_LEVELS] = {nullptr}, *G[LN_MAX_LEVELS] = {nullptr},
-         *Hh[LN_MAX_LEVELS] = {nullptr};
-    float *y = nullptr, *gl = nullptr, *rb = nullptr, *eb = nullptr;   // [bcap][N]
+    bf16* E[2][LN_MAX_LEARNED] = {{nullptr}};        // smoother activations per run
+    bf16 *U[LN_MAX_LEARNED] = {nullptr}, *V[LN_MAX_LEARNED] = {nullptr}, *SU[LN_MAX_LEARNED] = {nullptr},
+         *SV[LN_MAX_LEARNED] = {nullptr};
+    float *y0 = nullptr, *gl = nullptr, *rb = nullptr, *eb = nullptr;   // [bcap][N]
     float* d_part = nullptr; int64_t part_cap = 0;
     double* d_lpart = nullptr; double* d_lsum = nullptr;
     int* d_idx = nullptr;
+    double* d_eab = nullptr;                         // e^T A' e of the batch entries
+    uint32_t rough_step = 0;                         // counter of synthetic pairs drawn
     // data set of harvested pairs (ring)
     float *ds_r = nullptr, *ds_e = nullptr; double* ds_ea = nullptr;
     int ds_cap = 0, ds_n = 0, ds_head = 0;
@@ -526,17 +736,18 @@ struct LearnNet {
     double* d_hpart = nullptr;
     // FCG work vectors (padded node space)
     void* d_fvec = nullptr; float *z = nullptr, *w = nullptr, *q = nullptr;
-    // statistics of the last learned solve
-    int last_iters = 0, last_fallback = 0; long long fallbacks = 0;
+    // statistics; PIC_FCG_TRACE=1 prints each learned solve's residual history
+    int last_iters = 0; long long fallbacks = 0;
     double last_loss = 0.0;
+    bool trace = false;
 };
 static LearnNet ln;
 
-static int64_t ln_npix0() { return (int64_t)ln.H[0] * ln.W[0]; }
+static int64_t ln_npix0() { return ln.lv[0].n; }
 
 static void free_learn() {
-    void* ptrs[] = {ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part, ln.d_lpart, ln.d_lsum,
-                    ln.d_idx, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
+    void* ptrs[] = {ln.d_inv, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
+                    ln.d_lpart, ln.d_lsum, ln.d_idx, ln.d_eab, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     ln = LearnNet();
 }
@@ -554,62 +765,78 @@ static int launch_conv(const ConvArgs& a, int batch) {
     return PIC_OK;
 }
 
-// Forward of the network on `batch` inputs in_f32 (node-space, stride in_bs) into out_f32
-// (stride out_bs; may be null: then only the encoder/decoder tensors E, D are produced).
-static int net_forward(int batch, const float* in_f32, int64_t in_bs, float* out_f32, int64_t out_bs, bf16** E,
-                       bf16** D, const double* done) {
-    const int L = ln.L;
+// lift: out (bf16, C channels) = conv SAME of Dh u with weights w ([9][C][16])
+static int conv_lift(int l, int batch, const float* u, int64_t u_bs, const bf16* w, bf16* out, const double* done) {
+    const MgLevel& lv = ln.lv[l];
     ConvArgs a;
-    // E_0 = In r
     std::memset(&a, 0, sizeof(a));
-    a.in_f32 = in_f32; a.Cin = 16; a.Hin = ln.H[0]; a.Win = ln.W[0]; a.in_bs = in_bs;
-    a.w = ln.pin; a.out = E[0]; a.Cout = ln.C[0]; a.Hout = ln.H[0]; a.Wout = ln.W[0];
-    a.out_bs = ln.npix[0] * ln.C[0]; a.mode = CM_SAME; a.done = done;
-    int rc = launch_conv(a, batch); if (rc) return rc;
-    for (int l = 1; l < L; ++l) {
-        std::memset(&a, 0, sizeof(a));
-        a.in = E[l - 1]; a.Cin = ln.C[l - 1]; a.Hin = ln.H[l - 1]; a.Win = ln.W[l - 1]; a.in_bs = ln.npix[l - 1] * ln.C[l - 1];
-        a.w = ln.pdn[l]; a.out = E[l]; a.Cout = ln.C[l]; a.Hout = ln.H[l]; a.Wout = ln.W[l];
-        a.out_bs = ln.npix[l] * ln.C[l]; a.mode = CM_DOWN; a.done = done;
-        rc = launch_conv(a, batch); if (rc) return rc;
-    }
-    for (int l = L - 1; l >= 0; --l) {
-        std::memset(&a, 0, sizeof(a));
-        if (l == L - 1) {
-            a.mode = CM_NONE;
-        } else {
-            a.mode = CM_UP;
-            a.in = D[l + 1]; a.Cin = ln.C[l + 1]; a.Hin = ln.H[l + 1]; a.Win = ln.W[l + 1];
-            a.in_bs = ln.npix[l + 1] * ln.C[l + 1]; a.w = ln.pdnT[l + 1];
+    a.in_f32 = u; a.Cin = 16; a.Hin = lv.H; a.Win = lv.W; a.in_bs = u_bs; a.in_scale = lv.dh;
+    a.w = w; a.out = out; a.Cout = ln.C[l]; a.Hout = lv.H; a.Wout = lv.W; a.out_bs = lv.n * ln.C[l];
+    a.mode = CM_SAME; a.done = done;
+    return launch_conv(a, batch);
+}
+
+// project: y (+)= Dh In^T E
+static int conv_project(int l, int batch, const bf16* E, float* y, int64_t y_bs, bool add, const double* done) {
+    const MgLevel& lv = ln.lv[l];
+    ConvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = E; a.Cin = ln.C[l]; a.Hin = lv.H; a.Win = lv.W; a.in_bs = lv.n * ln.C[l];
+    a.w = ln.pinT[l]; a.out_f32 = y; a.Cout = 16; a.Hout = lv.H; a.Wout = lv.W; a.out_bs = y_bs;
+    a.out_scale = lv.dh; a.add_f32 = add ? y : nullptr; a.mode = CM_SAME_T; a.done = done;
+    return launch_conv(a, batch);
+}
+
+static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, int64_t y_bs, const double* done) {
+    const MgLevel& f = ln.lv[l];
+    const MgLevel& c = ln.lv[l + 1];
+    MgArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.Hf = f.H; a.Wf = f.W; a.Hc = c.H; a.Wc = c.W;
+    a.dia = f.dia; a.dinv = f.dinv; a.r = r; a.x = f.x[run];
+    a.rc = c.r[run]; a.ec = c.y[run]; a.y = y; a.s2 = f.s2[run];
+    a.fs = f.n; a.cs = c.n; a.r_bs = r_bs; a.y_bs = y_bs;
+    a.omega = LN_OMEGA; a.jacobi = l >= ln.Ls; a.done = done;
+    return a;
+}
+
+// y = M r for `batch` inputs (level-0 vectors with batch stride bs); run selects the work set.
+static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done) {
+    const int L = ln.L;
+    for (int l = 0; l < L - 1; ++l) {
+        const float* r = l == 0 ? r0 : ln.lv[l].r[run];
+        int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
+        if (l < ln.Ls) {
+            int rc = conv_lift(l, batch, r, r_bs, ln.pSin[l], ln.E[run][l], done); if (rc) return rc;
+            rc = conv_project(l, batch, ln.E[run][l], ln.lv[l].x[run], ln.lv[l].n, false, done); if (rc) return rc;
         }
-        a.in2 = E[l]; a.Cin2 = ln.C[l]; a.in2_bs = ln.npix[l] * ln.C[l]; a.w2 = ln.pS[l];
-        a.out = D[l]; a.Cout = ln.C[l]; a.Hout = ln.H[l]; a.Wout = ln.W[l]; a.out_bs = ln.npix[l] * ln.C[l];
-        a.done = done;
-        rc = launch_conv(a, batch); if (rc) return rc;
+        MgArgs a = mg_args(l, run, r, r_bs, nullptr, 0, done);
+        hipLaunchKernelGGL(k_mg_down, dim3((unsigned)((ln.lv[l + 1].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
     }
-    if (out_f32) {
-        std::memset(&a, 0, sizeof(a));
-        a.in = D[0]; a.Cin = ln.C[0]; a.Hin = ln.H[0]; a.Win = ln.W[0]; a.in_bs = ln.npix[0] * ln.C[0];
-        a.w = ln.pinT; a.out_f32 = out_f32; a.Cout = 16; a.Hout = ln.H[0]; a.Wout = ln.W[0]; a.out_bs = out_bs;
-        a.mask = g.d_sinv; a.mode = CM_SAME_T; a.done = done;
-        rc = launch_conv(a, batch); if (rc) return rc;
+    const MgLevel& cl = ln.lv[L - 1];
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1, batch), dim3(256), 0, g.stream, (int)cl.n, ln.d_inv, cl.r[run], cl.y[run],
+                       cl.n, done);
+    for (int l = L - 2; l >= 0; --l) {
+        const float* r = l == 0 ? r0 : ln.lv[l].r[run];
+        int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
+        float* y = l == 0 ? y0 : ln.lv[l].y[run];
+        int64_t y_bs = l == 0 ? bs : ln.lv[l].n;
+        MgArgs a = mg_args(l, run, r, r_bs, y, y_bs, done);
+        hipLaunchKernelGGL(k_mg_up, dim3((unsigned)((ln.lv[l].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
+        if (l < ln.Ls) {
+            int rc = conv_lift(l, batch, ln.lv[l].s2[run], ln.lv[l].n, ln.pSin[l], ln.E[run][l], done); if (rc) return rc;
+            rc = conv_project(l, batch, ln.E[run][l], y, y_bs, true, done); if (rc) return rc;
+        }
     }
+    HIPCHK(hipGetLastError());
     return PIC_OK;
 }
 
 static int pack_weights() {
-    const int L = ln.L;
-    int n = 9 * ln.C[0] * 16;
-    hipLaunchKernelGGL(k_pack_conv, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[0], 16, ln.d_p + ln.off_in,
-                       ln.pin, ln.pinT);
-    for (int l = 1; l < L; ++l) {
-        n = 9 * ln.C[l] * ln.C[l - 1];
-        hipLaunchKernelGGL(k_pack_conv, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.C[l - 1],
-                           ln.d_p + ln.off_dn[l], ln.pdn[l], ln.pdnT[l]);
-    }
-    for (int l = 0; l < L; ++l) {
-        n = ln.C[l] * ln.C[l];
-        hipLaunchKernelGGL(k_pack_s, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.d_p + ln.off_q[l], ln.pS[l]);
+    for (int l = 0; l < ln.Ls; ++l) {
+        int n = 9 * ln.C[l] * 16;
+        hipLaunchKernelGGL(k_pack_smoother, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l],
+                           ln.d_p + ln.off_q[l], ln.pin[l], ln.pinT[l], ln.pSin[l]);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
@@ -618,110 +845,234 @@ static int pack_weights() {
 static int ensure_batch(int batch) {
     if (batch <= ln.bcap) return PIC_OK;
     if (ln.d_act) { (void)hipFree(ln.d_act); ln.d_act = nullptr; }
-    size_t per = 0;
-    for (int l = 0; l < ln.L; ++l) per += (size_t)ln.npix[l] * ln.C[l] * sizeof(bf16);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    size_t bytes = 0;
+    for (int l = 0; l < ln.L; ++l) bytes += 8 * al((size_t)ln.lv[l].n * sizeof(float) * batch);
+    for (int l = 0; l < ln.Ls; ++l) bytes += 6 * al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch);
     const size_t N = (size_t)ln_npix0();
-    size_t bytes = 4 * per * batch + 4 * N * sizeof(float) * batch + 256 * 64;
+    bytes += 4 * al(N * sizeof(float) * batch);
     HIPCHK(hipMalloc(&ln.d_act, bytes));
     HIPCHK(hipMemset(ln.d_act, 0, bytes));
     char* p = (char*)ln.d_act;
-    bf16** sets[4] = {ln.E, ln.D, ln.G, ln.Hh};
-    for (int s = 0; s < 4; ++s)
-        for (int l = 0; l < ln.L; ++l) {
-            sets[s][l] = (bf16*)p;
-            p += ((size_t)ln.npix[l] * ln.C[l] * sizeof(bf16) * batch + 255) / 256 * 256;
-        }
-    float** fs[4] = {&ln.y, &ln.gl, &ln.rb, &ln.eb};
-    for (int s = 0; s < 4; ++s) { *fs[s] = (float*)p; p += (N * sizeof(float) * batch + 255) / 256 * 256; }
+    for (int l = 0; l < ln.L; ++l) {
+        MgLevel& v = ln.lv[l];
+        float** vs[8] = {&v.r[0], &v.r[1], &v.x[0], &v.x[1], &v.y[0], &v.y[1], &v.s2[0], &v.s2[1]};
+        for (int k = 0; k < 8; ++k) { *vs[k] = (float*)p; p += al((size_t)v.n * sizeof(float) * batch); }
+    }
+    for (int l = 0; l < ln.Ls; ++l) {
+        bf16** bs[6] = {&ln.E[0][l], &ln.E[1][l], &ln.U[l], &ln.V[l], &ln.SU[l], &ln.SV[l]};
+        for (int k = 0; k < 6; ++k) { *bs[k] = (bf16*)p; p += al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch); }
+    }
+    float** fs[4] = {&ln.y0, &ln.gl, &ln.rb, &ln.eb};
+    for (int s = 0; s < 4; ++s) { *fs[s] = (float*)p; p += al(N * sizeof(float) * batch); }
     ln.bcap = batch;
     return PIC_OK;
 }
 
-// Dia8 view of the scaled operator on node space
+// Dia8 view of the scaled level-0 operator (off-diagonals of the one-kernel CG)
 static Dia8 scaled_op() {
     Dia8 A;
     for (int k = 0; k < 8; ++k) { A.c[k] = g.d_cg_off[k]; A.off[k] = g.cg_off[k]; }
     return A;
 }
 
-// Initial parameters: channel 0 is a multilevel preconditioner (identity lift, full
-// weighting restrictions, S_l = e_0 e_0^T), the other channels small and random.
+// ---- hierarchy on the host (FP64)
+static void galerkin(int Hf, int Wf, const std::vector<double>& af, int Hc, int Wc, std::vector<double>& ac) {
+    const int64_t nf = (int64_t)Hf * Wf, nc = (int64_t)Hc * Wc;
+    ac.assign(9 * nc, 0.0);
+    auto mask = [&](int i, int j) { return af[4 * nf + (int64_t)i * Wf + j] != 0.0; };
+    for (int I = 0; I < Hc; ++I)
+        for (int J = 0; J < Wc; ++J) {
+            const int64_t c = (int64_t)I * Wc + J;
+            for (int u = -1; u <= 1; ++u)
+                for (int v = -1; v <= 1; ++v) {
+                    int f1i = 2 * I + u, f1j = 2 * J + v;
+                    if (f1i < 0 || f1i >= Hf || f1j < 0 || f1j >= Wf || !mask(f1i, f1j)) continue;
+                    double w1 = (u == 0 ? 1.0 : 0.5) * (v == 0 ? 1.0 : 0.5);
+                    int64_t f1 = (int64_t)f1i * Wf + f1j;
+                    for (int t = 0; t < 9; ++t) {
+                        double a12 = af[t * nf + f1];
+                        if (a12 == 0.0) continue;
+                        int f2i = f1i + t / 3 - 1, f2j = f1j + t % 3 - 1;
+                        if (f2i < 0 || f2i >= Hf || f2j < 0 || f2j >= Wf || !mask(f2i, f2j)) continue;
+                        // parents of f2
+                        int i0 = f2i >> 1, j0 = f2j >> 1;
+                        bool oi = f2i & 1, oj = f2j & 1;
+                        for (int a = 0; a < (oi ? 2 : 1); ++a)
+                            for (int b = 0; b < (oj ? 2 : 1); ++b) {
+                                int I2 = i0 + a, J2 = j0 + b;
+                                if (I2 >= Hc || J2 >= Wc) continue;
+                                double w2 = (oi ? 0.5 : 1.0) * (oj ? 0.5 : 1.0);
+                                int dI = I2 - I, dJ = J2 - J;
+                                ac[(int64_t)(3 * (dI + 1) + (dJ + 1)) * nc + c] += w1 * a12 * w2;
+                            }
+                    }
+                }
+        }
+}
+
+static bool dense_inverse(std::vector<double>& a, int n) {
+    std::vector<double> inv(n * (size_t)n, 0.0);
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r) if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)piv * n + c])) piv = r;
+        if (a[(size_t)piv * n + c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) { std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]); std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]); }
+        double d = 1.0 / a[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) { a[(size_t)c * n + k] *= d; inv[(size_t)c * n + k] *= d; }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            double f = a[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) { a[(size_t)r * n + k] -= f * a[(size_t)c * n + k]; inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k]; }
+        }
+    }
+    a.swap(inv);
+    return true;
+}
+
+// Initial parameters: In = identity tap on channel 0 (+ small random taps on the others),
+// Q = sqrt(omega) e0 e0^T (+ small random), so the untrained smoother is omega/diag.
 static void init_params(std::vector<float>& P, uint64_t seed) {
     uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
     auto rnd = [&]() {                                // uniform in [-1, 1)
         st ^= st << 13; st ^= st >> 7; st ^= st << 17;
         return (float)((double)(st >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
     };
-    const float fw[9] = {0.25f, 0.5f, 0.25f, 0.5f, 1.0f, 0.5f, 0.25f, 0.5f, 0.25f};
     const float small = 0.01f;
-    // In [9][C0][16]: only input channel 0 exists
-    for (int t = 0; t < 9; ++t)
-        for (int co = 0; co < ln.C[0]; ++co)
-            P[ln.off_in + (t * ln.C[0] + co) * 16 + 0] = (co == 0) ? (t == 4 ? 1.0f : 0.0f) : small * rnd();
-    for (int l = 1; l < ln.L; ++l)
+    for (int l = 0; l < ln.Ls; ++l) {
+        const int C = ln.C[l];
         for (int t = 0; t < 9; ++t)
-            for (int co = 0; co < ln.C[l]; ++co)
-                for (int ci = 0; ci < ln.C[l - 1]; ++ci)
-                    P[ln.off_dn[l] + ((int64_t)t * ln.C[l] + co) * ln.C[l - 1] + ci] =
-                        (co == 0 && ci == 0) ? fw[t] : small * rnd();
-    for (int l = 0; l < ln.L; ++l)
-        for (int i = 0; i < ln.C[l]; ++i)
-            for (int j = 0; j < ln.C[l]; ++j)
-                P[ln.off_q[l] + i * ln.C[l] + j] = (i == 0 && j == 0) ? 1.0f : small * rnd();
+            for (int co = 0; co < C; ++co)
+                P[ln.off_in[l] + ((int64_t)t * C + co) * 16] = (co == 0) ? (t == 4 ? 1.0f : 0.0f) : small * rnd();
+        for (int i = 0; i < C; ++i)
+            for (int j = 0; j < C; ++j)
+                P[ln.off_q[l] + i * C + j] = (i == 0 && j == 0) ? std::sqrt(LN_OMEGA) : small * rnd();
+    }
 }
 
 int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed) {
     if (!g.have_domain) return pic_fail(PIC_ERR_STATE, "pic_setup_domain not called");
-    if (max_levels < 1 || max_levels > LN_MAX_LEVELS) return pic_fail(PIC_ERR_ARG, "levels must be 1..10");
+    if (max_levels < 2 || max_levels > LN_MAX_LEVELS) return pic_fail(PIC_ERR_ARG, "levels must be 2..12");
+    if (nch < 0 || nch > LN_MAX_LEARNED) return pic_fail(PIC_ERR_ARG, "0..4 learned smoother levels");
     free_learn();
     // level sizes: coarse node I <-> fine node 2I, down to about 3 x 3
-    ln.H[0] = g.mesh.nnx; ln.W[0] = g.mesh.nny;
+    int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
+    H[0] = g.mesh.nnx; W[0] = g.mesh.nny;
     int L = 1;
-    while (L < max_levels && std::min(ln.H[L - 1], ln.W[L - 1]) > 3) {
-        ln.H[L] = (ln.H[L - 1] + 1) / 2; ln.W[L] = (ln.W[L - 1] + 1) / 2; L++;
-    }
+    while (L < max_levels && std::min(H[L - 1], W[L - 1]) > 3) { H[L] = (H[L - 1] + 1) / 2; W[L] = (W[L - 1] + 1) / 2; L++; }
+    if (L < 2) return pic_fail(PIC_ERR_UNSUPPORTED, "mesh too small for a multigrid hierarchy");
+    if ((int64_t)H[L - 1] * W[L - 1] > LN_MAX_COARSE) return pic_fail(PIC_ERR_UNSUPPORTED, "coarsest level too large");
     ln.L = L;
-    for (int l = 0; l < L; ++l) {
-        int c = channels && nch > 0 ? channels[std::min(l, nch - 1)] : (l < 2 ? 16 : 32);
+    ln.Ls = std::min(nch, L - 1);
+    for (int l = 0; l < ln.Ls; ++l) {
+        int c = channels[l];
         if (c != 16 && c != 32 && c != 64) return pic_fail(PIC_ERR_ARG, "channels must be 16, 32 or 64");
         ln.C[l] = c;
-        ln.npix[l] = (int64_t)ln.H[l] * ln.W[l];
     }
+    // level-0 operator: A' = D^-1/2 A D^-1/2 on unknowns, nine node-space diagonals
+    const int64_t N = (int64_t)H[0] * W[0];
+    std::vector<std::vector<double>> ops(L);
+    ops[0].assign(9 * N, 0.0);
+    {
+        const int nu = (int)g.h_A_diag.size();
+        for (int u = 0; u < nu; ++u) {
+            int fu = g.h_unknown_node[u] - 1;
+            double su = 1.0 / std::sqrt(g.h_A_diag[u]);
+            for (int k = g.h_A_rowptr[u]; k < g.h_A_rowptr[u + 1]; ++k) {
+                int v = g.h_A_col[k];
+                int fv = g.h_unknown_node[v] - 1;
+                int di = fv / W[0] - fu / W[0], dj = fv % W[0] - fu % W[0];
+                if (di < -1 || di > 1 || dj < -1 || dj > 1) return pic_fail(PIC_ERR_UNSUPPORTED, "stiffness off the 9-point stencil");
+                ops[0][(int64_t)(3 * (di + 1) + (dj + 1)) * N + fu] = g.h_A_val[k] * su / std::sqrt(g.h_A_diag[v]);
+            }
+        }
+    }
+    for (int l = 1; l < L; ++l) galerkin(H[l - 1], W[l - 1], ops[l - 1], H[l], W[l], ops[l]);
+    // coarsest dense inverse over its unknowns
+    const int nc = H[L - 1] * W[L - 1];
+    std::vector<float> invf((size_t)nc * nc, 0.0f);
+    {
+        const std::vector<double>& a = ops[L - 1];
+        std::vector<int> idx;
+        for (int i = 0; i < nc; ++i) if (a[4 * (int64_t)nc + i] != 0.0) idx.push_back(i);
+        const int m = (int)idx.size();
+        if (m > 0) {
+            std::vector<double> d((size_t)m * m, 0.0);
+            for (int r = 0; r < m; ++r) {
+                int i = idx[r], ii = i / W[L - 1], jj = i % W[L - 1];
+                for (int t = 0; t < 9; ++t) {
+                    int ni = ii + t / 3 - 1, nj = jj + t % 3 - 1;
+                    if (ni < 0 || ni >= H[L - 1] || nj < 0 || nj >= W[L - 1]) continue;
+                    int j = ni * W[L - 1] + nj;
+                    auto it = std::lower_bound(idx.begin(), idx.end(), j);
+                    if (it == idx.end() || *it != j) continue;
+                    d[(size_t)r * m + (it - idx.begin())] += a[(int64_t)t * nc + i];
+                }
+            }
+            if (!dense_inverse(d, m)) return pic_fail(PIC_ERR_STATE, "singular coarsest operator");
+            for (int r = 0; r < m; ++r)
+                for (int c = 0; c < m; ++c) invf[(size_t)idx[r] * nc + idx[c]] = (float)d[(size_t)r * m + c];
+        }
+    }
+    // upload operators: [dia 9n | dinv n | dh n] per level
+    size_t tot = 0;
+    for (int l = 0; l < L; ++l) tot += 11 * (size_t)H[l] * W[l];
+    HIPCHK(hipMalloc(&ln.d_ops, tot * sizeof(float)));
+    {
+        std::vector<float> host(tot);
+        float* hp = host.data();
+        float* dp = (float*)ln.d_ops;
+        for (int l = 0; l < L; ++l) {
+            MgLevel& v = ln.lv[l];
+            v.H = H[l]; v.W = W[l]; v.n = (int64_t)H[l] * W[l];
+            for (int64_t k = 0; k < 9 * v.n; ++k) hp[k] = (float)ops[l][k];
+            for (int64_t i = 0; i < v.n; ++i) {
+                double d = ops[l][4 * v.n + i];
+                hp[9 * v.n + i] = d != 0.0 ? (float)(1.0 / d) : 0.0f;
+                hp[10 * v.n + i] = d > 0.0 ? (float)(1.0 / std::sqrt(d)) : 0.0f;
+            }
+            v.dia = dp; v.dinv = dp + 9 * v.n; v.dh = dp + 10 * v.n;
+            hp += 11 * v.n; dp += 11 * v.n;
+        }
+        HIPCHK(hipMemcpy(ln.d_ops, host.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMalloc(&ln.d_inv, invf.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(ln.d_inv, invf.data(), invf.size() * sizeof(float), hipMemcpyHostToDevice));
+    ln.ncoarse = nc;
+    // parameters
     int64_t o = 0;
-    ln.off_in = o; o += 9 * ln.C[0] * 16;
-    for (int l = 1; l < L; ++l) { ln.off_dn[l] = o; o += 9LL * ln.C[l] * ln.C[l - 1]; }
-    for (int l = 0; l < L; ++l) { ln.off_q[l] = o; o += (int64_t)ln.C[l] * ln.C[l]; }
+    for (int l = 0; l < ln.Ls; ++l) { ln.off_in[l] = o; o += 9LL * ln.C[l] * 16; }
+    for (int l = 0; l < ln.Ls; ++l) { ln.off_q[l] = o; o += (int64_t)ln.C[l] * ln.C[l]; }
     ln.nparams = o;
-    std::vector<float> P(o, 0.0f);
+    std::vector<float> P(std::max<int64_t>(o, 1), 0.0f);
     init_params(P, seed);
-    HIPCHK(hipMalloc(&ln.d_p, o * sizeof(float))); HIPCHK(hipMalloc(&ln.d_m, o * sizeof(float)));
-    HIPCHK(hipMalloc(&ln.d_v, o * sizeof(float))); HIPCHK(hipMalloc(&ln.d_g, o * sizeof(float)));
-    HIPCHK(hipMalloc(&ln.d_dS, o * sizeof(float)));
-    HIPCHK(hipMemcpy(ln.d_p, P.data(), o * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(ln.d_m, 0, o * sizeof(float))); HIPCHK(hipMemset(ln.d_v, 0, o * sizeof(float)));
-    HIPCHK(hipMemset(ln.d_g, 0, o * sizeof(float))); HIPCHK(hipMemset(ln.d_dS, 0, o * sizeof(float)));
-    // packed weights: In, In^T, Down_l, Down_l^T, S_l
-    size_t nb = 2 * 9 * ln.C[0] * 16;
-    for (int l = 1; l < L; ++l) nb += 2 * 9 * (size_t)ln.C[l] * ln.C[l - 1];
-    for (int l = 0; l < L; ++l) nb += (size_t)ln.C[l] * ln.C[l];
+    const int64_t no = std::max<int64_t>(o, 1);
+    HIPCHK(hipMalloc(&ln.d_p, no * sizeof(float))); HIPCHK(hipMalloc(&ln.d_m, no * sizeof(float)));
+    HIPCHK(hipMalloc(&ln.d_v, no * sizeof(float))); HIPCHK(hipMalloc(&ln.d_g, no * sizeof(float)));
+    HIPCHK(hipMalloc(&ln.d_dS, no * sizeof(float)));
+    HIPCHK(hipMemcpy(ln.d_p, P.data(), no * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ln.d_m, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_v, 0, no * sizeof(float)));
+    HIPCHK(hipMemset(ln.d_g, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_dS, 0, no * sizeof(float)));
+    size_t nb = 1;
+    for (int l = 0; l < ln.Ls; ++l) nb += 3 * 9 * (size_t)ln.C[l] * 16;
     HIPCHK(hipMalloc(&ln.d_pk, nb * sizeof(bf16)));
     bf16* q = ln.d_pk;
-    ln.pin = q; q += 9 * ln.C[0] * 16; ln.pinT = q; q += 9 * ln.C[0] * 16;
-    for (int l = 1; l < L; ++l) {
-        size_t s = 9 * (size_t)ln.C[l] * ln.C[l - 1];
-        ln.pdn[l] = q; q += s; ln.pdnT[l] = q; q += s;
+    for (int l = 0; l < ln.Ls; ++l) {
+        size_t s = 9 * (size_t)ln.C[l] * 16;
+        ln.pin[l] = q; q += s; ln.pinT[l] = q; q += s; ln.pSin[l] = q; q += s;
     }
-    for (int l = 0; l < L; ++l) { ln.pS[l] = q; q += (size_t)ln.C[l] * ln.C[l]; }
-    // gradient partials: two correlations per conv tensor, <= 256 blocks each
-    int64_t pc = 2LL * 256 * 9 * ln.C[0] * 16;
-    for (int l = 1; l < L; ++l) pc += 2LL * 256 * 9 * ln.C[l] * ln.C[l - 1];
-    for (int l = 0; l < L; ++l) pc += 256LL * ln.C[l] * ln.C[l];
+    // gradient partials: per learned level 4 SAME correlations + 2 CENTER, <= 256 blocks each
+    int64_t pc = 64;
+    for (int l = 0; l < ln.Ls; ++l) pc += 256LL * (4 * 9 * ln.C[l] * 16 + 2 * ln.C[l] * ln.C[l]);
     ln.part_cap = pc;
     HIPCHK(hipMalloc(&ln.d_part, pc * sizeof(float)));
-    int64_t N = ln_npix0();
     HIPCHK(hipMalloc(&ln.d_lpart, ((N + 255) / 256 + 64) * sizeof(double)));
     HIPCHK(hipMalloc(&ln.d_lsum, 64 * sizeof(double)));
     HIPCHK(hipMalloc(&ln.d_idx, 1024 * sizeof(int)));
+    HIPCHK(hipMalloc(&ln.d_eab, 1024 * sizeof(double)));
     HIPCHK(hipMalloc(&ln.d_hpart, ((N + 255) / 256 + 64) * sizeof(double)));
     // FCG vectors with the padding of the CG vectors
     size_t pad = ((size_t)g.mesh.nny + 2 + 63) / 64 * 64, len = (size_t)N + 2 * pad;
@@ -732,6 +1083,8 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     int rc = pack_weights(); if (rc) return rc;
     rc = ensure_batch(1); if (rc) return rc;
     HIPCHK(hipStreamSynchronize(g.stream));
+    const char* tr = std::getenv("PIC_FCG_TRACE");
+    ln.trace = tr && tr[0] == '1';
     ln.ready = true;
     return PIC_OK;
 }
@@ -817,64 +1170,77 @@ static int launch_corr(CorrArgs a, int64_t& part_off, CorrPlan& plan) {
     const int mt = a.Cout / 16, nt = a.Cin / 16;
     dim3 grid(nblk), blk(256);
 #define CORR_CASE(M_, N_) if (mt == M_ && nt == N_) { hipLaunchKernelGGL((k_corr<M_, N_>), grid, blk, 0, g.stream, a); HIPCHK(hipGetLastError()); return PIC_OK; }
-    CORR_CASE(1, 1) CORR_CASE(1, 2) CORR_CASE(1, 4) CORR_CASE(2, 1) CORR_CASE(2, 2) CORR_CASE(2, 4)
-    CORR_CASE(4, 1) CORR_CASE(4, 2) CORR_CASE(4, 4)
+    CORR_CASE(1, 1) CORR_CASE(2, 1) CORR_CASE(4, 1) CORR_CASE(2, 2) CORR_CASE(4, 4)
 #undef CORR_CASE
     return pic_fail(PIC_ERR_ARG, "learned network: unsupported channel pair");
 }
 
-static CorrArgs corr_args(const bf16* og, int lo, const bf16* in, const float* in_f32, int li, int Cin, int mode, int batch) {
+// og (C channels at level l) against the FP32 input u (scaled by Dh), 9 taps
+static CorrArgs corr_in(int l, const bf16* og, const float* u, int64_t u_bs, int batch) {
     CorrArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.og = og; a.Cout = ln.C[lo]; a.Hout = ln.H[lo]; a.Wout = ln.W[lo]; a.og_bs = ln.npix[lo] * ln.C[lo];
-    a.in = in; a.in_f32 = in_f32; a.Cin = Cin; a.Hin = ln.H[li]; a.Win = ln.W[li];
-    a.in_bs = in_f32 ? ln.npix[li] : ln.npix[li] * Cin;
-    a.mode = mode; a.batch = batch;
+    const MgLevel& v = ln.lv[l];
+    a.og = og; a.Cout = ln.C[l]; a.Hout = v.H; a.Wout = v.W; a.og_bs = v.n * ln.C[l];
+    a.in_f32 = u; a.in_scale = v.dh; a.Cin = 16; a.Hin = v.H; a.Win = v.W; a.in_bs = u_bs;
+    a.mode = CM_SAME; a.batch = batch;
+    return a;
+}
+// og against in (both C channels at level l), centre tap
+static CorrArgs corr_center(int l, const bf16* og, const bf16* in, int batch) {
+    CorrArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const MgLevel& v = ln.lv[l];
+    a.og = og; a.Cout = ln.C[l]; a.Hout = v.H; a.Wout = v.W; a.og_bs = v.n * ln.C[l];
+    a.in = in; a.Cin = ln.C[l]; a.Hin = v.H; a.Win = v.W; a.in_bs = v.n * ln.C[l];
+    a.mode = CM_CENTER; a.batch = batch;
     return a;
 }
 
-// Loss and gradient for `batch` pairs already in ln.rb / ln.eb (indices idx into ds_ea),
-// gradient written to ln.d_g (sum over the batch). Returns the summed loss in *loss.
+// Loss and gradient for `batch` pairs already in ln.rb / ln.eb (indices in d_idx), gradient
+// summed over the batch into ln.d_g. Returns the summed loss in *loss.
 static int loss_and_gradient(int batch, double* loss) {
     const int n = (int)ln_npix0();
-    const int L = ln.L;
     const int64_t N = n;
-    int rc = net_forward(batch, ln.rb, N, ln.y, N, ln.E, ln.D, nullptr); if (rc) return rc;
+    int rc = net_apply(batch, ln.rb, ln.y0, N, 0, nullptr); if (rc) return rc;
     const int nb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, g.stream, n, batch, scaled_op(), g.d_sinv, ln.y, ln.eb,
-                       ln.ds_ea, ln.d_idx, ln.gl, ln.d_lpart);
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, g.stream, n, batch, scaled_op(), g.d_sinv, ln.y0, ln.eb,
+                       ln.d_eab, ln.gl, ln.d_lpart);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.stream, nb, ln.d_lpart, ln.d_lsum);
     HIPCHK(hipGetLastError());
-    // backward: G = encoder(g), H = decoder(G)
-    rc = net_forward(batch, ln.gl, N, nullptr, 0, ln.G, ln.Hh, nullptr); if (rc) return rc;
-    // correlations
+    // the same V-cycle on g (its level-0 output is not needed; it lands in ln.y0's twin below)
+    rc = net_apply(batch, ln.gl, ln.lv[0].y[1], N, 1, nullptr); if (rc) return rc;
     int64_t po = 0;
-    CorrPlan pin1, pin2, pdn1[LN_MAX_LEVELS], pdn2[LN_MAX_LEVELS], pq[LN_MAX_LEVELS];
-    rc = launch_corr(corr_args(ln.Hh[0], 0, nullptr, ln.rb, 0, 16, CM_SAME, batch), po, pin1); if (rc) return rc;
-    rc = launch_corr(corr_args(ln.D[0], 0, nullptr, ln.gl, 0, 16, CM_SAME, batch), po, pin2); if (rc) return rc;
-    for (int l = 1; l < L; ++l) {
-        rc = launch_corr(corr_args(ln.Hh[l], l, ln.E[l - 1], nullptr, l - 1, ln.C[l - 1], CM_DOWN, batch), po, pdn1[l]);
-        if (rc) return rc;
-        rc = launch_corr(corr_args(ln.D[l], l, ln.G[l - 1], nullptr, l - 1, ln.C[l - 1], CM_DOWN, batch), po, pdn2[l]);
-        if (rc) return rc;
-    }
-    for (int l = 0; l < L; ++l) {
-        rc = launch_corr(corr_args(ln.G[l], l, ln.E[l], nullptr, l, ln.C[l], CM_CENTER, batch), po, pq[l]);
-        if (rc) return rc;
-    }
-    // reductions (fixed order)
-    auto reduce = [&](int64_t size, CorrPlan a, CorrPlan b, int nreg, float* dst) {
-        GradRegion r0{a.off, a.nblk}, r1{b.off, b.nblk};
-        hipLaunchKernelGGL(k_reduce_grad, dim3((unsigned)((size + 255) / 256)), dim3(256), 0, g.stream, size, ln.d_part,
-                           r0, r1, nreg, dst);
-    };
-    reduce(9LL * ln.C[0] * 16, pin1, pin2, 2, ln.d_g + ln.off_in);
-    for (int l = 1; l < L; ++l) reduce(9LL * ln.C[l] * ln.C[l - 1], pdn1[l], pdn2[l], 2, ln.d_g + ln.off_dn[l]);
-    for (int l = 0; l < L; ++l) {
-        int C = ln.C[l];
-        reduce((int64_t)C * C, pq[l], pq[l], 1, ln.d_dS + ln.off_q[l]);
-        hipLaunchKernelGGL(k_qgrad, dim3((C * C + 255) / 256), dim3(256), 0, g.stream, C, ln.d_dS + ln.off_q[l],
-                           ln.d_p + ln.off_q[l], ln.d_g + ln.off_q[l]);
+    for (int l = 0; l < ln.Ls; ++l) {
+        const MgLevel& v = ln.lv[l];
+        const float* r_r = l == 0 ? ln.rb : v.r[0];
+        const float* r_g = l == 0 ? ln.gl : v.r[1];
+        const int64_t rbs = l == 0 ? N : v.n;
+        // pairings (u, v): (s2(g), r(r)) and (r(g), s2(r))
+        const float* us[2] = {v.s2[1], r_g};
+        const int64_t ubs[2] = {v.n, rbs};
+        const float* vs[2] = {r_r, v.s2[0]};
+        const int64_t vbs[2] = {rbs, v.n};
+        CorrPlan pw[4], ps[2];
+        for (int k = 0; k < 2; ++k) {
+            rc = conv_lift(l, batch, us[k], ubs[k], ln.pin[l], ln.U[l], nullptr); if (rc) return rc;
+            rc = conv_lift(l, batch, vs[k], vbs[k], ln.pin[l], ln.V[l], nullptr); if (rc) return rc;
+            rc = conv_lift(l, batch, us[k], ubs[k], ln.pSin[l], ln.SU[l], nullptr); if (rc) return rc;
+            rc = conv_lift(l, batch, vs[k], vbs[k], ln.pSin[l], ln.SV[l], nullptr); if (rc) return rc;
+            rc = launch_corr(corr_center(l, ln.U[l], ln.V[l], batch), po, ps[k]); if (rc) return rc;
+            rc = launch_corr(corr_in(l, ln.SV[l], us[k], ubs[k], batch), po, pw[2 * k]); if (rc) return rc;
+            rc = launch_corr(corr_in(l, ln.SU[l], vs[k], vbs[k], batch), po, pw[2 * k + 1]); if (rc) return rc;
+        }
+        GradRegions rw, rs;
+        rw.n = 4; rs.n = 2;
+        for (int k = 0; k < 4; ++k) { rw.off[k] = pw[k].off; rw.nblk[k] = pw[k].nblk; }
+        for (int k = 0; k < 2; ++k) { rs.off[k] = ps[k].off; rs.nblk[k] = ps[k].nblk; }
+        const int64_t nw = 9LL * ln.C[l] * 16, ns = (int64_t)ln.C[l] * ln.C[l];
+        hipLaunchKernelGGL(k_reduce_grad, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, g.stream, nw, ln.d_part, rw,
+                           ln.d_g + ln.off_in[l]);
+        hipLaunchKernelGGL(k_reduce_grad, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, g.stream, ns, ln.d_part, rs,
+                           ln.d_dS + ln.off_q[l]);
+        hipLaunchKernelGGL(k_qgrad, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, g.stream, ln.C[l],
+                           ln.d_dS + ln.off_q[l], ln.d_p + ln.off_q[l], ln.d_g + ln.off_q[l]);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(loss, ln.d_lsum, sizeof(double), hipMemcpyDeviceToHost, g.stream));
@@ -882,30 +1248,53 @@ static int loss_and_gradient(int batch, double* loss) {
     return PIC_OK;
 }
 
-static int gather_batch(int batch, const int* idx) {
+// batch entries [0, nh) from the ring (indices idx), [nh, batch) synthetic rough pairs
+static int gather_batch(int batch, int nh, const int* idx) {
     const size_t N = (size_t)ln_npix0();
-    for (int b = 0; b < batch; ++b) {
+    const int n = (int)N, nbk = (n + 255) / 256;
+    for (int b = 0; b < nh; ++b) {
         HIPCHK(hipMemcpyAsync(ln.rb + b * N, ln.ds_r + (size_t)idx[b] * N, N * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
         HIPCHK(hipMemcpyAsync(ln.eb + b * N, ln.ds_e + (size_t)idx[b] * N, N * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
     }
-    HIPCHK(hipMemcpyAsync(ln.d_idx, idx, batch * sizeof(int), hipMemcpyHostToDevice, g.stream));
+    if (nh > 0) {
+        HIPCHK(hipMemcpyAsync(ln.d_idx, idx, nh * sizeof(int), hipMemcpyHostToDevice, g.stream));
+        hipLaunchKernelGGL(k_gather_ea, dim3(1), dim3(64), 0, g.stream, nh, ln.d_idx, ln.ds_ea, ln.d_eab);
+    }
+    Dia8 A = scaled_op();
+    for (int b = nh; b < batch; ++b) {
+        const uint32_t step = ln.rough_step++;
+        float* e = ln.eb + b * N;
+        float* tmp = ln.rb + b * N;
+        hipLaunchKernelGGL(k_rough_e, dim3(nbk), dim3(256), 0, g.stream, n, (uint64_t)0x5EEDull, step, 0x10000u,
+                           g.d_sinv, e);
+        const int sweeps = (int)(step % 9);            // 0..8 smoothing sweeps: a spread of frequencies
+        for (int s = 0; s < sweeps; ++s) {
+            hipLaunchKernelGGL(k_smooth_e, dim3(nbk), dim3(256), 0, g.stream, n, A, e, tmp);
+            HIPCHK(hipMemcpyAsync(e, tmp, N * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
+        }
+        hipLaunchKernelGGL(k_pair_from_e, dim3(nbk), dim3(256), 0, g.stream, n, A, e, ln.rb + b * N, ln.d_hpart);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.stream, nbk, ln.d_hpart, ln.d_eab + b);
+    }
+    HIPCHK(hipGetLastError());
     return PIC_OK;
 }
 
 int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
     if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
     if (ln.ds_n < 1) return pic_fail(PIC_ERR_STATE, "learned network: empty data set");
+    if (ln.nparams == 0) return pic_fail(PIC_ERR_STATE, "learned network: no learned smoother levels");
     if (batch < 1 || batch > 64 || steps < 0) return pic_fail(PIC_ERR_ARG, "batch must be 1..64");
     int rc = ensure_batch(batch); if (rc) return rc;
     uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
     std::vector<int> idx(batch);
+    const int nh = batch - batch / 2;                // half of every minibatch is synthetic rough pairs
     const float b1 = 0.9f, b2 = 0.999f;
     for (int s = 0; s < steps; ++s) {
-        for (int b = 0; b < batch; ++b) {
+        for (int b = 0; b < nh; ++b) {
             st ^= st << 13; st ^= st >> 7; st ^= st << 17;
             idx[b] = (int)(st % (uint64_t)ln.ds_n);
         }
-        rc = gather_batch(batch, idx.data()); if (rc) return rc;
+        rc = gather_batch(batch, nh, idx.data()); if (rc) return rc;
         double loss = 0.0;
         rc = loss_and_gradient(batch, &loss); if (rc) return rc;
         loss /= batch;
@@ -932,10 +1321,10 @@ int learn_eval(int npairs, double* mean_loss) {
     const int nb = (n + 255) / 256;
     double tot = 0.0;
     for (int k = 0; k < npairs; ++k) {
-        rc = gather_batch(1, &k); if (rc) return rc;
-        rc = net_forward(1, ln.rb, n, ln.y, n, ln.E, ln.D, nullptr); if (rc) return rc;
-        hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, g.stream, n, 1, scaled_op(), g.d_sinv, ln.y, ln.eb,
-                           ln.ds_ea, ln.d_idx, ln.gl, ln.d_lpart);
+        rc = gather_batch(1, 1, &k); if (rc) return rc;
+        rc = net_apply(1, ln.rb, ln.y0, n, 0, nullptr); if (rc) return rc;
+        hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, g.stream, n, 1, scaled_op(), g.d_sinv, ln.y0, ln.eb,
+                           ln.d_eab, ln.gl, ln.d_lpart);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.stream, nb, ln.d_lpart, ln.d_lsum);
         double l = 0.0;
         HIPCHK(hipMemcpyAsync(&l, ln.d_lsum, sizeof(double), hipMemcpyDeviceToHost, g.stream));
@@ -973,7 +1362,7 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
         HIPCHK(hipEventRecord(g.ev[12], g.stream));
         for (int c = 0; c < CH; ++c, ++k) {
             hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, k, scal, part, ps);
-            int rc = net_forward(1, r, n, ln.z, n, ln.E, ln.D, scal + F_S_DONE); if (rc) return rc;
+            int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
             hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
             hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, k, ln.z, ln.w, p, ln.q, x, r,
                                scal, part, ps);
@@ -985,6 +1374,9 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, g.ev[12], g.ev[13]));
         g.kstat[4] += ms; g.kstat[5] += CH;
+        if (ln.trace)
+            fprintf(stderr, "fcg k=%d rr/rr0=%.3e done=%g\n", k, g.h_scal_pinned[F_S_NORMR] / g.h_scal_pinned[F_S_NORMRHS],
+                    g.h_scal_pinned[F_S_DONE]);
         if (g.h_scal_pinned[F_S_DONE] != 0.0) break;
         double rr = g.h_scal_pinned[F_S_NORMR];
         // stagnation: less than a 2x residual-norm reduction over a chunk, three times
@@ -1008,8 +1400,8 @@ int learn_apply_host(const float* r, float* z, int64_t n) {
     if (n != ln_npix0()) return pic_fail(PIC_ERR_ARG, "n must equal the node count");
     int rc = ensure_batch(1); if (rc) return rc;
     HIPCHK(hipMemcpyAsync(ln.rb, r, n * sizeof(float), hipMemcpyHostToDevice, g.stream));
-    rc = net_forward(1, ln.rb, n, ln.y, n, ln.E, ln.D, nullptr); if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(z, ln.y, n * sizeof(float), hipMemcpyDeviceToHost, g.stream));
+    rc = net_apply(1, ln.rb, ln.y0, n, 0, nullptr); if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(z, ln.y0, n * sizeof(float), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return PIC_OK;
 }
@@ -1032,13 +1424,12 @@ int learn_params(float* dst, const float* src, int64_t n, int64_t* nparams) {
 
 int learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets) {
     if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
-    *levels = ln.L;
-    for (int l = 0; l < ln.L; ++l) { H[l] = ln.H[l]; W[l] = ln.W[l]; C[l] = ln.C[l]; }
-    // offsets: [0] In, [1..L-1] Down_l, [L..2L-1] Q_l, [2L] total
-    offsets[0] = ln.off_in;
-    for (int l = 1; l < ln.L; ++l) offsets[l] = ln.off_dn[l];
-    for (int l = 0; l < ln.L; ++l) offsets[ln.L + l] = ln.off_q[l];
-    offsets[2 * ln.L] = ln.nparams;
+    // levels[0] = L, levels[1] = learned smoother levels Ls; H, W per level; C per learned level;
+    // offsets: [0..Ls) W_in, [Ls..2Ls) Q, [2Ls] total
+    levels[0] = ln.L; levels[1] = ln.Ls;
+    for (int l = 0; l < ln.L; ++l) { H[l] = ln.lv[l].H; W[l] = ln.lv[l].W; }
+    for (int l = 0; l < ln.Ls; ++l) { C[l] = ln.C[l]; offsets[l] = ln.off_in[l]; offsets[ln.Ls + l] = ln.off_q[l]; }
+    offsets[2 * ln.Ls] = ln.nparams;
     return PIC_OK;
 }
 
@@ -1075,10 +1466,11 @@ int learn_get_pair(int k, float* r, float* e, double* ea) {
 int learn_gradient(int np, const int32_t* idx, double* loss, float* grad, int64_t n) {
     if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
     if (np < 1 || np > 64) return pic_fail(PIC_ERR_ARG, "1..64 pairs");
+    if (ln.nparams == 0) return pic_fail(PIC_ERR_STATE, "learned network: no learned smoother levels");
     if (n < ln.nparams) return pic_fail(PIC_ERR_ARG, "gradient buffer too small");
     for (int k = 0; k < np; ++k) if (idx[k] < 0 || idx[k] >= ln.ds_n) return pic_fail(PIC_ERR_ARG, "pair index out of range");
     int rc = ensure_batch(np); if (rc) return rc;
-    rc = gather_batch(np, idx); if (rc) return rc;
+    rc = gather_batch(np, np, idx); if (rc) return rc;
     rc = loss_and_gradient(np, loss); if (rc) return rc;
     HIPCHK(hipMemcpy(grad, ln.d_g, ln.nparams * sizeof(float), hipMemcpyDeviceToHost));
     return PIC_OK;

@@ -1,6 +1,7 @@
-"""CPU checks of the learned-preconditioner oracle (oracle/learned.py): adjointness of the
-transposed convolutions, symmetry and semi-definiteness of M, and the analytic gradient
-against central differences of the FP64 loss."""
+"""CPU checks of the learned-multigrid oracle (oracle/learned.py): adjointness of the
+transposed convolution, the Galerkin hierarchy, symmetry and definiteness of the V-cycle,
+the untrained network equal to the classical Jacobi V-cycle, preconditioning quality, and
+the analytic gradient against central differences of the FP64 loss."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -8,121 +9,163 @@ import scipy.sparse as sp
 import learned as lrn
 
 
-def q1_scaled_operator(n):
-    """Jacobi-scaled Q1 stiffness (8/3 centre, -1/3 neighbours) on an n x n node grid with
-    Dirichlet boundary nodes: A' on node vectors, mask of the unknown nodes."""
+def q1_operator(n, holes=True):
+    """Jacobi-scaled Q1 stiffness (unit centre, -1/8 neighbours) on an n x n node grid,
+    Dirichlet walls and (optionally) three Dirichlet discs; returns A' and the unknown mask."""
     N = n * n
-    mask = np.zeros(N)
-    rows, cols, vals = [], [], []
-    idx = lambda i, j: i * n + j
+    xs = np.arange(n) / (n - 1)
+    circ = [(0.25, 0.5, 0.08), (0.5, 0.5, 0.08), (0.75, 0.5, 0.08)] if holes else []
+    unk = np.zeros((n, n), bool)
     for i in range(1, n - 1):
         for j in range(1, n - 1):
-            mask[idx(i, j)] = 1
+            unk[i, j] = not any((xs[i] - cx) ** 2 + (xs[j] - cy) ** 2 <= r * r for cx, cy, r in circ)
+    rows, cols, vals = [], [], []
+    for i in range(1, n - 1):
+        for j in range(1, n - 1):
+            if not unk[i, j]:
+                continue
             for di in (-1, 0, 1):
                 for dj in (-1, 0, 1):
-                    ii, jj = i + di, j + dj
-                    if 1 <= ii < n - 1 and 1 <= jj < n - 1:
-                        rows.append(idx(i, j)); cols.append(idx(ii, jj))
+                    if unk[i + di, j + dj]:
+                        rows.append(i * n + j); cols.append((i + di) * n + j + dj)
                         vals.append(1.0 if (di == 0 and dj == 0) else -1.0 / 8.0)
-    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N)), mask
+    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N)), unk.reshape(-1).astype(float)
 
 
-def random_params(lay, rng, scale=0.3):
-    p = rng.standard_normal(lay["nparams"]) * scale
-    # only input channel 0 of In exists
-    w = p[lay["off_in"]: lay["off_in"] + 9 * lay["C"][0] * 16].reshape(9, lay["C"][0], 16)
-    w[:, :, 1:] = 0.0
-    return p
+def pcg_iterations(A, b, Mf, tol=1e-8, maxit=2000):
+    x = np.zeros_like(b); r = b.copy(); z = Mf(r); p = z.copy(); rz = r @ z; nb = np.linalg.norm(b)
+    for k in range(maxit):
+        q = A @ p; a = rz / (p @ q); x += a * p; r -= a * q
+        if np.linalg.norm(r) / nb < tol:
+            return k + 1
+        z = Mf(r); rz2 = r @ z; p = z + rz2 / rz * p; rz = rz2
+    return maxit
 
 
-def test_transposed_convolutions_are_adjoints():
+def test_transposed_convolution_is_adjoint():
     rng = np.random.default_rng(0)
-    for H, W in ((9, 9), (10, 7)):
-        Ho, Wo = (H + 1) // 2, (W + 1) // 2
-        x = rng.standard_normal((H, W, 16))
-        y = rng.standard_normal((Ho, Wo, 32))
-        w = rng.standard_normal((9, 32, 16))
-        lhs = np.sum(lrn.conv_down(x, w, Ho, Wo) * y)
-        rhs = np.sum(x * lrn.conv_up(y, w, H, W))
-        assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
-        z = rng.standard_normal((H, W, 32))
-        lhs = np.sum(lrn.conv_same(x, w) * z)
-        rhs = np.sum(x * lrn.conv_same(z, np.transpose(w, (0, 2, 1)), sign=-1))
-        assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    x = rng.standard_normal((9, 7, 16))
+    z = rng.standard_normal((9, 7, 32))
+    w = rng.standard_normal((9, 32, 16))
+    lhs = np.sum(lrn.conv_same(x, w) * z)
+    rhs = np.sum(x * lrn.conv_same(z, np.transpose(w, (0, 2, 1)), sign=-1))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
 
 
-def test_network_is_symmetric_positive_semidefinite():
-    rng = np.random.default_rng(1)
-    n = 17
-    lay = lrn.layout(n, n, [16, 16, 32], max_levels=4)
-    p = random_params(lay, rng)
-    _, mask = q1_scaled_operator(n)
-    u = rng.standard_normal(n * n) * mask
-    v = rng.standard_normal(n * n) * mask
-    Mu = lrn.forward(p, lay, u, mask)[0]
-    Mv = lrn.forward(p, lay, v, mask)[0]
-    assert abs(u @ Mv - v @ Mu) <= 1e-10 * (abs(u @ Mv) + 1e-30)
-    assert v @ Mv >= -1e-12 * np.linalg.norm(v) ** 2
-
-
-def test_level_sizes_follow_vertex_coarsening():
+def test_level_sizes_and_galerkin_hierarchy():
     H, W = lrn.level_sizes(513, 513)
     assert H == [513, 257, 129, 65, 33, 17, 9, 5, 3]
-    H, W = lrn.level_sizes(65, 65, 10)
-    assert H == [65, 33, 17, 9, 5, 3]
+    A, mask = q1_operator(17)
+    lev = lrn.build_hierarchy(A, 17, 17)
+    assert [l["H"] for l in lev] == [17, 9, 5, 3]
+    for l in range(len(lev) - 1):
+        Ac = lev[l + 1]["A"].toarray()
+        assert np.allclose(Ac, Ac.T)                      # Galerkin operators stay symmetric
+        P = lev[l]["P"]
+        # bilinear interpolation reproduces constants away from Dirichlet nodes
+        ones_c = np.ones(P.shape[1])
+        f = P @ ones_c
+        inner = (lev[l]["mask"] > 0)
+        assert np.all(f[inner] <= 1.0 + 1e-12)
+    # exact coarsest solve
+    last = lev[-1]
+    idx = np.nonzero(last["mask"])[0]
+    Ad = last["A"].toarray()[np.ix_(idx, idx)]
+    assert np.allclose(last["inv"][np.ix_(idx, idx)] @ Ad, np.eye(len(idx)))
+
+
+def test_untrained_network_is_the_jacobi_vcycle_and_preconditions():
+    n = 65
+    A, mask = q1_operator(n)
+    lev = lrn.build_hierarchy(A, n, n)
+    lay = lrn.layout(n, n, [16], learned_levels=2)
+    p = lrn.initial_params(lay, small=0.0)
+    lay0 = lrn.layout(n, n, [16], learned_levels=0)
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(n * n) * mask
+    y1, _ = lrn.forward(p, lay, lev, r)
+    y0, _ = lrn.forward(np.zeros(0), lay0, lev, r)
+    assert np.allclose(y1, y0, rtol=1e-12, atol=1e-14)
+    it_mg = pcg_iterations(A, r, lambda v: lrn.forward(p, lay, lev, v)[0])
+    it_j = pcg_iterations(A, r, lambda v: v)
+    assert it_mg <= 12 and it_j >= 8 * it_mg, (it_mg, it_j)
+
+
+def test_vcycle_is_symmetric_positive_definite():
+    rng = np.random.default_rng(1)
+    n = 33
+    A, mask = q1_operator(n)
+    lev = lrn.build_hierarchy(A, n, n)
+    lay = lrn.layout(n, n, [16, 32], learned_levels=2)
+    p = lrn.initial_params(lay, small=0.05)
+    u = rng.standard_normal(n * n) * mask
+    v = rng.standard_normal(n * n) * mask
+    Mu = lrn.forward(p, lay, lev, u)[0]
+    Mv = lrn.forward(p, lay, lev, v)[0]
+    assert abs(u @ Mv - v @ Mu) <= 1e-10 * abs(u @ Mv)
+    assert v @ Mv > 0
+
+
+def _fd(p, lay, lev, pairs, d, eps=1e-6):
+    return (lrn.loss_only(p + eps * d, lay, lev, pairs) - lrn.loss_only(p - eps * d, lay, lev, pairs)) / (2 * eps)
 
 
 def test_gradient_matches_central_differences():
     rng = np.random.default_rng(2)
-    n = 13
-    lay = lrn.layout(n, n, [16, 16, 32], max_levels=3)
-    A, mask = q1_scaled_operator(n)
-    p = random_params(lay, rng)
+    n = 17
+    A, mask = q1_operator(n)
+    lev = lrn.build_hierarchy(A, n, n)
+    lay = lrn.layout(n, n, [16, 32], learned_levels=2)
+    p = lrn.initial_params(lay, small=0.2)
     pairs = []
     for _ in range(2):
         e = rng.standard_normal(n * n) * mask
         pairs.append((A @ e, e))
-    loss, grad = lrn.loss_and_grad(p, lay, pairs, A, mask)
-    assert np.isclose(loss, lrn.loss_only(p, lay, pairs, A, mask), rtol=1e-12)
-    eps = 1e-6
-    # one coordinate in every tensor, plus random directions
-    coords = [lay["off_in"] + (4 * lay["C"][0] + 3) * 16]
-    coords += [lay["off_down"][l] + 5 for l in range(1, lay["levels"])]
-    coords += [lay["off_q"][l] + 7 for l in range(lay["levels"])]
+    loss, grad = lrn.loss_and_grad(p, lay, lev, pairs)
+    assert np.isclose(loss, lrn.loss_only(p, lay, lev, pairs), rtol=1e-12)
+    coords = []
+    for l in range(lay["learned"]):
+        C = lay["C"][l]
+        coords += [lay["off_in"][l] + (4 * C + 0) * 16, lay["off_in"][l] + (2 * C + 3) * 16,
+                   lay["off_q"][l], lay["off_q"][l] + C + 2]
     for c in coords:
         d = np.zeros_like(p); d[c] = 1.0
-        fd = (lrn.loss_only(p + eps * d, lay, pairs, A, mask) - lrn.loss_only(p - eps * d, lay, pairs, A, mask)) / (2 * eps)
-        assert abs(fd - grad[c]) <= 1e-5 * max(1.0, abs(fd)), (c, fd, grad[c])
+        fd = _fd(p, lay, lev, pairs, d)
+        assert abs(fd - grad[c]) <= 1e-5 * max(1e-3, abs(fd)), (c, fd, grad[c])
     for _ in range(3):
-        d = random_params(lay, rng, 1.0)
-        fd = (lrn.loss_only(p + eps * d, lay, pairs, A, mask) - lrn.loss_only(p - eps * d, lay, pairs, A, mask)) / (2 * eps)
+        d = rng.standard_normal(p.size)
+        for l in range(lay["learned"]):
+            C = lay["C"][l]
+            d[lay["off_in"][l]: lay["off_in"][l] + 9 * C * 16].reshape(9, C, 16)[:, :, 1:] = 0.0
+        fd = _fd(p, lay, lev, pairs, d)
         assert abs(fd - grad @ d) <= 1e-6 * max(1.0, abs(fd))
 
 
-def test_planted_sign_error_in_gradient_is_caught():
-    """The central-difference check detects a dropped transposed-use term (the kind of
-    mistake the adjoint derivation invites)."""
+def test_planted_mistake_in_gradient_is_caught():
+    """Dropping one of the two site pairings (the post-smoother's) breaks the
+    central-difference agreement: the check can see that mistake."""
     rng = np.random.default_rng(3)
-    n = 9
-    lay = lrn.layout(n, n, [16], max_levels=2)
-    A, mask = q1_scaled_operator(n)
-    p = random_params(lay, rng)
+    n = 17
+    A, mask = q1_operator(n)
+    lev = lrn.build_hierarchy(A, n, n)
+    lay = lrn.layout(n, n, [16], learned_levels=1)
+    p = lrn.initial_params(lay, small=0.2)
     e = rng.standard_normal(n * n) * mask
     pairs = [(A @ e, e)]
-    _, grad = lrn.loss_and_grad(p, lay, pairs, A, mask)
-    # recompute dIn without the corr(D_0, g) term
-    w_in, w_dn, q = lrn.unpack(p, lay)
-    y, E, D = lrn.forward(p, lay, pairs[0][0], mask)
+    _, grad = lrn.loss_and_grad(p, lay, lev, pairs)
+    net = lrn._Net(p, lay, lev)
+    sr, sg = {}, {}
+    y = net.vcycle(0, pairs[0][0], sr)
     ea = e @ (A @ e)
     g = 2 * mask * (A @ (y - e)) / ea
-    S = [qq @ qq.T for qq in q]
-    G = lrn._encode(g, w_in, w_dn, lay, lambda a: a)
-    Hh = lrn._decode(G, w_dn, S, lay, lambda a: a)
-    rin = np.zeros((n, n, 16)); rin[:, :, 0] = pairs[0][0].reshape(n, n)
-    wrong = lrn.corr_same(Hh[0], rin).reshape(-1)
-    c = lay["off_in"] + (4 * lay["C"][0] + 0) * 16
+    net.vcycle(0, g, sg)
+    u, v = sg[0][1], sr[0][0]                   # only the first pairing
+    uu = lrn._as_chan(lev[0]["dh"] * u, n, n); vv = lrn._as_chan(lev[0]["dh"] * v, n, n)
+    U = lrn.conv_same(uu, net.w_in[0]); V = lrn.conv_same(vv, net.w_in[0])
+    dS = np.einsum("pqo,pqi->oi", U, V)
+    wrong = ((dS + dS.T) @ net.q[0]).reshape(-1)
+    c = lay["off_q"][0]
     d = np.zeros_like(p); d[c] = 1.0
-    eps = 1e-6
-    fd = (lrn.loss_only(p + eps * d, lay, pairs, A, mask) - lrn.loss_only(p - eps * d, lay, pairs, A, mask)) / (2 * eps)
-    assert abs(fd - grad[c]) <= 1e-5 * max(1.0, abs(fd))
-    assert abs(fd - wrong[c - lay["off_in"]]) > 1e-3 * max(1.0, abs(fd))
+    fd = _fd(p, lay, lev, pairs, d)
+    assert abs(fd - grad[c]) <= 1e-5 * max(1e-3, abs(fd))
+    assert abs(fd - wrong[0]) > 1e-2 * abs(fd)
