@@ -95,11 +95,13 @@ def cpu_baseline(pkg, pl, w, sample_particles: int) -> dict:
 
 def solve_compare(pl, reps: int) -> dict:
     """Plain (mode 1) vs learned (mode 2) inner solver on the same RHS and warm start: the
-    charge density of the current state and the potential of the previous step."""
+    charge density deposited in one step and the potential of the step before it (what the
+    solve inside a step sees)."""
     import time as _t
-    dom = pl.dom
-    rho = pl.field("RHO")
+    pl.set_solver(1)
     phi0 = pl.field("PHI")
+    pl.step(1)
+    rho = pl.field("RHO")
     res = {}
     for name, mode in (("plain", 1), ("learned", 2)):
         pl.set_solver(mode)

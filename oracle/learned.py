@@ -139,22 +139,24 @@ def _as_chan(v, H, W):
 
 # ------------------------------------------------------------------ network
 class _Net:
+    """The V-cycle in FP64 (the library applies the smoother as the collapsed FP32 stencil,
+    which equals the factored form up to rounding). BF16 emulation, when asked for, applies
+    to the factored lifts of the gradient, where the library uses BF16 MFMA."""
     def __init__(self, params, lay, levels, emulate_bf16=False):
         self.lay, self.levels = lay, levels
         self.rnd = bf16 if emulate_bf16 else (lambda a: a)
         w_in, q = unpack(params, lay)
         self.w_in, self.q = w_in, q
         self.S = [qq @ qq.T for qq in q]
-        rnd = self.rnd
-        self.SW = [rnd(np.einsum("oc,tci->toi", S, w)) for S, w in zip(self.S, w_in)]   # S In, [9][C][16]
-        self.WT = [rnd(np.transpose(w, (0, 2, 1))) for w in w_in]                          # In^T, [9][16][C]
+        self.SW = [np.einsum("oc,tci->toi", S, w) for S, w in zip(self.S, w_in)]   # S In, [9][C][16]
+        self.WT = [np.transpose(w, (0, 2, 1)) for w in w_in]                          # In^T, [9][16][C]
 
     def smooth(self, l, v):
         lev = self.levels[l]
         if l < self.lay["learned"]:
             H, W = lev["H"], lev["W"]
-            u = _as_chan(self.rnd(lev["dh"] * v), H, W)
-            E = self.rnd(conv_same(u, self.SW[l]))
+            u = _as_chan(lev["dh"] * v, H, W)
+            E = conv_same(u, self.SW[l])
             return conv_same(E, self.WT[l], sign=-1)[:, :, 0].reshape(-1) * lev["dh"]
         return OMEGA * lev["dinv"] * v
 
@@ -172,9 +174,32 @@ class _Net:
         return x2 + self.smooth(l, s2)
 
 
-def forward(params, lay, levels, r, emulate_bf16: bool = False):
+def collapsed_pairs(w_in, q):
+    """The learned smoother In^T S In as 81 tap-pair weights k81[t][t2] (the library's form)."""
+    S = q @ q.T
+    W = w_in[:, :, 0]
+    return W @ S @ W.T
+
+
+def apply_collapsed(k81, u):
+    """y(p) = sum_t [p - d_t on the grid] sum_t2 k81[t][t2] u(p - d_t + d_t2) on a 2-D field."""
+    H, W = u.shape
+    up = np.zeros((H + 4, W + 4)); up[2:-2, 2:-2] = u
+    y = np.zeros((H, W))
+    for t, (ti, tj) in enumerate(TAPS):
+        valid = np.zeros((H, W), bool)
+        valid[max(0, ti):H + min(0, ti), max(0, tj):W + min(0, tj)] = True   # p - d_t on the grid
+        acc = np.zeros((H, W))
+        for t2, (si, sj) in enumerate(TAPS):
+            oi, oj = 2 - ti + si, 2 - tj + sj
+            acc += k81[t, t2] * up[oi:oi + H, oj:oj + W]
+        y += np.where(valid, acc, 0.0)
+    return y
+
+
+def forward(params, lay, levels, r):
     """y = M r on a flat node vector; returns y and the smoother inputs per learned level."""
-    net = _Net(params, lay, levels, emulate_bf16)
+    net = _Net(params, lay, levels)
     sites = {}
     y = net.vcycle(0, np.asarray(r, np.float64), sites)
     return y, sites
@@ -209,8 +234,8 @@ def loss_and_grad(params, lay, levels, pairs, emulate_bf16: bool = False):
                 vv = _as_chan(net.rnd(dh * v), H, W)
                 U = net.rnd(conv_same(uu, net.rnd(net.w_in[l])))
                 V = net.rnd(conv_same(vv, net.rnd(net.w_in[l])))
-                SU = net.rnd(conv_same(uu, net.SW[l]))
-                SV = net.rnd(conv_same(vv, net.SW[l]))
+                SU = net.rnd(conv_same(uu, net.rnd(net.SW[l])))
+                SV = net.rnd(conv_same(vv, net.rnd(net.SW[l])))
                 dS += np.einsum("pqo,pqi->oi", U, V)
                 dW += corr_same(SV, uu) + corr_same(SU, vv)
             dQ = (dS + dS.T) @ net.q[l]

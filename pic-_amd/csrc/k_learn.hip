@@ -27,9 +27,13 @@
 //   dS += sum_p U V^T,  dIn += corr(S V, Dh u) + corr(S U, Dh v),  dQ = (dS + dS^T) Q.
 // corr(o, x)[t][co][ci] = sum_p o[co](p) x[ci](p + d_t).
 //
-// Activations of the learned smoothers are NHWC bf16 (pixel-major, C contiguous, C a
-// multiple of 16), one pixel per node. MFMA 16x16x16 bf16: a wave computes 16 pixels x 16
-// output channels per instruction with K = 16 input channels of one tap.
+// The smoother is linear, so In^T S In collapses to 81 tap-pair weights (a symmetric 5x5
+// stencil away from the grid edge), recomputed from the FP32 master weights after every
+// update; the V-cycle (solve and training forward) applies it in FP32. The factored network runs on MFMA where the factors matter, in the
+// gradient: the lifts U, V, S U, S V are NHWC bf16 (pixel-major, C contiguous, C a multiple
+// of 16, one pixel per node) and the weight correlations are MFMA reductions over pixels.
+// MFMA 16x16x16 bf16: a wave computes 16 pixels x 16 channels per instruction with K = 16
+// input channels of one tap (lifts) or K = 16 pixels (correlations).
 #include "pic_device.h"
 #include <algorithm>
 #include <cmath>
@@ -300,6 +304,41 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, cons
     }
 }
 
+// Learned smoother, collapsed: out = [out +] Dh (In^T S In) (Dh v) on one level with the
+// 81 tap-pair weights k81 (zero outside the grid, as the factored form).
+__global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k81, const float* dh, const float* v,
+                                                    int64_t v_bs, float* out, int64_t o_bs, int add, const double* done) {
+    if (done && *done != 0.0) return;
+    const int n = H * W;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float* vb = v + (int64_t)blockIdx.y * v_bs;
+    float* ob = out + (int64_t)blockIdx.y * o_bs;
+    const float dp = dh[p];
+    float s = 0.0f;
+    if (dp != 0.0f) {
+        const int i = p / W, j = p - (p / W) * W;
+        float u[5][5];
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const int ii = i + a - 2, jj = j + b - 2;
+                u[a][b] = (ii >= 0 && ii < H && jj >= 0 && jj < W) ? dh[ii * W + jj] * vb[ii * W + jj] : 0.0f;
+            }
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);   // intermediate pixel p - d_t
+            if (qi < 0 || qi >= H || qj < 0 || qj >= W) continue;
+#pragma unroll
+            for (int t2 = 0; t2 < 9; ++t2)
+                s = s + k81[9 * t + t2] * u[2 - (t / 3 - 1) + (t2 / 3 - 1)][2 - (t % 3 - 1) + (t2 % 3 - 1)];
+        }
+        s = dp * s;
+    }
+    ob[p] = add ? ob[p] + s : s;
+}
+
 // ---------------------------------------------------------------- weight-gradient correlation (MFMA)
 // part[blk][t][co][ci] = sum_{b, p in block} og[b][p][co] * in[b][src(p, t)][ci]
 struct CorrArgs {
@@ -424,14 +463,33 @@ __global__ void k_adam(int64_t n, float* p, const float* gr, float* m, float* v,
     p[k] = p[k] - lr * (mk / c1) / (sqrtf(vk / c2) + eps);
 }
 
-// Packed bf16 weights of one learned smoother from W_in [9][C][16] (input channel 0) and
-// Q [C][C]: pin = W_in, pinT = W_in^T per tap [9][16][C], pSin = (Q Q^T) W_in per tap.
-__global__ void k_pack_smoother(int C, const float* W, const float* Q, bf16* pin, bf16* pinT, bf16* pSin) {
+// Collapsed smoother: In^T S In through the channels is, tap pair by tap pair,
+//   k81[t][t2] = sum_{c, c2} W[t][c] S[c][c2] W[t2][c2]      (FP32 from the master weights)
+// applied as y(p) = sum_t [p - d_t on the grid] sum_t2 k81[t][t2] u(p - d_t + d_t2), which is
+// exactly the factored form (its intermediate channels exist on the grid only). One block.
+__global__ void k_pack_k81(int C, const float* W, const float* Q, float* k81) {
+    const int d = threadIdx.x;
+    if (d >= 81) return;
+    const int t = d / 9, t2 = d - t * 9;
+    float acc = 0.0f;
+    for (int c = 0; c < C; ++c) {
+        float wc = W[((int64_t)t * C + c) * 16];
+        for (int c2 = 0; c2 < C; ++c2) {
+            float sc = 0.0f;                             // S[c][c2]
+            for (int m = 0; m < C; ++m) sc += Q[c * C + m] * Q[c2 * C + m];
+            acc += wc * sc * W[((int64_t)t2 * C + c2) * 16];
+        }
+    }
+    k81[d] = acc;
+}
+
+// Packed bf16 weights of one learned smoother (the lifts of the gradient) from W_in
+// [9][C][16] (input channel 0) and Q [C][C]: pin = W_in, pSin = (Q Q^T) W_in per tap.
+__global__ void k_pack_smoother(int C, const float* W, const float* Q, bf16* pin, bf16* pSin) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 9 * C * 16) return;
     int t = k / (C * 16), r = k - t * C * 16, co = r / 16, ci = r - co * 16;
     pin[k] = f2bf(W[k]);
-    pinT[(int64_t)t * 16 * C + (int64_t)ci * C + co] = f2bf(W[k]);
     float s = 0.0f;
     for (int c = 0; c < C; ++c) {
         float sc = 0.0f;                                 // S[co][c] = sum_m Q[co][m] Q[c][m]
@@ -712,12 +770,12 @@ struct LearnNet {
     int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
     float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr, *d_dS = nullptr;
     bf16* d_pk = nullptr;
-    bf16 *pin[LN_MAX_LEARNED] = {nullptr}, *pinT[LN_MAX_LEARNED] = {nullptr}, *pSin[LN_MAX_LEARNED] = {nullptr};
+    bf16 *pin[LN_MAX_LEARNED] = {nullptr}, *pSin[LN_MAX_LEARNED] = {nullptr};
+    float* d_k81 = nullptr;                          // collapsed smoother tap-pair weights [Ls][81]
     int adam_t = 0;
     // activations for a batch of bcap samples
     int bcap = 0;
     void* d_act = nullptr;
-    bf16* E[2][LN_MAX_LEARNED] = {{nullptr}};        // smoother activations per run
     bf16 *U[LN_MAX_LEARNED] = {nullptr}, *V[LN_MAX_LEARNED] = {nullptr}, *SU[LN_MAX_LEARNED] = {nullptr},
          *SV[LN_MAX_LEARNED] = {nullptr};
     float *y0 = nullptr, *gl = nullptr, *rb = nullptr, *eb = nullptr;   // [bcap][N]
@@ -746,7 +804,7 @@ static LearnNet ln;
 static int64_t ln_npix0() { return ln.lv[0].n; }
 
 static void free_learn() {
-    void* ptrs[] = {ln.d_inv, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
+    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
                     ln.d_lpart, ln.d_lsum, ln.d_idx, ln.d_eab, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     ln = LearnNet();
@@ -776,17 +834,6 @@ static int conv_lift(int l, int batch, const float* u, int64_t u_bs, const bf16*
     return launch_conv(a, batch);
 }
 
-// project: y (+)= Dh In^T E
-static int conv_project(int l, int batch, const bf16* E, float* y, int64_t y_bs, bool add, const double* done) {
-    const MgLevel& lv = ln.lv[l];
-    ConvArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.in = E; a.Cin = ln.C[l]; a.Hin = lv.H; a.Win = lv.W; a.in_bs = lv.n * ln.C[l];
-    a.w = ln.pinT[l]; a.out_f32 = y; a.Cout = 16; a.Hout = lv.H; a.Wout = lv.W; a.out_bs = y_bs;
-    a.out_scale = lv.dh; a.add_f32 = add ? y : nullptr; a.mode = CM_SAME_T; a.done = done;
-    return launch_conv(a, batch);
-}
-
 static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, int64_t y_bs, const double* done) {
     const MgLevel& f = ln.lv[l];
     const MgLevel& c = ln.lv[l + 1];
@@ -800,16 +847,19 @@ static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, in
     return a;
 }
 
+static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, int64_t o_bs, bool add, const double* done) {
+    const MgLevel& lv = ln.lv[l];
+    hipLaunchKernelGGL(k_smooth_k81, dim3((unsigned)((lv.n + 255) / 256), batch), dim3(256), 0, g.stream, lv.H, lv.W,
+                       ln.d_k81 + 81 * l, lv.dh, v, v_bs, out, o_bs, add ? 1 : 0, done);
+}
+
 // y = M r for `batch` inputs (level-0 vectors with batch stride bs); run selects the work set.
 static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done) {
     const int L = ln.L;
     for (int l = 0; l < L - 1; ++l) {
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
-        if (l < ln.Ls) {
-            int rc = conv_lift(l, batch, r, r_bs, ln.pSin[l], ln.E[run][l], done); if (rc) return rc;
-            rc = conv_project(l, batch, ln.E[run][l], ln.lv[l].x[run], ln.lv[l].n, false, done); if (rc) return rc;
-        }
+        if (l < ln.Ls) smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done);
         MgArgs a = mg_args(l, run, r, r_bs, nullptr, 0, done);
         hipLaunchKernelGGL(k_mg_down, dim3((unsigned)((ln.lv[l + 1].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
     }
@@ -823,10 +873,7 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
         int64_t y_bs = l == 0 ? bs : ln.lv[l].n;
         MgArgs a = mg_args(l, run, r, r_bs, y, y_bs, done);
         hipLaunchKernelGGL(k_mg_up, dim3((unsigned)((ln.lv[l].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
-        if (l < ln.Ls) {
-            int rc = conv_lift(l, batch, ln.lv[l].s2[run], ln.lv[l].n, ln.pSin[l], ln.E[run][l], done); if (rc) return rc;
-            rc = conv_project(l, batch, ln.E[run][l], y, y_bs, true, done); if (rc) return rc;
-        }
+        if (l < ln.Ls) smooth5(l, batch, ln.lv[l].s2[run], ln.lv[l].n, y, y_bs, true, done);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
@@ -836,7 +883,9 @@ static int pack_weights() {
     for (int l = 0; l < ln.Ls; ++l) {
         int n = 9 * ln.C[l] * 16;
         hipLaunchKernelGGL(k_pack_smoother, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l],
-                           ln.d_p + ln.off_q[l], ln.pin[l], ln.pinT[l], ln.pSin[l]);
+                           ln.d_p + ln.off_q[l], ln.pin[l], ln.pSin[l]);
+        hipLaunchKernelGGL(k_pack_k81, dim3(1), dim3(128), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l], ln.d_p + ln.off_q[l],
+                           ln.d_k81 + 81 * l);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
@@ -848,7 +897,7 @@ static int ensure_batch(int batch) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     size_t bytes = 0;
     for (int l = 0; l < ln.L; ++l) bytes += 8 * al((size_t)ln.lv[l].n * sizeof(float) * batch);
-    for (int l = 0; l < ln.Ls; ++l) bytes += 6 * al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch);
+    for (int l = 0; l < ln.Ls; ++l) bytes += 4 * al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch);
     const size_t N = (size_t)ln_npix0();
     bytes += 4 * al(N * sizeof(float) * batch);
     HIPCHK(hipMalloc(&ln.d_act, bytes));
@@ -860,8 +909,8 @@ static int ensure_batch(int batch) {
         for (int k = 0; k < 8; ++k) { *vs[k] = (float*)p; p += al((size_t)v.n * sizeof(float) * batch); }
     }
     for (int l = 0; l < ln.Ls; ++l) {
-        bf16** bs[6] = {&ln.E[0][l], &ln.E[1][l], &ln.U[l], &ln.V[l], &ln.SU[l], &ln.SV[l]};
-        for (int k = 0; k < 6; ++k) { *bs[k] = (bf16*)p; p += al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch); }
+        bf16** bs[4] = {&ln.U[l], &ln.V[l], &ln.SU[l], &ln.SV[l]};
+        for (int k = 0; k < 4; ++k) { *bs[k] = (bf16*)p; p += al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch); }
     }
     float** fs[4] = {&ln.y0, &ln.gl, &ln.rb, &ln.eb};
     for (int s = 0; s < 4; ++s) { *fs[s] = (float*)p; p += al(N * sizeof(float) * batch); }
@@ -1057,12 +1106,13 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     HIPCHK(hipMemset(ln.d_m, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_v, 0, no * sizeof(float)));
     HIPCHK(hipMemset(ln.d_g, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_dS, 0, no * sizeof(float)));
     size_t nb = 1;
-    for (int l = 0; l < ln.Ls; ++l) nb += 3 * 9 * (size_t)ln.C[l] * 16;
+    for (int l = 0; l < ln.Ls; ++l) nb += 2 * 9 * (size_t)ln.C[l] * 16;
     HIPCHK(hipMalloc(&ln.d_pk, nb * sizeof(bf16)));
+    HIPCHK(hipMalloc(&ln.d_k81, 25 * LN_MAX_LEARNED * sizeof(float)));
     bf16* q = ln.d_pk;
     for (int l = 0; l < ln.Ls; ++l) {
         size_t s = 9 * (size_t)ln.C[l] * 16;
-        ln.pin[l] = q; q += s; ln.pinT[l] = q; q += s; ln.pSin[l] = q; q += s;
+        ln.pin[l] = q; q += s; ln.pSin[l] = q; q += s;
     }
     // gradient partials: per learned level 4 SAME correlations + 2 CENTER, <= 256 blocks each
     int64_t pc = 64;
@@ -1354,7 +1404,7 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
     hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, b64, g.d_sinv, r, x, scal, part, ps);
     HIPCHK(hipGetLastError());
     Dia8 A = scaled_op();
-    const int CH = 4;
+    const int CH = 3;
     int k = 0;
     double prev_rr = -1.0;
     int slow = 0;
@@ -1381,6 +1431,7 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
         double rr = g.h_scal_pinned[F_S_NORMR];
         // stagnation: less than a 2x residual-norm reduction over a chunk, three times
         if (prev_rr > 0.0 && !(rr < 0.25 * prev_rr)) slow++;
+        if (ln.trace) fprintf(stderr, "fcg chunk rr=%.3e slow=%d\n", rr, slow);
         prev_rr = rr;
         if (slow >= 3) { *stagnated = 1; g.h_scal_pinned[F_S_ITER] = k; g.h_scal_pinned[F_S_IERR] = 0.0; break; }
         if (k > maxit + CH) { g.h_scal_pinned[F_S_IERR] = 1.0; g.h_scal_pinned[F_S_ITER] = k; break; }

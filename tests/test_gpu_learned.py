@@ -1,6 +1,7 @@
 """GPU checks of the learned multigrid preconditioner: Galerkin hierarchy and network output
-against the numpy oracle (BF16 emulated; stated bound 1e-2 relative), the training gradient
-against the oracle (5e-2 relative per tensor), training on pairs harvested from the plain
+(FP32 V-cycle with the collapsed 5x5 smoother) against the FP64 numpy oracle (stated bound
+1e-4 relative), the training gradient (BF16 MFMA lifts and correlations) against the oracle
+with BF16 emulated (5e-2 relative per tensor), training on pairs harvested from the plain
 solver, the learned FCG against the plain CG at the same 1e-8 residual, and the fallback to
 the plain CG when the network is unusable."""
 import numpy as np
@@ -72,7 +73,7 @@ def test_untrained_network_is_the_jacobi_vcycle(gpu_lib, net):
     y_or, _ = lrn.forward(np.zeros(0), lay0, net["levels"], r.astype(np.float64))
     err = np.linalg.norm(y_gpu - y_or) / np.linalg.norm(y_or)
     say(f"  rel err vs classical V-cycle {err:.2e}")
-    assert err < 1e-2
+    assert err < 1e-4
     assert np.all(y_gpu[mask == 0] == 0)
 
 
@@ -85,10 +86,10 @@ def test_forward_matches_oracle(gpu_lib, net):
     for trial in range(2):
         r = (rng.standard_normal(mask.size) * mask).astype(np.float32)
         y_gpu = pl.learn_apply(r).astype(np.float64)
-        y_or, _ = lrn.forward(p.astype(np.float64), lay, net["levels"], r.astype(np.float64), emulate_bf16=True)
+        y_or, _ = lrn.forward(p.astype(np.float64), lay, net["levels"], r.astype(np.float64))
         err = np.linalg.norm(y_gpu - y_or) / np.linalg.norm(y_or)
         say(f"  forward trial {trial}: rel err {err:.2e}")
-        assert err < 1e-2
+        assert err < 1e-4
 
 
 def test_gradient_matches_oracle(gpu_lib, net):

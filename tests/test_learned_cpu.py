@@ -169,3 +169,20 @@ def test_planted_mistake_in_gradient_is_caught():
     fd = _fd(p, lay, lev, pairs, d)
     assert abs(fd - grad[c]) <= 1e-5 * max(1e-3, abs(fd))
     assert abs(fd - wrong[0]) > 1e-2 * abs(fd)
+
+
+def test_collapsed_smoother_equals_factored_smoother():
+    """In^T S In applied through the channels equals the 81 tap-pair weights the library
+    applies, up to the grid edge (and the pair matrix is symmetric)."""
+    rng = np.random.default_rng(5)
+    n = 11
+    lay = lrn.layout(n, n, [16], learned_levels=1)
+    p = lrn.initial_params(lay, small=0.3)
+    w_in, q = lrn.unpack(p, lay)
+    k81 = lrn.collapsed_pairs(w_in[0], q[0])
+    assert np.allclose(k81, k81.T)
+    v = rng.standard_normal((n, n))
+    E = lrn.conv_same(lrn._as_chan(v.reshape(-1), n, n), np.einsum("oc,tci->toi", q[0] @ q[0].T, w_in[0]))
+    y_fact = lrn.conv_same(E, np.transpose(w_in[0], (0, 2, 1)), sign=-1)[:, :, 0]
+    y_col = lrn.apply_collapsed(k81, v)
+    assert np.allclose(y_fact, y_col, rtol=1e-12, atol=1e-12)
