@@ -177,117 +177,141 @@ struct MgArgs {
     const double* done;
 };
 
-__device__ inline float rweight(int a) { return a == 0 ? 1.0f : 0.5f; }
-
-// Down: one thread per coarse node (I, J): x on the owned fine nodes (Jacobi levels),
-// rc = R (r - A x) over the 3x3 fine nodes around (2I, 2J).
+// Down, tiled: a block owns CI x CJ coarse nodes. It stages x on the fine nodes
+// [2I0-2, 2I0+2CI] x [2J0-2, 2J0+2CJ] (Jacobi levels: omega/diag r, computed here and written
+// back for the owned fine nodes) and r on the restriction footprint in LDS, forms the fine
+// residual s = r - A x on the footprint, then rc = R s.
+#define MGD_CI 8
+#define MGD_CJ 32
+#define MGD_XI (2 * MGD_CI + 3)
+#define MGD_XJ (2 * MGD_CJ + 3)
+#define MGD_SI (2 * MGD_CI + 1)
+#define MGD_SJ (2 * MGD_CJ + 1)
 __global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
+    __shared__ float xs[MGD_XI * MGD_XJ];
+    __shared__ float rs[MGD_SI * MGD_SJ];
+    __shared__ float ss[MGD_SI * MGD_SJ];
     if (a.done && *a.done != 0.0) return;
-    const int b = blockIdx.y;
-    const int nc = a.Hc * a.Wc, nf = a.Hf * a.Wf;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int I = c / a.Wc, J = c - I * a.Wc;
-    const int fi0 = 2 * I, fj0 = 2 * J;
+    const int b = blockIdx.z;
+    const int I0 = blockIdx.y * MGD_CI, J0 = blockIdx.x * MGD_CJ;
+    const int nf = a.Hf * a.Wf;
     const float* r = a.r + (int64_t)b * a.r_bs;
     float* x = a.x + (int64_t)b * a.fs;
-    float xw[5][5];
-#pragma unroll
-    for (int u = 0; u < 5; ++u)
-#pragma unroll
-        for (int v = 0; v < 5; ++v) {
-            int fi = fi0 + u - 2, fj = fj0 + v - 2;
-            float val = 0.0f;
-            if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
-                int f = fi * a.Wf + fj;
-                val = a.jacobi ? a.omega * a.dinv[f] * r[f] : x[f];
-            }
-            xw[u][v] = val;
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;
+    for (int k = threadIdx.x; k < MGD_XI * MGD_XJ; k += 256) {
+        const int u = k / MGD_XJ, v = k - u * MGD_XJ;
+        const int fi = fi0 + u, fj = fj0 + v;
+        float val = 0.0f;
+        if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
+            const int f = fi * a.Wf + fj;
+            val = a.jacobi ? a.omega * a.dinv[f] * r[f] : x[f];
         }
-    if (a.jacobi) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                int fi = fi0 + u, fj = fj0 + v;
-                if (fi < a.Hf && fj < a.Wf) x[fi * a.Wf + fj] = xw[u + 2][v + 2];
-            }
+        xs[k] = val;
     }
-    float acc = 0.0f;
-#pragma unroll
-    for (int u = -1; u <= 1; ++u)
-#pragma unroll
-        for (int v = -1; v <= 1; ++v) {
-            int fi = fi0 + u, fj = fj0 + v;
-            if (fi < 0 || fi >= a.Hf || fj < 0 || fj >= a.Wf) continue;
-            int f = fi * a.Wf + fj;
-            if (a.dia[4 * (int64_t)nf + f] == 0.0f) continue;          // P row of a Dirichlet node is zero
-            float s = r[f];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) s = s - a.dia[(int64_t)t * nf + f] * xw[u + 2 + t / 3 - 1][v + 2 + t % 3 - 1];
-            acc = acc + rweight(u) * rweight(v) * s;
-        }
-    a.rc[(int64_t)b * a.cs + c] = acc;
-}
-
-// x2 at fine node (gi, gj): x + P ec (zero on Dirichlet nodes)
-__device__ inline float mg_x2(const MgArgs& a, const float* x, const float* ec, int gi, int gj) {
-    if (gi < 0 || gi >= a.Hf || gj < 0 || gj >= a.Wf) return 0.0f;
-    const int nf = a.Hf * a.Wf;
-    const int g = gi * a.Wf + gj;
-    if (a.dia[4 * (int64_t)nf + g] == 0.0f) return 0.0f;
-    float pe = 0.0f;
-    const int i0 = gi >> 1, j0 = gj >> 1;
-    const bool oi = gi & 1, oj = gj & 1;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        if (u == 1 && !oi) break;
-        int I = i0 + u;
-        if (I >= a.Hc) continue;
-        float wi = oi ? 0.5f : 1.0f;
-#pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            if (v == 1 && !oj) break;
-            int J = j0 + v;
-            if (J >= a.Wc) continue;
-            float wj = oj ? 0.5f : 1.0f;
-            pe = pe + wi * wj * ec[I * a.Wc + J];
+    for (int k = threadIdx.x; k < MGD_SI * MGD_SJ; k += 256) {
+        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
+        rs[k] = (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) ? r[fi * a.Wf + fj] : 0.0f;
+    }
+    __syncthreads();
+    if (a.jacobi) {                                  // owned fine nodes [2I0, 2I0+2CI) x [2J0, 2J0+2CJ)
+        for (int k = threadIdx.x; k < 4 * MGD_CI * MGD_CJ; k += 256) {
+            const int u = k / (2 * MGD_CJ), v = k - u * (2 * MGD_CJ);
+            const int fi = 2 * I0 + u, fj = 2 * J0 + v;
+            if (fi < a.Hf && fj < a.Wf) x[fi * a.Wf + fj] = xs[(u + 2) * MGD_XJ + (v + 2)];
         }
     }
-    return x[g] + pe;
+    for (int k = threadIdx.x; k < MGD_SI * MGD_SJ; k += 256) {
+        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
+        float sv = 0.0f;
+        if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
+            const int f = fi * a.Wf + fj;
+            if (a.dia[4 * (int64_t)nf + f] != 0.0f) {
+                sv = rs[k];
+#pragma unroll
+                for (int t = 0; t < 9; ++t)
+                    sv = sv - a.dia[(int64_t)t * nf + f] * xs[(u + t / 3) * MGD_XJ + (v + t % 3)];
+            }
+        }
+        ss[k] = sv;
+    }
+    __syncthreads();
+    const int ti = threadIdx.x / MGD_CJ, tj = threadIdx.x - (threadIdx.x / MGD_CJ) * MGD_CJ;
+    const int I = I0 + ti, J = J0 + tj;
+    if (I < a.Hc && J < a.Wc) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int u = -1; u <= 1; ++u)
+#pragma unroll
+            for (int v = -1; v <= 1; ++v)
+                acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * ss[(2 * ti + 1 + u) * MGD_SJ + (2 * tj + 1 + v)];
+        a.rc[(int64_t)b * a.cs + I * a.Wc + J] = acc;
+    }
 }
 
-// Up: one thread per fine node: x2 = x + P ec; Jacobi levels y = x2 + omega/diag (r - A x2),
-// learned levels y = x2 and s2 = r - A x2 (the learned post-smoothing adds K s2 to y).
+// Up, tiled: a block owns FI x FJ fine nodes; it stages ec on the covering coarse nodes and x
+// on the fine halo in LDS, forms x2 = x + P ec on the halo, then (Jacobi levels)
+// y = x2 + omega/diag (r - A x2), or (learned levels) y = x2 and s2 = r - A x2.
+#define MGU_FI 4
+#define MGU_FJ 64
+#define MGU_CI (MGU_FI / 2 + 2)
+#define MGU_CJ (MGU_FJ / 2 + 2)
 __global__ __launch_bounds__(256) void k_mg_up(MgArgs a) {
+    __shared__ float es[MGU_CI * MGU_CJ];
+    __shared__ float x2s[(MGU_FI + 2) * (MGU_FJ + 2)];
     if (a.done && *a.done != 0.0) return;
-    const int b = blockIdx.y;
+    const int b = blockIdx.z;
+    const int fi0 = blockIdx.y * MGU_FI, fj0 = blockIdx.x * MGU_FJ;
     const int nf = a.Hf * a.Wf;
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= nf) return;
-    const int fi = f / a.Wf, fj = f - fi * a.Wf;
     const float* r = a.r + (int64_t)b * a.r_bs;
     const float* x = a.x + (int64_t)b * a.fs;
     const float* ec = a.ec + (int64_t)b * a.cs;
     float* y = a.y + (int64_t)b * a.y_bs;
+    const int ci0 = (fi0 - 1) >> 1, cj0 = (fj0 - 1) >> 1;    // floor
+    for (int k = threadIdx.x; k < MGU_CI * MGU_CJ; k += 256) {
+        const int u = k / MGU_CJ, v = k - u * MGU_CJ;
+        const int I = ci0 + u, J = cj0 + v;
+        es[k] = (I >= 0 && I < a.Hc && J >= 0 && J < a.Wc) ? ec[I * a.Wc + J] : 0.0f;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < (MGU_FI + 2) * (MGU_FJ + 2); k += 256) {
+        const int u = k / (MGU_FJ + 2), v = k - u * (MGU_FJ + 2);
+        const int gi = fi0 - 1 + u, gj = fj0 - 1 + v;
+        float val = 0.0f;
+        if (gi >= 0 && gi < a.Hf && gj >= 0 && gj < a.Wf) {
+            const int g = gi * a.Wf + gj;
+            if (a.dia[4 * (int64_t)nf + g] != 0.0f) {
+                const int i0 = gi >> 1, j0 = gj >> 1;
+                const bool oi = gi & 1, oj = gj & 1;
+                float pe = 0.0f;
+                for (int p = 0; p < (oi ? 2 : 1); ++p)
+                    for (int q = 0; q < (oj ? 2 : 1); ++q)
+                        pe = pe + (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * es[(i0 + p - ci0) * MGU_CJ + (j0 + q - cj0)];
+                val = x[g] + pe;
+            }
+        }
+        x2s[k] = val;
+    }
+    __syncthreads();
+    const int ti = threadIdx.x / MGU_FJ, tj = threadIdx.x - (threadIdx.x / MGU_FJ) * MGU_FJ;
+    const int fi = fi0 + ti, fj = fj0 + tj;
+    if (fi >= a.Hf || fj >= a.Wf) return;
+    const int f = fi * a.Wf + fj;
+    const float xc = x2s[(ti + 1) * (MGU_FJ + 2) + (tj + 1)];
     if (a.dia[4 * (int64_t)nf + f] == 0.0f) {
         y[f] = 0.0f;
         if (!a.jacobi) a.s2[(int64_t)b * a.fs + f] = 0.0f;
         return;
     }
-    float s = r[f];
-    float xc = 0.0f;
+    float sres = r[f];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        float xv = mg_x2(a, x, ec, fi + t / 3 - 1, fj + t % 3 - 1);
-        if (t == 4) xc = xv;
-        s = s - a.dia[(int64_t)t * nf + f] * xv;
-    }
+    for (int t = 0; t < 9; ++t) sres = sres - a.dia[(int64_t)t * nf + f] * x2s[(ti + t / 3) * (MGU_FJ + 2) + (tj + t % 3)];
     if (a.jacobi) {
-        y[f] = xc + a.omega * a.dinv[f] * s;
+        y[f] = xc + a.omega * a.dinv[f] * sres;
     } else {
         y[f] = xc;
-        a.s2[(int64_t)b * a.fs + f] = s;
+        a.s2[(int64_t)b * a.fs + f] = sres;
     }
 }
 
@@ -304,35 +328,170 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, cons
     }
 }
 
+// Levels lc .. L-1 (Jacobi smoothers, all <= 33 x 33 nodes) in ONE workgroup per batch entry:
+// the down sweeps, the exact coarsest solve and the up sweeps of the V-cycle, phase by phase
+// with block barriers, every vector in LDS (operators read through the cache). Replaces ~2
+// launches per level whose run time is pure latency at these sizes.
+#define MG_CB_THREADS 1024
+#define MG_CB_MAXN 1089
+struct MgCoarseArgs {
+    int lc, L;
+    int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
+    int off[LN_MAX_LEVELS];                      // LDS offset (floats) of each level's 4 vectors r, x, y, t
+    const float* dia[LN_MAX_LEVELS];
+    const float* dinv[LN_MAX_LEVELS];
+    const float* rin; int64_t rin_bs;            // residual of level lc (global)
+    float* yout; int64_t yout_bs;                // correction of level lc (global)
+    const float* inv; float omega;
+    const double* done;
+};
+
+__global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs a) {
+    extern __shared__ float lds[];
+    if (a.done && *a.done != 0.0) return;
+    const int b = blockIdx.y;
+    {
+        const int n = a.H[a.lc] * a.W[a.lc];
+        float* r = lds + a.off[a.lc];
+        for (int f = threadIdx.x; f < n; f += MG_CB_THREADS) r[f] = a.rin[(int64_t)b * a.rin_bs + f];
+    }
+    __syncthreads();
+    for (int l = a.lc; l < a.L - 1; ++l) {
+        const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
+        const int nf = Hf * Wf, nc = Hc * Wc;
+        float* r = lds + a.off[l];
+        float* x = r + nf;
+        float* s = r + 3 * nf;
+        float* rc = lds + a.off[l + 1];
+        const float* dia = a.dia[l];
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) x[f] = a.omega * a.dinv[l][f] * r[f];
+        __syncthreads();
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
+            const int fi = f / Wf, fj = f - (f / Wf) * Wf;
+            float v = 0.0f;
+            if (dia[4 * nf + f] != 0.0f) {
+                v = r[f];
+                for (int t = 0; t < 9; ++t) {
+                    const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
+                    if (gi < 0 || gi >= Hf || gj < 0 || gj >= Wf) continue;
+                    v = v - dia[t * nf + f] * x[gi * Wf + gj];
+                }
+            }
+            s[f] = v;
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < nc; c += MG_CB_THREADS) {
+            const int I = c / Wc, J = c - (c / Wc) * Wc;
+            float acc = 0.0f;
+            for (int u = -1; u <= 1; ++u)
+                for (int v = -1; v <= 1; ++v) {
+                    const int fi = 2 * I + u, fj = 2 * J + v;
+                    if (fi < 0 || fi >= Hf || fj < 0 || fj >= Wf) continue;
+                    acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * s[fi * Wf + fj];
+                }
+            rc[c] = acc;
+        }
+        __syncthreads();
+    }
+    {
+        const int l = a.L - 1;
+        const int n = a.H[l] * a.W[l];
+        const float* r = lds + a.off[l];
+        float* y = lds + a.off[l] + 2 * n;
+        for (int i = threadIdx.x; i < n; i += MG_CB_THREADS) {
+            float sacc = 0.0f;
+            for (int j = 0; j < n; ++j) sacc = sacc + a.inv[i * n + j] * r[j];
+            y[i] = sacc;
+        }
+        __syncthreads();
+    }
+    for (int l = a.L - 2; l >= a.lc; --l) {
+        const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
+        const int nf = Hf * Wf;
+        const float* r = lds + a.off[l];
+        const float* x = r + nf;
+        float* y = lds + a.off[l] + 2 * nf;
+        float* x2 = lds + a.off[l] + 3 * nf;
+        const float* yc = lds + a.off[l + 1] + 2 * Hc * Wc;
+        const float* dia = a.dia[l];
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
+            const int gi = f / Wf, gj = f - (f / Wf) * Wf;
+            float v = 0.0f;
+            if (dia[4 * nf + f] != 0.0f) {
+                float pe = 0.0f;
+                const int i0 = gi >> 1, j0 = gj >> 1;
+                const bool oi = gi & 1, oj = gj & 1;
+                for (int u = 0; u < (oi ? 2 : 1); ++u) {
+                    const int I = i0 + u;
+                    if (I >= Hc) continue;
+                    for (int w = 0; w < (oj ? 2 : 1); ++w) {
+                        const int J = j0 + w;
+                        if (J >= Wc) continue;
+                        pe = pe + (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * yc[I * Wc + J];
+                    }
+                }
+                v = x[f] + pe;
+            }
+            x2[f] = v;
+        }
+        __syncthreads();
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
+            const int fi = f / Wf, fj = f - (f / Wf) * Wf;
+            float v = 0.0f;
+            if (dia[4 * nf + f] != 0.0f) {
+                float sres = r[f];
+                for (int t = 0; t < 9; ++t) {
+                    const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
+                    if (gi < 0 || gi >= Hf || gj < 0 || gj >= Wf) continue;
+                    sres = sres - dia[t * nf + f] * x2[gi * Wf + gj];
+                }
+                v = x2[f] + a.omega * a.dinv[l][f] * sres;
+            }
+            y[f] = v;
+        }
+        __syncthreads();
+    }
+    {
+        const int n = a.H[a.lc] * a.W[a.lc];
+        const float* y = lds + a.off[a.lc] + 2 * n;
+        for (int f = threadIdx.x; f < n; f += MG_CB_THREADS) a.yout[(int64_t)b * a.yout_bs + f] = y[f];
+    }
+}
+
 // Learned smoother, collapsed: out = [out +] Dh (In^T S In) (Dh v) on one level with the
-// 81 tap-pair weights k81 (zero outside the grid, as the factored form).
+// 81 tap-pair weights k81 (zero outside the grid, as the factored form). A block owns
+// SMI x SMJ nodes and stages Dh v on the tile with a halo of 2 in LDS.
+#define SMI 4
+#define SMJ 64
 __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k81, const float* dh, const float* v,
                                                     int64_t v_bs, float* out, int64_t o_bs, int add, const double* done) {
+    __shared__ float us[(SMI + 4) * (SMJ + 4)];
+    __shared__ float kw[81];
     if (done && *done != 0.0) return;
-    const int n = H * W;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const float* vb = v + (int64_t)blockIdx.y * v_bs;
-    float* ob = out + (int64_t)blockIdx.y * o_bs;
+    const float* vb = v + (int64_t)blockIdx.z * v_bs;
+    float* ob = out + (int64_t)blockIdx.z * o_bs;
+    const int i0 = blockIdx.y * SMI, j0 = blockIdx.x * SMJ;
+    if (threadIdx.x < 81) kw[threadIdx.x] = k81[threadIdx.x];
+    for (int k = threadIdx.x; k < (SMI + 4) * (SMJ + 4); k += 256) {
+        const int u = k / (SMJ + 4), w = k - u * (SMJ + 4);
+        const int ii = i0 - 2 + u, jj = j0 - 2 + w;
+        us[k] = (ii >= 0 && ii < H && jj >= 0 && jj < W) ? dh[ii * W + jj] * vb[ii * W + jj] : 0.0f;
+    }
+    __syncthreads();
+    const int ti = threadIdx.x / SMJ, tj = threadIdx.x - (threadIdx.x / SMJ) * SMJ;
+    const int i = i0 + ti, j = j0 + tj;
+    if (i >= H || j >= W) return;
+    const int p = i * W + j;
     const float dp = dh[p];
     float s = 0.0f;
     if (dp != 0.0f) {
-        const int i = p / W, j = p - (p / W) * W;
-        float u[5][5];
-#pragma unroll
-        for (int a = 0; a < 5; ++a)
-#pragma unroll
-            for (int b = 0; b < 5; ++b) {
-                const int ii = i + a - 2, jj = j + b - 2;
-                u[a][b] = (ii >= 0 && ii < H && jj >= 0 && jj < W) ? dh[ii * W + jj] * vb[ii * W + jj] : 0.0f;
-            }
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);   // intermediate pixel p - d_t
             if (qi < 0 || qi >= H || qj < 0 || qj >= W) continue;
 #pragma unroll
             for (int t2 = 0; t2 < 9; ++t2)
-                s = s + k81[9 * t + t2] * u[2 - (t / 3 - 1) + (t2 / 3 - 1)][2 - (t % 3 - 1) + (t2 % 3 - 1)];
+                s = s + kw[9 * t + t2] * us[(ti + 2 - (t / 3 - 1) + (t2 / 3 - 1)) * (SMJ + 4) + (tj + 2 - (t % 3 - 1) + (t2 % 3 - 1))];
         }
         s = dp * s;
     }
@@ -762,6 +921,8 @@ struct MgLevel {
 struct LearnNet {
     bool ready = false;
     int L = 0, Ls = 0;                               // levels, learned smoother levels
+    int lc = 0;                                      // first level of the single-workgroup coarse part
+    size_t lc_lds = 0;                               // its LDS bytes
     MgLevel lv[LN_MAX_LEVELS];
     int C[LN_MAX_LEARNED] = {0};
     float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator
@@ -849,30 +1010,48 @@ static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, in
 
 static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, int64_t o_bs, bool add, const double* done) {
     const MgLevel& lv = ln.lv[l];
-    hipLaunchKernelGGL(k_smooth_k81, dim3((unsigned)((lv.n + 255) / 256), batch), dim3(256), 0, g.stream, lv.H, lv.W,
-                       ln.d_k81 + 81 * l, lv.dh, v, v_bs, out, o_bs, add ? 1 : 0, done);
+    dim3 grid((unsigned)((lv.W + SMJ - 1) / SMJ), (unsigned)((lv.H + SMI - 1) / SMI), batch);
+    hipLaunchKernelGGL(k_smooth_k81, grid, dim3(256), 0, g.stream, lv.H, lv.W, ln.d_k81 + 81 * l, lv.dh, v, v_bs, out,
+                       o_bs, add ? 1 : 0, done);
 }
 
 // y = M r for `batch` inputs (level-0 vectors with batch stride bs); run selects the work set.
 static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done) {
-    const int L = ln.L;
-    for (int l = 0; l < L - 1; ++l) {
+    const int L = ln.L, lc = ln.lc;
+    for (int l = 0; l < lc; ++l) {
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
         if (l < ln.Ls) smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done);
         MgArgs a = mg_args(l, run, r, r_bs, nullptr, 0, done);
-        hipLaunchKernelGGL(k_mg_down, dim3((unsigned)((ln.lv[l + 1].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
+        const MgLevel& c = ln.lv[l + 1];
+        dim3 grid((unsigned)((c.W + MGD_CJ - 1) / MGD_CJ), (unsigned)((c.H + MGD_CI - 1) / MGD_CI), batch);
+        hipLaunchKernelGGL(k_mg_down, grid, dim3(256), 0, g.stream, a);
     }
-    const MgLevel& cl = ln.lv[L - 1];
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1, batch), dim3(256), 0, g.stream, (int)cl.n, ln.d_inv, cl.r[run], cl.y[run],
-                       cl.n, done);
-    for (int l = L - 2; l >= 0; --l) {
+    if (lc == 0) return pic_fail(PIC_ERR_STATE, "learned network: coarse part without a fine level");
+    {
+        MgCoarseArgs c;
+        std::memset(&c, 0, sizeof(c));
+        c.lc = lc; c.L = L;
+        int off = 0;
+        for (int l = lc; l < L; ++l) {
+            const MgLevel& v = ln.lv[l];
+            c.H[l] = v.H; c.W[l] = v.W; c.dia[l] = v.dia; c.dinv[l] = v.dinv;
+            c.off[l] = off; off += 4 * (int)v.n;
+        }
+        c.rin = ln.lv[lc].r[run]; c.rin_bs = ln.lv[lc].n;
+        c.yout = ln.lv[lc].y[run]; c.yout_bs = ln.lv[lc].n;
+        c.inv = ln.d_inv; c.omega = LN_OMEGA; c.done = done;
+        hipLaunchKernelGGL(k_mg_coarse_block, dim3(1, batch), dim3(MG_CB_THREADS), ln.lc_lds, g.stream, c);
+    }
+    for (int l = lc - 1; l >= 0; --l) {
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
         float* y = l == 0 ? y0 : ln.lv[l].y[run];
         int64_t y_bs = l == 0 ? bs : ln.lv[l].n;
         MgArgs a = mg_args(l, run, r, r_bs, y, y_bs, done);
-        hipLaunchKernelGGL(k_mg_up, dim3((unsigned)((ln.lv[l].n + 255) / 256), batch), dim3(256), 0, g.stream, a);
+        const MgLevel& f = ln.lv[l];
+        dim3 grid((unsigned)((f.W + MGU_FJ - 1) / MGU_FJ), (unsigned)((f.H + MGU_FI - 1) / MGU_FI), batch);
+        hipLaunchKernelGGL(k_mg_up, grid, dim3(256), 0, g.stream, a);
         if (l < ln.Ls) smooth5(l, batch, ln.lv[l].s2[run], ln.lv[l].n, y, y_bs, true, done);
     }
     HIPCHK(hipGetLastError());
@@ -1016,6 +1195,14 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     if ((int64_t)H[L - 1] * W[L - 1] > LN_MAX_COARSE) return pic_fail(PIC_ERR_UNSUPPORTED, "coarsest level too large");
     ln.L = L;
     ln.Ls = std::min(nch, L - 1);
+    ln.lc = L - 1;                                   // levels of <= 33 x 33 nodes go to one workgroup
+    while (ln.lc > std::max(ln.Ls, 1) && (int64_t)H[ln.lc - 1] * W[ln.lc - 1] <= MG_CB_MAXN) ln.lc--;
+    {
+        size_t lds = 0;
+        for (int l = ln.lc; l < L; ++l) lds += 4 * (size_t)H[l] * W[l] * sizeof(float);
+        if (lds > 64 * 1024) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse levels exceed the workgroup LDS");
+        ln.lc_lds = lds;
+    }
     for (int l = 0; l < ln.Ls; ++l) {
         int c = channels[l];
         if (c != 16 && c != 32 && c != 64) return pic_fail(PIC_ERR_ARG, "channels must be 16, 32 or 64");
