@@ -383,9 +383,15 @@ def test_library_exports_every_declared_symbol(pkg):
 
 
 def test_workload_definitions(pkg):
+    from patch_ref import element_and_node_classes
     w = pkg.workload("cfg3")
-    assert w["domain"].nnx == 513 and len(w["domain"].objects) == 3 and w["ppc"] == 64
-    assert (w["domain"].nnx - 1) ** 2 * w["ppc"] == 16777216
+    d = w["domain"]
+    assert d.nnx == 513 and len(d.objects) == 3 and w["ppc"] == 71
+    # 16.8M particles per species: ppc per element outside the conductors (the interface
+    # elements keep the particles that fall outside the objects)
+    xs = np.arange(d.nnx, dtype=float)
+    _, ei = element_and_node_classes(d.nnx, d.nny, xs, xs, [(o.x, o.y, o.r) for o in d.objects])
+    assert (ei == -1).sum() * w["ppc"] < 16777216 < (ei != -2).sum() * w["ppc"]
     w1 = pkg.workload("cfg1")
     assert (w1["domain"].nnx - 1) ** 2 * w1["ppc"] == 409600
 
