@@ -166,7 +166,9 @@ int pic_init(int device) {
     if (device < 0 || device >= c) return pic_fail(PIC_ERR_ARG, "device index out of range");
     HIPCHK(hipSetDevice(device));
     g.device = device;
-    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    // A blocking stream: the synchronous hipMemcpy / hipMemset calls of the API (null stream)
+    // are then ordered with the library's kernels on both sides.
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamDefault));
     HIPCHK(hipHostMalloc((void**)&g.h_scal_pinned, 64 * sizeof(double), hipHostMallocDefault));
     HIPCHK(hipMalloc((void**)&g.d_scal, 64 * sizeof(double)));
     HIPCHK(hipMemset(g.d_scal, 0, 64 * sizeof(double)));

@@ -786,7 +786,9 @@ __global__ __launch_bounds__(256) void k_harvest(int n, Dia8 A, const float* xf,
 // ---------------------------------------------------------------- FCG(1) on the scaled system
 // scal slots (shared with the CG): NORMRHS, DONE, ITER, IERR, TOL2, MAXIT; pq in GAM0/GAM1
 enum { F_S_NORMRHS = 4, F_S_NORMR = 5, F_S_DONE = 6, F_S_ITER = 7, F_S_IERR = 8, F_S_TOL2 = 9, F_S_MAXIT = 10,
-       F_S_PQ0 = 12, F_S_PQ1 = 13 };
+       F_S_PQ0 = 12, F_S_PQ1 = 13, F_S_KCUR = 20, F_S_KNEXT = 21 };
+// The iteration index lives on the device (KNEXT advanced by the check kernel, KCUR read by
+// the update), so every chunk of iterations is the same launch sequence.
 #define FCG_BLOCK 256
 
 __device__ inline void fcg_block_partials(int nv, double* v, double* part, int pstride) {
@@ -827,16 +829,21 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_init(int n, const double* b64
         v[0] = (double)ri * (double)ri;
     }
     fcg_block_partials(1, v, part, pstride);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[F_S_DONE] = 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[F_S_DONE] = 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0; scal[F_S_KNEXT] = 0.0;
+    }
 }
 
-// convergence test of the current r before iteration k (one block): done if
-// ||r||^2/||b||^2 <= tol^2 or k >= maxit
-__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_check(int nb, int k, double* scal, const double* part, int pstride) {
+// convergence test of the current r before iteration k = KNEXT (one block): done if
+// ||r||^2/||b||^2 <= tol^2 or k >= maxit; publishes k as KCUR and advances KNEXT
+__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_check(int nb, double* scal, const double* part, int pstride) {
     if (scal[F_S_DONE] != 0.0) return;
+    const int k = (int)scal[F_S_KNEXT];
     double rr;
     fcg_reduce(1, part, pstride, nb, &rr);
     if (threadIdx.x != 0) return;
+    scal[F_S_KCUR] = (double)k;
+    scal[F_S_KNEXT] = (double)(k + 1);
     if (k == 0) scal[F_S_NORMRHS] = rr;
     double nb2 = (k == 0) ? rr : scal[F_S_NORMRHS];
     scal[F_S_NORMR] = rr;
@@ -868,7 +875,7 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_aw(int n, Dia8 A, const float
 // beta = -(z, q_prev)/(p_prev, q_prev); p = z + beta p; q = w + beta q;
 // pq = (z,w) + 2 beta (z,q_prev) + beta^2 pq_prev; alpha = (z, r)/pq; x += alpha p; r -= alpha q;
 // partial r.r (region 0)
-__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, int kk, const float* z, const float* w, float* p,
+__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, const float* z, const float* w, float* p,
                                                           float* q, float* x, float* r, double* scal, double* part,
                                                           int pstride) {
     __shared__ double s_done;
@@ -876,6 +883,7 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, int kk,
     double d[3];
     fcg_reduce(3, part + (size_t)pstride, pstride, nb, d);     // contains a barrier after s_done
     if (s_done != 0.0) return;
+    const int kk = (int)scal[F_S_KCUR];
     const double k = (double)kk;
     const int par = kk & 1;
     double zr = d[0], zw = d[1], zq = d[2];
@@ -1592,19 +1600,25 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
     HIPCHK(hipGetLastError());
     Dia8 A = scaled_op();
     const int CH = 3;
+    // one chunk: CH x (check, V-cycle, A z + dots, update); every argument is fixed
+    auto launch_chunk = [&]() -> int {
+        for (int c = 0; c < CH; ++c) {
+            hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, scal, part, ps);
+            int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
+            hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, ln.z, ln.w, p, ln.q, x, r,
+                               scal, part, ps);
+        }
+        HIPCHK(hipGetLastError());
+        return PIC_OK;
+    };
     int k = 0;
     double prev_rr = -1.0;
     int slow = 0;
     for (;;) {
         HIPCHK(hipEventRecord(g.ev[12], g.stream));
-        for (int c = 0; c < CH; ++c, ++k) {
-            hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, k, scal, part, ps);
-            int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
-            hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, k, ln.z, ln.w, p, ln.q, x, r,
-                               scal, part, ps);
-        }
-        HIPCHK(hipGetLastError());
+        int rc = launch_chunk(); if (rc) return rc;
+        k += CH;
         HIPCHK(hipEventRecord(g.ev[13], g.stream));
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, 16 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
