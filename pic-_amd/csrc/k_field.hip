@@ -612,7 +612,8 @@ int run_pcg_mixed() {
         int it = 0, ierr = 0;
         if (use_learned) {
             int stagnated = 0;
-            rc = learn_fcg_run(n, g.d_rn, inner_tol, std::min(g.learn_maxit, g.pcg_maxit - total), &it, &ierr, &stagnated);
+            rc = learn_fcg_run(n, g.d_rn, learn_inner_tol(), std::min(g.learn_maxit, g.pcg_maxit - total), &it, &ierr,
+                               &stagnated);
             if (rc) return rc;
             g.last_learned_iters += it;
             if (stagnated) { use_learned = false; g.last_fallback = 1; learn_count_fallback(); }

@@ -967,6 +967,9 @@ struct LearnNet {
     int last_iters = 0; long long fallbacks = 0;
     double last_loss = 0.0;
     bool trace = false;
+    double inner_tol = 1e-3;                         // FCG tolerance per refinement pass (the FP64 outer
+                                                     // residual decides convergence)
+    int chunk = 3;                                   // FCG iterations between host checks
 };
 static LearnNet ln;
 
@@ -1330,6 +1333,8 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     HIPCHK(hipStreamSynchronize(g.stream));
     const char* tr = std::getenv("PIC_FCG_TRACE");
     ln.trace = tr && tr[0] == '1';
+    if (const char* e = std::getenv("PIC_LEARN_INNER_TOL")) { double v = std::atof(e); if (v > 0 && v < 1) ln.inner_tol = v; }
+    if (const char* e = std::getenv("PIC_FCG_CHUNK")) { int v = std::atoi(e); if (v >= 1 && v <= 16) ln.chunk = v; }
     ln.ready = true;
     return PIC_OK;
 }
@@ -1599,7 +1604,7 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
     hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, b64, g.d_sinv, r, x, scal, part, ps);
     HIPCHK(hipGetLastError());
     Dia8 A = scaled_op();
-    const int CH = 3;
+    const int CH = ln.chunk;
     // one chunk: CH x (check, V-cycle, A z + dots, update); every argument is fixed
     auto launch_chunk = [&]() -> int {
         for (int c = 0; c < CH; ++c) {
@@ -1645,6 +1650,7 @@ int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, i
 }
 
 bool learn_ready() { return ln.ready; }
+double learn_inner_tol() { return ln.inner_tol; }
 
 // --------------------------------------------------------------- test/diagnostic access
 int learn_apply_host(const float* r, float* z, int64_t n) {

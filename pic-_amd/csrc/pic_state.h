@@ -256,6 +256,7 @@ int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_fir
 int learn_eval(int npairs, double* mean_loss);
 int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, int* ierr, int* stagnated);
 bool learn_ready();
+double learn_inner_tol();
 int learn_apply_host(const float* r, float* z, int64_t n);
 int learn_params(float* dst, const float* src, int64_t n, int64_t* nparams);
 int learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets);
