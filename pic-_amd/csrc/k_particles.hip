@@ -40,6 +40,7 @@ template <typename R> struct PushArgs {
     unsigned int* err;
     PicErrInfo* errinfo;
     int species;
+    int* wall_list; unsigned int* wall_count;   // particles whose first pass reaches a wall
 };
 
 __device__ inline long long shfl_xor_ll(long long v, int d) {
@@ -293,6 +294,115 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
     return live;
 }
 
+// First pass of MoveAll's loop for one particle (gather + push, MoveOne.f90:77-146, 213-219,
+// 242-245). Returns true when the pass ends outside [xmin, xmax] x [ymin, ymax]: the particle
+// then goes to the wall kernel, which runs the whole loop from the original state (the same
+// arithmetic, so the result is identical). Otherwise the loop would end here: the object
+// test of AdjustParticleBoundary (MCCInterface.f90:486-499) completes it.
+template <typename R>
+__device__ inline bool push_fast(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, bool& absorbed_obj) {
+    const MeshParams& m = a.m;
+    const R T = a.dt;
+    int ci = e_old / m.ney, cj = e_old - ci * m.ney;
+    R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
+    R hxp = x2 - x1, hyp = y2 - y1;
+    R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
+    R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
+    R xm = (R)1 - dx, ym = (R)1 - dy;
+    R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
+    int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+    R Ex, Ey;
+    if (m.element_index[e_old] > 0) {
+        R ex = 0, ey = 0;
+        int count = 0;
+        const int nd[4] = {n1, n2, n3, n4};
+        const R nx_[4] = {x1, x2, x2, x1}, ny_[4] = {y1, y1, y2, y2};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (m.node_index[nd[k]] == -1) {
+                const R* c = a.coef5 + (size_t)nd[k] * 5;
+                count++;
+                R xr = X - nx_[k], yr = Y - ny_[k];
+                ex = ex + c[0] + (R)2 * c[2] * xr + c[3] * yr;
+                ey = ey + c[1] + (R)2 * c[4] * yr + c[3] * xr;
+            }
+        }
+        ex = ex / (R)count;
+        ey = ey / (R)count;
+        Ex = -ex; Ey = -ey;
+    } else {
+        Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
+        Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
+    }
+    R VX1 = VX + a.qm * Ex * T;
+    R VY1 = VY + a.qm * Ey * T;
+    R X1 = X + T * VX1;
+    R Y1 = Y + T * VY1;
+    if (X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
+    absorbed_obj = false;
+    for (int k = 0; k < m.n_objects; ++k) {
+        const DevObject o = a.objs[k];
+        double xd = (double)X1, yd = (double)Y1;
+        bool in = false;
+        if (o.shape == 3) in = xd > o.x0 && xd < o.x1 && yd > o.y0 && yd < o.y1;
+        else if (o.shape == 1) in = (xd - o.x0) * (xd - o.x0) + (yd - o.y0) * (yd - o.y0) < o.r2;
+        if (in) { X1 = (R)-2000; absorbed_obj = true; }
+    }
+    X = X1; Y = Y1; VX = VX1; VY = VY1;
+    return false;
+}
+
+// One particle's lookup and deposit (GetParChg_2D.f90:137-163, 788-843) into the global
+// node accumulators: the path of the particles the wall kernel moves.
+template <typename R>
+__device__ inline int locate_deposit_global(const PushArgs<R>& a, int64_t p, int e_old, R X, R Y, R VX, R VY, R VZ) {
+    const MeshParams& m = a.m;
+    int bad = 0;
+    double Xd = (double)X, Yd = (double)Y;
+    int e_new = pic_locate_elem(Xd, Yd, m, &bad);
+    if (bad) {
+        atomicOr(a.err, PIC_EBIT_POSITION);
+        pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, e_new, a.species);
+    }
+    double W[4];
+    deposit_weights(Xd, Yd, e_new, m, W);
+    double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
+    double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
+    energy = energy * a.v_ref * a.v_ref;
+    energy = energy / PIC_JTOEV;
+    int ci = e_new / m.ney, cj = e_new - ci * m.ney;
+    int nd[4] = {ci * m.nny + cj, (ci + 1) * m.nny + cj, (ci + 1) * m.nny + cj + 1, ci * m.nny + cj + 1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        atomicAdd((unsigned long long*)&a.acc_w[nd[q]], (unsigned long long)pic_fix(W[q]));
+        atomicAdd((unsigned long long*)&a.acc_e[nd[q]], (unsigned long long)pic_fix(W[q] * energy));
+    }
+    return e_new;
+}
+
+// The particles whose first pass reached a wall: the whole MoveOne/AdjustParticleBoundary
+// loop, lookup, deposit, write-back. Grid-stride over the list the push kernel filled.
+template <typename R>
+__global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
+    const unsigned int nw = *a.wall_count;
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
+        const int64_t p = a.wall_list[i];
+        const int e_old = a.cell[p];
+        R X = a.x[p], Y = a.y[p], VX = a.vx[p], VY = a.vy[p], VZ = a.vz[p];
+        bool vzc = false;
+        PushEvents ev = {0, 0, 0, 0, 0};
+        bool live = move_one<R>(a, p, e_old, X, Y, VX, VY, VZ, vzc, ev);
+        a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
+        if (vzc) a.vz[p] = VZ;
+        a.cell_next[p] = live ? locate_deposit_global<R>(a, p, e_old, X, Y, VX, VY, VZ) : -1;
+        if (ev.emit) atomicAdd(&a.counters[PIC_C_EMIT], (unsigned long long)ev.emit);
+        if (ev.right) atomicAdd(&a.counters[PIC_C_RIGHT], (unsigned long long)ev.right);
+        if (ev.object) atomicAdd(&a.counters[PIC_C_OBJECT], (unsigned long long)ev.object);
+        if (ev.reflect) atomicAdd(&a.counters[PIC_C_REFLECT], (unsigned long long)ev.reflect);
+        if (ev.deleted) atomicAdd(&a.counters[PIC_C_DELETED], (unsigned long long)ev.deleted);
+    }
+}
+
 template <typename R> struct Vec8;
 template <> struct Vec8<float> {
     static __device__ inline void load(const float* p, float v[8]) {
@@ -383,23 +493,40 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     // ---------------- move, locate, deposit (run accumulation in registers)
     PushEvents ev = {0, 0, 0, 0, 0};
     int next[PUSH_PPT];
-    int vzmask = 0;
     int rkey = -1;
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
+    const int lane = threadIdx.x & 63;
     for (int k = 0; k < PUSH_PPT; ++k) {
-        int e_old = cel[k];
+        const int e_old = cel[k];
+        const int64_t p = p0 + k;
         next[k] = -1;
-        if (e_old < 0) continue;
-        int64_t p = p0 + k;
-        bool live = true;
-        if (DO_PUSH) {
-            bool vzc = false;
-            live = move_one<R>(a, p, e_old, X[k], Y[k], VX[k], VY[k], VZ[k], vzc, ev);
-            if (vzc) vzmask |= 1 << k;
+        bool live = e_old >= 0;
+        bool wall = false;
+        if (DO_PUSH && live) {
+            bool absorbed_obj = false;
+            wall = push_fast<R>(a, e_old, X[k], Y[k], VX[k], VY[k], absorbed_obj);
+            if (!wall) {
+                live = !(X[k] <= a.xmin);           // MoveAll deletion test, MCCInterface.f90:617-620
+                if (!(X[k] == X[k]) || !(Y[k] == Y[k]) || !(VX[k] == VX[k]) || !(VY[k] == VY[k])) {
+                    atomicOr(a.err, PIC_EBIT_NONFINITE);
+                    live = false;
+                }
+                ev.object += absorbed_obj && !live;
+                ev.deleted += !live;
+            }
         }
-        if (!live) continue;
+        if (DO_PUSH) {                               // wave-aggregated append to the wall list
+            const unsigned long long wm = __ballot(wall);
+            if (wm) {
+                unsigned int wb = 0;
+                if (lane == 0) wb = atomicAdd(a.wall_count, (unsigned int)__popcll(wm));
+                wb = __shfl(wb, 0, 64);
+                if (wall) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)p;
+            }
+        }
+        if (!live || wall) continue;
         int bad = 0;
         double Xd = (double)X[k], Yd = (double)Y[k];
         int e_new = pic_locate_elem(Xd, Yd, m, &bad);
@@ -445,14 +572,8 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
             if (DO_PUSH) { a.x[p] = X[k]; a.y[p] = Y[k]; a.vx[p] = VX[k]; a.vy[p] = VY[k]; }
         }
     }
-    if (DO_PUSH && vzmask) {
-        for (int k = 0; k < PUSH_PPT; ++k) if (vzmask & (1 << k)) a.vz[p0 + k] = VZ[k];
-    }
     if (DO_PUSH) {
-        wave_add(&a.counters[PIC_C_EMIT], ev.emit);
-        wave_add(&a.counters[PIC_C_RIGHT], ev.right);
         wave_add(&a.counters[PIC_C_OBJECT], ev.object);
-        wave_add(&a.counters[PIC_C_REFLECT], ev.reflect);
         wave_add(&a.counters[PIC_C_DELETED], ev.deleted);
     }
 
@@ -497,6 +618,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.acc_w = S.acc_w; a.acc_e = S.acc_e;
     a.counters = S.counters; a.err = g.d_err;
     a.errinfo = (PicErrInfo*)g.d_errinfo; a.species = isp;
+    a.wall_list = g.d_wall_list; a.wall_count = g.d_wall_count;
     return a;
 }
 
@@ -521,15 +643,35 @@ int launch_push_deposit(int isp, bool do_push) {
     if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
     if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
     int blocks = (int)((S.nslots + PUSH_SLOTS - 1) / PUSH_SLOTS);
-    if (do_push) HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
+    if (do_push) {
+        // the wall list holds at most every slot once
+        if (g.wall_cap < S.nslots) {
+            if (g.d_wall_list) (void)hipFree(g.d_wall_list);
+            g.d_wall_list = nullptr;
+            HIPCHK(hipMalloc(&g.d_wall_list, sizeof(int) * (size_t)S.capacity));
+            g.wall_cap = S.capacity;
+        }
+        if (!g.d_wall_count) HIPCHK(hipMalloc(&g.d_wall_count, 64 * sizeof(unsigned int)));
+        HIPCHK(hipMemsetAsync(g.d_wall_count, 0, sizeof(unsigned int), g.stream));
+        HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
+    }
+    const int wall_blocks = 512;
     if (g.particle_bits == 64) {
         PushArgs<double> a = make_push_args<double>(isp);
-        if (do_push) hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-        else hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        if (do_push) {
+            hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+            hipLaunchKernelGGL((k_push_walls<double>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        }
     } else {
         PushArgs<float> a = make_push_args<float>(isp);
-        if (do_push) hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-        else hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        if (do_push) {
+            hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+            hipLaunchKernelGGL((k_push_walls<float>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+        }
     }
     HIPCHK(hipGetLastError());
     if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream)); g.ev_push_pending[isp] = true; }

@@ -175,6 +175,9 @@ struct PicState {
     float *d_cg_r[2] = {nullptr, nullptr}, *d_cg_w[2] = {nullptr, nullptr}, *d_cg_s[2] = {nullptr, nullptr};
     float *d_cg_p = nullptr, *d_cg_x = nullptr;
     int cg_rpt = 1;                    // rows per thread of the CG kernels
+    int* d_wall_list = nullptr;        // particles the push hands to the wall kernel
+    unsigned int* d_wall_count = nullptr;
+    int64_t wall_cap = 0;
     int learn_maxit = 400;             // FCG iterations per inner solve before handing over
     int last_learned_iters = 0, last_fallback = 0;
     // kernel statistics of the last pic_step: [0] ms in CG iteration kernels (chunk events),
