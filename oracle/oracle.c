@@ -519,14 +519,16 @@ void or_stiffness_apply(int nnx, int nny, const double* xs, const double* ys, in
 #define OR_PI 3.141592653589793238
 #define OR_MINREAL 1.0e-15
 
-/* Selection: one Philox word of counter (slot, step, tag, 0) per live slot against
- * floor(CollisionRatio * 2^32). Writes the selected slots in slot order. */
+/* Selection: slot s of a live particle is selected when word s % 4 of the Philox block
+ * with counter (s / 4, step, tag, 0) is below floor(CollisionRatio * 2^32) (a Bernoulli
+ * trial per particle standing in for SelectParticle's count + shuffle, MCCPublic.F90:113-171).
+ * Writes the selected slots in slot order. */
 int64_t or_mcc_select(int64_t n, const int* cell, uint32_t threshold, uint64_t seed, uint32_t step, uint32_t tag, int* out) {
     int64_t k = 0;
     for (int64_t s = 0; s < n; ++s) {
         if (cell[s] < 0) continue;
-        or_p4 o = or_philox((uint32_t)s, step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
-        if (o.v[0] < threshold) out[k++] = (int)s;
+        or_p4 o = or_philox((uint32_t)(s >> 2), step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+        if (o.v[s & 3] < threshold) out[k++] = (int)s;
     }
     return k;
 }

@@ -588,6 +588,83 @@ int run_pcg_fp64(bool) {
 // FP32 CG on the Jacobi-scaled correction equation inside FP64 iterative refinement; the
 // outer residual is recomputed in FP64 with the unscaled operator and decides convergence
 // at pcg_tol.
+// FP64 residual r = b - A x with rr = ||r||^2, bb = ||b||^2 left at d_scal[S_NSCAL], [S_NSCAL+1]
+template <typename Op>
+static void resid64_launch(int n, const Op& A, const double* b, const double* x, double* r) {
+    int nb = pcg_blocks(n);
+    hipLaunchKernelGGL((k_resid64<Op>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, A, b, x, r, g.d_partials, g.pcg_pstride);
+    hipLaunchKernelGGL(k_sum2, dim3(1), dim3(PCG_BLOCK), 0, g.stream, nb, g.d_partials, g.pcg_pstride, g.d_scal + S_NSCAL);
+}
+
+// Iterative refinement with the learned FCG as the inner solver. A batch of passes (as many
+// as the last learned solve needed, each with the FCG iterations its counterpart needed) is
+// launched with no host synchronisation in between: every pass tests the FP64 residual
+// before it on the device and does nothing once rr <= tol^2 bb (or after a failed pass).
+// One synchronisation per batch reads the records; a pass that ran out of iterations gets
+// two more next time, and further batches of one pass follow until the tolerance is met.
+// *fallback: the network failed (NaN, breakdown) or stagnated; the caller finishes with CG.
+static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, int* outer, double* rr, double* bb,
+                                  bool* fallback) {
+    const double tol2 = g.pcg_tol * g.pcg_tol;
+    const double inner = learn_inner_tol();
+    const int cap = std::max(1, std::min(g.learn_maxit, 64));
+    *fallback = false;
+    int batch = std::min(std::max(g.lp_passes, 1), FCG_MAX_PASS);
+    bool first = true;
+    int done_passes = 0;
+    for (;;) {
+        HIPCHK(hipEventRecord(g.ev[12], g.stream));
+        resid64_launch(n, A64, g.d_bn, g.d_xn, g.d_rn);
+        int K[FCG_MAX_PASS];
+        for (int p = 0; p < batch; ++p) {
+            const int idx = std::min(done_passes + p, FCG_MAX_PASS - 1);
+            K[p] = std::min(cap, std::max(1, g.lp_iters[idx]) + (p == batch - 1 ? 1 : 0));
+            int rc = learn_fcg_pass(n, g.d_rn, inner, K[p], tol2, g.d_scal + S_NSCAL, first && p == 0 ? 1 : 0, p,
+                                    g.d_xn);
+            if (rc) return rc;
+            resid64_launch(n, A64, g.d_bn, g.d_xn, g.d_rn);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(g.ev[13], g.stream));
+        HIPCHK(hipMemcpyAsync(g.h_scal_pinned, g.d_scal, 64 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        const double* h = g.h_scal_pinned;
+        *rr = h[S_NSCAL]; *bb = h[S_NSCAL + 1];
+        int its = 0;
+        for (int p = 0; p < batch; ++p) {
+            const double* rec = h + FCG_REC_BASE + FCG_RECW * p;
+            if (rec[4] != 0.0) break;                       // skipped: converged before (or failed earlier)
+            const int it = (int)rec[0];
+            const int idx = std::min(done_passes, FCG_MAX_PASS - 1);
+            its += it; *total += it; (*outer)++; done_passes++;
+            g.last_learned_iters += it;
+            if (learn_trace_on())
+                fprintf(stderr, "learned pass %d: %d of %d iterations, done %g, err %g, rr/rr0 %.3e\n", done_passes, it,
+                        K[p], rec[1], rec[2], rec[3]);
+            if (rec[2] == 2.0 || rec[2] == 3.0) { *fallback = true; break; }
+            if (rec[1] == 0.0) {
+                // out of iterations: stagnation if the residual norm fell less than 10x
+                if (K[p] >= 4 && !(rec[3] < 1e-2)) { *fallback = true; break; }
+                g.lp_iters[idx] = K[p] + 2;
+            } else {
+                g.lp_iters[idx] = std::max(1, it);
+            }
+        }
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, g.ev[12], g.ev[13]));
+        g.kstat[4] += ms; g.kstat[5] += its;
+        learn_note_iterations(its);
+        if (*fallback) return PIC_OK;
+        if (*bb == 0.0 || *rr <= tol2 * *bb) { g.lp_passes = std::max(1, done_passes); return PIC_OK; }
+        if (*outer >= 30 || *total >= g.pcg_maxit) {
+            g.last_iter = *total; g.last_outer = *outer; g.last_relres = std::sqrt(*rr / *bb);
+            return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
+        }
+        first = false;
+        batch = 1;
+    }
+}
+
 int run_pcg_mixed() {
     int n = g.mesh.nnodes;
     int nb256 = (n + 255) / 256;
@@ -596,10 +673,23 @@ int run_pcg_mixed() {
     int total = 0, outer = 0;
     double rr = 0, bb = 0;
     const double inner_tol = 1e-4;
-    // solver_mode 2: the learned FCG is the inner solver until it stagnates; the plain CG
-    // then finishes this solve. The FP64 outer residual judges convergence either way.
-    bool use_learned = g.solver_mode == 2 && learn_ready();
     g.last_learned_iters = 0; g.last_fallback = 0;
+    // solver_mode 2: the learned FCG is the inner solver; when it fails or stagnates the plain
+    // CG finishes this solve. The FP64 outer residual judges convergence either way.
+    if (g.solver_mode == 2 && learn_ready()) {
+        bool fallback = false;
+        int rc = run_learned_refinement(n, A64, &total, &outer, &rr, &bb, &fallback);
+        if (rc) return rc;
+        if (!fallback) {
+            hipLaunchKernelGGL(k_xn_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_xn, g.d_phi);
+            HIPCHK(hipGetLastError());
+            g.last_iter = total; g.last_outer = outer;
+            g.last_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            return PIC_OK;
+        }
+        g.last_fallback = 1;
+        learn_count_fallback();
+    }
     for (;;) {
         int rc = resid64(n, A64, g.d_bn, g.d_xn, g.d_rn, &rr, &bb);
         if (rc) return rc;
@@ -610,19 +700,8 @@ int run_pcg_mixed() {
             return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
         }
         int it = 0, ierr = 0;
-        if (use_learned) {
-            int stagnated = 0;
-            rc = learn_fcg_run(n, g.d_rn, learn_inner_tol(), std::min(g.learn_maxit, g.pcg_maxit - total), &it, &ierr,
-                               &stagnated);
-            if (rc) return rc;
-            g.last_learned_iters += it;
-            if (stagnated) { use_learned = false; g.last_fallback = 1; learn_count_fallback(); }
-            if (ierr == 2 || ierr == 3) ierr = 0;           // handed over to the plain CG
-            if (ierr == 1) { use_learned = false; ierr = 0; }
-        } else {
-            rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
-            if (rc) return rc;
-        }
+        rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_cg_apply, dim3(nb256), dim3(256), 0, g.stream, n, g.d_sinv, g.d_cg_x, g.d_xn);
         total += it; outer++;
         if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "inner PCG breakdown");

@@ -187,54 +187,74 @@ struct MgArgs {
 #define MGD_XJ (2 * MGD_CJ + 3)
 #define MGD_SI (2 * MGD_CI + 1)
 #define MGD_SJ (2 * MGD_CJ + 1)
-__global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
-    __shared__ float xs[MGD_XI * MGD_XJ];
-    __shared__ float rs[MGD_SI * MGD_SJ];
-    __shared__ float ss[MGD_SI * MGD_SJ];
-    if (a.done && *a.done != 0.0) return;
+#define MGD_NX (MGD_XI * MGD_XJ)
+#define MGD_NS (MGD_SI * MGD_SJ)
+#define MGD_XIT ((MGD_NX + 255) / 256)
+#define MGD_SIT ((MGD_NS + 255) / 256)
+// Every global load of the kernel (operator rows of the residual nodes, r and x / 1/diag on
+// the staged nodes) is issued before the first barrier, so the block waits for memory once.
+template <bool JAC>
+__device__ inline void mg_down_body(const MgArgs& a, float* xs, float* rx, float* ss) {
     const int b = blockIdx.z;
     const int I0 = blockIdx.y * MGD_CI, J0 = blockIdx.x * MGD_CJ;
     const int nf = a.Hf * a.Wf;
     const float* r = a.r + (int64_t)b * a.r_bs;
     float* x = a.x + (int64_t)b * a.fs;
     const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;
-    for (int k = threadIdx.x; k < MGD_XI * MGD_XJ; k += 256) {
+    float dv[MGD_SIT][9];
+#pragma unroll
+    for (int it = 0; it < MGD_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
+        const bool in = k < MGD_NS && fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf;
+        const int f = in ? fi * a.Wf + fj : 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) dv[it][t] = a.dia[(int64_t)t * nf + f];
+        if (!in) dv[it][4] = 0.0f;                      // no residual outside the grid
+    }
+    float xv[MGD_XIT], rv[MGD_XIT];
+#pragma unroll
+    for (int it = 0; it < MGD_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
         const int u = k / MGD_XJ, v = k - u * MGD_XJ;
         const int fi = fi0 + u, fj = fj0 + v;
-        float val = 0.0f;
-        if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
-            const int f = fi * a.Wf + fj;
-            val = a.jacobi ? a.omega * a.dinv[f] * r[f] : x[f];
-        }
-        xs[k] = val;
+        const bool in = k < MGD_NX && fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf;
+        const int f = in ? fi * a.Wf + fj : 0;
+        rv[it] = r[f];
+        xv[it] = JAC ? a.dinv[f] : x[f];
+        if (!in) { rv[it] = 0.0f; xv[it] = 0.0f; }
     }
-    for (int k = threadIdx.x; k < MGD_SI * MGD_SJ; k += 256) {
-        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
-        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
-        rs[k] = (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) ? r[fi * a.Wf + fj] : 0.0f;
-    }
-    __syncthreads();
-    if (a.jacobi) {                                  // owned fine nodes [2I0, 2I0+2CI) x [2J0, 2J0+2CJ)
-        for (int k = threadIdx.x; k < 4 * MGD_CI * MGD_CJ; k += 256) {
-            const int u = k / (2 * MGD_CJ), v = k - u * (2 * MGD_CJ);
-            const int fi = 2 * I0 + u, fj = 2 * J0 + v;
-            if (fi < a.Hf && fj < a.Wf) x[fi * a.Wf + fj] = xs[(u + 2) * MGD_XJ + (v + 2)];
-        }
-    }
-    for (int k = threadIdx.x; k < MGD_SI * MGD_SJ; k += 256) {
-        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
-        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
-        float sv = 0.0f;
-        if (fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf) {
-            const int f = fi * a.Wf + fj;
-            if (a.dia[4 * (int64_t)nf + f] != 0.0f) {
-                sv = rs[k];
+    if (a.done && *a.done != 0.0) return;
 #pragma unroll
-                for (int t = 0; t < 9; ++t)
-                    sv = sv - a.dia[(int64_t)t * nf + f] * xs[(u + t / 3) * MGD_XJ + (v + t % 3)];
+    for (int it = 0; it < MGD_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGD_NX) {
+            const float val = JAC ? a.omega * xv[it] * rv[it] : xv[it];
+            xs[k] = val;
+            rx[k] = rv[it];
+            if (JAC) {                                   // owned fine nodes [2I0, 2I0+2CI) x [2J0, 2J0+2CJ)
+                const int u = k / MGD_XJ - 2, v = k - (k / MGD_XJ) * MGD_XJ - 2;
+                const int fi = 2 * I0 + u, fj = 2 * J0 + v;
+                if (u >= 0 && u < 2 * MGD_CI && v >= 0 && v < 2 * MGD_CJ && fi < a.Hf && fj < a.Wf)
+                    x[fi * a.Wf + fj] = val;
             }
         }
-        ss[k] = sv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MGD_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGD_NS) {
+            const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+            float sv = 0.0f;
+            if (dv[it][4] != 0.0f) {
+                sv = rx[(u + 1) * MGD_XJ + (v + 1)];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) sv = sv - dv[it][t] * xs[(u + t / 3) * MGD_XJ + (v + t % 3)];
+            }
+            ss[k] = sv;
+        }
     }
     __syncthreads();
     const int ti = threadIdx.x / MGD_CJ, tj = threadIdx.x - (threadIdx.x / MGD_CJ) * MGD_CJ;
@@ -250,6 +270,14 @@ __global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
+    __shared__ float xs[MGD_NX];
+    __shared__ float rx[MGD_NX];
+    __shared__ float ss[MGD_NS];
+    if (a.jacobi) mg_down_body<true>(a, xs, rx, ss);
+    else mg_down_body<false>(a, xs, rx, ss);
+}
+
 // Up, tiled: a block owns FI x FJ fine nodes; it stages ec on the covering coarse nodes and x
 // on the fine halo in LDS, forms x2 = x + P ec on the halo, then (Jacobi levels)
 // y = x2 + omega/diag (r - A x2), or (learned levels) y = x2 and s2 = r - A x2.
@@ -257,10 +285,13 @@ __global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
 #define MGU_FJ 64
 #define MGU_CI (MGU_FI / 2 + 2)
 #define MGU_CJ (MGU_FJ / 2 + 2)
+#define MGU_NE (MGU_CI * MGU_CJ)
+#define MGU_NX ((MGU_FI + 2) * (MGU_FJ + 2))
+#define MGU_XIT ((MGU_NX + 255) / 256)
 __global__ __launch_bounds__(256) void k_mg_up(MgArgs a) {
-    __shared__ float es[MGU_CI * MGU_CJ];
-    __shared__ float x2s[(MGU_FI + 2) * (MGU_FJ + 2)];
-    if (a.done && *a.done != 0.0) return;
+    static_assert(MGU_NE <= 256 && MGU_FI * MGU_FJ == 256, "k_mg_up tile");
+    __shared__ float es[MGU_NE];
+    __shared__ float x2s[MGU_NX];
     const int b = blockIdx.z;
     const int fi0 = blockIdx.y * MGU_FI, fj0 = blockIdx.x * MGU_FJ;
     const int nf = a.Hf * a.Wf;
@@ -269,46 +300,72 @@ __global__ __launch_bounds__(256) void k_mg_up(MgArgs a) {
     const float* ec = a.ec + (int64_t)b * a.cs;
     float* y = a.y + (int64_t)b * a.y_bs;
     const int ci0 = (fi0 - 1) >> 1, cj0 = (fj0 - 1) >> 1;    // floor
-    for (int k = threadIdx.x; k < MGU_CI * MGU_CJ; k += 256) {
+    // all global loads first: coarse correction, halo x and its Dirichlet marks, the own row
+    float ev;
+    {
+        const int k = threadIdx.x;
         const int u = k / MGU_CJ, v = k - u * MGU_CJ;
         const int I = ci0 + u, J = cj0 + v;
-        es[k] = (I >= 0 && I < a.Hc && J >= 0 && J < a.Wc) ? ec[I * a.Wc + J] : 0.0f;
+        const bool in = k < MGU_NE && I >= 0 && I < a.Hc && J >= 0 && J < a.Wc;
+        ev = ec[in ? I * a.Wc + J : 0];
+        if (!in) ev = 0.0f;
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < (MGU_FI + 2) * (MGU_FJ + 2); k += 256) {
+    float hx[MGU_XIT], hd[MGU_XIT];
+#pragma unroll
+    for (int it = 0; it < MGU_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
         const int u = k / (MGU_FJ + 2), v = k - u * (MGU_FJ + 2);
         const int gi = fi0 - 1 + u, gj = fj0 - 1 + v;
-        float val = 0.0f;
-        if (gi >= 0 && gi < a.Hf && gj >= 0 && gj < a.Wf) {
-            const int g = gi * a.Wf + gj;
-            if (a.dia[4 * (int64_t)nf + g] != 0.0f) {
+        const bool in = k < MGU_NX && gi >= 0 && gi < a.Hf && gj >= 0 && gj < a.Wf;
+        const int g = in ? gi * a.Wf + gj : 0;
+        hd[it] = a.dia[4 * (int64_t)nf + g];
+        hx[it] = x[g];
+        if (!in) hd[it] = 0.0f;
+    }
+    const int ti = threadIdx.x / MGU_FJ, tj = threadIdx.x - (threadIdx.x / MGU_FJ) * MGU_FJ;
+    const int fi = fi0 + ti, fj = fj0 + tj;
+    const bool own = fi < a.Hf && fj < a.Wf;
+    const int f = own ? fi * a.Wf + fj : 0;
+    float dv[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) dv[t] = a.dia[(int64_t)t * nf + f];
+    const float rf = r[f];
+    const float di = a.jacobi ? a.dinv[f] : 0.0f;
+    if (a.done && *a.done != 0.0) return;
+    if (threadIdx.x < MGU_NE) es[threadIdx.x] = ev;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MGU_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGU_NX) {
+            const int u = k / (MGU_FJ + 2), v = k - u * (MGU_FJ + 2);
+            const int gi = fi0 - 1 + u, gj = fj0 - 1 + v;
+            float val = 0.0f;
+            if (hd[it] != 0.0f) {
                 const int i0 = gi >> 1, j0 = gj >> 1;
                 const bool oi = gi & 1, oj = gj & 1;
                 float pe = 0.0f;
                 for (int p = 0; p < (oi ? 2 : 1); ++p)
                     for (int q = 0; q < (oj ? 2 : 1); ++q)
                         pe = pe + (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * es[(i0 + p - ci0) * MGU_CJ + (j0 + q - cj0)];
-                val = x[g] + pe;
+                val = hx[it] + pe;
             }
+            x2s[k] = val;
         }
-        x2s[k] = val;
     }
     __syncthreads();
-    const int ti = threadIdx.x / MGU_FJ, tj = threadIdx.x - (threadIdx.x / MGU_FJ) * MGU_FJ;
-    const int fi = fi0 + ti, fj = fj0 + tj;
-    if (fi >= a.Hf || fj >= a.Wf) return;
-    const int f = fi * a.Wf + fj;
+    if (!own) return;
     const float xc = x2s[(ti + 1) * (MGU_FJ + 2) + (tj + 1)];
-    if (a.dia[4 * (int64_t)nf + f] == 0.0f) {
+    if (dv[4] == 0.0f) {
         y[f] = 0.0f;
         if (!a.jacobi) a.s2[(int64_t)b * a.fs + f] = 0.0f;
         return;
     }
-    float sres = r[f];
+    float sres = rf;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) sres = sres - a.dia[(int64_t)t * nf + f] * x2s[(ti + t / 3) * (MGU_FJ + 2) + (tj + t % 3)];
+    for (int t = 0; t < 9; ++t) sres = sres - dv[t] * x2s[(ti + t / 3) * (MGU_FJ + 2) + (tj + t % 3)];
     if (a.jacobi) {
-        y[f] = xc + a.omega * a.dinv[f] * sres;
+        y[f] = xc + a.omega * di * sres;
     } else {
         y[f] = xc;
         a.s2[(int64_t)b * a.fs + f] = sres;
@@ -330,16 +387,24 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, cons
 
 // Levels lc .. L-1 (Jacobi smoothers, all <= 33 x 33 nodes) in ONE workgroup per batch entry:
 // the down sweeps, the exact coarsest solve and the up sweeps of the V-cycle, phase by phase
-// with block barriers, every vector in LDS (operators read through the cache). Replaces ~2
-// launches per level whose run time is pure latency at these sizes.
+// with block barriers. The operators of these levels (nine diagonals, 1/diag, and the dense
+// coarsest inverse when it fits) are packed on the host in the LDS order and copied in with
+// a few float4 loads per thread at the start, so after that first wait every phase works
+// from LDS. Replaces ~2 launches per level whose run time is pure latency at these sizes.
 #define MG_CB_THREADS 1024
 #define MG_CB_MAXN 1089
+#define MG_CB_OPS_IT 5                               // float4 loads per thread for the operators
+#define MG_CB_OPS_CAP (MG_CB_OPS_IT * MG_CB_THREADS * 4)
+#define MG_CB_RIT ((MG_CB_MAXN + MG_CB_THREADS - 1) / MG_CB_THREADS)
+static_assert(MG_CB_RIT * MG_CB_THREADS >= LN_MAX_COARSE, "coarse workgroup stages the coarsest level too");
 struct MgCoarseArgs {
     int lc, L;
     int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
-    int off[LN_MAX_LEVELS];                      // LDS offset (floats) of each level's 4 vectors r, x, y, t
-    const float* dia[LN_MAX_LEVELS];
-    const float* dinv[LN_MAX_LEVELS];
+    int voff[LN_MAX_LEVELS];                     // LDS offset (floats) of each level's vectors r, x, y, t
+    int ooff[LN_MAX_LEVELS];                     // LDS offset of each level's dia [9][n] then dinv [n]
+    int ioff;                                    // LDS offset of the coarsest inverse, -1: read from inv
+    int nops4;                                   // packed operator float4s (<= MG_CB_OPS_IT * threads)
+    const float4* ops;
     const float* rin; int64_t rin_bs;            // residual of level lc (global)
     float* yout; int64_t yout_bs;                // correction of level lc (global)
     const float* inv; float omega;
@@ -348,33 +413,52 @@ struct MgCoarseArgs {
 
 __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs a) {
     extern __shared__ float lds[];
-    if (a.done && *a.done != 0.0) return;
     const int b = blockIdx.y;
-    {
-        const int n = a.H[a.lc] * a.W[a.lc];
-        float* r = lds + a.off[a.lc];
-        for (int f = threadIdx.x; f < n; f += MG_CB_THREADS) r[f] = a.rin[(int64_t)b * a.rin_bs + f];
+    float4 ov[MG_CB_OPS_IT];
+#pragma unroll
+    for (int q = 0; q < MG_CB_OPS_IT; ++q) {
+        const int k = threadIdx.x + q * MG_CB_THREADS;
+        ov[q] = a.ops[k < a.nops4 ? k : 0];
+    }
+    const int n0 = a.H[a.lc] * a.W[a.lc];
+    float rv[MG_CB_RIT];
+#pragma unroll
+    for (int q = 0; q < MG_CB_RIT; ++q) {
+        const int k = threadIdx.x + q * MG_CB_THREADS;
+        rv[q] = a.rin[(int64_t)b * a.rin_bs + (k < n0 ? k : 0)];
+    }
+    if (a.done && *a.done != 0.0) return;
+#pragma unroll
+    for (int q = 0; q < MG_CB_OPS_IT; ++q) {
+        const int k = threadIdx.x + q * MG_CB_THREADS;
+        if (k < a.nops4) reinterpret_cast<float4*>(lds)[k] = ov[q];
+    }
+#pragma unroll
+    for (int q = 0; q < MG_CB_RIT; ++q) {
+        const int k = threadIdx.x + q * MG_CB_THREADS;
+        if (k < n0) lds[a.voff[a.lc] + k] = rv[q];
     }
     __syncthreads();
     for (int l = a.lc; l < a.L - 1; ++l) {
         const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
         const int nf = Hf * Wf, nc = Hc * Wc;
-        float* r = lds + a.off[l];
+        float* r = lds + a.voff[l];
         float* x = r + nf;
         float* s = r + 3 * nf;
-        float* rc = lds + a.off[l + 1];
-        const float* dia = a.dia[l];
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) x[f] = a.omega * a.dinv[l][f] * r[f];
+        float* rc = lds + a.voff[l + 1];
+        const float* dia = lds + a.ooff[l];
+        const float* dinv = dia + 9 * nf;
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) x[f] = a.omega * dinv[f] * r[f];
         __syncthreads();
         for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
             const int fi = f / Wf, fj = f - (f / Wf) * Wf;
             float v = 0.0f;
             if (dia[4 * nf + f] != 0.0f) {
                 v = r[f];
+#pragma unroll
                 for (int t = 0; t < 9; ++t) {
                     const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
-                    if (gi < 0 || gi >= Hf || gj < 0 || gj >= Wf) continue;
-                    v = v - dia[t * nf + f] * x[gi * Wf + gj];
+                    if (gi >= 0 && gi < Hf && gj >= 0 && gj < Wf) v = v - dia[t * nf + f] * x[gi * Wf + gj];
                 }
             }
             s[f] = v;
@@ -383,11 +467,13 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
         for (int c = threadIdx.x; c < nc; c += MG_CB_THREADS) {
             const int I = c / Wc, J = c - (c / Wc) * Wc;
             float acc = 0.0f;
+#pragma unroll
             for (int u = -1; u <= 1; ++u)
+#pragma unroll
                 for (int v = -1; v <= 1; ++v) {
                     const int fi = 2 * I + u, fj = 2 * J + v;
-                    if (fi < 0 || fi >= Hf || fj < 0 || fj >= Wf) continue;
-                    acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * s[fi * Wf + fj];
+                    if (fi >= 0 && fi < Hf && fj >= 0 && fj < Wf)
+                        acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * s[fi * Wf + fj];
                 }
             rc[c] = acc;
         }
@@ -396,11 +482,12 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
     {
         const int l = a.L - 1;
         const int n = a.H[l] * a.W[l];
-        const float* r = lds + a.off[l];
-        float* y = lds + a.off[l] + 2 * n;
+        const float* r = lds + a.voff[l];
+        float* y = lds + a.voff[l] + 2 * n;
+        const float* inv = a.ioff >= 0 ? lds + a.ioff : a.inv;
         for (int i = threadIdx.x; i < n; i += MG_CB_THREADS) {
             float sacc = 0.0f;
-            for (int j = 0; j < n; ++j) sacc = sacc + a.inv[i * n + j] * r[j];
+            for (int j = 0; j < n; ++j) sacc = sacc + inv[i * n + j] * r[j];
             y[i] = sacc;
         }
         __syncthreads();
@@ -408,12 +495,13 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
     for (int l = a.L - 2; l >= a.lc; --l) {
         const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
         const int nf = Hf * Wf;
-        const float* r = lds + a.off[l];
+        const float* r = lds + a.voff[l];
         const float* x = r + nf;
-        float* y = lds + a.off[l] + 2 * nf;
-        float* x2 = lds + a.off[l] + 3 * nf;
-        const float* yc = lds + a.off[l + 1] + 2 * Hc * Wc;
-        const float* dia = a.dia[l];
+        float* y = lds + a.voff[l] + 2 * nf;
+        float* x2 = lds + a.voff[l] + 3 * nf;
+        const float* yc = lds + a.voff[l + 1] + 2 * Hc * Wc;
+        const float* dia = lds + a.ooff[l];
+        const float* dinv = dia + 9 * nf;
         for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
             const int gi = f / Wf, gj = f - (f / Wf) * Wf;
             float v = 0.0f;
@@ -440,12 +528,12 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
             float v = 0.0f;
             if (dia[4 * nf + f] != 0.0f) {
                 float sres = r[f];
+#pragma unroll
                 for (int t = 0; t < 9; ++t) {
                     const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
-                    if (gi < 0 || gi >= Hf || gj < 0 || gj >= Wf) continue;
-                    sres = sres - dia[t * nf + f] * x2[gi * Wf + gj];
+                    if (gi >= 0 && gi < Hf && gj >= 0 && gj < Wf) sres = sres - dia[t * nf + f] * x2[gi * Wf + gj];
                 }
-                v = x2[f] + a.omega * a.dinv[l][f] * sres;
+                v = x2[f] + a.omega * dinv[f] * sres;
             }
             y[f] = v;
         }
@@ -453,7 +541,7 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
     }
     {
         const int n = a.H[a.lc] * a.W[a.lc];
-        const float* y = lds + a.off[a.lc] + 2 * n;
+        const float* y = lds + a.voff[a.lc] + 2 * n;
         for (int f = threadIdx.x; f < n; f += MG_CB_THREADS) a.yout[(int64_t)b * a.yout_bs + f] = y[f];
     }
 }
@@ -463,26 +551,45 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
 // SMI x SMJ nodes and stages Dh v on the tile with a halo of 2 in LDS.
 #define SMI 4
 #define SMJ 64
+#define SM_NU ((SMI + 4) * (SMJ + 4))
+#define SM_UIT ((SM_NU + 255) / 256)
 __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k81, const float* dh, const float* v,
                                                     int64_t v_bs, float* out, int64_t o_bs, int add, const double* done) {
-    __shared__ float us[(SMI + 4) * (SMJ + 4)];
+    static_assert(SMI * SMJ == 256, "k_smooth_k81 tile");
+    __shared__ float us[SM_NU];
     __shared__ float kw[81];
-    if (done && *done != 0.0) return;
     const float* vb = v + (int64_t)blockIdx.z * v_bs;
     float* ob = out + (int64_t)blockIdx.z * o_bs;
     const int i0 = blockIdx.y * SMI, j0 = blockIdx.x * SMJ;
-    if (threadIdx.x < 81) kw[threadIdx.x] = k81[threadIdx.x];
-    for (int k = threadIdx.x; k < (SMI + 4) * (SMJ + 4); k += 256) {
+    // all global loads first: weights, the scaled halo, the own node's scale and output
+    const float kv = k81[threadIdx.x < 81 ? threadIdx.x : 0];
+    float hd[SM_UIT], hv[SM_UIT];
+#pragma unroll
+    for (int it = 0; it < SM_UIT; ++it) {
+        const int k = threadIdx.x + it * 256;
         const int u = k / (SMJ + 4), w = k - u * (SMJ + 4);
         const int ii = i0 - 2 + u, jj = j0 - 2 + w;
-        us[k] = (ii >= 0 && ii < H && jj >= 0 && jj < W) ? dh[ii * W + jj] * vb[ii * W + jj] : 0.0f;
+        const bool in = k < SM_NU && ii >= 0 && ii < H && jj >= 0 && jj < W;
+        const int q = in ? ii * W + jj : 0;
+        hd[it] = dh[q];
+        hv[it] = vb[q];
+        if (!in) { hd[it] = 0.0f; hv[it] = 0.0f; }
     }
-    __syncthreads();
     const int ti = threadIdx.x / SMJ, tj = threadIdx.x - (threadIdx.x / SMJ) * SMJ;
     const int i = i0 + ti, j = j0 + tj;
-    if (i >= H || j >= W) return;
-    const int p = i * W + j;
+    const bool own = i < H && j < W;
+    const int p = own ? i * W + j : 0;
     const float dp = dh[p];
+    const float o0 = add ? ob[p] : 0.0f;
+    if (done && *done != 0.0) return;
+    if (threadIdx.x < 81) kw[threadIdx.x] = kv;
+#pragma unroll
+    for (int it = 0; it < SM_UIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < SM_NU) us[k] = hd[it] * hv[it];
+    }
+    __syncthreads();
+    if (!own) return;
     float s = 0.0f;
     if (dp != 0.0f) {
 #pragma unroll
@@ -495,7 +602,7 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
         }
         s = dp * s;
     }
-    ob[p] = add ? ob[p] + s : s;
+    ob[p] = add ? o0 + s : s;
 }
 
 // ---------------------------------------------------------------- weight-gradient correlation (MFMA)
@@ -786,9 +893,12 @@ __global__ __launch_bounds__(256) void k_harvest(int n, Dia8 A, const float* xf,
 // ---------------------------------------------------------------- FCG(1) on the scaled system
 // scal slots (shared with the CG): NORMRHS, DONE, ITER, IERR, TOL2, MAXIT; pq in GAM0/GAM1
 enum { F_S_NORMRHS = 4, F_S_NORMR = 5, F_S_DONE = 6, F_S_ITER = 7, F_S_IERR = 8, F_S_TOL2 = 9, F_S_MAXIT = 10,
-       F_S_PQ0 = 12, F_S_PQ1 = 13, F_S_KCUR = 20, F_S_KNEXT = 21 };
+       F_S_PQ0 = 12, F_S_PQ1 = 13, F_S_KCUR = 20, F_S_KNEXT = 21, F_S_FAIL = 22 };
 // The iteration index lives on the device (KNEXT advanced by the check kernel, KCUR read by
-// the update), so every chunk of iterations is the same launch sequence.
+// the update), so every iteration is the same launch sequence. A refinement pass leaves a
+// record at FCG_REC_BASE + FCG_RECW * slot: iterations, done, error code, last ||r||^2/||r0||^2,
+// skipped (the outer residual met its tolerance before the pass, or an earlier pass failed).
+// FAIL is set by a pass that ends in a NaN or a breakdown.
 #define FCG_BLOCK 256
 
 __device__ inline void fcg_block_partials(int nv, double* v, double* part, int pstride) {
@@ -818,9 +928,13 @@ __device__ inline void fcg_reduce(int nv, const double* part, int pstride, int n
     __syncthreads();
 }
 
-// r = D^-1/2 b64, x = 0; partial r.r (region 0)
+// r = D^-1/2 b64, x = 0; partial r.r (region 0). The pass is skipped (DONE at once) when the
+// FP64 residual before it meets the outer tolerance, rr <= otol2 * bb (outer_rrbb = rr, bb),
+// or when an earlier pass of this solve failed; first = 1 starts a solve (clears FAIL).
 __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_init(int n, const double* b64, const double* sinv, float* r, float* x,
-                                                        double* scal, double* part, int pstride) {
+                                                        double* scal, double* part, int pstride, double tol2,
+                                                        double maxit, double otol2, const double* outer_rrbb,
+                                                        int first, int slot) {
     int i = blockIdx.x * FCG_BLOCK + threadIdx.x;
     double v[1] = {0.0};
     if (i < n) {
@@ -830,7 +944,39 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_init(int n, const double* b64
     }
     fcg_block_partials(1, v, part, pstride);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        scal[F_S_DONE] = 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0; scal[F_S_KNEXT] = 0.0;
+        const double orr = outer_rrbb[0], obb = outer_rrbb[1];
+        const bool failed = !first && scal[F_S_FAIL] != 0.0;
+        const bool skip = failed || obb == 0.0 || orr <= otol2 * obb;
+        if (first) scal[F_S_FAIL] = 0.0;
+        scal[F_S_DONE] = skip ? 1.0 : 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0; scal[F_S_KNEXT] = 0.0;
+        scal[F_S_TOL2] = tol2; scal[F_S_MAXIT] = maxit;
+        scal[F_S_NORMR] = 0.0; scal[F_S_NORMRHS] = 0.0;
+        scal[FCG_REC_BASE + FCG_RECW * slot + 4] = skip ? 1.0 : 0.0;
+    }
+}
+
+// end of a pass: xn += D^-1/2 x (zero when skipped); block 0 writes the pass record. A pass
+// that used its whole budget is still recorded as done when the residual of its last update
+// (region-0 partials) meets the inner tolerance.
+__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_finish(int n, int nb, const double* sinv, const float* x, double* xn,
+                                                          double* scal, const double* part, int pstride, int slot) {
+    int i = blockIdx.x * FCG_BLOCK + threadIdx.x;
+    if (i < n) xn[i] = xn[i] + sinv[i] * (double)x[i];
+    if (blockIdx.x != 0) return;
+    double rr;
+    fcg_reduce(1, part, pstride, nb, &rr);
+    if (threadIdx.x == 0) {
+        double* rec = scal + FCG_REC_BASE + FCG_RECW * slot;
+        const double ierr = scal[F_S_IERR];
+        const double rr0 = scal[F_S_NORMRHS];
+        double done = scal[F_S_DONE];
+        double ratio = rr0 > 0.0 ? scal[F_S_NORMR] / rr0 : 0.0;
+        if (done == 0.0 && rr0 > 0.0) {
+            ratio = rr / rr0;
+            if (ratio <= scal[F_S_TOL2]) done = 1.0;
+        }
+        rec[0] = scal[F_S_ITER]; rec[1] = done; rec[2] = ierr; rec[3] = ratio;
+        if (ierr == 2.0 || ierr == 3.0) scal[F_S_FAIL] = 1.0;
     }
 }
 
@@ -934,6 +1080,8 @@ struct LearnNet {
     MgLevel lv[LN_MAX_LEVELS];
     int C[LN_MAX_LEARNED] = {0};
     float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator
+    float* d_cops = nullptr; int cops_n = 0;         // coarse workgroup operators in its LDS order
+    int c_ooff[LN_MAX_LEVELS] = {0}, c_voff[LN_MAX_LEVELS] = {0}, c_ioff = -1;
     void* d_ops = nullptr;                           // one allocation: dia, dinv, dh of all levels
     // parameters (FP32, flat): per learned level W_in [9][C][16] then Q [C][C]
     int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
@@ -969,14 +1117,13 @@ struct LearnNet {
     bool trace = false;
     double inner_tol = 1e-3;                         // FCG tolerance per refinement pass (the FP64 outer
                                                      // residual decides convergence)
-    int chunk = 3;                                   // FCG iterations between host checks
 };
 static LearnNet ln;
 
 static int64_t ln_npix0() { return ln.lv[0].n; }
 
 static void free_learn() {
-    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
+    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_cops, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
                     ln.d_lpart, ln.d_lsum, ln.d_idx, ln.d_eab, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     ln = LearnNet();
@@ -1043,12 +1190,11 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
         MgCoarseArgs c;
         std::memset(&c, 0, sizeof(c));
         c.lc = lc; c.L = L;
-        int off = 0;
         for (int l = lc; l < L; ++l) {
             const MgLevel& v = ln.lv[l];
-            c.H[l] = v.H; c.W[l] = v.W; c.dia[l] = v.dia; c.dinv[l] = v.dinv;
-            c.off[l] = off; off += 4 * (int)v.n;
+            c.H[l] = v.H; c.W[l] = v.W; c.voff[l] = ln.c_voff[l]; c.ooff[l] = ln.c_ooff[l];
         }
+        c.ioff = ln.c_ioff; c.nops4 = ln.cops_n / 4; c.ops = (const float4*)ln.d_cops;
         c.rin = ln.lv[lc].r[run]; c.rin_bs = ln.lv[lc].n;
         c.yout = ln.lv[lc].y[run]; c.yout_bs = ln.lv[lc].n;
         c.inv = ln.d_inv; c.omega = LN_OMEGA; c.done = done;
@@ -1197,6 +1343,8 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     if (max_levels < 2 || max_levels > LN_MAX_LEVELS) return pic_fail(PIC_ERR_ARG, "levels must be 2..12");
     if (nch < 0 || nch > LN_MAX_LEARNED) return pic_fail(PIC_ERR_ARG, "0..4 learned smoother levels");
     free_learn();
+    g.lp_passes = 2;                                 // launch budget of the first learned solve
+    for (int k = 0; k < FCG_MAX_PASS; ++k) g.lp_iters[k] = 8;
     // level sizes: coarse node I <-> fine node 2I, down to about 3 x 3
     int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
     H[0] = g.mesh.nnx; W[0] = g.mesh.nny;
@@ -1208,12 +1356,6 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     ln.Ls = std::min(nch, L - 1);
     ln.lc = L - 1;                                   // levels of <= 33 x 33 nodes go to one workgroup
     while (ln.lc > std::max(ln.Ls, 1) && (int64_t)H[ln.lc - 1] * W[ln.lc - 1] <= MG_CB_MAXN) ln.lc--;
-    {
-        size_t lds = 0;
-        for (int l = ln.lc; l < L; ++l) lds += 4 * (size_t)H[l] * W[l] * sizeof(float);
-        if (lds > 64 * 1024) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse levels exceed the workgroup LDS");
-        ln.lc_lds = lds;
-    }
     for (int l = 0; l < ln.Ls; ++l) {
         int c = channels[l];
         if (c != 16 && c != 32 && c != 64) return pic_fail(PIC_ERR_ARG, "channels must be 16, 32 or 64");
@@ -1289,6 +1431,36 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     HIPCHK(hipMalloc(&ln.d_inv, invf.size() * sizeof(float)));
     HIPCHK(hipMemcpy(ln.d_inv, invf.data(), invf.size() * sizeof(float), hipMemcpyHostToDevice));
     ln.ncoarse = nc;
+    // the coarse workgroup's LDS: [dia 9n | dinv n] of levels lc..L-1, the coarsest inverse
+    // when it fits, then the vectors r, x, y, t of each level
+    {
+        std::vector<float> cp;
+        for (int l = ln.lc; l < L; ++l) {
+            const MgLevel& v = ln.lv[l];
+            std::vector<float> lvl(10 * v.n);
+            HIPCHK(hipMemcpy(lvl.data(), v.dia, 10 * v.n * sizeof(float), hipMemcpyDeviceToHost));
+            ln.c_ooff[l] = (int)cp.size();
+            cp.insert(cp.end(), lvl.begin(), lvl.end());
+        }
+        ln.c_ioff = -1;
+        if (cp.size() + invf.size() <= (size_t)MG_CB_OPS_CAP) {
+            ln.c_ioff = (int)cp.size();
+            cp.insert(cp.end(), invf.begin(), invf.end());
+        }
+        while (cp.size() % 4) cp.push_back(0.0f);
+        if (cp.size() > (size_t)MG_CB_OPS_CAP) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse operators exceed the workgroup staging");
+        ln.cops_n = (int)cp.size();
+        size_t off = cp.size();
+        for (int l = ln.lc; l < L; ++l) { ln.c_voff[l] = (int)off; off += 4 * (size_t)ln.lv[l].n; }
+        ln.lc_lds = off * sizeof(float);
+        int maxlds = 0;
+        HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, g.device));
+        if (ln.lc_lds > (size_t)maxlds) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse levels exceed the workgroup LDS");
+        HIPCHK(hipFuncSetAttribute((const void*)k_mg_coarse_block, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)ln.lc_lds));
+        HIPCHK(hipMalloc((void**)&ln.d_cops, cp.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(ln.d_cops, cp.data(), cp.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     // parameters
     int64_t o = 0;
     for (int l = 0; l < ln.Ls; ++l) { ln.off_in[l] = o; o += 9LL * ln.C[l] * 16; }
@@ -1334,7 +1506,6 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     const char* tr = std::getenv("PIC_FCG_TRACE");
     ln.trace = tr && tr[0] == '1';
     if (const char* e = std::getenv("PIC_LEARN_INNER_TOL")) { double v = std::atof(e); if (v > 0 && v < 1) ln.inner_tol = v; }
-    if (const char* e = std::getenv("PIC_FCG_CHUNK")) { int v = std::atoi(e); if (v >= 1 && v <= 16) ln.chunk = v; }
     ln.ready = true;
     return PIC_OK;
 }
@@ -1586,68 +1757,40 @@ int learn_eval(int npairs, double* mean_loss) {
 }
 
 // --------------------------------------------------------------- FCG solve
-// FCG(1) on the scaled correction equation A' y = D^-1/2 b64, network preconditioner.
-// Returns PIC_OK with *stagnated set when the residual stops decreasing (the caller then
-// continues with the plain CG).
-int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, int* ierr, int* stagnated) {
-    *stagnated = 0;
+// One refinement pass of FCG(1) on the scaled correction equation A' y = D^-1/2 b64 with the
+// network as preconditioner, launched without any host synchronisation: init (outer test on
+// outer_rrbb against otol2), K iterations (each: convergence test, V-cycle, A z + dots,
+// update; all no-ops once the pass is done), then xn += D^-1/2 y and the pass record in
+// record slot `slot`. The inner tolerance is inner_tol on the residual norm.
+int learn_fcg_pass(int n, const double* b64, double inner_tol, int K, double otol2, const double* outer_rrbb,
+                   int first, int slot, double* xn) {
     const int nb = (n + FCG_BLOCK - 1) / FCG_BLOCK;
     if (nb > g.pcg_pstride) return pic_fail(PIC_ERR_STATE, "FCG partial buffer too small");
+    if (slot < 0 || slot >= FCG_MAX_PASS) return pic_fail(PIC_ERR_ARG, "FCG pass record slot out of range");
     double* scal = g.d_scal;
     double* part = g.d_partials;
     const int ps = g.pcg_pstride;
-    double h2[2] = {tol * tol, (double)maxit};
-    HIPCHK(hipMemcpyAsync(scal + F_S_TOL2, h2, 2 * sizeof(double), hipMemcpyHostToDevice, g.stream));
     float* r = g.d_cg_r[0];
     float* x = g.d_cg_x;
     float* p = g.d_cg_p;
-    hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, b64, g.d_sinv, r, x, scal, part, ps);
-    HIPCHK(hipGetLastError());
-    Dia8 A = scaled_op();
-    const int CH = ln.chunk;
-    // one chunk: CH x (check, V-cycle, A z + dots, update); every argument is fixed
-    auto launch_chunk = [&]() -> int {
-        for (int c = 0; c < CH; ++c) {
-            hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, scal, part, ps);
-            int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
-            hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
-            hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, ln.z, ln.w, p, ln.q, x, r,
-                               scal, part, ps);
-        }
-        HIPCHK(hipGetLastError());
-        return PIC_OK;
-    };
-    int k = 0;
-    double prev_rr = -1.0;
-    int slow = 0;
-    for (;;) {
-        HIPCHK(hipEventRecord(g.ev[12], g.stream));
-        int rc = launch_chunk(); if (rc) return rc;
-        k += CH;
-        HIPCHK(hipEventRecord(g.ev[13], g.stream));
-        HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, 16 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-        HIPCHK(hipStreamSynchronize(g.stream));
-        float ms = 0.0f;
-        HIPCHK(hipEventElapsedTime(&ms, g.ev[12], g.ev[13]));
-        g.kstat[4] += ms; g.kstat[5] += CH;
-        if (ln.trace)
-            fprintf(stderr, "fcg k=%d rr/rr0=%.3e done=%g\n", k, g.h_scal_pinned[F_S_NORMR] / g.h_scal_pinned[F_S_NORMRHS],
-                    g.h_scal_pinned[F_S_DONE]);
-        if (g.h_scal_pinned[F_S_DONE] != 0.0) break;
-        double rr = g.h_scal_pinned[F_S_NORMR];
-        // stagnation: less than a 2x residual-norm reduction over a chunk, three times
-        if (prev_rr > 0.0 && !(rr < 0.25 * prev_rr)) slow++;
-        if (ln.trace) fprintf(stderr, "fcg chunk rr=%.3e slow=%d\n", rr, slow);
-        prev_rr = rr;
-        if (slow >= 3) { *stagnated = 1; g.h_scal_pinned[F_S_ITER] = k; g.h_scal_pinned[F_S_IERR] = 0.0; break; }
-        if (k > maxit + CH) { g.h_scal_pinned[F_S_IERR] = 1.0; g.h_scal_pinned[F_S_ITER] = k; break; }
+    hipLaunchKernelGGL(k_fcg_init, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, b64, g.d_sinv, r, x, scal, part, ps,
+                       inner_tol * inner_tol, (double)K + 1.0, otol2, outer_rrbb, first, slot);
+    const Dia8 A = scaled_op();
+    for (int c = 0; c < K; ++c) {
+        hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, scal, part, ps);
+        int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
+        hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
+        hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, ln.z, ln.w, p, ln.q, x, r,
+                           scal, part, ps);
     }
-    *iters = (int)g.h_scal_pinned[F_S_ITER];
-    *ierr = (int)g.h_scal_pinned[F_S_IERR];
-    if (*ierr == 2 || *ierr == 3) *stagnated = 1;        // NaN or breakdown: hand over to the plain CG
-    ln.last_iters = *iters;
+    hipLaunchKernelGGL(k_fcg_finish, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, g.d_sinv, x, xn, scal, part, ps,
+                       slot);
+    HIPCHK(hipGetLastError());
     return PIC_OK;
 }
+
+bool learn_trace_on() { return ln.trace; }
+void learn_note_iterations(int it) { ln.last_iters = it; }
 
 bool learn_ready() { return ln.ready; }
 double learn_inner_tol() { return ln.inner_tol; }

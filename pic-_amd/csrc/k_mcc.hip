@@ -4,78 +4,117 @@
 // floor(CollisionRatio * 2^32) (a Bernoulli(CollisionRatio) trial per particle, the GPU
 // form of SelectParticle's INT(N*P) + Bernoulli(residue) count with a partial
 // Fisher-Yates shuffle, MCCPublic.F90:113-171: same expected count, no serial shuffle).
-// The selected slots are compacted in slot order (block counts, one-block scan, write).
+// Slots 4g..4g+3 share one Philox call of counter (g, step, tag, 0); slot s tests word
+// v[s & 3]. A mask pass writes one bit per slot and per-block counts, one workgroup
+// scans the counts, and a write pass compacts the selected slots in slot order.
 // Collision: one thread per selected particle, FP64 kinematics of TypeMCC / MCCElectron /
 // MCCIon / MCCChi drawing from the particle's own (slot, step, species) Philox stream.
 // Ionisation products are staged per selected particle and appended in selected order.
 #include "pic_device.h"
 #include <hipcub/hipcub.hpp>
 
-#define SEL_PER_THREAD 4
+// A wave owns 2048 consecutive slots as 8 chunks of 256; in chunk j lane l tests slots
+// wave_base + 256 j + 4 l + (0..3) (one int4 of cell, one Philox call) and keeps the
+// four results in bits 4 j .. 4 j + 3 of its mask word.
 #define SEL_BLOCK 256
-#define SEL_TILE (SEL_PER_THREAD * SEL_BLOCK)
+#define SEL_CHUNKS 8
+#define SEL_WAVE_SLOTS (SEL_CHUNKS * 256)
+#define SEL_BLOCK_SLOTS (SEL_WAVE_SLOTS * (SEL_BLOCK / 64))
+#define SCAN_THREADS 1024
 
-__device__ inline bool sel_test(int64_t slot, const int* cell, uint32_t thr, uint64_t seed, uint32_t step, uint32_t tag) {
-    if (cell[slot] < 0) return false;
-    philox4 o = pic_philox4x32_10((uint32_t)slot, step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
-    return o.v[0] < thr;
+__device__ inline uint32_t sel_nibble(int64_t s, int64_t n, const int* cell, uint32_t thr, uint64_t seed,
+                                      uint32_t step, uint32_t tag) {
+    int c[4];
+    if (s + 3 < n) {
+        int4 q = *(const int4*)(cell + s);
+        c[0] = q.x; c[1] = q.y; c[2] = q.z; c[3] = q.w;
+    } else {
+        for (int k = 0; k < 4; ++k) c[k] = (s + k < n) ? cell[s + k] : -1;
+    }
+    if ((c[0] & c[1] & c[2] & c[3]) < 0) return 0u;          // all four dead (or past n)
+    philox4 o = pic_philox4x32_10((uint32_t)(s >> 2), step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t m = 0;
+    for (int k = 0; k < 4; ++k) m |= (c[k] >= 0 && o.v[k] < thr) ? (1u << k) : 0u;
+    return m;
 }
 
-__global__ __launch_bounds__(SEL_BLOCK) void k_sel_count(int64_t n, const int* cell, uint32_t thr, uint64_t seed,
-                                                         uint32_t step, uint32_t tag, int* blockcount) {
+__device__ inline int wave_sum(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// mask word per lane (wave-major), selected count per block
+__global__ __launch_bounds__(SEL_BLOCK) void k_sel_mask(int64_t n, const int* cell, uint32_t thr, uint64_t seed,
+                                                        uint32_t step, uint32_t tag, uint32_t* mask, int* blockcount) {
     __shared__ int sh[SEL_BLOCK / 64];
-    int64_t base = (int64_t)blockIdx.x * SEL_TILE + (int64_t)threadIdx.x * SEL_PER_THREAD;
-    int c = 0;
-    for (int k = 0; k < SEL_PER_THREAD; ++k) { int64_t s = base + k; if (s < n && sel_test(s, cell, thr, seed, step, tag)) c++; }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    int lane = threadIdx.x & 63;
+    int64_t wave = (int64_t)blockIdx.x * (SEL_BLOCK / 64) + (threadIdx.x >> 6);
+    int64_t base = wave * SEL_WAVE_SLOTS + 4 * lane;
+    uint32_t word = 0;
+#pragma unroll
+    for (int j = 0; j < SEL_CHUNKS; ++j) word |= sel_nibble(base + 256 * j, n, cell, thr, seed, step, tag) << (4 * j);
+    mask[wave * 64 + lane] = word;
+    int c = wave_sum(__popc(word));
+    if (lane == 0) sh[threadIdx.x >> 6] = c;
     __syncthreads();
-    if (threadIdx.x == 0) { int t = 0; for (int w = 0; w < SEL_BLOCK / 64; ++w) t += sh[w]; blockcount[blockIdx.x] = t; }
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < SEL_BLOCK / 64; ++w) t += sh[w];
+        blockcount[blockIdx.x] = t;
+    }
 }
 
-// exclusive scan of nb block counts in one block; total at blockcount[nb]
-__global__ __launch_bounds__(1024) void k_scan_counts(int nb, int* blockcount) {
-    __shared__ int carry;
-    __shared__ int sh[1024];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < nb ? blockcount[i] : 0;
-        sh[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        int incl = sh[threadIdx.x];
-        if (i < nb) blockcount[i] = carry + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += incl;
-        __syncthreads();
+// Exclusive scan of nb counts in one workgroup (each thread a contiguous run of counts);
+// the total goes to blockcount[nb] and *total_out, and mul * total is added to *counter.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_counts(int nb, int* blockcount, int* total_out,
+                                                              unsigned long long* counter, int mul) {
+    __shared__ int wt[SCAN_THREADS / 64];
+    int per = (nb + SCAN_THREADS - 1) / SCAN_THREADS;
+    int b0 = threadIdx.x * per, b1 = min(nb, b0 + per);
+    int own = 0;
+    for (int i = b0; i < b1; ++i) own += blockcount[i];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = own;
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
     }
-    if (threadIdx.x == 0) blockcount[nb] = carry;
+    if (lane == 63) wt[w] = incl;
+    __syncthreads();
+    int wbase = 0, total = 0;
+    for (int k = 0; k < SCAN_THREADS / 64; ++k) { int v = wt[k]; if (k < w) wbase += v; total += v; }
+    int run = wbase + incl - own;
+    for (int i = b0; i < b1; ++i) { int v = blockcount[i]; blockcount[i] = run; run += v; }
+    if (threadIdx.x == 0) {
+        blockcount[nb] = total;
+        if (total_out) *total_out = total;
+        if (counter) *counter += (unsigned long long)mul * (unsigned long long)total;
+    }
 }
 
-__global__ __launch_bounds__(SEL_BLOCK) void k_sel_write(int64_t n, const int* cell, uint32_t thr, uint64_t seed,
-                                                         uint32_t step, uint32_t tag, const int* blockoff, int* out) {
-    __shared__ int sh[SEL_BLOCK];
-    int64_t base = (int64_t)blockIdx.x * SEL_TILE + (int64_t)threadIdx.x * SEL_PER_THREAD;
-    bool f[SEL_PER_THREAD];
-    int c = 0;
-    for (int k = 0; k < SEL_PER_THREAD; ++k) { int64_t s = base + k; f[k] = s < n && sel_test(s, cell, thr, seed, step, tag); c += f[k]; }
-    sh[threadIdx.x] = c;
+// compacted slot list in slot order from the mask words and the scanned block offsets
+__global__ __launch_bounds__(SEL_BLOCK) void k_sel_write(const uint32_t* mask, const int* blockoff, int* out) {
+    __shared__ int sh[SEL_BLOCK / 64];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t wave = (int64_t)blockIdx.x * (SEL_BLOCK / 64) + w;
+    uint32_t word = mask[wave * 64 + lane];
+    int c = wave_sum(__popc(word));
+    if (lane == 0) sh[w] = c;
     __syncthreads();
-    for (int off = 1; off < SEL_BLOCK; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += t;
-        __syncthreads();
+    int run = blockoff[blockIdx.x];
+    for (int k = 0; k < w; ++k) run += sh[k];
+    if (__ballot(word != 0u) == 0ull) return;                  // wave-uniform: nothing selected here
+    const uint64_t lt = (1ull << lane) - 1ull;
+    int64_t base = wave * SEL_WAVE_SLOTS + 4 * lane;
+#pragma unroll
+    for (int j = 0; j < SEL_CHUNKS; ++j) {
+        uint32_t nib = (word >> (4 * j)) & 15u;
+        uint64_t b0 = __ballot(nib & 1u), b1 = __ballot(nib & 2u), b2 = __ballot(nib & 4u), b3 = __ballot(nib & 8u);
+        int pos = run + __popcll(b0 & lt) + __popcll(b1 & lt) + __popcll(b2 & lt) + __popcll(b3 & lt);
+        for (int q = 0; q < 4; ++q)
+            if (nib & (1u << q)) out[pos++] = (int)(base + 256 * j + q);
+        run += __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
     }
-    int pos = blockoff[blockIdx.x] + sh[threadIdx.x] - c;
-    for (int k = 0; k < SEL_PER_THREAD; ++k) if (f[k]) out[pos++] = (int)(base + k);
 }
 
 // ---------------------------------------------------------------- collision
@@ -96,8 +135,9 @@ struct MccArgs {
     unsigned long long* events;   // [32]
     unsigned long long* counters;
     unsigned int* err;
-    // staged ionisation products, one pair per selected particle
-    unsigned char* newflag;
+    // staged ionisation products, one pair per selected particle; bit k of newmask marks
+    // selected particle k as having created a pair (null when the species creates none)
+    unsigned long long* newmask;
     double* new_e;          // 6 per selected: x, y, vx, vy, vz, cell
     double* new_i;
 };
@@ -170,10 +210,9 @@ __device__ inline void velocity_updater(MccState& M, double f) {
     M.v.x *= f; M.v.y *= f; M.v.z *= f;
 }
 
+// collision of selected particle k; true when it created an electron-ion pair
 template <typename R>
-__global__ __launch_bounds__(128) void k_mcc_collide(MccArgs a) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nsel) return;
+__device__ inline bool collide_one(const MccArgs& a, int k) {
     int slot = a.sel[k];
     R* X = (R*)a.x; R* Y = (R*)a.y; R* VX = (R*)a.vx; R* VY = (R*)a.vy; R* VZ = (R*)a.vz;
     pic_stream s; pic_stream_init(&s, a.seed, (uint32_t)slot, a.step, a.tag);
@@ -249,42 +288,83 @@ __global__ __launch_bounds__(128) void k_mcc_collide(MccArgs a) {
             else if (type == 103 || type == 102) M.v = M.vt;
         }
     }
-    a.newflag[k] = created ? 1 : 0;
     // VelRes(1/VFactor)
     double vf = 1.0 / a.v_ref;
     VX[slot] = (R)(M.v.x * vf); VY[slot] = (R)(M.v.y * vf); VZ[slot] = (R)(M.v.z * vf);
+    return created;
 }
 
-// append staged products: flags -> offsets (single block scan is enough: nsel is small)
 template <typename R>
-__global__ __launch_bounds__(1024) void k_append(int nsel, const unsigned char* flag, const double* stage, int64_t base,
-                                                 R* x, R* y, R* vx, R* vy, R* vz, int* cell, int* count_out) {
-    __shared__ int sh[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int b0 = 0; b0 < nsel; b0 += 1024) {
-        int k = b0 + threadIdx.x;
-        int f = (k < nsel) ? flag[k] : 0;
-        sh[threadIdx.x] = f;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-            __syncthreads();
-            sh[threadIdx.x] += t;
-            __syncthreads();
-        }
-        int incl = sh[threadIdx.x];
-        if (f) {
-            int64_t d = base + carry + incl - 1;
-            const double* st = stage + (size_t)k * 6;
-            x[d] = (R)st[0]; y[d] = (R)st[1]; vx[d] = (R)st[2]; vy[d] = (R)st[3]; vz[d] = (R)st[4]; cell[d] = (int)st[5];
-        }
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += incl;
-        __syncthreads();
+__global__ __launch_bounds__(128) void k_mcc_collide(MccArgs a) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool created = (k < a.nsel) && collide_one<R>(a, k);
+    if (a.newmask) {
+        unsigned long long b = __ballot(created);
+        if ((threadIdx.x & 63) == 0 && k < a.nsel) a.newmask[k >> 6] = b;
     }
-    if (threadIdx.x == 0) *count_out = carry;
+}
+
+// Ionisation products: bit k of the collision's mask marks selected particle k as a
+// creator; per-block counts, the shared scan, then each word's pairs are written at the
+// end of the electron and ion arrays in selected order (MCCAddNewParticle order).
+__global__ __launch_bounds__(SEL_BLOCK) void k_mask_count(int nwords, const unsigned long long* mask, int* blockcount) {
+    __shared__ int sh[SEL_BLOCK / 64];
+    int w = blockIdx.x * SEL_BLOCK + threadIdx.x;
+    int c = wave_sum(w < nwords ? __popcll(mask[w]) : 0);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < SEL_BLOCK / 64; ++k) t += sh[k];
+        blockcount[blockIdx.x] = t;
+    }
+}
+
+struct AppendDst {
+    void *x, *y, *vx, *vy, *vz;
+    int* cell;
+    int64_t base;           // first free slot
+};
+
+template <typename R>
+__device__ inline void put_product(const AppendDst& d, int64_t i, const double* st) {
+    ((R*)d.x)[i] = (R)st[0]; ((R*)d.y)[i] = (R)st[1];
+    ((R*)d.vx)[i] = (R)st[2]; ((R*)d.vy)[i] = (R)st[3]; ((R*)d.vz)[i] = (R)st[4];
+    d.cell[i] = (int)st[5];
+}
+
+// room = free slots of the fuller species; nothing is written when the pairs do not fit
+template <typename R>
+__global__ __launch_bounds__(SEL_BLOCK) void k_append_pairs(int nwords, const unsigned long long* mask,
+                                                            const int* blockoff, const double* new_e,
+                                                            const double* new_i, AppendDst E, AppendDst I,
+                                                            int64_t room) {
+    __shared__ int sh[SEL_BLOCK / 64];
+    if ((int64_t)blockoff[gridDim.x] > room) return;
+    int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int w = blockIdx.x * SEL_BLOCK + threadIdx.x;
+    unsigned long long word = w < nwords ? mask[w] : 0ull;
+    int own = __popcll(word), incl = own;
+    for (int off = 1; off < 64; off <<= 1) {
+        int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) sh[wv] = incl;
+    __syncthreads();
+    int64_t pos = blockoff[blockIdx.x] + incl - own;
+    for (int k = 0; k < wv; ++k) pos += sh[k];
+    while (word) {
+        int b = __ffsll((long long)word) - 1;
+        int64_t k = (int64_t)w * 64 + b;
+        put_product<R>(E, E.base + pos, new_e + k * 6);
+        put_product<R>(I, I.base + pos, new_i + k * 6);
+        ++pos;
+        word &= word - 1ull;
+    }
+}
+
+__global__ void k_add_counter(unsigned long long* c, unsigned long long v) {
+    if (threadIdx.x == 0) *c += v;
 }
 
 static int ensure_buf(void** p, int64_t* cap, int64_t need_bytes) {
@@ -296,40 +376,23 @@ static int ensure_buf(void** p, int64_t* cap, int64_t need_bytes) {
     return PIC_OK;
 }
 
-static int select_species(int isp, int* nsel_out) {
+// mask pass + scan of one species; its selected count lands in *info, its counter
+// PIC_C_MCC_SELECTED grows by the same amount on the device
+static void launch_select(int isp, int64_t n, int nb, uint32_t* mask, int* bc, int* info) {
     SpeciesHost& S = g.sp[isp];
-    int64_t n = S.nslots;
-    if (n == 0) { *nsel_out = 0; return PIC_OK; }
-    int nb = (int)((n + SEL_TILE - 1) / SEL_TILE);
-    int rc = ensure_buf((void**)&g.d_blockcount, &g.blockcount_cap, (int64_t)(nb + 1) * sizeof(int));
-    if (rc) return rc;
     uint32_t tag = pic_stream_tag(isp, PIC_RNG_SELECT);
-    hipLaunchKernelGGL(k_sel_count, dim3(nb), dim3(SEL_BLOCK), 0, g.stream, n, S.cell, S.select_threshold, g.seed,
-                       (uint32_t)g.step, tag, g.d_blockcount);
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, g.stream, nb, g.d_blockcount);
-    HIPCHK(hipGetLastError());
-    int total = 0;
-    HIPCHK(hipMemcpyAsync(&total, g.d_blockcount + nb, sizeof(int), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    rc = ensure_buf((void**)&g.d_sel, &g.sel_cap, (int64_t)std::max(total, 1) * sizeof(int));
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_sel_write, dim3(nb), dim3(SEL_BLOCK), 0, g.stream, n, S.cell, S.select_threshold, g.seed,
-                       (uint32_t)g.step, tag, g.d_blockcount, g.d_sel);
-    HIPCHK(hipGetLastError());
-    *nsel_out = total;
-    return PIC_OK;
+    hipLaunchKernelGGL(k_sel_mask, dim3(nb), dim3(SEL_BLOCK), 0, g.stream, n, S.cell, S.select_threshold, g.seed,
+                       (uint32_t)g.step, tag, mask, bc);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_THREADS), 0, g.stream, nb, bc, info,
+                       S.counters + PIC_C_MCC_SELECTED, 1);
 }
 
 template <typename R>
-static int collide_species(int isp, int nsel, int* ncreated) {
+static void launch_collide(int isp, int nsel, const int* sel, unsigned long long* newmask, double* new_e,
+                           double* new_i) {
     SpeciesHost& S = g.sp[isp];
-    *ncreated = 0;
-    if (nsel == 0) return PIC_OK;
-    int64_t need = (int64_t)nsel * (1 + 12 * sizeof(double)) + 64;
-    int rc = ensure_buf(&g.d_newbuf, &g.newbuf_cap, need);
-    if (rc) return rc;
     MccArgs a;
-    a.nsel = nsel; a.sel = g.d_sel; a.nslots = S.nslots;
+    a.nsel = nsel; a.sel = sel; a.nslots = S.nslots;
     a.x = S.x; a.y = S.y; a.vx = S.vx; a.vy = S.vy; a.vz = S.vz; a.cell = S.cell;
     a.nreact = S.nreact; a.nsigma = S.nsigma; a.emin = S.emin; a.eint = S.eint; a.emax = S.emax;
     a.prob = S.prob; a.rtype = S.rtype; a.thr = S.thr;
@@ -337,89 +400,88 @@ static int collide_species(int isp, int nsel, int* ncreated) {
     a.electron = (isp == g.e_species) ? 1 : 0;
     a.seed = g.seed; a.step = (uint32_t)g.step; a.tag = pic_stream_tag(isp, PIC_RNG_COLLIDE);
     a.events = S.mcc_events; a.counters = S.counters; a.err = g.d_err;
-    double* dbl = (double*)g.d_newbuf;
-    a.new_e = dbl; a.new_i = dbl + (size_t)nsel * 6;
-    a.newflag = (unsigned char*)(dbl + (size_t)nsel * 12);
+    a.newmask = newmask; a.new_e = new_e; a.new_i = new_i;
     hipLaunchKernelGGL((k_mcc_collide<R>), dim3((nsel + 127) / 128), dim3(128), 0, g.stream, a);
-    HIPCHK(hipGetLastError());
-    return PIC_OK;
 }
 
-template <typename R>
-static int append_products(int nsel, int esp, int isp_ion) {
-    if (nsel == 0) return PIC_OK;
-    double* dbl = (double*)g.d_newbuf;
-    const double* new_e = dbl; const double* new_i = dbl + (size_t)nsel * 6;
-    const unsigned char* flag = (const unsigned char*)(dbl + (size_t)nsel * 12);
-    // count first (host) to check capacity before writing
-    std::vector<unsigned char> hf(nsel);
-    HIPCHK(hipMemcpyAsync(hf.data(), flag, nsel, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    int nnew = 0;
-    for (int k = 0; k < nsel; ++k) nnew += hf[k];
-    if (nnew == 0) return PIC_OK;
-    SpeciesHost& E = g.sp[esp];
-    SpeciesHost& I = g.sp[isp_ion];
-    if (E.nslots + nnew > E.capacity || I.nslots + nnew > I.capacity)
-        return pic_fail(PIC_ERR_CAPACITY, "MCC products exceed particle capacity");
-    int* dcount = (int*)g.d_blockcount;
-    hipLaunchKernelGGL((k_append<R>), dim3(1), dim3(1024), 0, g.stream, nsel, flag, new_e, E.nslots, (R*)E.x, (R*)E.y,
-                       (R*)E.vx, (R*)E.vy, (R*)E.vz, E.cell, dcount);
-    hipLaunchKernelGGL((k_append<R>), dim3(1), dim3(1024), 0, g.stream, nsel, flag, new_i, I.nslots, (R*)I.x, (R*)I.y,
-                       (R*)I.vx, (R*)I.vy, (R*)I.vz, I.cell, dcount);
-    HIPCHK(hipGetLastError());
-    E.nslots += nnew; E.nlive += nnew;
-    I.nslots += nnew; I.nlive += nnew;
-    unsigned long long add = (unsigned long long)nnew;
-    // created counters (electron species counts both products it created)
-    HIPCHK(hipMemcpyAsync(g.h_scal_pinned, E.counters + PIC_C_MCC_CREATED, 8, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    unsigned long long cur; memcpy(&cur, g.h_scal_pinned, 8); cur += 2 * add;
-    HIPCHK(hipMemcpy(E.counters + PIC_C_MCC_CREATED, &cur, 8, hipMemcpyHostToDevice));
-    return PIC_OK;
-}
-
-// MCC order of MCCPublic.F90:50-105: for each gas (one), species 0..Ns with a table;
-// products of all species are appended after the loop (MCCAddNewParticle).
+// MCC order of MCCPublic.F90:50-105 for the one gas: electrons then ions, products of
+// the electron collisions appended after both (MCCAddNewParticle). Ion selection only
+// looks at ion slots that exist before the products arrive and the two species' random
+// streams are disjoint, so selecting and colliding both species before the append gives
+// the same result as the reference's interleaving.
+// Host synchronisations: one after the two selection counts (sizes of the slot list and
+// the staging), one after the product count (new slot counts).
 template <typename R>
 static int run_mcc_t() {
     int e = g.e_species, ion = g.i_species;
-    int nsel_e = 0, nsel_i = 0, nc = 0;
-    int rc;
-    // electrons: select, collide (staging kept in d_newbuf), then ions would overwrite the
-    // shared selection buffers, so the electron products are appended before the ions
-    // are selected. Appending after selection of ions or before is equivalent because ion
-    // selection only looks at ion slots that existed before this call.
-    if (g.sp[e].has_mcc) {
-        int64_t ion_slots_before = g.sp[ion].nslots;
-        rc = select_species(e, &nsel_e); if (rc) return rc;
-        rc = collide_species<R>(e, nsel_e, &nc); if (rc) return rc;
-        unsigned long long c = nsel_e;
-        (void)c;
-        // ions selected against the slots present before the products arrive
-        int64_t save = g.sp[ion].nslots;
-        rc = append_products<R>(nsel_e, e, ion); if (rc) return rc;
-        int64_t added = g.sp[ion].nslots - save;
-        if (g.sp[ion].has_mcc) {
-            g.sp[ion].nslots = ion_slots_before;
-            rc = select_species(ion, &nsel_i);
-            g.sp[ion].nslots = ion_slots_before + added;
-            if (rc) return rc;
-            rc = collide_species<R>(ion, nsel_i, &nc); if (rc) return rc;
-        }
-    } else if (g.sp[ion].has_mcc) {
-        rc = select_species(ion, &nsel_i); if (rc) return rc;
-        rc = collide_species<R>(ion, nsel_i, &nc); if (rc) return rc;
+    SpeciesHost& E = g.sp[e];
+    SpeciesHost& I = g.sp[ion];
+    int64_t n_e = E.has_mcc ? E.nslots : 0, n_i = I.has_mcc ? I.nslots : 0;
+    int nb_e = (int)((n_e + SEL_BLOCK_SLOTS - 1) / SEL_BLOCK_SLOTS);
+    int nb_i = (int)((n_i + SEL_BLOCK_SLOTS - 1) / SEL_BLOCK_SLOTS);
+    if (nb_e + nb_i == 0) return PIC_OK;
+    int rc = ensure_buf((void**)&g.d_selmask, &g.selmask_cap, (int64_t)(nb_e + nb_i) * SEL_BLOCK * sizeof(uint32_t));
+    if (rc) return rc;
+    rc = ensure_buf((void**)&g.d_blockcount, &g.blockcount_cap, (int64_t)(nb_e + nb_i + 2) * sizeof(int));
+    if (rc) return rc;
+    rc = ensure_buf((void**)&g.d_mcc_info, &g.mcc_info_cap, 4 * sizeof(int));
+    if (rc) return rc;
+    int* info = g.d_mcc_info;
+    HIPCHK(hipMemsetAsync(info, 0, 4 * sizeof(int), g.stream));
+    uint32_t* mask_e = g.d_selmask;
+    uint32_t* mask_i = mask_e + (size_t)nb_e * SEL_BLOCK;
+    int* bc_e = g.d_blockcount;
+    int* bc_i = bc_e + nb_e + 1;
+    if (nb_e) launch_select(e, n_e, nb_e, mask_e, bc_e, info + 0);
+    if (nb_i) launch_select(ion, n_i, nb_i, mask_i, bc_i, info + 1);
+    HIPCHK(hipGetLastError());
+    int* hinfo = (int*)g.h_scal_pinned;
+    HIPCHK(hipMemcpyAsync(hinfo, info, 2 * sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    int nsel_e = hinfo[0], nsel_i = hinfo[1];
+    if (nsel_e + nsel_i == 0) return PIC_OK;
+    rc = ensure_buf((void**)&g.d_sel, &g.sel_cap, (int64_t)(nsel_e + nsel_i) * sizeof(int));
+    if (rc) return rc;
+    int nwords = (nsel_e + 63) / 64;
+    rc = ensure_buf(&g.d_newbuf, &g.newbuf_cap, (int64_t)nsel_e * 12 * sizeof(double) + (int64_t)nwords * 8 + 64);
+    if (rc) return rc;
+    double* new_e = (double*)g.d_newbuf;
+    double* new_i = new_e + (size_t)nsel_e * 6;
+    unsigned long long* newmask = (unsigned long long*)(new_e + (size_t)nsel_e * 12);
+    if (nsel_e) {
+        hipLaunchKernelGGL(k_sel_write, dim3(nb_e), dim3(SEL_BLOCK), 0, g.stream, mask_e, bc_e, g.d_sel);
+        launch_collide<R>(e, nsel_e, g.d_sel, newmask, new_e, new_i);
     }
-    // selected counters
-    for (int s = 0; s < g.nspecies; ++s) {
-        int ns = (s == e) ? nsel_e : (s == ion ? nsel_i : 0);
-        if (!ns) continue;
-        unsigned long long cur;
-        HIPCHK(hipMemcpy(&cur, g.sp[s].counters + PIC_C_MCC_SELECTED, 8, hipMemcpyDeviceToHost));
-        cur += (unsigned long long)ns;
-        HIPCHK(hipMemcpy(g.sp[s].counters + PIC_C_MCC_SELECTED, &cur, 8, hipMemcpyHostToDevice));
+    if (nsel_i) {
+        hipLaunchKernelGGL(k_sel_write, dim3(nb_i), dim3(SEL_BLOCK), 0, g.stream, mask_i, bc_i, g.d_sel + nsel_e);
+        launch_collide<R>(ion, nsel_i, g.d_sel + nsel_e, nullptr, nullptr, nullptr);
     }
+    HIPCHK(hipGetLastError());
+    if (nsel_e == 0) return PIC_OK;
+    // ionisation pairs
+    int nbp = (nwords + SEL_BLOCK - 1) / SEL_BLOCK;
+    rc = ensure_buf((void**)&g.d_pcount, &g.pcount_cap, (int64_t)(nbp + 1) * sizeof(int));
+    if (rc) return rc;
+    int64_t room = std::min(E.capacity - E.nslots, I.capacity - I.nslots);
+    hipLaunchKernelGGL(k_mask_count, dim3(nbp), dim3(SEL_BLOCK), 0, g.stream, nwords, newmask, g.d_pcount);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_THREADS), 0, g.stream, nbp, g.d_pcount, info + 2,
+                       (unsigned long long*)nullptr, 0);
+    AppendDst de{E.x, E.y, E.vx, E.vy, E.vz, E.cell, E.nslots};
+    AppendDst di{I.x, I.y, I.vx, I.vy, I.vz, I.cell, I.nslots};
+    hipLaunchKernelGGL((k_append_pairs<R>), dim3(nbp), dim3(SEL_BLOCK), 0, g.stream, nwords, newmask, g.d_pcount,
+                       new_e, new_i, de, di, room);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hinfo, info + 2, sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    int64_t nnew = hinfo[0];
+    if (nnew > room) return pic_fail(PIC_ERR_CAPACITY, "MCC products exceed particle capacity");
+    if (nnew == 0) return PIC_OK;
+    E.nslots += nnew; E.nlive += nnew;
+    I.nslots += nnew; I.nlive += nnew;
+    // created counter: the electron species counts both products of each pair
+    hipLaunchKernelGGL(k_add_counter, dim3(1), dim3(64), 0, g.stream, E.counters + PIC_C_MCC_CREATED,
+                       (unsigned long long)(2 * nnew));
+    HIPCHK(hipGetLastError());
     return PIC_OK;
 }
 

@@ -179,6 +179,10 @@ struct PicState {
     unsigned int* d_wall_count = nullptr;
     int64_t wall_cap = 0;
     int learn_maxit = 400;             // FCG iterations per inner solve before handing over
+    // learned refinement: passes and FCG iterations per pass of the last learned solve, the
+    // launch budget of the next one (all its passes go out before one host synchronisation)
+    int lp_passes = 2;
+    int lp_iters[6] = {8, 8, 8, 8, 8, 8};
     int last_learned_iters = 0, last_fallback = 0;
     // kernel statistics of the last pic_step: [0] ms in CG iteration kernels (chunk events),
     // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches,
@@ -212,6 +216,9 @@ struct PicState {
     // scratch for selection / compaction
     int* d_sel = nullptr; int64_t sel_cap = 0;
     int* d_blockcount = nullptr; int64_t blockcount_cap = 0;
+    uint32_t* d_selmask = nullptr; int64_t selmask_cap = 0;   // MCC selection bits, both species
+    int* d_pcount = nullptr; int64_t pcount_cap = 0;          // MCC product block counts
+    int* d_mcc_info = nullptr; int64_t mcc_info_cap = 0;      // selected e / i, pairs created
     void* d_newbuf = nullptr; int64_t newbuf_cap = 0;
     void* d_sort_tmp = nullptr; size_t sort_tmp_bytes = 0;
     void* d_sort_buf = nullptr; size_t sort_buf_bytes = 0;
@@ -250,6 +257,11 @@ int count_live_slots(int64_t n, const int* cell, int64_t* out);
 int launch_locate_points(int64_t n, const double* dx, const double* dy, int* dout, int mode);
 int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, const int* ele, double* ex, double* ey);
 // learned preconditioner (k_learn.hip)
+// records of the learned refinement passes in the device scalar block d_scal (64 doubles):
+// slot k at FCG_REC_BASE + FCG_RECW * k = iterations, done, error code, ||r||^2/||r0||^2, skipped
+#define FCG_REC_BASE 32
+#define FCG_RECW 5
+#define FCG_MAX_PASS 6
 int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed);
 int learn_set_harvest(int enable, int capacity, int every);
 bool learn_harvesting();
@@ -257,7 +269,10 @@ int learn_snapshot(const float* x, int chunk);
 int learn_harvest_finish(const float* xfinal);
 int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
 int learn_eval(int npairs, double* mean_loss);
-int learn_fcg_run(int n, const double* b64, double tol, int maxit, int* iters, int* ierr, int* stagnated);
+int learn_fcg_pass(int n, const double* b64, double inner_tol, int K, double otol2, const double* outer_rrbb,
+                   int first, int slot, double* xn);
+bool learn_trace_on();
+void learn_note_iterations(int it);
 bool learn_ready();
 double learn_inner_tol();
 int learn_apply_host(const float* r, float* z, int64_t n);

@@ -371,6 +371,25 @@ def test_mcc_selection_rate(orc):
     assert np.all(np.diff(sel) > 0)
 
 
+def test_mcc_selection_stream(orc):
+    """Slot s draws word s % 4 of the Philox block with counter (s // 4, step, tag, 0):
+    brute force over the blocks, with dead slots (cell < 0) never selected."""
+    n = 1003
+    rng = np.random.default_rng(9)
+    cell = rng.integers(0, 50, n).astype(np.int32)
+    cell[rng.random(n) < 0.2] = -1
+    seed, step, tg = 0x0123456789ABCDEF, 7, orc.tag(1, orc.RNG_SELECT)
+    thr = int(0.3 * 2 ** 32)
+    words = np.concatenate([orc.philox([g, step, tg, 0], [seed & 0xFFFFFFFF, seed >> 32])
+                            for g in range((n + 3) // 4)])[:n]
+    expect = np.nonzero((words < thr) & (cell >= 0))[0]
+    sel = orc.mcc_select(cell, thr, seed, step, tg)
+    assert np.array_equal(sel, expect) and 200 < len(sel) < 300
+    # every live slot at the top threshold, none at zero
+    assert np.array_equal(orc.mcc_select(cell, 2 ** 32 - 1, seed, step, tg), np.nonzero(cell >= 0)[0])
+    assert len(orc.mcc_select(cell, 0, seed, step, tg)) == 0
+
+
 # ------------------------------------------------------------------ library exports
 def test_library_exports_every_declared_symbol(pkg):
     lib = ctypes.CDLL(str(pkg.LIB_PATH))
