@@ -77,6 +77,7 @@ __global__ void k_load(int64_t n, const double* x, const double* y, const double
 static MeshParams dev_mesh() {
     MeshParams m = g.mesh;
     m.xs = g.d_xs; m.ys = g.d_ys; m.xsf = g.d_xsf; m.ysf = g.d_ysf;
+    m.elem_clean = g.d_elem_clean;
     m.element_index = g.d_element_index; m.node_index = g.d_node_index;
     return m;
 }
@@ -142,7 +143,7 @@ __global__ void k_d2f_n(int64_t n, const double* a, float* b) { int64_t i = (int
 static void free_species(int s) {
     SpeciesHost& S = g.sp[s];
     void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob, S.rtype,
-                    S.thr, S.mcc_events};
+                    S.thr, S.mcc_events, S.spare[0], S.spare[1], S.spare[2], S.spare[3], S.spare[4]};
     for (void* p : ptrs) if (p) hipFree(p);
     S = SpeciesHost();
 }
@@ -234,6 +235,32 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     {
         std::vector<DevObject> ob(g.mesh.obj, g.mesh.obj + PIC_MAX_OBJECTS);
         TRY(dupload(&g.d_objects, ob));
+    }
+    {
+        // elements where a particle can need the object branches of MoveOne /
+        // AdjustParticleBoundary: cut elements and every element whose box, widened by
+        // 1e-6 h, meets an object (the push kernel hands their particles to the full loop)
+        std::vector<unsigned char> clean(m.nelem, 1);
+        for (int e = 0; e < m.nelem; ++e) {
+            const int ci = e / m.ney, cj = e - ci * m.ney;
+            if (g.h_element_index[e] > 0) { clean[e] = 0; continue; }
+            const double mx = 1e-6 * m.hx, my = 1e-6 * m.hy;
+            const double bx0 = g.h_xs[ci] - mx, bx1 = g.h_xs[ci + 1] + mx;
+            const double by0 = g.h_ys[cj] - my, by1 = g.h_ys[cj + 1] + my;
+            for (int o = 0; o < m.n_objects; ++o) {
+                const DevObject& d = m.obj[o];
+                bool touch = false;
+                if (d.shape == 3) {
+                    touch = bx0 < d.x1 && bx1 > d.x0 && by0 < d.y1 && by1 > d.y0;
+                } else if (d.shape == 1) {
+                    const double dx = std::max(std::max(bx0 - d.x0, 0.0), d.x0 - bx1);
+                    const double dy = std::max(std::max(by0 - d.y0, 0.0), d.y0 - by1);
+                    touch = dx * dx + dy * dy < d.r2 * (1.0 + 1e-9);
+                }
+                if (touch) { clean[e] = 0; break; }
+            }
+        }
+        TRY(dupload(&g.d_elem_clean, clean));
     }
     TRY(dupload(&g.d_node_type2, g.h_node_type2));
     std::vector<int> un0(g.h_unknown_node.size());

@@ -492,7 +492,8 @@ int run_mcc() {
 
 // ---------------------------------------------------------------- cell sort
 // Stable radix sort of (tile-major element key, slot) with dead slots keyed past the end,
-// then a gather of every SoA array; occupied slots shrink to the live count.
+// then one gather of every SoA array into spare arrays that are swapped in (group-transposed
+// layout, pic_sorted_position); occupied slots shrink to the live count.
 __global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsigned int* vals, unsigned int dead_key,
                             int ney, int nty) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,10 +502,17 @@ __global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsi
     keys[i] = c >= 0 ? pic_sort_key(c, ney, nty) : dead_key;
     vals[i] = (unsigned int)i;
 }
-template <typename T>
-__global__ void k_gather(int64_t n, const unsigned int* perm, const T* src, T* dst) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[perm[i]];
+// Gather of every particle array in one pass: slot d takes the particle at sorted position
+// pic_sorted_position(d) (the group-transposed layout the push kernel walks).
+template <typename R>
+__global__ __launch_bounds__(256) void k_sort_gather(int64_t n, int64_t nfull, const unsigned int* perm,
+                                                     const R* x, const R* y, const R* vx, const R* vy, const R* vz,
+                                                     const int* cell, R* ox, R* oy, R* ovx, R* ovy, R* ovz,
+                                                     int* ocell) {
+    int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= n) return;
+    const unsigned int s = perm[pic_sorted_position(d, nfull)];
+    ox[d] = x[s]; oy[d] = y[s]; ovx[d] = vx[s]; ovy[d] = vy[s]; ovz[d] = vz[s]; ocell[d] = cell[s];
 }
 #define TRY_MCC(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
 
@@ -517,7 +525,7 @@ static int sort_species(int isp) {
     int end_bit = 1;
     while ((1u << end_bit) <= dead_key) end_bit++;
     size_t esz = (g.particle_bits == 64) ? 8 : 4;
-    size_t need = (size_t)n * 16 + (size_t)n * esz + 256;
+    size_t need = (size_t)n * 16 + 256;
     if (g.sort_buf_bytes < need) {
         if (g.d_sort_buf) hipFree(g.d_sort_buf);
         g.d_sort_buf = nullptr;
@@ -528,7 +536,6 @@ static int sort_species(int isp) {
     unsigned int* k1 = k0 + n;
     unsigned int* v0 = k1 + n;
     unsigned int* v1 = v0 + n;
-    void* tmpdata = (void*)(v1 + n);
     int blocks = (int)((n + 255) / 256);
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, g.stream, n, S.cell, k0, v0, dead_key, g.mesh.ney, nty);
     size_t tb = 0;
@@ -540,19 +547,29 @@ static int sort_species(int isp) {
         g.sort_tmp_bytes = tb;
     }
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(g.d_sort_tmp, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
-    // live count
+    // live count; the live particles occupy the first `live` sorted positions
     int64_t live = 0;
     TRY_MCC(count_live_slots(n, S.cell, &live));
-    // gather arrays through the temporary
-    void* arrays[5] = {S.x, S.y, S.vx, S.vy, S.vz};
-    for (int a = 0; a < 5; ++a) {
-        if (esz == 8) hipLaunchKernelGGL(k_gather<double>, dim3(blocks), dim3(256), 0, g.stream, n, v1, (const double*)arrays[a], (double*)tmpdata);
-        else hipLaunchKernelGGL(k_gather<float>, dim3(blocks), dim3(256), 0, g.stream, n, v1, (const float*)arrays[a], (float*)tmpdata);
-        HIPCHK(hipMemcpyAsync(arrays[a], tmpdata, n * esz, hipMemcpyDeviceToDevice, g.stream));
+    // gather into the spare arrays (and cell_next), then swap them in
+    for (int a = 0; a < 5; ++a)
+        if (!S.spare[a]) HIPCHK(hipMalloc(&S.spare[a], (size_t)S.capacity * esz));
+    const int64_t nfull = live / PIC_SORT_GROUP * PIC_SORT_GROUP;
+    const int gblocks = (int)((live + 255) / 256);
+    if (live > 0) {
+        if (esz == 8)
+            hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
+                               (const double*)S.x, (const double*)S.y, (const double*)S.vx, (const double*)S.vy,
+                               (const double*)S.vz, S.cell, (double*)S.spare[0], (double*)S.spare[1],
+                               (double*)S.spare[2], (double*)S.spare[3], (double*)S.spare[4], S.cell_next);
+        else
+            hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
+                               (const float*)S.x, (const float*)S.y, (const float*)S.vx, (const float*)S.vy,
+                               (const float*)S.vz, S.cell, (float*)S.spare[0], (float*)S.spare[1],
+                               (float*)S.spare[2], (float*)S.spare[3], (float*)S.spare[4], S.cell_next);
+        HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_gather<int>, dim3(blocks), dim3(256), 0, g.stream, n, v1, (const int*)S.cell, (int*)tmpdata);
-    HIPCHK(hipMemcpyAsync(S.cell, tmpdata, n * 4, hipMemcpyDeviceToDevice, g.stream));
-    HIPCHK(hipGetLastError());
+    std::swap(S.x, S.spare[0]); std::swap(S.y, S.spare[1]); std::swap(S.vx, S.spare[2]);
+    std::swap(S.vy, S.spare[3]); std::swap(S.vz, S.spare[4]); std::swap(S.cell, S.cell_next);
     HIPCHK(hipStreamSynchronize(g.stream));
     S.nslots = (int64_t)live;
     S.nlive = (int64_t)live;

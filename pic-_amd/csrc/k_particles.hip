@@ -4,8 +4,9 @@
 // normalisation) and the batched drop-ins (point lookup, PPR point gather).
 //
 // Particles are structure-of-arrays slots kept in tile-major element order (periodic
-// cell sort). A thread owns PUSH_PPT consecutive slots and moves them with vector loads
-// and stores (a wave reads and writes contiguous 2 KiB runs of every array). Its
+// cell sort). A wave owns a group of PIC_SORT_GROUP slots and a thread PUSH_PPT of them,
+// 64 slots apart, so every access is lane-contiguous; the sort places consecutive sorted
+// particles on a thread's slots. Its
 // deposits are summed in registers while consecutive particles share an element; the
 // per-thread runs are merged by one segmented wave reduction keyed by element, and run
 // totals go to an LDS node window of the block (flushed once with global atomics) or,
@@ -14,7 +15,7 @@
 #include "pic_device.h"
 
 #define PUSH_BLOCK 256
-#define PUSH_PPT 8
+#define PUSH_PPT (PIC_SORT_GROUP / 64)
 #define PUSH_SLOTS (PUSH_BLOCK * PUSH_PPT)
 #define WIN_NODES (3 * SORT_TILE + 1) * (3 * SORT_TILE + 1)
 
@@ -295,13 +296,18 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
 }
 
 // First pass of MoveAll's loop for one particle (gather + push, MoveOne.f90:77-146, 213-219,
-// 242-245). Returns true when the pass ends outside [xmin, xmax] x [ymin, ymax]: the particle
-// then goes to the wall kernel, which runs the whole loop from the original state (the same
-// arithmetic, so the result is identical). Otherwise the loop would end here: the object
-// test of AdjustParticleBoundary (MCCInterface.f90:486-499) completes it.
+// 242-245) and the element lookup of the new position, for the common case only. Returns
+// true (state untouched) when the particle needs any other branch of MoveOne /
+// AdjustParticleBoundary: its element is cut by or near an object (the object-side field
+// of Get_ParticleEField_PPR.f90:19-33), the pass ends outside [xmin, xmax] x [ymin, ymax]
+// (walls), or it ends in an element near an object (the object test,
+// MCCInterface.f90:486-499, could fire) or the lookup fails. The wall kernel then runs the
+// whole loop from the original state with the same arithmetic. Otherwise the reference's
+// loop ends after this pass with no boundary or object event, and *e_new is the element.
 template <typename R>
-__device__ inline bool push_fast(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, bool& absorbed_obj) {
+__device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, int* e_new) {
     const MeshParams& m = a.m;
+    if (!m.elem_clean[e_old]) return true;
     const R T = a.dt;
     int ci = e_old / m.ney, cj = e_old - ci * m.ney;
     R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
@@ -311,44 +317,18 @@ __device__ inline bool push_fast(const PushArgs<R>& a, int e_old, R& X, R& Y, R&
     R xm = (R)1 - dx, ym = (R)1 - dy;
     R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
     int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-    R Ex, Ey;
-    if (m.element_index[e_old] > 0) {
-        R ex = 0, ey = 0;
-        int count = 0;
-        const int nd[4] = {n1, n2, n3, n4};
-        const R nx_[4] = {x1, x2, x2, x1}, ny_[4] = {y1, y1, y2, y2};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (m.node_index[nd[k]] == -1) {
-                const R* c = a.coef5 + (size_t)nd[k] * 5;
-                count++;
-                R xr = X - nx_[k], yr = Y - ny_[k];
-                ex = ex + c[0] + (R)2 * c[2] * xr + c[3] * yr;
-                ey = ey + c[1] + (R)2 * c[4] * yr + c[3] * xr;
-            }
-        }
-        ex = ex / (R)count;
-        ey = ey / (R)count;
-        Ex = -ex; Ey = -ey;
-    } else {
-        Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
-        Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
-    }
+    R Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
+    R Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
     R VX1 = VX + a.qm * Ex * T;
     R VY1 = VY + a.qm * Ey * T;
     R X1 = X + T * VX1;
     R Y1 = Y + T * VY1;
     if (X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
-    absorbed_obj = false;
-    for (int k = 0; k < m.n_objects; ++k) {
-        const DevObject o = a.objs[k];
-        double xd = (double)X1, yd = (double)Y1;
-        bool in = false;
-        if (o.shape == 3) in = xd > o.x0 && xd < o.x1 && yd > o.y0 && yd < o.y1;
-        else if (o.shape == 1) in = (xd - o.x0) * (xd - o.x0) + (yd - o.y0) * (yd - o.y0) < o.r2;
-        if (in) { X1 = (R)-2000; absorbed_obj = true; }
-    }
+    int bad = 0;
+    int en = pic_locate_elem((double)X1, (double)Y1, m, &bad);
+    if (bad || !m.elem_clean[en]) return true;
     X = X1; Y = Y1; VX = VX1; VY = VY1;
+    *e_new = en;
     return false;
 }
 
@@ -403,36 +383,6 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
     }
 }
 
-template <typename R> struct Vec8;
-template <> struct Vec8<float> {
-    static __device__ inline void load(const float* p, float v[8]) {
-        float4 a = *(const float4*)p, b = *(const float4*)(p + 4);
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    }
-    static __device__ inline void store(float* p, const float v[8]) {
-        *(float4*)p = make_float4(v[0], v[1], v[2], v[3]);
-        *(float4*)(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
-    }
-};
-template <> struct Vec8<double> {
-    static __device__ inline void load(const double* p, double v[8]) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { double2 a = *(const double2*)(p + 2 * k); v[2 * k] = a.x; v[2 * k + 1] = a.y; }
-    }
-    static __device__ inline void store(double* p, const double v[8]) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) *(double2*)(p + 2 * k) = make_double2(v[2 * k], v[2 * k + 1]);
-    }
-};
-__device__ inline void load8i(const int* p, int v[8]) {
-    int4 a = *(const int4*)p, b = *(const int4*)(p + 4);
-    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-}
-__device__ inline void store8i(int* p, const int v[8]) {
-    *(int4*)p = make_int4(v[0], v[1], v[2], v[3]);
-    *(int4*)(p + 4) = make_int4(v[4], v[5], v[6], v[7]);
-}
-
 // Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
 template <typename R, int DO_PUSH>
 __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
@@ -469,113 +419,85 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     const int wn = w.nx * w.ny;
     for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) { winW[k] = 0; winE[k] = 0; }
 
-    // ---------------- load this thread's PUSH_PPT consecutive slots
-    const int64_t p0 = base + (int64_t)threadIdx.x * PUSH_PPT;
-    const bool full = p0 + PUSH_PPT <= a.n;
-    R X[PUSH_PPT], Y[PUSH_PPT], VX[PUSH_PPT], VY[PUSH_PPT], VZ[PUSH_PPT];
-    int cel[PUSH_PPT];
-    if (full) {
-        load8i(a.cell + p0, cel);
-        Vec8<R>::load(a.x + p0, X); Vec8<R>::load(a.y + p0, Y);
-        Vec8<R>::load(a.vx + p0, VX); Vec8<R>::load(a.vy + p0, VY); Vec8<R>::load(a.vz + p0, VZ);
-    } else {
-#pragma unroll
-        for (int k = 0; k < PUSH_PPT; ++k) {
-            int64_t p = p0 + k;
-            bool in = p < a.n;
-            cel[k] = in ? a.cell[p] : -1;
-            X[k] = in ? a.x[p] : (R)0; Y[k] = in ? a.y[p] : (R)0;
-            VX[k] = in ? a.vx[p] : (R)0; VY[k] = in ? a.vy[p] : (R)0; VZ[k] = in ? a.vz[p] : (R)0;
-        }
-    }
-    __syncthreads();     // window zeroed before any deposit
-
     // ---------------- move, locate, deposit (run accumulation in registers)
-    PushEvents ev = {0, 0, 0, 0, 0};
-    int next[PUSH_PPT];
+    // Wave w of the block owns the 512-slot group at wbase; iteration k handles slot
+    // wbase + 64 k + lane, so every load and store is lane-contiguous. The cell sort lays a
+    // full group out transposed (sorted position 8 lane + k at slot 64 k + lane), so a
+    // thread's iterations walk consecutive sorted particles, which mostly share an element.
+    const int lane = threadIdx.x & 63;
+    const int64_t wbase = base + (int64_t)(threadIdx.x >> 6) * PIC_SORT_GROUP;
     int rkey = -1;
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
-    const int lane = threadIdx.x & 63;
+    // all PUSH_PPT slots' state is loaded up front (one wait); the loop is unrolled so the
+    // per-slot registers are indexed statically
+    int cel[PUSH_PPT];
+    R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT];
+#pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
+        const int64_t p = wbase + 64 * k + lane;
+        cel[k] = -1; xs_[k] = (R)0; ys_[k] = (R)0; vxs_[k] = (R)0; vys_[k] = (R)0; vzs_[k] = (R)0;
+        if (p < a.n) { cel[k] = a.cell[p]; xs_[k] = a.x[p]; ys_[k] = a.y[p]; vxs_[k] = a.vx[p]; vys_[k] = a.vy[p]; vzs_[k] = a.vz[p]; }
+    }
+    __syncthreads();     // window zeroed before any deposit
+#pragma unroll
+    for (int k = 0; k < PUSH_PPT; ++k) {
+        const int64_t p = wbase + 64 * k + lane;
+        const bool in = p < a.n;
         const int e_old = cel[k];
-        const int64_t p = p0 + k;
-        next[k] = -1;
-        bool live = e_old >= 0;
-        bool wall = false;
-        if (DO_PUSH && live) {
-            bool absorbed_obj = false;
-            wall = push_fast<R>(a, e_old, X[k], Y[k], VX[k], VY[k], absorbed_obj);
-            if (!wall) {
-                live = !(X[k] <= a.xmin);           // MoveAll deletion test, MCCInterface.f90:617-620
-                if (!(X[k] == X[k]) || !(Y[k] == Y[k]) || !(VX[k] == VX[k]) || !(VY[k] == VY[k])) {
-                    atomicOr(a.err, PIC_EBIT_NONFINITE);
-                    live = false;
-                }
-                ev.object += absorbed_obj && !live;
-                ev.deleted += !live;
-            }
-        }
-        if (DO_PUSH) {                               // wave-aggregated append to the wall list
-            const unsigned long long wm = __ballot(wall);
+        R X = xs_[k], Y = ys_[k], VX = vxs_[k], VY = vys_[k];
+        const R VZ = vzs_[k];
+        int next = -1;
+        bool dep = false;
+        if (DO_PUSH) {
+            bool defer = false;
+            if (e_old >= 0) { defer = push_hot<R>(a, e_old, X, Y, VX, VY, &next); dep = !defer; }
+            const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
             if (wm) {
                 unsigned int wb = 0;
                 if (lane == 0) wb = atomicAdd(a.wall_count, (unsigned int)__popcll(wm));
                 wb = __shfl(wb, 0, 64);
-                if (wall) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)p;
+                if (defer) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)p;
+            }
+        } else if (e_old >= 0) {
+            int bad = 0;
+            next = pic_locate_elem((double)X, (double)Y, m, &bad);
+            if (bad) {
+                atomicOr(a.err, PIC_EBIT_POSITION);
+                pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, next, a.species);
+            }
+            dep = true;
+        }
+        if (dep) {
+            // GetParChg_2D.f90:788-843: weights and kinetic energy (ParticleOne.f90:339-340, :128)
+            double W[4];
+            deposit_weights((double)X, (double)Y, next, m, W);
+            double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
+            double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
+            energy = energy * a.v_ref * a.v_ref;
+            energy = energy / PIC_JTOEV;
+            long long qv[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { qv[q] = pic_fix(W[q]); qv[4 + q] = pic_fix(W[q] * energy); }
+            if (next != rkey) {
+                if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
+                rkey = next;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] = qv[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[q] += qv[q];
             }
         }
-        if (!live || wall) continue;
-        int bad = 0;
-        double Xd = (double)X[k], Yd = (double)Y[k];
-        int e_new = pic_locate_elem(Xd, Yd, m, &bad);
-        if (bad) {
-            atomicOr(a.err, PIC_EBIT_POSITION);
-            pic_record(a.errinfo, PIC_EBIT_POSITION, p, X[k], Y[k], VX[k], VY[k], 0, e_old, e_new, a.species);
-        }
-        next[k] = e_new;
-        // GetParChg_2D.f90:788-843: weights and kinetic energy (ParticleOne.f90:339-340, :128)
-        double W[4];
-        deposit_weights(Xd, Yd, e_new, m, W);
-        double vxd = (double)VX[k], vyd = (double)VY[k], vzd = (double)VZ[k];
-        double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
-        energy = energy * a.v_ref * a.v_ref;
-        energy = energy / PIC_JTOEV;
-        long long qv[8];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { qv[q] = pic_fix(W[q]); qv[4 + q] = pic_fix(W[q] * energy); }
-        if (e_new != rkey) {
-            if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
-            rkey = e_new;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc[q] = qv[q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) acc[q] += qv[q];
+        // write back (a deferred particle keeps its old state for the wall kernel, which
+        // also writes its cell_next)
+        if (in) {
+            a.cell_next[p] = next;
+            if (DO_PUSH && dep) { a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY; }
         }
     }
     wave_deposit(rkey, m, acc, a.acc_w, a.acc_e, w, winW, winE);
-
-    // ---------------- write back
-    if (full) {
-        store8i(a.cell_next + p0, next);
-        if (DO_PUSH) {
-            Vec8<R>::store(a.x + p0, X); Vec8<R>::store(a.y + p0, Y);
-            Vec8<R>::store(a.vx + p0, VX); Vec8<R>::store(a.vy + p0, VY);
-        }
-    } else {
-        for (int k = 0; k < PUSH_PPT; ++k) {
-            int64_t p = p0 + k;
-            if (p >= a.n) break;
-            a.cell_next[p] = next[k];
-            if (DO_PUSH) { a.x[p] = X[k]; a.y[p] = Y[k]; a.vx[p] = VX[k]; a.vy[p] = VY[k]; }
-        }
-    }
-    if (DO_PUSH) {
-        wave_add(&a.counters[PIC_C_OBJECT], ev.object);
-        wave_add(&a.counters[PIC_C_DELETED], ev.deleted);
-    }
 
     // ---------------- flush the window
     __syncthreads();
@@ -598,6 +520,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.m = g.mesh;
     a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
     a.m.element_index = g.d_element_index; a.m.node_index = g.d_node_index;
+    a.m.elem_clean = g.d_elem_clean;
     a.objs = g.d_objects;
     if (sizeof(R) == 8) {
         a.efx = (const R*)g.d_efx; a.efy = (const R*)g.d_efy; a.coef5 = (const R*)g.d_coef5;

@@ -24,8 +24,8 @@ PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
     double modx, mody;
     if (m.pow2_map) {
         // exact: a power-of-two divisor (1/mapx is exact and x*(1/mapx) == x/mapx)
-        modx = rxp - m.mapx * (double)(long long)(rxp * m.imapx);
-        mody = ryp - m.mapy * (double)(long long)(ryp * m.imapy);
+        modx = rxp - m.mapx * trunc(rxp * m.imapx);
+        mody = ryp - m.mapy * trunc(ryp * m.imapy);
     } else {
         modx = fmod(rxp, m.mapx);
         mody = fmod(ryp, m.mapy);
@@ -44,6 +44,18 @@ PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
     return (j - 1) + (i - 1) * m.ney;
 }
 
+// Slot layout after a cell sort: within every full group of PIC_SORT_GROUP slots the sorted
+// position 8 l + k (l = 0..63, k = 0..7) sits at slot 64 k + l, so lane l of a wave that
+// walks the group 64 slots at a time meets consecutive sorted particles (a partial last
+// group keeps the sorted order).
+#define PIC_SORT_GROUP 512
+PIC_HD int64_t pic_sorted_position(int64_t slot, int64_t nfull) {
+    if (slot >= nfull) return slot;
+    const int64_t g = slot & ~(int64_t)(PIC_SORT_GROUP - 1);
+    const int r = (int)(slot - g);
+    return g + (int64_t)((r & 63) * (PIC_SORT_GROUP / 64) + (r >> 6));
+}
+
 // Sort tiles: SORT_TILE x SORT_TILE elements, tiles visited column by column with the
 // direction alternating (serpentine), elements inside a tile column-major.
 #define SORT_TILE 16
@@ -58,8 +70,18 @@ PIC_HD unsigned int pic_sort_key(int e, int ney, int nty) {
 // Fixed-point quantum of deposition: value * 2^40 rounded to nearest (even), as int64.
 // Integer sums are exact and independent of order, so the node moments are
 // bit-identical whatever the particle order, atomic interleaving or kernel fusion.
+// For |x| < 2^51, x + 1.5*2^52 rounds x to an integer (nearest even) in the low mantissa
+// bits, so the bit pattern minus that of 1.5*2^52 is the rounded value: one add and one
+// integer subtract instead of the generic double -> int64 conversion sequence.
 #if defined(__HIPCC__)
-__device__ inline long long pic_fix(double v) { return __double2ll_rn(v * PIC_FIX_SCALE); }
+__device__ inline long long pic_fix(double v) {
+    const double x = v * PIC_FIX_SCALE;
+    if (fabs(x) < 0x1p51) {
+        const double t = x + 0x1.8p52;
+        return __double_as_longlong(t) - __double_as_longlong(0x1.8p52);
+    }
+    return __double2ll_rn(x);
+}
 #endif
 
 // Device error bits.

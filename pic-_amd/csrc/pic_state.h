@@ -63,6 +63,7 @@ struct MeshParams {
     const float* xsf; const float* ysf;
     const int* element_index;           // nelem
     const int* node_index;              // nnodes
+    const unsigned char* elem_clean;    // nelem: 1 if no object cuts or comes near the element
     int n_objects;
     DevObject obj[PIC_MAX_OBJECTS];
 };
@@ -77,6 +78,7 @@ struct SpeciesHost {
     void* x = nullptr; void* y = nullptr;
     void* vx = nullptr; void* vy = nullptr; void* vz = nullptr;
     int* cell = nullptr; int* cell_next = nullptr;
+    void* spare[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // sort targets, swapped with x..vz
     long long* acc_w = nullptr;         // nnodes fixed-point sum W
     long long* acc_e = nullptr;         // nnodes fixed-point sum W*E
     bool acc_dirty = false;
@@ -211,6 +213,7 @@ struct PicState {
     uint64_t seed = 12345;
     int mcc_enabled = 1;
     int64_t step = 0;
+    unsigned char* d_elem_clean = nullptr;   // MeshParams::elem_clean
     unsigned int* d_err = nullptr;     // device error flags
     void* d_errinfo = nullptr;         // PicErrInfo of the first failing particle
     // scratch for selection / compaction
