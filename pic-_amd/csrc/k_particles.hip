@@ -332,55 +332,59 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& 
     return false;
 }
 
-// One particle's lookup and deposit (GetParChg_2D.f90:137-163, 788-843) into the global
-// node accumulators: the path of the particles the wall kernel moves.
-template <typename R>
-__device__ inline int locate_deposit_global(const PushArgs<R>& a, int64_t p, int e_old, R X, R Y, R VX, R VY, R VZ) {
-    const MeshParams& m = a.m;
-    int bad = 0;
-    double Xd = (double)X, Yd = (double)Y;
-    int e_new = pic_locate_elem(Xd, Yd, m, &bad);
-    if (bad) {
-        atomicOr(a.err, PIC_EBIT_POSITION);
-        pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, e_new, a.species);
-    }
-    double W[4];
-    deposit_weights(Xd, Yd, e_new, m, W);
-    double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
-    double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
-    energy = energy * a.v_ref * a.v_ref;
-    energy = energy / PIC_JTOEV;
-    int ci = e_new / m.ney, cj = e_new - ci * m.ney;
-    int nd[4] = {ci * m.nny + cj, (ci + 1) * m.nny + cj, (ci + 1) * m.nny + cj + 1, ci * m.nny + cj + 1};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        atomicAdd((unsigned long long*)&a.acc_w[nd[q]], (unsigned long long)pic_fix(W[q]));
-        atomicAdd((unsigned long long*)&a.acc_e[nd[q]], (unsigned long long)pic_fix(W[q] * energy));
-    }
-    return e_new;
-}
-
-// The particles whose first pass reached a wall: the whole MoveOne/AdjustParticleBoundary
-// loop, lookup, deposit, write-back. Grid-stride over the list the push kernel filled.
+// The particles the push kernel handed over: the whole MoveOne/AdjustParticleBoundary loop,
+// lookup, write-back, and the deposit merged per wave (segmented by element, the list is
+// in slot order within each wave's append, so neighbours mostly share an element) before
+// the global atomics. Grid-stride over the list, every lane of a wave in every round.
 template <typename R>
 __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
+    const MeshParams& m = a.m;
     const unsigned int nw = *a.wall_count;
-    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) {
-        const int64_t p = a.wall_list[i];
-        const int e_old = a.cell[p];
-        R X = a.x[p], Y = a.y[p], VX = a.vx[p], VY = a.vy[p], VZ = a.vz[p];
-        bool vzc = false;
-        PushEvents ev = {0, 0, 0, 0, 0};
-        bool live = move_one<R>(a, p, e_old, X, Y, VX, VY, VZ, vzc, ev);
-        a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
-        if (vzc) a.vz[p] = VZ;
-        a.cell_next[p] = live ? locate_deposit_global<R>(a, p, e_old, X, Y, VX, VY, VZ) : -1;
-        if (ev.emit) atomicAdd(&a.counters[PIC_C_EMIT], (unsigned long long)ev.emit);
-        if (ev.right) atomicAdd(&a.counters[PIC_C_RIGHT], (unsigned long long)ev.right);
-        if (ev.object) atomicAdd(&a.counters[PIC_C_OBJECT], (unsigned long long)ev.object);
-        if (ev.reflect) atomicAdd(&a.counters[PIC_C_REFLECT], (unsigned long long)ev.reflect);
-        if (ev.deleted) atomicAdd(&a.counters[PIC_C_DELETED], (unsigned long long)ev.deleted);
+    const DepWindow none = {0, 0, 0, 0};
+    const unsigned int lane = threadIdx.x & 63;
+    const unsigned int wave0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
+    const unsigned int stride = gridDim.x * blockDim.x;
+    PushEvents ev = {0, 0, 0, 0, 0};
+    for (unsigned int base = wave0; base < nw; base += stride) {
+        const unsigned int i = base + lane;
+        int key = -1;
+        long long v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = 0;
+        if (i < nw) {
+            const int64_t p = a.wall_list[i];
+            const int e_old = a.cell[p];
+            R X = a.x[p], Y = a.y[p], VX = a.vx[p], VY = a.vy[p], VZ = a.vz[p];
+            bool vzc = false;
+            bool live = move_one<R>(a, p, e_old, X, Y, VX, VY, VZ, vzc, ev);
+            a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
+            if (vzc) a.vz[p] = VZ;
+            if (live) {
+                int bad = 0;
+                double Xd = (double)X, Yd = (double)Y;
+                key = pic_locate_elem(Xd, Yd, m, &bad);
+                if (bad) {
+                    atomicOr(a.err, PIC_EBIT_POSITION);
+                    pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, key, a.species);
+                }
+                double W[4];
+                deposit_weights(Xd, Yd, key, m, W);
+                double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
+                double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
+                energy = energy * a.v_ref * a.v_ref;
+                energy = energy / PIC_JTOEV;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { v[q] = pic_fix(W[q]); v[4 + q] = pic_fix(W[q] * energy); }
+            }
+            a.cell_next[p] = key;
+        }
+        wave_deposit(key, m, v, a.acc_w, a.acc_e, none, nullptr, nullptr);
     }
+    wave_add(&a.counters[PIC_C_EMIT], ev.emit);
+    wave_add(&a.counters[PIC_C_RIGHT], ev.right);
+    wave_add(&a.counters[PIC_C_OBJECT], ev.object);
+    wave_add(&a.counters[PIC_C_REFLECT], ev.reflect);
+    wave_add(&a.counters[PIC_C_DELETED], ev.deleted);
 }
 
 // Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
