@@ -197,6 +197,27 @@ def apply_collapsed(k81, u):
     return y
 
 
+def collapsed_stencil(k81):
+    """The 5 x 5 stencil k25[di+2][dj+2] = sum over tap pairs with d_t2 - d_t = (di, dj) of
+    k81[t][t2]: equal to the tap-pair form wherever every p - d_t is on the grid."""
+    k25 = np.zeros((5, 5))
+    for t, (ti, tj) in enumerate(TAPS):
+        for t2, (si, sj) in enumerate(TAPS):
+            k25[si - ti + 2, sj - tj + 2] += k81[t, t2]
+    return k25
+
+
+def apply_stencil(k25, u):
+    """y(p) = sum_delta k25[delta] u(p + delta), zero outside the grid."""
+    H, W = u.shape
+    up = np.zeros((H + 4, W + 4)); up[2:-2, 2:-2] = u
+    y = np.zeros((H, W))
+    for di in range(5):
+        for dj in range(5):
+            y += k25[di, dj] * up[di:di + H, dj:dj + W]
+    return y
+
+
 def forward(params, lay, levels, r):
     """y = M r on a flat node vector; returns y and the smoother inputs per learned level."""
     net = _Net(params, lay, levels)

@@ -48,6 +48,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 #define LN_MAX_LEVELS 12
 #define LN_MAX_LEARNED 4
+#define LN_KW 128                                    // smoother weights per level: k81 then k25
+#define LN_K25 96
 #define LN_OMEGA 0.8f
 #define LN_MAX_COARSE 2048
 
@@ -175,12 +177,15 @@ struct MgArgs {
     float omega;
     int jacobi;                  // 1: x = omega/diag r here (down) and the post-smoothing too (up)
     const double* done;
+    const float* dh;             // learned levels: diag^-1/2 and the 81 tap-pair smoother weights
+    const float* k81;
 };
 
 // Down, tiled: a block owns CI x CJ coarse nodes. It stages x on the fine nodes
 // [2I0-2, 2I0+2CI] x [2J0-2, 2J0+2CJ] (Jacobi levels: omega/diag r, computed here and written
-// back for the owned fine nodes) and r on the restriction footprint in LDS, forms the fine
-// residual s = r - A x on the footprint, then rc = R s.
+// back for the owned fine nodes; learned levels: the smoother's output) and r on the
+// restriction footprint in LDS, forms the fine residual s = r - A x on the footprint, then
+// rc = R s.
 #define MGD_CI 8
 #define MGD_CJ 32
 #define MGD_XI (2 * MGD_CI + 3)
@@ -276,6 +281,127 @@ __global__ __launch_bounds__(256) void k_mg_down(MgArgs a) {
     __shared__ float ss[MGD_NS];
     if (a.jacobi) mg_down_body<true>(a, xs, rx, ss);
     else mg_down_body<false>(a, xs, rx, ss);
+}
+
+// Down on a learned level with the pre-smoothing fused in: the block computes the learned
+// smoother x = Dh K Dh r (81 tap pairs, the arithmetic of k_smooth_k81) on the staged fine
+// nodes from Dh r on a further halo of 2, writes x on its owned fine nodes, then forms the
+// residual and restricts it exactly as k_mg_down does.
+#define MGF_UI (MGD_XI + 4)
+#define MGF_UJ (MGD_XJ + 4)
+#define MGF_NU (MGF_UI * MGF_UJ)
+#define MGF_UIT ((MGF_NU + 255) / 256)
+__global__ __launch_bounds__(256) void k_mg_smooth_down(MgArgs a) {
+    __shared__ float us[MGF_NU];                 // Dh r on the U region (0 outside the grid)
+    __shared__ float ru[MGF_NU];                 // r on the U region
+    __shared__ float du[MGF_NU];                 // Dh on the U region
+    __shared__ float xs[MGD_NX];
+    __shared__ float ss[MGD_NS];
+    __shared__ float kw[LN_KW];
+    const int b = blockIdx.z;
+    const int I0 = blockIdx.y * MGD_CI, J0 = blockIdx.x * MGD_CJ;
+    const int nf = a.Hf * a.Wf;
+    const float* r = a.r + (int64_t)b * a.r_bs;
+    float* x = a.x + (int64_t)b * a.fs;
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;          // X region origin
+    const int ui0 = fi0 - 2, uj0 = fj0 - 2;                // U region origin
+    // ---- all global loads first
+    float dv[MGD_SIT][9];
+#pragma unroll
+    for (int it = 0; it < MGD_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+        const int fi = fi0 + 1 + u, fj = fj0 + 1 + v;
+        const bool in = k < MGD_NS && fi >= 0 && fi < a.Hf && fj >= 0 && fj < a.Wf;
+        const int f = in ? fi * a.Wf + fj : 0;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) dv[it][t] = a.dia[(int64_t)t * nf + f];
+        if (!in) dv[it][4] = 0.0f;
+    }
+    float hr[MGF_UIT], hd[MGF_UIT];
+#pragma unroll
+    for (int it = 0; it < MGF_UIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        const int u = k / MGF_UJ, v = k - u * MGF_UJ;
+        const int gi = ui0 + u, gj = uj0 + v;
+        const bool in = k < MGF_NU && gi >= 0 && gi < a.Hf && gj >= 0 && gj < a.Wf;
+        const int g = in ? gi * a.Wf + gj : 0;
+        hr[it] = r[g];
+        hd[it] = a.dh[g];
+        if (!in) { hr[it] = 0.0f; hd[it] = 0.0f; }
+    }
+    const float kv = a.k81[threadIdx.x < LN_KW ? threadIdx.x : 0];
+    if (a.done && *a.done != 0.0) return;
+#pragma unroll
+    for (int it = 0; it < MGF_UIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGF_NU) { us[k] = hd[it] * hr[it]; ru[k] = hr[it]; du[k] = hd[it]; }
+    }
+    if (threadIdx.x < LN_KW) kw[threadIdx.x] = kv;
+    __syncthreads();
+    // ---- learned smoother on the X region
+#pragma unroll
+    for (int it = 0; it < MGD_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGD_NX) {
+            const int u = k / MGD_XJ, v = k - u * MGD_XJ;
+            const int i = fi0 + u, j = fj0 + v;
+            float val = 0.0f;
+            if (i >= 0 && i < a.Hf && j >= 0 && j < a.Wf) {
+                const float dp = du[(u + 2) * MGF_UJ + (v + 2)];
+                float sm = 0.0f;
+                if (dp != 0.0f) {
+                    if (i >= 1 && i <= a.Hf - 2 && j >= 1 && j <= a.Wf - 2) {
+#pragma unroll
+                        for (int d = 0; d < 25; ++d)
+                            sm = sm + kw[LN_K25 + d] * us[(u + d / 5) * MGF_UJ + (v + d % 5)];
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 9; ++t) {
+                            const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);   // intermediate pixel p - d_t
+                            if (qi < 0 || qi >= a.Hf || qj < 0 || qj >= a.Wf) continue;
+#pragma unroll
+                            for (int t2 = 0; t2 < 9; ++t2)
+                                sm = sm + kw[9 * t + t2] * us[(u + 2 - (t / 3 - 1) + (t2 / 3 - 1)) * MGF_UJ + (v + 2 - (t % 3 - 1) + (t2 % 3 - 1))];
+                        }
+                    }
+                    sm = dp * sm;
+                }
+                val = sm;
+                const int ou = u - 2, ov = v - 2;                  // owned fine nodes
+                if (ou >= 0 && ou < 2 * MGD_CI && ov >= 0 && ov < 2 * MGD_CJ) x[i * a.Wf + j] = val;
+            }
+            xs[k] = val;
+        }
+    }
+    __syncthreads();
+    // ---- residual on the restriction footprint, then restriction (as k_mg_down)
+#pragma unroll
+    for (int it = 0; it < MGD_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGD_NS) {
+            const int u = k / MGD_SJ, v = k - u * MGD_SJ;
+            float sv = 0.0f;
+            if (dv[it][4] != 0.0f) {
+                sv = ru[(u + 3) * MGF_UJ + (v + 3)];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) sv = sv - dv[it][t] * xs[(u + t / 3) * MGD_XJ + (v + t % 3)];
+            }
+            ss[k] = sv;
+        }
+    }
+    __syncthreads();
+    const int ti = threadIdx.x / MGD_CJ, tj = threadIdx.x - (threadIdx.x / MGD_CJ) * MGD_CJ;
+    const int I = I0 + ti, J = J0 + tj;
+    if (I < a.Hc && J < a.Wc) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int u = -1; u <= 1; ++u)
+#pragma unroll
+            for (int v = -1; v <= 1; ++v)
+                acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * ss[(2 * ti + 1 + u) * MGD_SJ + (2 * tj + 1 + v)];
+        a.rc[(int64_t)b * a.cs + I * a.Wc + J] = acc;
+    }
 }
 
 // Up, tiled: a block owns FI x FJ fine nodes; it stages ec on the covering coarse nodes and x
@@ -393,7 +519,7 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, cons
 // from LDS. Replaces ~2 launches per level whose run time is pure latency at these sizes.
 #define MG_CB_THREADS 1024
 #define MG_CB_MAXN 1089
-#define MG_CB_OPS_IT 5                               // float4 loads per thread for the operators
+#define MG_CB_OPS_IT 6                               // float4 loads per thread for the operators
 #define MG_CB_OPS_CAP (MG_CB_OPS_IT * MG_CB_THREADS * 4)
 #define MG_CB_RIT ((MG_CB_MAXN + MG_CB_THREADS - 1) / MG_CB_THREADS)
 static_assert(MG_CB_RIT * MG_CB_THREADS >= LN_MAX_COARSE, "coarse workgroup stages the coarsest level too");
@@ -547,7 +673,8 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
 }
 
 // Learned smoother, collapsed: out = [out +] Dh (In^T S In) (Dh v) on one level with the
-// 81 tap-pair weights k81 (zero outside the grid, as the factored form). A block owns
+// 5 x 5 stencil k25 on interior nodes and the 81 tap-pair weights k81 on the outer ring (zero
+// outside the grid, as the factored form). A block owns
 // SMI x SMJ nodes and stages Dh v on the tile with a halo of 2 in LDS.
 #define SMI 4
 #define SMJ 64
@@ -557,12 +684,12 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
                                                     int64_t v_bs, float* out, int64_t o_bs, int add, const double* done) {
     static_assert(SMI * SMJ == 256, "k_smooth_k81 tile");
     __shared__ float us[SM_NU];
-    __shared__ float kw[81];
+    __shared__ float kw[LN_KW];
     const float* vb = v + (int64_t)blockIdx.z * v_bs;
     float* ob = out + (int64_t)blockIdx.z * o_bs;
     const int i0 = blockIdx.y * SMI, j0 = blockIdx.x * SMJ;
     // all global loads first: weights, the scaled halo, the own node's scale and output
-    const float kv = k81[threadIdx.x < 81 ? threadIdx.x : 0];
+    const float kv = k81[threadIdx.x < LN_KW ? threadIdx.x : 0];
     float hd[SM_UIT], hv[SM_UIT];
 #pragma unroll
     for (int it = 0; it < SM_UIT; ++it) {
@@ -582,7 +709,7 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
     const float dp = dh[p];
     const float o0 = add ? ob[p] : 0.0f;
     if (done && *done != 0.0) return;
-    if (threadIdx.x < 81) kw[threadIdx.x] = kv;
+    if (threadIdx.x < LN_KW) kw[threadIdx.x] = kv;
 #pragma unroll
     for (int it = 0; it < SM_UIT; ++it) {
         const int k = threadIdx.x + it * 256;
@@ -592,13 +719,18 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
     if (!own) return;
     float s = 0.0f;
     if (dp != 0.0f) {
+        if (i >= 1 && i <= H - 2 && j >= 1 && j <= W - 2) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);   // intermediate pixel p - d_t
-            if (qi < 0 || qi >= H || qj < 0 || qj >= W) continue;
+            for (int d = 0; d < 25; ++d) s = s + kw[LN_K25 + d] * us[(ti + d / 5) * (SMJ + 4) + (tj + d % 5)];
+        } else {
 #pragma unroll
-            for (int t2 = 0; t2 < 9; ++t2)
-                s = s + kw[9 * t + t2] * us[(ti + 2 - (t / 3 - 1) + (t2 / 3 - 1)) * (SMJ + 4) + (tj + 2 - (t % 3 - 1) + (t2 % 3 - 1))];
+            for (int t = 0; t < 9; ++t) {
+                const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);   // intermediate pixel p - d_t
+                if (qi < 0 || qi >= H || qj < 0 || qj >= W) continue;
+#pragma unroll
+                for (int t2 = 0; t2 < 9; ++t2)
+                    s = s + kw[9 * t + t2] * us[(ti + 2 - (t / 3 - 1) + (t2 / 3 - 1)) * (SMJ + 4) + (tj + 2 - (t % 3 - 1) + (t2 % 3 - 1))];
+            }
         }
         s = dp * s;
     }
@@ -732,21 +864,36 @@ __global__ void k_adam(int64_t n, float* p, const float* gr, float* m, float* v,
 // Collapsed smoother: In^T S In through the channels is, tap pair by tap pair,
 //   k81[t][t2] = sum_{c, c2} W[t][c] S[c][c2] W[t2][c2]      (FP32 from the master weights)
 // applied as y(p) = sum_t [p - d_t on the grid] sum_t2 k81[t][t2] u(p - d_t + d_t2), which is
-// exactly the factored form (its intermediate channels exist on the grid only). One block.
-__global__ void k_pack_k81(int C, const float* W, const float* Q, float* k81) {
+// exactly the factored form (its intermediate channels exist on the grid only). Where every
+// p - d_t is on the grid (all but the outer ring of nodes) this is the 5 x 5 stencil
+//   k25[delta] = sum_{d_t2 - d_t = delta} k81[t][t2]           (summed in FP64, rounded once)
+// One block per level writes both: kw[0..80] = k81, kw[LN_K25 .. LN_K25+24] = k25.
+__global__ void k_pack_k81(int C, const float* W, const float* Q, float* kw) {
+    __shared__ float k81s[81];
     const int d = threadIdx.x;
-    if (d >= 81) return;
-    const int t = d / 9, t2 = d - t * 9;
-    float acc = 0.0f;
-    for (int c = 0; c < C; ++c) {
-        float wc = W[((int64_t)t * C + c) * 16];
-        for (int c2 = 0; c2 < C; ++c2) {
-            float sc = 0.0f;                             // S[c][c2]
-            for (int m = 0; m < C; ++m) sc += Q[c * C + m] * Q[c2 * C + m];
-            acc += wc * sc * W[((int64_t)t2 * C + c2) * 16];
+    if (d < 81) {
+        const int t = d / 9, t2 = d - t * 9;
+        float acc = 0.0f;
+        for (int c = 0; c < C; ++c) {
+            float wc = W[((int64_t)t * C + c) * 16];
+            for (int c2 = 0; c2 < C; ++c2) {
+                float sc = 0.0f;                         // S[c][c2]
+                for (int m = 0; m < C; ++m) sc += Q[c * C + m] * Q[c2 * C + m];
+                acc += wc * sc * W[((int64_t)t2 * C + c2) * 16];
+            }
         }
+        kw[d] = acc;
+        k81s[d] = acc;
     }
-    k81[d] = acc;
+    __syncthreads();
+    if (d < 25) {
+        const int di = d / 5 - 2, dj = d % 5 - 2;
+        double acc = 0.0;
+        for (int t = 0; t < 9; ++t)
+            for (int t2 = 0; t2 < 9; ++t2)
+                if (t2 / 3 - t / 3 == di && t2 % 3 - t % 3 == dj) acc += (double)k81s[9 * t + t2];
+        kw[LN_K25 + d] = (float)acc;
+    }
 }
 
 // Packed bf16 weights of one learned smoother (the lifts of the gradient) from W_in
@@ -1088,7 +1235,7 @@ struct LearnNet {
     float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr, *d_dS = nullptr;
     bf16* d_pk = nullptr;
     bf16 *pin[LN_MAX_LEARNED] = {nullptr}, *pSin[LN_MAX_LEARNED] = {nullptr};
-    float* d_k81 = nullptr;                          // collapsed smoother tap-pair weights [Ls][81]
+    float* d_k81 = nullptr;                          // collapsed smoother weights [Ls][LN_KW] (k81, k25)
     int adam_t = 0;
     // activations for a batch of bcap samples
     int bcap = 0;
@@ -1169,7 +1316,7 @@ static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, in
 static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, int64_t o_bs, bool add, const double* done) {
     const MgLevel& lv = ln.lv[l];
     dim3 grid((unsigned)((lv.W + SMJ - 1) / SMJ), (unsigned)((lv.H + SMI - 1) / SMI), batch);
-    hipLaunchKernelGGL(k_smooth_k81, grid, dim3(256), 0, g.stream, lv.H, lv.W, ln.d_k81 + 81 * l, lv.dh, v, v_bs, out,
+    hipLaunchKernelGGL(k_smooth_k81, grid, dim3(256), 0, g.stream, lv.H, lv.W, ln.d_k81 + LN_KW * l, lv.dh, v, v_bs, out,
                        o_bs, add ? 1 : 0, done);
 }
 
@@ -1179,11 +1326,19 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
     for (int l = 0; l < lc; ++l) {
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
-        if (l < ln.Ls) smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done);
         MgArgs a = mg_args(l, run, r, r_bs, nullptr, 0, done);
         const MgLevel& c = ln.lv[l + 1];
         dim3 grid((unsigned)((c.W + MGD_CJ - 1) / MGD_CJ), (unsigned)((c.H + MGD_CI - 1) / MGD_CI), batch);
-        hipLaunchKernelGGL(k_mg_down, grid, dim3(256), 0, g.stream, a);
+        // the fused smoother + down pays where its grid fills the GPU; on smaller levels the
+        // smoother's own launch spreads the 25-tap work over more blocks
+        const bool fuse = l < ln.Ls && (int64_t)grid.x * grid.y * batch >= 256;
+        if (l < ln.Ls && !fuse) smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done);
+        if (fuse) {
+            a.dh = ln.lv[l].dh; a.k81 = ln.d_k81 + LN_KW * l;
+            hipLaunchKernelGGL(k_mg_smooth_down, grid, dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL(k_mg_down, grid, dim3(256), 0, g.stream, a);
+        }
     }
     if (lc == 0) return pic_fail(PIC_ERR_STATE, "learned network: coarse part without a fine level");
     {
@@ -1221,7 +1376,7 @@ static int pack_weights() {
         hipLaunchKernelGGL(k_pack_smoother, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l],
                            ln.d_p + ln.off_q[l], ln.pin[l], ln.pSin[l]);
         hipLaunchKernelGGL(k_pack_k81, dim3(1), dim3(128), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l], ln.d_p + ln.off_q[l],
-                           ln.d_k81 + 81 * l);
+                           ln.d_k81 + LN_KW * l);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
@@ -1478,7 +1633,7 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     size_t nb = 1;
     for (int l = 0; l < ln.Ls; ++l) nb += 2 * 9 * (size_t)ln.C[l] * 16;
     HIPCHK(hipMalloc(&ln.d_pk, nb * sizeof(bf16)));
-    HIPCHK(hipMalloc(&ln.d_k81, 25 * LN_MAX_LEARNED * sizeof(float)));
+    HIPCHK(hipMalloc(&ln.d_k81, LN_KW * LN_MAX_LEARNED * sizeof(float)));
     bf16* q = ln.d_pk;
     for (int l = 0; l < ln.Ls; ++l) {
         size_t s = 9 * (size_t)ln.C[l] * 16;

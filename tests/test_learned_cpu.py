@@ -186,3 +186,21 @@ def test_collapsed_smoother_equals_factored_smoother():
     y_fact = lrn.conv_same(E, np.transpose(w_in[0], (0, 2, 1)), sign=-1)[:, :, 0]
     y_col = lrn.apply_collapsed(k81, v)
     assert np.allclose(y_fact, y_col, rtol=1e-12, atol=1e-12)
+
+
+def test_stencil_equals_tap_pairs_on_interior_nodes():
+    """The 5 x 5 stencil the library applies away from the grid edge equals the 81 tap-pair
+    form there (not on the outer ring, where the pair form drops off-grid intermediates)."""
+    rng = np.random.default_rng(6)
+    n = 12
+    lay = lrn.layout(n, n, [16], learned_levels=1)
+    p = lrn.initial_params(lay, small=0.3)
+    w_in, q = lrn.unpack(p, lay)
+    k81 = lrn.collapsed_pairs(w_in[0], q[0])
+    k25 = lrn.collapsed_stencil(k81)
+    assert np.isclose(k25.sum(), k81.sum())
+    v = rng.standard_normal((n, n))
+    y81 = lrn.apply_collapsed(k81, v)
+    y25 = lrn.apply_stencil(k25, v)
+    assert np.allclose(y81[1:-1, 1:-1], y25[1:-1, 1:-1], rtol=1e-12, atol=1e-12)
+    assert not np.allclose(y81[0, :], y25[0, :])
