@@ -2,12 +2,14 @@
 """Benchmark of the MI355X IFE-PIC hot path on one GPU.
 
 Workload (default): 512 x 512 Cartesian mesh, three immersed conductors at -1, 0.5, 0,
-16.8M electrons + 16.8M Ar+ ions (71 per vacuum element per species), FP32 gather / push /
-MCC with fixed-point (int64) deposition, mixed-precision field solve (FP32 inner, FP64
-refinement) to a 1e-8 relative residual, PPR recovery, null-collision MCC against 300 K
-argon at 3.2e20 m^-3, periodic cell sort. A step is one full explicit PIC step (MoveAll x2,
-GetParChg_2D, IFE_Solve_2D, GetEField_SIDG_PPR, MCC, sort every 20 steps) with every input
-resident in HBM. Prints one JSON line.
+16.8M electrons + 16.8M Ar+ ions (71 per vacuum element per species) plus both species
+injected through the left wall (InjectBeams_2D: Maxwellian flux drifting inward at 0.1 v_th,
+at the plasma density), FP32 gather / push / MCC with fixed-point (int64) deposition,
+mixed-precision field solve (FP32 inner, FP64 refinement) to a 1e-8 relative residual, PPR
+recovery, null-collision MCC against 300 K argon at 3.2e20 m^-3, periodic cell sort. A step is
+one full explicit PIC step (MoveAll x2, GetParChg_2D, IFE_Solve_2D, GetEField_SIDG_PPR,
+InjectBeams_2D, MCC, sort every 20 steps) with every input resident in HBM. Prints one JSON
+line.
 
 The timed steps use the learned multigrid preconditioner (FCG inner solver, plain CG
 fallback), trained on the GPU on pairs the plain solver harvests during the warm-up; the
@@ -96,6 +98,7 @@ def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
     return {"value": n_total / t_step, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
             "sample": f"oracle.c scalar FP64 step: mover+lookup+deposit timed on {sample_particles} particles per "
                       f"species and scaled to {n_total}; RHS + MY_PCG (1e-8, {it} it) + PPR on the full mesh; "
+                      f"MCC and the wall injection (~1e3 new particles per step) not included; "
                       f"{t_step:.2f} s per step"}
 
 
@@ -246,7 +249,8 @@ def main() -> int:
         "warmup": args.warmup + (2 if learned else 0), "ms_per_step": main_run["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: 512x512 Cartesian, 3 conductors, 2 species x 16.8M particles, "
-                               "FP32 particles + mixed-precision field solve to 1e-8, MCC, sort/20",
+                               "left-wall injection (drifting Maxwellian, 0.1 v_th), FP32 particles + "
+                               "mixed-precision field solve to 1e-8, MCC, sort/20",
                    "mesh": [dom.nnx - 1, dom.nny - 1], "particles_per_species": int(main_run["n0"] // 2),
                    "solver": "learned multigrid FCG (plain CG fallback)" if learned else "plain one-kernel CG",
                    "parallelism": "single GPU"},

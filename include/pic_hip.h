@@ -229,10 +229,32 @@ int  pic_mcc_set_table(int isp, int nreact, int nsigma, double energy_min_log10,
                        double sigma_max, double collision_ratio, int product_species);
 int  pic_mcc_get_table(int isp, double* prob, int64_t n, double* sigma_max, double* collision_ratio);
 
+/* Wall-injection source of one species: the injection block of pic.inp as Input_2D.f90:333-344
+ * reads it and :489-494 normalises it (density dens0 in m^-3, temperature tmpj in eV, drift
+ * vd in m/s), the slab start R_start and the wall's y extent [f_left_wall(2), f_right_wall(2)]
+ * in mesh units. density_m3 = 0 removes the source. Cartesian only (delta = 0). */
+typedef struct {
+    double density_m3;
+    double temp_eV;
+    double drift_ms[3];
+    double r_start;
+    double y_lo, y_hi;
+} pic_injection;
+int  pic_set_injection(int isp, const pic_injection* p);
+/* InjectBeams_2D(it, delta, PB) (IMPIC/InjectBeams_2D.f90:1-160): for every registered
+ * source, add_N = INT(n |b_amb| v_ref dt t_ref (y_hi-y_lo) L_ref / Weight + R) new particles
+ * with the inward velocity of a drifting Maxwellian flux (Bird's sampler, :89-106), the
+ * transverse ones from Loadv (+ the ion drift), a uniform position in the slab
+ * [R_start, R_start + b_amb dt / 2] x [y_lo, y_hi], located by InjectPositioning and appended
+ * after the occupied slots. `it` is the random stream's step word. added (optional, 4
+ * entries, one per species slot) receives the counts. */
+int  pic_InjectBeams_2D(int it, int delta, int64_t* added);
+
 /* One full explicit step (Main_IFE_Test_2.f90:329-341 followed by MCC, the order of
  * MCC_jw/code/OneStep.F90:49-67): MoveAll for every species, GetParChg_2D,
- * IFE_Solve_2D, GetEField_SIDG_PPR, MCC (if tables are registered), and the periodic
- * cell sort. nsteps steps back to back. */
+ * IFE_Solve_2D, GetEField_SIDG_PPR, InjectBeams_2D (when a source is registered, at the
+ * place of the reference's call, Main_IFE_Test_2.f90:344), MCC (if tables are registered),
+ * and the periodic cell sort. nsteps steps back to back. */
 int  pic_step(int nsteps);
 /* Initial field: GetParChg_2D + IFE_Solve_2D + GetEField_SIDG_PPR (Main:221-233). */
 int  pic_initial_field(void);

@@ -661,3 +661,86 @@ double or_mcc_table(int nreact, int nsigma, double emin, double eint, double mas
     for (size_t k = 0; k < (size_t)nsigma * nreact; ++k) prob[k] = prob[k] / mx;
     return mx;
 }
+
+/* ------------------------------------------------------------------ wall injection */
+/* InjectBeams_2D (IMPIC/InjectBeams_2D.f90:40-143) with the build's per-particle Philox
+ * streams. Constants (:42-50, :91-95, Loadv.f90:14; SQRT(2.) is single precision in the
+ * reference): consts = {b_amb, v_r, FS1, FS2, QA, sqrt(2) vt, VMP, XL, XU}. */
+void or_inject_consts(double vd1, double vt, double tmpj, double xm, double r_start, double dt, double* c) {
+    const double s2 = (double)sqrtf(2.0f);
+    double b;
+    if (vd1 > 0.0)
+        b = vt / sqrt(2.0 * OR_PI) * exp(-(vd1 * vd1) / (2.0 * vt * vt)) + vd1 / 2.0 * (1.0 + erf(vd1 / (s2 * vt)));
+    else
+        b = -vt / sqrt(2.0 * OR_PI) * exp(-(vd1 * vd1) / (2.0 * vt * vt)) + vd1 / 2.0 * (1.0 - erf(vd1 / (s2 * vt)));
+    const double v_r = fabs(vd1) / (s2 * vt);
+    const double fs1 = v_r + sqrt(v_r * v_r + 2.0);
+    c[0] = b; c[1] = v_r; c[2] = fs1; c[3] = 0.5 * (1.0 + v_r * (2.0 * v_r - fs1));
+    c[4] = v_r < -3.0 ? fabs(v_r) + 1.0 : 3.0;
+    c[5] = s2 * vt;
+    c[6] = sqrt(2.0 * tmpj / xm);
+    const double x2 = r_start + 0.5 * b * dt;
+    c[7] = r_start < x2 ? r_start : x2; c[8] = r_start < x2 ? x2 : r_start;
+}
+
+/* add_N = INT(dens0 n_ref |b_amb| v_ref dt t_ref (y_hi - y_lo) L_ref / Weight + R) (:56-57),
+ * R = the first draw of the stream at slot 0xFFFFFFFF. */
+int64_t or_inject_count(double expect_no_r, uint64_t seed, uint32_t step, uint32_t tag) {
+    or_stream s;
+    or_stream_init(&s, seed, 0xFFFFFFFFu, step, tag);
+    return (int64_t)(expect_no_r + or_u01(&s));
+}
+
+/* Candidate i of a rejection loop takes the two uniforms of Philox block i of its stream
+ * (a rejected candidate's second uniform goes unused). */
+static void or_block_u01(uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag, uint32_t blk, double* ra,
+                         double* rb) {
+    or_p4 o = or_philox(slot, step, tag, blk, (uint32_t)seed, (uint32_t)(seed >> 32));
+    *ra = (((double)((((uint64_t)o.v[0]) << 20) | (uint64_t)(o.v[1] >> 12))) + 0.5) * (1.0 / 4503599627370496.0);
+    *rb = (((double)((((uint64_t)o.v[2]) << 20) | (uint64_t)(o.v[3] >> 12))) + 0.5) * (1.0 / 4503599627370496.0);
+}
+
+static double or_loadv(uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag, double vmp) {   /* Loadv.f90:14-23 */
+    for (uint32_t i = 0;; ++i) {
+        double R, R1;
+        or_block_u01(seed, slot, step, tag, i, &R, &R1);
+        double v = (6.0 * R - 3.0) * vmp;
+        double pacc = exp(-v * v / (vmp * vmp));
+        if (!(R1 > pacc)) return v;
+    }
+}
+
+/* n particles of one source, each from four streams (slot 4k + c): c = 0 Bird's inward
+ * component (:91-106), c = 1, 2 Loadv for Vy, Vz (+ ion drift, :121-126), c = 3 PosInit's X
+ * then Y (ParticleOne.f90:82-89); positions rounded to the particle precision (bits 32:
+ * float) before InjectPositioning (InjectPositioning.f90:13-22, no clamp; cell -1 when
+ * outside). vd = drift {vx, vy, vz}; mesh: xmin, ymin, mapx, mapy, nex, ney. */
+void or_inject(int bits, int64_t n, int electron, const double* c, const double* vd, double y_lo, double y_hi,
+               uint64_t seed, uint32_t step, uint32_t tag, double xmin, double ymin, double mapx, double mapy,
+               int nex, int ney, double* x, double* y, double* vx, double* vy, double* vz, int* cell) {
+    for (int64_t k = 0; k < n; ++k) {
+        double UN = 0.0;
+        for (uint32_t i = 0;; ++i) {
+            double R, R2;
+            or_block_u01(seed, (uint32_t)(4 * k), step, tag, i, &R, &R2);
+            double UU = -c[4] + 2.0 * c[4] * R;
+            UN = UU + c[1];
+            if (UN < 0.0) continue;
+            double GG = (2.0 * UN / c[2]) * exp(c[3] - UU * UU);
+            if (GG < R2) continue;
+            break;
+        }
+        double Vx = UN * c[5];
+        double Vy = or_loadv(seed, (uint32_t)(4 * k + 1), step, tag, c[6]);
+        double Vz = or_loadv(seed, (uint32_t)(4 * k + 2), step, tag, c[6]);
+        if (!electron) { Vy = vd[1] + Vy; Vz = vd[2] + Vz; }
+        double RX, RY;
+        or_block_u01(seed, (uint32_t)(4 * k + 3), step, tag, 0, &RX, &RY);
+        double X = c[7] + (c[8] - c[7]) * RX;
+        double Y = y_lo + (y_hi - y_lo) * RY;
+        if (bits == 32) { X = (double)(float)X; Y = (double)(float)Y; Vx = (double)(float)Vx; Vy = (double)(float)Vy; Vz = (double)(float)Vz; }
+        int ii = (int)((X - xmin) / mapx) + 1, jj = (int)((Y - ymin) / mapy) + 1;
+        x[k] = X; y[k] = Y; vx[k] = Vx; vy[k] = Vy; vz[k] = Vz;
+        cell[k] = (ii < 1 || ii > nex || jj < 1 || jj > ney) ? -1 : (jj - 1) + (ii - 1) * ney;
+    }
+}

@@ -67,6 +67,14 @@ def lib():
         _lib.or_mcc_collide.argtypes = [C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_inject_consts.restype = None
+        _lib.or_inject_consts.argtypes = [C.c_double] * 6 + [C.c_void_p]
+        _lib.or_inject_count.restype = C.c_int64
+        _lib.or_inject_count.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]
+        _lib.or_inject.restype = None
+        _lib.or_inject.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                   C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 6
         _lib.or_mcc_table.restype = C.c_double
         _lib.or_mcc_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_void_p, C.c_void_p]
@@ -214,3 +222,27 @@ def mcc_table(nreact, nsigma, emin, eint, mass, gas_mass, density, sigma):
     smax = lib().or_mcc_table(nreact, nsigma, emin, eint, mass, gas_mass, density,
                               _p(np.ascontiguousarray(sigma, dtype=np.float64)), _p(prob))
     return prob, smax
+
+
+RNG_INJECT = 5
+
+
+def inject_consts(vd1, vt, tmpj, xm, r_start, dt):
+    """b_amb, v_r, FS1, FS2, QA, sqrt(2) vt, VMP, XL, XU of InjectBeams_2D / Loadv."""
+    c = np.zeros(9)
+    lib().or_inject_consts(vd1, vt, tmpj, xm, r_start, dt, _p(c))
+    return c
+
+
+def inject_count(expect_no_r, seed, step, tag_):
+    return int(lib().or_inject_count(expect_no_r, seed, step, tag_))
+
+
+def inject(bits, n, electron, consts, vd, y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney):
+    """n injected particles of one source: x, y, vx, vy, vz, 0-based cell (-1 outside)."""
+    out = [np.zeros(n) for _ in range(5)] + [np.zeros(n, dtype=np.int32)]
+    c = np.ascontiguousarray(consts, dtype=np.float64)
+    v = np.ascontiguousarray(vd, dtype=np.float64)
+    lib().or_inject(bits, n, electron, _p(c), _p(v), y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney,
+                    *[_p(a) for a in out])
+    return out

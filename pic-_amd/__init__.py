@@ -77,6 +77,11 @@ class GasParams(C.Structure):
     _fields_ = [("mass_kg", C.c_double), ("temperature_K", C.c_double), ("density_m3", C.c_double)]
 
 
+class Injection(C.Structure):
+    _fields_ = [("density_m3", C.c_double), ("temp_eV", C.c_double), ("drift_ms", C.c_double * 3),
+                ("r_start", C.c_double), ("y_lo", C.c_double), ("y_hi", C.c_double)]
+
+
 _P = C.POINTER
 _dp = _P(C.c_double)
 _ip = _P(C.c_int32)
@@ -111,6 +116,8 @@ _PROTOS = {
     "pic_mcc_set_table": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _dp, _ip, _dp, C.c_double,
                                     C.c_double, C.c_int]),
     "pic_mcc_get_table": (C.c_int, [C.c_int, _dp, C.c_int64, _dp, _dp]),
+    "pic_set_injection": (C.c_int, [C.c_int, _P(Injection)]),
+    "pic_InjectBeams_2D": (C.c_int, [C.c_int, C.c_int, _i64p]),
     "pic_step": (C.c_int, [C.c_int]),
     "pic_initial_field": (C.c_int, []),
     "pic_set_run_params": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32]),
@@ -267,8 +274,9 @@ def workload(name: str) -> dict:
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),
                      objects=[Circle(0.25 * n, 0.30 * n, 0.10 * n, -1.0), Circle(0.65 * n, 0.65 * n, 0.12 * n, 0.5),
                               Circle(0.70 * n, 0.22 * n, 0.08 * n, 0.0)])
-        # 71 per vacuum element: 16.8M particles per species (the conductors take 9.7% of the area)
-        return dict(domain=dom, ppc=71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026)
+        # 71 per vacuum element: 16.8M particles per species (the conductors take 9.7% of the area);
+        # both species injected through the left wall as a Maxwellian drifting inward at 0.1 v_th
+        return dict(domain=dom, ppc=71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026, inject_drift_vth=0.1)
     if name.startswith("small"):  # small parity cases: small<N>
         n = int(name[5:] or 16)
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),
@@ -403,6 +411,18 @@ class PicLibrary:
 
     def MCC(self):
         check(lib().pic_MCC())
+
+    def set_injection(self, isp: int, density_m3: float, temp_eV: float, drift_ms=(0.0, 0.0, 0.0),
+                      r_start: float = 0.0, y_lo: float = 0.0, y_hi: float = 1.0):
+        """Wall-injection source of species isp (pic.inp injection block); density 0 removes it."""
+        p = Injection(density_m3, temp_eV, (C.c_double * 3)(*drift_ms), r_start, y_lo, y_hi)
+        check(lib().pic_set_injection(isp, C.byref(p)))
+
+    def InjectBeams_2D(self, it: int, delta: int = 0) -> list:
+        """One injection of every registered source; returns the particles added per species."""
+        added = (C.c_int64 * 4)()
+        check(lib().pic_InjectBeams_2D(it, delta, added))
+        return list(added)
 
     def set_run_params(self, sort_every: int = 20, seed: int = 12345, mcc_enabled: int = 1):
         check(lib().pic_set_run_params(sort_every, seed, mcc_enabled))
@@ -593,5 +613,12 @@ def build_workload(pl: PicLibrary, name: str, seed: int = 2024, ppc: int | None 
     pl.load_maxwellian(1, ppc, w["Ti"], seed + 1)
     if mcc:
         pl.MCCBundleInit(ARGON_ION["mass_kg"], 300.0, gas_density, 0, 1)
+    if w.get("inject_drift_vth"):
+        # InjectBeams_2D source at x = xmin over the whole wall, at the plasma density
+        v_ref = dom.normalisation()["v_ref"]
+        for isp, (sp, temp) in enumerate(((ELECTRON, w["Te"]), (ARGON_ION, w["Ti"]))):
+            vt = np.sqrt(temp / dom.T_ref_eV / (sp["mass_kg"] / ELECTRON_MASS))
+            pl.set_injection(isp, dom.n_ref, temp, (w["inject_drift_vth"] * vt * v_ref, 0.0, 0.0),
+                             r_start=dom.xmin, y_lo=dom.ymin, y_hi=dom.ymax)
     pl.set_run_params(20, seed, 1 if mcc else 0)
     return w

@@ -55,6 +55,7 @@ static int check_err_flags() {
     if (e & PIC_EBIT_POSITION) return pic_fail(PIC_ERR_POSITION, "particle located outside the mesh" + where);
     if (e & PIC_EBIT_EKRHO) return pic_fail(PIC_ERR_POSITION, "Ek_tot /= 0 with rho == 0 (GetParChg_2D.f90:887-890)");
     if (e & PIC_EBIT_MCCLOOP) return pic_fail(PIC_ERR_BOUNDARY, "MCC rejection loop did not end");
+    if (e & PIC_EBIT_SAMPLER) return pic_fail(PIC_ERR_BOUNDARY, "injection rejection sampler did not end");
     return pic_fail(PIC_ERR_HIP, "device error flag " + std::to_string(e));
 }
 
@@ -146,6 +147,7 @@ static void free_species(int s) {
                     S.thr, S.mcc_events, S.spare[0], S.spare[1], S.spare[2], S.spare[3], S.spare[4]};
     for (void* p : ptrs) if (p) hipFree(p);
     S = SpeciesHost();
+    g.inj[s] = InjectHost();
 }
 
 extern "C" {
@@ -737,6 +739,62 @@ int pic_initial_field(void) {
     return check_err_flags();
 }
 
+int pic_set_injection(int isp, const pic_injection* p) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    if (!p) return pic_fail(PIC_ERR_ARG, "null injection parameters");
+    InjectHost& q = g.inj[isp];
+    q = InjectHost();
+    if (p->density_m3 == 0.0) return PIC_OK;
+    if (g.mesh.delta != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "wall injection is implemented for Cartesian domains");
+    if (!(p->density_m3 > 0) || !(p->temp_eV > 0)) return pic_fail(PIC_ERR_ARG, "injection density and temperature must be > 0");
+    if (!(p->y_hi > p->y_lo) || p->y_lo < g.mesh.ymin || p->y_hi > g.mesh.ymax || p->r_start < g.mesh.xmin ||
+        p->r_start > g.mesh.xmax)
+        return pic_fail(PIC_ERR_ARG, "injection slab outside the domain");
+    const SpeciesHost& S = g.sp[isp];
+    // Input_2D.f90:489-494
+    q.dens = p->density_m3 / g.cfg.n_ref;
+    q.tmpj = p->temp_eV * PIC_EVTOK / (g.cfg.T_ref_eV * PIC_EVTOK);
+    q.vt = std::sqrt(q.tmpj / S.xm);
+    for (int d = 0; d < 3; ++d) q.vd[d] = p->drift_ms[d] / g.v_ref;
+    q.r_start = p->r_start; q.y_lo = p->y_lo; q.y_hi = p->y_hi;
+    // InjectBeams_2D.f90:42-50 (SQRT(2.) is single precision there)
+    const double s2 = (double)std::sqrt(2.0f);
+    const double vd1 = q.vd[0], vt = q.vt;
+    if (vd1 > 0.0)
+        q.b_amb = vt / std::sqrt(2.0 * PIC_PI) * std::exp(-(vd1 * vd1) / (2.0 * vt * vt)) +
+                  vd1 / 2.0 * (1.0 + std::erf(vd1 / (s2 * vt)));
+    else
+        q.b_amb = -vt / std::sqrt(2.0 * PIC_PI) * std::exp(-(vd1 * vd1) / (2.0 * vt * vt)) +
+                  vd1 / 2.0 * (1.0 - std::erf(vd1 / (s2 * vt)));
+    // :91-95, :106
+    q.v_r = std::fabs(vd1) / (s2 * vt);
+    q.fs1 = q.v_r + std::sqrt(q.v_r * q.v_r + 2.0);
+    q.fs2 = 0.5 * (1.0 + q.v_r * (2.0 * q.v_r - q.fs1));
+    q.qa = q.v_r < -3.0 ? std::fabs(q.v_r) + 1.0 : 3.0;
+    q.sq2vt = s2 * vt;
+    q.vmp = std::sqrt(2.0 * q.tmpj / S.xm);                  // Loadv.f90:14
+    const double x2 = q.r_start + 0.5 * q.b_amb * g.cfg.dt;  // :73-76
+    q.xl = std::min(q.r_start, x2); q.xu = std::max(q.r_start, x2);
+    q.on = true;
+    return PIC_OK;
+}
+
+int pic_InjectBeams_2D(int it, int delta, int64_t* added) {
+    REQUIRE_DOMAIN();
+    if (delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "delta differs from the domain");
+    for (int s = 0; s < g.nspecies; ++s)
+        if (g.sp[s].pending) return pic_fail(PIC_ERR_STATE, "injection between MoveAll and GetParChg_2D");
+    TRY(run_injection(it, added));
+    return check_err_flags();
+}
+
+static bool injection_on() {
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s)
+        if (g.inj[s].on) return true;
+    return false;
+}
+
 int pic_step(int nsteps) {
     REQUIRE_DOMAIN();
     double acc[8] = {0};
@@ -757,6 +815,7 @@ int pic_step(int nsteps) {
         TRY(g.solver_mode == 0 ? run_pcg_fp64(true) : run_pcg_mixed());
         HIPCHK(hipEventRecord(g.ev[4], g.stream));
         TRY(launch_ppr());
+        if (injection_on()) TRY(run_injection((int)g.step, nullptr));     // Main_IFE_Test_2.f90:344
         HIPCHK(hipEventRecord(g.ev[5], g.stream));
         if (g.mcc_enabled && g.mcc_ready) TRY(run_mcc());
         HIPCHK(hipEventRecord(g.ev[6], g.stream));

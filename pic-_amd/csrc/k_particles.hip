@@ -732,3 +732,144 @@ int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, cons
     HIPCHK(hipGetLastError());
     return PIC_OK;
 }
+
+// ---------------------------------------------------------------- wall injection
+// InjectBeams_2D (IMPIC/InjectBeams_2D.f90:85-143). Four lanes per injected particle k, each
+// with its own Philox stream (slot 4k + c, step it, purpose PIC_RNG_INJECT | species):
+// c = 0 Bird's inward flux component (:91-106), c = 1 and 2 Loadv for Vy and Vz
+// (Loadv.f90:14-23; the ion drift added after, :121-126), c = 3 PosInit's X then Y
+// (ParticleOne.f90:82-89) and InjectPositioning (InjectPositioning.f90:13-22). Candidate i
+// of a rejection loop takes the two uniforms of Philox block i of its stream (a rejected
+// candidate's second uniform goes unused), so a lane evaluates four candidates at a time and
+// keeps the first accepted one: the same value as trying them in order. One launch serves
+// every registered source.
+struct InjectSource {
+    InjectHost p;
+    int electron;
+    int64_t base;                        // first free slot
+    int n;                               // particles to add
+    uint32_t tag;
+    void *x, *y, *vx, *vy, *vz;
+    int* cell;
+};
+struct InjectArgs {
+    InjectSource src[PIC_MAX_SPECIES];
+    int nsrc;
+    int64_t first[PIC_MAX_SPECIES + 1];  // thread offsets of the sources (4 threads per particle)
+    uint64_t seed; uint32_t step;
+    MeshParams m;
+    unsigned int* err;
+};
+
+#define PIC_SAMPLER_MAXIT 1024
+#define PIC_SAMPLER_ILP 4
+
+template <typename R>
+__global__ void k_inject(InjectArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.first[a.nsrc]) return;
+    int si = 0;
+    while (si + 1 < a.nsrc && t >= a.first[si + 1]) ++si;
+    const InjectSource& S = a.src[si];
+    const InjectHost& p = S.p;
+    const int64_t local = t - a.first[si];
+    const int64_t k = local >> 2;
+    const int c = (int)(local & 3);
+    const uint32_t slot = (uint32_t)(4 * k + c);
+    const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32);
+    const int64_t d = S.base + k;
+    if (c < 3) {
+        double val = 0.0;
+        bool found = false;
+        for (int i0 = 0; i0 < PIC_SAMPLER_MAXIT && !found; i0 += PIC_SAMPLER_ILP) {
+            double cv[PIC_SAMPLER_ILP];
+            bool ok[PIC_SAMPLER_ILP];
+#pragma unroll
+            for (int j = 0; j < PIC_SAMPLER_ILP; ++j) {
+                const philox4 o = pic_philox4x32_10(slot, a.step, S.tag, (uint32_t)(i0 + j), k0, k1);
+                const double Ra = pic_u01(o.v[0], o.v[1]), Rb = pic_u01(o.v[2], o.v[3]);
+                if (c == 0) {
+                    const double UU = -p.qa + 2.0 * p.qa * Ra;
+                    const double UN = UU + p.v_r;
+                    cv[j] = UN;
+                    ok[j] = !(UN < 0.0) && !((2.0 * UN / p.fs1) * exp(p.fs2 - UU * UU) < Rb);
+                } else {
+                    const double v = (6.0 * Ra - 3.0) * p.vmp;
+                    cv[j] = v;
+                    ok[j] = !(Rb > exp(-v * v / (p.vmp * p.vmp)));
+                }
+            }
+#pragma unroll
+            for (int j = PIC_SAMPLER_ILP - 1; j >= 0; --j)
+                if (ok[j]) { val = cv[j]; found = true; }
+        }
+        if (!found) atomicOr(a.err, PIC_EBIT_SAMPLER);
+        if (c == 0) {
+            ((R*)S.vx)[d] = (R)(val * p.sq2vt);
+        } else {
+            if (!S.electron) val = p.vd[c] + val;
+            if (c == 1) ((R*)S.vy)[d] = (R)val;
+            else ((R*)S.vz)[d] = (R)val;
+        }
+    } else {
+        const philox4 o = pic_philox4x32_10(slot, a.step, S.tag, 0u, k0, k1);
+        const double X = p.xl + (p.xu - p.xl) * pic_u01(o.v[0], o.v[1]);
+        const double Y = p.y_lo + (p.y_hi - p.y_lo) * pic_u01(o.v[2], o.v[3]);
+        const R Xs = (R)X, Ys = (R)Y;
+        // InjectPositioning on the stored position (no clamp)
+        const MeshParams& m = a.m;
+        const double rx = ((double)Xs - m.xmin) / m.mapx;
+        const int ii = (int)rx + 1;
+        const double ry = ((double)Ys - m.ymin) / m.mapy;
+        const int jj = (int)ry + 1;
+        int e = -1;
+        if (ii < 1 || ii > m.nex || jj < 1 || jj > m.ney) atomicOr(a.err, PIC_EBIT_POSITION);
+        else e = (jj - 1) + (ii - 1) * m.ney;
+        ((R*)S.x)[d] = Xs; ((R*)S.y)[d] = Ys;
+        S.cell[d] = e;
+    }
+}
+
+// add_N of every registered source (InjectBeams_2D.f90:40-64, Cartesian), then one launch;
+// the rounding draw is the first word of the species' stream at slot 0xFFFFFFFF.
+int run_injection(int it, int64_t* added) {
+    int64_t ns[PIC_MAX_SPECIES] = {0};
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
+        if (added) added[s] = 0;
+        const InjectHost& p = g.inj[s];
+        if (!p.on || !g.sp[s].defined) continue;
+        const SpeciesHost& S = g.sp[s];
+        pic_stream rs;
+        pic_stream_init(&rs, g.seed, 0xFFFFFFFFu, (uint32_t)it, pic_stream_tag(s, PIC_RNG_INJECT));
+        const double ranum = pic_stream_u01(&rs);
+        const double expect = p.dens * g.cfg.n_ref * std::fabs(p.b_amb) * g.v_ref * g.cfg.dt * g.time_ref *
+                              (p.y_hi - p.y_lo) * g.L_ref / S.p.weight;
+        const int64_t n = (int64_t)(expect + ranum);
+        if (n <= 0) continue;
+        if (S.nslots + n > S.capacity) return pic_fail(PIC_ERR_CAPACITY, "injection exceeds particle capacity");
+        if (n >= (int64_t)1 << 29) return pic_fail(PIC_ERR_CAPACITY, "injection count out of range");
+        ns[s] = n;
+    }
+    InjectArgs a;
+    a.nsrc = 0; a.first[0] = 0;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) {
+        if (ns[s] == 0) continue;
+        SpeciesHost& S = g.sp[s];
+        InjectSource& q = a.src[a.nsrc];
+        q.p = g.inj[s]; q.electron = (s == g.e_species) ? 1 : 0;
+        q.base = S.nslots; q.n = (int)ns[s]; q.tag = pic_stream_tag(s, PIC_RNG_INJECT);
+        q.x = S.x; q.y = S.y; q.vx = S.vx; q.vy = S.vy; q.vz = S.vz; q.cell = S.cell;
+        a.first[a.nsrc + 1] = a.first[a.nsrc] + 4 * ns[s];
+        a.nsrc++;
+        S.nslots += ns[s]; S.nlive += ns[s];
+        if (added) added[s] = ns[s];
+    }
+    if (a.nsrc == 0) return PIC_OK;
+    a.seed = g.seed; a.step = (uint32_t)it;
+    a.m = g.mesh; a.err = g.d_err;
+    const int blocks = (int)((a.first[a.nsrc] + 255) / 256);
+    if (g.particle_bits == 64) hipLaunchKernelGGL(k_inject<double>, dim3(blocks), dim3(256), 0, g.stream, a);
+    else hipLaunchKernelGGL(k_inject<float>, dim3(blocks), dim3(256), 0, g.stream, a);
+    HIPCHK(hipGetLastError());
+    return PIC_OK;
+}

@@ -93,7 +93,8 @@ enum {
     PIC_EBIT_EKRHO = 16u,
     PIC_EBIT_OVERFLOW = 32u,
     PIC_EBIT_CAPACITY = 64u,
-    PIC_EBIT_MCCLOOP = 128u
+    PIC_EBIT_MCCLOOP = 128u,
+    PIC_EBIT_SAMPLER = 256u
 };
 
 // First failing particle, recorded for the error message (first writer wins).

@@ -100,6 +100,18 @@ struct SpeciesHost {
     unsigned long long* mcc_events = nullptr; // device [32]
 };
 
+// Wall-injection source of one species (normalised as Input_2D.f90:489-494) with the
+// constants of InjectBeams_2D.f90 precomputed on the host.
+struct InjectHost {
+    bool on = false;
+    double dens = 0, tmpj = 0, vt = 0, vd[3] = {0, 0, 0};   // normalised
+    double r_start = 0, y_lo = 0, y_hi = 0;
+    double b_amb = 0;                   // mean inward velocity of the drifting flux (:42-50)
+    double v_r = 0, fs1 = 0, fs2 = 0, qa = 0, sq2vt = 0;     // Bird's sampler (:91-95, :106)
+    double vmp = 0;                     // Loadv's VMP_Sheath (Loadv.f90:14)
+    double xl = 0, xu = 0;              // slab (:73-76)
+};
+
 struct PprClass {
     int n;                        // patch size
     int di[PIC_PATCH_MAX], dj[PIC_PATCH_MAX];
@@ -214,6 +226,7 @@ struct PicState {
     int mcc_enabled = 1;
     int64_t step = 0;
     unsigned char* d_elem_clean = nullptr;   // MeshParams::elem_clean
+    InjectHost inj[PIC_MAX_SPECIES];
     unsigned int* d_err = nullptr;     // device error flags
     void* d_errinfo = nullptr;         // PicErrInfo of the first failing particle
     // scratch for selection / compaction
@@ -265,6 +278,7 @@ int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, cons
 #define FCG_REC_BASE 32
 #define FCG_RECW 5
 #define FCG_MAX_PASS 6
+int run_injection(int it, int64_t* added);
 int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed);
 int learn_set_harvest(int enable, int capacity, int every);
 bool learn_harvesting();

@@ -44,6 +44,13 @@ module pic_hip_mod
         real(c_double) :: mass_kg, temperature_K, density_m3
     end type pic_gas_params
 
+    ! pic_injection (pic.inp injection block, Input_2D.f90:333-344)
+    type, bind(c) :: pic_injection
+        real(c_double) :: density_m3, temp_eV
+        real(c_double) :: drift_ms(3)
+        real(c_double) :: r_start, y_lo, y_hi
+    end type pic_injection
+
     interface
         integer(c_int) function pic_init(device) bind(c, name="pic_init")
             import :: c_int
@@ -123,6 +130,17 @@ module pic_hip_mod
         end function
         integer(c_int) function pic_MCC() bind(c, name="pic_MCC")
             import :: c_int
+        end function
+        ! InjectBeams_2D(it, delta, PB), IMPIC/InjectBeams_2D.f90:1
+        integer(c_int) function pic_set_injection(isp, p) bind(c, name="pic_set_injection")
+            import :: pic_injection, c_int
+            integer(c_int), value :: isp
+            type(pic_injection), intent(in) :: p
+        end function
+        integer(c_int) function pic_InjectBeams_2D(it, delta, added) bind(c, name="pic_InjectBeams_2D")
+            import :: c_int, c_int64_t
+            integer(c_int), value :: it, delta
+            integer(c_int64_t), intent(out) :: added(4)
         end function
         integer(c_int) function pic_MCCBundleInit(gas, electron_species, ion_species) bind(c, name="pic_MCCBundleInit")
             import :: pic_gas_params, c_int

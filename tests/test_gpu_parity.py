@@ -322,3 +322,60 @@ def test_full_step_deterministic_and_conserving(gpu_lib, pkg):
         assert np.array_equal(a, b)
     c0 = pl.counts(0)
     assert c0["nlive"] == len(outs[1][2])
+
+
+# ------------------------------------------------------------------ wall injection
+@pytest.mark.parametrize("bits", [64, 32])
+def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
+    """InjectBeams_2D: the count, and every appended particle's position, velocity and element,
+    bit-exact against the oracle's restatement on the same streams (FP64 and FP32 particles)."""
+    pl = gpu_lib
+    say(f"case: InjectBeams_2D small32 bits={bits}")
+    w = pkg.workload("small32")
+    dom = w["domain"]
+    pl.setup(dom, bits=bits, solver=0, tol=1e-10)
+    nel = (dom.nnx - 1) * (dom.nny - 1)
+    weight = pkg.species_weight(dom, 8)
+    masses = {0: pkg.ELECTRON["mass_kg"], 1: pkg.ARGON_ION["mass_kg"]}
+    temps = {0: 2.0, 1: 0.026}
+    pl.set_species(0, pkg.ELECTRON["charge_C"], masses[0], weight, temps[0] * pkg.EVTOK, nel * 16)
+    pl.set_species(1, pkg.ARGON_ION["charge_C"], masses[1], weight, temps[1] * pkg.EVTOK, nel * 16)
+    pl.load_maxwellian(0, 8, temps[0], 5)
+    pl.load_maxwellian(1, 8, temps[1], 6)
+    pl.set_run_params(20, 77, 0)
+    norm = dom.normalisation()
+    ylen = float(dom.ymax) - 0.0
+    src = {}
+    for s in (0, 1):
+        xm = masses[s] / pkg.ELECTRON_MASS
+        tmpj = temps[s] * pkg.EVTOK / (dom.T_ref_eV * pkg.EVTOK)
+        vt = np.sqrt(tmpj / xm)
+        drift = (0.1 * vt * norm["v_ref"] * (1 if s == 0 else 20), 0.02 * norm["v_ref"], -0.01 * norm["v_ref"])
+        vd = [d / norm["v_ref"] for d in drift]
+        c = orc.inject_consts(vd[0], vt, tmpj, xm, 0.0, dom.dt)
+        per_dens = abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * ylen * norm["L_ref"] / weight
+        dens = (2000.0 if s == 0 else 400.0) / per_dens       # about 2000 electrons, 400 ions per call
+        pl.set_injection(s, dens, temps[s], drift, r_start=0.0, y_lo=0.0, y_hi=ylen)
+        expect = (dens / dom.n_ref) * dom.n_ref * abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * ylen * \
+            norm["L_ref"] / weight
+        src[s] = (c, vd, expect)
+    before = [pl.counts(s)["nslots"] for s in (0, 1)]
+    added = pl.InjectBeams_2D(7)
+    say(f"  added {added[:2]}")
+    assert added[0] > 1000 and added[1] > 100
+    for s in (0, 1):
+        c, vd, expect = src[s]
+        tg = orc.tag(s, orc.RNG_INJECT)
+        assert added[s] == orc.inject_count(expect, 77, 7, tg)
+        x, y, vx, vy, vz, cell = orc.inject(bits, added[s], 1 if s == 0 else 0, c, vd, 0.0, ylen, 77, 7, tg,
+                                            dom.xmin, dom.ymin, 1.0, 1.0, dom.nnx - 1, dom.nny - 1)
+        p = pl.particles(s)
+        assert pl.counts(s)["nslots"] == before[s] + added[s]
+        n = added[s]
+        for k, ref in (("x", x), ("y", y), ("vx", vx), ("vy", vy), ("vz", vz)):
+            assert np.array_equal(p[k][-n:], ref), (s, k)
+        assert np.array_equal(p["loc"][-n:] - 1, cell), s
+    # the injected particles move and deposit like any other in the next step
+    pl.initial_field()
+    pl.step(1)
+    assert pl.counts(0)["nlive"] > 0

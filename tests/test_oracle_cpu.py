@@ -462,3 +462,56 @@ def test_push_wall_fuzz_always_settles(orc, bits):
         state = [a[live].astype(np.float64) for a in (X, Y, VX, VY, VZ)]
         cell, bad = orc.locate(xs, ys, state[0], state[1])
         assert bad == 0
+
+
+# ------------------------------------------------------------------ wall injection
+def test_injection_flux_constants_closed_form(orc):
+    """b_amb of InjectBeams_2D.f90:42-50 is the mean inward flux velocity of a drifting
+    Maxwellian, int_0^inf v N(v; vd, vt) dv (within the single-precision SQRT(2.) the
+    reference uses); Bird's FS1/FS2 and Loadv's VMP follow their definitions."""
+    from scipy import integrate
+    for vd, vt in ((0.1, 1.0), (0.3, 2.0), (1.5, 0.7)):
+        c = orc.inject_consts(vd, vt, vt * vt, 1.0, 3.0, 0.05)
+        ref, _ = integrate.quad(lambda v: v * np.exp(-(v - vd) ** 2 / (2 * vt * vt)) / np.sqrt(2 * np.pi) / vt,
+                                0, np.inf)
+        assert c[0] == pytest.approx(ref, rel=1e-6)
+        vr = vd / (np.sqrt(2) * vt)
+        assert c[1] == pytest.approx(vr, rel=1e-7) and c[2] == pytest.approx(vr + np.sqrt(vr * vr + 2), rel=1e-7)
+        assert c[6] == pytest.approx(np.sqrt(2) * vt, rel=1e-12)            # VMP with tmpj = vt^2, xm = 1
+        assert c[7] == 3.0 and c[8] == pytest.approx(3.0 + 0.5 * ref * 0.05, rel=1e-6)
+    # no drift: the reference takes the negative branch (slab behind R_start, |b| in the count)
+    c = orc.inject_consts(0.0, 1.0, 1.0, 1.0, 3.0, 0.05)
+    assert c[0] == pytest.approx(-1.0 / np.sqrt(2 * np.pi), rel=1e-12) and c[8] == 3.0
+
+
+def test_injection_sampler_statistics(orc):
+    """Bird's sampler draws the flux-weighted inward component v exp(-(v-vd)^2/2vt^2), v > 0;
+    Loadv draws N(0, vt) transverse components (plus the ion drift); positions fill the slab
+    and InjectPositioning matches a brute-force element search."""
+    from scipy import integrate
+    n, vd, vt = 60000, 0.2, 1.0
+    c = orc.inject_consts(vd, vt, vt * vt, 1.0, 0.0, 0.05)
+    x, y, vx, vy, vz, cell = orc.inject(64, n, 0, c, [vd, 0.25, -0.5], 0.0, 16.0, 9, 4, orc.tag(1, orc.RNG_INJECT),
+                                        0.0, 0.0, 1.0, 1.0, 16, 16)
+    w = lambda v: v * np.exp(-(v - vd) ** 2 / (2 * vt * vt))
+    m1 = integrate.quad(lambda v: v * w(v), 0, np.inf)[0] / integrate.quad(w, 0, np.inf)[0]
+    m2 = integrate.quad(lambda v: v * v * w(v), 0, np.inf)[0] / integrate.quad(w, 0, np.inf)[0]
+    assert np.all(vx > 0)
+    assert abs(vx.mean() - m1) < 5 * np.sqrt((m2 - m1 * m1) / n)
+    for v, drift in ((vy, 0.25), (vz, -0.5)):
+        assert abs(v.mean() - drift) < 5 * vt / np.sqrt(n)
+        assert abs(v.std() - vt) < 0.02
+    assert np.all((x >= c[7]) & (x <= c[8]) & (y >= 0) & (y <= 16))
+    brute = np.floor(y).astype(int) + np.floor(x).astype(int) * 16
+    assert np.array_equal(cell, brute)
+    # electrons get no transverse drift
+    _, _, _, vy_e, _, _ = orc.inject(64, 4000, 1, c, [vd, 0.25, -0.5], 0.0, 16.0, 9, 4, orc.tag(0, orc.RNG_INJECT),
+                                     0.0, 0.0, 1.0, 1.0, 16, 16)
+    assert abs(vy_e.mean()) < 5 * vt / np.sqrt(4000)
+
+
+def test_injection_count_rounding(orc):
+    """add_N = INT(expect + R): floor or floor + 1, with mean expect over many steps."""
+    counts = np.array([orc.inject_count(12.37, 3, s, orc.tag(0, orc.RNG_INJECT)) for s in range(4000)])
+    assert set(np.unique(counts)) <= {12, 13}
+    assert abs(counts.mean() - 12.37) < 5 * 0.5 / np.sqrt(4000)
