@@ -367,12 +367,16 @@ __global__ void k_add_counter(unsigned long long* c, unsigned long long v) {
     if (threadIdx.x == 0) *c += v;
 }
 
+// Scratch that grows with the particle count: reallocated (which synchronises the device)
+// only when outgrown, then with 25% and 1 MiB of headroom so a slowly growing population
+// does not reallocate every step.
 static int ensure_buf(void** p, int64_t* cap, int64_t need_bytes) {
     if (*cap >= need_bytes) return PIC_OK;
-    if (*p) hipFree(*p);
+    if (*p) (void)hipFree(*p);
     *p = nullptr;
-    HIPCHK(hipMalloc(p, (size_t)need_bytes));
-    *cap = need_bytes;
+    int64_t bytes = need_bytes + need_bytes / 4 + ((int64_t)1 << 20);
+    HIPCHK(hipMalloc(p, (size_t)bytes));
+    *cap = bytes;
     return PIC_OK;
 }
 
@@ -525,9 +529,11 @@ static int sort_species(int isp) {
     int end_bit = 1;
     while ((1u << end_bit) <= dead_key) end_bit++;
     size_t esz = (g.particle_bits == 64) ? 8 : 4;
-    size_t need = (size_t)n * 16 + 256;
+    // keys and values sized for the species' capacity, so a growing population does not
+    // reallocate between sorts
+    size_t need = (size_t)std::max<int64_t>(n, S.capacity) * 16 + 256;
     if (g.sort_buf_bytes < need) {
-        if (g.d_sort_buf) hipFree(g.d_sort_buf);
+        if (g.d_sort_buf) (void)hipFree(g.d_sort_buf);
         g.d_sort_buf = nullptr;
         HIPCHK(hipMalloc(&g.d_sort_buf, need));
         g.sort_buf_bytes = need;
@@ -541,10 +547,11 @@ static int sort_species(int isp) {
     size_t tb = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
     if (g.sort_tmp_bytes < tb) {
-        if (g.d_sort_tmp) hipFree(g.d_sort_tmp);
+        if (g.d_sort_tmp) (void)hipFree(g.d_sort_tmp);
         g.d_sort_tmp = nullptr;
-        HIPCHK(hipMalloc(&g.d_sort_tmp, tb));
-        g.sort_tmp_bytes = tb;
+        const size_t bytes = tb + tb / 4 + ((size_t)1 << 20);
+        HIPCHK(hipMalloc(&g.d_sort_tmp, bytes));
+        g.sort_tmp_bytes = bytes;
     }
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(g.d_sort_tmp, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
     // live count; the live particles occupy the first `live` sorted positions
