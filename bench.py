@@ -50,13 +50,21 @@ def traffic_from_profiles(name: str) -> dict:
             re.finditer(r"traffic_bytes_per_launch\s+(\S+):\s*([0-9.eE+]+)", p.read_text())}
 
 
-def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
-    """One full explicit step of the CPU oracle (scalar C, 1 core) on a bounded sample of the
-    same workload state: MoveAll + lookup + deposition for `sample_particles` particles per
-    species, and the field half of the step (RHS, MY_PCG to 1e-8 in FP64, PPR) on the full
-    512^2 mesh. Scaled to particle-pushes/s of the whole workload."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle as orc
+def cpu_threads() -> int:
+    """Host threads for the multi-core CPU timing: the CPUs this process may run on, capped
+    at 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _oracle_step(pkg, pl, w, orc, sample_particles: int, phi_prev) -> tuple:
+    """One explicit step of the CPU oracle on the workload state: MoveAll + lookup +
+    deposition for `sample_particles` particles per species (time scaled to the whole
+    population), then RHS, MY_PCG to 1e-8 in FP64 and PPR on the full mesh. Runs with the
+    oracle's current thread count. Returns (seconds per step, PCG iterations, particles)."""
     dom = w["domain"]
     hp = pl.setup_array("HP")
     xs = hp[0::2][:: dom.nny].copy()
@@ -94,12 +102,32 @@ def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
     phi2[un] = xsol
     orc.ppr(xs, ys, pl.setup_array("POINT_PATCH"), phi2)
     t_field = time.perf_counter() - t0
-    t_step = t_part + t_field
-    return {"value": n_total / t_step, "unit": "particle-pushes/s", "cores": 1, "kind": "port",
-            "sample": f"oracle.c scalar FP64 step: mover+lookup+deposit timed on {sample_particles} particles per "
-                      f"species and scaled to {n_total}; RHS + MY_PCG (1e-8, {it} it) + PPR on the full mesh; "
-                      f"MCC and the wall injection (~1e3 new particles per step) not included; "
-                      f"{t_step:.2f} s per step"}
+    return t_part + t_field, it, n_total
+
+
+def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
+    """The CPU oracle's full explicit step (C, FP64) on a bounded sample of the same
+    workload state, timed twice: on one core in the reference's serial order, and on every
+    host core of this process (OpenMP over particles, nodes and rows, at most 16 threads).
+    The all-core figure is the reported baseline; the one-core figure rides along. Both are
+    scaled to particle-pushes/s of the whole workload."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+    nthr = cpu_threads()
+    orc.set_threads(1)
+    t1, it1, n_total = _oracle_step(pkg, pl, w, orc, sample_particles, phi_prev)
+    try:
+        orc.set_threads(nthr)
+        tn, itn, _ = _oracle_step(pkg, pl, w, orc, sample_particles, phi_prev)
+    finally:
+        orc.set_threads(1)
+    return {"value": n_total / tn, "unit": "particle-pushes/s", "cores": nthr, "kind": "port",
+            "nproc": os.cpu_count(),
+            "single_core": {"value": n_total / t1, "cores": 1, "s_per_step": t1, "pcg_iterations": it1},
+            "sample": f"oracle.c FP64 step on {nthr} OpenMP threads ({tn:.2f} s per step; 1 core: {t1:.2f} s): "
+                      f"mover+lookup+deposit timed on {sample_particles} particles per species and scaled to "
+                      f"{n_total}; RHS + MY_PCG (1e-8, {itn} it) + PPR on the full mesh; MCC and the wall "
+                      f"injection (~1e3 new particles per step) not included"}
 
 
 def solve_compare(pl, reps: int) -> dict:

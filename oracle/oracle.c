@@ -110,6 +110,16 @@ static void mk_mesh(or_mesh* m, int nnx, int nny, const double* xs, const double
     }
 }
 
+/* ------------------------------------------------------------------ threads */
+/* OpenMP thread count of the particle, lookup, deposition, RHS, PCG and PPR loops. 1 (the
+ * default, used by every parity test) runs each loop serially in the reference's order; > 1
+ * exists for the multi-core CPU timing. Integer results are identical for every count;
+ * floating-point sums that are split over threads (PCG dot products, RHS scatter) may
+ * differ in the last bits. */
+static int or_threads = 1;
+void or_set_threads(int n) { or_threads = n < 1 ? 1 : n; }
+int or_get_threads(void) { return or_threads; }
+
 /* ------------------------------------------------------------------ MoveAll */
 #define REAL double
 #define FMOD fmod
@@ -184,6 +194,8 @@ int or_locate(int nnx, int nny, const double* xs, const double* ys, int64_t n, c
     or_mesh m;
     mk_mesh(&m, nnx, nny, xs, ys, NULL, NULL, 0, NULL);
     int bad = 0;
+    const int nt = or_threads;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(| : bad)
     for (int64_t p = 0; p < n; ++p) out[p] = locate1(&m, x[p], y[p], &bad);
     return bad;
 }
@@ -205,6 +217,10 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
     double* ek = (double*)calloc((size_t)4 * nelem, sizeof(double));
     long long* qs = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
     long long* qe = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
+    /* threads only for the fixed-point sums, which are order independent; the FP64 sums
+     * stay sequential in particle order */
+    const int nt = fix ? or_threads : 1;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
     for (int64_t p = 0; p < n; ++p) {
         int e = cell[p];
         if (e < 0) continue;
@@ -232,7 +248,13 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
             W[2] = dx * (R * R - R1 * R1) / den; W[3] = xm * (R * R - R1 * R1) / den;
         }
         for (int k = 0; k < 4; ++k) {
-            if (fix) {
+            if (fix && nt > 1) {
+                long long a = llrint(W[k] * 1099511627776.0), b = llrint(W[k] * Energy * 1099511627776.0);
+#pragma omp atomic
+                qs[4 * (size_t)e + k] += a;
+#pragma omp atomic
+                qe[4 * (size_t)e + k] += b;
+            } else if (fix) {
                 qs[4 * (size_t)e + k] += llrint(W[k] * 1099511627776.0);
                 qe[4 * (size_t)e + k] += llrint(W[k] * Energy * 1099511627776.0);
             } else {
@@ -244,10 +266,26 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
     if (fix) {
         long long* nw = (long long*)calloc(nnodes, sizeof(long long));
         long long* ne = (long long*)calloc(nnodes, sizeof(long long));
-        for (int e = 0; e < nelem; ++e) {
-            int ci = e / ney, cj = e % ney;
-            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
-            for (int k = 0; k < 4; ++k) { nw[ht[k]] += qs[4 * (size_t)e + k]; ne[ht[k]] += qe[4 * (size_t)e + k]; }
+        if (nt > 1) {
+            /* node-side gather of the (up to) four corner sums touching each node */
+#pragma omp parallel for num_threads(nt) schedule(static)
+            for (int i = 0; i < nnodes; ++i) {
+                int ni = i / nny, nj = i % nny;
+                long long sw = 0, se = 0;
+                for (int k = 0; k < 4; ++k) {
+                    int ci = ni - (k == 1 || k == 2), cj = nj - (k >= 2);   /* corner k of element (ci,cj) is node i */
+                    if (ci < 0 || ci >= nex || cj < 0 || cj >= ney) continue;
+                    size_t e = (size_t)ci * ney + cj;
+                    sw += qs[4 * e + k]; se += qe[4 * e + k];
+                }
+                nw[i] = sw; ne[i] = se;
+            }
+        } else {
+            for (int e = 0; e < nelem; ++e) {
+                int ci = e / ney, cj = e % ney;
+                int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+                for (int k = 0; k < 4; ++k) { nw[ht[k]] += qs[4 * (size_t)e + k]; ne[ht[k]] += qe[4 * (size_t)e + k]; }
+            }
         }
         for (int i = 0; i < nnodes; ++i) {
             w_node[i] = (double)nw[i] * (1.0 / 1099511627776.0);
@@ -321,8 +359,10 @@ int or_brinv(double* a, int n) {
 int or_ppr(int nnx, int nny, const double* xs, const double* ys, const int* point_patch, const double* phi,
            double* coef, double* efx, double* efy) {
     int nnodes = nnx * nny, nsing = 0;
-    double pnts[2][64], AT[6][64], A_MUL[36], ATn[6][64];
+    const int nt = or_threads;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+ : nsing)
     for (int i = 0; i < nnodes; ++i) {
+        double pnts[2][64], AT[6][64], A_MUL[36], ATn[6][64];
         int cnt = point_patch[i];
         for (int r = 0; r < 6; ++r) coef[6 * (size_t)i + r] = 0.0;
         efx[i] = 0.0; efy[i] = 0.0;
@@ -404,29 +444,52 @@ int or_pcg(int n, const int* rowptr, const int* col, const double* val, const do
     double *P = malloc(sizeof(double) * n), *Q = malloc(sizeof(double) * n);
     double alpha, beta, rho = 0, rho0 = 0, normR, normRHS = 0, RELTOL2 = reltol * reltol;
     int i, j, ierr = 0;
+    /* with or_threads > 1 every vector loop is split over threads (dot products then sum in
+     * per-thread partials); with 1 thread each loop runs in row order */
+    const int nt = or_threads;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+ : normRHS)
     for (j = 0; j < n; ++j) normRHS = normRHS + b[j] * b[j];
-    for (j = 0; j < n; ++j) X0[j] = x[j];
-    for (j = 0; j < n; ++j) { double s = 0; for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * X0[col[k]]; R0[j] = s; }
-    for (j = 0; j < n; ++j) R0[j] = b[j] - R0[j];
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+    for (j = 0; j < n; ++j) {
+        X0[j] = x[j];
+    }
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+    for (j = 0; j < n; ++j) {
+        double s = 0;
+        for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * X0[col[k]];
+        R0[j] = b[j] - s;
+    }
     for (i = 1; i <= maxiter; ++i) {
-        for (j = 0; j < n; ++j) W[j] = R0[j] / diag[j];
         rho = 0;
-        for (j = 0; j < n; ++j) rho = rho + R0[j] * W[j];
-        if (i == 1) { for (j = 0; j < n; ++j) P[j] = W[j]; }
-        else {
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+ : rho)
+        for (j = 0; j < n; ++j) { W[j] = R0[j] / diag[j]; rho = rho + R0[j] * W[j]; }
+        if (i == 1) {
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+            for (j = 0; j < n; ++j) P[j] = W[j];
+        } else {
             if (rho0 == 0) { ierr = 2; break; }
             beta = rho / rho0;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
             for (j = 0; j < n; ++j) P[j] = W[j] + beta * P[j];
         }
-        for (j = 0; j < n; ++j) { double s = 0; for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * P[col[k]]; Q[j] = s; }
         alpha = 0;
-        for (j = 0; j < n; ++j) alpha = alpha + P[j] * Q[j];
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+ : alpha)
+        for (j = 0; j < n; ++j) {
+            double s = 0;
+            for (int k = rowptr[j]; k < rowptr[j + 1]; ++k) s = s + val[k] * P[col[k]];
+            Q[j] = s;
+            alpha = alpha + P[j] * s;
+        }
         if (alpha == 0) { ierr = 3; break; }
         alpha = rho / alpha;
-        for (j = 0; j < n; ++j) { X0[j] = X0[j] + alpha * P[j]; R0[j] = R0[j] - alpha * Q[j]; }
-        rho0 = rho;
         normR = 0;
-        for (j = 0; j < n; ++j) normR = normR + R0[j] * R0[j];
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static) reduction(+ : normR)
+        for (j = 0; j < n; ++j) {
+            X0[j] = X0[j] + alpha * P[j];
+            R0[j] = R0[j] - alpha * Q[j];
+            normR = normR + R0[j] * R0[j];
+        }
+        rho0 = rho;
         if (normR / normRHS <= RELTOL2) break;
     }
     if (i >= maxiter) ierr = 1;      /* MY_PCG.f90:129-133: ITER left unset */
@@ -471,22 +534,28 @@ void or_element_matrices(int delta, double x0, double y0, double hx, double hy, 
 void or_rhs(int nnx, int nny, const double* xs, const double* ys, int delta, const int* node_type2,
             const double* dphi, const double* rho, int nunk, double* rhs) {
     int nex = nnx - 1, ney = nny - 1;
-    double K[16], M[16];
     for (int u = 0; u < nunk; ++u) rhs[u] = 0.0;
     double* ebc = calloc(nunk, sizeof(double));
-    for (int ci = 0; ci < nex; ++ci)
-        for (int cj = 0; cj < ney; ++cj) {
-            int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
-            or_element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj], K, M);
-            for (int a = 0; a < 4; ++a) {
-                int u = node_type2[ht[a]];
-                if (u <= 0) continue;
-                double s = 0.0;
-                for (int b = 0; b < 4; ++b) s += M[a * 4 + b] * rho[ht[b]];
-                rhs[u - 1] += s;
-                for (int b = 0; b < 4; ++b) if (node_type2[ht[b]] <= 0) ebc[u - 1] += K[a * 4 + b] * dphi[ht[b]];
+    const int nt = or_threads;
+    /* with threads, element columns of one parity share no node: even columns, then odd */
+    for (int par = 0; par < (nt > 1 ? 2 : 1); ++par) {
+        const int c0 = nt > 1 ? par : 0, cs = nt > 1 ? 2 : 1;
+#pragma omp parallel for num_threads(nt) if (nt > 1) schedule(static)
+        for (int ci = c0; ci < nex; ci += cs)
+            for (int cj = 0; cj < ney; ++cj) {
+                double K[16], M[16];
+                int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
+                or_element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj], K, M);
+                for (int a = 0; a < 4; ++a) {
+                    int u = node_type2[ht[a]];
+                    if (u <= 0) continue;
+                    double s = 0.0;
+                    for (int b = 0; b < 4; ++b) s += M[a * 4 + b] * rho[ht[b]];
+                    rhs[u - 1] += s;
+                    for (int b = 0; b < 4; ++b) if (node_type2[ht[b]] <= 0) ebc[u - 1] += K[a * 4 + b] * dphi[ht[b]];
+                }
             }
-        }
+    }
     for (int u = 0; u < nunk; ++u) rhs[u] = rhs[u] - ebc[u];
     free(ebc);
 }

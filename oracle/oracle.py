@@ -29,6 +29,8 @@ def lib():
         if not LIB.exists():
             build()
         _lib = C.CDLL(str(LIB))
+        _lib.or_set_threads.argtypes = [C.c_int]
+        _lib.or_get_threads.restype = C.c_int
         _lib.or_philox_block.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib.or_move_all.restype = C.c_int
         _lib.or_move_all.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -79,6 +81,16 @@ def lib():
         _lib.or_mcc_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_void_p, C.c_void_p]
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the particle, lookup, deposition, RHS, PCG and PPR loops (1 = the
+    serial reference order, the default)."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().or_get_threads())
 
 
 def _p(a):

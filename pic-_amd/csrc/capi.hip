@@ -79,6 +79,7 @@ static MeshParams dev_mesh() {
     MeshParams m = g.mesh;
     m.xs = g.d_xs; m.ys = g.d_ys; m.xsf = g.d_xsf; m.ysf = g.d_ysf;
     m.elem_clean = g.d_elem_clean;
+    m.elem_nbr = g.d_elem_nbr;
     m.element_index = g.d_element_index; m.node_index = g.d_node_index;
     return m;
 }
@@ -263,6 +264,19 @@ int pic_setup_domain(const pic_domain_config* cfg) {
             }
         }
         TRY(dupload(&g.d_elem_clean, clean));
+        std::vector<unsigned short> nbr(m.nelem, 0);
+        for (int e = 0; e < m.nelem; ++e) {
+            const int ci = e / m.ney, cj = e - ci * m.ney;
+            unsigned short b = 0;
+            for (int di = -1; di <= 1; ++di)
+                for (int dj = -1; dj <= 1; ++dj) {
+                    const int i = ci + di, j = cj + dj;
+                    if (i < 0 || i >= m.nex || j < 0 || j >= m.ney) continue;
+                    if (clean[(size_t)i * m.ney + j]) b |= (unsigned short)(1u << (3 * (di + 1) + (dj + 1)));
+                }
+            nbr[e] = b;
+        }
+        TRY(dupload(&g.d_elem_nbr, nbr));
     }
     TRY(dupload(&g.d_node_type2, g.h_node_type2));
     std::vector<int> un0(g.h_unknown_node.size());

@@ -17,7 +17,17 @@
 #define PUSH_BLOCK 256
 #define PUSH_PPT (PIC_SORT_GROUP / 64)
 #define PUSH_SLOTS (PUSH_BLOCK * PUSH_PPT)
-#define WIN_NODES (3 * SORT_TILE + 1) * (3 * SORT_TILE + 1)
+// LDS deposition window of a push block: element columns ci - M .. ci + 1 + M around the
+// element of the block's middle particle, and the rows of its sort tile +- M, with M =
+// PUSH_WIN_MARGIN cells of drift since the last sort. A block's PUSH_SLOTS sorted particles
+// cover one or two element columns of one tile; deposits outside the window go to global
+// atomics.
+#ifndef PUSH_WIN_MARGIN
+#define PUSH_WIN_MARGIN 16
+#endif
+#define WIN_NX (2 * PUSH_WIN_MARGIN + 3)
+#define WIN_NY (SORT_TILE + 2 * PUSH_WIN_MARGIN + 1)
+#define WIN_NODES (WIN_NX * WIN_NY)
 
 template <typename R> struct PushArgs {
     R *x, *y, *vx, *vy, *vz;
@@ -70,7 +80,8 @@ struct DepWindow { int x0, y0, nx, ny; };
 // or to the global node accumulators.
 __device__ inline void add_element(int key, const long long v[8], const MeshParams& m, long long* acc_w,
                                    long long* acc_e, const DepWindow& w, long long* winW, long long* winE) {
-    int ci = key / m.ney, cj = key - ci * m.ney;
+    int ci, cj;
+    pic_elem_ij(key, m, &ci, &cj);
     int li = ci - w.x0, lj = cj - w.y0;
     if (li >= 0 && lj >= 0 && li + 1 < w.nx && lj + 1 < w.ny) {
         int l1 = li * w.ny + lj, l2 = l1 + w.ny, l3 = l2 + 1, l4 = l1 + 1;
@@ -144,9 +155,8 @@ __device__ inline void fold_y(R& Y, R& VY, R ymin, R ymax) {
 
 // Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64. With a
 // power-of-two spacing the division is the (bit-identical) multiplication by 1/h.
-__device__ inline void deposit_weights(double X, double Y, int e, const MeshParams& m, double W[4]) {
-    int ci = e / m.ney, cj = e - ci * m.ney;
-    double x1 = m.xs[ci], x2 = m.xs[ci + 1], y1 = m.ys[cj], y2 = m.ys[cj + 1];
+__device__ inline void deposit_weights_ij(double X, double Y, int ci, int cj, const MeshParams& m, double W[4]) {
+    double x1 = pic_ld(m.xs, ci), x2 = pic_ld(m.xs, ci + 1), y1 = pic_ld(m.ys, cj), y2 = pic_ld(m.ys, cj + 1);
     double hxp = x2 - x1, hyp = y2 - y1;
     double dx = (m.pow2_map && hxp == m.hx) ? (X - x1) * m.hxi : (X - x1) / hxp;
     double dy = (m.pow2_map && hyp == m.hy) ? (Y - y1) * m.hyi : (Y - y1) / hyp;
@@ -161,6 +171,12 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
         W[2] = dx * (Rr * Rr - R1 * R1) / den;
         W[3] = xm * (Rr * Rr - R1 * R1) / den;
     }
+}
+
+__device__ inline void deposit_weights(double X, double Y, int e, const MeshParams& m, double W[4]) {
+    int ci, cj;
+    pic_elem_ij(e, m, &ci, &cj);
+    deposit_weights_ij(X, Y, ci, cj, m, W);
 }
 
 struct PushEvents { int emit, right, object, reflect, deleted; };
@@ -305,28 +321,39 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
 // whole loop from the original state with the same arithmetic. Otherwise the reference's
 // loop ends after this pass with no boundary or object event, and *e_new is the element.
 template <typename R>
-__device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, int* e_new) {
+__device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, int* e_new, int* ci_new,
+                                int* cj_new) {
     const MeshParams& m = a.m;
-    if (!m.elem_clean[e_old]) return true;
     const R T = a.dt;
-    int ci = e_old / m.ney, cj = e_old - ci * m.ney;
-    R x1 = a.xsR[ci], x2 = a.xsR[ci + 1], y1 = a.ysR[cj], y2 = a.ysR[cj + 1];
+    int ci, cj;
+    pic_elem_ij(e_old, m, &ci, &cj);
+    // every load of the pass is issued before the first test (all addresses are valid for
+    // any element), so one wait covers them
+    const unsigned int nbr = pic_ld(m.elem_nbr, e_old);     // flags of the 3x3 neighbourhood
+    R x1 = pic_ld(a.xsR, ci), x2 = pic_ld(a.xsR, ci + 1), y1 = pic_ld(a.ysR, cj), y2 = pic_ld(a.ysR, cj + 1);
+    const int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+    const R e1x = pic_ld(a.efx, n1), e2x = pic_ld(a.efx, n2), e3x = pic_ld(a.efx, n3), e4x = pic_ld(a.efx, n4);
+    const R e1y = pic_ld(a.efy, n1), e2y = pic_ld(a.efy, n2), e3y = pic_ld(a.efy, n3), e4y = pic_ld(a.efy, n4);
     R hxp = x2 - x1, hyp = y2 - y1;
     R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
     R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
     R xm = (R)1 - dx, ym = (R)1 - dy;
     R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
-    int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-    R Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
-    R Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
+    R Ex = W1 * e1x + W2 * e2x + W3 * e3x + W4 * e4x;
+    R Ey = W1 * e1y + W2 * e2y + W3 * e3y + W4 * e4y;
     R VX1 = VX + a.qm * Ex * T;
     R VY1 = VY + a.qm * Ey * T;
     R X1 = X + T * VX1;
     R Y1 = Y + T * VY1;
-    if (X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
+    // the pass is computed for every particle and the element flag tested with the walls,
+    // so no branch separates the flag load from the field loads
+    if (!(nbr & 16u) || X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
     int bad = 0;
-    int en = pic_locate_elem((double)X1, (double)Y1, m, &bad);
-    if (bad || !m.elem_clean[en]) return true;
+    int en = pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
+    // the new element's flag from the neighbourhood mask (no second load); a particle that
+    // moved further than one element is handed over like a flagged one
+    const int di = *ci_new - ci + 1, dj = *cj_new - cj + 1;
+    if (bad || (unsigned int)di > 2u || (unsigned int)dj > 2u || !((nbr >> (3 * di + dj)) & 1u)) return true;
     X = X1; Y = Y1; VX = VX1; VY = VY1;
     *e_new = en;
     return false;
@@ -388,14 +415,24 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 }
 
 // Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
+// at 8 slots per thread the register budget of 4 waves per SIMD is asked for explicitly
+// (the unconstrained allocation lands a few registers above it, at 3 waves)
+#if !defined(PUSH_WAVES_PER_EU) && PIC_SORT_GROUP == 512
+#define PUSH_WAVES_PER_EU 4
+#endif
+#ifdef PUSH_WAVES_PER_EU
+#define PUSH_ATTR __attribute__((amdgpu_waves_per_eu(PUSH_WAVES_PER_EU, 8)))
+#else
+#define PUSH_ATTR
+#endif
 template <typename R, int DO_PUSH>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
+__global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<R> a) {
     __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
     __shared__ int s_center;
     const MeshParams& m = a.m;
     const int64_t base = (int64_t)blockIdx.x * PUSH_SLOTS;
-    // window: the 3x3 sort tiles around the tile of the first live particle at or after
-    // the block's middle slot
+    // window around the element of the first live particle at or after the block's middle
+    // slot (WIN_NX x WIN_NY nodes)
     if (threadIdx.x < 64) {
         int64_t mid = base + PUSH_SLOTS / 2;
         if (mid >= a.n) mid = base;
@@ -411,9 +448,9 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     w.nx = 0; w.ny = 0; w.x0 = 0; w.y0 = 0;
     if (s_center >= 0) {
         int ci = s_center / m.ney, cj = s_center - ci * m.ney;
-        int ti = ci / SORT_TILE, tj = cj / SORT_TILE;
-        int x0 = (ti - 1) * SORT_TILE, y0 = (tj - 1) * SORT_TILE;
-        int x1 = (ti + 2) * SORT_TILE, y1 = (tj + 2) * SORT_TILE;
+        int tj = cj / SORT_TILE;
+        int x0 = ci - PUSH_WIN_MARGIN, y0 = tj * SORT_TILE - PUSH_WIN_MARGIN;
+        int x1 = x0 + WIN_NX - 1, y1 = y0 + WIN_NY - 1;
         if (x0 < 0) x0 = 0;
         if (y0 < 0) y0 = 0;
         if (x1 > m.nnx - 1) x1 = m.nnx - 1;
@@ -424,12 +461,19 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) { winW[k] = 0; winE[k] = 0; }
 
     // ---------------- move, locate, deposit (run accumulation in registers)
-    // Wave w of the block owns the 512-slot group at wbase; iteration k handles slot
-    // wbase + 64 k + lane, so every load and store is lane-contiguous. The cell sort lays a
-    // full group out transposed (sorted position 8 lane + k at slot 64 k + lane), so a
-    // thread's iterations walk consecutive sorted particles, which mostly share an element.
+    // Wave w of the block owns the PIC_SORT_GROUP-slot group at wbase; iteration k handles
+    // slot wbase + 64 k + lane, so every load and store is lane-contiguous. The cell sort lays
+    // a full group out transposed (sorted position PUSH_PPT lane + k at slot 64 k + lane), so
+    // a thread's iterations walk consecutive sorted particles, which mostly share an element.
+    // Slot arrays are addressed from the block's first slot with 32-bit offsets.
     const int lane = threadIdx.x & 63;
-    const int64_t wbase = base + (int64_t)(threadIdx.x >> 6) * PIC_SORT_GROUP;
+    const unsigned int o0 = (threadIdx.x >> 6) * PIC_SORT_GROUP + lane;
+    const int64_t nrem = a.n - base;
+    const unsigned int nb = nrem < (int64_t)PUSH_SLOTS ? (unsigned int)nrem : (unsigned int)PUSH_SLOTS;
+    const int* cellb = a.cell + base;
+    int* nextb = a.cell_next + base;
+    R* xb = a.x + base; R* yb = a.y + base; R* vxb = a.vx + base; R* vyb = a.vy + base;
+    const R* vzb = a.vz + base;
     int rkey = -1;
     long long acc[8];
 #pragma unroll
@@ -440,50 +484,52 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
     R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT];
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
-        const int64_t p = wbase + 64 * k + lane;
+        const unsigned int o = o0 + 64 * k;
         cel[k] = -1; xs_[k] = (R)0; ys_[k] = (R)0; vxs_[k] = (R)0; vys_[k] = (R)0; vzs_[k] = (R)0;
-        if (p < a.n) { cel[k] = a.cell[p]; xs_[k] = a.x[p]; ys_[k] = a.y[p]; vxs_[k] = a.vx[p]; vys_[k] = a.vy[p]; vzs_[k] = a.vz[p]; }
+        if (o < nb) {
+            cel[k] = pic_ld(cellb, o); xs_[k] = pic_ld((const R*)xb, o); ys_[k] = pic_ld((const R*)yb, o);
+            vxs_[k] = pic_ld((const R*)vxb, o); vys_[k] = pic_ld((const R*)vyb, o); vzs_[k] = pic_ld(vzb, o);
+        }
     }
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
-        const int64_t p = wbase + 64 * k + lane;
-        const bool in = p < a.n;
+        const unsigned int o = o0 + 64 * k;
+        const bool in = o < nb;
         const int e_old = cel[k];
         R X = xs_[k], Y = ys_[k], VX = vxs_[k], VY = vys_[k];
         const R VZ = vzs_[k];
-        int next = -1;
+        int next = -1, nci = 0, ncj = 0;
         bool dep = false;
         if (DO_PUSH) {
             bool defer = false;
-            if (e_old >= 0) { defer = push_hot<R>(a, e_old, X, Y, VX, VY, &next); dep = !defer; }
+            if (e_old >= 0) { defer = push_hot<R>(a, e_old, X, Y, VX, VY, &next, &nci, &ncj); dep = !defer; }
             const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
             if (wm) {
                 unsigned int wb = 0;
                 if (lane == 0) wb = atomicAdd(a.wall_count, (unsigned int)__popcll(wm));
                 wb = __shfl(wb, 0, 64);
-                if (defer) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)p;
+                if (defer) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)(base + o);
             }
         } else if (e_old >= 0) {
             int bad = 0;
-            next = pic_locate_elem((double)X, (double)Y, m, &bad);
+            next = pic_locate_ij((double)X, (double)Y, m, &bad, &nci, &ncj);
             if (bad) {
                 atomicOr(a.err, PIC_EBIT_POSITION);
-                pic_record(a.errinfo, PIC_EBIT_POSITION, p, X, Y, VX, VY, 0, e_old, next, a.species);
+                pic_record(a.errinfo, PIC_EBIT_POSITION, base + o, X, Y, VX, VY, 0, e_old, next, a.species);
             }
             dep = true;
         }
         if (dep) {
             // GetParChg_2D.f90:788-843: weights and kinetic energy (ParticleOne.f90:339-340, :128)
             double W[4];
-            deposit_weights((double)X, (double)Y, next, m, W);
+            deposit_weights_ij((double)X, (double)Y, nci, ncj, m, W);
             double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
             double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
             energy = energy * a.v_ref * a.v_ref;
             energy = energy / PIC_JTOEV;
             long long qv[8];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) { qv[q] = pic_fix(W[q]); qv[4 + q] = pic_fix(W[q] * energy); }
+            pic_fix8(W, energy, qv);
             if (next != rkey) {
                 if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
                 rkey = next;
@@ -497,8 +543,8 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_push_deposit(PushArgs<R> a) {
         // write back (a deferred particle keeps its old state for the wall kernel, which
         // also writes its cell_next)
         if (in) {
-            a.cell_next[p] = next;
-            if (DO_PUSH && dep) { a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY; }
+            pic_st(nextb, o, next);
+            if (DO_PUSH && dep) { pic_st(xb, o, X); pic_st(yb, o, Y); pic_st(vxb, o, VX); pic_st(vyb, o, VY); }
         }
     }
     wave_deposit(rkey, m, acc, a.acc_w, a.acc_e, w, winW, winE);
@@ -525,6 +571,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
     a.m.element_index = g.d_element_index; a.m.node_index = g.d_node_index;
     a.m.elem_clean = g.d_elem_clean;
+    a.m.elem_nbr = g.d_elem_nbr;
     a.objs = g.d_objects;
     if (sizeof(R) == 8) {
         a.efx = (const R*)g.d_efx; a.efy = (const R*)g.d_efy; a.coef5 = (const R*)g.d_coef5;

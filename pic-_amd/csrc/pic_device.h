@@ -14,7 +14,7 @@
 // MOD of reals is taken as the exact remainder (fmod), which is what a == INT(a/p)*p
 // rounds to whenever p is a power of two. Returns the 0-based element, sets *bad when
 // the result falls outside the mesh.
-PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
+PIC_HD int pic_locate_ij(double X, double Y, const MeshParams& m, int* bad, int* ci, int* cj) {
     double rxp = (X - m.vox) * m.hxi;
     int i = (int)rxp;
     double ryp = (Y - m.voy) * m.hyi;
@@ -41,14 +41,51 @@ PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
         i = i < 1 ? 1 : (i > m.nex ? m.nex : i);
         j = j < 1 ? 1 : (j > m.ney ? m.ney : j);
     }
+    *ci = i - 1; *cj = j - 1;
     return (j - 1) + (i - 1) * m.ney;
 }
 
-// Slot layout after a cell sort: within every full group of PIC_SORT_GROUP slots the sorted
-// position 8 l + k (l = 0..63, k = 0..7) sits at slot 64 k + l, so lane l of a wave that
-// walks the group 64 slots at a time meets consecutive sorted particles (a partial last
-// group keeps the sorted order).
-#define PIC_SORT_GROUP 512
+PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
+    int ci, cj;
+    return pic_locate_ij(X, Y, m, bad, &ci, &cj);
+}
+
+// Column and row of element e (ci = e / ney, cj = e mod ney), 0 <= e < nelem. Below 2^24
+// elements the FP32 product e * RN(1/ney) is within 1 of e / ney (e is exact in FP32, two
+// roundings give a relative error under 2^-23 + 2^-48, and e / ney < nex < 2^23 when ney > 1;
+// ney = 1 is exact), so truncation is off by at most one, which the remainder test corrects.
+PIC_HD void pic_elem_ij(int e, const MeshParams& m, int* ci, int* cj) {
+    int q, r;
+    if (m.ij_exact_div) {
+        q = e / m.ney; r = e - q * m.ney;
+    } else {
+        q = (int)((float)e * m.ney_rcp);
+        r = e - q * m.ney;
+        q += (r >= m.ney) - (r < 0);
+        r = e - q * m.ney;
+    }
+    *ci = q; *cj = r;
+}
+
+// Loads and stores at base[i] with a 32-bit byte offset (i * sizeof(T) < 2^32): the address
+// is the wave-uniform base plus a zero-extended 32-bit lane offset, one instruction shorter
+// than 64-bit index arithmetic per access.
+template <typename T>
+PIC_HD T pic_ld(const T* base, unsigned int i) {
+    return *(const T*)((const char*)base + (size_t)(i * (unsigned int)sizeof(T)));
+}
+template <typename T>
+PIC_HD void pic_st(T* base, unsigned int i, T v) {
+    *(T*)((char*)base + (size_t)(i * (unsigned int)sizeof(T))) = v;
+}
+
+// Slot layout after a cell sort: within every full group of PIC_SORT_GROUP = 64 K slots the
+// sorted position K l + k (l = 0..63, k = 0..K-1) sits at slot 64 k + l, so lane l of a wave
+// that walks the group 64 slots at a time meets K consecutive sorted particles (a partial
+// last group keeps the sorted order). K = 4 keeps the push kernel at 5 waves per SIMD.
+#ifndef PIC_SORT_GROUP
+#define PIC_SORT_GROUP 256
+#endif
 PIC_HD int64_t pic_sorted_position(int64_t slot, int64_t nfull) {
     if (slot >= nfull) return slot;
     const int64_t g = slot & ~(int64_t)(PIC_SORT_GROUP - 1);
@@ -81,6 +118,27 @@ __device__ inline long long pic_fix(double v) {
         return __double_as_longlong(t) - __double_as_longlong(0x1.8p52);
     }
     return __double2ll_rn(x);
+}
+
+// The eight quanta of one particle's deposit: W[q] and W[q] * energy for the four corners,
+// each equal to pic_fix of the same value. When |W| < 2 and |energy| < 1000 every value v
+// has |v| < 2^11, so v + 1.5*2^12 lies in [2^12, 2^13) where the spacing is 2^-40: the add
+// rounds v*2^40 to the nearest (even) integer, held in the low mantissa bits (the same
+// rounding as pic_fix, one FP64 add and one 32-bit subtract on the high word per value).
+// The test is one uniform branch per particle instead of one per value.
+__device__ inline void pic_fix8(const double W[4], double energy, long long q[8]) {
+    const double wmax = fmax(fmax(fabs(W[0]), fabs(W[1])), fmax(fabs(W[2]), fabs(W[3])));
+    if (wmax < 2.0 && fabs(energy) < 1000.0) {
+        const long long off = __double_as_longlong(0x1.8p12);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            q[k] = __double_as_longlong(__dadd_rn(W[k], 0x1.8p12)) - off;
+            q[4 + k] = __double_as_longlong(__dadd_rn(__dmul_rn(W[k], energy), 0x1.8p12)) - off;   // never fused
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { q[k] = pic_fix(W[k]); q[4 + k] = pic_fix(W[k] * energy); }
+    }
 }
 #endif
 

@@ -59,11 +59,15 @@ struct MeshParams {
     double mapx, mapy;                  // hx / 2^r
     double imapx, imapy;                // 1 / mapx, 1 / mapy
     int pow2_map;                       // 1 if mapx, mapy are powers of two (exact division)
+    float ney_rcp;                      // 1 / ney rounded to FP32 (element -> column, pic_elem_ij)
+    int ij_exact_div;                   // 1 when nelem >= 2^24: pic_elem_ij divides
     const double* xs; const double* ys; // node coordinate lines
     const float* xsf; const float* ysf;
     const int* element_index;           // nelem
     const int* node_index;              // nnodes
     const unsigned char* elem_clean;    // nelem: 1 if no object cuts or comes near the element
+    const unsigned short* elem_nbr;     // nelem: bit 3 (di + 1) + (dj + 1) = elem_clean of element
+                                        // (ci + di, cj + dj), di, dj in -1..1 (0 outside the mesh)
     int n_objects;
     DevObject obj[PIC_MAX_OBJECTS];
 };
@@ -226,6 +230,7 @@ struct PicState {
     int mcc_enabled = 1;
     int64_t step = 0;
     unsigned char* d_elem_clean = nullptr;   // MeshParams::elem_clean
+    unsigned short* d_elem_nbr = nullptr;    // MeshParams::elem_nbr
     InjectHost inj[PIC_MAX_SPECIES];
     unsigned int* d_err = nullptr;     // device error flags
     void* d_errinfo = nullptr;         // PicErrInfo of the first failing particle

@@ -77,6 +77,9 @@ static void q1_axi(double x0, double y0, double hx, double hy, double K[4][4], d
 // ------------------------------------------------------------------ domain build
 int setup_build_domain(const pic_domain_config* c) {
     if (c->nnx < 3 || c->nny < 3) return pic_fail(PIC_ERR_ARG, "nnx, nny must be >= 3");
+    // node-indexed device arrays are addressed with 32-bit byte offsets (8-byte entries)
+    if ((int64_t)c->nnx * (int64_t)c->nny >= ((int64_t)1 << 29))
+        return pic_fail(PIC_ERR_ARG, "nnx * nny must be below 2^29");
     if (!(c->xmax > c->xmin) || !(c->ymax > c->ymin)) return pic_fail(PIC_ERR_ARG, "empty domain");
     if (c->n_objects < 0 || c->n_objects > PIC_MAX_OBJECTS) return pic_fail(PIC_ERR_ARG, "n_objects out of range");
     if (c->delta != 0 && c->delta != 1) return pic_fail(PIC_ERR_ARG, "delta must be 0 or 1");
@@ -114,6 +117,8 @@ int setup_build_domain(const pic_domain_config* c) {
     m.imapx = 1.0 / hx; m.imapy = 1.0 / hy;
     int ex_;
     m.pow2_map = (std::frexp(hx, &ex_) == 0.5 && std::frexp(hy, &ex_) == 0.5) ? 1 : 0;
+    m.ney_rcp = 1.0f / (float)ney;
+    m.ij_exact_div = ((int64_t)nelem >= ((int64_t)1 << 24)) ? 1 : 0;
     m.n_objects = c->n_objects;
     for (int k = 0; k < c->n_objects; ++k) {
         const pic_object& o = c->objects[k];

@@ -515,3 +515,65 @@ def test_injection_count_rounding(orc):
     counts = np.array([orc.inject_count(12.37, 3, s, orc.tag(0, orc.RNG_INJECT)) for s in range(4000)])
     assert set(np.unique(counts)) <= {12, 13}
     assert abs(counts.mean() - 12.37) < 5 * 0.5 / np.sqrt(4000)
+
+
+# ------------------------------------------------------------------ threaded oracle
+def _csr(A):
+    rows, cols = np.nonzero(np.abs(A) > 0)
+    rowptr = np.zeros(A.shape[0] + 1, dtype=np.int32)
+    np.add.at(rowptr, rows + 1, 1)
+    return np.cumsum(rowptr).astype(np.int32), cols.astype(np.int32), A[rows, cols], np.diag(A).copy()
+
+
+def test_threaded_oracle_matches_serial(orc):
+    """The OpenMP split of the oracle's loops (used only for the multi-core CPU timing)
+    reproduces the serial reference order: push, lookup, fixed-point deposition and PPR are
+    bit-identical; PCG and the RHS, whose sums are split over threads, agree to rounding."""
+    n = 16
+    xs, ys, ni, ei, pp, _ = _mesh(n, circles=[(8.0, 8.0, 2.5)])
+    nn = (n + 1) ** 2
+    rng = np.random.default_rng(23)
+    m = 40000
+    x = 0.05 + (n - 0.1) * rng.random(m)
+    y = (n - 1e-3) * rng.random(m)
+    v = rng.normal(size=(3, m)) * rng.choice([0.1, 2.0, 20.0], size=m)
+    ex, ey = rng.normal(size=(2, nn))
+    coef5 = rng.normal(size=5 * nn)
+    obj = np.array([[1, 8.0, 8.0, 0, 0, 6.25]])
+    cell, _ = orc.locate(xs, ys, x, y)
+    phi = rng.normal(size=nn)
+    _, _, nt2, A = _poisson_system(orc, n)
+    rp, col, val, diag = _csr(A)
+    b = rng.normal(size=A.shape[0])
+    rho = rng.normal(size=nn)
+    dphi = rng.normal(size=nn)
+
+    def run():
+        mv = orc.move_all(64, xs, ys, ei, ni, obj, ex, ey, coef5, -1.0, 0.05, 419396.657327663, 9.1095e-31,
+                          2 * 11605.0, 7, 3, orc.tag(0, 2), x, y, *v, cell)
+        keep = mv[5].astype(bool)
+        loc, bad = orc.locate(xs, ys, mv[0][keep], mv[1][keep])
+        dep = orc.deposit(64, xs, ys, 0, *(a[keep] for a in mv[:5]), loc, 9.1095e-31, 4.19e5, True)
+        pr = orc.ppr(xs, ys, pp, phi)
+        pc = orc.pcg(rp, col, val, diag, b, np.zeros_like(b), 1e-10, 5000)
+        rh = orc.rhs(xs, ys, 0, nt2, dphi, rho, A.shape[0])
+        return mv, loc, bad, dep, pr, pc, rh
+
+    ref = run()
+    try:
+        orc.set_threads(4)
+        assert orc.get_threads() == 4
+        got = run()
+    finally:
+        orc.set_threads(1)
+    for a, b_ in zip(ref[0][:6], got[0][:6]):
+        assert np.array_equal(a, b_)
+    assert np.array_equal(ref[0][6], got[0][6]) and ref[0][7] == got[0][7]
+    assert np.array_equal(ref[1], got[1]) and ref[2] == got[2]
+    for a, b_ in zip(ref[3], got[3]):
+        assert np.array_equal(a, b_)
+    for a, b_ in zip(ref[4][:3], got[4][:3]):
+        assert np.array_equal(a, b_)
+    np.testing.assert_allclose(got[5][0], ref[5][0], rtol=1e-9, atol=1e-12)
+    assert abs(got[5][2] - ref[5][2]) <= 1
+    np.testing.assert_allclose(got[6], ref[6], rtol=1e-13, atol=1e-14)
