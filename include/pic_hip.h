@@ -154,7 +154,9 @@ int  pic_set_species(int isp, const pic_species_params* p, int64_t capacity);
 
 /* Uniform positions per cell and Maxwellian (Box-Muller) velocities at temp_eV from
  * the Philox stream (seed, species): ppc particles in every element, generated in
- * element order. Replaces the RANDOM_NUMBER loaders of MCCInterface.f90:146-188. */
+ * element order. Replaces the RANDOM_NUMBER loaders of MCCInterface.f90:146-188. On an
+ * axisymmetric domain the radius is uniform in r^2 over the cell, r = sqrt(r0^2 + (r1 +
+ * r0)(r1 - r0) R), and the azimuth uniform on [0, 2 pi) (ParticleOne.f90:97-101). */
 int  pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed);
 
 /* Load / store a ParticleBundle in the reference's AoS record layout
@@ -172,6 +174,14 @@ int  pic_load_particles(int isp, int64_t n, const double* x, const double* y,
 int  pic_get_particles(int isp, double* x, double* y, double* vx, double* vy, double* vz,
                        int32_t* loc, int64_t capacity, int64_t* n);
 
+/* The azimuth PO%Z (ParticleOne.f90:4-6; radians) of an axisymmetric domain (delta = 1),
+ * where MoveOne moves in Cartesian coordinates and returns to (r, PO%Z)
+ * (MoveOne.f90:246-263). Set: slots 0..n-1 in load order (after pic_load_particles);
+ * get: the live particles in the order of pic_get_particles. PIC_ERR_STATE on a Cartesian
+ * domain, which keeps no azimuth. pic_load_bundle / pic_store_bundle carry Z through these. */
+int  pic_set_azimuth(int isp, int64_t n, const double* th);
+int  pic_get_azimuth(int isp, double* th, int64_t capacity, int64_t* n);
+
 /* Live particles, occupied slots, and event counters of a species:
  * counters[0] nLoss(1) left-wall re-emissions, [1] nLoss(2) right-wall absorptions,
  * [2] object absorptions, [3] wall reflections, [4] MCC selected (null incl.),
@@ -188,7 +198,9 @@ int  pic_get_mcc_events(int isp, int64_t* events, int nmax);
  * pushers are not on this path). Objects come from pic_setup_domain. The same
  * kernel also computes each particle's new element and its deposition weights, kept
  * pending until GetParChg_2D commits them (the element seen by the next gather is
- * the one GetParChg_2D stores, exactly as in the reference). */
+ * the one GetParChg_2D stores, exactly as in the reference). delta = 1 is the
+ * axisymmetric mover (x axial, y radial, velocities in a fixed Cartesian frame, the
+ * azimuth PO%Z updated; MoveOne.f90:94-104, :221-236, :246-263). */
 int  pic_MoveAll(int isp, int delta, int PushFlag);
 
 /* GetParChg_2D, code/PIC/GetParChg_2D.f90:1-952: per-particle element lookup

@@ -78,6 +78,55 @@ static double or_u01(or_stream* s) {
 static uint32_t or_tag(int species, int purpose) { return ((uint32_t)purpose << 16) | (uint32_t)(species & 0xFFFF); }
 enum { OR_RNG_LOAD = 1, OR_RNG_EMIT = 2, OR_RNG_SELECT = 3, OR_RNG_COLLIDE = 4 };
 
+/* ------------------------------------------------------------------ trig */
+/* DCOS / DSIN / DATAN of the axisymmetric MoveOne (MoveOne.f90:174-176, :248-262) as the
+ * build evaluates them: a fixed sequence of IEEE operations (Cody-Waite reduction by a
+ * three-part pi/2, Taylor series through r^15 / r^16 for sin / cos on [-pi/4, pi/4]; atan by
+ * the reductions 1/t and (t-1)/(t+1) to |u| <= tan(pi/8) and the series through u^41/41).
+ * Pinned against the C library to about 1 ulp by tests/test_oracle_cpu.py. */
+void or_sincos(double x, double* s, double* c) {
+    const double kd = rint(x * 0x1.45f306dc9c883p-1);
+    const double r = ((x - kd * 0x1.921fb54400000p+0) - kd * 0x1.0b4611a600000p-34) - kd * 0x1.3198a2e037073p-69;
+    const double z = r * r;
+    const double ps = -0x1.5555555555555p-3 + z * (0x1.1111111111111p-7 + z * (-0x1.a01a01a01a01ap-13 +
+                      z * (0x1.71de3a556c734p-19 + z * (-0x1.ae64567f544e4p-26 + z * (0x1.6124613a86d09p-33 +
+                      z * -0x1.ae7f3e733b81fp-41)))));
+    const double sr = r + r * z * ps;
+    const double pc = 0x1.5555555555555p-5 + z * (-0x1.6c16c16c16c17p-10 + z * (0x1.a01a01a01a01ap-16 +
+                      z * (-0x1.27e4fb7789f5cp-22 + z * (0x1.1eed8eff8d898p-29 + z * (-0x1.93974a8c07c9dp-37 +
+                      z * 0x1.ae7f3e733b81fp-45)))));
+    const double hz = 0.5 * z, w = 1.0 - hz;
+    const double cr = w + (((1.0 - w) - hz) + z * z * pc);
+    switch ((int)kd & 3) {
+        case 0: *s = sr; *c = cr; break;
+        case 1: *s = cr; *c = -sr; break;
+        case 2: *s = -sr; *c = -cr; break;
+        default: *s = -cr; *c = sr; break;
+    }
+}
+
+double or_atan(double t) {
+    static const double A[20] = {-0x1.5555555555555p-2, 0x1.999999999999ap-3, -0x1.2492492492492p-3,
+        0x1.c71c71c71c71cp-4, -0x1.745d1745d1746p-4, 0x1.3b13b13b13b14p-4, -0x1.1111111111111p-4,
+        0x1.e1e1e1e1e1e1ep-5, -0x1.af286bca1af28p-5, 0x1.8618618618618p-5, -0x1.642c8590b2164p-5,
+        0x1.47ae147ae147bp-5, -0x1.2f684bda12f68p-5, 0x1.1a7b9611a7b96p-5, -0x1.0842108421084p-5,
+        0x1.f07c1f07c1f08p-6, -0x1.d41d41d41d41dp-6, 0x1.bacf914c1bad0p-6, -0x1.a41a41a41a41ap-6,
+        0x1.8f9c18f9c18fap-6};                                  /* (-1)^n / (2n+1), n = 1..20 */
+    if (!(t == t)) return t;
+    double a = fabs(t);
+    const int inv = a > 1.0;
+    if (inv) a = 1.0 / a;
+    const int red = a > 0x1.a827999fcef32p-2;                 /* tan(pi/8) */
+    const double u = red ? (a - 1.0) / (a + 1.0) : a;
+    const double z = u * u;
+    double q = A[19];
+    for (int n = 18; n >= 0; --n) q = A[n] + z * q;
+    double p = u + u * z * q;
+    if (red) p = 0x1.921fb54442d18p-1 + (p + 0x1.1a62633145c07p-55);       /* pi/4 hi + lo */
+    if (inv) p = 0x1.921fb54442d18p+0 - (p - 0x1.1a62633145c07p-54);       /* pi/2 hi - lo */
+    return t < 0.0 ? -p : p;
+}
+
 /* ------------------------------------------------------------------ mesh */
 typedef struct { int shape; double x0, y0, x1, y1, r2; } or_object;
 
@@ -137,26 +186,26 @@ int or_get_threads(void) { return or_threads; }
 #undef FN
 
 int or_move_all(int bits, int nnx, int nny, const double* xs, const double* ys, const int* element_index,
-                const int* node_index, int n_objects, const double* objects,
+                const int* node_index, int n_objects, const double* objects, int delta,
                 const void* efx, const void* efy, const void* coef5, double qm, double dt,
                 double v_ref, double mass, double init_temp_K, uint64_t seed, uint32_t step, uint32_t tag_emit,
-                int64_t n, void* x, void* y, void* vx, void* vy, void* vz, const int* cell, int* alive,
+                int64_t n, void* x, void* y, void* vx, void* vy, void* vz, void* th, const int* cell, int* alive,
                 int64_t* counters) {
     or_mesh m;
     mk_mesh(&m, nnx, nny, xs, ys, element_index, node_index, n_objects, objects);
     int err = 0;
     if (bits == 64) {
-        or_move_all_f64(&m, (const double*)efx, (const double*)efy, (const double*)coef5, xs, ys, qm, dt, v_ref, mass,
-                        init_temp_K, seed, step, tag_emit, n, (double*)x, (double*)y, (double*)vx, (double*)vy,
-                        (double*)vz, cell, alive, counters, &err);
+        or_move_all_f64(&m, delta, (const double*)efx, (const double*)efy, (const double*)coef5, xs, ys, qm, dt, v_ref,
+                        mass, init_temp_K, seed, step, tag_emit, n, (double*)x, (double*)y, (double*)vx, (double*)vy,
+                        (double*)vz, (double*)th, cell, alive, counters, &err);
     } else {
         float* xsf = (float*)malloc(sizeof(float) * nnx);
         float* ysf = (float*)malloc(sizeof(float) * nny);
         for (int i = 0; i < nnx; ++i) xsf[i] = (float)xs[i];
         for (int j = 0; j < nny; ++j) ysf[j] = (float)ys[j];
-        or_move_all_f32(&m, (const float*)efx, (const float*)efy, (const float*)coef5, xsf, ysf, (float)qm, (float)dt,
-                        v_ref, mass, init_temp_K, seed, step, tag_emit, n, (float*)x, (float*)y, (float*)vx,
-                        (float*)vy, (float*)vz, cell, alive, counters, &err);
+        or_move_all_f32(&m, delta, (const float*)efx, (const float*)efy, (const float*)coef5, xsf, ysf, (float)qm,
+                        (float)dt, v_ref, mass, init_temp_K, seed, step, tag_emit, n, (float*)x, (float*)y, (float*)vx,
+                        (float*)vy, (float*)vz, (float*)th, cell, alive, counters, &err);
         free(xsf); free(ysf);
     }
     return err;

@@ -29,15 +29,19 @@ def lib():
         if not LIB.exists():
             build()
         _lib = C.CDLL(str(LIB))
+        _lib.or_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _lib.or_sincos.restype = None
+        _lib.or_atan.argtypes = [C.c_double]
+        _lib.or_atan.restype = C.c_double
         _lib.or_set_threads.argtypes = [C.c_int]
         _lib.or_get_threads.restype = C.c_int
         _lib.or_philox_block.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib.or_move_all.restype = C.c_int
         _lib.or_move_all.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                     C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                     C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
                                      C.c_uint32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.or_locate.restype = C.c_int
         _lib.or_locate.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
@@ -83,6 +87,16 @@ def lib():
     return _lib
 
 
+def sincos(x: float) -> tuple[float, float]:
+    s, c = C.c_double(), C.c_double()
+    lib().or_sincos(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def atan(t: float) -> float:
+    return lib().or_atan(float(t))
+
+
 def set_threads(n: int) -> None:
     """OpenMP threads of the particle, lookup, deposition, RHS, PCG and PPR loops (1 = the
     serial reference order, the default)."""
@@ -113,18 +127,23 @@ RNG_EMIT, RNG_SELECT, RNG_COLLIDE = 2, 3, 4
 
 
 def move_all(bits, xs, ys, element_index, node_index, objects, efx, efy, coef5, qm, dt, v_ref, mass, init_temp_K,
-             seed, step, tag_emit, x, y, vx, vy, vz, cell):
-    """MoveAll on copies; returns (x, y, vx, vy, vz, alive, counters, err)."""
+             seed, step, tag_emit, x, y, vx, vy, vz, cell, delta=0, th=None):
+    """MoveAll on copies; returns (x, y, vx, vy, vz, alive, counters, err), plus the azimuth th
+    when delta = 1 (axisymmetric: x axial, y radial, th = PO%Z)."""
     ft = np.float64 if bits == 64 else np.float32
     st = [np.array(a, dtype=ft) for a in (x, y, vx, vy, vz)]
+    tha = np.array(th if th is not None else np.zeros(len(cell)), dtype=ft)
     f = [np.ascontiguousarray(a, dtype=ft) for a in (efx, efy, coef5)]
     cell = np.ascontiguousarray(cell, dtype=np.int32)
     alive = np.zeros(len(cell), dtype=np.int32)
     cnt = np.zeros(8, dtype=np.int64)
     objs = np.ascontiguousarray(objects, dtype=np.float64)
     err = lib().or_move_all(bits, len(xs), len(ys), _p(xs), _p(ys), _p(element_index), _p(node_index),
-                            len(objects), _p(objs), _p(f[0]), _p(f[1]), _p(f[2]), qm, dt, v_ref, mass, init_temp_K,
-                            seed, step, tag_emit, len(cell), *[_p(a) for a in st], _p(cell), _p(alive), _p(cnt))
+                            len(objects), _p(objs), int(delta), _p(f[0]), _p(f[1]), _p(f[2]), qm, dt, v_ref, mass,
+                            init_temp_K, seed, step, tag_emit, len(cell), *[_p(a) for a in st], _p(tha), _p(cell),
+                            _p(alive), _p(cnt))
+    if delta == 1:
+        return (*st, alive, cnt, err, tha)
     return (*st, alive, cnt, err)
 
 

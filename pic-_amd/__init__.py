@@ -103,6 +103,8 @@ _PROTOS = {
     "pic_store_bundle": (C.c_int, [C.c_int, _P(ParticleOne), C.c_int64, _i64p]),
     "pic_load_particles": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]),
     "pic_get_particles": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int64, _i64p]),
+    "pic_set_azimuth": (C.c_int, [C.c_int, C.c_int64, _dp]),
+    "pic_get_azimuth": (C.c_int, [C.c_int, _dp, C.c_int64, _i64p]),
     "pic_get_counts": (C.c_int, [C.c_int, _i64p, _i64p, _i64p]),
     "pic_get_mcc_events": (C.c_int, [C.c_int, _i64p, C.c_int]),
     "pic_MoveAll": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -277,6 +279,11 @@ def workload(name: str) -> dict:
         # 71 per vacuum element: 16.8M particles per species (the conductors take 9.7% of the area);
         # both species injected through the left wall as a Maxwellian drifting inward at 0.1 v_th
         return dict(domain=dom, ppc=71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026, inject_drift_vth=0.1)
+    if name.startswith("rz"):     # axisymmetric parity cases rz<N>: x axial, y radial, the axis (bottom) natural
+        n = int(name[2:] or 16)
+        dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1,
+                     objects=[Circle(0.5 * n, 0.5 * n, 0.15 * n, -1.0)], wall_dirichlet=(1, 1, 0, 1))
+        return dict(domain=dom, ppc=8, bits=64, solver=0, tol=1e-12, Te=2.0, Ti=0.026)
     if name.startswith("small"):  # small parity cases: small<N>
         n = int(name[5:] or 16)
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),
@@ -332,10 +339,23 @@ class PicLibrary:
     def load_maxwellian(self, isp: int, ppc: int, temp_eV: float, seed: int):
         check(lib().pic_load_maxwellian(isp, ppc, temp_eV, seed))
 
-    def load_particles(self, isp, x, y, vx, vy, vz, loc=None):
+    def load_particles(self, isp, x, y, vx, vy, vz, loc=None, th=None):
+        """th: the azimuth PO%Z of each particle (axisymmetric domains)."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, vx, vy, vz)]
         lp = None if loc is None else _i(np.ascontiguousarray(loc, dtype=np.int32))
         check(lib().pic_load_particles(isp, len(arrs[0]), *[_d(a) for a in arrs], lp))
+        if th is not None:
+            t = np.ascontiguousarray(th, dtype=np.float64)
+            check(lib().pic_set_azimuth(isp, len(t), _d(t)))
+
+    def azimuth(self, isp: int) -> np.ndarray:
+        """PO%Z of the live particles, in the order of particles() (axisymmetric domains)."""
+        ns = C.c_int64()
+        check(lib().pic_get_counts(isp, None, C.byref(ns), None))
+        out = np.zeros(max(ns.value, 1))
+        n = C.c_int64()
+        check(lib().pic_get_azimuth(isp, _d(out), len(out), C.byref(n)))
+        return out[:n.value]
 
     def particles(self, isp: int) -> dict:
         nl, ns = C.c_int64(), C.c_int64()
@@ -375,14 +395,15 @@ class PicLibrary:
         check(lib().pic_set_field(FIELD[name], _d(a), a.size))
 
     # reference-named drop-ins
-    def MoveAll(self, isp: int, delta: int = 0, PushFlag: int = 0):
-        check(lib().pic_MoveAll(isp, delta, PushFlag))
+    def MoveAll(self, isp: int, delta: int | None = None, PushFlag: int = 0):
+        check(lib().pic_MoveAll(isp, self.dom.delta if delta is None else delta, PushFlag))
 
-    def GetParChg_2D(self, delta: int = 0):
-        check(lib().pic_GetParChg_2D(delta))
+    def GetParChg_2D(self, delta: int | None = None):
+        check(lib().pic_GetParChg_2D(self.dom.delta if delta is None else delta))
 
-    def IFE_Solve_2D(self, Rho=None, Phi=None, delta: int = 0, time: float = 0.0):
+    def IFE_Solve_2D(self, Rho=None, Phi=None, delta: int | None = None, time: float = 0.0):
         d = self.dom
+        delta = d.delta if delta is None else delta
         r = None if Rho is None else _d(np.ascontiguousarray(Rho, dtype=np.float64))
         p = None
         if Phi is not None:

@@ -144,8 +144,9 @@ __global__ void k_d2f_n(int64_t n, const double* a, float* b) { int64_t i = (int
 
 static void free_species(int s) {
     SpeciesHost& S = g.sp[s];
-    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob, S.rtype,
-                    S.thr, S.mcc_events, S.spare[0], S.spare[1], S.spare[2], S.spare[3], S.spare[4]};
+    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.th, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob,
+                    S.rtype, S.thr, S.mcc_events, S.spare[0], S.spare[1], S.spare[2], S.spare[3], S.spare[4],
+                    S.spare[5]};
     for (void* p : ptrs) if (p) hipFree(p);
     S = SpeciesHost();
     g.inj[s] = InjectHost();
@@ -442,11 +443,18 @@ int pic_set_species(int isp, const pic_species_params* p, int64_t capacity) {
     S.capacity = capacity;
     S.nslots = 0; S.nlive = 0;
     size_t esz = g.particle_bits == 64 ? 8 : 4;
-    void** arr[5] = {&S.x, &S.y, &S.vx, &S.vy, &S.vz};
-    for (int a = 0; a < 5; ++a) {
+    void** arr[6] = {&S.x, &S.y, &S.vx, &S.vy, &S.vz, &S.th};
+    for (int a = 0; a < 6; ++a) {
         if (*arr[a]) hipFree(*arr[a]);
         *arr[a] = nullptr;
-        HIPCHK(hipMalloc(arr[a], (size_t)capacity * esz));
+        // the azimuth PO%Z is kept for axisymmetric domains only
+        if (a < 5 || g.mesh.delta == 1) HIPCHK(hipMalloc(arr[a], (size_t)capacity * esz));
+    }
+    if (S.th) HIPCHK(hipMemset(S.th, 0, (size_t)capacity * esz));
+    // the sort's spare arrays are sized for the capacity: drop them, the next sort reallocates
+    for (int a = 0; a < 6; ++a) {
+        if (S.spare[a]) (void)hipFree(S.spare[a]);
+        S.spare[a] = nullptr;
     }
     TRY(dalloc(&S.cell, (size_t)capacity)); TRY(dalloc(&S.cell_next, (size_t)capacity));
     TRY(dalloc(&S.acc_w, (size_t)g.mesh.nnodes)); TRY(dalloc(&S.acc_e, (size_t)g.mesh.nnodes));
@@ -492,9 +500,13 @@ int pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed) {
     if (ppc <= 0 || !(temp_eV >= 0)) return pic_fail(PIC_ERR_ARG, "bad ppc / temperature");
     std::vector<double> x, y, vx, vy, vz;
     std::vector<int> cell;
-    setup_maxwellian(isp, ppc, temp_eV, seed, x, y, vx, vy, vz, cell);
+    std::vector<double> th;
+    const bool rz = g.mesh.delta == 1;
+    setup_maxwellian(isp, ppc, temp_eV, seed, x, y, vx, vy, vz, cell, rz ? &th : nullptr);
     if ((int64_t)x.size() > g.sp[isp].capacity) return pic_fail(PIC_ERR_CAPACITY, "load exceeds capacity");
-    return pic_load_particles(isp, (int64_t)x.size(), x.data(), y.data(), vx.data(), vy.data(), vz.data(), nullptr);
+    TRY(pic_load_particles(isp, (int64_t)x.size(), x.data(), y.data(), vx.data(), vy.data(), vz.data(), nullptr));
+    if (rz) TRY(pic_set_azimuth(isp, (int64_t)th.size(), th.data()));
+    return PIC_OK;
 }
 
 
@@ -529,14 +541,57 @@ int pic_get_particles(int isp, double* x, double* y, double* vx, double* vy, dou
     return PIC_OK;
 }
 
+int pic_set_azimuth(int isp, int64_t n, const double* th) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    SpeciesHost& S = g.sp[isp];
+    if (g.mesh.delta != 1 || !S.th) return pic_fail(PIC_ERR_STATE, "the azimuth is kept for axisymmetric domains only");
+    if (n < 0 || n > S.nslots || (n > 0 && !th)) return pic_fail(PIC_ERR_ARG, "azimuth count exceeds the occupied slots");
+    if (n == 0) return PIC_OK;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (g.particle_bits == 64) {
+        HIPCHK(hipMemcpy(S.th, th, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(th, th + n);
+        HIPCHK(hipMemcpy(S.th, f.data(), (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return PIC_OK;
+}
+
+int pic_get_azimuth(int isp, double* th, int64_t capacity, int64_t* n) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    SpeciesHost& S = g.sp[isp];
+    if (g.mesh.delta != 1 || !S.th) return pic_fail(PIC_ERR_STATE, "the azimuth is kept for axisymmetric domains only");
+    HIPCHK(hipStreamSynchronize(g.stream));
+    const int64_t ns = S.nslots;
+    std::vector<double> a;
+    if (g.particle_bits == 64) to_host_double<double>(S.th, ns, a);
+    else to_host_double<float>(S.th, ns, a);
+    std::vector<int> c(ns);
+    if (ns) HIPCHK(hipMemcpy(c.data(), S.cell, ns * sizeof(int), hipMemcpyDeviceToHost));
+    int64_t k = 0;
+    for (int64_t i = 0; i < ns; ++i) {
+        if (c[i] < 0) continue;
+        if (k >= capacity) return pic_fail(PIC_ERR_CAPACITY, "destination too small");
+        if (th) th[k] = a[i];
+        ++k;
+    }
+    *n = k;
+    return PIC_OK;
+}
+
 int pic_load_bundle(int isp, const pic_particle_one* po, int64_t npar) {
     if (npar < 0 || (npar > 0 && !po)) return pic_fail(PIC_ERR_ARG, "bad bundle");
-    std::vector<double> x(npar), y(npar), vx(npar), vy(npar), vz(npar);
+    std::vector<double> x(npar), y(npar), z(npar), vx(npar), vy(npar), vz(npar);
     std::vector<int> loc(npar);
     for (int64_t i = 0; i < npar; ++i) {
-        x[i] = po[i].X; y[i] = po[i].Y; vx[i] = po[i].Vx; vy[i] = po[i].Vy; vz[i] = po[i].Vz; loc[i] = po[i].Location;
+        x[i] = po[i].X; y[i] = po[i].Y; z[i] = po[i].Z;
+        vx[i] = po[i].Vx; vy[i] = po[i].Vy; vz[i] = po[i].Vz; loc[i] = po[i].Location;
     }
-    return pic_load_particles(isp, npar, x.data(), y.data(), vx.data(), vy.data(), vz.data(), loc.data());
+    TRY(pic_load_particles(isp, npar, x.data(), y.data(), vx.data(), vy.data(), vz.data(), loc.data()));
+    if (g.mesh.delta == 1) TRY(pic_set_azimuth(isp, npar, z.data()));
+    return PIC_OK;
 }
 
 int pic_store_bundle(int isp, pic_particle_one* po, int64_t capacity, int64_t* npar) {
@@ -548,10 +603,16 @@ int pic_store_bundle(int isp, pic_particle_one* po, int64_t capacity, int64_t* n
     int64_t n = 0;
     TRY(pic_get_particles(isp, x.data(), y.data(), vx.data(), vy.data(), vz.data(), loc.data(), cap, &n));
     if (n > capacity) return pic_fail(PIC_ERR_CAPACITY, "destination too small");
+    std::vector<double> z(g.mesh.delta == 1 ? cap : 0);
+    if (g.mesh.delta == 1) {
+        int64_t nz = 0;
+        TRY(pic_get_azimuth(isp, z.data(), cap, &nz));
+    }
     for (int64_t i = 0; i < n; ++i) {
         pic_particle_one& o = po[i];
         memset(&o, 0, sizeof(o));
         o.X = x[i]; o.Y = y[i]; o.Vx = vx[i]; o.Vy = vy[i]; o.Vz = vz[i]; o.Location = loc[i];
+        if (g.mesh.delta == 1) o.Z = z[i];
     }
     *npar = n;
     return PIC_OK;
@@ -590,7 +651,6 @@ int pic_MoveAll(int isp, int delta, int PushFlag) {
     if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
     if (PushFlag != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "only the explicit mover (PushFlag = 0) is on this path");
     if (delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "delta differs from the set-up domain");
-    if (delta == 1) return pic_fail(PIC_ERR_UNSUPPORTED, "axisymmetric mover not built");
     SpeciesHost& S = g.sp[isp];
     if (S.pending) S.acc_dirty = true;   // a repeated MoveAll discards the previous pending deposit
     TRY(launch_push_deposit(isp, true));

@@ -118,6 +118,7 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_sel_write(const uint32_t* mask, c
 }
 
 // ---------------------------------------------------------------- collision
+#define MCC_PROD_W 7
 struct MccArgs {
     int nsel;
     const int* sel;
@@ -138,8 +139,9 @@ struct MccArgs {
     // staged ionisation products, one pair per selected particle; bit k of newmask marks
     // selected particle k as having created a pair (null when the species creates none)
     unsigned long long* newmask;
-    double* new_e;          // 6 per selected: x, y, vx, vy, vz, cell
+    double* new_e;          // MCC_PROD_W per selected: x, y, vx, vy, vz, cell, azimuth
     double* new_i;
+    const void* th;         // azimuth of the species (axisymmetric domains), else null
 };
 
 struct Vel3 { double x, y, z; };
@@ -273,8 +275,10 @@ __device__ inline bool collide_one(const MccArgs& a, int k) {
                 post_collision(M, &s, 0);
                 double EnergyNew = EnergyTemp - EnergyOld;
                 double f2 = sqrt(EnergyNew / Eorig);
-                double* ne = a.new_e + (size_t)k * 6;
-                double* ni = a.new_i + (size_t)k * 6;
+                double* ne = a.new_e + (size_t)k * MCC_PROD_W;
+                double* ni = a.new_i + (size_t)k * MCC_PROD_W;
+                // both products start at the creator's position, azimuth included
+                ne[6] = ni[6] = a.th ? (double)((const R*)a.th)[slot] : 0.0;
                 ne[0] = (double)X[slot]; ne[1] = (double)Y[slot];
                 ne[2] = vsolid.x * f2 / a.v_ref; ne[3] = vsolid.y * f2 / a.v_ref; ne[4] = vsolid.z * f2 / a.v_ref;
                 ne[5] = (double)a.cell[slot];
@@ -324,6 +328,7 @@ struct AppendDst {
     void *x, *y, *vx, *vy, *vz;
     int* cell;
     int64_t base;           // first free slot
+    void* th;               // azimuth (axisymmetric domains), else null
 };
 
 template <typename R>
@@ -331,6 +336,7 @@ __device__ inline void put_product(const AppendDst& d, int64_t i, const double* 
     ((R*)d.x)[i] = (R)st[0]; ((R*)d.y)[i] = (R)st[1];
     ((R*)d.vx)[i] = (R)st[2]; ((R*)d.vy)[i] = (R)st[3]; ((R*)d.vz)[i] = (R)st[4];
     d.cell[i] = (int)st[5];
+    if (d.th) ((R*)d.th)[i] = (R)st[6];
 }
 
 // room = free slots of the fuller species; nothing is written when the pairs do not fit
@@ -356,8 +362,8 @@ __global__ __launch_bounds__(SEL_BLOCK) void k_append_pairs(int nwords, const un
     while (word) {
         int b = __ffsll((long long)word) - 1;
         int64_t k = (int64_t)w * 64 + b;
-        put_product<R>(E, E.base + pos, new_e + k * 6);
-        put_product<R>(I, I.base + pos, new_i + k * 6);
+        put_product<R>(E, E.base + pos, new_e + k * MCC_PROD_W);
+        put_product<R>(I, I.base + pos, new_i + k * MCC_PROD_W);
         ++pos;
         word &= word - 1ull;
     }
@@ -405,6 +411,7 @@ static void launch_collide(int isp, int nsel, const int* sel, unsigned long long
     a.seed = g.seed; a.step = (uint32_t)g.step; a.tag = pic_stream_tag(isp, PIC_RNG_COLLIDE);
     a.events = S.mcc_events; a.counters = S.counters; a.err = g.d_err;
     a.newmask = newmask; a.new_e = new_e; a.new_i = new_i;
+    a.th = g.mesh.delta == 1 ? S.th : nullptr;
     hipLaunchKernelGGL((k_mcc_collide<R>), dim3((nsel + 127) / 128), dim3(128), 0, g.stream, a);
 }
 
@@ -447,11 +454,12 @@ static int run_mcc_t() {
     rc = ensure_buf((void**)&g.d_sel, &g.sel_cap, (int64_t)(nsel_e + nsel_i) * sizeof(int));
     if (rc) return rc;
     int nwords = (nsel_e + 63) / 64;
-    rc = ensure_buf(&g.d_newbuf, &g.newbuf_cap, (int64_t)nsel_e * 12 * sizeof(double) + (int64_t)nwords * 8 + 64);
+    rc = ensure_buf(&g.d_newbuf, &g.newbuf_cap,
+                    (int64_t)nsel_e * 2 * MCC_PROD_W * sizeof(double) + (int64_t)nwords * 8 + 64);
     if (rc) return rc;
     double* new_e = (double*)g.d_newbuf;
-    double* new_i = new_e + (size_t)nsel_e * 6;
-    unsigned long long* newmask = (unsigned long long*)(new_e + (size_t)nsel_e * 12);
+    double* new_i = new_e + (size_t)nsel_e * MCC_PROD_W;
+    unsigned long long* newmask = (unsigned long long*)(new_e + (size_t)nsel_e * 2 * MCC_PROD_W);
     if (nsel_e) {
         hipLaunchKernelGGL(k_sel_write, dim3(nb_e), dim3(SEL_BLOCK), 0, g.stream, mask_e, bc_e, g.d_sel);
         launch_collide<R>(e, nsel_e, g.d_sel, newmask, new_e, new_i);
@@ -470,8 +478,9 @@ static int run_mcc_t() {
     hipLaunchKernelGGL(k_mask_count, dim3(nbp), dim3(SEL_BLOCK), 0, g.stream, nwords, newmask, g.d_pcount);
     hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(SCAN_THREADS), 0, g.stream, nbp, g.d_pcount, info + 2,
                        (unsigned long long*)nullptr, 0);
-    AppendDst de{E.x, E.y, E.vx, E.vy, E.vz, E.cell, E.nslots};
-    AppendDst di{I.x, I.y, I.vx, I.vy, I.vz, I.cell, I.nslots};
+    const bool rz = g.mesh.delta == 1;
+    AppendDst de{E.x, E.y, E.vx, E.vy, E.vz, E.cell, E.nslots, rz ? E.th : nullptr};
+    AppendDst di{I.x, I.y, I.vx, I.vy, I.vz, I.cell, I.nslots, rz ? I.th : nullptr};
     hipLaunchKernelGGL((k_append_pairs<R>), dim3(nbp), dim3(SEL_BLOCK), 0, g.stream, nwords, newmask, g.d_pcount,
                        new_e, new_i, de, di, room);
     HIPCHK(hipGetLastError());
@@ -511,12 +520,13 @@ __global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsi
 template <typename R>
 __global__ __launch_bounds__(256) void k_sort_gather(int64_t n, int64_t nfull, const unsigned int* perm,
                                                      const R* x, const R* y, const R* vx, const R* vy, const R* vz,
-                                                     const int* cell, R* ox, R* oy, R* ovx, R* ovy, R* ovz,
-                                                     int* ocell) {
+                                                     const int* cell, const R* th, R* ox, R* oy, R* ovx, R* ovy,
+                                                     R* ovz, int* ocell, R* oth) {
     int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= n) return;
     const unsigned int s = perm[pic_sorted_position(d, nfull)];
     ox[d] = x[s]; oy[d] = y[s]; ovx[d] = vx[s]; ovy[d] = vy[s]; ovz[d] = vz[s]; ocell[d] = cell[s];
+    if (th) oth[d] = th[s];
 }
 #define TRY_MCC(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
 
@@ -558,7 +568,8 @@ static int sort_species(int isp) {
     int64_t live = 0;
     TRY_MCC(count_live_slots(n, S.cell, &live));
     // gather into the spare arrays (and cell_next), then swap them in
-    for (int a = 0; a < 5; ++a)
+    const bool rz = g.mesh.delta == 1 && S.th;
+    for (int a = 0; a < (rz ? 6 : 5); ++a)
         if (!S.spare[a]) HIPCHK(hipMalloc(&S.spare[a], (size_t)S.capacity * esz));
     const int64_t nfull = live / PIC_SORT_GROUP * PIC_SORT_GROUP;
     const int gblocks = (int)((live + 255) / 256);
@@ -566,17 +577,20 @@ static int sort_species(int isp) {
         if (esz == 8)
             hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
                                (const double*)S.x, (const double*)S.y, (const double*)S.vx, (const double*)S.vy,
-                               (const double*)S.vz, S.cell, (double*)S.spare[0], (double*)S.spare[1],
-                               (double*)S.spare[2], (double*)S.spare[3], (double*)S.spare[4], S.cell_next);
+                               (const double*)S.vz, S.cell, rz ? (const double*)S.th : nullptr,
+                               (double*)S.spare[0], (double*)S.spare[1], (double*)S.spare[2], (double*)S.spare[3],
+                               (double*)S.spare[4], S.cell_next, rz ? (double*)S.spare[5] : nullptr);
         else
             hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
                                (const float*)S.x, (const float*)S.y, (const float*)S.vx, (const float*)S.vy,
-                               (const float*)S.vz, S.cell, (float*)S.spare[0], (float*)S.spare[1],
-                               (float*)S.spare[2], (float*)S.spare[3], (float*)S.spare[4], S.cell_next);
+                               (const float*)S.vz, S.cell, rz ? (const float*)S.th : nullptr,
+                               (float*)S.spare[0], (float*)S.spare[1], (float*)S.spare[2], (float*)S.spare[3],
+                               (float*)S.spare[4], S.cell_next, rz ? (float*)S.spare[5] : nullptr);
         HIPCHK(hipGetLastError());
     }
     std::swap(S.x, S.spare[0]); std::swap(S.y, S.spare[1]); std::swap(S.vx, S.spare[2]);
     std::swap(S.vy, S.spare[3]); std::swap(S.vz, S.spare[4]); std::swap(S.cell, S.cell_next);
+    if (rz) std::swap(S.th, S.spare[5]);
     HIPCHK(hipStreamSynchronize(g.stream));
     S.nslots = (int64_t)live;
     S.nlive = (int64_t)live;

@@ -13,6 +13,7 @@
 // outside the window, straight to global atomics. All sums are int64 fixed point, so
 // the node moments are identical whatever path or order a deposit takes.
 #include "pic_device.h"
+#include "pic_trig.h"
 
 #define PUSH_BLOCK 256
 #define PUSH_PPT (PIC_SORT_GROUP / 64)
@@ -31,6 +32,7 @@
 
 template <typename R> struct PushArgs {
     R *x, *y, *vx, *vy, *vz;
+    R* th;                    // azimuth (delta = 1), else null
     const int* cell;
     int* cell_next;
     int64_t n;
@@ -52,6 +54,7 @@ template <typename R> struct PushArgs {
     PicErrInfo* errinfo;
     int species;
     int* wall_list; unsigned int* wall_count;   // particles whose first pass reaches a wall
+    int all_slots;            // k_push_walls over every slot (axisymmetric mover) instead of the list
 };
 
 __device__ inline long long shfl_xor_ll(long long v, int d) {
@@ -184,7 +187,7 @@ struct PushEvents { int emit, right, object, reflect, deleted; };
 // MoveAll for one particle (MCCInterface.f90:591-615 with MoveOne + AdjustParticleBoundary):
 // state in registers, returns false when the particle is deleted.
 template <typename R>
-__device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X, R& Y, R& VX, R& VY, R& VZ,
+__device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X, R& Y, R& VX, R& VY, R& VZ, R& TH,
                                 bool& vz_changed, PushEvents& ev) {
     const MeshParams& m = a.m;
     R T = a.dt;
@@ -206,7 +209,18 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
             R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
             R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
             R xm = (R)1 - dx, ym = (R)1 - dy;
-            R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
+            R W1, W2, W3, W4;
+            if (m.delta == 0) {
+                W1 = xm * ym; W2 = dx * ym; W3 = dx * dy; W4 = xm * dy;
+            } else {
+                // MoveOne.f90:94-104: R1, R2 = dymin + node radii, no axis factor here
+                const R R1 = (R)m.ymin + y1, R2 = (R)m.ymin + y2, Rp = Y;
+                const R den = R2 * R2 - R1 * R1;
+                W1 = xm * (R2 * R2 - Rp * Rp) / den;
+                W2 = dx * (R2 * R2 - Rp * Rp) / den;
+                W3 = dx * (Rp * Rp - R1 * R1) / den;
+                W4 = xm * (Rp * Rp - R1 * R1) / den;
+            }
             int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
             R Ex, Ey;
             if (m.element_index[n] > 0) {
@@ -232,12 +246,40 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
                 Ex = W1 * a.efx[n1] + W2 * a.efx[n2] + W3 * a.efx[n3] + W4 * a.efx[n4];
                 Ey = W1 * a.efy[n1] + W2 * a.efy[n2] + W3 * a.efy[n3] + W4 * a.efy[n4];
             }
-            VX = VX + a.qm * Ex * T;
-            VY = VY + a.qm * Ey * T;
+            if (m.delta == 0) {
+                VX = VX + a.qm * Ex * T;
+                VY = VY + a.qm * Ey * T;
+            } else {
+                // MoveOne.f90:221-236: E_r turned into the fixed Cartesian frame by the
+                // azimuth; the tangential field Efield(3) = efz is zero in the electrostatic model
+                double sn, cs;
+                pic_sincos((double)TH, &sn, &cs);
+                const R xe = Ey * (R)cs - (R)0 * (R)sn;
+                const R ye = Ey * (R)sn + (R)0 * (R)cs;
+                VX = VX + a.qm * Ex * T;
+                VY = VY + a.qm * xe * T;
+                VZ = VZ + a.qm * ye * T;
+                vz_changed = true;
+            }
             ivel = 0;
         }
-        X = X + T * VX;
-        Y = Y + T * VY;
+        if (m.delta == 0) {
+            X = X + T * VX;
+            Y = Y + T * VY;
+        } else {
+            // MoveOne.f90:246-263: move in Cartesian coordinates (in FP64), back to (r, th)
+            double sn, cs;
+            pic_sincos((double)TH, &sn, &cs);
+            double xc = (double)Y * cs, yc = (double)Y * sn;
+            xc = xc + (double)T * (double)VY;
+            if (xc == 0.0) xc = m.hy * (double)1.0e-5f;      // hx(2)*1.0E-5 (a default-real literal)
+            yc = yc + (double)T * (double)VZ;
+            X = X + T * VX;
+            const double rr = sqrt(xc * xc + yc * yc);
+            double tt = pic_atan(yc / xc);
+            if (xc <= 0.0) tt = tt + 3.14159265358979;         // PI_LY
+            Y = (R)rr; TH = (R)tt;
+        }
         ipos = 0;
         // ---------------- AdjustParticleBoundary (MCCInterface.f90:361-499)
         // Back-up times lie in [0, T] whenever the reference's arithmetic is well posed;
@@ -293,6 +335,10 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
                 } else {
                     T = 0; ipos = 0; fold_y(Y, VY, a.ymin, a.ymax);
                 }
+            } else {
+                // axisymmetric outer radius: the reference has no branch (the particle would
+                // leave the mesh); absorbed like the x > xmax wall
+                X = (R)-2000;
             }
         }
         for (int k = 0; k < m.n_objects; ++k) {
@@ -362,11 +408,13 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& 
 // The particles the push kernel handed over: the whole MoveOne/AdjustParticleBoundary loop,
 // lookup, write-back, and the deposit merged per wave (segmented by element, the list is
 // in slot order within each wave's append, so neighbours mostly share an element) before
-// the global atomics. Grid-stride over the list, every lane of a wave in every round.
+// the global atomics. Grid-stride over the list, every lane of a wave in every round. With
+// all_slots the kernel runs over every slot instead (the axisymmetric mover, whose every
+// particle takes this path; dead slots are skipped).
 template <typename R>
 __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
     const MeshParams& m = a.m;
-    const unsigned int nw = *a.wall_count;
+    const unsigned int nw = a.all_slots ? (unsigned int)a.n : *a.wall_count;
     const DepWindow none = {0, 0, 0, 0};
     const unsigned int lane = threadIdx.x & 63;
     const unsigned int wave0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
@@ -378,14 +426,17 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
         long long v[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = 0;
-        if (i < nw) {
-            const int64_t p = a.wall_list[i];
-            const int e_old = a.cell[p];
+        const int64_t p = i < nw ? (a.all_slots ? (int64_t)i : (int64_t)a.wall_list[i]) : 0;
+        const int e_old = i < nw ? a.cell[p] : -1;
+        if (i < nw && e_old < 0) a.cell_next[p] = -1;       // dead slot (all_slots only)
+        if (e_old >= 0) {
             R X = a.x[p], Y = a.y[p], VX = a.vx[p], VY = a.vy[p], VZ = a.vz[p];
+            R TH = a.th ? a.th[p] : (R)0;
             bool vzc = false;
-            bool live = move_one<R>(a, p, e_old, X, Y, VX, VY, VZ, vzc, ev);
+            bool live = move_one<R>(a, p, e_old, X, Y, VX, VY, VZ, TH, vzc, ev);
             a.x[p] = X; a.y[p] = Y; a.vx[p] = VX; a.vy[p] = VY;
             if (vzc) a.vz[p] = VZ;
+            if (a.th) a.th[p] = TH;
             if (live) {
                 int bad = 0;
                 double Xd = (double)X, Yd = (double)Y;
@@ -566,6 +617,8 @@ static PushArgs<R> make_push_args(int isp) {
     SpeciesHost& S = g.sp[isp];
     PushArgs<R> a;
     a.x = (R*)S.x; a.y = (R*)S.y; a.vx = (R*)S.vx; a.vy = (R*)S.vy; a.vz = (R*)S.vz;
+    a.th = g.mesh.delta == 1 ? (R*)S.th : nullptr;
+    a.all_slots = 0;
     a.cell = S.cell; a.cell_next = S.cell_next; a.n = S.nslots;
     a.m = g.mesh;
     a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
@@ -630,7 +683,20 @@ int launch_push_deposit(int isp, bool do_push) {
         HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
     }
     const int wall_blocks = 512;
-    if (g.particle_bits == 64) {
+    if (do_push && g.mesh.delta == 1) {
+        // axisymmetric: every particle takes the full MoveOne / AdjustParticleBoundary loop
+        if (!S.th) return pic_fail(PIC_ERR_STATE, "axisymmetric species without azimuth storage");
+        const int ab = (int)std::min<int64_t>((S.nslots + 255) / 256, 8192);
+        if (g.particle_bits == 64) {
+            PushArgs<double> a = make_push_args<double>(isp);
+            a.all_slots = 1;
+            hipLaunchKernelGGL((k_push_walls<double>), dim3(ab), dim3(256), 0, g.stream, a);
+        } else {
+            PushArgs<float> a = make_push_args<float>(isp);
+            a.all_slots = 1;
+            hipLaunchKernelGGL((k_push_walls<float>), dim3(ab), dim3(256), 0, g.stream, a);
+        }
+    } else if (g.particle_bits == 64) {
         PushArgs<double> a = make_push_args<double>(isp);
         if (do_push) {
             hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);

@@ -82,7 +82,8 @@ struct SpeciesHost {
     void* x = nullptr; void* y = nullptr;
     void* vx = nullptr; void* vy = nullptr; void* vz = nullptr;
     int* cell = nullptr; int* cell_next = nullptr;
-    void* spare[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // sort targets, swapped with x..vz
+    void* th = nullptr;                 // azimuth PO%Z, axisymmetric domains only (delta = 1)
+    void* spare[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // sort targets, swapped with x..vz, th
     long long* acc_w = nullptr;         // nnodes fixed-point sum W
     long long* acc_e = nullptr;         // nnodes fixed-point sum W*E
     bool acc_dirty = false;
@@ -262,7 +263,7 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
                       std::vector<double>& x, std::vector<double>& y,
                       std::vector<double>& vx, std::vector<double>& vy, std::vector<double>& vz,
-                      std::vector<int>& cell);
+                      std::vector<int>& cell, std::vector<double>* th);
 
 // device launchers
 int launch_push_deposit(int isp, bool do_push);

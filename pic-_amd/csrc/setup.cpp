@@ -470,11 +470,12 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
                       std::vector<double>& x, std::vector<double>& y,
                       std::vector<double>& vx, std::vector<double>& vy, std::vector<double>& vz,
-                      std::vector<int>& cell) {
+                      std::vector<int>& cell, std::vector<double>* th) {
     const MeshParams& m = g.mesh;
     const SpeciesHost& S = g.sp[isp];
     double vth = std::sqrt(temp_eV / g.cfg.T_ref_eV / S.xm);   // normalised thermal speed
     x.clear(); y.clear(); vx.clear(); vy.clear(); vz.clear(); cell.clear();
+    if (th) th->clear();
     size_t cap = (size_t)m.nelem * ppc;
     x.reserve(cap); y.reserve(cap); vx.reserve(cap); vy.reserve(cap); vz.reserve(cap); cell.reserve(cap);
     const uint32_t tag = pic_stream_tag(isp, PIC_RNG_LOAD);
@@ -493,7 +494,9 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
             uint32_t slot = (uint32_t)((uint64_t)e * ppc + k);
             pic_stream s; pic_stream_init(&s, seed, slot, 0u, tag);
             double px = x0 + (x1 - x0) * pic_stream_u01(&s);
-            double py = y0 + (y1 - y0) * pic_stream_u01(&s);
+            double uy = pic_stream_u01(&s);
+            // axisymmetric: uniform in r^2 over the cell (ParticleOne.f90:97-101)
+            double py = m.delta == 1 ? std::sqrt(y0 * y0 + (y1 + y0) * (y1 - y0) * uy) : y0 + (y1 - y0) * uy;
             double u1 = pic_stream_u01(&s), u2 = pic_stream_u01(&s);
             double u3 = pic_stream_u01(&s), u4 = pic_stream_u01(&s);
             double r1 = std::sqrt(-2.0 * std::log(u1)), r2 = std::sqrt(-2.0 * std::log(u3));
@@ -502,6 +505,7 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
             for (int o = 0; o < g.cfg.n_objects; ++o) inside = inside || inside_object(g.cfg.objects[o], px, py);
             if (inside) continue;
             x.push_back(px); y.push_back(py);
+            if (th) th->push_back(2.0 * PIC_PI * pic_stream_u01(&s));
             vx.push_back(vth * r1 * std::cos(a)); vy.push_back(vth * r1 * std::sin(a)); vz.push_back(vth * r2 * std::cos(b));
             cell.push_back(e);
         }

@@ -96,6 +96,19 @@ module pic_hip_mod
             integer(c_int64_t), value :: capacity
             integer(c_int64_t) :: npar
         end function
+        integer(c_int) function pic_set_azimuth(isp, n, th) bind(c, name="pic_set_azimuth")
+            import :: c_int, c_int64_t, c_double
+            integer(c_int), value :: isp
+            integer(c_int64_t), value :: n
+            real(c_double) :: th(*)
+        end function
+        integer(c_int) function pic_get_azimuth(isp, th, capacity, n) bind(c, name="pic_get_azimuth")
+            import :: c_int, c_int64_t, c_double
+            integer(c_int), value :: isp
+            real(c_double) :: th(*)
+            integer(c_int64_t), value :: capacity
+            integer(c_int64_t) :: n
+        end function
         integer(c_int) function pic_get_counts(isp, nlive, nslots, counters) bind(c, name="pic_get_counts")
             import :: c_int, c_int64_t
             integer(c_int), value :: isp

@@ -111,12 +111,74 @@ def test_moveall_bit_exact(gpu_lib, pkg, orc, bits):
         say(f"  species {isp}: {keep.sum()} live, emitted {cnt[0]}, absorbed {cnt[1] + cnt[2]}, reflected {cnt[3]}")
 
 
-# ------------------------------------------------------------------ deposition
 @pytest.mark.parametrize("bits", [64, 32])
-def test_getparchg_moments(gpu_lib, pkg, orc, bits):
+def test_moveall_axisymmetric_bit_exact(gpu_lib, pkg, orc, bits):
+    """The r-z mover (MoveOne.f90:94-104, :221-236, :246-263; x axial, y radial, velocities
+    in a fixed Cartesian frame, the azimuth PO%Z moved with them) against the oracle, bit for
+    bit, azimuth included, with the field strong enough to reach the axial walls, the outer
+    radius (absorbing in the build; the reference has no branch there) and the conductor. The
+    particles are reloaded with their azimuths through load_particles(th=...)."""
     pl = gpu_lib
-    say(f"case: GetParChg moments bits={bits}")
-    w = pkg.build_workload(pl, "small24", seed=21, ppc=16, mcc=False)
+    say(f"case: MoveAll axisymmetric bits={bits}")
+    w = pkg.workload("rz24")
+    dom = w["domain"]
+    pl.setup(dom, bits=bits, solver=0, tol=1e-12)
+    weight = pkg.species_weight(dom, 16)
+    nel = (dom.nnx - 1) * (dom.nny - 1)
+    species = [(pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], 2.0),
+               (pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], 0.026)]
+    for isp, (chg, mass, T) in enumerate(species):
+        pl.set_species(isp, chg, mass, weight, T * pkg.EVTOK, nel * 32)
+        pl.load_maxwellian(isp, 16, T, 5 + isp)
+        p, th = pl.particles(isp), pl.azimuth(isp)
+        assert len(th) == len(p["x"]) and np.all((th >= 0) & (th < 2 * np.pi))
+        pl.load_particles(isp, p["x"], p["y"], p["vx"], p["vy"], p["vz"], loc=p["loc"], th=th)
+        assert np.array_equal(pl.azimuth(isp), th)
+    pl.set_run_params(0, 77, 0)
+    pl.set_step(3)
+    pl.initial_field()
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    coef = pl.field("COEF_PPR").reshape(-1, 6)
+    efx, efy = pl.field("EFX") * 20.0, pl.field("EFY") * 20.0
+    pl.set_field("EFX", efx)
+    pl.set_field("EFY", efy)
+    coef[:, 1:] *= 20.0
+    pl.set_field("COEF_PPR", coef.ravel())
+    norm = dom.normalisation()
+    ft = np.float64 if bits == 64 else np.float32
+    for isp, (chg, mass, T) in enumerate(species):
+        p0, th0 = pl.particles(isp), pl.azimuth(isp)
+        qm = (chg / pkg.ELECTRON_CHARGE) / (mass / pkg.ELECTRON_MASS)
+        x, y, vx, vy, vz, alive, cnt, err, th = orc.move_all(
+            bits, xs, ys, ei, ni, dom.object_array(), efx.astype(ft), efy.astype(ft), coef[:, 1:].astype(ft).ravel(),
+            qm, dom.dt, norm["v_ref"], mass, T * pkg.EVTOK, 77, 3, orc.tag(isp, orc.RNG_EMIT),
+            p0["x"], p0["y"], p0["vx"], p0["vy"], p0["vz"], p0["loc"] - 1, delta=1, th=th0)
+        assert err == 0
+        pl.MoveAll(isp)
+        pl.GetParChg_2D()
+        p1, th1 = pl.particles(isp), pl.azimuth(isp)
+        keep = alive.astype(bool)
+        assert len(p1["x"]) == keep.sum()
+        for k, a in zip(("x", "y", "vx", "vy", "vz"), (x, y, vx, vy, vz)):
+            assert np.array_equal(p1[k], a[keep].astype(np.float64)), k
+        assert np.array_equal(th1, th[keep].astype(np.float64))
+        c = pl.counts(isp)
+        assert (c["emit"], c["right"], c["object"], c["reflect"], c["deleted"]) == \
+               tuple(int(v) for v in (cnt[0], cnt[1], cnt[2], cnt[3], cnt[7]))
+        loc, bad = orc.locate(xs, ys, x[keep].astype(np.float64), y[keep].astype(np.float64))
+        assert bad == 0 and np.array_equal(p1["loc"] - 1, loc)
+        moved = np.abs(th[keep] - th0[keep]) > 0
+        say(f"  species {isp}: {keep.sum()} live, emitted {cnt[0]}, absorbed {cnt[1] + cnt[2]}, "
+            f"azimuth changed for {moved.sum()}")
+        assert moved.any()
+
+
+# ------------------------------------------------------------------ deposition
+@pytest.mark.parametrize("bits,case", [(64, "small24"), (32, "small24"), (64, "rz24"), (32, "rz24")])
+def test_getparchg_moments(gpu_lib, pkg, orc, bits, case):
+    pl = gpu_lib
+    say(f"case: GetParChg moments bits={bits} {case}")
+    w = pkg.build_workload(pl, case, seed=21, ppc=16, mcc=False)
     dom = w["domain"]
     pl.setup(dom, bits=bits, solver=0, tol=1e-12)
     weight = pkg.species_weight(dom, 16)
@@ -138,16 +200,31 @@ def test_getparchg_moments(gpu_lib, pkg, orc, bits):
         loc, bad = orc.locate(xs, ys, p["x"], p["y"])
         assert bad == 0 and np.array_equal(loc, p["loc"] - 1)
         ft = np.float64 if bits == 64 else np.float32
-        args = (bits, xs, ys, 0, p["x"].astype(ft), p["y"].astype(ft), p["vx"].astype(ft), p["vy"].astype(ft),
+        args = (bits, xs, ys, dom.delta, p["x"].astype(ft), p["y"].astype(ft), p["vx"].astype(ft), p["vy"].astype(ft),
                 p["vz"].astype(ft), loc, mass, norm["v_ref"])
         wq, eq = orc.deposit(*args, fix=True)
         wf, ef = orc.deposit(*args, fix=False)
         assert np.array_equal(rho_p[isp * nn:(isp + 1) * nn], wq)            # bit-exact
-        np.testing.assert_allclose(rho_p[isp * nn:(isp + 1) * nn], wf, rtol=1e-12, atol=1e-12)   # vs FP64 sums
-        rs, e = orc.normalize(0, weight, chg / pkg.ELECTRON_CHARGE, norm["L_ref"], dom.n_ref, vol, wq, eq, rho_or)
+        # vs the FP64 sums: each fixed-point quantum is within 2^-41 of its term, and a node sums at
+        # most 4 x (particles of its four elements) terms
+        nterm = 4 * np.bincount(loc, minlength=(dom.nnx - 1) * (dom.nny - 1)).max()
+        np.testing.assert_allclose(rho_p[isp * nn:(isp + 1) * nn], wf, rtol=0, atol=nterm * 2.0 ** -41)
+        rs, e = orc.normalize(dom.delta, weight, chg / pkg.ELECTRON_CHARGE, norm["L_ref"], dom.n_ref, vol, wq, eq,
+                              rho_or)
         assert np.array_equal(rho_s[isp * nn:(isp + 1) * nn], rs)
         assert np.array_equal(ek[isp * nn:(isp + 1) * nn], e)
-        assert wq.sum() == pytest.approx(len(p["x"]), rel=1e-12)
+        if dom.delta == 0:
+            assert wq.sum() == pytest.approx(len(p["x"]), rel=1e-12)
+        else:
+            # the axis row's lower-node weights carry beta = 0.75 (GetParChg_2D.f90:803-821):
+            # a particle's weights sum to beta (R2^2 - R^2)/den + (R^2 - R1^2)/den there, 1 elsewhere
+            cj = loc % (dom.nny - 1)
+            r1, r2 = dom.ymin + ys[cj], dom.ymin + ys[cj + 1]
+            beta = np.where(np.abs(ys[cj] - dom.ymin) < 1e-12, 0.75, 1.0)
+            den = r2 * r2 - r1 * r1
+            expect = beta * (r2 * r2 - p["y"] ** 2) / den + (p["y"] ** 2 - r1 * r1) / den
+            assert (beta < 1).any()
+            assert wq.sum() == pytest.approx(expect.sum(), rel=1e-12)
     assert np.array_equal(pl.field("RHO"), rho_or)
 
 
@@ -307,21 +384,32 @@ def test_positioning_dropins(gpu_lib, pkg, orc):
 
 
 # ------------------------------------------------------------------ full step determinism
-def test_full_step_deterministic_and_conserving(gpu_lib, pkg):
+@pytest.mark.parametrize("case", ["small32", "rz32"])
+def test_full_step_deterministic_and_conserving(gpu_lib, pkg, case):
+    """Seven full steps (mover, deposit, solve, PPR, MCC with ionisation products, a cell sort
+    every 3 steps) twice from the same state: identical bits, the live count consistent; on
+    the axisymmetric domain the azimuths travel with their particles through the sort and the
+    MCC products."""
     pl = gpu_lib
-    say("case: full step determinism small32")
+    say(f"case: full step determinism {case}")
     outs = []
     for rep in range(2):
-        w = pkg.build_workload(pl, "small32", seed=41, ppc=16, mcc=True, gas_density=3.2e24)
+        w = pkg.build_workload(pl, case, seed=41, ppc=16, mcc=True, gas_density=3.2e24)
         pl.set_run_params(3, 41, 1)
         pl.set_step(0)
         pl.initial_field()
         pl.step(7)
-        outs.append((pl.field("PHI"), pl.field("RHO"), pl.particles(0)["x"], pl.particles(1)["vx"]))
+        o = [pl.field("PHI"), pl.field("RHO"), pl.particles(0)["x"], pl.particles(1)["vx"]]
+        if w["domain"].delta == 1:
+            o += [pl.azimuth(0), pl.azimuth(1)]
+            assert len(o[-2]) == len(o[2]) and np.all(np.isfinite(o[-2]))
+        outs.append(o)
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
     c0 = pl.counts(0)
     assert c0["nlive"] == len(outs[1][2])
+    if case.startswith("rz"):
+        assert c0["mcc_created"] > 0
 
 
 # ------------------------------------------------------------------ wall injection

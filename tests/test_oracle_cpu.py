@@ -577,3 +577,100 @@ def test_threaded_oracle_matches_serial(orc):
     np.testing.assert_allclose(got[5][0], ref[5][0], rtol=1e-9, atol=1e-12)
     assert abs(got[5][2] - ref[5][2]) <= 1
     np.testing.assert_allclose(got[6], ref[6], rtol=1e-13, atol=1e-14)
+
+
+# ------------------------------------------------------------------ axisymmetric mover
+def test_trig_against_libm(orc):
+    """The mover's own sin / cos / atan (one IEEE sequence shared by the device and the
+    oracle) stay within 1 ulp (sin, cos) and 2 ulp (atan) of the C library, with the exact
+    special values and symmetries."""
+    import math
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-8.0, 8.0, 4000), rng.uniform(-1e-6, 1e-6, 200), np.arange(-12, 13) * np.pi / 4])
+    for x in xs:
+        s, c = orc.sincos(x)
+        for got, ref in ((s, math.sin(x)), (c, math.cos(x))):
+            assert abs(got - ref) <= math.ulp(ref), (x, got, ref)
+    ts = np.concatenate([rng.uniform(-3.0, 3.0, 4000), 10.0 ** rng.uniform(-12, 12, 4000), [0.41421356237309503]])
+    for t in ts:
+        for tt in (t, -t):
+            assert abs(orc.atan(tt) - math.atan(tt)) <= 2 * math.ulp(math.atan(tt)), tt
+        assert orc.atan(-t) == -orc.atan(t)
+    assert orc.sincos(0.0) == (0.0, 1.0) and orc.atan(0.0) == 0.0
+    assert orc.atan(1.0) == math.pi / 4 and orc.atan(math.inf) == math.pi / 2
+    assert math.isnan(orc.atan(math.nan))
+
+
+def _rz_case(orc, n_part=400, seed=5):
+    n = 16
+    xs = np.arange(n + 1, dtype=np.float64)            # axial 0..16
+    ys = np.arange(n + 1, dtype=np.float64)            # radial 0..16
+    from patch_ref import element_and_node_classes
+    ni, ei = element_and_node_classes(n + 1, n + 1, xs, ys, ())
+    rng = np.random.default_rng(seed)
+    x = 4.0 + 8.0 * rng.random(n_part)
+    y = 3.0 + 8.0 * rng.random(n_part)
+    th = rng.uniform(-1.5, 4.6, n_part)
+    v = rng.normal(size=(3, n_part)) * 0.6
+    cell, bad = orc.locate(xs, ys, x, y)
+    assert bad == 0
+    return xs, ys, ni, ei, x, y, th, v, cell
+
+
+def test_axisymmetric_push_field_free_is_a_straight_line(orc):
+    """With no field the r-z mover moves the particle on a straight line of 3-D space:
+    z += T vx, (r cos th, r sin th) += T (vy, vz), then back to (r, th) with th = atan(Y/X)
+    (+ PI_LY when X <= 0), MoveOne.f90:246-263."""
+    xs, ys, ni, ei, x, y, th, v, cell = _rz_case(orc)
+    nn = len(xs) * len(ys)
+    z = np.zeros(nn)
+    dt = 0.25
+    X, Y, VX, VY, VZ, alive, cnt, err, TH = orc.move_all(64, xs, ys, ei, ni, np.zeros((0, 6)), z, z, np.zeros(5 * nn),
+                                                         -1.0, dt, 4e5, 9.1e-31, 2 * 11605.0, 1, 0, 0, x, y, *v,
+                                                         cell, delta=1, th=th)
+    assert err == 0 and alive.all()
+    np.testing.assert_array_equal(VX, v[0]); np.testing.assert_array_equal(VY, v[1]); np.testing.assert_array_equal(VZ, v[2])
+    cx = y * np.cos(th) + dt * v[1]
+    cy = y * np.sin(th) + dt * v[2]
+    np.testing.assert_allclose(X, x + dt * v[0], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(Y, np.hypot(cx, cy), rtol=1e-14)
+    expect_th = np.arctan(cy / cx) + np.where(cx <= 0, 3.14159265358979, 0.0)
+    np.testing.assert_allclose(TH, expect_th, rtol=0, atol=1e-13)
+
+
+def test_axisymmetric_push_uniform_fields_and_angular_momentum(orc):
+    """Uniform E_z (efx) and E_r (efy): vx += qm E_z T, (vy, vz) += qm E_r T (cos th, sin th)
+    (MoveOne.f90:221-236); a radial kick leaves the angular momentum about the axis
+    x vz - y vy (Cartesian frame, before the move) unchanged."""
+    xs, ys, ni, ei, x, y, th, v, cell = _rz_case(orc, seed=8)
+    nn = len(xs) * len(ys)
+    Ez, Er, qm, dt = 0.3, -0.7, -1.0, 0.05
+    out = orc.move_all(64, xs, ys, ei, ni, np.zeros((0, 6)), np.full(nn, Ez), np.full(nn, Er), np.zeros(5 * nn), qm,
+                       dt, 4e5, 9.1e-31, 2 * 11605.0, 1, 0, 0, x, y, *v, cell, delta=1, th=th)
+    X, Y, VX, VY, VZ, alive, cnt, err, TH = out
+    assert err == 0 and alive.all()
+    np.testing.assert_allclose(VX, v[0] + qm * Ez * dt, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(VY, v[1] + qm * Er * dt * np.cos(th), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(VZ, v[2] + qm * Er * dt * np.sin(th), rtol=0, atol=1e-15)
+    px, py = y * np.cos(th), y * np.sin(th)
+    np.testing.assert_allclose(px * VZ - py * VY, px * v[2] - py * v[1], rtol=1e-13, atol=1e-13)
+    # (r, th) after the move is the kicked straight line in the Cartesian frame
+    cx, cy = px + dt * VY, py + dt * VZ
+    np.testing.assert_allclose(Y * np.cos(TH), cx, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Y * np.sin(TH), cy, rtol=0, atol=1e-12)
+
+
+def test_axisymmetric_outer_radius_absorbs(orc):
+    """delta = 1: a particle carried past r = ymax is deleted (the build's outer-radius wall;
+    the reference has no branch there), not reflected, and counts only as a deletion."""
+    xs, ys, ni, ei, x, y, th, v, cell = _rz_case(orc, n_part=3)
+    nn = len(xs) * len(ys)
+    z = np.zeros(nn)
+    x = np.array([8.0, 8.0, 8.0]); y = np.array([15.9, 15.9, 5.0]); th = np.array([0.0, 1.0, 2.0])
+    vx = np.zeros(3); vy = np.array([4.0, 4.0 * np.cos(1.0), 0.1]); vz = np.array([0.0, 4.0 * np.sin(1.0), 0.0])
+    cell, _ = orc.locate(xs, ys, x, y)
+    out = orc.move_all(64, xs, ys, ei, ni, np.zeros((0, 6)), z, z, np.zeros(5 * nn), -1.0, 0.05, 4e5, 9.1e-31,
+                       2 * 11605.0, 1, 0, 0, x, y, vx, vy, vz, cell, delta=1, th=th)
+    X, Y, VX, VY, VZ, alive, cnt, err, TH = out
+    assert err == 0 and alive.tolist() == [0, 0, 1]
+    assert cnt[7] == 2 and cnt[1] == 0 and cnt[3] == 0
