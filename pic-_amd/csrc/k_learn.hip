@@ -513,10 +513,15 @@ __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, cons
 
 // Levels lc .. L-1 (Jacobi smoothers, all <= 33 x 33 nodes) in ONE workgroup per batch entry:
 // the down sweeps, the exact coarsest solve and the up sweeps of the V-cycle, phase by phase
-// with block barriers. The operators of these levels (nine diagonals, 1/diag, and the dense
-// coarsest inverse when it fits) are packed on the host in the LDS order and copied in with
-// a few float4 loads per thread at the start, so after that first wait every phase works
-// from LDS. Replaces ~2 launches per level whose run time is pure latency at these sizes.
+// with block barriers. The operators of these levels (nine diagonals and 1/diag, node order,
+// and the dense coarsest inverse when it fits) are packed on the host in the LDS order and
+// copied in with a few float4 loads per thread at the start, so after that first wait every
+// phase works from LDS. The vectors r, x, y, t of each level are stored with a one-node zero
+// halo ((H + 2) x (W + 2), zeroed once), so the stencils, restriction and interpolation read
+// without bounds tests; a node's row is (f + 1/2) * RN(1/W) truncated (exact for these sizes)
+// instead of an integer division. One CU's issue rate bounds these phases, so every
+// instruction per node counts; the arithmetic uses FMA (the solve is not bit-compared).
+// Replaces ~2 launches per level whose run time is pure latency at these sizes.
 #define MG_CB_THREADS 1024
 #define MG_CB_MAXN 1089
 #define MG_CB_OPS_IT 6                               // float4 loads per thread for the operators
@@ -526,8 +531,10 @@ static_assert(MG_CB_RIT * MG_CB_THREADS >= LN_MAX_COARSE, "coarse workgroup stag
 struct MgCoarseArgs {
     int lc, L;
     int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
-    int voff[LN_MAX_LEVELS];                     // LDS offset (floats) of each level's vectors r, x, y, t
+    float rw[LN_MAX_LEVELS];                     // RN(1 / W)
+    int voff[LN_MAX_LEVELS];                     // LDS offset (floats) of each level's padded vectors r, x, y, t
     int ooff[LN_MAX_LEVELS];                     // LDS offset of each level's dia [9][n] then dinv [n]
+    int vtot;                                    // floats of all padded vectors (zeroed at the start)
     int ioff;                                    // LDS offset of the coarsest inverse, -1: read from inv
     int nops4;                                   // packed operator float4s (<= MG_CB_OPS_IT * threads)
     const float4* ops;
@@ -536,6 +543,8 @@ struct MgCoarseArgs {
     const float* inv; float omega;
     const double* done;
 };
+
+__device__ inline int mgcb_row(int f, float rw) { return (int)(((float)f + 0.5f) * rw); }
 
 __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs a) {
     extern __shared__ float lds[];
@@ -546,7 +555,8 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
         const int k = threadIdx.x + q * MG_CB_THREADS;
         ov[q] = a.ops[k < a.nops4 ? k : 0];
     }
-    const int n0 = a.H[a.lc] * a.W[a.lc];
+    const int H0 = a.H[a.lc], W0 = a.W[a.lc];
+    const int n0 = H0 * W0;
     float rv[MG_CB_RIT];
 #pragma unroll
     for (int q = 0; q < MG_CB_RIT; ++q) {
@@ -559,116 +569,114 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
         const int k = threadIdx.x + q * MG_CB_THREADS;
         if (k < a.nops4) reinterpret_cast<float4*>(lds)[k] = ov[q];
     }
+    float* vbase = lds + a.voff[a.lc];
+    for (int k = threadIdx.x; k < a.vtot; k += MG_CB_THREADS) vbase[k] = 0.0f;
+    __syncthreads();
+    {
+        float* r = lds + a.voff[a.lc];
+        const int Wp = W0 + 2;
 #pragma unroll
-    for (int q = 0; q < MG_CB_RIT; ++q) {
-        const int k = threadIdx.x + q * MG_CB_THREADS;
-        if (k < n0) lds[a.voff[a.lc] + k] = rv[q];
+        for (int q = 0; q < MG_CB_RIT; ++q) {
+            const int k = threadIdx.x + q * MG_CB_THREADS;
+            if (k < n0) {
+                const int fi = mgcb_row(k, a.rw[a.lc]), fj = k - fi * W0;
+                r[(fi + 1) * Wp + fj + 1] = rv[q];
+            }
+        }
     }
     __syncthreads();
     for (int l = a.lc; l < a.L - 1; ++l) {
         const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
         const int nf = Hf * Wf, nc = Hc * Wc;
+        const int Wp = Wf + 2, Pf = (Hf + 2) * Wp, Wpc = Wc + 2;
+        const float rwf = a.rw[l], rwc = a.rw[l + 1];
         float* r = lds + a.voff[l];
-        float* x = r + nf;
-        float* s = r + 3 * nf;
+        float* x = r + Pf;
+        float* s = r + 3 * Pf;
         float* rc = lds + a.voff[l + 1];
         const float* dia = lds + a.ooff[l];
         const float* dinv = dia + 9 * nf;
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) x[f] = a.omega * dinv[f] * r[f];
+        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
+            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
+            const int p = (fi + 1) * Wp + fj + 1;
+            x[p] = a.omega * dinv[f] * r[p];
+        }
         __syncthreads();
         for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int fi = f / Wf, fj = f - (f / Wf) * Wf;
-            float v = 0.0f;
-            if (dia[4 * nf + f] != 0.0f) {
-                v = r[f];
+            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
+            const int p = (fi + 1) * Wp + fj + 1;
+            float v = r[p];
 #pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
-                    if (gi >= 0 && gi < Hf && gj >= 0 && gj < Wf) v = v - dia[t * nf + f] * x[gi * Wf + gj];
-                }
-            }
-            s[f] = v;
+            for (int t = 0; t < 9; ++t) v = fmaf(-dia[t * nf + f], x[p + (t / 3 - 1) * Wp + (t % 3 - 1)], v);
+            s[p] = dia[4 * nf + f] != 0.0f ? v : 0.0f;
         }
         __syncthreads();
         for (int c = threadIdx.x; c < nc; c += MG_CB_THREADS) {
-            const int I = c / Wc, J = c - (c / Wc) * Wc;
-            float acc = 0.0f;
-#pragma unroll
-            for (int u = -1; u <= 1; ++u)
-#pragma unroll
-                for (int v = -1; v <= 1; ++v) {
-                    const int fi = 2 * I + u, fj = 2 * J + v;
-                    if (fi >= 0 && fi < Hf && fj >= 0 && fj < Wf)
-                        acc = acc + (u == 0 ? 1.0f : 0.5f) * (v == 0 ? 1.0f : 0.5f) * s[fi * Wf + fj];
-                }
-            rc[c] = acc;
+            const int I = mgcb_row(c, rwc), J = c - I * Wc;
+            const float* s0 = s + (2 * I) * Wp + 2 * J;          // fine node (2I - 1, 2J - 1), padded
+            const float acc = 0.25f * (s0[0] + s0[2] + s0[2 * Wp] + s0[2 * Wp + 2]) +
+                              0.5f * (s0[1] + s0[Wp] + s0[Wp + 2] + s0[2 * Wp + 1]) + s0[Wp + 1];
+            rc[(I + 1) * Wpc + J + 1] = acc;
         }
         __syncthreads();
     }
     {
         const int l = a.L - 1;
-        const int n = a.H[l] * a.W[l];
+        const int Wl = a.W[l], n = a.H[l] * Wl, Wp = Wl + 2, P = (a.H[l] + 2) * Wp;
         const float* r = lds + a.voff[l];
-        float* y = lds + a.voff[l] + 2 * n;
+        float* y = lds + a.voff[l] + 2 * P;
         const float* inv = a.ioff >= 0 ? lds + a.ioff : a.inv;
         for (int i = threadIdx.x; i < n; i += MG_CB_THREADS) {
             float sacc = 0.0f;
-            for (int j = 0; j < n; ++j) sacc = sacc + inv[i * n + j] * r[j];
-            y[i] = sacc;
+            for (int j = 0; j < n; ++j) {
+                const int jr = mgcb_row(j, a.rw[l]);
+                sacc = fmaf(inv[i * n + j], r[(jr + 1) * Wp + (j - jr * Wl) + 1], sacc);
+            }
+            const int ir = mgcb_row(i, a.rw[l]);
+            y[(ir + 1) * Wp + (i - ir * Wl) + 1] = sacc;
         }
         __syncthreads();
     }
     for (int l = a.L - 2; l >= a.lc; --l) {
         const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
         const int nf = Hf * Wf;
+        const int Wp = Wf + 2, Pf = (Hf + 2) * Wp, Wpc = Wc + 2, Pc = (Hc + 2) * Wpc;
+        const float rwf = a.rw[l];
         const float* r = lds + a.voff[l];
-        const float* x = r + nf;
-        float* y = lds + a.voff[l] + 2 * nf;
-        float* x2 = lds + a.voff[l] + 3 * nf;
-        const float* yc = lds + a.voff[l + 1] + 2 * Hc * Wc;
+        const float* x = r + Pf;
+        float* y = lds + a.voff[l] + 2 * Pf;
+        float* x2 = lds + a.voff[l] + 3 * Pf;
+        const float* yc = lds + a.voff[l + 1] + 2 * Pc;
         const float* dia = lds + a.ooff[l];
         const float* dinv = dia + 9 * nf;
         for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int gi = f / Wf, gj = f - (f / Wf) * Wf;
-            float v = 0.0f;
-            if (dia[4 * nf + f] != 0.0f) {
-                float pe = 0.0f;
-                const int i0 = gi >> 1, j0 = gj >> 1;
-                const bool oi = gi & 1, oj = gj & 1;
-                for (int u = 0; u < (oi ? 2 : 1); ++u) {
-                    const int I = i0 + u;
-                    if (I >= Hc) continue;
-                    for (int w = 0; w < (oj ? 2 : 1); ++w) {
-                        const int J = j0 + w;
-                        if (J >= Wc) continue;
-                        pe = pe + (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * yc[I * Wc + J];
-                    }
-                }
-                v = x[f] + pe;
-            }
-            x2[f] = v;
+            const int gi = mgcb_row(f, rwf), gj = f - gi * Wf;
+            const int p = (gi + 1) * Wp + gj + 1;
+            // bilinear interpolation: weights (1, 0) on an even index, (1/2, 1/2) on an odd one
+            const int i0 = gi >> 1, j0 = gj >> 1;
+            const float ai = (gi & 1) ? 0.5f : 1.0f, bi = (gi & 1) ? 0.5f : 0.0f;
+            const float aj = (gj & 1) ? 0.5f : 1.0f, bj = (gj & 1) ? 0.5f : 0.0f;
+            const float* e0 = yc + (i0 + 1) * Wpc + j0 + 1;
+            const float pe = ai * fmaf(aj, e0[0], bj * e0[1]) + bi * fmaf(aj, e0[Wpc], bj * e0[Wpc + 1]);
+            x2[p] = dia[4 * nf + f] != 0.0f ? x[p] + pe : 0.0f;
         }
         __syncthreads();
         for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int fi = f / Wf, fj = f - (f / Wf) * Wf;
-            float v = 0.0f;
-            if (dia[4 * nf + f] != 0.0f) {
-                float sres = r[f];
+            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
+            const int p = (fi + 1) * Wp + fj + 1;
+            float sres = r[p];
 #pragma unroll
-                for (int t = 0; t < 9; ++t) {
-                    const int gi = fi + t / 3 - 1, gj = fj + t % 3 - 1;
-                    if (gi >= 0 && gi < Hf && gj >= 0 && gj < Wf) sres = sres - dia[t * nf + f] * x2[gi * Wf + gj];
-                }
-                v = x2[f] + a.omega * dinv[f] * sres;
-            }
-            y[f] = v;
+            for (int t = 0; t < 9; ++t) sres = fmaf(-dia[t * nf + f], x2[p + (t / 3 - 1) * Wp + (t % 3 - 1)], sres);
+            y[p] = dia[4 * nf + f] != 0.0f ? fmaf(a.omega * dinv[f], sres, x2[p]) : 0.0f;
         }
         __syncthreads();
     }
     {
-        const int n = a.H[a.lc] * a.W[a.lc];
-        const float* y = lds + a.voff[a.lc] + 2 * n;
-        for (int f = threadIdx.x; f < n; f += MG_CB_THREADS) a.yout[(int64_t)b * a.yout_bs + f] = y[f];
+        const float* y = lds + a.voff[a.lc] + 2 * (H0 + 2) * (W0 + 2);
+        for (int f = threadIdx.x; f < n0; f += MG_CB_THREADS) {
+            const int fi = mgcb_row(f, a.rw[a.lc]), fj = f - fi * W0;
+            a.yout[(int64_t)b * a.yout_bs + f] = y[(fi + 1) * (W0 + 2) + fj + 1];
+        }
     }
 }
 
@@ -1228,7 +1236,7 @@ struct LearnNet {
     int C[LN_MAX_LEARNED] = {0};
     float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator
     float* d_cops = nullptr; int cops_n = 0;         // coarse workgroup operators in its LDS order
-    int c_ooff[LN_MAX_LEVELS] = {0}, c_voff[LN_MAX_LEVELS] = {0}, c_ioff = -1;
+    int c_ooff[LN_MAX_LEVELS] = {0}, c_voff[LN_MAX_LEVELS] = {0}, c_ioff = -1, c_vtot = 0;
     void* d_ops = nullptr;                           // one allocation: dia, dinv, dh of all levels
     // parameters (FP32, flat): per learned level W_in [9][C][16] then Q [C][C]
     int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
@@ -1348,8 +1356,9 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
         for (int l = lc; l < L; ++l) {
             const MgLevel& v = ln.lv[l];
             c.H[l] = v.H; c.W[l] = v.W; c.voff[l] = ln.c_voff[l]; c.ooff[l] = ln.c_ooff[l];
+            c.rw[l] = 1.0f / (float)v.W;
         }
-        c.ioff = ln.c_ioff; c.nops4 = ln.cops_n / 4; c.ops = (const float4*)ln.d_cops;
+        c.ioff = ln.c_ioff; c.nops4 = ln.cops_n / 4; c.ops = (const float4*)ln.d_cops; c.vtot = ln.c_vtot;
         c.rin = ln.lv[lc].r[run]; c.rin_bs = ln.lv[lc].n;
         c.yout = ln.lv[lc].y[run]; c.yout_bs = ln.lv[lc].n;
         c.inv = ln.d_inv; c.omega = LN_OMEGA; c.done = done;
@@ -1606,7 +1615,11 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
         if (cp.size() > (size_t)MG_CB_OPS_CAP) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse operators exceed the workgroup staging");
         ln.cops_n = (int)cp.size();
         size_t off = cp.size();
-        for (int l = ln.lc; l < L; ++l) { ln.c_voff[l] = (int)off; off += 4 * (size_t)ln.lv[l].n; }
+        for (int l = ln.lc; l < L; ++l) {
+            ln.c_voff[l] = (int)off;
+            off += 4 * (size_t)(ln.lv[l].H + 2) * (size_t)(ln.lv[l].W + 2);   // padded r, x, y, t
+        }
+        ln.c_vtot = (int)(off - cp.size());
         ln.lc_lds = off * sizeof(float);
         int maxlds = 0;
         HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, g.device));
