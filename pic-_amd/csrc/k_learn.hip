@@ -162,6 +162,73 @@ __global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
 }
 
 // ---------------------------------------------------------------- multigrid transfer kernels
+// ---------------------------------------------------------------- FCG(1) on the scaled system
+// scal slots (shared with the CG): NORMRHS, DONE, ITER, IERR, TOL2, MAXIT; pq in GAM0/GAM1
+enum { F_S_NORMRHS = 4, F_S_NORMR = 5, F_S_DONE = 6, F_S_ITER = 7, F_S_IERR = 8, F_S_TOL2 = 9, F_S_MAXIT = 10,
+       F_S_PQ0 = 12, F_S_PQ1 = 13, F_S_FAIL = 22 };
+// The iteration index k of a pass is a launch argument of the first V-cycle kernel, which
+// runs the convergence test (FcgCheck), and of the update. A refinement pass leaves a
+// record at FCG_REC_BASE + FCG_RECW * slot: iterations, done, error code, last ||r||^2/||r0||^2,
+// skipped (the outer residual met its tolerance before the pass, or an earlier pass failed).
+// FAIL is set by a pass that ends in a NaN or a breakdown.
+#define FCG_BLOCK 256
+
+__device__ inline void fcg_block_partials(int nv, double* v, double* part, int pstride) {
+    __shared__ double sh[4][FCG_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < nv; ++k) {
+        double s = v[k];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) sh[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < nv; ++k) part[(size_t)k * pstride + blockIdx.x] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
+}
+__device__ inline void fcg_reduce(int nv, const double* part, int pstride, int nb, double* out) {
+    __shared__ double sh[4][FCG_BLOCK / 64];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int q = threadIdx.x; q < nb; q += FCG_BLOCK)
+        for (int k = 0; k < nv; ++k) s[k] += part[(size_t)k * pstride + q];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int k = 0; k < nv; ++k) {
+        for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
+        if (lane == 0) sh[k][wid] = s[k];
+    }
+    __syncthreads();
+    for (int k = 0; k < nv; ++k) out[k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
+    __syncthreads();
+}
+
+
+// Convergence test of the current r before FCG iteration k, run by every block of the first
+// V-cycle kernel in its prologue (the partial sums of r.r from the previous update or the
+// pass start are reduced in the same fixed order by every block, so all blocks take the same
+// decision; block 0 publishes it): done when ||r||^2 / ||r_0||^2 <= tol^2 (k = 0 sets
+// ||r_0||^2), at k >= maxit or on a NaN. The later kernels of the V-cycle and the FCG vector
+// kernels read DONE. part == nullptr: no test (training and evaluation passes).
+struct FcgCheck { const double* part; int pstride, nb, k; double* scal; };
+
+__device__ inline bool fcg_check_block(const FcgCheck& c) {
+    const double pre = c.scal[F_S_DONE];
+    double rr;
+    fcg_reduce(1, c.part, c.pstride, c.nb, &rr);
+    if (pre != 0.0) return true;
+    const double nb2 = c.k == 0 ? rr : c.scal[F_S_NORMRHS];
+    const bool conv = nb2 == 0.0 || rr / nb2 <= c.scal[F_S_TOL2];
+    const bool stop = conv || (double)c.k >= c.scal[F_S_MAXIT] || !(rr == rr);
+    if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) {
+        if (c.k == 0) c.scal[F_S_NORMRHS] = rr;
+        c.scal[F_S_NORMR] = rr;
+        if (stop) {
+            c.scal[F_S_ITER] = (double)c.k;
+            if (!conv) c.scal[F_S_IERR] = rr == rr ? 1.0 : 2.0;
+            c.scal[F_S_DONE] = 1.0;
+        }
+    }
+    return stop;
+}
+
 struct MgArgs {
     int Hf, Wf, Hc, Wc;
     const float* dia;            // fine operator, nine diagonals [9][Hf*Wf], tap t <-> (t/3-1, t%3-1)
@@ -179,6 +246,7 @@ struct MgArgs {
     const double* done;
     const float* dh;             // learned levels: diag^-1/2 and the 81 tap-pair smoother weights
     const float* k81;
+    FcgCheck chk;                // first kernel of an FCG V-cycle: the convergence test (else part == nullptr)
 };
 
 // Down, tiled: a block owns CI x CJ coarse nodes. It stages x on the fine nodes
@@ -230,7 +298,7 @@ __device__ inline void mg_down_body(const MgArgs& a, float* xs, float* rx, float
         xv[it] = JAC ? a.dinv[f] : x[f];
         if (!in) { rv[it] = 0.0f; xv[it] = 0.0f; }
     }
-    if (a.done && *a.done != 0.0) return;
+    if (a.chk.part ? fcg_check_block(a.chk) : (a.done && *a.done != 0.0)) return;
 #pragma unroll
     for (int it = 0; it < MGD_XIT; ++it) {
         const int k = threadIdx.x + it * 256;
@@ -331,7 +399,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth_down(MgArgs a) {
         if (!in) { hr[it] = 0.0f; hd[it] = 0.0f; }
     }
     const float kv = a.k81[threadIdx.x < LN_KW ? threadIdx.x : 0];
-    if (a.done && *a.done != 0.0) return;
+    if (a.chk.part ? fcg_check_block(a.chk) : (a.done && *a.done != 0.0)) return;
 #pragma unroll
     for (int it = 0; it < MGF_UIT; ++it) {
         const int k = threadIdx.x + it * 256;
@@ -689,7 +757,8 @@ __global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs 
 #define SM_NU ((SMI + 4) * (SMJ + 4))
 #define SM_UIT ((SM_NU + 255) / 256)
 __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k81, const float* dh, const float* v,
-                                                    int64_t v_bs, float* out, int64_t o_bs, int add, const double* done) {
+                                                    int64_t v_bs, float* out, int64_t o_bs, int add, const double* done,
+                                                    FcgCheck chk) {
     static_assert(SMI * SMJ == 256, "k_smooth_k81 tile");
     __shared__ float us[SM_NU];
     __shared__ float kw[LN_KW];
@@ -716,7 +785,7 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
     const int p = own ? i * W + j : 0;
     const float dp = dh[p];
     const float o0 = add ? ob[p] : 0.0f;
-    if (done && *done != 0.0) return;
+    if (chk.part ? fcg_check_block(chk) : (done && *done != 0.0)) return;
     if (threadIdx.x < LN_KW) kw[threadIdx.x] = kv;
 #pragma unroll
     for (int it = 0; it < SM_UIT; ++it) {
@@ -1046,42 +1115,7 @@ __global__ __launch_bounds__(256) void k_harvest(int n, Dia8 A, const float* xf,
 }
 
 // ---------------------------------------------------------------- FCG(1) on the scaled system
-// scal slots (shared with the CG): NORMRHS, DONE, ITER, IERR, TOL2, MAXIT; pq in GAM0/GAM1
-enum { F_S_NORMRHS = 4, F_S_NORMR = 5, F_S_DONE = 6, F_S_ITER = 7, F_S_IERR = 8, F_S_TOL2 = 9, F_S_MAXIT = 10,
-       F_S_PQ0 = 12, F_S_PQ1 = 13, F_S_KCUR = 20, F_S_KNEXT = 21, F_S_FAIL = 22 };
-// The iteration index lives on the device (KNEXT advanced by the check kernel, KCUR read by
-// the update), so every iteration is the same launch sequence. A refinement pass leaves a
-// record at FCG_REC_BASE + FCG_RECW * slot: iterations, done, error code, last ||r||^2/||r0||^2,
-// skipped (the outer residual met its tolerance before the pass, or an earlier pass failed).
-// FAIL is set by a pass that ends in a NaN or a breakdown.
-#define FCG_BLOCK 256
-
-__device__ inline void fcg_block_partials(int nv, double* v, double* part, int pstride) {
-    __shared__ double sh[4][FCG_BLOCK / 64];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = 0; k < nv; ++k) {
-        double s = v[k];
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) sh[k][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < nv; ++k) part[(size_t)k * pstride + blockIdx.x] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
-}
-__device__ inline void fcg_reduce(int nv, const double* part, int pstride, int nb, double* out) {
-    __shared__ double sh[4][FCG_BLOCK / 64];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int q = threadIdx.x; q < nb; q += FCG_BLOCK)
-        for (int k = 0; k < nv; ++k) s[k] += part[(size_t)k * pstride + q];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int k = 0; k < nv; ++k) {
-        for (int off = 32; off > 0; off >>= 1) s[k] += __shfl_xor(s[k], off, 64);
-        if (lane == 0) sh[k][wid] = s[k];
-    }
-    __syncthreads();
-    for (int k = 0; k < nv; ++k) out[k] = sh[k][0] + sh[k][1] + sh[k][2] + sh[k][3];
-    __syncthreads();
-}
+// (scalar slots, block reductions and the merged convergence test are defined with the V-cycle)
 
 // r = D^-1/2 b64, x = 0; partial r.r (region 0). The pass is skipped (DONE at once) when the
 // FP64 residual before it meets the outer tolerance, rr <= otol2 * bb (outer_rrbb = rr, bb),
@@ -1103,7 +1137,7 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_init(int n, const double* b64
         const bool failed = !first && scal[F_S_FAIL] != 0.0;
         const bool skip = failed || obb == 0.0 || orr <= otol2 * obb;
         if (first) scal[F_S_FAIL] = 0.0;
-        scal[F_S_DONE] = skip ? 1.0 : 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0; scal[F_S_KNEXT] = 0.0;
+        scal[F_S_DONE] = skip ? 1.0 : 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0;
         scal[F_S_TOL2] = tol2; scal[F_S_MAXIT] = maxit;
         scal[F_S_NORMR] = 0.0; scal[F_S_NORMRHS] = 0.0;
         scal[FCG_REC_BASE + FCG_RECW * slot + 4] = skip ? 1.0 : 0.0;
@@ -1135,25 +1169,6 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_finish(int n, int nb, const d
     }
 }
 
-// convergence test of the current r before iteration k = KNEXT (one block): done if
-// ||r||^2/||b||^2 <= tol^2 or k >= maxit; publishes k as KCUR and advances KNEXT
-__global__ __launch_bounds__(FCG_BLOCK) void k_fcg_check(int nb, double* scal, const double* part, int pstride) {
-    if (scal[F_S_DONE] != 0.0) return;
-    const int k = (int)scal[F_S_KNEXT];
-    double rr;
-    fcg_reduce(1, part, pstride, nb, &rr);
-    if (threadIdx.x != 0) return;
-    scal[F_S_KCUR] = (double)k;
-    scal[F_S_KNEXT] = (double)(k + 1);
-    if (k == 0) scal[F_S_NORMRHS] = rr;
-    double nb2 = (k == 0) ? rr : scal[F_S_NORMRHS];
-    scal[F_S_NORMR] = rr;
-    if (nb2 == 0.0 || rr / nb2 <= scal[F_S_TOL2]) { scal[F_S_DONE] = 1.0; scal[F_S_ITER] = (double)k; return; }
-    if ((double)k >= scal[F_S_MAXIT] || !(rr == rr)) {
-        scal[F_S_DONE] = 1.0; scal[F_S_ITER] = (double)k; scal[F_S_IERR] = rr == rr ? 1.0 : 2.0;
-    }
-}
-
 // w = A' z; partials (z, r), (z, w), (z, q_prev) in regions 1, 2, 3
 __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_aw(int n, Dia8 A, const float* z, const float* r, const float* q,
                                                       float* w, const double* scal, double* part, int pstride) {
@@ -1178,13 +1193,12 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_aw(int n, Dia8 A, const float
 // partial r.r (region 0)
 __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, const float* z, const float* w, float* p,
                                                           float* q, float* x, float* r, double* scal, double* part,
-                                                          int pstride) {
+                                                          int pstride, int kk) {
     __shared__ double s_done;
     if (threadIdx.x == 0) s_done = scal[F_S_DONE];
     double d[3];
     fcg_reduce(3, part + (size_t)pstride, pstride, nb, d);     // contains a barrier after s_done
     if (s_done != 0.0) return;
-    const int kk = (int)scal[F_S_KCUR];
     const double k = (double)kk;
     const int par = kk & 1;
     double zr = d[0], zw = d[1], zq = d[2];
@@ -1321,26 +1335,35 @@ static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, in
     return a;
 }
 
-static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, int64_t o_bs, bool add, const double* done) {
+static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, int64_t o_bs, bool add, const double* done,
+                    const FcgCheck& chk) {
     const MgLevel& lv = ln.lv[l];
     dim3 grid((unsigned)((lv.W + SMJ - 1) / SMJ), (unsigned)((lv.H + SMI - 1) / SMI), batch);
     hipLaunchKernelGGL(k_smooth_k81, grid, dim3(256), 0, g.stream, lv.H, lv.W, ln.d_k81 + LN_KW * l, lv.dh, v, v_bs, out,
-                       o_bs, add ? 1 : 0, done);
+                       o_bs, add ? 1 : 0, done, chk);
 }
 
 // y = M r for `batch` inputs (level-0 vectors with batch stride bs); run selects the work set.
-static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done) {
+// chk (FCG solve only): the convergence test, run by the first kernel of the V-cycle.
+static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done,
+                     const FcgCheck* chk = nullptr) {
     const int L = ln.L, lc = ln.lc;
+    const FcgCheck none = {nullptr, 0, 0, 0, nullptr};
     for (int l = 0; l < lc; ++l) {
+        const FcgCheck& c0 = (l == 0 && chk) ? *chk : none;
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
         MgArgs a = mg_args(l, run, r, r_bs, nullptr, 0, done);
+        a.chk = c0;
         const MgLevel& c = ln.lv[l + 1];
         dim3 grid((unsigned)((c.W + MGD_CJ - 1) / MGD_CJ), (unsigned)((c.H + MGD_CI - 1) / MGD_CI), batch);
         // the fused smoother + down pays where its grid fills the GPU; on smaller levels the
         // smoother's own launch spreads the 25-tap work over more blocks
         const bool fuse = l < ln.Ls && (int64_t)grid.x * grid.y * batch >= 256;
-        if (l < ln.Ls && !fuse) smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done);
+        if (l < ln.Ls && !fuse) {
+            smooth5(l, batch, r, r_bs, ln.lv[l].x[run], ln.lv[l].n, false, done, c0);
+            a.chk = none;                            // tested by the smoother's launch
+        }
         if (fuse) {
             a.dh = ln.lv[l].dh; a.k81 = ln.d_k81 + LN_KW * l;
             hipLaunchKernelGGL(k_mg_smooth_down, grid, dim3(256), 0, g.stream, a);
@@ -1373,7 +1396,7 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
         const MgLevel& f = ln.lv[l];
         dim3 grid((unsigned)((f.W + MGU_FJ - 1) / MGU_FJ), (unsigned)((f.H + MGU_FI - 1) / MGU_FI), batch);
         hipLaunchKernelGGL(k_mg_up, grid, dim3(256), 0, g.stream, a);
-        if (l < ln.Ls) smooth5(l, batch, ln.lv[l].s2[run], ln.lv[l].n, y, y_bs, true, done);
+        if (l < ln.Ls) smooth5(l, batch, ln.lv[l].s2[run], ln.lv[l].n, y, y_bs, true, done, none);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
@@ -1945,11 +1968,11 @@ int learn_fcg_pass(int n, const double* b64, double inner_tol, int K, double oto
                        inner_tol * inner_tol, (double)K + 1.0, otol2, outer_rrbb, first, slot);
     const Dia8 A = scaled_op();
     for (int c = 0; c < K; ++c) {
-        hipLaunchKernelGGL(k_fcg_check, dim3(1), dim3(FCG_BLOCK), 0, g.stream, nb, scal, part, ps);
-        int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE); if (rc) return rc;
+        const FcgCheck chk = {part, ps, nb, c, scal};     // the V-cycle's first kernel tests convergence
+        int rc = net_apply(1, r, ln.z, n, 0, scal + F_S_DONE, &chk); if (rc) return rc;
         hipLaunchKernelGGL(k_fcg_aw, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, A, ln.z, r, ln.q, ln.w, scal, part, ps);
         hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, ln.z, ln.w, p, ln.q, x, r,
-                           scal, part, ps);
+                           scal, part, ps, c);
     }
     hipLaunchKernelGGL(k_fcg_finish, dim3(nb), dim3(FCG_BLOCK), 0, g.stream, n, nb, g.d_sinv, x, xn, scal, part, ps,
                        slot);
