@@ -182,6 +182,25 @@ int  pic_get_particles(int isp, double* x, double* y, double* vx, double* vy, do
 int  pic_set_azimuth(int isp, int64_t n, const double* th);
 int  pic_get_azimuth(int isp, double* th, int64_t capacity, int64_t* n);
 
+/* --------------------------------------------------------------------- files
+ * The reference's on-disk formats (little-endian Fortran unformatted records framed by
+ * 4-byte lengths where the reference writes FORM='UNFORMATTED').
+ * DumpParticleBundle / LoadParticleBundle (MCC_jw ParticleBundle.f90:286-375): records
+ * [NPar, NParNormal], [Charge, Mass, Weight], then [X, Y, Z, Vx, Vy, Vz, Ax, Ay, Az, Wq,
+ * Location] per particle, X in metres and velocities in m/s (the bundle's PosRes/VelRes:
+ * only X is rescaled, Y and Z stay normalised); the load converts back and needs the file's
+ * charge, mass and weight to match the species. Dump_2D / Restart_2D (code/PIC/Dump_2D.f90,
+ * Restart_2D.f90): <dir>/var<lap:7>dump (list-directed counters) and <dir>/phi<lap:7>dump
+ * (Phi). Output_To_Tecplot_IJK_2D option 1: <dir>/field_IJ_<it:6>.dat. */
+int  pic_DumpParticleBundle(int isp, const char* path);
+int  pic_LoadParticleBundle(int isp, const char* path, int64_t* npar);
+int  pic_Dump_2D(int lap, const char* dir);
+int  pic_Restart_2D(int lap, const char* dir);
+int  pic_Output_To_Tecplot_IJ(int it, const char* dir);
+/* The Fortran Ew.d edit descriptor these writers use ([-]0.ddd E+ee, no E past exponent 99),
+ * into out (at least w + 1 bytes). Host only. */
+int  pic_format_fortran_e(double v, int w, int d, char* out);
+
 /* Live particles, occupied slots, and event counters of a species:
  * counters[0] nLoss(1) left-wall re-emissions, [1] nLoss(2) right-wall absorptions,
  * [2] object absorptions, [3] wall reflections, [4] MCC selected (null incl.),
@@ -287,7 +306,9 @@ enum {
     PIC_F_RHO_S = 6,     /* rho_s(nnodes, nspecies) */
     PIC_F_RHO_P = 7,     /* rho_p(nnodes, nspecies) */
     PIC_F_EK_TOT = 8,    /* Ek_tot(nnodes, nspecies) */
-    PIC_F_RHS = 9        /* rhs(nunknowns) of the last solve */
+    PIC_F_RHS = 9,       /* rhs(nunknowns) of the last solve */
+    PIC_F_EK_S = 10      /* Ek_s(nnodes, nspecies): Ek_tot / rho before normalisation (eV per
+                          * particle; GetParChg_2D.f90:885-886), 0 where either sum is 0 */
 };
 int  pic_get_field(int which, double* dst, int64_t n);
 int  pic_set_field(int which, const double* src, int64_t n);

@@ -29,7 +29,7 @@ ERRORS = {
 SETUP = dict(HP=1, HT=2, ELEMENT_INDEX=3, NODE_INDEX=4, NODE_TYPE=5, CELL_VOLUME=6, A_ROWPTR=7, A_COL=8, A_VAL=9,
              A_DIAG=10, B_ROWPTR=11, B_COL=12, B_VAL=13, RHS_CONST=14, DIRICHLET_PHI=15, POINT_PATCH=16,
              UNKNOWN_NODE=17)
-FIELD = dict(RHO=1, PHI=2, EFX=3, EFY=4, COEF_PPR=5, RHO_S=6, RHO_P=7, EK_TOT=8, RHS=9)
+FIELD = dict(RHO=1, PHI=2, EFX=3, EFY=4, COEF_PPR=5, RHO_S=6, RHO_P=7, EK_TOT=8, RHS=9, EK_S=10)
 
 # physical constants of the reference (MCC_jw/code/base/Constants.f90)
 ELECTRON_MASS = 9.1095e-31
@@ -104,6 +104,12 @@ _PROTOS = {
     "pic_load_particles": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]),
     "pic_get_particles": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp, _dp, _ip, C.c_int64, _i64p]),
     "pic_set_azimuth": (C.c_int, [C.c_int, C.c_int64, _dp]),
+    "pic_DumpParticleBundle": (C.c_int, [C.c_int, C.c_char_p]),
+    "pic_LoadParticleBundle": (C.c_int, [C.c_int, C.c_char_p, _i64p]),
+    "pic_Dump_2D": (C.c_int, [C.c_int, C.c_char_p]),
+    "pic_Restart_2D": (C.c_int, [C.c_int, C.c_char_p]),
+    "pic_Output_To_Tecplot_IJ": (C.c_int, [C.c_int, C.c_char_p]),
+    "pic_format_fortran_e": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_char_p]),
     "pic_get_azimuth": (C.c_int, [C.c_int, _dp, C.c_int64, _i64p]),
     "pic_get_counts": (C.c_int, [C.c_int, _i64p, _i64p, _i64p]),
     "pic_get_mcc_events": (C.c_int, [C.c_int, _i64p, C.c_int]),
@@ -348,6 +354,24 @@ class PicLibrary:
             t = np.ascontiguousarray(th, dtype=np.float64)
             check(lib().pic_set_azimuth(isp, len(t), _d(t)))
 
+    # files in the reference's formats
+    def dump_bundle(self, isp: int, path: str):
+        check(lib().pic_DumpParticleBundle(isp, str(path).encode()))
+
+    def load_bundle_file(self, isp: int, path: str) -> int:
+        n = C.c_int64()
+        check(lib().pic_LoadParticleBundle(isp, str(path).encode(), C.byref(n)))
+        return n.value
+
+    def dump_2d(self, lap: int, directory: str):
+        check(lib().pic_Dump_2D(lap, str(directory).encode()))
+
+    def restart_2d(self, lap: int, directory: str):
+        check(lib().pic_Restart_2D(lap, str(directory).encode()))
+
+    def tecplot_field(self, it: int, directory: str):
+        check(lib().pic_Output_To_Tecplot_IJ(it, str(directory).encode()))
+
     def azimuth(self, isp: int) -> np.ndarray:
         """PO%Z of the live particles, in the order of particles() (axisymmetric domains)."""
         ns = C.c_int64()
@@ -385,7 +409,8 @@ class PicLibrary:
     def field(self, name: str) -> np.ndarray:
         s = self.sizes()
         nn = s["nnodes"]
-        n = dict(COEF_PPR=6 * nn, RHO_S=4 * nn, RHO_P=4 * nn, EK_TOT=4 * nn, RHS=s["nunknowns"]).get(name, nn)
+        n = dict(COEF_PPR=6 * nn, RHO_S=4 * nn, RHO_P=4 * nn, EK_TOT=4 * nn, EK_S=4 * nn,
+                 RHS=s["nunknowns"]).get(name, nn)
         a = np.zeros(n)
         check(lib().pic_get_field(FIELD[name], _d(a), n))
         return a
@@ -616,6 +641,13 @@ def species_weight(dom: Domain, ppc: int, n0: float = 1.0) -> float:
     L = dom.normalisation()["L_ref"]
     h = (dom.xmax - dom.xmin) / (dom.nnx - 1)
     return n0 * dom.n_ref * (L * h) ** 2 / ppc
+
+
+def format_fortran_e(v: float, w: int = 15, d: int = 6) -> str:
+    """Fortran Ew.d text of v as the library's file writers produce it (host only)."""
+    out = C.create_string_buffer(w + 1)
+    check(lib().pic_format_fortran_e(float(v), w, d, out))
+    return out.value.decode()
 
 
 def build_workload(pl: PicLibrary, name: str, seed: int = 2024, ppc: int | None = None, capacity_factor: float = 1.5,

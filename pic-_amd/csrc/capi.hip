@@ -293,6 +293,7 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     TRY(dalloc(&g.d_rho_s, (size_t)m.nnodes * PIC_MAX_SPECIES));
     TRY(dalloc(&g.d_rho_p, (size_t)m.nnodes * PIC_MAX_SPECIES));
     TRY(dalloc(&g.d_ek, (size_t)m.nnodes * PIC_MAX_SPECIES));
+    TRY(dalloc(&g.d_eks, (size_t)m.nnodes * PIC_MAX_SPECIES));
     // PPR
     TRY(dupload(&g.d_ppr_class, g.h_ppr_class));
     TRY(dupload(&g.d_ppr_classes, g.h_ppr_classes));
@@ -933,6 +934,7 @@ int pic_get_field(int which, double* dst, int64_t n) {
     case PIC_F_RHO_S: src = g.d_rho_s; cnt = nn * g.nspecies; break;
     case PIC_F_RHO_P: src = g.d_rho_p; cnt = nn * g.nspecies; break;
     case PIC_F_EK_TOT: src = g.d_ek; cnt = nn * g.nspecies; break;
+    case PIC_F_EK_S: src = g.d_eks; cnt = nn * g.nspecies; break;
     case PIC_F_RHS: src = g.d_b; cnt = g.nunknowns; break;
     default: return pic_fail(PIC_ERR_ARG, "unknown field");
     }

@@ -377,7 +377,10 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& 
     // any element), so one wait covers them
     const unsigned int nbr = pic_ld(m.elem_nbr, e_old);     // flags of the 3x3 neighbourhood
     R x1 = pic_ld(a.xsR, ci), x2 = pic_ld(a.xsR, ci + 1), y1 = pic_ld(a.ysR, cj), y2 = pic_ld(a.ysR, cj + 1);
-    const int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+    // a 24-bit product (mesh sides < 2^24): the generic 32-bit multiply-add is a 64-bit
+    // v_mad_u64_u32 whose addend pair can take the register of the load above still in flight,
+    // which makes the field gathers wait for it
+    const int n1 = (int)__umul24((unsigned)ci, (unsigned)m.nny) + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
     const R e1x = pic_ld(a.efx, n1), e2x = pic_ld(a.efx, n2), e3x = pic_ld(a.efx, n3), e4x = pic_ld(a.efx, n4);
     const R e1y = pic_ld(a.efy, n1), e2y = pic_ld(a.efy, n2), e3y = pic_ld(a.efy, n3), e4y = pic_ld(a.efy, n4);
     R hxp = x2 - x1, hyp = y2 - y1;
@@ -732,6 +735,7 @@ struct FinalArgs {
     double L_ref, n_ref;
     const double* vol;
     double *rho, *rho_s, *rho_p, *ek;
+    double* eks;              // Ek_s: mean energy per particle (eV) at the node, before normalisation
     unsigned int* err;
 };
 
@@ -756,6 +760,7 @@ __global__ void k_finalize(FinalArgs a) {
         }
         size_t o = (size_t)s * a.nnodes + n;
         a.rho_p[o] = w; a.rho_s[o] = rs; a.ek[o] = ek;
+        a.eks[o] = (e != 0.0 && w != 0.0) ? e / w : 0.0;      // GetParChg_2D.f90:885-886
         rho = rho + rs;
     }
     a.rho[n] = rho;
@@ -774,7 +779,7 @@ int launch_finalize_deposit() {
     }
     a.L_ref = g.L_ref; a.n_ref = g.cfg.n_ref;
     a.vol = g.d_cell_volume;
-    a.rho = g.d_rho; a.rho_s = g.d_rho_s; a.rho_p = g.d_rho_p; a.ek = g.d_ek;
+    a.rho = g.d_rho; a.rho_s = g.d_rho_s; a.rho_p = g.d_rho_p; a.ek = g.d_ek; a.eks = g.d_eks;
     a.err = g.d_err;
     hipLaunchKernelGGL(k_finalize, dim3((a.nnodes + 255) / 256), dim3(256), 0, g.stream, a);
     HIPCHK(hipGetLastError());

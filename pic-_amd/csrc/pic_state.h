@@ -165,6 +165,7 @@ struct PicState {
     double* d_coef5 = nullptr;                                                         // c2..c6 node-major (5 x nnodes)
     float *d_efxf = nullptr, *d_efyf = nullptr, *d_coeff = nullptr;                  // coeff: 5 x nnodes (c2..c6)
     double *d_rho = nullptr, *d_rho_s = nullptr, *d_rho_p = nullptr, *d_ek = nullptr;
+    double* d_eks = nullptr;                 // Ek_s per species (eV per particle at the node)
 
     // PPR
     int* d_ppr_class = nullptr;

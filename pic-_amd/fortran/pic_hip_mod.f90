@@ -96,6 +96,33 @@ module pic_hip_mod
             integer(c_int64_t), value :: capacity
             integer(c_int64_t) :: npar
         end function
+        ! files in the reference's formats; names are NUL-terminated (trim(name)//c_null_char)
+        integer(c_int) function pic_DumpParticleBundle(isp, path) bind(c, name="pic_DumpParticleBundle")
+            import :: c_int, c_char
+            integer(c_int), value :: isp
+            character(kind=c_char) :: path(*)
+        end function
+        integer(c_int) function pic_LoadParticleBundle(isp, path, npar) bind(c, name="pic_LoadParticleBundle")
+            import :: c_int, c_char, c_int64_t
+            integer(c_int), value :: isp
+            character(kind=c_char) :: path(*)
+            integer(c_int64_t) :: npar
+        end function
+        integer(c_int) function pic_Dump_2D(lap, dir) bind(c, name="pic_Dump_2D")
+            import :: c_int, c_char
+            integer(c_int), value :: lap
+            character(kind=c_char) :: dir(*)
+        end function
+        integer(c_int) function pic_Restart_2D(lap, dir) bind(c, name="pic_Restart_2D")
+            import :: c_int, c_char
+            integer(c_int), value :: lap
+            character(kind=c_char) :: dir(*)
+        end function
+        integer(c_int) function pic_Output_To_Tecplot_IJ(it, dir) bind(c, name="pic_Output_To_Tecplot_IJ")
+            import :: c_int, c_char
+            integer(c_int), value :: it
+            character(kind=c_char) :: dir(*)
+        end function
         integer(c_int) function pic_set_azimuth(isp, n, th) bind(c, name="pic_set_azimuth")
             import :: c_int, c_int64_t, c_double
             integer(c_int), value :: isp
