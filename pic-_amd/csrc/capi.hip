@@ -113,20 +113,28 @@ __global__ __launch_bounds__(1024) void k_sum_counts(int nb, const int* blockcou
 }
 
 // number of slots with cell >= 0 among the first n
+// Live (cell >= 0) slots among the first n: per-block counts into a scratch array kept
+// between calls (regrown only when a larger population needs more blocks), one reduction,
+// one pinned read-back.
 int count_live_slots(int64_t n, const int* cell, int64_t* out) {
     if (n == 0) { *out = 0; return PIC_OK; }
     int nb = (int)((n + 1023) / 1024);
-    int* bc = nullptr;
-    HIPCHK(hipMalloc(&bc, (size_t)nb * sizeof(int) + 16));
+    const int64_t need = (int64_t)nb * (int64_t)sizeof(int) + 16;
+    if (g.live_bc_cap < need) {
+        if (g.d_live_bc) (void)hipFree(g.d_live_bc);
+        g.d_live_bc = nullptr;
+        const int64_t bytes = need + need / 4 + 4096;
+        HIPCHK(hipMalloc(&g.d_live_bc, (size_t)bytes));
+        g.live_bc_cap = bytes;
+    }
     long long* d = (long long*)g.d_partials;
-    hipLaunchKernelGGL(k_count_live_blocks, dim3(nb), dim3(256), 0, g.stream, n, cell, bc);
-    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, g.stream, nb, bc, d);
+    hipLaunchKernelGGL(k_count_live_blocks, dim3(nb), dim3(256), 0, g.stream, n, cell, g.d_live_bc);
+    hipLaunchKernelGGL(k_sum_counts, dim3(1), dim3(1024), 0, g.stream, nb, g.d_live_bc, d);
     HIPCHK(hipGetLastError());
-    long long v = 0;
-    HIPCHK(hipMemcpyAsync(&v, d, 8, hipMemcpyDeviceToHost, g.stream));
+    long long* h = (long long*)g.h_scal_pinned;
+    HIPCHK(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
-    (void)hipFree(bc);
-    *out = (int64_t)v;
+    *out = (int64_t)h[0];
     return PIC_OK;
 }
 

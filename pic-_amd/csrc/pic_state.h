@@ -166,6 +166,7 @@ struct PicState {
     float *d_efxf = nullptr, *d_efyf = nullptr, *d_coeff = nullptr;                  // coeff: 5 x nnodes (c2..c6)
     double *d_rho = nullptr, *d_rho_s = nullptr, *d_rho_p = nullptr, *d_ek = nullptr;
     double* d_eks = nullptr;                 // Ek_s per species (eV per particle at the node)
+    int* d_live_bc = nullptr; int64_t live_bc_cap = 0;   // count_live_slots per-block counts
 
     // PPR
     int* d_ppr_class = nullptr;
