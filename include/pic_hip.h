@@ -316,7 +316,8 @@ int  pic_set_field(int which, const double* src, int64_t n);
 /* Per-phase device time of the last pic_step call, ms: [0] push+locate+deposit,
  * [1] deposit finalise, [2] rhs, [3] PCG, [4] PPR, [5] MCC, [6] sort, [7] total; with
  * n >= 12 also kernel statistics: [8] ms in the CG iteration kernel, [9] its launches,
- * [10] ms in the fused push kernel, [11] its launches (HIP events on the library's
+ * [10] ms in the fused push kernel alone (events on both sides of its launch; the wall
+ * kernel after it is not included), [11] its launches (HIP events on the library's
  * stream). */
 int  pic_get_phase_times(double* ms, int n);
 

@@ -703,6 +703,7 @@ int launch_push_deposit(int isp, bool do_push) {
         PushArgs<double> a = make_push_args<double>(isp);
         if (do_push) {
             hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
             hipLaunchKernelGGL((k_push_walls<double>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
         } else {
             hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
@@ -711,13 +712,16 @@ int launch_push_deposit(int isp, bool do_push) {
         PushArgs<float> a = make_push_args<float>(isp);
         if (do_push) {
             hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
+            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
             hipLaunchKernelGGL((k_push_walls<float>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
         } else {
             hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
         }
     }
     HIPCHK(hipGetLastError());
-    if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream)); g.ev_push_pending[isp] = true; }
+    // axisymmetric: no fused push kernel, the interval covers the all-slot mover
+    if (do_push && g.mesh.delta == 1) HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
+    if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][2], g.stream)); g.ev_push_pending[isp] = true; }
     S.acc_dirty = true;
     return PIC_OK;
 }

@@ -209,7 +209,7 @@ struct PicState {
     // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches,
     // [4] ms in learned-FCG iterations (chunk events), [5] FCG iterations launched
     double kstat[6] = {0, 0, 0, 0, 0, 0};
-    hipEvent_t ev_push[PIC_MAX_SPECIES][2];
+    hipEvent_t ev_push[PIC_MAX_SPECIES][3];   // before the push kernel, after it, after the wall kernel
     bool ev_push_pending[PIC_MAX_SPECIES] = {false};
     double* d_scal = nullptr;          // PCG scalars (device)
     double* d_partials = nullptr;      // block partial sums
