@@ -818,6 +818,15 @@ static void or_block_u01(uint64_t seed, uint32_t slot, uint32_t step, uint32_t t
     *rb = (((double)((((uint64_t)o.v[2]) << 20) | (uint64_t)(o.v[3] >> 12))) + 0.5) * (1.0 / 4503599627370496.0);
 }
 
+/* v if v < hi, else the largest value below hi of the particle precision (bits 32: float) */
+static double or_below(double v, double hi, int bits) {
+    if (v < hi) return v;
+    if (bits == 64) return nextafter(hi, -INFINITY);
+    float b = (float)hi;
+    while ((double)b >= hi) b = nextafterf(b, -INFINITY);
+    return (double)b;
+}
+
 static double or_loadv(uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag, double vmp) {   /* Loadv.f90:14-23 */
     for (uint32_t i = 0;; ++i) {
         double R, R1;
@@ -832,7 +841,8 @@ static double or_loadv(uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag
  * component (:91-106), c = 1, 2 Loadv for Vy, Vz (+ ion drift, :121-126), c = 3 PosInit's X
  * then Y (ParticleOne.f90:82-89); positions rounded to the particle precision (bits 32:
  * float) before InjectPositioning (InjectPositioning.f90:13-22, no clamp; cell -1 when
- * outside). vd = drift {vx, vy, vz}; mesh: xmin, ymin, mapx, mapy, nex, ney. */
+ * outside); a stored X or Y that rounds onto the slab's upper bound takes the largest value
+ * below it, keeping the particle strictly inside the slab as sampled. vd = drift {vx, vy, vz}; mesh: xmin, ymin, mapx, mapy, nex, ney. */
 void or_inject(int bits, int64_t n, int electron, const double* c, const double* vd, double y_lo, double y_hi,
                uint64_t seed, uint32_t step, uint32_t tag, double xmin, double ymin, double mapx, double mapy,
                int nex, int ney, double* x, double* y, double* vx, double* vy, double* vz, int* cell) {
@@ -857,6 +867,7 @@ void or_inject(int bits, int64_t n, int electron, const double* c, const double*
         double X = c[7] + (c[8] - c[7]) * RX;
         double Y = y_lo + (y_hi - y_lo) * RY;
         if (bits == 32) { X = (double)(float)X; Y = (double)(float)Y; Vx = (double)(float)Vx; Vy = (double)(float)Vy; Vz = (double)(float)Vz; }
+        X = or_below(X, c[8], bits); Y = or_below(Y, y_hi, bits);
         int ii = (int)((X - xmin) / mapx) + 1, jj = (int)((Y - ymin) / mapy) + 1;
         x[k] = X; y[k] = Y; vx[k] = Vx; vy[k] = Vy; vz[k] = Vz;
         cell[k] = (ii < 1 || ii > nex || jj < 1 || jj > ney) ? -1 : (jj - 1) + (ii - 1) * ney;

@@ -871,6 +871,7 @@ struct InjectSource {
     int64_t base;                        // first free slot
     int n;                               // particles to add
     uint32_t tag;
+    int species;
     void *x, *y, *vx, *vy, *vz;
     int* cell;
 };
@@ -881,7 +882,19 @@ struct InjectArgs {
     uint64_t seed; uint32_t step;
     MeshParams m;
     unsigned int* err;
+    PicErrInfo* errinfo;
 };
+
+// v if v < hi, else the largest value of v's type below hi
+__device__ inline double pic_below(double v, double hi) {
+    return v < hi ? v : nextafter(hi, -INFINITY);
+}
+__device__ inline float pic_below(float v, double hi) {
+    if ((double)v < hi) return v;
+    float b = (float)hi;
+    while ((double)b >= hi) b = nextafterf(b, -INFINITY);
+    return b;
+}
 
 #define PIC_SAMPLER_MAXIT 1024
 #define PIC_SAMPLER_ILP 4
@@ -937,7 +950,10 @@ __global__ void k_inject(InjectArgs a) {
         const philox4 o = pic_philox4x32_10(slot, a.step, S.tag, 0u, k0, k1);
         const double X = p.xl + (p.xu - p.xl) * pic_u01(o.v[0], o.v[1]);
         const double Y = p.y_lo + (p.y_hi - p.y_lo) * pic_u01(o.v[2], o.v[3]);
-        const R Xs = (R)X, Ys = (R)Y;
+        // a stored coordinate that rounds onto the slab's upper bound (FP32 slots: Y just below
+        // y_hi becomes y_hi) takes the largest value below it, so the particle stays strictly
+        // inside the slab, as the sampled value is
+        const R Xs = pic_below((R)X, p.xu), Ys = pic_below((R)Y, p.y_hi);
         // InjectPositioning on the stored position (no clamp)
         const MeshParams& m = a.m;
         const double rx = ((double)Xs - m.xmin) / m.mapx;
@@ -945,8 +961,12 @@ __global__ void k_inject(InjectArgs a) {
         const double ry = ((double)Ys - m.ymin) / m.mapy;
         const int jj = (int)ry + 1;
         int e = -1;
-        if (ii < 1 || ii > m.nex || jj < 1 || jj > m.ney) atomicOr(a.err, PIC_EBIT_POSITION);
-        else e = (jj - 1) + (ii - 1) * m.ney;
+        if (ii < 1 || ii > m.nex || jj < 1 || jj > m.ney) {
+            atomicOr(a.err, PIC_EBIT_POSITION);
+            pic_record(a.errinfo, PIC_EBIT_POSITION, d, Xs, Ys, 0.0, 0.0, 0, -1, -1, S.species);
+        } else {
+            e = (jj - 1) + (ii - 1) * m.ney;
+        }
         ((R*)S.x)[d] = Xs; ((R*)S.y)[d] = Ys;
         S.cell[d] = e;
     }
@@ -980,6 +1000,7 @@ int run_injection(int it, int64_t* added) {
         InjectSource& q = a.src[a.nsrc];
         q.p = g.inj[s]; q.electron = (s == g.e_species) ? 1 : 0;
         q.base = S.nslots; q.n = (int)ns[s]; q.tag = pic_stream_tag(s, PIC_RNG_INJECT);
+        q.species = s;
         q.x = S.x; q.y = S.y; q.vx = S.vx; q.vy = S.vy; q.vz = S.vz; q.cell = S.cell;
         a.first[a.nsrc + 1] = a.first[a.nsrc] + 4 * ns[s];
         a.nsrc++;
@@ -988,7 +1009,7 @@ int run_injection(int it, int64_t* added) {
     }
     if (a.nsrc == 0) return PIC_OK;
     a.seed = g.seed; a.step = (uint32_t)it;
-    a.m = g.mesh; a.err = g.d_err;
+    a.m = g.mesh; a.err = g.d_err; a.errinfo = (PicErrInfo*)g.d_errinfo;
     const int blocks = (int)((a.first[a.nsrc] + 255) / 256);
     if (g.particle_bits == 64) hipLaunchKernelGGL(k_inject<double>, dim3(blocks), dim3(256), 0, g.stream, a);
     else hipLaunchKernelGGL(k_inject<float>, dim3(blocks), dim3(256), 0, g.stream, a);

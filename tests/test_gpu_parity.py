@@ -413,12 +413,14 @@ def test_full_step_deterministic_and_conserving(gpu_lib, pkg, case):
 
 
 # ------------------------------------------------------------------ wall injection
-@pytest.mark.parametrize("bits", [64, 32])
-def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
+@pytest.mark.parametrize("bits,slab", [(64, "full"), (32, "full"), (32, "top"), (64, "top")])
+def test_injection_matches_oracle(gpu_lib, pkg, orc, bits, slab):
     """InjectBeams_2D: the count, and every appended particle's position, velocity and element,
-    bit-exact against the oracle's restatement on the same streams (FP64 and FP32 particles)."""
+    bit-exact against the oracle's restatement on the same streams (FP64 and FP32 particles).
+    The "top" slab is 1e-3 high under the mesh's upper edge, where about one FP32 Y in 2000
+    rounds onto the edge and is stored as the float below it."""
     pl = gpu_lib
-    say(f"case: InjectBeams_2D small32 bits={bits}")
+    say(f"case: InjectBeams_2D small32 bits={bits} slab={slab}")
     w = pkg.workload("small32")
     dom = w["domain"]
     pl.setup(dom, bits=bits, solver=0, tol=1e-10)
@@ -432,7 +434,9 @@ def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
     pl.load_maxwellian(1, 8, temps[1], 6)
     pl.set_run_params(20, 77, 0)
     norm = dom.normalisation()
-    ylen = float(dom.ymax) - 0.0
+    y_hi = float(dom.ymax)
+    y_lo = 0.0 if slab == "full" else y_hi - 1e-3
+    ylen = y_hi - y_lo
     src = {}
     for s in (0, 1):
         xm = masses[s] / pkg.ELECTRON_MASS
@@ -443,7 +447,7 @@ def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
         c = orc.inject_consts(vd[0], vt, tmpj, xm, 0.0, dom.dt)
         per_dens = abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * ylen * norm["L_ref"] / weight
         dens = (2000.0 if s == 0 else 400.0) / per_dens       # about 2000 electrons, 400 ions per call
-        pl.set_injection(s, dens, temps[s], drift, r_start=0.0, y_lo=0.0, y_hi=ylen)
+        pl.set_injection(s, dens, temps[s], drift, r_start=0.0, y_lo=y_lo, y_hi=y_hi)
         expect = (dens / dom.n_ref) * dom.n_ref * abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * ylen * \
             norm["L_ref"] / weight
         src[s] = (c, vd, expect)
@@ -455,7 +459,7 @@ def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
         c, vd, expect = src[s]
         tg = orc.tag(s, orc.RNG_INJECT)
         assert added[s] == orc.inject_count(expect, 77, 7, tg)
-        x, y, vx, vy, vz, cell = orc.inject(bits, added[s], 1 if s == 0 else 0, c, vd, 0.0, ylen, 77, 7, tg,
+        x, y, vx, vy, vz, cell = orc.inject(bits, added[s], 1 if s == 0 else 0, c, vd, y_lo, y_hi, 77, 7, tg,
                                             dom.xmin, dom.ymin, 1.0, 1.0, dom.nnx - 1, dom.nny - 1)
         p = pl.particles(s)
         assert pl.counts(s)["nslots"] == before[s] + added[s]
@@ -463,6 +467,7 @@ def test_injection_matches_oracle(gpu_lib, pkg, orc, bits):
         for k, ref in (("x", x), ("y", y), ("vx", vx), ("vy", vy), ("vz", vz)):
             assert np.array_equal(p[k][-n:], ref), (s, k)
         assert np.array_equal(p["loc"][-n:] - 1, cell), s
+        assert np.all(cell >= 0) and np.all(y < y_hi)
     # the injected particles move and deposit like any other in the next step
     pl.initial_field()
     pl.step(1)

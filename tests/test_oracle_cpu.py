@@ -510,6 +510,22 @@ def test_injection_sampler_statistics(orc):
     assert abs(vy_e.mean()) < 5 * vt / np.sqrt(4000)
 
 
+def test_injection_upper_edge_stays_inside(orc):
+    """A slab 1e-3 high under the mesh's upper edge y = 16: about one FP32 Y in 2000 rounds onto
+    16.0; those are stored as the float below 16, so every particle keeps an element (the top
+    row) and none sits on the edge. FP64 Y never reaches the edge here."""
+    c = orc.inject_consts(0.2, 1.0, 1.0, 1.0, 0.0, 0.05)
+    below = float(np.nextafter(np.float32(16.0), np.float32(0.0)))
+    for bits in (32, 64):
+        x, y, vx, vy, vz, cell = orc.inject(bits, 40000, 0, c, [0.2, 0.0, 0.0], 16.0 - 1e-3, 16.0, 9, 4,
+                                            orc.tag(1, orc.RNG_INJECT), 0.0, 0.0, 1.0, 1.0, 16, 16)
+        assert np.all(y < 16.0) and np.all(cell >= 0) and np.all(cell % 16 == 15)
+        hits = int(np.count_nonzero(y == below))
+        if bits == 32:
+            assert hits > 5                   # the clamp was exercised
+            assert np.all(y == y.astype(np.float32))
+
+
 def test_injection_count_rounding(orc):
     """add_N = INT(expect + R): floor or floor + 1, with mean expect over many steps."""
     counts = np.array([orc.inject_count(12.37, 3, s, orc.tag(0, orc.RNG_INJECT)) for s in range(4000)])
