@@ -577,6 +577,172 @@ void or_element_matrices(int delta, double x0, double y0, double hx, double hy, 
     (void)x0;
 }
 
+/* ------------------------------------------------------------------ IFE interface element
+ * Bilinear immersed-finite-element matrices of one square element cut by a circle (the
+ * interface of a dielectric object), following the reference's construction:
+ *   - classification and vertex order (El_Object_Intersection_info_2D_BL.f90): A1..A4 is the
+ *     element's LL, LR, UR, UL rotated so that A1 is the lone vertex (type 1: the circle cuts
+ *     two adjacent edges) or, for type 2 (two opposite edges cut), A1 = LL when the bottom and
+ *     top edges are cut and A1 = UL when the left and right edges are; E lies on edge A1A2, D
+ *     on edge A4A1 (type 1) or A4A3 (type 2); piece 1 holds A1 (and A4 for type 2);
+ *   - cut points: the circle's root on each cut edge (Line_Circle_Intersection_BL);
+ *   - basis (Linear_IFE_Basis_Coeff_2D.f90): per piece c1 + c2 x + c3 y + c4 xy; nodal values;
+ *     continuity at D, E and at the midpoint of DE (c4 equal when DE is axis-parallel); zero
+ *     jump of beta * flux integrated along DE;
+ *   - stiffness K_ij = sum over pieces beta int grad phi_i . grad phi_j (times r when
+ *     axisymmetric, Gauss_integration_local_stiffness_IFE.f90), integrated exactly;
+ *   - RHS operator M_ik (IFE_RHS_2D.f90 with Inter_Tetra_Partition_2D and Gauss_Nodes_2D): the
+ *     pieces split into the reference's sub-triangles, each integrated by the edge-midpoint
+ *     rule (weights area/3), rho_h = sum_k rho_k phi_k, so rhs_i = sum_k M_ik rho_k.
+ * Coordinates inside are X = (x - x0)/h, Y = (y - y0)/h. inside[k]: vertex k (LL, LR, UR, UL)
+ * inside the circle. Outputs K, M (4x4, row = test node, LL..UL), coef[4][2][4] (basis i,
+ * piece, c1..c4 in X, Y), pts[4] = D, E in X, Y, order[4] = local index of A1..A4.
+ * Returns the type (1, 2), 0 when the element is not cut, -1 for a saddle (two diagonal
+ * vertices inside), -2 when a cut point falls within 1e-12 of a vertex. */
+static int or_solve8(double A[8][8], double b[8], double x[8]) {
+    for (int c = 0; c < 8; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 8; ++r) if (fabs(A[r][c]) > fabs(A[p][c])) p = r;
+        if (A[p][c] == 0.0) return -1;
+        if (p != c) {
+            for (int k = 0; k < 8; ++k) { double t = A[c][k]; A[c][k] = A[p][k]; A[p][k] = t; }
+            double t = b[c]; b[c] = b[p]; b[p] = t;
+        }
+        for (int r = c + 1; r < 8; ++r) {
+            double f = A[r][c] / A[c][c];
+            for (int k = c; k < 8; ++k) A[r][k] -= f * A[c][k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int c = 7; c >= 0; --c) {
+        double s = b[c];
+        for (int k = c + 1; k < 8; ++k) s -= A[c][k] * x[k];
+        x[c] = s / A[c][c];
+    }
+    return 0;
+}
+
+static double or_circle_cut(double xi, double yi, double xo, double yo, double cx, double cy, double r) {
+    /* parameter t in (0, 1) of the circle crossing on the segment inside point -> outside point */
+    double dx = xo - xi, dy = yo - yi, fx = xi - cx, fy = yi - cy;
+    double a = dx * dx + dy * dy, bh = fx * dx + fy * dy, c = fx * fx + fy * fy - r * r;
+    return (-bh + sqrt(bh * bh - a * c)) / a;
+}
+
+int or_ife_element(int delta, double x0, double y0, double h, const int* inside, double cx, double cy, double rad,
+                   double beta_in, double beta_out, double* K, double* M, double* coef, double* pts, int* order) {
+    static const double VX[4] = {0, 1, 1, 0}, VY[4] = {0, 0, 1, 1};
+    int kin = inside[0] + inside[1] + inside[2] + inside[3];
+    if (kin == 0 || kin == 4) return 0;
+    int rot, type;
+    if (kin == 1 || kin == 3) {
+        type = 1; rot = 0;
+        for (int k = 0; k < 4; ++k) {
+            int same = 0;
+            for (int m = 0; m < 4; ++m) same += (inside[m] == inside[k]);
+            if (same == 1) rot = k;
+        }
+    } else {
+        if (inside[0] == inside[2]) return -1;
+        type = 2;
+        rot = (inside[0] == inside[3]) ? 0 : 3;   /* LL, UL alike: bottom and top edges cut */
+    }
+    int A[4];
+    for (int k = 0; k < 4; ++k) A[k] = (rot + k) & 3;
+    /* cut points: E on A1A2, D on A4A1 (type 1) or A4A3 (type 2) */
+    int eD0 = A[3], eD1 = (type == 1) ? A[0] : A[2], eE0 = A[0], eE1 = A[1];
+    double P[2][2];
+    int ends[2][2] = {{eD0, eD1}, {eE0, eE1}};
+    for (int q = 0; q < 2; ++q) {
+        int u = ends[q][0], v = ends[q][1];
+        int vi = inside[u] ? u : v, vo = inside[u] ? v : u;
+        double t = or_circle_cut(x0 + h * VX[vi], y0 + h * VY[vi], x0 + h * VX[vo], y0 + h * VY[vo], cx, cy, rad);
+        if (!(t > 1e-12 && t < 1.0 - 1e-12)) return -2;
+        P[q][0] = VX[vi] + t * (VX[vo] - VX[vi]);
+        P[q][1] = VY[vi] + t * (VY[vo] - VY[vi]);
+    }
+    const double Dx = P[0][0], Dy = P[0][1], Ex = P[1][0], Ey = P[1][1];
+    int piece[4];
+    for (int k = 0; k < 4; ++k) piece[A[k]] = (k == 0 || (type == 2 && k == 3)) ? 0 : 1;
+    double beta[2];
+    beta[0] = inside[A[0]] ? beta_in : beta_out;
+    beta[1] = inside[A[0]] ? beta_out : beta_in;
+    double C[4][2][4];
+    for (int i = 0; i < 4; ++i) {
+        double S[8][8] = {{0}}, b[8] = {0}, x[8];
+        for (int k = 0; k < 4; ++k) {
+            int o = 4 * piece[k];
+            S[k][o] = 1; S[k][o + 1] = VX[k]; S[k][o + 2] = VY[k]; S[k][o + 3] = VX[k] * VY[k];
+            b[k] = (k == i) ? 1.0 : 0.0;
+        }
+        double cp[3][2] = {{Dx, Dy}, {Ex, Ey}, {0.5 * (Dx + Ex), 0.5 * (Dy + Ey)}};
+        for (int q = 0; q < 3; ++q) {
+            double X = cp[q][0], Y = cp[q][1];
+            S[4 + q][0] = 1; S[4 + q][1] = X; S[4 + q][2] = Y; S[4 + q][3] = X * Y;
+            S[4 + q][4] = -1; S[4 + q][5] = -X; S[4 + q][6] = -Y; S[4 + q][7] = -X * Y;
+        }
+        if (Dx == Ex || Dy == Ey) {
+            for (int k = 0; k < 8; ++k) S[6][k] = 0;
+            S[6][3] = 1; S[6][7] = -1;
+        }
+        double fq = 0.5 * (Ey * Ey - Dy * Dy - Ex * Ex + Dx * Dx);
+        S[7][1] = beta[0] * (Ey - Dy); S[7][2] = -beta[0] * (Ex - Dx); S[7][3] = beta[0] * fq;
+        S[7][5] = -beta[1] * (Ey - Dy); S[7][6] = beta[1] * (Ex - Dx); S[7][7] = -beta[1] * fq;
+        if (or_solve8(S, b, x)) return -3;
+        for (int pc = 0; pc < 2; ++pc)
+            for (int k = 0; k < 4; ++k) C[i][pc][k] = x[4 * pc + k];
+    }
+    /* the reference's sub-triangles, points as indices: 0..3 = A1..A4, 4 = D, 5 = E */
+    double Q[6][2];
+    for (int k = 0; k < 4; ++k) { Q[k][0] = VX[A[k]]; Q[k][1] = VY[A[k]]; }
+    Q[4][0] = Dx; Q[4][1] = Dy; Q[5][0] = Ex; Q[5][1] = Ey;
+    static const int T1[4][4] = {{0, 4, 5, 0}, {1, 5, 2, 1}, {5, 2, 4, 1}, {2, 4, 3, 1}};
+    static const int T2[4][4] = {{0, 5, 3, 0}, {3, 4, 5, 0}, {5, 2, 4, 1}, {2, 5, 1, 1}};
+    const int (*T)[4] = (type == 1) ? T1 : T2;
+    for (int a = 0; a < 16; ++a) { K[a] = 0; M[a] = 0; }
+    const double gq = sqrt(0.6), gw[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+    const double gx[3] = {0.5 * (1 - gq), 0.5, 0.5 * (1 + gq)};
+    for (int st = 0; st < 4; ++st) {
+        const double *p1 = Q[T[st][0]], *p2 = Q[T[st][1]], *p3 = Q[T[st][2]];
+        int pc = T[st][3];
+        double area = 0.5 * fabs((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+        /* stiffness: collapsed 3x3 Gauss-Legendre, exact to total degree 4 */
+        for (int qa = 0; qa < 3; ++qa)
+            for (int qb = 0; qb < 3; ++qb) {
+                double u = gx[qa], v = gx[qb] * (1.0 - u);
+                double w = 2.0 * area * gw[qa] * gw[qb] * (1.0 - u);
+                double X = p1[0] + u * (p2[0] - p1[0]) + v * (p3[0] - p1[0]);
+                double Y = p1[1] + u * (p2[1] - p1[1]) + v * (p3[1] - p1[1]);
+                if (delta == 1) w *= y0 + h * Y;
+                double gxv[4], gyv[4];
+                for (int i = 0; i < 4; ++i) {
+                    gxv[i] = C[i][pc][1] + C[i][pc][3] * Y;
+                    gyv[i] = C[i][pc][2] + C[i][pc][3] * X;
+                }
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) K[i * 4 + j] += w * beta[pc] * (gxv[i] * gxv[j] + gyv[i] * gyv[j]);
+            }
+        /* RHS operator: edge midpoints, weight area/3 (physical area h^2 area) */
+        const double* tv[3] = {p1, p2, p3};
+        for (int g = 0; g < 3; ++g) {
+            double X = 0.5 * (tv[g][0] + tv[(g + 1) % 3][0]), Y = 0.5 * (tv[g][1] + tv[(g + 1) % 3][1]);
+            double w = h * h * area / 3.0;
+            if (delta == 1) w *= y0 + h * Y;
+            double phi[4];
+            for (int i = 0; i < 4; ++i) phi[i] = C[i][pc][0] + C[i][pc][1] * X + C[i][pc][2] * Y + C[i][pc][3] * X * Y;
+            for (int i = 0; i < 4; ++i)
+                for (int k = 0; k < 4; ++k) M[i * 4 + k] += w * phi[i] * phi[k];
+        }
+    }
+    if (coef)
+        for (int i = 0; i < 4; ++i)
+            for (int pc = 0; pc < 2; ++pc)
+                for (int k = 0; k < 4; ++k) coef[(i * 2 + pc) * 4 + k] = C[i][pc][k];
+    if (pts) { pts[0] = Dx; pts[1] = Dy; pts[2] = Ex; pts[3] = Ey; }
+    if (order) for (int k = 0; k < 4; ++k) order[k] = A[k];
+    return type;
+}
+
 /* rhs_i = sum_e int N_i rho_h - (Dirichlet lift) over unknown rows; node_type2: unknown id
  * (1-based) or <= 0 for Dirichlet nodes with values dphi. Global_RHS_PDE_2D.f90:78-140 +
  * Global_RHS_EBC_2D_bjw.f90:66-78 restated by element loops. */

@@ -62,6 +62,10 @@ def lib():
                                 C.c_double, C.c_int, C.POINTER(C.c_int)]
         _lib.or_element_matrices.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p,
                                              C.c_void_p]
+        _lib.or_ife_element.restype = C.c_int
+        _lib.or_ife_element.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
         _lib.or_rhs.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int, C.c_void_p]
         _lib.or_stiffness_apply.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
@@ -213,6 +217,72 @@ def element_matrices(delta, x0, y0, hx, hy):
     K, M = np.zeros(16), np.zeros(16)
     lib().or_element_matrices(delta, x0, y0, hx, hy, _p(K), _p(M))
     return K.reshape(4, 4), M.reshape(4, 4)
+
+
+def ife_element(delta, x0, y0, h, inside, cx, cy, rad, beta_in, beta_out):
+    """One circle-cut element: (type, K 4x4, M 4x4, coef[4 basis][2 pieces][4] in X=(x-x0)/h,
+    Y=(y-y0)/h, cut points D, E in X, Y, local indices of A1..A4)."""
+    K, M, coef, pts = np.zeros(16), np.zeros(16), np.zeros(32), np.zeros(4)
+    order = np.zeros(4, dtype=np.int32)
+    ins = np.ascontiguousarray(inside, dtype=np.int32)
+    t = lib().or_ife_element(delta, x0, y0, h, _p(ins), cx, cy, rad, beta_in, beta_out, _p(K), _p(M), _p(coef),
+                             _p(pts), _p(order))
+    return t, K.reshape(4, 4), M.reshape(4, 4), coef.reshape(4, 2, 4), pts.reshape(2, 2), order
+
+
+def assemble_ife(xs, ys, delta, circles, node_type2, dphi, mistake=None):
+    """Element-by-element restatement of the stiffness on unknowns A, the Dirichlet lift c (A_xt
+    times the Dirichlet values) and the RHS operator B (unknowns x nodes) for a square mesh with
+    circular objects (cx, cy, r, eps, conductor): vacuum elements Q1 with beta 1, elements inside
+    an object Q1 with beta eps, elements cut by a dielectric through ife_element, elements cut by
+    a conductor Q1 (Generate_stiffness_matrix_local_IFE.f90, Global_RHS_PDE_2D.f90).
+    mistake plants an error for the tests that show the checks catch it: "q1_cut" treats the
+    dielectric's cut elements as plain Q1, "swap" swaps the two permittivities on them."""
+    import scipy.sparse as sp
+    nnx, nny = len(xs), len(ys)
+    h = xs[1] - xs[0]
+    nu = int(node_type2.max())
+    inside = np.full(nnx * nny, -1)
+    for k, (cx, cy, r, eps, cond) in enumerate(circles):
+        X, Y = np.meshgrid(xs, ys, indexing="ij")
+        m = ((X - cx) ** 2 + (Y - cy) ** 2 < r * r).ravel() & (inside < 0)
+        inside[m] = k
+    Ar, Ac, Av, Br, Bc, Bv = [], [], [], [], [], []
+    c = np.zeros(nu)
+    types = []
+    for ci in range(nnx - 1):
+        for cj in range(nny - 1):
+            nodes = [ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1]
+            objs = [inside[n] for n in nodes]
+            kin = sum(o >= 0 for o in objs)
+            K, M = element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj])
+            if kin == 4:
+                cx, cy, r, eps, cond = circles[objs[0]]
+                K = K * (1.0 if cond else eps)
+            elif kin > 0:
+                k = next(o for o in objs if o >= 0)
+                cx, cy, r, eps, cond = circles[k]
+                if not cond and mistake != "q1_cut":
+                    b_in, b_out = (1.0, eps) if mistake == "swap" else (eps, 1.0)
+                    t, K2, M2, _, _, _ = ife_element(delta, xs[ci], ys[cj], h, [o >= 0 for o in objs], cx, cy, r,
+                                                     b_in, b_out)
+                    assert t in (1, 2), t
+                    K, M = K2, M2
+                    types.append(t)
+            for a in range(4):
+                u = node_type2[nodes[a]]
+                if u <= 0:
+                    continue
+                for b in range(4):
+                    ub = node_type2[nodes[b]]
+                    if ub > 0:
+                        Ar.append(u - 1); Ac.append(ub - 1); Av.append(K[a, b])
+                    else:
+                        c[u - 1] += K[a, b] * dphi[nodes[b]]
+                    Br.append(u - 1); Bc.append(nodes[b]); Bv.append(M[a, b])
+    A = sp.csr_matrix((Av, (Ar, Ac)), shape=(nu, nu))
+    B = sp.csr_matrix((Bv, (Br, Bc)), shape=(nu, nnx * nny))
+    return A, c, B, types
 
 
 def rhs(xs, ys, delta, node_type2, dphi, rho, nunk):
