@@ -285,6 +285,21 @@ def workload(name: str) -> dict:
         # 71 per vacuum element: 16.8M particles per species (the conductors take 9.7% of the area);
         # both species injected through the left wall as a Maxwellian drifting inward at 0.1 v_th
         return dict(domain=dom, ppc=71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026, inject_drift_vth=0.1)
+    if name == "cfg2":            # 256^2 r-z, a conductor at -1 and a dielectric disk (eps 4), mixed precision
+        n = 256
+        dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1,
+                     objects=[Circle(0.30 * n, 0.45 * n, 0.10 * n, -1.0),
+                              Circle(0.68 * n, 0.50 * n, 0.12 * n, 0.0, conductor=False, eps=4.0)],
+                     wall_dirichlet=(1, 1, 0, 1))
+        return dict(domain=dom, ppc=64, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026)
+    if name.startswith("diel"):   # dielectric parity cases diel<N> (Cartesian), dielrz<N> (r-z)
+        rz = name.startswith("dielrz")
+        n = int(name[6 if rz else 4:] or 24)
+        dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1 if rz else 0,
+                     objects=[Circle(0.27 * n, 0.48 * n, 0.12 * n, -1.0),
+                              Circle(0.66 * n, 0.47 * n, 0.19 * n, 0.0, conductor=False, eps=4.0)],
+                     wall_dirichlet=(1, 1, 0, 1) if rz else (1, 1, 1, 1))
+        return dict(domain=dom, ppc=8, bits=64, solver=0, tol=1e-12, Te=2.0, Ti=0.026)
     if name.startswith("rz"):     # axisymmetric parity cases rz<N>: x axial, y radial, the axis (bottom) natural
         n = int(name[2:] or 16)
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1,

@@ -137,6 +137,7 @@ struct PicState {
     pic_domain_config cfg{};
     MeshParams mesh{};
     int nunknowns = 0;
+    int n_ife_elements = 0;          // elements cut by a dielectric, IFE element matrices
     double L_ref = 0, time_ref = 0, v_ref = 0, phi_ref = 0;
 
     // host copies of set-up arrays (Fortran conventions where exported)

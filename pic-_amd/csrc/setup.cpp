@@ -74,6 +74,135 @@ static void q1_axi(double x0, double y0, double hx, double hy, double K[4][4], d
     (void)x0;
 }
 
+// ------------------------------------------------------------------ IFE interface element
+// A square element (side h, lower-left (x0, y0)) cut by the circle of a dielectric object:
+// bilinear immersed-finite-element basis on the reference square of the reference's element
+// orientation (El_Object_Intersection_info_2D_BL.f90: A1 = the lone vertex for a corner cut;
+// for a cut through two opposite edges A1 = LL when bottom and top are cut, UL when left and
+// right are; E on A1A2, D on A4A1 or A4A3; piece 1 = the side of A1), defined by
+// Retangular_reference_basis_IFE's conditions: nodal values, continuity at D and E, one
+// cross coefficient on both pieces, beta * normal flux continuous at the midpoint of DE.
+// K = sum over pieces beta int grad phi_i . grad phi_j (times r when axisymmetric) by a
+// degree-4 rule, exact here; M = the RHS operator of IFE_RHS_2D (rho_h = sum rho_k phi_k on
+// the reference's sub-triangles, edge-midpoint rule). K and M rows/columns are LL, LR, UR, UL.
+// Returns 1 or 2 (the cut type), -1 for two diagonal vertices inside (no bilinear IFE
+// element), -2 if a cut point lies within 1e-12 h of a vertex.
+static int ife_element(int delta, double x0, double y0, double h, const bool inside[4], double cx, double cy,
+                       double rad, double beta_in, double beta_out, double K[4][4], double M[4][4]) {
+    static const double VX[4] = {0, 1, 1, 0}, VY[4] = {0, 0, 1, 1};
+    int kin = (int)inside[0] + inside[1] + inside[2] + inside[3];
+    int type, rot = 0;
+    if (kin == 1 || kin == 3) {
+        type = 1;
+        for (int k = 0; k < 4; ++k)
+            if (inside[k] != inside[(k + 1) & 3] && inside[k] != inside[(k + 3) & 3]) rot = k;
+    } else {
+        if (inside[0] == inside[2]) return -1;
+        type = 2;
+        rot = (inside[0] == inside[3]) ? 0 : 3;
+    }
+    int A[4];
+    for (int k = 0; k < 4; ++k) A[k] = (rot + k) & 3;
+    // reference frame: local (X, Y) = V[A1] + xi e1 + eta e2
+    const double e1x = VX[A[1]] - VX[A[0]], e1y = VY[A[1]] - VY[A[0]];
+    const double e2x = VX[A[3]] - VX[A[0]], e2y = VY[A[3]] - VY[A[0]];
+    auto toY = [&](double xi, double eta) { return VY[A[0]] + xi * e1y + eta * e2y; };
+    // edge crossing: parameter from vertex u towards vertex v
+    auto cross = [&](int u, int v) {
+        const int vi = inside[u] ? u : v, vo = inside[u] ? v : u;
+        const double px = x0 + h * VX[vi], py = y0 + h * VY[vi];
+        const double dx = h * (VX[vo] - VX[vi]), dy = h * (VY[vo] - VY[vi]);
+        const double fx = px - cx, fy = py - cy;
+        const double a2 = dx * dx + dy * dy, b1 = fx * dx + fy * dy, c0 = fx * fx + fy * fy - rad * rad;
+        const double t = (-b1 + std::sqrt(b1 * b1 - a2 * c0)) / a2;      // from the inside vertex
+        return (vi == u) ? t : 1.0 - t;
+    };
+    const double a = cross(A[0], A[1]);                                      // E = (a, 0)
+    const double b = (type == 1) ? 1.0 - cross(A[3], A[0]) : cross(A[3], A[2]);  // D = (0, b) or (b, 1)
+    if (!(a > 1e-12 && a < 1 - 1e-12 && b > 1e-12 && b < 1 - 1e-12)) return -2;
+    const double Dxi = (type == 1) ? 0.0 : b, Deta = (type == 1) ? b : 1.0, Exi = a, Eeta = 0.0;
+    const double beta1 = inside[A[0]] ? beta_in : beta_out, beta2 = inside[A[0]] ? beta_out : beta_in;
+    const double bp[2] = {beta1, beta2};
+    static const double RX[4] = {0, 1, 1, 0}, RY[4] = {0, 0, 1, 1};
+    auto pieceof = [&](int k) { return (k == 0 || (type == 2 && k == 3)) ? 0 : 1; };
+    // basis coefficients c[k][piece][0..3] of c0 + c1 xi + c2 eta + c3 xi eta for reference vertex k
+    double c[4][2][4];
+    const double mx = 0.5 * (Dxi + Exi), my = 0.5 * (Deta + Eeta);
+    const double nx = Eeta - Deta, ny = -(Exi - Dxi);
+    for (int k = 0; k < 4; ++k) {
+        double S[8][9] = {{0}};
+        for (int m = 0; m < 4; ++m) {
+            const int o = 4 * pieceof(m);
+            S[m][o] = 1; S[m][o + 1] = RX[m]; S[m][o + 2] = RY[m]; S[m][o + 3] = RX[m] * RY[m];
+            S[m][8] = (m == k) ? 1.0 : 0.0;
+        }
+        const double cp[2][2] = {{Dxi, Deta}, {Exi, Eeta}};
+        for (int q = 0; q < 2; ++q) {
+            const double X = cp[q][0], Y = cp[q][1], f[4] = {1, X, Y, X * Y};
+            for (int j = 0; j < 4; ++j) { S[4 + q][j] = f[j]; S[4 + q][4 + j] = -f[j]; }
+        }
+        S[6][3] = 1; S[6][7] = -1;
+        // beta1 grad p1 . n - beta2 grad p2 . n at the midpoint: grad = (c1 + c3 eta, c2 + c3 xi)
+        for (int pc = 0; pc < 2; ++pc) {
+            const double sg = pc == 0 ? bp[0] : -bp[1];
+            S[7][4 * pc + 1] = sg * nx; S[7][4 * pc + 2] = sg * ny; S[7][4 * pc + 3] = sg * (nx * my + ny * mx);
+        }
+        for (int col = 0; col < 8; ++col) {                 // Gauss-Jordan with partial pivoting
+            int pv = col;
+            for (int r = col + 1; r < 8; ++r) if (std::fabs(S[r][col]) > std::fabs(S[pv][col])) pv = r;
+            if (S[pv][col] == 0.0) return -3;
+            if (pv != col) for (int j = 0; j < 9; ++j) std::swap(S[pv][j], S[col][j]);
+            for (int r = 0; r < 8; ++r) {
+                if (r == col || S[r][col] == 0.0) continue;
+                const double f = S[r][col] / S[col][col];
+                for (int j = col; j < 9; ++j) S[r][j] -= f * S[col][j];
+            }
+        }
+        for (int pc = 0; pc < 2; ++pc)
+            for (int j = 0; j < 4; ++j) c[k][pc][j] = S[4 * pc + j][8] / S[4 * pc + j][4 * pc + j];
+    }
+    // sub-triangles of Inter_Tetra_Partition_2D: points 0..3 = A1..A4, 4 = D, 5 = E; last = piece
+    const double P[6][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}, {Dxi, Deta}, {Exi, Eeta}};
+    static const int T1[4][4] = {{0, 4, 5, 0}, {1, 5, 2, 1}, {5, 2, 4, 1}, {2, 4, 3, 1}};
+    static const int T2[4][4] = {{0, 5, 3, 0}, {3, 4, 5, 0}, {5, 2, 4, 1}, {2, 5, 1, 1}};
+    const int (*T)[4] = (type == 1) ? T1 : T2;
+    // symmetric 6-point rule on the triangle, degree 4 (Dunavant 1985)
+    static const double DW[2] = {0.223381589678011, 0.109951743655322};
+    static const double DA[2] = {0.445948490915965, 0.091576213509771};
+    double Kr[4][4] = {{0}}, Mr[4][4] = {{0}};
+    for (int st = 0; st < 4; ++st) {
+        const double* p1 = P[T[st][0]]; const double* p2 = P[T[st][1]]; const double* p3 = P[T[st][2]];
+        const int pc = T[st][3];
+        const double area = 0.5 * std::fabs((p2[0] - p1[0]) * (p3[1] - p1[1]) - (p3[0] - p1[0]) * (p2[1] - p1[1]));
+        for (int g2 = 0; g2 < 2; ++g2)
+            for (int perm = 0; perm < 3; ++perm) {
+                double l[3] = {DA[g2], DA[g2], DA[g2]};
+                l[perm] = 1.0 - 2.0 * DA[g2];
+                const double xi = l[0] * p1[0] + l[1] * p2[0] + l[2] * p3[0];
+                const double eta = l[0] * p1[1] + l[1] * p2[1] + l[2] * p3[1];
+                double w = DW[g2] * area * bp[pc];
+                if (delta == 1) w *= y0 + h * toY(xi, eta);
+                double gx[4], gy[4];
+                for (int k = 0; k < 4; ++k) { gx[k] = c[k][pc][1] + c[k][pc][3] * eta; gy[k] = c[k][pc][2] + c[k][pc][3] * xi; }
+                for (int i = 0; i < 4; ++i)
+                    for (int j = 0; j < 4; ++j) Kr[i][j] += w * (gx[i] * gx[j] + gy[i] * gy[j]);
+            }
+        const double* tv[3] = {p1, p2, p3};
+        for (int e = 0; e < 3; ++e) {
+            const double xi = 0.5 * (tv[e][0] + tv[(e + 1) % 3][0]), eta = 0.5 * (tv[e][1] + tv[(e + 1) % 3][1]);
+            double w = h * h * area / 3.0;
+            if (delta == 1) w *= y0 + h * toY(xi, eta);
+            double ph[4];
+            for (int k = 0; k < 4; ++k) ph[k] = c[k][pc][0] + c[k][pc][1] * xi + c[k][pc][2] * eta + c[k][pc][3] * xi * eta;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) Mr[i][j] += w * ph[i] * ph[j];
+        }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) { K[A[i]][A[j]] = Kr[i][j]; M[A[i]][A[j]] = Mr[i][j]; }
+    return type;
+}
+
 // ------------------------------------------------------------------ domain build
 int setup_build_domain(const pic_domain_config* c) {
     if (c->nnx < 3 || c->nny < 3) return pic_fail(PIC_ERR_ARG, "nnx, nny must be >= 3");
@@ -87,10 +216,12 @@ int setup_build_domain(const pic_domain_config* c) {
     for (int k = 0; k < c->n_objects; ++k) {
         const pic_object& o = c->objects[k];
         if (o.shape != 1 && o.shape != 3) return pic_fail(PIC_ERR_UNSUPPORTED, "object shape must be 1 (circle) or 3 (box)");
-        if (!o.conductor && o.eps != 1.0)
-            return pic_fail(PIC_ERR_UNSUPPORTED, "dielectric objects (eps != 1) need the IFE jump basis, not built");
+        if (!o.conductor && !(o.eps > 0.0)) return pic_fail(PIC_ERR_ARG, "a dielectric needs eps > 0");
+        if (!o.conductor && o.eps != 1.0 && o.shape != 1)
+            return pic_fail(PIC_ERR_UNSUPPORTED, "dielectric objects other than circles (shape 1) are not supported");
     }
     g.cfg = *c;
+    g.n_ife_elements = 0;
     const int nnx = c->nnx, nny = c->nny, nex = nnx - 1, ney = nny - 1;
     const int nnodes = nnx * nny, nelem = nex * ney;
 
@@ -103,6 +234,9 @@ int setup_build_domain(const pic_domain_config* c) {
 
     const double hx = (c->xmax - c->xmin) / (double)nex;
     const double hy = (c->ymax - c->ymin) / (double)ney;
+    for (int k = 0; k < c->n_objects; ++k)
+        if (!c->objects[k].conductor && c->objects[k].eps != 1.0 && std::fabs(hx - hy) > 1e-12 * hx)
+            return pic_fail(PIC_ERR_UNSUPPORTED, "dielectric objects need square elements (hx == hy)");
     g.h_xs.resize(nnx); g.h_ys.resize(nny);
     for (int i = 0; i < nnx; ++i) g.h_xs[i] = c->xmin + hx * (double)i;     // x_s + h_x*(count-1)
     for (int j = 0; j < nny; ++j) g.h_ys[j] = c->ymin + hy * (double)j;
@@ -217,6 +351,36 @@ int setup_build_domain(const pic_domain_config* c) {
             double K[4][4], M[4][4];
             if (c->delta == 0) { q1_stiffness_cart(hxp, hyp, K); q1_mass_cart(hxp, hyp, M); }
             else q1_axi(g.h_xs[ci], g.h_ys[cj], hxp, hyp, K, M);
+            // elements inside or cut by a dielectric: beta = eps inside, the IFE element on the
+            // cut (Generate_stiffness_matrix_local_IFE.f90:79-118, IFE_RHS_2D.f90); conductors keep
+            // Q1 on their cut elements (their inside nodes are Dirichlet)
+            bool ins[4];
+            int kin = 0, ko = -1;
+            for (int a = 0; a < 4; ++a) {
+                const int k = node_obj[nodes[a]];
+                ins[a] = k >= 0;
+                if (k < 0) continue;
+                ++kin;
+                if (ko < 0) ko = k;
+                else if (ko != k) return pic_fail(PIC_ERR_UNSUPPORTED, "an element touches two objects");
+            }
+            if (ko >= 0 && !c->objects[ko].conductor && c->objects[ko].eps != 1.0) {
+                const pic_object& o = c->objects[ko];
+                if (kin == 4) {
+                    for (int a = 0; a < 4; ++a)
+                        for (int b = 0; b < 4; ++b) K[a][b] *= o.eps;
+                } else {
+                    double Ki[4][4], Mi[4][4];
+                    const int t = ife_element(c->delta, g.h_xs[ci], g.h_ys[cj], hxp, ins, o.locations[0][0],
+                                              o.locations[0][1], o.dimensions[0], o.eps, 1.0, Ki, Mi);
+                    if (t == -1) return pic_fail(PIC_ERR_UNSUPPORTED, "a dielectric cuts an element through two diagonal vertices");
+                    if (t == -3) return pic_fail(PIC_ERR_ARG, "singular IFE basis system");
+                    if (t > 0) {                 // t == -2 (a cut within 1e-12 h of a vertex): Q1 kept
+                        std::memcpy(K, Ki, sizeof(K)); std::memcpy(M, Mi, sizeof(M));
+                        ++g.n_ife_elements;
+                    }
+                }
+            }
             for (int a = 0; a < 4; ++a) {
                 int u = g.h_node_type2[nodes[a]];
                 if (u <= 0) continue;

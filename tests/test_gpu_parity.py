@@ -58,6 +58,67 @@ def test_setup_matches_restatements(gpu_lib, pkg, orc, small):
     np.testing.assert_allclose(b_lib, b_or, rtol=1e-12, atol=1e-12)
 
 
+# ------------------------------------------------------------------ dielectric (IFE) set-up and solve
+def _dielectric_system(pl, pkg, orc, case):
+    w = pkg.workload(case)
+    dom = w["domain"]
+    pl.setup(dom, bits=64, solver=0, tol=1e-14)
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    nt = pl.setup_array("NODE_TYPE").reshape(-1, 2)[:, 1]
+    circles = [(o.x, o.y, o.r, o.eps, o.conductor) for o in dom.objects]
+    A, c, B, types = orc.assemble_ife(xs, ys, dom.delta, circles, nt, pl.setup_array("DIRICHLET_PHI"))
+    return dom, xs, ys, ni, nt, A, c, B, types
+
+
+@pytest.mark.parametrize("case", ["diel24", "dielrz24"])
+def test_dielectric_setup_matches_oracle(gpu_lib, pkg, orc, case):
+    """A dielectric disk (eps 4): its inside nodes are unknowns (particles are absorbed there,
+    node_index -2), elements inside carry beta = eps, and the elements the circle cuts take the
+    IFE element matrices; the stiffness, the Dirichlet lift and the RHS operator agree with the
+    oracle's element-by-element assembly within 1e-12 relative (FP64, different quadratures)."""
+    import scipy.sparse as sp
+    pl = gpu_lib
+    say(f"case: dielectric set-up {case}")
+    dom, xs, ys, ni, nt, A, c, B, types = _dielectric_system(pl, pkg, orc, case)
+    assert set(types) == {1, 2} and len(types) > 20
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    d = dom.objects[1]
+    inside = ((X - d.x) ** 2 + (Y - d.y) ** 2 < d.r * d.r).ravel()
+    assert np.all(ni[inside] == -2) and np.all(nt[inside] > 0)
+    rp, col, val = pl.setup_array("A_ROWPTR"), pl.setup_array("A_COL"), pl.setup_array("A_VAL")
+    nu = len(rp) - 1
+    A_lib = sp.csr_matrix((val, col, rp), shape=(nu, nu))
+    scale = abs(A).max()
+    assert abs(A_lib - A).max() <= 1e-12 * scale
+    np.testing.assert_allclose(pl.setup_array("RHS_CONST"), c, rtol=0, atol=1e-12 * max(1.0, abs(c).max()))
+    brp, bcol, bval = pl.setup_array("B_ROWPTR"), pl.setup_array("B_COL"), pl.setup_array("B_VAL")
+    B_lib = sp.csr_matrix((bval, bcol, brp), shape=(nu, dom.nnx * dom.nny))
+    assert abs(B_lib - B).max() <= 1e-12 * abs(B).max()
+    # the jump changes the operator: without the IFE element the cut rows differ
+    A_q1, _, _, _ = orc.assemble_ife(xs, ys, dom.delta, [(o.x, o.y, o.r, o.eps, o.conductor) for o in dom.objects],
+                                     nt, pl.setup_array("DIRICHLET_PHI"), mistake="q1_cut")
+    assert abs(A_q1 - A).max() > 0.1 * scale
+
+
+@pytest.mark.parametrize("case", ["diel24", "dielrz24"])
+def test_dielectric_solve_matches_oracle(gpu_lib, pkg, orc, case):
+    """IFE_Solve_2D on the dielectric domain: the FP64 solution within 1e-10 relative of a
+    direct sparse solve of the oracle's system on the same charge."""
+    import scipy.sparse.linalg as spl
+    pl = gpu_lib
+    say(f"case: dielectric solve {case}")
+    dom, xs, ys, ni, nt, A, c, B, types = _dielectric_system(pl, pkg, orc, case)
+    rng = np.random.default_rng(5)
+    rho = rng.normal(size=dom.nnx * dom.nny)
+    pl.set_field("RHO", rho)
+    pl.IFE_Solve_2D()
+    phi = pl.field("PHI")
+    u = spl.spsolve(A.tocsc(), B @ rho - c)
+    ref = pl.setup_array("DIRICHLET_PHI").copy()
+    ref[nt > 0] = u[nt[nt > 0] - 1]
+    np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-10 * abs(ref).max())
+
+
 # ------------------------------------------------------------------ mover
 @pytest.mark.parametrize("bits", [64, 32])
 def test_moveall_bit_exact(gpu_lib, pkg, orc, bits):
@@ -384,7 +445,7 @@ def test_positioning_dropins(gpu_lib, pkg, orc):
 
 
 # ------------------------------------------------------------------ full step determinism
-@pytest.mark.parametrize("case", ["small32", "rz32"])
+@pytest.mark.parametrize("case", ["small32", "rz32", "diel32", "dielrz32"])
 def test_full_step_deterministic_and_conserving(gpu_lib, pkg, case):
     """Seven full steps (mover, deposit, solve, PPR, MCC with ionisation products, a cell sort
     every 3 steps) twice from the same state: identical bits, the live count consistent; on
