@@ -64,7 +64,10 @@ typedef struct {
     double dimensions[2];
     double locations[4][2];   /* C order: locations[k][d] == Fortran Locations(k+1,d+1) */
     double phi;               /* normalised potential of a conductor */
-    double eps;               /* relative permittivity (Global_Beta of the object region) */
+    double eps;               /* relative permittivity (Global_Beta of the object region); a
+                                 dielectric (conductor 0, eps != 1) must be a circle on square
+                                 elements: its cut elements take the bilinear IFE element
+                                 (Generate_stiffness_matrix_local_IFE.f90, IFE_RHS_2D.f90) */
 } pic_object;
 
 /* Synthetic domain description (what INPUT/mesh.inp + object.inp + pic.inp carry,
@@ -118,7 +121,9 @@ int  pic_set_precision(int particle_bits, int solver_mode);
  * Global_RHS_EBC_2D_bjw (as a precomputed matrix and constant vector) and the PPR
  * point patches of Get_PointPatch (Get_PointPatch.f90:83-590) produce, and uploads
  * them to the device. Uniform rectangular Q1 meshes without refinement (the shipped
- * IDG_inf.inp: repeat_refinement = 0). */
+ * IDG_inf.inp: repeat_refinement = 0); elements cut by a dielectric circle carry IFE
+ * element matrices. PIC_ERR_UNSUPPORTED: a dielectric box, a dielectric on non-square
+ * elements, an element touching two objects or cut through two diagonal vertices. */
 int  pic_setup_domain(const pic_domain_config* cfg);
 
 /* Sizes of the set-up problem. */
