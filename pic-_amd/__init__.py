@@ -652,9 +652,15 @@ def sparse_from_csr(rowptr0, col0, val) -> tuple:
 
 def species_weight(dom: Domain, ppc: int, n0: float = 1.0) -> float:
     """Real particles per macro-particle giving density n0*n_ref at ppc per element:
-    rho_s = Weight*qs*ppc/(L_ref^2 n_ref h^2) (GetParChg_2D.f90:896)."""
+    rho_s = Weight*qs*ppc/(L_ref^2 n_ref h^2) (GetParChg_2D.f90:896); r-z (per element on
+    average, the loader following the element volumes): Weight = n0 n_ref V / N with
+    V = pi (ymax^2 - ymin^2)(xmax - xmin) L_ref^3 (GetParChg_2D.f90:899-905)."""
     L = dom.normalisation()["L_ref"]
     h = (dom.xmax - dom.xmin) / (dom.nnx - 1)
+    if dom.delta == 1:
+        nel = (dom.nnx - 1) * (dom.nny - 1)
+        vol = np.pi * (dom.ymax ** 2 - dom.ymin ** 2) * (dom.xmax - dom.xmin) * L ** 3
+        return n0 * dom.n_ref * vol / (ppc * nel)
     return n0 * dom.n_ref * (L * h) ** 2 / ppc
 
 

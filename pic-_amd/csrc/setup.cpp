@@ -628,7 +628,8 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 }
 
 // ------------------------------------------------------------------ particle loader
-// ppc particles per vacuum element, elements in sort order; uniform position inside the element,
+// ppc particles per vacuum element (r-z: ppc on average, in proportion to the element's
+// volume), elements in sort order; uniform position inside the element,
 // Maxwellian velocity (three Box-Muller normals) at temp_eV. Particles falling inside an
 // object are dropped (the reference's AdjustBoundary_2D pre-filter, Main_IFE_Test_2.f90:218).
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
@@ -650,12 +651,21 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return pic_sort_key(a, m.ney, nty) < pic_sort_key(b, m.ney, nty);
     });
+    // axisymmetric: an element's count follows its volume share (ppc on average), so the
+    // density is uniform as with the reference's volume-uniform draw over the domain
+    const double vol_all = m.ymax * m.ymax - m.ymin * m.ymin;
+    const int stride = m.delta == 1 ? 2 * ppc + 1 : ppc;
     for (int e : order) {
         if (g.h_element_index[e] == -2) continue;
         int ci = e / m.ney, cj = e % m.ney;
         double x0 = g.h_xs[ci], x1 = g.h_xs[ci + 1], y0 = g.h_ys[cj], y1 = g.h_ys[cj + 1];
-        for (int k = 0; k < ppc; ++k) {
-            uint32_t slot = (uint32_t)((uint64_t)e * ppc + k);
+        int count = ppc;
+        if (m.delta == 1) {
+            const double q = (double)ppc * m.ney * (y1 * y1 - y0 * y0) / vol_all;
+            count = std::min(stride, (int)(std::floor((ci + 1) * q) - std::floor(ci * q)));
+        }
+        for (int k = 0; k < count; ++k) {
+            uint32_t slot = (uint32_t)((uint64_t)e * stride + k);
             pic_stream s; pic_stream_init(&s, seed, slot, 0u, tag);
             double px = x0 + (x1 - x0) * pic_stream_u01(&s);
             double uy = pic_stream_u01(&s);
