@@ -64,10 +64,11 @@ typedef struct {
     double dimensions[2];
     double locations[4][2];   /* C order: locations[k][d] == Fortran Locations(k+1,d+1) */
     double phi;               /* normalised potential of a conductor */
-    double eps;               /* relative permittivity (Global_Beta of the object region); a
-                                 dielectric (conductor 0, eps != 1) must be a circle on square
-                                 elements: its cut elements take the bilinear IFE element
-                                 (Generate_stiffness_matrix_local_IFE.f90, IFE_RHS_2D.f90) */
+    double eps;               /* relative permittivity (Global_Beta of the object region); an
+                                 object with eps != 1 must be a circle on square elements: its
+                                 cut elements take the bilinear IFE element
+                                 (Generate_stiffness_matrix_local_IFE.f90, IFE_RHS_2D.f90), a
+                                 conductor keeping its inside nodes Dirichlet */
 } pic_object;
 
 /* Synthetic domain description (what INPUT/mesh.inp + object.inp + pic.inp carry,

@@ -234,8 +234,9 @@ def assemble_ife(xs, ys, delta, circles, node_type2, dphi, mistake=None):
     """Element-by-element restatement of the stiffness on unknowns A, the Dirichlet lift c (A_xt
     times the Dirichlet values) and the RHS operator B (unknowns x nodes) for a square mesh with
     circular objects (cx, cy, r, eps, conductor): vacuum elements Q1 with beta 1, elements inside
-    an object Q1 with beta eps, elements cut by a dielectric through ife_element, elements cut by
-    a conductor Q1 (Generate_stiffness_matrix_local_IFE.f90, Global_RHS_PDE_2D.f90).
+    an object Q1 with beta eps, elements cut by an object of eps != 1 through ife_element (a
+    conductor's inside nodes stay Dirichlet), cut elements of eps = 1 Q1
+    (Generate_stiffness_matrix_local_IFE.f90, Global_RHS_PDE_2D.f90).
     mistake plants an error for the tests that show the checks catch it: "q1_cut" treats the
     dielectric's cut elements as plain Q1, "swap" swaps the two permittivities on them."""
     import scipy.sparse as sp
@@ -258,11 +259,11 @@ def assemble_ife(xs, ys, delta, circles, node_type2, dphi, mistake=None):
             K, M = element_matrices(delta, xs[ci], ys[cj], xs[ci + 1] - xs[ci], ys[cj + 1] - ys[cj])
             if kin == 4:
                 cx, cy, r, eps, cond = circles[objs[0]]
-                K = K * (1.0 if cond else eps)
+                K = K * eps
             elif kin > 0:
                 k = next(o for o in objs if o >= 0)
                 cx, cy, r, eps, cond = circles[k]
-                if not cond and mistake != "q1_cut":
+                if eps != 1.0 and mistake != "q1_cut":
                     b_in, b_out = (1.0, eps) if mistake == "swap" else (eps, 1.0)
                     t, K2, M2, _, _, _ = ife_element(delta, xs[ci], ys[cj], h, [o >= 0 for o in objs], cx, cy, r,
                                                      b_in, b_out)

@@ -217,8 +217,8 @@ int setup_build_domain(const pic_domain_config* c) {
         const pic_object& o = c->objects[k];
         if (o.shape != 1 && o.shape != 3) return pic_fail(PIC_ERR_UNSUPPORTED, "object shape must be 1 (circle) or 3 (box)");
         if (!o.conductor && !(o.eps > 0.0)) return pic_fail(PIC_ERR_ARG, "a dielectric needs eps > 0");
-        if (!o.conductor && o.eps != 1.0 && o.shape != 1)
-            return pic_fail(PIC_ERR_UNSUPPORTED, "dielectric objects other than circles (shape 1) are not supported");
+        if (o.eps != 1.0 && o.shape != 1)
+            return pic_fail(PIC_ERR_UNSUPPORTED, "objects with eps != 1 must be circles (shape 1)");
     }
     g.cfg = *c;
     g.n_ife_elements = 0;
@@ -235,8 +235,8 @@ int setup_build_domain(const pic_domain_config* c) {
     const double hx = (c->xmax - c->xmin) / (double)nex;
     const double hy = (c->ymax - c->ymin) / (double)ney;
     for (int k = 0; k < c->n_objects; ++k)
-        if (!c->objects[k].conductor && c->objects[k].eps != 1.0 && std::fabs(hx - hy) > 1e-12 * hx)
-            return pic_fail(PIC_ERR_UNSUPPORTED, "dielectric objects need square elements (hx == hy)");
+        if (c->objects[k].eps != 1.0 && std::fabs(hx - hy) > 1e-12 * hx)
+            return pic_fail(PIC_ERR_UNSUPPORTED, "objects with eps != 1 need square elements (hx == hy)");
     g.h_xs.resize(nnx); g.h_ys.resize(nny);
     for (int i = 0; i < nnx; ++i) g.h_xs[i] = c->xmin + hx * (double)i;     // x_s + h_x*(count-1)
     for (int j = 0; j < nny; ++j) g.h_ys[j] = c->ymin + hy * (double)j;
@@ -351,9 +351,9 @@ int setup_build_domain(const pic_domain_config* c) {
             double K[4][4], M[4][4];
             if (c->delta == 0) { q1_stiffness_cart(hxp, hyp, K); q1_mass_cart(hxp, hyp, M); }
             else q1_axi(g.h_xs[ci], g.h_ys[cj], hxp, hyp, K, M);
-            // elements inside or cut by a dielectric: beta = eps inside, the IFE element on the
-            // cut (Generate_stiffness_matrix_local_IFE.f90:79-118, IFE_RHS_2D.f90); conductors keep
-            // Q1 on their cut elements (their inside nodes are Dirichlet)
+            // elements inside or cut by an object of eps != 1: beta = eps inside, the IFE element on
+            // the cut (Generate_stiffness_matrix_local_IFE.f90:79-118, IFE_RHS_2D.f90). A conductor
+            // keeps its inside nodes Dirichlet; with eps = 1 its cut elements are plain Q1
             bool ins[4];
             int kin = 0, ko = -1;
             for (int a = 0; a < 4; ++a) {
@@ -364,7 +364,7 @@ int setup_build_domain(const pic_domain_config* c) {
                 if (ko < 0) ko = k;
                 else if (ko != k) return pic_fail(PIC_ERR_UNSUPPORTED, "an element touches two objects");
             }
-            if (ko >= 0 && !c->objects[ko].conductor && c->objects[ko].eps != 1.0) {
+            if (ko >= 0 && c->objects[ko].eps != 1.0) {
                 const pic_object& o = c->objects[ko];
                 if (kin == 4) {
                     for (int a = 0; a < 4; ++a)
