@@ -182,7 +182,7 @@ def timed_steps(pl, steps: int) -> dict:
             "ms_per_step": 1e3 * elapsed / steps, "phase": phase, "ks": ks, "solve": solve}
 
 
-def rooflines(run: dict, steps: int, nnodes: int, traffic: dict) -> tuple:
+def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0) -> tuple:
     """Rooflines of the push kernel and of the plain CG iteration kernel (the two single
     kernels that can dominate a step). Algorithmic bytes per launch:
       push: 44 B per particle (read x, y, vx, vy, vz, cell; write x, y, vx, vy, cell_next, FP32)
@@ -194,7 +194,9 @@ def rooflines(run: dict, steps: int, nnodes: int, traffic: dict) -> tuple:
         b = 44.0 * (0.5 * (run["n0"] + run["n1"])) / 2
         out.append((ks["push_ms"], {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "traffic": traffic.get("k_push_deposit"),
-                                    "kernel": "k_push_deposit<float,1>", "bytes_per_launch": b, "ms_per_launch": ms,
+                                    # r-z: every slot moves through the general mover
+                                    "kernel": "k_push_walls<float>" if delta == 1 else "k_push_deposit<float,1>",
+                                    "bytes_per_launch": b, "ms_per_launch": ms,
                                     "launches": ks["push_launches"], "share_of_step": ks["push_ms"] / phase[7]}))
     if ks["cg_launches"]:
         ms = ks["cg_ms"] / ks["cg_launches"]
@@ -280,8 +282,8 @@ def main() -> int:
         pl.step(1)
         log(f"bench: timing {args.steps} steps (plain solver)")
         plain_run = timed_steps(pl, args.steps)
-    roofs = rooflines(main_run, args.steps, nnodes, traffic)
-    roofs_plain = rooflines(plain_run, args.steps, nnodes, traffic)
+    roofs = rooflines(main_run, args.steps, nnodes, traffic, dom.delta)
+    roofs_plain = rooflines(plain_run, args.steps, nnodes, traffic, dom.delta)
     other = [r for r in roofs_plain if r["kernel"] != roofs[0]["kernel"]]
     ph = main_run["phase"]
     out = {
