@@ -238,8 +238,8 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes_each_way": x.numel() * 4, "launches": 5}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); the sharded design is "
-              "described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); a run over several "
+              "GPUs would shard the mesh into bands of element columns, particles following their elements", file=sys.stderr)
         return 2
 
     import __graft_entry__ as ge

@@ -25,7 +25,16 @@
  *                              the uniform Q1 mesh
  *   or_mcc_*                   MCCPublic.F90, TypeMCC.f90, MCCElectron.F90, MCCIon.F90,
  *                              MCCChi.f90 with the counter-based selection of the build
- * Parity status of each function is listed in DESIGN.md.
+ *   or_inject*                 IMPIC/InjectBeams_2D.f90, Loadv.f90, InjectPositioning.f90
+ *   or_sincos / or_atan        DCOS / DSIN / DATAN of the axisymmetric mover
+ *   or_ife_element             the bilinear IFE interface element (Linear_IFE_Basis_Coeff_2D,
+ *                              Gauss_integration_local_stiffness_IFE, IFE_RHS_2D)
+ * Every function is pinned by checks under tests/ that do not retype it: published
+ * known-answer vectors (Philox), closed forms and invariants (uniform-field pushes, charge
+ * conservation, quadratic reproduction of PPR, direct solves for PCG, the defining
+ * conditions and the circular-interface convergence of the IFE element), brute force on
+ * small inputs (lookup, MCC selection), the C library within 1-2 ulp (trigonometry) and
+ * files written by a Fortran compiler (the file formats).
  */
 #include <math.h>
 #include <stdint.h>
