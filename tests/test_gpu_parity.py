@@ -125,6 +125,29 @@ def test_dielectric_solve_matches_oracle(gpu_lib, pkg, orc, case):
     np.testing.assert_allclose(phi, ref, rtol=0, atol=1e-10 * abs(ref).max())
 
 
+def test_dielectric_limits_are_refused(gpu_lib, pkg):
+    """Configurations without a bilinear IFE element are refused at set-up with
+    PIC_ERR_UNSUPPORTED (host-side checks): a dielectric box, a dielectric on non-square
+    elements, an element touching two objects."""
+    import ctypes as C
+    say("case: dielectric limits")
+    base = pkg.workload("diel24")["domain"]
+    c = base.to_c()
+    c.objects[1].shape = 3
+    c.objects[1].locations[0][0], c.objects[1].locations[0][1] = 12.0, 8.0
+    c.objects[1].locations[1][0], c.objects[1].locations[1][1] = 18.0, 14.0
+    assert pkg.lib().pic_setup_domain(C.byref(c)) == 9
+    c = base.to_c()
+    c.ymax = 36.0                                   # hy = 1.5 hx
+    assert pkg.lib().pic_setup_domain(C.byref(c)) == 9
+    c = base.to_c()
+    c.objects[0].locations[0][0] = 10.9             # the conductor now overlaps the disk's elements
+    c.objects[0].dimensions[0] = 4.0
+    assert pkg.lib().pic_setup_domain(C.byref(c)) == 9
+    # the unmodified domain is accepted afterwards
+    assert pkg.lib().pic_setup_domain(C.byref(base.to_c())) == 0
+
+
 def test_axisymmetric_load_follows_volume(gpu_lib, pkg):
     """r-z loads: element (i, j) receives floor((i+1) q_j) - floor(i q_j) particles, q_j = ppc ney
     (y_{j+1}^2 - y_j^2) / (ymax^2 - ymin^2), so rows away from the object hold counts in
