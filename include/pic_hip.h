@@ -124,7 +124,9 @@ int  pic_set_precision(int particle_bits, int solver_mode);
  * them to the device. Uniform rectangular Q1 meshes without refinement (the shipped
  * IDG_inf.inp: repeat_refinement = 0); elements cut by a dielectric circle carry IFE
  * element matrices. PIC_ERR_UNSUPPORTED: a dielectric box, a dielectric on non-square
- * elements, an element touching two objects or cut through two diagonal vertices. */
+ * elements, an element touching two objects or cut through two diagonal vertices.
+ * A new domain starts a new run: species, MCC tables, the learned network and the step
+ * counter are reset (pic_set_step sets the counter for a restart). */
 int  pic_setup_domain(const pic_domain_config* cfg);
 
 /* Sizes of the set-up problem. */

@@ -225,11 +225,12 @@ int pic_set_precision(int particle_bits, int solver_mode) {
 int pic_setup_domain(const pic_domain_config* cfg) {
     REQUIRE_INIT();
     if (!cfg) return pic_fail(PIC_ERR_ARG, "cfg is NULL");
-    // species, MCC tables and the learned network belong to a domain (sized by its nodes)
+    // species, MCC tables, the learned network and the step counter belong to a domain's run
     HIPCHK(hipStreamSynchronize(g.stream));
     learn_free();
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
     g.nspecies = 0;
+    g.step = 0;
     g.mcc_ready = false;
     g.have_domain = false;
     TRY(setup_build_domain(cfg));

@@ -148,6 +148,34 @@ def test_dielectric_limits_are_refused(gpu_lib, pkg):
     assert pkg.lib().pic_setup_domain(C.byref(base.to_c())) == 0
 
 
+def test_empty_species_steps(gpu_lib, pkg):
+    """A species with no particles (ions not loaded) steps, sorts and deposits: its moments
+    are zero and the electrons' step is unchanged by its presence."""
+    pl = gpu_lib
+    say("case: empty species small24")
+    outs = []
+    for with_ions in (False, True):
+        w = pkg.workload("small24")
+        dom = w["domain"]
+        pl.setup(dom, bits=64, solver=0, tol=1e-12)
+        nel = (dom.nnx - 1) * (dom.nny - 1)
+        weight = pkg.species_weight(dom, 8)
+        pl.set_species(0, pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], weight, 2 * pkg.EVTOK, nel * 16)
+        if with_ions:
+            pl.set_species(1, pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], weight, 0.026 * pkg.EVTOK,
+                           nel * 16)
+        pl.load_maxwellian(0, 8, 2.0, 5)
+        pl.set_run_params(2, 77, 0)
+        pl.sort_particles()
+        pl.initial_field()
+        pl.step(3)
+        if with_ions:
+            assert pl.counts(1)["nlive"] == 0
+        outs.append((pl.particles(0)["x"].copy(), pl.field("RHO").copy(), pl.field("PHI").copy()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 def test_axisymmetric_load_follows_volume(gpu_lib, pkg):
     """r-z loads: element (i, j) receives floor((i+1) q_j) - floor(i q_j) particles, q_j = ppc ney
     (y_{j+1}^2 - y_j^2) / (ymax^2 - ymin^2), so rows away from the object hold counts in
