@@ -169,13 +169,15 @@ int  pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed);
 
 /* Load / store a ParticleBundle in the reference's AoS record layout
  * (ParticleBundle%PO(1:NPar), ParticleBundle.f90:7-43). Load converts to device SoA;
- * Location is taken as given (1-based element). */
+ * Location (1-based element) must be the element GetParChg_2D stores for the position
+ * (PIC_ERR_POSITION otherwise, nothing loaded). */
 int  pic_load_bundle(int isp, const pic_particle_one* po, int64_t npar);
 int  pic_store_bundle(int isp, pic_particle_one* po, int64_t capacity, int64_t* npar);
 
 /* SoA load/store (x, y, vx, vy, vz in normalised units; loc 1-based, may be NULL
- * on load: then every particle is located as GetParChg_2D would). Store writes the
- * live particles in slot order. */
+ * on load: then every particle is located as GetParChg_2D would; a given loc must equal
+ * that element, PIC_ERR_POSITION otherwise). Store writes the live particles in slot
+ * order. */
 int  pic_load_particles(int isp, int64_t n, const double* x, const double* y,
                         const double* vx, const double* vy, const double* vz,
                         const int32_t* loc);
@@ -252,7 +254,10 @@ int  pic_set_pcg_params(double tol, int32_t maxit);
 int  pic_GetEField_SIDG_PPR(void);
 
 /* MCC, MCC_jw/code/mc/MCCPublic.F90:35-111, against the registered gas for all
- * species with a table; new particles appended as in MCCAddNewParticle (:173-189). */
+ * species with a table; new particles appended as in MCCAddNewParticle (:173-189).
+ * MCC ends a time step: on success the step counter (the key of every random stream)
+ * advances by one, so a caller of the drop-ins in the reference's order draws new numbers
+ * every step, the same numbers pic_step draws. */
 int  pic_MCC(void);
 
 /* MCCBundleInit (MCCInit.F90:14-152) for the analytic argon cross sections: builds the
@@ -300,7 +305,9 @@ int  pic_initial_field(void);
 /* Cell sort / compaction period in steps (0 = never) and the RNG seed of the run. */
 int  pic_set_run_params(int32_t sort_every, uint64_t seed, int32_t mcc_enabled);
 int  pic_sort_particles(void);
-/* Current step counter (the RNG counter word) and its setter (restart). */
+/* Current step counter (the RNG counter word) and its setter (restart). The counter
+ * advances after MCC (pic_MCC, or inside pic_step), and also when a species is moved a
+ * second time at the same counter value (a caller that runs no MCC). */
 int  pic_get_step(int64_t* step);
 int  pic_set_step(int64_t step);
 

@@ -263,8 +263,17 @@ int or_locate(int nnx, int nny, const double* xs, const double* ys, int64_t n, c
  * (Energy = 0.5*Mass*|v|^2*v_ref^2/JtoeV, ParticleOne.f90:333-341, :128) accumulated to
  * element corners (:823-843), corners summed to nodes in element order (:849-866).
  * fix != 0: every term quantised to round(term*2^40) and summed as int64 (the build's
- * accumulation, order independent); fix == 0: the reference's FP64 sums in particle
- * order. Output node sums w_node, e_node (FP64), before normalisation. */
+ * accumulation, order independent); a W*Energy term >= 2^11 eV is held as its whole-eV part
+ * (summed apart) plus the quantised fraction, so no single term can wrap the 2^40 sum, and
+ * the node value is sum_lo * 2^-40 + sum_hi. fix == 0: the reference's FP64 sums in
+ * particle order. Output node sums w_node, e_node (FP64), before normalisation. */
+static void or_fix_split(double v, long long* lo, long long* hi) {
+    if (fabs(v) < 2048.0) { *hi = 0; *lo = llrint(v * 1099511627776.0); return; }
+    double ip = floor(v);
+    *hi = (long long)ip;
+    *lo = llrint((v - ip) * 1099511627776.0);
+}
+
 int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, int delta, int64_t n,
                const void* x, const void* y, const void* vx, const void* vy, const void* vz, const int* cell,
                double mass, double v_ref, int fix, double* w_node, double* e_node) {
@@ -275,6 +284,7 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
     double* ek = (double*)calloc((size_t)4 * nelem, sizeof(double));
     long long* qs = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
     long long* qe = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
+    long long* qh = (long long*)calloc((size_t)4 * nelem, sizeof(long long));
     /* threads only for the fixed-point sums, which are order independent; the FP64 sums
      * stay sequential in particle order */
     const int nt = fix ? or_threads : 1;
@@ -306,15 +316,19 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
             W[2] = dx * (R * R - R1 * R1) / den; W[3] = xm * (R * R - R1 * R1) / den;
         }
         for (int k = 0; k < 4; ++k) {
-            if (fix && nt > 1) {
-                long long a = llrint(W[k] * 1099511627776.0), b = llrint(W[k] * Energy * 1099511627776.0);
+            if (fix) {
+                long long a = llrint(W[k] * 1099511627776.0), b, h;
+                or_fix_split(W[k] * Energy, &b, &h);
+                if (nt > 1) {
 #pragma omp atomic
-                qs[4 * (size_t)e + k] += a;
+                    qs[4 * (size_t)e + k] += a;
 #pragma omp atomic
-                qe[4 * (size_t)e + k] += b;
-            } else if (fix) {
-                qs[4 * (size_t)e + k] += llrint(W[k] * 1099511627776.0);
-                qe[4 * (size_t)e + k] += llrint(W[k] * Energy * 1099511627776.0);
+                    qe[4 * (size_t)e + k] += b;
+#pragma omp atomic
+                    qh[4 * (size_t)e + k] += h;
+                } else {
+                    qs[4 * (size_t)e + k] += a; qe[4 * (size_t)e + k] += b; qh[4 * (size_t)e + k] += h;
+                }
             } else {
                 rs[4 * (size_t)e + k] = rs[4 * (size_t)e + k] + W[k];
                 ek[4 * (size_t)e + k] = ek[4 * (size_t)e + k] + W[k] * Energy;
@@ -324,32 +338,35 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
     if (fix) {
         long long* nw = (long long*)calloc(nnodes, sizeof(long long));
         long long* ne = (long long*)calloc(nnodes, sizeof(long long));
+        long long* nh = (long long*)calloc(nnodes, sizeof(long long));
         if (nt > 1) {
             /* node-side gather of the (up to) four corner sums touching each node */
 #pragma omp parallel for num_threads(nt) schedule(static)
             for (int i = 0; i < nnodes; ++i) {
                 int ni = i / nny, nj = i % nny;
-                long long sw = 0, se = 0;
+                long long sw = 0, se = 0, sh = 0;
                 for (int k = 0; k < 4; ++k) {
                     int ci = ni - (k == 1 || k == 2), cj = nj - (k >= 2);   /* corner k of element (ci,cj) is node i */
                     if (ci < 0 || ci >= nex || cj < 0 || cj >= ney) continue;
                     size_t e = (size_t)ci * ney + cj;
-                    sw += qs[4 * e + k]; se += qe[4 * e + k];
+                    sw += qs[4 * e + k]; se += qe[4 * e + k]; sh += qh[4 * e + k];
                 }
-                nw[i] = sw; ne[i] = se;
+                nw[i] = sw; ne[i] = se; nh[i] = sh;
             }
         } else {
             for (int e = 0; e < nelem; ++e) {
                 int ci = e / ney, cj = e % ney;
                 int ht[4] = {ci * nny + cj, (ci + 1) * nny + cj, (ci + 1) * nny + cj + 1, ci * nny + cj + 1};
-                for (int k = 0; k < 4; ++k) { nw[ht[k]] += qs[4 * (size_t)e + k]; ne[ht[k]] += qe[4 * (size_t)e + k]; }
+                for (int k = 0; k < 4; ++k) {
+                    nw[ht[k]] += qs[4 * (size_t)e + k]; ne[ht[k]] += qe[4 * (size_t)e + k]; nh[ht[k]] += qh[4 * (size_t)e + k];
+                }
             }
         }
         for (int i = 0; i < nnodes; ++i) {
             w_node[i] = (double)nw[i] * (1.0 / 1099511627776.0);
-            e_node[i] = (double)ne[i] * (1.0 / 1099511627776.0);
+            e_node[i] = (double)ne[i] * (1.0 / 1099511627776.0) + (double)nh[i];
         }
-        free(nw); free(ne);
+        free(nw); free(ne); free(nh);
     } else {
         for (int i = 0; i < nnodes; ++i) { w_node[i] = 0.0; e_node[i] = 0.0; }
         for (int e = 0; e < nelem; ++e) {
@@ -361,7 +378,7 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
             }
         }
     }
-    free(rs); free(ek); free(qs); free(qe);
+    free(rs); free(ek); free(qs); free(qe); free(qh);
     return 0;
 }
 

@@ -8,6 +8,7 @@
 
 PicState g;
 
+
 int pic_fail(int code, const std::string& msg) {
     g.err = msg;
     return code;
@@ -56,6 +57,9 @@ static int check_err_flags() {
     if (e & PIC_EBIT_EKRHO) return pic_fail(PIC_ERR_POSITION, "Ek_tot /= 0 with rho == 0 (GetParChg_2D.f90:887-890)");
     if (e & PIC_EBIT_MCCLOOP) return pic_fail(PIC_ERR_BOUNDARY, "MCC rejection loop did not end");
     if (e & PIC_EBIT_SAMPLER) return pic_fail(PIC_ERR_BOUNDARY, "injection rejection sampler did not end");
+    if (e & PIC_EBIT_OVERFLOW)
+        return pic_fail(PIC_ERR_OVERFLOW, "deposition out of the fixed-point range (a weight x energy term that is "
+                                          "not finite or >= 2^62 eV, or a node sum past its int64 range)" + where);
     return pic_fail(PIC_ERR_HIP, "device error flag " + std::to_string(e));
 }
 
@@ -152,7 +156,7 @@ __global__ void k_d2f_n(int64_t n, const double* a, float* b) { int64_t i = (int
 
 static void free_species(int s) {
     SpeciesHost& S = g.sp[s];
-    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.th, S.cell, S.cell_next, S.acc_w, S.acc_e, S.counters, S.prob,
+    void* ptrs[] = {S.x, S.y, S.vx, S.vy, S.vz, S.th, S.cell, S.cell_next, S.acc_w, S.acc_e, S.acc_h, S.counters, S.prob,
                     S.rtype, S.thr, S.mcc_events, S.spare[0], S.spare[1], S.spare[2], S.spare[3], S.spare[4],
                     S.spare[5]};
     for (void* p : ptrs) if (p) hipFree(p);
@@ -231,6 +235,7 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
     g.nspecies = 0;
     g.step = 0;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.moved[s] = false;
     g.mcc_ready = false;
     g.have_domain = false;
     TRY(setup_build_domain(cfg));
@@ -468,6 +473,7 @@ int pic_set_species(int isp, const pic_species_params* p, int64_t capacity) {
     }
     TRY(dalloc(&S.cell, (size_t)capacity)); TRY(dalloc(&S.cell_next, (size_t)capacity));
     TRY(dalloc(&S.acc_w, (size_t)g.mesh.nnodes)); TRY(dalloc(&S.acc_e, (size_t)g.mesh.nnodes));
+    TRY(dalloc(&S.acc_h, (size_t)g.mesh.nnodes));
     TRY(dalloc(&S.counters, 16)); TRY(dalloc(&S.mcc_events, 32));
     S.acc_dirty = false; S.pending = false; S.defined = true;
     g.nspecies = std::max(g.nspecies, isp + 1);
@@ -481,6 +487,25 @@ int pic_load_particles(int isp, int64_t n, const double* x, const double* y, con
     if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
     SpeciesHost& S = g.sp[isp];
     if (n < 0 || n > S.capacity) return pic_fail(PIC_ERR_CAPACITY, "more particles than capacity");
+    if (n > 0 && (!x || !y || !vx || !vy || !vz)) return pic_fail(PIC_ERR_ARG, "NULL particle array");
+    if (loc) {
+        // A given Location must be the element GetParChg_2D would store for the position as
+        // the device keeps it (rounded to the particle precision): every kernel indexes the
+        // element arrays with it, and the gather of the next MoveOne uses it as its element.
+        for (int64_t i = 0; i < n; ++i) {
+            const double xr = g.particle_bits == 32 ? (double)(float)x[i] : x[i];
+            const double yr = g.particle_bits == 32 ? (double)(float)y[i] : y[i];
+            int bad = 0, ci = 0, cj = 0;
+            const int e = pic_locate_ij(xr, yr, g.mesh, &bad, &ci, &cj);
+            if (loc[i] < 1 || loc[i] > g.mesh.nelem || bad || loc[i] != e + 1) {
+                char buf[256];
+                snprintf(buf, sizeof buf, "particle %lld: Location %d is not the element containing (%.17g, %.17g)%s",
+                         (long long)i, (int)loc[i], xr, yr, bad ? " (outside the mesh)" : "");
+                S.nslots = 0; S.nlive = 0; S.pending = false;
+                return pic_fail(PIC_ERR_POSITION, buf);
+            }
+        }
+    }
     S.nslots = n; S.nlive = n; S.pending = false;
     if (n == 0) return PIC_OK;
     double* d = nullptr;
@@ -656,15 +681,28 @@ int pic_get_mcc_events(int isp, int64_t* events, int nmax) {
 }
 
 // ----------------------------------------------------------------------- per step
+// The step counter keys every random stream (kappa re-emission, MCC selection and
+// collision). It advances where a time step ends: after MCC (the last random consumer of
+// the reference's step, MCC_jw OneStep.F90:49-67), or, for a caller that runs no MCC, when
+// a species moves a second time at the same counter value. pic_step and a caller of the
+// drop-ins in the reference's order (MoveAll per species, GetParChg_2D, IFE_Solve_2D,
+// GetEField_SIDG_PPR, InjectBeams_2D, MCC) therefore draw the same numbers.
+static void advance_step() {
+    g.step++;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.moved[s] = false;
+}
+
 int pic_MoveAll(int isp, int delta, int PushFlag) {
     REQUIRE_DOMAIN();
     if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
     if (PushFlag != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "only the explicit mover (PushFlag = 0) is on this path");
     if (delta != g.mesh.delta) return pic_fail(PIC_ERR_ARG, "delta differs from the set-up domain");
     SpeciesHost& S = g.sp[isp];
+    if (g.moved[isp]) advance_step();
     if (S.pending) S.acc_dirty = true;   // a repeated MoveAll discards the previous pending deposit
     TRY(launch_push_deposit(isp, true));
     S.pending = true;
+    g.moved[isp] = true;
     return PIC_OK;
 }
 
@@ -796,7 +834,9 @@ int pic_MCC(void) {
     for (int s = 0; s < g.nspecies; ++s)
         if (g.sp[s].pending) return pic_fail(PIC_ERR_STATE, "MCC between MoveAll and GetParChg_2D");
     TRY(run_mcc());
-    return check_err_flags();
+    TRY(check_err_flags());
+    advance_step();
+    return PIC_OK;
 }
 
 int pic_set_run_params(int32_t sort_every, uint64_t seed, int32_t mcc_enabled) {
@@ -813,7 +853,12 @@ int pic_sort_particles(void) {
 }
 
 int pic_get_step(int64_t* step) { *step = g.step; return PIC_OK; }
-int pic_set_step(int64_t step) { if (step < 0) return pic_fail(PIC_ERR_ARG, "step < 0"); g.step = step; return PIC_OK; }
+int pic_set_step(int64_t step) {
+    if (step < 0) return pic_fail(PIC_ERR_ARG, "step < 0");
+    g.step = step;
+    for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.moved[s] = false;
+    return PIC_OK;
+}
 
 int pic_initial_field(void) {
     REQUIRE_DOMAIN();
@@ -903,7 +948,7 @@ int pic_step(int nsteps) {
         HIPCHK(hipEventRecord(g.ev[5], g.stream));
         if (g.mcc_enabled && g.mcc_ready) TRY(run_mcc());
         HIPCHK(hipEventRecord(g.ev[6], g.stream));
-        g.step++;
+        advance_step();
         if (g.sort_every > 0 && g.step % g.sort_every == 0) TRY(run_sort_all());
         HIPCHK(hipEventRecord(g.ev[7], g.stream));
         HIPCHK(hipEventSynchronize(g.ev[7]));
@@ -976,20 +1021,31 @@ int pic_set_field(int which, const double* src, int64_t n) {
 }
 
 // ----------------------------------------------------------------------- drop-ins
+// Host check of the reference's SPARSE array (allocates nothing): row pointers SROW(1..N+1)
+// start at 1, never decrease and end within NZ + 1; every used JCOL lies in 1..N. Returns
+// the stored-entry count SROW(N+1) - 1 in *nnz.
+static int check_sparse(const pic_sparse* matrix, int NZ, int N, int* nnz) {
+    if (matrix[0].SROW != 1 || matrix[N].SROW - 1 > NZ) return pic_fail(PIC_ERR_ARG, "SROW row pointers inconsistent");
+    for (int i = 0; i < N; ++i)
+        if (matrix[i + 1].SROW < matrix[i].SROW) return pic_fail(PIC_ERR_ARG, "SROW row pointers decrease");
+    const int n = matrix[N].SROW - 1;
+    for (int k = 0; k < n; ++k)
+        if (matrix[k].JCOL < 1 || matrix[k].JCOL > N) return pic_fail(PIC_ERR_ARG, "JCOL out of range");
+    *nnz = n;
+    return PIC_OK;
+}
+
 int pic_MY_JPCG_Solver(const pic_sparse* matrix, double* R, double* X, const double* M, int NZ, int N, int* ITER) {
     REQUIRE_INIT();
-    if (!matrix || !R || !X || !M || N <= 0 || NZ < N + 1) return pic_fail(PIC_ERR_ARG, "bad MY_JPCG_Solver arguments");
+    if (!matrix || !R || !X || !M || !ITER || N <= 0 || NZ < N + 1)
+        return pic_fail(PIC_ERR_ARG, "bad MY_JPCG_Solver arguments");
+    int nnz = 0;
+    TRY(check_sparse(matrix, NZ, N, &nnz));
     std::vector<int> rowptr(N + 1);
     for (int i = 0; i <= N; ++i) rowptr[i] = matrix[i].SROW - 1;
-    if (rowptr[0] != 0 || rowptr[N] > NZ) return pic_fail(PIC_ERR_ARG, "SROW row pointers inconsistent");
-    int nnz = rowptr[N];
     std::vector<int> col(nnz);
     std::vector<double> val(nnz);
-    for (int k = 0; k < nnz; ++k) {
-        col[k] = matrix[k].JCOL - 1;
-        val[k] = matrix[k].K;
-        if (col[k] < 0 || col[k] >= N) return pic_fail(PIC_ERR_ARG, "JCOL out of range");
-    }
+    for (int k = 0; k < nnz; ++k) { col[k] = matrix[k].JCOL - 1; val[k] = matrix[k].K; }
     int *drp = nullptr, *dcol = nullptr;
     double *dval = nullptr, *ddiag = nullptr, *db = nullptr, *dx = nullptr, *dr = nullptr;
     HIPCHK(hipMalloc(&drp, (N + 1) * sizeof(int))); HIPCHK(hipMalloc(&dcol, nnz * sizeof(int)));

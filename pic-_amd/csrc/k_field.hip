@@ -511,10 +511,10 @@ static inline int pcg_blocks(int n) { return (n + PCG_BLOCK - 1) / PCG_BLOCK; }
 // iterations in chunks and reads the done flag between chunks.
 template <typename T, class Op>
 static int pcg_run(int n, const Op& A, const T* diag, const T* b, T* x, T* r, T* z, T* p, T* p2, T* q, double tol,
-                   int maxit, double* partials, int pstride, int* iters, int* ierr) {
+                   int maxit, double* partials, int pstride, int* iters, int* ierr, double* scal = nullptr) {
     int nb = pcg_blocks(n);
     if (nb > pstride) return pic_fail(PIC_ERR_STATE, "PCG partial buffer too small");
-    double* scal = g.d_scal;
+    if (!scal) scal = g.d_scal;
     double h2[2] = {tol * tol, (double)maxit};
     HIPCHK(hipMemcpyAsync(scal + S_TOL2, h2, 2 * sizeof(double), hipMemcpyHostToDevice, g.stream));
     hipLaunchKernelGGL((k_pcg_init<T, Op>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, A, diag, b, x, r, z, scal,

@@ -49,6 +49,7 @@ template <typename R> struct PushArgs {
     double mass, v_ref;       // species mass (kg) and v_ref for the energy moment
     uint64_t seed; uint32_t step; uint32_t tag_emit;
     long long* acc_w; long long* acc_e;
+    long long* acc_h;         // whole-eV parts of the rare W E quanta >= 2^11 eV (pic_fix_split)
     unsigned long long* counters;
     unsigned int* err;
     PicErrInfo* errinfo;
@@ -183,6 +184,21 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
 }
 
 struct PushEvents { int emit, right, object, reflect, deleted; };
+
+// The whole-eV parts of a particle's energy quanta (pic_fix8 returned fx != 0): global
+// atomics on the element's corner nodes (rare: W E >= 2^11 eV), and the range flag.
+// The nodes come from the element index (not from registers the common path would have to
+// keep live).
+__device__ inline void deposit_hi(int fx, int e, const MeshParams& m, const long long hv[4], long long* acc_h,
+                                  unsigned int* err) {
+    if (fx & 2) atomicOr(err, PIC_EBIT_OVERFLOW);
+    int ci, cj;
+    pic_elem_ij(e, m, &ci, &cj);
+    const int n1 = ci * m.nny + cj, nd[4] = {n1, n1 + m.nny, n1 + m.nny + 1, n1 + 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (hv[k] != 0) atomicAdd((unsigned long long*)&acc_h[nd[k]], (unsigned long long)hv[k]);
+}
 
 // MoveAll for one particle (MCCInterface.f90:591-615 with MoveOne + AdjustParticleBoundary):
 // state in registers, returns false when the particle is deleted.
@@ -454,8 +470,15 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
                 double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
                 energy = energy * a.v_ref * a.v_ref;
                 energy = energy / PIC_JTOEV;
+                long long hv[4];
+                int fx = 0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) { v[q] = pic_fix(W[q]); v[4 + q] = pic_fix(W[q] * energy); }
+                for (int q = 0; q < 4; ++q) {
+                    v[q] = pic_fix(W[q]);
+                    if (!pic_fix_split(W[q] * energy, &v[4 + q], &hv[q])) fx |= 2;
+                    if (hv[q] != 0) fx |= 1;
+                }
+                if (fx) deposit_hi(fx, key, m, hv, a.acc_h, a.err);
             }
             a.cell_next[p] = key;
         }
@@ -583,7 +606,7 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
             energy = energy * a.v_ref * a.v_ref;
             energy = energy / PIC_JTOEV;
             long long qv[8];
-            pic_fix8(W, energy, qv);
+            pic_fix8(W, energy, qv, next, m, a.acc_h, a.err);
             if (next != rkey) {
                 if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
                 rkey = next;
@@ -645,7 +668,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.kappa_c = (1.0 * 2.0 - 1.5) / beta;
     a.mass = S.p.mass_kg; a.v_ref = g.v_ref;
     a.seed = g.seed; a.step = (uint32_t)g.step; a.tag_emit = pic_stream_tag(isp, PIC_RNG_EMIT);
-    a.acc_w = S.acc_w; a.acc_e = S.acc_e;
+    a.acc_w = S.acc_w; a.acc_e = S.acc_e; a.acc_h = S.acc_h;
     a.counters = S.counters; a.err = g.d_err;
     a.errinfo = (PicErrInfo*)g.d_errinfo; a.species = isp;
     a.wall_list = g.d_wall_list; a.wall_count = g.d_wall_count;
@@ -663,6 +686,7 @@ int launch_zero_acc(int isp) {
     int blocks = (int)((n + 255) / 256);
     hipLaunchKernelGGL(k_zero_ll, dim3(blocks), dim3(256), 0, g.stream, S.acc_w, n);
     hipLaunchKernelGGL(k_zero_ll, dim3(blocks), dim3(256), 0, g.stream, S.acc_e, n);
+    hipLaunchKernelGGL(k_zero_ll, dim3(blocks), dim3(256), 0, g.stream, S.acc_h, n);
     HIPCHK(hipGetLastError());
     S.acc_dirty = false;
     return PIC_OK;
@@ -734,6 +758,7 @@ struct FinalArgs {
     int nnodes, nspecies, delta;
     long long* acc_w[PIC_MAX_SPECIES];
     long long* acc_e[PIC_MAX_SPECIES];
+    long long* acc_h[PIC_MAX_SPECIES];
     double weight_qs[PIC_MAX_SPECIES];
     double weight[PIC_MAX_SPECIES];
     double L_ref, n_ref;
@@ -749,10 +774,11 @@ __global__ void k_finalize(FinalArgs a) {
     double vol = a.vol[n];
     double rho = 0.0;
     for (int s = 0; s < a.nspecies; ++s) {
-        long long qw = a.acc_w[s][n], qe = a.acc_e[s][n];
-        a.acc_w[s][n] = 0; a.acc_e[s][n] = 0;
+        long long qw = a.acc_w[s][n], qe = a.acc_e[s][n], qh = a.acc_h[s][n];
+        a.acc_w[s][n] = 0; a.acc_e[s][n] = 0; a.acc_h[s][n] = 0;
+        if (qw < 0 || qe < 0 || qh < 0) atomicOr(a.err, PIC_EBIT_OVERFLOW);
         double w = (double)qw * PIC_FIX_INV;
-        double e = (double)qe * PIC_FIX_INV;
+        double e = (double)qe * PIC_FIX_INV + (double)qh;
         if (e != 0.0 && w == 0.0) atomicOr(a.err, PIC_EBIT_EKRHO);
         double rs, ek;
         if (a.delta == 0) {
@@ -776,7 +802,7 @@ int launch_finalize_deposit() {
     for (int s = 0; s < g.nspecies; ++s) {
         SpeciesHost& S = g.sp[s];
         if (!S.defined) return pic_fail(PIC_ERR_STATE, "species indices must be contiguous from 0");
-        a.acc_w[s] = S.acc_w; a.acc_e[s] = S.acc_e;
+        a.acc_w[s] = S.acc_w; a.acc_e[s] = S.acc_e; a.acc_h[s] = S.acc_h;
         a.weight_qs[s] = S.p.weight * S.qs;     // Weight * qs(isp), evaluated left to right
         a.weight[s] = S.p.weight;
         S.acc_dirty = false;

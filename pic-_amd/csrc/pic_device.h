@@ -104,6 +104,19 @@ PIC_HD unsigned int pic_sort_key(int e, int ney, int nty) {
     return tile * (SORT_TILE * SORT_TILE) + (unsigned int)((ci % SORT_TILE) * SORT_TILE + (cj % SORT_TILE));
 }
 
+// Device error bits.
+enum {
+    PIC_EBIT_POSITION = 1u,
+    PIC_EBIT_BOUNDARY = 2u,
+    PIC_EBIT_NONFINITE = 4u,
+    PIC_EBIT_WRONGCROSS = 8u,
+    PIC_EBIT_EKRHO = 16u,
+    PIC_EBIT_OVERFLOW = 32u,
+    PIC_EBIT_CAPACITY = 64u,
+    PIC_EBIT_MCCLOOP = 128u,
+    PIC_EBIT_SAMPLER = 256u
+};
+
 // Fixed-point quantum of deposition: value * 2^40 rounded to nearest (even), as int64.
 // Integer sums are exact and independent of order, so the node moments are
 // bit-identical whatever the particle order, atomic interleaving or kernel fusion.
@@ -120,13 +133,38 @@ __device__ inline long long pic_fix(double v) {
     return __double2ll_rn(x);
 }
 
-// The eight quanta of one particle's deposit: W[q] and W[q] * energy for the four corners,
-// each equal to pic_fix of the same value. When |W| < 2 and |energy| < 1000 every value v
-// has |v| < 2^11, so v + 1.5*2^12 lies in [2^12, 2^13) where the spacing is 2^-40: the add
-// rounds v*2^40 to the nearest (even) integer, held in the low mantissa bits (the same
-// rounding as pic_fix, one FP64 add and one 32-bit subtract on the high word per value).
-// The test is one uniform branch per particle instead of one per value.
-__device__ inline void pic_fix8(const double W[4], double energy, long long q[8]) {
+// Range of the fixed-point deposition. Each node keeps three int64 sums per species: the
+// weights W and the weight x energy quanta W E at scale 2^40, and the whole-eV parts of the
+// rare quanta with W E >= 2^11 eV (a kappa re-emission draws energies up to ~1e16 eV), whose
+// fraction stays in the 2^40 sum. The node's energy moment is the exact integer sum of all
+// quanta, held as lo + 2^40 hi, so it does not depend on the order of the additions and no
+// single particle can wrap the 2^40 sum. Out of range (PIC_EBIT_OVERFLOW): a non-finite
+// quantum, W E >= 2^62 eV, or a node sum that comes out negative (every quantum is >= 0).
+#define PIC_FIX_SPLIT 2048.0          // 2^11: W E at or above it is split into whole eV + fraction
+#define PIC_FIX_HI_LIMIT 0x1p62
+
+// round(v 2^40) as lo + 2^40 hi: hi = 0 below 2^11, otherwise hi = floor(v) (exact integer) and
+// lo = round((v - floor(v)) 2^40) (v - floor(v) is exact: floor(v) lies within [v/2, v]).
+// Returns false when v is out of range (both parts 0 then).
+__device__ inline bool pic_fix_split(double v, long long* lo, long long* hi) {
+    if (fabs(v) < PIC_FIX_SPLIT) { *hi = 0; *lo = pic_fix(v); return true; }
+    if (!(fabs(v) < PIC_FIX_HI_LIMIT)) { *hi = 0; *lo = 0; return false; }
+    const double ip = floor(v);
+    *hi = (long long)ip;
+    *lo = pic_fix(v - ip);
+    return true;
+}
+
+// The quanta of one particle's deposit on element e: q[k] = round(W[k] 2^40) and q[4 + k] +
+// 2^40 hi = round(W[k] energy 2^40) for the four corners (LL, LR, UR, UL). When |W| < 2 and
+// |energy| < 1000 every value v has |v| < 2^11, so v + 1.5*2^12 lies in [2^12, 2^13) where the
+// spacing is 2^-40: the add rounds v*2^40 to the nearest (even) integer, held in the low
+// mantissa bits (the same rounding as pic_fix, one FP64 add and one 32-bit subtract on the
+// high word per value), and every hi part is 0. The test is one uniform branch per particle
+// instead of one per value. On the other path a nonzero hi part goes straight to acc_h of its
+// corner node (a global atomic: rare), and a value out of range sets PIC_EBIT_OVERFLOW.
+__device__ inline void pic_fix8(const double W[4], double energy, long long q[8], int e, const MeshParams& m,
+                                long long* acc_h, unsigned int* err) {
     const double wmax = fmax(fmax(fabs(W[0]), fabs(W[1])), fmax(fabs(W[2]), fabs(W[3])));
     if (wmax < 2.0 && fabs(energy) < 1000.0) {
         const long long off = __double_as_longlong(0x1.8p12);
@@ -135,25 +173,22 @@ __device__ inline void pic_fix8(const double W[4], double energy, long long q[8]
             q[k] = __double_as_longlong(__dadd_rn(W[k], 0x1.8p12)) - off;
             q[4 + k] = __double_as_longlong(__dadd_rn(__dmul_rn(W[k], energy), 0x1.8p12)) - off;   // never fused
         }
-    } else {
+        return;
+    }
+    int ci, cj;
+    pic_elem_ij(e, m, &ci, &cj);
+    const int n1 = ci * m.nny + cj;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { q[k] = pic_fix(W[k]); q[4 + k] = pic_fix(W[k] * energy); }
+    for (int k = 0; k < 4; ++k) {
+        long long hi;
+        q[k] = pic_fix(W[k]);
+        if (!pic_fix_split(W[k] * energy, &q[4 + k], &hi)) atomicOr(err, PIC_EBIT_OVERFLOW);
+        const int node = n1 + (k == 1 || k == 2 ? m.nny : 0) + (k >= 2 ? 1 : 0);
+        if (hi != 0) atomicAdd((unsigned long long*)&acc_h[node], (unsigned long long)hi);
     }
 }
 #endif
 
-// Device error bits.
-enum {
-    PIC_EBIT_POSITION = 1u,
-    PIC_EBIT_BOUNDARY = 2u,
-    PIC_EBIT_NONFINITE = 4u,
-    PIC_EBIT_WRONGCROSS = 8u,
-    PIC_EBIT_EKRHO = 16u,
-    PIC_EBIT_OVERFLOW = 32u,
-    PIC_EBIT_CAPACITY = 64u,
-    PIC_EBIT_MCCLOOP = 128u,
-    PIC_EBIT_SAMPLER = 256u
-};
 
 // First failing particle, recorded for the error message (first writer wins).
 struct PicErrInfo {

@@ -86,6 +86,7 @@ struct SpeciesHost {
     void* spare[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // sort targets, swapped with x..vz, th
     long long* acc_w = nullptr;         // nnodes fixed-point sum W
     long long* acc_e = nullptr;         // nnodes fixed-point sum W*E
+    long long* acc_h = nullptr;         // nnodes whole-eV parts of the W*E quanta >= 2^11 eV
     bool acc_dirty = false;
     bool pending = false;               // MoveAll has filled acc and cell_next
     unsigned long long* counters = nullptr;  // device [16]
@@ -233,6 +234,9 @@ struct PicState {
     uint64_t seed = 12345;
     int mcc_enabled = 1;
     int64_t step = 0;
+    // species moved at the current step counter: a second MoveAll of the same species before
+    // the counter advances starts a new step (the counter keys every random stream)
+    bool moved[PIC_MAX_SPECIES] = {false, false, false, false};
     unsigned char* d_elem_clean = nullptr;   // MeshParams::elem_clean
     unsigned short* d_elem_nbr = nullptr;    // MeshParams::elem_nbr
     InjectHost inj[PIC_MAX_SPECIES];

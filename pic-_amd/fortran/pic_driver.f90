@@ -74,8 +74,7 @@ program pic_driver
         call pic_check(pic_GetParChg_2D(0_c_int), 'GetParChg_2D')
         call pic_check(pic_IFE_Solve_2D(c_null_ptr, c_null_ptr, cfg%nnx, cfg%nny, 0_c_int, dble(it)), 'IFE_Solve_2D')
         call pic_check(pic_GetEField_SIDG_PPR(), 'GetEField_SIDG_PPR')
-        call pic_check(pic_MCC(), 'MCC')
-        call pic_check(pic_set_step(int(it, c_int64_t)), 'pic_set_step')
+        call pic_check(pic_MCC(), 'MCC')     ! ends the step: the random-stream counter advances
         if (mod(it, 20) == 0) call pic_check(pic_sort_particles(), 'sort')
         do isp = 0, 1
             call pic_check(pic_get_counts(int(isp, c_int), nlive(isp), nslots, counters), 'counts')

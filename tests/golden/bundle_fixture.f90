@@ -2,7 +2,9 @@
 ! bundle dump: WRITE(unit) NPar, NParNormal / WRITE(unit) Charge, Mass, Weight / one
 ! WRITE(unit) of each particle record (X, Y, Z, Vx, Vy, Vz, Ax, Ay, Az, Wq as REAL(8) and
 ! Location as the default INTEGER). Compiled and run by make_bundle_fixture.sh; its output,
-! bundle_flang.dat, is the fixture that pins the Fortran unformatted record layout.
+! bundle_flang.dat, is the fixture that pins the Fortran unformatted record layout. On a
+! 24 x 24 mesh of unit cells at n_ref = 1e21 m^-3, T_ref = 1 eV (L_ref = 2.3509e-7 m) the
+! particles lie in elements 97, 193 and 289, the Locations written.
 program bundle_fixture
     implicit none
     type :: rec_t
@@ -18,7 +20,7 @@ program bundle_fixture
     mass = 9.1093837d-31
     weight = 2.5d6
     do i = 1, 3
-        po(i)%X = 1.0d-5 * i
+        po(i)%X = 1.0d-6 * i
         po(i)%Y = 0.25d0 * i
         po(i)%Z = 0.5d0 * i
         po(i)%Vx = 1.0d5 * i
@@ -28,7 +30,7 @@ program bundle_fixture
         po(i)%Ay = 0.0d0
         po(i)%Az = 0.0d0
         po(i)%Wq = 0.0d0
-        po(i)%Location = 10 * i + 1
+        po(i)%Location = 96 * i + 1
     end do
     open(10, form='UNFORMATTED', file='bundle_flang.dat')
     write(10) npar, npar_normal

@@ -45,10 +45,10 @@ def test_bundle_fixture_layout():
     assert (npar, nnorm) == (3, 1000)
     assert phys == (-1.60217663e-19, 9.1093837e-31, 2.5e6)
     i = np.arange(1, 4)
-    np.testing.assert_array_equal(parts[:, 0], 1.0e-5 * i)
+    np.testing.assert_array_equal(parts[:, 0], 1.0e-6 * i)
     np.testing.assert_array_equal(parts[:, 3], 1.0e5 * i)
     np.testing.assert_array_equal(parts[:, 4], -2.0e5 * i)
-    np.testing.assert_array_equal(loc, 10 * i + 1)
+    np.testing.assert_array_equal(loc, 96 * i + 1)
 
 
 def test_fortran_e15_matches_flang(pkg):
@@ -132,10 +132,12 @@ def test_bundle_from_flang_fixture(gpu_lib, pkg):
     q = pl.particles(0)
     norm = dom.normalisation()
     i = np.arange(1, 4)
-    np.testing.assert_allclose(q["x"], 1.0e-5 * i / norm["L_ref"], rtol=1e-12)
+    np.testing.assert_allclose(q["x"], 1.0e-6 * i / norm["L_ref"], rtol=1e-12)
     np.testing.assert_array_equal(q["y"], 0.25 * i)
     np.testing.assert_allclose(q["vy"], -2.0e5 * i / norm["v_ref"], rtol=1e-12)
-    np.testing.assert_array_equal(q["loc"], 10 * i + 1)
+    # the Locations written are the elements holding the positions (the load checks them)
+    np.testing.assert_array_equal(q["loc"], 96 * i + 1)
+    np.testing.assert_array_equal(pl.locate(q["x"], q["y"]), q["loc"])
 
 
 @pytest.mark.gpu

@@ -323,6 +323,31 @@ def test_deposit_particle_on_node(orc):
     assert w[2 * 5 + 3] == 1.0 and w.sum() == 1.0
 
 
+def test_deposit_extreme_energies_exact(orc):
+    """The heavy tail of the kappa re-emission (energies 0.5 kT (1/R - 1), up to ~1e16 eV for
+    the smallest R): terms of weight x energy >= 2^11 eV keep their whole-eV part apart, so
+    the fixed-point node sums equal the FP64 sums to rounding and stay order independent
+    (one int64 at 2^40 holds only 2^23 eV)."""
+    xs = np.arange(5, dtype=float)
+    rng = np.random.default_rng(9)
+    m = 400
+    x, y = rng.random(m) * 4, rng.random(m) * 4
+    v = rng.normal(size=(3, m))
+    v[0, [3, 50, 51, 200]] = [3.0e3, 4.0e5, 2.0e7, 9.0e7]      # 4.5 MeV ... 2.6e15 eV
+    cell, _ = orc.locate(xs, xs, x, y)
+    wq, eq = orc.deposit(64, xs, xs, 0, x, y, *v, cell, 9.1095e-31, 4.19e5, fix=True)
+    wf, ef = orc.deposit(64, xs, xs, 0, x, y, *v, cell, 9.1095e-31, 4.19e5, fix=False)
+    assert ef.max() > 2.0 ** 40
+    np.testing.assert_allclose(eq, ef, rtol=1e-13, atol=1e-10)
+    perm = rng.permutation(m)
+    wp, ep = orc.deposit(64, xs, xs, 0, x[perm], y[perm], *(v[:, perm]), cell[perm], 9.1095e-31, 4.19e5, fix=True)
+    assert np.array_equal(ep, eq)
+    # planted: one 2^40-scale word per term (no whole-eV part) wraps on these energies
+    q = np.array([int(np.rint(t * 2.0 ** 40)) & (2 ** 64 - 1) for t in [ef.max()]])
+    wrapped = (q[0] - 2 ** 64 if q[0] >= 2 ** 63 else q[0]) * 2.0 ** -40
+    assert not np.isclose(wrapped, ef.max(), rtol=1e-6)
+
+
 # ------------------------------------------------------------------ MCC
 def test_mcc_table_cumulative(orc):
     ns, nr = 101, 3
