@@ -345,6 +345,18 @@ int  pic_get_phase_times(double* ms, int n);
 int  pic_MY_JPCG_Solver(const pic_sparse* matrix, double* R, double* X, const double* M,
                         int NZ, int N, int* ITER);
 
+/* No counterpart in the reference: pic_MY_JPCG_Solver for callers that cannot allocate
+ * device memory (work set up around HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments, results and status codes as pic_MY_JPCG_Solver;
+ * the matrix, the vectors and the partial sums are staged in a fixed device scratch array
+ * of the library (54.6 MB per GPU, held until the process ends) instead of new
+ * allocations, and no allocating routine is called. Limits: N <= 262144 unknowns and
+ * NZ <= 2359296 stored entries (PIC_ERR_CAPACITY beyond, nothing run). Every call on one
+ * GPU shares that GPU's scratch, which keeps the last call's contents: calls that overlap
+ * on one GPU (several host threads) race. */
+int  pic_MY_JPCG_Solver_noalloc(const pic_sparse* matrix, double* R, double* X, const double* M,
+                                int NZ, int N, int* ITER);
+
 /* Get_ParticleEField_PPR (PPR/Get_ParticleEField_PPR.f90:1-50) for n points against the
  * current device coef_ppr; EleIndex 1-based. */
 int  pic_Get_ParticleEField_PPR(int64_t n, const double* x, const double* y,

@@ -136,6 +136,7 @@ _PROTOS = {
     "pic_set_field": (C.c_int, [C.c_int, _dp, C.c_int64]),
     "pic_get_phase_times": (C.c_int, [_dp, C.c_int]),
     "pic_MY_JPCG_Solver": (C.c_int, [_P(Sparse), _dp, _dp, _dp, C.c_int, C.c_int, _ip]),
+    "pic_MY_JPCG_Solver_noalloc": (C.c_int, [_P(Sparse), _dp, _dp, _dp, C.c_int, C.c_int, _ip]),
     "pic_Get_ParticleEField_PPR": (C.c_int, [C.c_int64, _dp, _dp, _ip, _dp, _dp]),
     "pic_InjectPositioning": (C.c_int, [C.c_int64, _dp, _dp, _ip]),
     "pic_ParticlePositioning": (C.c_int, [C.c_int64, _dp, _dp, _ip, _ip]),
@@ -634,6 +635,14 @@ class PicLibrary:
         R, X updated in place. Returns ITER."""
         it = C.c_int()
         check(lib().pic_MY_JPCG_Solver(matrix, _d(R), _d(X), _d(M), NZ, N, C.byref(it)))
+        return it.value
+
+    @staticmethod
+    def MY_JPCG_Solver_noalloc(matrix, R, X, M, NZ: int, N: int) -> int:
+        """MY_JPCG_Solver staged in the library's fixed device scratch (no allocation;
+        N <= 262144, NZ <= 2359296). R, X updated in place. Returns ITER."""
+        it = C.c_int()
+        check(lib().pic_MY_JPCG_Solver_noalloc(matrix, _d(R), _d(X), _d(M), NZ, N, C.byref(it)))
         return it.value
 
 

@@ -8,6 +8,35 @@
 
 PicState g;
 
+// ---------------------------------------------------------------- allocation-free scratch
+// Device scratch of pic_MY_JPCG_Solver_noalloc: the staged SPARSE array, the eight PCG
+// vectors, the block partial sums and the scalar block, for systems of up to
+// PIC_NOALLOC_MAX_N unknowns and PIC_NOALLOC_MAX_NZ stored entries (54.6 MB, held by every
+// GPU the library runs on until the process ends). In host code `scratch` names a host-side
+// stand-in, not the device array: the device address comes from hipGetSymbolAddress, once
+// per device at pic_init, and only that address is handed to copies and kernels.
+#define PIC_NOALLOC_MAX_N 262144
+#define PIC_NOALLOC_MAX_NZ (9 * PIC_NOALLOC_MAX_N)
+#define PIC_NOALLOC_PSTRIDE (PIC_NOALLOC_MAX_N / 256 + 2)
+static constexpr size_t kNoallocMatBytes = (size_t)PIC_NOALLOC_MAX_NZ * sizeof(pic_sparse);
+static constexpr size_t kNoallocVecBytes = (size_t)PIC_NOALLOC_MAX_N * sizeof(double);
+static constexpr size_t kNoallocPartBytes = ((size_t)4 * PIC_NOALLOC_PSTRIDE * sizeof(double) + 255) / 256 * 256;
+static constexpr size_t kNoallocScalBytes = 64 * sizeof(double);
+static constexpr size_t kNoallocBytes = kNoallocMatBytes + 8 * kNoallocVecBytes + kNoallocPartBytes + kNoallocScalBytes;
+static __device__ __align__(256) double scratch[kNoallocBytes / sizeof(double)];
+#define PIC_MAX_DEVICES 64
+static void* g_scratch_dev[PIC_MAX_DEVICES] = {nullptr};   // device address of `scratch` per device
+
+static int scratch_lookup(int device) {
+    if (device < 0 || device >= PIC_MAX_DEVICES) return pic_fail(PIC_ERR_ARG, "device index beyond the scratch table");
+    if (g_scratch_dev[device]) return PIC_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return pic_fail(PIC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = ptr;
+    return PIC_OK;
+}
 
 int pic_fail(int code, const std::string& msg) {
     g.err = msg;
@@ -197,6 +226,10 @@ int pic_init(int device) {
     HIPCHK(hipMemset(g.d_errinfo, 0, sizeof(PicErrInfo)));
     for (int i = 0; i < 16; ++i) HIPCHK(hipEventCreate(&g.ev[i]));
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&g.ev_push[s][i]));
+    {
+        int rc = scratch_lookup(device);   // with `device` current
+        if (rc) return rc;
+    }
     g.inited = true;
     return PIC_OK;
 }
@@ -206,6 +239,8 @@ int pic_finalize(void) {
     hipDeviceSynchronize();
     learn_free();
     hipDeviceReset();
+    // the reset unloads the code object: the scratch array gets a new address next time
+    if (g.device >= 0 && g.device < PIC_MAX_DEVICES) g_scratch_dev[g.device] = nullptr;
     g = PicState();
     return PIC_OK;
 }
@@ -1071,6 +1106,44 @@ int pic_MY_JPCG_Solver(const pic_sparse* matrix, double* R, double* X, const dou
     }
     hipFree(part); hipFree(drp); hipFree(dcol); hipFree(dval); hipFree(ddiag); hipFree(db); hipFree(dx); hipFree(dr);
     if (rc) return rc;
+    if (ierr == 1) return pic_fail(PIC_ERR_PCG_MAXITER, "MY_PCG IERR=1 (ITER left unset, as in the reference)");
+    *ITER = iters - 1;     // MY_PCG.f90:129-133: ITER = i-1
+    if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "MY_PCG IERR=2/3");
+    return PIC_OK;
+}
+
+int pic_MY_JPCG_Solver_noalloc(const pic_sparse* matrix, double* R, double* X, const double* M, int NZ, int N,
+                               int* ITER) {
+    REQUIRE_INIT();
+    if (!matrix || !R || !X || !M || !ITER || N <= 0 || NZ < N + 1)
+        return pic_fail(PIC_ERR_ARG, "bad MY_JPCG_Solver arguments");
+    if (N > PIC_NOALLOC_MAX_N || NZ > PIC_NOALLOC_MAX_NZ) {
+        char buf[200];
+        snprintf(buf, sizeof buf, "MY_JPCG_Solver_noalloc: N = %d, NZ = %d exceed the scratch limits N <= %d, NZ <= %d",
+                 N, NZ, PIC_NOALLOC_MAX_N, PIC_NOALLOC_MAX_NZ);
+        return pic_fail(PIC_ERR_CAPACITY, buf);
+    }
+    int nnz = 0;
+    TRY(check_sparse(matrix, NZ, N, &nnz));
+    char* base = (char*)g_scratch_dev[g.device];
+    if (!base) return pic_fail(PIC_ERR_STATE, "scratch address not set up (pic_init)");
+    // carve the scratch: SPARSE array, diag, b, x, r, z, p, p2, q, partials, scalars
+    pic_sparse* dmat = (pic_sparse*)base;
+    double* v[8];
+    for (int k = 0; k < 8; ++k) v[k] = (double*)(base + kNoallocMatBytes + (size_t)k * kNoallocVecBytes);
+    double* part = (double*)(base + kNoallocMatBytes + 8 * kNoallocVecBytes);
+    double* scal = (double*)(base + kNoallocMatBytes + 8 * kNoallocVecBytes + kNoallocPartBytes);
+    HIPCHK(hipMemcpyAsync(dmat, matrix, (size_t)NZ * sizeof(pic_sparse), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(v[0], M, (size_t)N * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(v[1], R, (size_t)N * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(v[2], X, (size_t)N * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    int iters = 0, ierr = 0;
+    int rc = pcg_sparse_aos_fp64(N, dmat, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], part, PIC_NOALLOC_PSTRIDE,
+                                 scal, g.pcg_tol, g.pcg_maxit, &iters, &ierr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(X, v[2], (size_t)N * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(R, v[3], (size_t)N * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
     if (ierr == 1) return pic_fail(PIC_ERR_PCG_MAXITER, "MY_PCG IERR=1 (ITER left unset, as in the reference)");
     *ITER = iters - 1;     // MY_PCG.f90:129-133: ITER = i-1
     if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "MY_PCG IERR=2/3");

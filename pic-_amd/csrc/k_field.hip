@@ -111,6 +111,18 @@ template <typename T> struct CsrOp {
         return s;
     }
 };
+// The reference's TYPE SPARSE array read as is (IFE_MAIN_PARAM.f90:8-11, row pointers in SROW
+// of entries 1..N+1, Sparse_Structure.f90:163-175; 1-based): the same terms in the same
+// ascending order as CsrOp, so both round identically.
+struct SparseAosOp {
+    const pic_sparse* m;
+    template <class F> __device__ inline double row(int i, F pj) const {
+        double s = 0;
+        const int k0 = m[i].SROW - 1, k1 = m[i + 1].SROW - 1;
+        for (int k = k0; k < k1; ++k) s = s + m[k].K * pj(m[k].JCOL - 1);
+        return s;
+    }
+};
 template <typename T> struct DiaOp {
     const T* c[9];
     int off[9];
@@ -727,4 +739,13 @@ int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val
     HIPCHK(hipStreamSynchronize(g.stream));
     (void)hipFree(r); (void)hipFree(z); (void)hipFree(p); (void)hipFree(p2); (void)hipFree(q); (void)hipFree(part);
     return rc;
+}
+
+// MY_PCG on the reference's SPARSE array already on the device (the allocation-free
+// MY_JPCG_Solver): every buffer is the caller's (carved from the library's fixed scratch).
+int pcg_sparse_aos_fp64(int n, const pic_sparse* dmat, const double* diag, const double* b, double* x, double* r,
+                        double* z, double* p, double* p2, double* q, double* partials, int pstride, double* scal,
+                        double tol, int maxit, int* iters, int* ierr) {
+    SparseAosOp A; A.m = dmat;
+    return pcg_run<double>(n, A, diag, b, x, r, z, p, p2, q, tol, maxit, partials, pstride, iters, ierr, scal);
 }

@@ -317,3 +317,6 @@ void learn_free();
 int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,
                      const double* b, double* x, double tol, int maxit, int* iters, int* ierr,
                      double* r_out);
+int pcg_sparse_aos_fp64(int n, const pic_sparse* dmat, const double* diag, const double* b, double* x, double* r,
+                        double* z, double* p, double* p2, double* q, double* partials, int pstride, double* scal,
+                        double tol, int maxit, int* iters, int* ierr);
