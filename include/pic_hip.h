@@ -216,6 +216,12 @@ int  pic_format_fortran_e(double v, int w, int d, char* out);
  * [2] object absorptions, [3] wall reflections, [4] MCC selected (null incl.),
  * [5] MCC real collisions, [6] particles created by MCC, [7] particles deleted. */
 int  pic_get_counts(int isp, int64_t* nlive, int64_t* nslots, int64_t counters[8]);
+/* The first n counters of a species: [0..7] as pic_get_counts, [8] particles added by
+ * InjectBeams_2D (the wall source, InjectBeams_2D.f90:56-143) since pic_set_species. The MCC
+ * creation counter [6] of the electron species counts both products of an ionisation (one
+ * electron, one ion), so per step nlive(e) grows by injected + [6]/2 - deleted, nlive(ion)
+ * by injected + [6](e)/2 - deleted. */
+int  pic_get_counters(int isp, int64_t* counters, int n);
 /* MCC events by reaction index 1..nreact of the species' table (exact counts). */
 int  pic_get_mcc_events(int isp, int64_t* events, int nmax);
 

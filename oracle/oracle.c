@@ -176,6 +176,13 @@ static void mk_mesh(or_mesh* m, int nnx, int nny, const double* xs, const double
  * differ in the last bits. */
 static int or_threads = 1;
 void or_set_threads(int n) { or_threads = n < 1 ? 1 : n; }
+
+/* Planted mistakes (0 = none), set only by the tests that show their checks catch them:
+ * 1 post-collision h vector without its G*Gz*sin(phi) term, 2 SelectProbility reading
+ * the table one row too high, 3 VelMaxInit taking the speed from the first uniform of the
+ * accepted pair, 4 cell volumes without the hy/4 shift of the first and last radial rows. */
+static int or_plant = 0;
+void or_set_plant(int code) { or_plant = code; }
 int or_get_threads(void) { return or_threads; }
 
 /* ------------------------------------------------------------------ MoveAll */
@@ -396,6 +403,38 @@ void or_normalize(int nnodes, int delta, double weight, double qs_norm, double L
         }
         rho[i] = rho[i] + rho_s[i];
     }
+}
+
+/* Cell_Volume_zwz(2,:) of SetupGrids_2D_QLL.f90:196-395 for the CG nodes of a mesh without
+ * refinement (num == 1, P_flag(3) == 0): Cartesian hx hy / 4 at corners, / 2 on walls, hx hy
+ * inside; axisymmetric (y = r) 2 Pi_LY (ymin + hy/4)(hy/2) hx on the first row (/2 at its
+ * corners), 2 Pi_LY (ymax - hy/4)(hy/2) hx on the last, 2 Pi_LY y hy hx / 2 on the x walls and
+ * 2 Pi_LY y hy hx inside. hx, hy: the spacing of the first element containing the node. */
+void or_cell_volume(int nnx, int nny, const double* xs, const double* ys, int delta, double* vol) {
+    const double PI_LY = 3.14159265358979, sv = 1e-12;
+    const double xmin = xs[0], xmax = xs[nnx - 1], ymin = ys[0], ymax = ys[nny - 1];
+    for (int i = 0; i < nnx; ++i)
+        for (int j = 0; j < nny; ++j) {
+            int ci = i < nnx - 1 ? i : nnx - 2, cj = j < nny - 1 ? j : nny - 2;
+            double hx = xs[ci + 1] - xs[ci], hy = ys[cj + 1] - ys[cj], x = xs[i], y = ys[j];
+            double q = or_plant == 4 ? 0.0 : hy / 4.0;
+            int xl = fabs(x - xmin) < sv, xr = fabs(x - xmax) < sv, yb = fabs(y - ymin) < sv, yt = fabs(y - ymax) < sv;
+            double v;
+            if ((xl || xr) && (yb || yt)) {
+                if (delta == 0) v = hx * hy / 4.0;
+                else if (yb) v = 2.0 * PI_LY * (ymin + q) * (hy / 2.0) * hx / 2.0;
+                else v = 2.0 * PI_LY * (ymax - q) * (hy / 2.0) * hx / 2.0;
+            } else if (xl || xr || yb || yt) {
+                if (delta == 0) v = hx * hy / 2.0;
+                else if (yb) v = 2.0 * PI_LY * (ymin + q) * (hy / 2.0) * hx;
+                else if (yt) v = 2.0 * PI_LY * (ymax - q) * (hy / 2.0) * hx;
+                else v = 2.0 * PI_LY * y * hy * hx / 2.0;
+            } else {
+                if (delta == 0) v = hx * hy / 1.0;
+                else v = 2.0 * PI_LY * y * hy * hx / 1.0;
+            }
+            vol[(size_t)i * nny + j] = v;
+        }
 }
 
 /* ------------------------------------------------------------------ PPR */
@@ -850,7 +889,9 @@ static or_v3 or_vel_maxwell(or_stream* s, double mass, double temp) {       /* P
     int it = 0;
     while (FuncA > FuncB) {
         R = or_u01(s); FuncA = R * R;
+        double R1 = R;
         R = or_u01(s); FuncB = -OR_EXP * R * log(R);
+        if (or_plant == 3 && !(FuncA > FuncB)) R = R1;
         if (++it > 1000) break;
     }
     double V = sqrt(-3.0 * log(R) / Beta / mass);
@@ -880,7 +921,7 @@ static void or_post(or_mcpo* M, or_stream* s, int iso) {                     /* 
     double FcosTheta = 1.0 - CosTheta, SinTheta = sqrt(1.0 - CosTheta * CosTheta);
     double R = or_u01(s), APhi = 2.0 * OR_PI * R, CosPhi = cos(APhi), SinPhi = sin(APhi);
     double hx = M->Gper * CosPhi;
-    double hy = -(M->Gx * M->Gy * CosPhi + M->G * M->Gz * SinPhi) / M->Gper;
+    double hy = -(M->Gx * M->Gy * CosPhi + (or_plant == 1 ? 0.0 : M->G * M->Gz * SinPhi)) / M->Gper;
     double hz = -(M->Gx * M->Gz * CosPhi - M->G * M->Gy * SinPhi) / M->Gper;
     M->v.x = M->v.x - M->MassRatio * (M->Gx * FcosTheta + hx * SinTheta);
     M->v.y = M->v.y - M->MassRatio * (M->Gy * FcosTheta + hy * SinTheta);
@@ -890,6 +931,53 @@ static void or_post(or_mcpo* M, or_stream* s, int iso) {                     /* 
 static void or_vupd(or_mcpo* M, double f) {                                   /* TypeMCC.f90:84-99 */
     M->Gx *= f; M->Gy *= f; M->Gz *= f; M->Gper *= f; M->G *= f;
     M->v.x *= f; M->v.y *= f; M->v.z *= f;
+}
+
+/* SelectProbility (TypeMCC.f90:155-198) before the draw: the reaction probabilities at
+ * Energy (eV) from Probility(nreact, nsigma) (column-major, rows on the log10 grid emin +
+ * i*eint): N = ceiling((log10 E - emin)/eint); N < 1: the first row; N < nsigma: linear
+ * interpolation between rows N and N+1 (1-based) in log10 E; otherwise the last row times
+ * E/EnergyMax; zero below each reaction's threshold. */
+void or_select_prob(double Energy, int nreact, int nsigma, double emin, double eint, double emax, const double* prob,
+                    const double* thr, double* P) {
+    double lE = log10(Energy);
+    int N = (int)ceil((lE - emin) / eint);
+    int sh = or_plant == 2 ? 1 : 0;
+    if (N < 1) for (int i = 0; i < nreact; ++i) P[i] = prob[i];
+    else if (N < nsigma - sh) {
+        double S1 = (double)N - (lE - emin) / eint, S2 = 1.0 - S1;
+        for (int i = 0; i < nreact; ++i)
+            P[i] = prob[(size_t)(N - 1 + sh) * nreact + i] * S1 + prob[(size_t)(N + sh) * nreact + i] * S2;
+    } else for (int i = 0; i < nreact; ++i) P[i] = prob[(size_t)(nsigma - 1) * nreact + i] * Energy / emax;
+    for (int i = 0; i < nreact; ++i) if (Energy < thr[i]) P[i] = 0.0;
+}
+
+/* Test hooks: PostCollisionVelocity (TypeMCC.f90:101-133) of a projectile at velocity v
+ * (m/s) with relative velocity g = v - vt (m/s), energy (eV) and mass ratio M/(m + M),
+ * elastic (iso 0: CosKaiVahedi, MCCChi.f90:13-25) or isotropic (iso 1), on the stream
+ * (seed; slot, 0, 0); and n target-gas velocities of VelMaxInit (ParticleOne.f90:135-152,
+ * :247-261) from the streams (seed; k, 0, 0). */
+void or_post_test(const double* g, const double* v, double energy, double mass_ratio, int iso, uint64_t seed,
+                  uint32_t slot, double* v_out) {
+    or_stream s;
+    or_stream_init(&s, seed, slot, 0u, 0u);
+    or_mcpo M;
+    M.Gx = g[0]; M.Gy = g[1]; M.Gz = g[2];
+    M.Gper = sqrt(g[1] * g[1] + g[2] * g[2]);
+    M.G = sqrt(M.Gper * M.Gper + g[0] * g[0]);
+    M.Energy = energy; M.MassRatio = mass_ratio;
+    M.v.x = v[0]; M.v.y = v[1]; M.v.z = v[2];
+    or_post(&M, &s, iso);
+    v_out[0] = M.v.x; v_out[1] = M.v.y; v_out[2] = M.v.z;
+}
+
+void or_vel_maxwell_test(uint64_t seed, int64_t n, double mass, double temp, double* v) {
+    for (int64_t k = 0; k < n; ++k) {
+        or_stream s;
+        or_stream_init(&s, seed, (uint32_t)k, 0u, 0u);
+        or_v3 u = or_vel_maxwell(&s, mass, temp);
+        v[3 * k] = u.x; v[3 * k + 1] = u.y; v[3 * k + 2] = u.z;
+    }
 }
 
 /* One collision (MCCPublic.F90:58-90) of selected particle `slot` (normalised velocity
@@ -915,14 +1003,7 @@ int or_mcc_collide(int electron, int slot, uint64_t seed, uint32_t step, uint32_
     M.G = sqrt(G2);
     M.Energy = 0.5 * Miu * G2 / OR_JTOEV;
     double P[4] = {0, 0, 0, 0};
-    double lE = log10(M.Energy);
-    int N = (int)ceil((lE - emin) / eint);
-    if (N < 1) for (int i = 0; i < nreact; ++i) P[i] = prob[i];
-    else if (N < nsigma) {
-        double S1 = (double)N - (lE - emin) / eint, S2 = 1.0 - S1;
-        for (int i = 0; i < nreact; ++i) P[i] = prob[(size_t)(N - 1) * nreact + i] * S1 + prob[(size_t)N * nreact + i] * S2;
-    } else for (int i = 0; i < nreact; ++i) P[i] = prob[(size_t)(nsigma - 1) * nreact + i] * M.Energy / emax;
-    for (int i = 0; i < nreact; ++i) if (M.Energy < thr[i]) P[i] = 0.0;
+    or_select_prob(M.Energy, nreact, nsigma, emin, eint, emax, prob, thr, P);
     double R = or_u01(&s);
     int index = 0;
     for (int i = 0; i < nreact; ++i) if (R < P[i]) { index = i + 1; break; }

@@ -85,6 +85,18 @@ def lib():
         _lib.or_inject.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                    C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
                                    C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 6
+        _lib.or_set_plant.argtypes = [C.c_int]
+        _lib.or_set_plant.restype = None
+        _lib.or_select_prob.restype = None
+        _lib.or_select_prob.argtypes = [C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib.or_post_test.restype = None
+        _lib.or_post_test.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_uint64,
+                                      C.c_uint32, C.c_void_p]
+        _lib.or_vel_maxwell_test.restype = None
+        _lib.or_vel_maxwell_test.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
+        _lib.or_cell_volume.restype = None
+        _lib.or_cell_volume.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _lib.or_mcc_table.restype = C.c_double
         _lib.or_mcc_table.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       C.c_void_p, C.c_void_p]
@@ -348,3 +360,52 @@ def inject(bits, n, electron, consts, vd, y_lo, y_hi, seed, step, tag_, xmin, ym
     lib().or_inject(bits, n, electron, _p(c), _p(v), y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney,
                     *[_p(a) for a in out])
     return out
+
+
+# ------------------------------------------------------------------ pins of the MCC and set-up restatements
+class planted:
+    """Context manager: a planted mistake in the oracle (codes in oracle.c, or_set_plant),
+    for the tests that show a check catches it."""
+
+    def __init__(self, code: int):
+        self.code = code
+
+    def __enter__(self):
+        lib().or_set_plant(self.code)
+        return self
+
+    def __exit__(self, *exc):
+        lib().or_set_plant(0)
+        return False
+
+
+def select_prob(energy, prob, thr, nreact, nsigma, emin, eint, emax):
+    """SelectProbility's reaction probabilities at `energy` (eV), before the draw."""
+    P = np.zeros(4)
+    lib().or_select_prob(float(energy), nreact, nsigma, emin, eint, emax, _p(np.ascontiguousarray(prob, dtype=float)),
+                         _p(np.ascontiguousarray(thr, dtype=float)), _p(P))
+    return P[:nreact]
+
+
+def post_collision(g, v, energy, mass_ratio, iso, seed, slot):
+    """PostCollisionVelocity of a projectile (m/s) with relative velocity g (m/s)."""
+    out = np.zeros(3)
+    lib().or_post_test(_p(np.ascontiguousarray(g, dtype=float)), _p(np.ascontiguousarray(v, dtype=float)),
+                       float(energy), float(mass_ratio), int(iso), seed, slot, _p(out))
+    return out
+
+
+def vel_maxwell(seed, n, mass, temp):
+    """n target-gas velocities (m/s) of VelMaxInit, one Philox stream each."""
+    v = np.zeros((n, 3))
+    lib().or_vel_maxwell_test(seed, n, mass, temp, _p(v))
+    return v
+
+
+def cell_volume(xs, ys, delta):
+    """Cell_Volume_zwz(2,:) in node order (i * nny + j)."""
+    xs = np.ascontiguousarray(xs, dtype=float)
+    ys = np.ascontiguousarray(ys, dtype=float)
+    vol = np.zeros(len(xs) * len(ys))
+    lib().or_cell_volume(len(xs), len(ys), _p(xs), _p(ys), int(delta), _p(vol))
+    return vol

@@ -113,6 +113,7 @@ _PROTOS = {
     "pic_get_azimuth": (C.c_int, [C.c_int, _dp, C.c_int64, _i64p]),
     "pic_get_counts": (C.c_int, [C.c_int, _i64p, _i64p, _i64p]),
     "pic_get_mcc_events": (C.c_int, [C.c_int, _i64p, C.c_int]),
+    "pic_get_counters": (C.c_int, [C.c_int, _i64p, C.c_int]),
     "pic_MoveAll": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "pic_GetParChg_2D": (C.c_int, [C.c_int]),
     "pic_IFE_Solve_2D": (C.c_int, [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double]),
@@ -278,14 +279,18 @@ def workload(name: str) -> dict:
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),
                      objects=[Circle(0.5 * n, 0.5 * n, 0.15 * n, -1.0)])
         return dict(domain=dom, ppc=100, bits=64, solver=0, tol=1e-10, Te=2.0, Ti=0.026)
-    if name == "cfg3":            # 512^2, three conductors, mixed precision
-        n = 512
+    if name in ("cfg3", "cfg5") or name.startswith("cfg3s"):
+        # three conductors at -1, 0.5, 0, mixed precision: cfg3 on 512^2, its layout scaled to
+        # 1024^2 for cfg5 and to n^2 for the cfg3s<n> parity cases
+        n = {"cfg3": 512, "cfg5": 1024}.get(name) or int(name[5:])
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),
                      objects=[Circle(0.25 * n, 0.30 * n, 0.10 * n, -1.0), Circle(0.65 * n, 0.65 * n, 0.12 * n, 0.5),
                               Circle(0.70 * n, 0.22 * n, 0.08 * n, 0.0)])
-        # 71 per vacuum element: 16.8M particles per species (the conductors take 9.7% of the area);
-        # both species injected through the left wall as a Maxwellian drifting inward at 0.1 v_th
-        return dict(domain=dom, ppc=71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026, inject_drift_vth=0.1)
+        # 71 per vacuum element: 16.8M particles per species at 512^2 (the conductors take 9.7% of
+        # the area); cfg5: 35 per vacuum element, 33.1M per species at 1024^2. Both species are
+        # injected through the left wall as a Maxwellian drifting inward at 0.1 v_th
+        return dict(domain=dom, ppc=35 if name == "cfg5" else 71, bits=32, solver=1, tol=1e-8, Te=2.0, Ti=0.026,
+                    inject_drift_vth=0.1)
     if name == "cfg2":            # 256^2 r-z, a conductor at -1 and a dielectric disk (eps 4), mixed precision
         n = 256
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1,
@@ -416,6 +421,13 @@ class PicLibrary:
         check(lib().pic_get_counts(isp, C.byref(nl), C.byref(ns), c))
         return dict(nlive=nl.value, nslots=ns.value, emit=c[0], right=c[1], object=c[2], reflect=c[3],
                     mcc_selected=c[4], mcc_real=c[5], mcc_created=c[6], deleted=c[7])
+
+    def counters(self, isp: int) -> dict:
+        """Event counters of a species, with the wall-injected particle count."""
+        a = np.zeros(9, dtype=np.int64)
+        check(lib().pic_get_counters(isp, a.ctypes.data_as(_i64p), 9))
+        return dict(emit=int(a[0]), right=int(a[1]), object=int(a[2]), reflect=int(a[3]), mcc_selected=int(a[4]),
+                    mcc_real=int(a[5]), mcc_created=int(a[6]), deleted=int(a[7]), injected=int(a[8]))
 
     def mcc_events(self, isp: int, n: int = 4) -> np.ndarray:
         a = np.zeros(n, dtype=np.int64)

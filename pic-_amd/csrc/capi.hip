@@ -705,6 +705,19 @@ int pic_get_counts(int isp, int64_t* nlive, int64_t* nslots, int64_t counters[8]
     return PIC_OK;
 }
 
+int pic_get_counters(int isp, int64_t* counters, int n) {
+    REQUIRE_DOMAIN();
+    if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
+    if (!counters || n < 0) return pic_fail(PIC_ERR_ARG, "bad counter buffer");
+    unsigned long long h[16];
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipMemcpy(h, g.sp[isp].counters, sizeof(h), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n && k < 8; ++k) counters[k] = (int64_t)h[k];
+    if (n > 8) counters[8] = g.sp[isp].injected;
+    for (int k = 9; k < n; ++k) counters[k] = 0;
+    return PIC_OK;
+}
+
 int pic_get_mcc_events(int isp, int64_t* events, int nmax) {
     REQUIRE_DOMAIN();
     if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");

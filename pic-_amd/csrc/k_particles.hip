@@ -1030,7 +1030,7 @@ int run_injection(int it, int64_t* added) {
         q.x = S.x; q.y = S.y; q.vx = S.vx; q.vy = S.vy; q.vz = S.vz; q.cell = S.cell;
         a.first[a.nsrc + 1] = a.first[a.nsrc] + 4 * ns[s];
         a.nsrc++;
-        S.nslots += ns[s]; S.nlive += ns[s];
+        S.nslots += ns[s]; S.nlive += ns[s]; S.injected += ns[s];
         if (added) added[s] = ns[s];
     }
     if (a.nsrc == 0) return PIC_OK;

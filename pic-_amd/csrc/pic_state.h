@@ -90,6 +90,7 @@ struct SpeciesHost {
     bool acc_dirty = false;
     bool pending = false;               // MoveAll has filled acc and cell_next
     unsigned long long* counters = nullptr;  // device [16]
+    int64_t injected = 0;               // particles added by InjectBeams_2D since pic_set_species
     // MCC table
     bool has_mcc = false;
     int nreact = 0, nsigma = 0;

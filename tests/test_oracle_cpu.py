@@ -438,6 +438,19 @@ def test_workload_definitions(pkg):
     assert (ei == -1).sum() * w["ppc"] < 16777216 < (ei != -2).sum() * w["ppc"]
     w1 = pkg.workload("cfg1")
     assert (w1["domain"].nnx - 1) ** 2 * w1["ppc"] == 409600
+    # cfg5: the cfg3 layout scaled to 1024^2, about 32M particles per species (element centres
+    # outside the conductors x ppc)
+    w5 = pkg.workload("cfg5")
+    d5 = w5["domain"]
+    c = np.arange(d5.nnx - 1) + 0.5
+    X, Y = np.meshgrid(c, c, indexing="ij")
+    out = np.ones_like(X, dtype=bool)
+    for o in d5.objects:
+        out &= (X - o.x) ** 2 + (Y - o.y) ** 2 >= o.r ** 2
+        assert o.r / d5.xmax == pytest.approx([q.r for q in d.objects][d5.objects.index(o)] / d.xmax)
+    assert 32.0e6 < out.sum() * w5["ppc"] < 34.0e6
+    s64 = pkg.workload("cfg3s64")["domain"]
+    assert s64.nnx == 65 and [o.phi for o in s64.objects] == [-1.0, 0.5, 0.0]
 
 
 @pytest.mark.parametrize("bits", [64, 32])
