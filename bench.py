@@ -59,6 +59,16 @@ def traffic_from_profiles(name: str) -> dict:
             re.finditer(r"traffic_bytes_per_launch\s+(\S+):\s*([0-9.eE+]+)", p.read_text())}
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_threads() -> int:
     """Host threads for the multi-core CPU timing: the CPUs this process may run on, capped
     at 16 (one GPU's share of the host)."""
@@ -72,8 +82,10 @@ def cpu_threads() -> int:
 def _oracle_step(pkg, pl, w, orc, sample_particles: int, phi_prev) -> tuple:
     """One explicit step of the CPU oracle on the workload state: MoveAll + lookup +
     deposition for `sample_particles` particles per species (time scaled to the whole
-    population), then RHS, MY_PCG to 1e-8 in FP64 and PPR on the full mesh. Runs with the
-    oracle's current thread count. Returns (seconds per step, PCG iterations, particles)."""
+    population), then RHS, MY_PCG to 1e-8 in FP64 and PPR on the full mesh, the wall
+    injection of the workload and MCC (selection over every particle, the collisions of the
+    selected ones). Runs with the oracle's current thread count. Returns (seconds per step,
+    PCG iterations, particles)."""
     dom = w["domain"]
     hp = pl.setup_array("HP")
     xs = hp[0::2][:: dom.nny].copy()
@@ -111,7 +123,41 @@ def _oracle_step(pkg, pl, w, orc, sample_particles: int, phi_prev) -> tuple:
     phi2[un] = xsol
     orc.ppr(xs, ys, pl.setup_array("POINT_PATCH"), phi2)
     t_field = time.perf_counter() - t0
-    return t_part + t_field, it, n_total
+    # wall injection (InjectBeams_2D) and MCC; the inputs are read from the device first (untimed)
+    ns_sigma = 10001
+    mcc_in = []
+    for isp in (0, 1):
+        q = pl.particles(isp)
+        mcc_in.append((q["loc"] - 1, q["vx"], q["vy"], q["vz"], pl.mcc_table(isp, 3 if isp == 0 else 2)))
+    t0 = time.perf_counter()
+    for isp, sp in enumerate((pkg.ELECTRON, pkg.ARGON_ION)):
+        if w.get("inject_drift_vth"):
+            xm = sp["mass_kg"] / pkg.ELECTRON_MASS
+            temp = w["Te"] if isp == 0 else w["Ti"]
+            tmpj = temp * pkg.EVTOK / (dom.T_ref_eV * pkg.EVTOK)
+            vt = np.sqrt(tmpj / xm)
+            vd = [w["inject_drift_vth"] * vt, 0.0, 0.0]
+            cc = orc.inject_consts(vd[0], vt, tmpj, xm, dom.xmin, dom.dt)
+            weight = pkg.species_weight(dom, w["ppc"])
+            expect = dom.n_ref * abs(cc[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * (dom.ymax - dom.ymin) * \
+                norm["L_ref"] / weight
+            tg = orc.tag(isp, orc.RNG_INJECT)
+            k = orc.inject_count(expect, 1, 0, tg)
+            if k > 0:
+                orc.inject(64, k, 1 if isp == 0 else 0, cc, vd, dom.ymin, dom.ymax, 1, 0, tg, dom.xmin, dom.ymin,
+                           1.0, 1.0, dom.nnx - 1, dom.nny - 1)
+        nr, emin = (3, -3.0) if isp == 0 else (2, -1.0)
+        cells, pvx, pvy, pvz, (prob, _, cr) = mcc_in[isp]
+        thr = int(min(np.floor(cr * 2 ** 32), 2 ** 32 - 1))
+        sel = orc.mcc_select(cells, thr, 1, 0, orc.tag(isp, orc.RNG_SELECT))
+        rt, th = ([11, 12, 13], [0.0, 11.55, 15.76]) if isp == 0 else ([101, 103], [0.0, 0.0])
+        for k in sel:
+            orc.mcc_collide(1 if isp == 0 else 0, int(k), 1, 0, orc.tag(isp, orc.RNG_COLLIDE), norm["v_ref"],
+                            sp["mass_kg"], pkg.ARGON_ION["mass_kg"], 300.0, nr, ns_sigma, emin,
+                            (3.0 - emin) / (ns_sigma - 1), 1e3, prob, rt, th,
+                            np.array([pvx[k], pvy[k], pvz[k]]))
+    t_src = time.perf_counter() - t0
+    return t_part + t_field + t_src, it, n_total
 
 
 def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
@@ -133,12 +179,13 @@ def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
     finally:
         orc.set_threads(1)
     return {"value": n_total / tn, "unit": "particle-pushes/s", "cores": nthr, "kind": "port",
-            "nproc": os.cpu_count(),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
             "single_core": {"value": n_total / t1, "cores": 1, "s_per_step": t1, "pcg_iterations": it1},
             "sample": f"oracle.c FP64 step on {nthr} OpenMP threads ({tn:.2f} s per step; 1 core: {t1:.2f} s): "
                       f"mover+lookup+deposit timed on {sample_particles} particles per species and scaled to "
-                      f"{n_total}; RHS + MY_PCG (1e-8, {itn} it) + PPR on the full mesh; MCC and the wall "
-                      f"injection (~1e3 new particles per step) not included"}
+                      f"{n_total}; RHS + MY_PCG (1e-8, {itn} it) + PPR on the full mesh; wall injection and MCC "
+                      f"(selection over every particle, collisions of the selected); a stated baseline, "
+                      f"not a target: the GPU's roofline fractions measure the kernels"}
 
 
 def solve_compare(pl, reps: int) -> dict:
@@ -213,6 +260,105 @@ def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0)
 
 PHASES = ["push_lookup_deposit", "deposit_finalize", "rhs", "pcg", "ppr", "mcc", "sort", "total"]
 
+# Algorithmic HBM bytes of one step, by phase (the figures behind "full_step"):
+#   push + gather + lookup + deposit (fused)  44 B per particle (FP32 SoA state read and written once)
+#   deposit finalisation                      per node: 3 int64 sums read and cleared per species, rho_s,
+#                                             rho_p, Ek_tot, Ek_s written per species, rho and the volume
+#   RHS (B rho - c)                           per unknown: 9 B entries (12 B) + rho + c + two rhs writes
+#   PPR                                       per node: phi + class id + 6 coefficients + 5 FP64 and 5 FP32
+#                                             gather copies + efx, efy in FP64 and FP32
+#   MCC                                       the selection reads the cell of every slot (4 B) and writes its
+#                                             mask bit; a collision reads and writes a particle (24 B)
+#   cell sort (every sort_every steps)        52 B per particle (keys, gather of the slot arrays), amortised
+#   solve, per iteration of the inner solver  plain one-kernel CG: 72 B per node; learned FCG: 240 B per node
+#                                             (FCG vectors 88 B, V-cycle 152 B over its levels), plus 96 B per
+#                                             node per FP64 refinement residual
+FIN_B_NODE = 2 * (24 + 24 + 32) + 16
+RHS_B_NODE = 9 * 12 + 32
+PPR_B_NODE = 8 + 4 + 48 + 60 + 24
+SORT_B_PART = 52.0
+CG_B_NODE, FCG_B_NODE, RES64_B_NODE = 72.0, 240.0, 96.0
+
+
+def full_step_roofline(run: dict, steps: int, nnodes: int, nunk: int, learned: bool, sort_every: int) -> dict:
+    """Summed algorithmic bytes per step / measured time per step, against the 8 TB/s HBM peak."""
+    npart = 0.5 * (run["n0"] + run["n1"])
+    ks, solve = run["ks"], run["solve"]
+    if learned:
+        iters = ks["fcg_iterations"] / steps
+        solve_b = iters * FCG_B_NODE * nnodes + (solve["outer"] + 1) * RES64_B_NODE * nnodes
+    else:
+        iters = ks["cg_launches"] / steps
+        solve_b = iters * CG_B_NODE * nnodes + (solve["outer"] + 1) * RES64_B_NODE * nnodes
+    by = {"push_lookup_deposit": 44.0 * npart, "deposit_finalize": FIN_B_NODE * nnodes, "rhs": RHS_B_NODE * nunk,
+          "pcg": solve_b, "ppr": PPR_B_NODE * nnodes, "mcc": 4.125 * npart,
+          "sort": (SORT_B_PART * npart / sort_every) if sort_every else 0.0}
+    total = sum(by.values())
+    t = run["ms_per_step"] * 1e-3
+    ach = total / t / 1e9
+    return {"bound": "hbm", "bytes_per_step": total, "ms_per_step": run["ms_per_step"], "achieved": ach,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "floor_ms": total / (HBM_PEAK_GBS * 1e9) * 1e3, "solver_iterations_per_step": iters,
+            "bytes_by_phase": by}
+
+
+def fp64_error(pkg, pl, w, args, learned: bool) -> dict:
+    """One step of the product configuration (FP32 particles, mixed-precision or learned
+    solve) against the same step in FP64 (FP64 particles, FP64 PCG to 1e-12) from identical
+    inputs: the particles after a cell sort (no dead slots, so both runs draw the same random
+    streams), the potential, the step counter. Errors relative to the FP64 step."""
+    dom = w["domain"]
+    pl.sort_particles()
+    step = pl.get_step()
+    parts = [pl.particles(s) for s in (0, 1)]
+    th = [pl.azimuth(s) for s in (0, 1)] if dom.delta == 1 else [None, None]
+    phi0 = pl.field("PHI")
+    pl.set_solver(2 if learned else 1)
+    pl.step(1)
+    nn = dom.nnx * dom.nny
+    vol = pl.setup_array("CELL_VOLUME")[1::2]
+    a = dict(phi=pl.field("PHI"), rho_s=pl.field("RHO_S")[:2 * nn], ek=pl.field("EK_TOT")[:2 * nn],
+             n=[pl.counts(s)["nlive"] for s in (0, 1)], relres=pl.last_solve()["relres"])
+    # the same state, FP64 throughout
+    pl.setup(dom, bits=64, solver=0, tol=1e-12)
+    weight = pkg.species_weight(dom, w["ppc"])
+    for s, (sp, temp) in enumerate(((pkg.ELECTRON, w["Te"]), (pkg.ARGON_ION, w["Ti"]))):
+        q = parts[s]
+        cap = int(len(q["x"]) * 1.05) + 4096
+        pl.set_species(s, sp["charge_C"], sp["mass_kg"], weight, temp * pkg.EVTOK, cap)
+        pl.load_particles(s, q["x"], q["y"], q["vx"], q["vy"], q["vz"], q["loc"], th=th[s])
+    pl.MCCBundleInit(pkg.ARGON_ION["mass_kg"], 300.0, 3.2e20, 0, 1)
+    if w.get("inject_drift_vth"):
+        v_ref = dom.normalisation()["v_ref"]
+        for s, (sp, temp) in enumerate(((pkg.ELECTRON, w["Te"]), (pkg.ARGON_ION, w["Ti"]))):
+            vt = np.sqrt(temp / dom.T_ref_eV / (sp["mass_kg"] / pkg.ELECTRON_MASS))
+            pl.set_injection(s, dom.n_ref, temp, (w["inject_drift_vth"] * vt * v_ref, 0.0, 0.0), r_start=dom.xmin,
+                             y_lo=dom.ymin, y_hi=dom.ymax)
+    pl.set_run_params(20, 2024, 1)
+    pl.set_field("PHI", phi0)
+    pl.GetEField_SIDG_PPR()
+    pl.set_step(step)
+    pl.step(1)
+    b = dict(phi=pl.field("PHI"), rho_s=pl.field("RHO_S")[:2 * nn], ek=pl.field("EK_TOT")[:2 * nn],
+             n=[pl.counts(s)["nlive"] for s in (0, 1)])
+
+    def rel(x, y):
+        return float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+
+    out = {"reference": "the same step from the same state with FP64 particles and FP64 PCG to 1e-12",
+           "phi_max_rel": float(np.abs(a["phi"] - b["phi"]).max() / np.abs(b["phi"]).max()),
+           "phi_rel_l2": rel(a["phi"], b["phi"]), "relres_mixed": a["relres"], "live_fp32": a["n"], "live_fp64": b["n"]}
+    for s, name in ((0, "electrons"), (1, "ions")):
+        sl = slice(s * nn, (s + 1) * nn)
+        q_a, q_b = (a["rho_s"][sl] * vol).sum(), (b["rho_s"][sl] * vol).sum()
+        e_a, e_b = (a["ek"][sl] * vol).sum(), (b["ek"][sl] * vol).sum()
+        out[name] = {"rho_s_rel_l2": rel(a["rho_s"][sl], b["rho_s"][sl]),
+                     "rho_s_max_rel": float(np.abs(a["rho_s"][sl] - b["rho_s"][sl]).max() / np.abs(b["rho_s"][sl]).max()),
+                     "ek_rel_l2": rel(a["ek"][sl], b["ek"][sl]),
+                     "total_charge_rel": float(abs(q_a - q_b) / abs(q_b)),
+                     "total_energy_rel": float(abs(e_a - e_b) / abs(e_b))}
+    return out
+
 
 def main() -> int:
     ap = argparse.ArgumentParser()
@@ -225,6 +371,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--solve-reps", type=int, default=5)
+    ap.add_argument("--no-fp64-error", action="store_true", help="skip the FP64 comparison step (it ends the run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -262,7 +409,8 @@ def main() -> int:
         # the network is trained on pairs the plain solver harvests during the untimed warm-up
         pl.learn_setup(12, (16, 16), seed=7)
         pl.learn_harvest(True, 512, 1)
-    pl.step(args.warmup)
+    n_learned_warm = min(2, max(args.warmup - 1, 0)) if learned else 0
+    pl.step(args.warmup - n_learned_warm)
     info = {}
     if learned:
         pl.learn_harvest(False, 512, 1)
@@ -272,7 +420,8 @@ def main() -> int:
         l0, l1 = pl.learn_train(args.train_steps, batch=4, lr=2e-3, seed=11)
         t_train = time.perf_counter() - t0
         pl.set_solver(2)
-        pl.step(2)                       # two more untimed steps, with the learned solver
+        if n_learned_warm:
+            pl.step(n_learned_warm)      # the last warm-up steps run the learned solver
     log(f"bench: timing {args.steps} steps ({'learned' if learned else 'plain'} solver)")
     main_run = timed_steps(pl, args.steps)
     fallbacks = pl.learn_info()["fallbacks"] if learned else 0
@@ -288,7 +437,7 @@ def main() -> int:
     ph = main_run["phase"]
     out = {
         "metric": METRIC, "value": main_run["value"], "unit": "particle-pushes/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup + (2 if learned else 0), "ms_per_step": main_run["ms_per_step"], "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": main_run["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: " + WORKLOAD_TEXT.get(args.workload, "see pic-_amd workload()"),
                    "mesh": [dom.nnx - 1, dom.nny - 1], "particles_per_species": int(main_run["n0"] // 2),
@@ -299,6 +448,7 @@ def main() -> int:
         "phase_ms_per_step": {k: ph[i] / args.steps for i, k in enumerate(PHASES)},
         "roofline": roofs[0],
         "roofline_other": other[0] if other else None,
+        "full_step": full_step_roofline(main_run, args.steps, nnodes, pl.sizes()["nunknowns"], learned, 20),
     }
     if learned:
         cmp_ = solve_compare(pl, args.solve_reps)
@@ -323,6 +473,9 @@ def main() -> int:
             out["cpu_baseline"] = cpu_baseline(pkg, pl, w, args.cpu_sample, phi_prev)
         except Exception as e:  # the baseline is reported, not the product
             out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    if not args.no_fp64_error and w["bits"] != 64:
+        log("bench: error of the product step against an FP64 step from the same state")
+        out["fp64_error"] = fp64_error(pkg, pl, w, args, learned)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -285,6 +285,9 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
                const void* x, const void* y, const void* vx, const void* vy, const void* vz, const int* cell,
                double mass, double v_ref, int fix, double* w_node, double* e_node) {
     const double JtoeV = 1.6022e-19;
+    /* 0.5*Mass*|v|^2*v_ref^2/JtoeV with the constant factors multiplied once (the build's
+     * order; a few ulps from the reference's left-to-right product) */
+    const double Ec = 0.5 * mass * v_ref * v_ref / JtoeV;
     int nex = nnx - 1, ney = nny - 1, nelem = nex * ney, nnodes = nnx * nny;
     double ymin = ys[0];
     double* rs = (double*)calloc((size_t)4 * nelem, sizeof(double));
@@ -307,9 +310,7 @@ int or_deposit(int bits, int nnx, int nny, const double* xs, const double* ys, i
             X = ((const float*)x)[p]; Y = ((const float*)y)[p];
             VX = ((const float*)vx)[p]; VY = ((const float*)vy)[p]; VZ = ((const float*)vz)[p];
         }
-        double Energy = 0.5 * mass * (VX * VX + VY * VY + VZ * VZ);
-        Energy = Energy * v_ref * v_ref;
-        Energy = Energy / JtoeV;
+        double Energy = (VX * VX + VY * VY + VZ * VZ) * Ec;
         int ci = e / ney, cj = e % ney;
         double hxp = xs[ci + 1] - xs[ci], hyp = ys[cj + 1] - ys[cj];
         double dx = (X - xs[ci]) / hxp, dy = (Y - ys[cj]) / hyp;
@@ -872,14 +873,38 @@ void or_stiffness_apply(int nnx, int nny, const double* xs, const double* ys, in
  * with counter (s / 4, step, tag, 0) is below floor(CollisionRatio * 2^32) (a Bernoulli
  * trial per particle standing in for SelectParticle's count + shuffle, MCCPublic.F90:113-171).
  * Writes the selected slots in slot order. */
+static int or_selected(int64_t s, const int* cell, uint32_t threshold, uint64_t seed, uint32_t step, uint32_t tag) {
+    if (cell[s] < 0) return 0;
+    or_p4 o = or_philox((uint32_t)(s >> 2), step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return o.v[s & 3] < threshold;
+}
+
 int64_t or_mcc_select(int64_t n, const int* cell, uint32_t threshold, uint64_t seed, uint32_t step, uint32_t tag, int* out) {
-    int64_t k = 0;
-    for (int64_t s = 0; s < n; ++s) {
-        if (cell[s] < 0) continue;
-        or_p4 o = or_philox((uint32_t)(s >> 2), step, tag, 0u, (uint32_t)seed, (uint32_t)(seed >> 32));
-        if (o.v[s & 3] < threshold) out[k++] = (int)s;
+    const int nt = or_threads;
+    if (nt <= 1) {
+        int64_t k = 0;
+        for (int64_t s = 0; s < n; ++s)
+            if (or_selected(s, cell, threshold, seed, step, tag)) out[k++] = (int)s;
+        return k;
     }
-    return k;
+    /* threaded: count per chunk, exclusive prefix, write in slot order (same output) */
+    int64_t cnt[65] = {0};
+    const int nc = nt < 64 ? nt : 64;
+    const int64_t len = (n + nc - 1) / nc;
+#pragma omp parallel for num_threads(nc) schedule(static)
+    for (int c = 0; c < nc; ++c) {
+        int64_t k = 0;
+        for (int64_t s = c * len; s < n && s < (c + 1) * len; ++s) k += or_selected(s, cell, threshold, seed, step, tag);
+        cnt[c + 1] = k;
+    }
+    for (int c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+#pragma omp parallel for num_threads(nc) schedule(static)
+    for (int c = 0; c < nc; ++c) {
+        int64_t k = cnt[c];
+        for (int64_t s = c * len; s < n && s < (c + 1) * len; ++s)
+            if (or_selected(s, cell, threshold, seed, step, tag)) out[k++] = (int)s;
+    }
+    return cnt[nc];
 }
 
 typedef struct { double x, y, z; } or_v3;

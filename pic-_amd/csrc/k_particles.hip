@@ -46,7 +46,7 @@ template <typename R> struct PushArgs {
     R hxR, hxiR, hyR, hyiR;               // mesh spacing and reciprocal in R (exact if pow2)
     double vref_inv;          // 1/v_ref  (VelRes(1.0/PB%VFactor))
     double kappa_c;           // (Kappa-1.5)/beta, beta = m/(2 kB T)   (SI)
-    double mass, v_ref;       // species mass (kg) and v_ref for the energy moment
+    double ek_c;              // 0.5 Mass v_ref^2 / JtoeV: kinetic energy (eV) = |v|^2 ek_c
     uint64_t seed; uint32_t step; uint32_t tag_emit;
     long long* acc_w; long long* acc_e;
     long long* acc_h;         // whole-eV parts of the rare W E quanta >= 2^11 eV (pic_fix_split)
@@ -467,9 +467,7 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
                 double W[4];
                 deposit_weights(Xd, Yd, key, m, W);
                 double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
-                double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
-                energy = energy * a.v_ref * a.v_ref;
-                energy = energy / PIC_JTOEV;
+                const double energy = (vxd * vxd + vyd * vyd + vzd * vzd) * a.ek_c;
                 long long hv[4];
                 int fx = 0;
 #pragma unroll
@@ -602,9 +600,7 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
             double W[4];
             deposit_weights_ij((double)X, (double)Y, nci, ncj, m, W);
             double vxd = (double)VX, vyd = (double)VY, vzd = (double)VZ;
-            double energy = 0.5 * a.mass * (vxd * vxd + vyd * vyd + vzd * vzd);
-            energy = energy * a.v_ref * a.v_ref;
-            energy = energy / PIC_JTOEV;
+            const double energy = (vxd * vxd + vyd * vyd + vzd * vzd) * a.ek_c;
             long long qv[8];
             pic_fix8(W, energy, qv, next, m, a.acc_h, a.err);
             if (next != rkey) {
@@ -666,7 +662,10 @@ static PushArgs<R> make_push_args(int isp) {
     a.vref_inv = 1.0 / g.v_ref;
     double beta = S.p.mass_kg / (2.0 * PIC_KB * S.p.init_temperature_K);
     a.kappa_c = (1.0 * 2.0 - 1.5) / beta;
-    a.mass = S.p.mass_kg; a.v_ref = g.v_ref;
+    // ParticleOne.f90:333-341 with GetParChg_2D.f90:128 (0.5 Mass |v|^2 VFactor^2 / JtoeV), the
+    // constant factors multiplied once here instead of per particle (the product differs from the
+    // reference's operation order by at most a few ulps; the oracle uses the same constant)
+    a.ek_c = 0.5 * S.p.mass_kg * g.v_ref * g.v_ref / PIC_JTOEV;
     a.seed = g.seed; a.step = (uint32_t)g.step; a.tag_emit = pic_stream_tag(isp, PIC_RNG_EMIT);
     a.acc_w = S.acc_w; a.acc_e = S.acc_e; a.acc_h = S.acc_h;
     a.counters = S.counters; a.err = g.d_err;

@@ -611,7 +611,10 @@ def test_threaded_oracle_matches_serial(orc):
         pr = orc.ppr(xs, ys, pp, phi)
         pc = orc.pcg(rp, col, val, diag, b, np.zeros_like(b), 1e-10, 5000)
         rh = orc.rhs(xs, ys, 0, nt2, dphi, rho, A.shape[0])
-        return mv, loc, bad, dep, pr, pc, rh
+        sel = orc.mcc_select(np.where(rng_cells >= 0, rng_cells, -1), 900000000, 7, 3, orc.tag(1, 3))
+        return mv, loc, bad, dep, pr, pc, rh, sel
+
+    rng_cells = np.random.default_rng(4).integers(-1, 9, size=m).astype(np.int32)
 
     ref = run()
     try:
@@ -631,6 +634,7 @@ def test_threaded_oracle_matches_serial(orc):
     np.testing.assert_allclose(got[5][0], ref[5][0], rtol=1e-9, atol=1e-12)
     assert abs(got[5][2] - ref[5][2]) <= 1
     np.testing.assert_allclose(got[6], ref[6], rtol=1e-13, atol=1e-14)
+    assert len(ref[7]) > 1000 and np.array_equal(ref[7], got[7])     # MCC selection, slot order kept
 
 
 # ------------------------------------------------------------------ axisymmetric mover
