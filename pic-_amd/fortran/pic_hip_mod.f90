@@ -2,8 +2,12 @@
 !
 ! Fortran callers of the PIC-IFE_GEC time loop bind the GPU drop-ins through this
 ! module. The derived types are layout-identical to the reference's TYPE SPARSE
-! (IFE_MAIN_PARAM.f90:8-11) and TYPE ParticleOne (MCC_jw/code/base/ParticleOne.f90:4-6),
-! so reference arrays can be handed over with C_LOC without copying.
+! (IFE_MAIN_PARAM.f90:8-11) and TYPE ParticleOne (MCC_jw/code/base/ParticleOne.f90:4-6).
+! The reference's own arrays go through without a copy: TYPE SPARSE (no type-bound
+! procedures) through the TYPE(*) dummies of MY_JPCG_Solver, and PB%PO(1:NPar) -- a
+! non-interoperable ParticleOne with type-bound procedures, which a TYPE(*) dummy may not
+! take, but the same storage (10 REAL(8), one INTEGER, 88-byte stride) -- by its address,
+! pic_load_bundle_loc(isp, c_loc(PB%PO(1)), npar) and pic_store_bundle_loc (PB a TARGET).
 module pic_hip_mod
     use iso_c_binding
     implicit none
@@ -84,17 +88,32 @@ module pic_hip_mod
             integer(c_int64_t), value :: seed
         end function
         integer(c_int) function pic_load_bundle(isp, po, npar) bind(c, name="pic_load_bundle")
-            import :: pic_particle_one, c_int, c_int64_t
+            import :: c_int, c_int64_t
             integer(c_int), value :: isp
-            type(pic_particle_one), intent(in) :: po(*)
+            type(*), intent(in) :: po(*)          ! pic_particle_one, or any type of that storage without procedures
             integer(c_int64_t), value :: npar
         end function
         integer(c_int) function pic_store_bundle(isp, po, capacity, npar) bind(c, name="pic_store_bundle")
-            import :: pic_particle_one, c_int, c_int64_t
+            import :: c_int, c_int64_t
             integer(c_int), value :: isp
-            type(pic_particle_one) :: po(*)
+            type(*) :: po(*)
             integer(c_int64_t), value :: capacity
             integer(c_int64_t) :: npar
+        end function
+        integer(c_int) function pic_get_particles(isp, x, y, vx, vy, vz, loc, capacity, n) &
+                bind(c, name="pic_get_particles")
+            import :: c_int, c_double, c_int32_t, c_int64_t
+            integer(c_int), value :: isp
+            real(c_double) :: x(*), y(*), vx(*), vy(*), vz(*)
+            integer(c_int32_t) :: loc(*)
+            integer(c_int64_t), value :: capacity
+            integer(c_int64_t) :: n
+        end function
+        integer(c_int) function pic_get_counters(isp, counters, n) bind(c, name="pic_get_counters")
+            import :: c_int, c_int64_t
+            integer(c_int), value :: isp
+            integer(c_int64_t) :: counters(*)
+            integer(c_int), value :: n
         end function
         ! files in the reference's formats; names are NUL-terminated (trim(name)//c_null_char)
         integer(c_int) function pic_DumpParticleBundle(isp, path) bind(c, name="pic_DumpParticleBundle")
@@ -206,8 +225,16 @@ module pic_hip_mod
             integer(c_int64_t), value :: n
         end function
         integer(c_int) function pic_MY_JPCG_Solver(matrix, R, X, M, NZ, N, ITER) bind(c, name="pic_MY_JPCG_Solver")
-            import :: pic_sparse, c_double, c_int
-            type(pic_sparse), intent(in) :: matrix(*)
+            import :: c_double, c_int
+            type(*), intent(in) :: matrix(*)      ! pic_sparse or the reference's TYPE SPARSE (16 B each)
+            real(c_double) :: R(*), X(*)
+            real(c_double), intent(in) :: M(*)
+            integer(c_int), value :: NZ, N
+            integer(c_int) :: ITER
+        end function
+        integer(c_int) function pic_MY_JPCG_Solver_noalloc(matrix, R, X, M, NZ, N, ITER) bind(c, name="pic_MY_JPCG_Solver_noalloc")
+            import :: c_double, c_int
+            type(*), intent(in) :: matrix(*)      ! pic_sparse or the reference's TYPE SPARSE (16 B each)
             real(c_double) :: R(*), X(*)
             real(c_double), intent(in) :: M(*)
             integer(c_int), value :: NZ, N
@@ -245,6 +272,27 @@ module pic_hip_mod
 contains
 
     ! STOP with the library's message on a failed call (the reference STOPs on errors).
+    ! pic_load_bundle / pic_store_bundle on the address of the first particle record of an
+    ! array with pic_particle_one's storage (the reference's PB%PO(1) with C_LOC)
+    integer(c_int) function pic_load_bundle_loc(isp, po_first, npar)
+        integer(c_int), intent(in) :: isp
+        type(c_ptr), intent(in) :: po_first
+        integer(c_int64_t), intent(in) :: npar
+        type(pic_particle_one), pointer :: po(:)
+        call c_f_pointer(po_first, po, [max(npar, 1_c_int64_t)])
+        pic_load_bundle_loc = pic_load_bundle(isp, po, npar)
+    end function pic_load_bundle_loc
+
+    integer(c_int) function pic_store_bundle_loc(isp, po_first, capacity, npar)
+        integer(c_int), intent(in) :: isp
+        type(c_ptr), intent(in) :: po_first
+        integer(c_int64_t), intent(in) :: capacity
+        integer(c_int64_t), intent(out) :: npar
+        type(pic_particle_one), pointer :: po(:)
+        call c_f_pointer(po_first, po, [max(capacity, 1_c_int64_t)])
+        pic_store_bundle_loc = pic_store_bundle(isp, po, capacity, npar)
+    end function pic_store_bundle_loc
+
     subroutine pic_check(rc, where)
         integer(c_int), intent(in) :: rc
         character(*), intent(in) :: where

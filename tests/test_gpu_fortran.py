@@ -10,6 +10,19 @@ import pytest
 pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parent.parent
 DRIVER = ROOT / "pic-_amd" / "pic_driver"
+DROPIN_CHECK = ROOT / "pic-_amd" / "dropin_check"
+
+
+def test_dropin_binding_technique(pkg):
+    """pic-_amd/fortran/dropin_check.f90: a bundle of a ParticleOne-shaped type with
+    type-bound procedures loads by C_LOC of its first record and stores back, a TYPE
+    SPARSE-shaped array goes through the TYPE(*) dummy of MY_JPCG_Solver, the species index
+    is read as PB%SO%SpecyIndex (what the INTEGRATION.md shims do)."""
+    if not DROPIN_CHECK.exists():
+        pytest.skip("flang was not available at build time: dropin_check not built")
+    print("case: Fortran drop-in binding check", flush=True)
+    out = subprocess.run([str(DROPIN_CHECK)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "dropin_check: ParticleOne-shaped bundle" in out.stdout, out.stdout + out.stderr
 
 
 def test_fortran_driver_matches_python_driver(pkg, tmp_path):
