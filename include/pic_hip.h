@@ -166,6 +166,13 @@ int  pic_set_species(int isp, const pic_species_params* p, int64_t capacity);
  * axisymmetric domain the radius is uniform in r^2 over the cell, r = sqrt(r0^2 + (r1 +
  * r0)(r1 - r0) R), and the azimuth uniform on [0, 2 pi) (ParticleOne.f90:97-101). */
 int  pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed);
+/* The same load with the velocity law the reference selects by MaxKappa
+ * (MCCInterface.f90:168-176): kind 1 the Maxwellian above; kind 0 VelKappaInit
+ * (ParticleOne.f90:154-204: kappa = 2, Vx by rejection, Vy = Vz = 0); kind 2 VelPolyInit
+ * (ParticleOne.f90:206-245, gamma = 3: Vx uniform on +-sqrt(3) s, Vy = Vz = 0). For kinds 0
+ * and 2, Vx is stored as the reference draws it, in units of sqrt(2 kB 11605 K / Me) (its
+ * rescale to v_ref is commented out), from the stream (seed; slot, 0, load-velocity purpose). */
+int  pic_load_distribution(int isp, int32_t ppc, double temp_eV, uint64_t seed, int kind);
 
 /* Load / store a ParticleBundle in the reference's AoS record layout
  * (ParticleBundle%PO(1:NPar), ParticleBundle.f90:7-43). Load converts to device SoA;
@@ -282,7 +289,10 @@ int  pic_mcc_get_table(int isp, double* prob, int64_t n, double* sigma_max, doub
 /* Wall-injection source of one species: the injection block of pic.inp as Input_2D.f90:333-344
  * reads it and :489-494 normalises it (density dens0 in m^-3, temperature tmpj in eV, drift
  * vd in m/s), the slab start R_start and the wall's y extent [f_left_wall(2), f_right_wall(2)]
- * in mesh units. density_m3 = 0 removes the source. Cartesian only (delta = 0). */
+ * in mesh units. density_m3 = 0 removes the source. On an axisymmetric domain (delta = 1)
+ * the wall is the disk annulus of radii [y_lo, y_hi] at x = r_start: InjectBeams_2D.f90:59-61
+ * counts with PI (y_hi - y_lo)^2 L_ref^2 (the squared radial extent, as written), and PosInit
+ * draws r uniform in r^2 and the azimuth PO%Z uniform (ParticleOne.f90:95-98). */
 typedef struct {
     double density_m3;
     double temp_eV;

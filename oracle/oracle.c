@@ -180,7 +180,9 @@ void or_set_threads(int n) { or_threads = n < 1 ? 1 : n; }
 /* Planted mistakes (0 = none), set only by the tests that show their checks catch them:
  * 1 post-collision h vector without its G*Gz*sin(phi) term, 2 SelectProbility reading
  * the table one row too high, 3 VelMaxInit taking the speed from the first uniform of the
- * accepted pair, 4 cell volumes without the hy/4 shift of the first and last radial rows. */
+ * accepted pair, 4 cell volumes without the hy/4 shift of the first and last radial rows,
+ * 5 axisymmetric injection drawing r uniform in r instead of r^2, 6 the kappa load's
+ * acceptance density with half its exponent. */
 static int or_plant = 0;
 void or_set_plant(int code) { or_plant = code; }
 int or_get_threads(void) { return or_threads; }
@@ -1005,6 +1007,33 @@ void or_vel_maxwell_test(uint64_t seed, int64_t n, double mass, double temp, dou
     }
 }
 
+/* VelKappaInit (kind 0, ParticleOne.f90:154-204) and VelPolyInit (kind 2, gama = 3, :206-245):
+ * n velocities (Vx; Vy = Vz = 0, as the reference stores them) for mass M (kg) at T (K), on the
+ * streams (seed; k, 0, 0). */
+void or_load_velocity(int kind, uint64_t seed, int64_t n, double T, double M, double* v) {
+    const double kB = 1.3807e-23, Te = 11605.0, Me = 9.1095e-31;
+    for (int64_t k = 0; k < n; ++k) {
+        or_stream s;
+        or_stream_init(&s, seed, (uint32_t)k, 0u, 0u);
+        double V = 0.0;
+        if (kind == 0) {
+            const double Kappa = 2.0;
+            const double Sigma = ((Kappa - 1.5) / Kappa) * (2.0 * kB * T / M);
+            const double Sigma_e = ((Kappa - 1.5) / Kappa) * (2.0 * kB * Te / Me);
+            const double ex = or_plant == 6 ? -0.5 * Kappa : -Kappa;
+            for (int it = 0; it < 100000; ++it) {
+                V = sqrt(Sigma / Sigma_e) * (20.0 * or_u01(&s) - 10.0);
+                double P = pow(1.0 + (V * V) / ((2 * Kappa - 3) * Sigma / Sigma_e), ex);
+                if (or_u01(&s) <= P) break;
+            }
+        } else {
+            const double gama = 3.0, Sigma = 2.0 * kB * T / M, Sigma_e = 2.0 * kB * Te / Me;
+            V = sqrt(Sigma / Sigma_e) * 2.0 * sqrt(gama) * (or_u01(&s) - 0.5);
+        }
+        v[3 * k] = V; v[3 * k + 1] = 0.0; v[3 * k + 2] = 0.0;
+    }
+}
+
 /* One collision (MCCPublic.F90:58-90) of selected particle `slot` (normalised velocity
  * v in/out). prob: Probility(nreact, nsigma) column-major. Returns the reaction index
  * (0 = null); for ionisation writes the new electron and ion velocities (normalised)
@@ -1143,7 +1172,9 @@ static double or_loadv(uint64_t seed, uint32_t slot, uint32_t step, uint32_t tag
  * below it, keeping the particle strictly inside the slab as sampled. vd = drift {vx, vy, vz}; mesh: xmin, ymin, mapx, mapy, nex, ney. */
 void or_inject(int bits, int64_t n, int electron, const double* c, const double* vd, double y_lo, double y_hi,
                uint64_t seed, uint32_t step, uint32_t tag, double xmin, double ymin, double mapx, double mapy,
-               int nex, int ney, double* x, double* y, double* vx, double* vy, double* vz, int* cell) {
+               int nex, int ney, double* x, double* y, double* vx, double* vy, double* vz, int* cell, double* th) {
+    /* th != NULL: axisymmetric PosInit (ParticleOne.f90:95-98), r = sqrt(YL^2 + (YU+YL)(YU-YL) R)
+     * and the azimuth PO%Z = 2 PI R' (R' the first uniform of block 1 of the position stream) */
     for (int64_t k = 0; k < n; ++k) {
         double UN = 0.0;
         for (uint32_t i = 0;; ++i) {
@@ -1163,7 +1194,13 @@ void or_inject(int bits, int64_t n, int electron, const double* c, const double*
         double RX, RY;
         or_block_u01(seed, (uint32_t)(4 * k + 3), step, tag, 0, &RX, &RY);
         double X = c[7] + (c[8] - c[7]) * RX;
-        double Y = y_lo + (y_hi - y_lo) * RY;
+        double Y = (th && or_plant != 5) ? sqrt(y_lo * y_lo + (y_hi + y_lo) * (y_hi - y_lo) * RY) : y_lo + (y_hi - y_lo) * RY;
+        if (th) {
+            double RZ, RZ2;
+            or_block_u01(seed, (uint32_t)(4 * k + 3), step, tag, 1, &RZ, &RZ2);
+            th[k] = 2.0 * OR_PI * RZ;
+            if (bits == 32) th[k] = (double)(float)th[k];
+        }
         if (bits == 32) { X = (double)(float)X; Y = (double)(float)Y; Vx = (double)(float)Vx; Vy = (double)(float)Vy; Vz = (double)(float)Vz; }
         X = or_below(X, c[8], bits); Y = or_below(Y, y_hi, bits);
         int ii = (int)((X - xmin) / mapx) + 1, jj = (int)((Y - ymin) / mapy) + 1;

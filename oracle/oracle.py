@@ -84,7 +84,7 @@ def lib():
         _lib.or_inject.restype = None
         _lib.or_inject.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
                                    C.c_uint64, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
-                                   C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 6
+                                   C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 7
         _lib.or_set_plant.argtypes = [C.c_int]
         _lib.or_set_plant.restype = None
         _lib.or_select_prob.restype = None
@@ -95,6 +95,8 @@ def lib():
                                       C.c_uint32, C.c_void_p]
         _lib.or_vel_maxwell_test.restype = None
         _lib.or_vel_maxwell_test.argtypes = [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
+        _lib.or_load_velocity.restype = None
+        _lib.or_load_velocity.argtypes = [C.c_int, C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_void_p]
         _lib.or_cell_volume.restype = None
         _lib.or_cell_volume.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         _lib.or_mcc_table.restype = C.c_double
@@ -352,14 +354,16 @@ def inject_count(expect_no_r, seed, step, tag_):
     return int(lib().or_inject_count(expect_no_r, seed, step, tag_))
 
 
-def inject(bits, n, electron, consts, vd, y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney):
-    """n injected particles of one source: x, y, vx, vy, vz, 0-based cell (-1 outside)."""
+def inject(bits, n, electron, consts, vd, y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney, delta=0):
+    """n injected particles of one source: x, y, vx, vy, vz, 0-based cell (-1 outside), and on
+    an axisymmetric domain (delta = 1) the azimuth as a seventh array."""
     out = [np.zeros(n) for _ in range(5)] + [np.zeros(n, dtype=np.int32)]
+    th = np.zeros(n) if delta == 1 else None
     c = np.ascontiguousarray(consts, dtype=np.float64)
     v = np.ascontiguousarray(vd, dtype=np.float64)
     lib().or_inject(bits, n, electron, _p(c), _p(v), y_lo, y_hi, seed, step, tag_, xmin, ymin, mapx, mapy, nex, ney,
-                    *[_p(a) for a in out])
-    return out
+                    *[_p(a) for a in out], _p(th))
+    return out + [th] if delta == 1 else out
 
 
 # ------------------------------------------------------------------ pins of the MCC and set-up restatements
@@ -399,6 +403,13 @@ def vel_maxwell(seed, n, mass, temp):
     """n target-gas velocities (m/s) of VelMaxInit, one Philox stream each."""
     v = np.zeros((n, 3))
     lib().or_vel_maxwell_test(seed, n, mass, temp, _p(v))
+    return v
+
+
+def load_velocity(kind, seed, n, temp_K, mass):
+    """n initial velocities of VelKappaInit (kind 0) or VelPolyInit (kind 2)."""
+    v = np.zeros((n, 3))
+    lib().or_load_velocity(int(kind), seed, n, float(temp_K), float(mass), _p(v))
     return v
 
 

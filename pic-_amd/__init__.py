@@ -99,6 +99,7 @@ _PROTOS = {
     "pic_get_setup_array": (C.c_int, [C.c_int, C.c_void_p, C.c_int64]),
     "pic_set_species": (C.c_int, [C.c_int, _P(SpeciesParams), C.c_int64]),
     "pic_load_maxwellian": (C.c_int, [C.c_int, C.c_int32, C.c_double, C.c_uint64]),
+    "pic_load_distribution": (C.c_int, [C.c_int, C.c_int32, C.c_double, C.c_uint64, C.c_int]),
     "pic_load_bundle": (C.c_int, [C.c_int, _P(ParticleOne), C.c_int64]),
     "pic_store_bundle": (C.c_int, [C.c_int, _P(ParticleOne), C.c_int64, _i64p]),
     "pic_load_particles": (C.c_int, [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _dp, _ip]),
@@ -365,6 +366,11 @@ class PicLibrary:
 
     def load_maxwellian(self, isp: int, ppc: int, temp_eV: float, seed: int):
         check(lib().pic_load_maxwellian(isp, ppc, temp_eV, seed))
+
+    def load_distribution(self, isp: int, ppc: int, temp_eV: float, seed: int, kind: int):
+        """Initial load with the reference's MaxKappa velocity law: 0 kappa, 1 Maxwellian,
+        2 polytropic (MCCInterface.f90:168-176)."""
+        check(lib().pic_load_distribution(isp, ppc, temp_eV, seed, kind))
 
     def load_particles(self, isp, x, y, vx, vy, vz, loc=None, th=None):
         """th: the azimuth PO%Z of each particle (axisymmetric domains)."""

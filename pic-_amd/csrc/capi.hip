@@ -565,14 +565,19 @@ int pic_load_particles(int isp, int64_t n, const double* x, const double* y, con
 }
 
 int pic_load_maxwellian(int isp, int32_t ppc, double temp_eV, uint64_t seed) {
+    return pic_load_distribution(isp, ppc, temp_eV, seed, 1);
+}
+
+int pic_load_distribution(int isp, int32_t ppc, double temp_eV, uint64_t seed, int kind) {
     REQUIRE_DOMAIN();
+    if (kind < 0 || kind > 2) return pic_fail(PIC_ERR_ARG, "kind must be 0 (kappa), 1 (Maxwellian) or 2 (polytropic)");
     if (isp < 0 || isp >= PIC_MAX_SPECIES || !g.sp[isp].defined) return pic_fail(PIC_ERR_ARG, "species not defined");
     if (ppc <= 0 || !(temp_eV >= 0)) return pic_fail(PIC_ERR_ARG, "bad ppc / temperature");
     std::vector<double> x, y, vx, vy, vz;
     std::vector<int> cell;
     std::vector<double> th;
     const bool rz = g.mesh.delta == 1;
-    setup_maxwellian(isp, ppc, temp_eV, seed, x, y, vx, vy, vz, cell, rz ? &th : nullptr);
+    setup_maxwellian(isp, ppc, temp_eV, seed, x, y, vx, vy, vz, cell, rz ? &th : nullptr, kind);
     if ((int64_t)x.size() > g.sp[isp].capacity) return pic_fail(PIC_ERR_CAPACITY, "load exceeds capacity");
     TRY(pic_load_particles(isp, (int64_t)x.size(), x.data(), y.data(), vx.data(), vy.data(), vz.data(), nullptr));
     if (rz) TRY(pic_set_azimuth(isp, (int64_t)th.size(), th.data()));
@@ -923,7 +928,6 @@ int pic_set_injection(int isp, const pic_injection* p) {
     InjectHost& q = g.inj[isp];
     q = InjectHost();
     if (p->density_m3 == 0.0) return PIC_OK;
-    if (g.mesh.delta != 0) return pic_fail(PIC_ERR_UNSUPPORTED, "wall injection is implemented for Cartesian domains");
     if (!(p->density_m3 > 0) || !(p->temp_eV > 0)) return pic_fail(PIC_ERR_ARG, "injection density and temperature must be > 0");
     if (!(p->y_hi > p->y_lo) || p->y_lo < g.mesh.ymin || p->y_hi > g.mesh.ymax || p->r_start < g.mesh.xmin ||
         p->r_start > g.mesh.xmax)

@@ -885,7 +885,9 @@ int launch_ppr_gather_points(int64_t n, const double* dx, const double* dy, cons
 // with its own Philox stream (slot 4k + c, step it, purpose PIC_RNG_INJECT | species):
 // c = 0 Bird's inward flux component (:91-106), c = 1 and 2 Loadv for Vy and Vz
 // (Loadv.f90:14-23; the ion drift added after, :121-126), c = 3 PosInit's X then Y
-// (ParticleOne.f90:82-89) and InjectPositioning (InjectPositioning.f90:13-22). Candidate i
+// (ParticleOne.f90:82-89; on an axisymmetric domain r = sqrt(YL^2 + (YU + YL)(YU - YL) R)
+// and the azimuth PO%Z = 2 PI R', :95-98, R' from the stream's second block) and
+// InjectPositioning (InjectPositioning.f90:13-22). Candidate i
 // of a rejection loop takes the two uniforms of Philox block i of its stream (a rejected
 // candidate's second uniform goes unused), so a lane evaluates four candidates at a time and
 // keeps the first accepted one: the same value as trying them in order. One launch serves
@@ -898,6 +900,7 @@ struct InjectSource {
     uint32_t tag;
     int species;
     void *x, *y, *vx, *vy, *vz;
+    void* th;                            // azimuth (axisymmetric domains), else nullptr
     int* cell;
 };
 struct InjectArgs {
@@ -974,7 +977,14 @@ __global__ void k_inject(InjectArgs a) {
     } else {
         const philox4 o = pic_philox4x32_10(slot, a.step, S.tag, 0u, k0, k1);
         const double X = p.xl + (p.xu - p.xl) * pic_u01(o.v[0], o.v[1]);
-        const double Y = p.y_lo + (p.y_hi - p.y_lo) * pic_u01(o.v[2], o.v[3]);
+        double Y;
+        if (S.th) {
+            Y = sqrt(p.y_lo * p.y_lo + (p.y_hi + p.y_lo) * (p.y_hi - p.y_lo) * pic_u01(o.v[2], o.v[3]));
+            const philox4 o2 = pic_philox4x32_10(slot, a.step, S.tag, 1u, k0, k1);
+            ((R*)S.th)[d] = (R)(2.0 * PIC_PI * pic_u01(o2.v[0], o2.v[1]));
+        } else {
+            Y = p.y_lo + (p.y_hi - p.y_lo) * pic_u01(o.v[2], o.v[3]);
+        }
         // a stored coordinate that rounds onto the slab's upper bound (FP32 slots: Y just below
         // y_hi becomes y_hi) takes the largest value below it, so the particle stays strictly
         // inside the slab, as the sampled value is
@@ -997,7 +1007,7 @@ __global__ void k_inject(InjectArgs a) {
     }
 }
 
-// add_N of every registered source (InjectBeams_2D.f90:40-64, Cartesian), then one launch;
+// add_N of every registered source (InjectBeams_2D.f90:40-64), then one launch;
 // the rounding draw is the first word of the species' stream at slot 0xFFFFFFFF.
 int run_injection(int it, int64_t* added) {
     int64_t ns[PIC_MAX_SPECIES] = {0};
@@ -1009,8 +1019,13 @@ int run_injection(int it, int64_t* added) {
         pic_stream rs;
         pic_stream_init(&rs, g.seed, 0xFFFFFFFFu, (uint32_t)it, pic_stream_tag(s, PIC_RNG_INJECT));
         const double ranum = pic_stream_u01(&rs);
-        const double expect = p.dens * g.cfg.n_ref * std::fabs(p.b_amb) * g.v_ref * g.cfg.dt * g.time_ref *
-                              (p.y_hi - p.y_lo) * g.L_ref / S.p.weight;
+        // InjectBeams_2D.f90:56-62: the slab's width (Cartesian) or the disk annulus PI (YU - YL)^2
+        // (axisymmetric, as written: the squared radial extent) crossed per step
+        const double dy = p.y_hi - p.y_lo;
+        const double expect = g.mesh.delta == 1
+            ? p.dens * g.cfg.n_ref * std::fabs(p.b_amb) * g.v_ref * g.cfg.dt * g.time_ref * PIC_PI * (dy * dy) *
+                  (g.L_ref * g.L_ref) / S.p.weight
+            : p.dens * g.cfg.n_ref * std::fabs(p.b_amb) * g.v_ref * g.cfg.dt * g.time_ref * dy * g.L_ref / S.p.weight;
         const int64_t n = (int64_t)(expect + ranum);
         if (n <= 0) continue;
         if (S.nslots + n > S.capacity) return pic_fail(PIC_ERR_CAPACITY, "injection exceeds particle capacity");
@@ -1027,6 +1042,7 @@ int run_injection(int it, int64_t* added) {
         q.base = S.nslots; q.n = (int)ns[s]; q.tag = pic_stream_tag(s, PIC_RNG_INJECT);
         q.species = s;
         q.x = S.x; q.y = S.y; q.vx = S.vx; q.vy = S.vy; q.vz = S.vz; q.cell = S.cell;
+        q.th = g.mesh.delta == 1 ? S.th : nullptr;
         a.first[a.nsrc + 1] = a.first[a.nsrc] + 4 * ns[s];
         a.nsrc++;
         S.nslots += ns[s]; S.nlive += ns[s]; S.injected += ns[s];

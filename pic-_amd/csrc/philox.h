@@ -81,7 +81,8 @@ enum {
     PIC_RNG_EMIT = 2,      // left-wall kappa re-emission inside the push
     PIC_RNG_SELECT = 3,    // null-collision selection (one word per slot)
     PIC_RNG_COLLIDE = 4,   // collision kinematics of a selected particle
-    PIC_RNG_INJECT = 5     // wall injection
+    PIC_RNG_INJECT = 5,    // wall injection
+    PIC_RNG_LOADV = 6      // kappa / polytropic velocities of the initial load
 };
 
 PIC_HD uint32_t pic_stream_tag(int species, int purpose) {

@@ -271,7 +271,7 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
                       std::vector<double>& x, std::vector<double>& y,
                       std::vector<double>& vx, std::vector<double>& vy, std::vector<double>& vz,
-                      std::vector<int>& cell, std::vector<double>* th);
+                      std::vector<int>& cell, std::vector<double>* th, int kind = 1);
 
 // device launchers
 int launch_push_deposit(int isp, bool do_push);

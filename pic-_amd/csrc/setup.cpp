@@ -628,14 +628,46 @@ void setup_analytic_tables(int isp, bool electron, const pic_gas_params& gas, do
 }
 
 // ------------------------------------------------------------------ particle loader
+// The other initial velocity laws of AllInitializationParticleBundleIFE (MaxKappa,
+// MCCInterface.f90:168-176), on the stream (seed; slot, 0, PIC_RNG_LOADV | species).
+// kind 0, VelocityKappaInitializationParticleOne (ParticleOne.f90:154-204): kappa = 2, Vx by
+// rejection on s (20 R - 10) against (1 + Vx^2 / ((2 kappa - 3) s^2))^-kappa, s^2 =
+// Sigma / Sigma_e with Sigma = ((kappa - 1.5)/kappa) 2 kB T / M and Sigma_e the same for an
+// electron at 11605 K; Vy = Vz = 0. kind 2, VelocityPolytropicInitializationParticleOne
+// (:206-245, gama = 3): Vx = s 2 sqrt(3) (R - 0.5), s^2 = (2 kB T / M) / (2 kB 11605 / Me);
+// Vy = Vz = 0. Both store Vx as drawn (the reference's rescale to v_ref is commented out).
+static void load_velocity_law(int kind, uint64_t seed, uint32_t slot, uint32_t tag, double T, double M,
+                              double* vx, double* vy, double* vz) {
+    const double kB = 1.3807e-23, Te = 11605.0, Me = 9.1095e-31;
+    pic_stream s; pic_stream_init(&s, seed, slot, 0u, tag);
+    double V = 0.0;
+    if (kind == 0) {
+        const double Kappa = 2.0;
+        const double Sigma = ((Kappa - 1.5) / Kappa) * (2.0 * kB * T / M);
+        const double Sigma_e = ((Kappa - 1.5) / Kappa) * (2.0 * kB * Te / Me);
+        for (int it = 0; it < 100000; ++it) {
+            V = std::sqrt(Sigma / Sigma_e) * (20.0 * pic_stream_u01(&s) - 10.0);
+            const double P = std::pow(1.0 + (V * V) / ((2 * Kappa - 3) * Sigma / Sigma_e), -Kappa);
+            if (pic_stream_u01(&s) <= P) break;
+        }
+    } else {
+        const double gama = 3.0;
+        const double Sigma = 2.0 * kB * T / M, Sigma_e = 2.0 * kB * Te / Me;
+        const double V_max = std::sqrt(gama);
+        V = std::sqrt(Sigma / Sigma_e) * 2.0 * V_max * (pic_stream_u01(&s) - 0.5);
+    }
+    *vx = V; *vy = 0.0; *vz = 0.0;
+}
+
 // ppc particles per vacuum element (r-z: ppc on average, in proportion to the element's
 // volume), elements in sort order; uniform position inside the element,
-// Maxwellian velocity (three Box-Muller normals) at temp_eV. Particles falling inside an
-// object are dropped (the reference's AdjustBoundary_2D pre-filter, Main_IFE_Test_2.f90:218).
+// Maxwellian velocity (three Box-Muller normals) at temp_eV (kind 1), or the kappa (kind 0)
+// or polytropic (kind 2) law above. Particles falling inside an object are dropped (the
+// reference's AdjustBoundary_2D pre-filter, Main_IFE_Test_2.f90:218).
 void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
                       std::vector<double>& x, std::vector<double>& y,
                       std::vector<double>& vx, std::vector<double>& vy, std::vector<double>& vz,
-                      std::vector<int>& cell, std::vector<double>* th) {
+                      std::vector<int>& cell, std::vector<double>* th, int kind) {
     const MeshParams& m = g.mesh;
     const SpeciesHost& S = g.sp[isp];
     double vth = std::sqrt(temp_eV / g.cfg.T_ref_eV / S.xm);   // normalised thermal speed
@@ -680,7 +712,14 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
             if (inside) continue;
             x.push_back(px); y.push_back(py);
             if (th) th->push_back(2.0 * PIC_PI * pic_stream_u01(&s));
-            vx.push_back(vth * r1 * std::cos(a)); vy.push_back(vth * r1 * std::sin(a)); vz.push_back(vth * r2 * std::cos(b));
+            if (kind == 1) {
+                vx.push_back(vth * r1 * std::cos(a)); vy.push_back(vth * r1 * std::sin(a)); vz.push_back(vth * r2 * std::cos(b));
+            } else {
+                double u, v, w;
+                load_velocity_law(kind, seed, slot, pic_stream_tag(isp, PIC_RNG_LOADV), temp_eV * PIC_EVTOK, S.p.mass_kg,
+                                  &u, &v, &w);
+                vx.push_back(u); vy.push_back(v); vz.push_back(w);
+            }
             cell.push_back(e);
         }
     }

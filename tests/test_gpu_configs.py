@@ -377,3 +377,38 @@ def test_cfg2_product_configuration_properties(gpu_lib, pkg):
     pl = gpu_lib
     say("case: cfg2 full size, product configuration, 24 steps of properties")
     _product_run(pl, pkg, "cfg2", 24, 200)
+
+
+def test_kappa_and_polytropic_initial_loads(gpu_lib, pkg):
+    """pic_load_distribution with the reference's MaxKappa laws (MCCInterface.f90:168-176): the
+    loaded Vx follow VelKappaInit's kappa = 2 law and VelPolyInit's uniform gamma = 3 law (the
+    laws the oracle's restatement is pinned to, tests/test_oracle_pins_cpu.py), Vy = Vz = 0,
+    positions as the Maxwellian load's, and the particles step."""
+    from scipy import stats
+    from test_oracle_pins_cpu import kappa_load_cdf, load_scale
+    pl = gpu_lib
+    say("case: kappa and polytropic initial loads")
+    w = pkg.workload("small32")
+    dom = w["domain"]
+    pl.setup(dom, bits=64, solver=0, tol=1e-10)
+    weight = pkg.species_weight(dom, 20)
+    pl.set_species(0, pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], weight, 2.0 * pkg.EVTOK, 32 * 32 * 40)
+    pl.set_species(1, pkg.ARGON_ION["charge_C"], pkg.ARGON_ION["mass_kg"], weight, 0.026 * pkg.EVTOK, 32 * 32 * 40)
+    pl.load_distribution(0, 20, 2.0, 9, 0)
+    pl.load_distribution(1, 20, 0.026, 10, 2)
+    pe, pi = pl.particles(0), pl.particles(1)
+    assert np.all(pe["vy"] == 0) and np.all(pe["vz"] == 0) and np.all(pi["vy"] == 0) and np.all(pi["vz"] == 0)
+    se = load_scale(2.0 * pkg.EVTOK, pkg.ELECTRON["mass_kg"], 0)
+    si = load_scale(0.026 * pkg.EVTOK, pkg.ARGON_ION["mass_kg"], 2)
+    assert stats.kstest(pe["vx"], kappa_load_cdf(se)).pvalue > 1e-3
+    assert stats.kstest(pi["vx"], stats.uniform(-np.sqrt(3) * si, 2 * np.sqrt(3) * si).cdf).pvalue > 1e-3
+    # positions: the Maxwellian load of the same seed puts the particles at the same places
+    pl.load_distribution(0, 20, 2.0, 9, 1)
+    pm = pl.particles(0)
+    assert np.array_equal(pm["x"], pe["x"]) and np.array_equal(pm["loc"], pe["loc"])
+    pl.load_distribution(0, 20, 2.0, 9, 0)
+    pl.set_run_params(0, 9, 0)
+    pl.initial_field()
+    pl.step(2)
+    with pytest.raises(pkg.PicError):
+        pl.load_distribution(0, 20, 2.0, 9, 3)

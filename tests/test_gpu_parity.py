@@ -615,3 +615,64 @@ def test_injection_matches_oracle(gpu_lib, pkg, orc, bits, slab):
     pl.initial_field()
     pl.step(1)
     assert pl.counts(0)["nlive"] > 0
+
+
+@pytest.mark.parametrize("bits", [64, 32])
+def test_injection_axisymmetric_matches_oracle(gpu_lib, pkg, orc, bits):
+    """InjectBeams_2D on the axisymmetric domain (InjectBeams_2D.f90:59-61): the count with the
+    disk area PI (y_hi - y_lo)^2 L_ref^2 (y_lo = 0: the axis), and every particle's position,
+    radius, azimuth (PO%Z), velocity and element bit-exact against the oracle."""
+    pl = gpu_lib
+    say(f"case: InjectBeams_2D rz32 bits={bits}")
+    w = pkg.workload("rz32")
+    dom = w["domain"]
+    pl.setup(dom, bits=bits, solver=0, tol=1e-10)
+    nel = (dom.nnx - 1) * (dom.nny - 1)
+    weight = pkg.species_weight(dom, 8)
+    masses = {0: pkg.ELECTRON["mass_kg"], 1: pkg.ARGON_ION["mass_kg"]}
+    temps = {0: 2.0, 1: 0.026}
+    pl.set_species(0, pkg.ELECTRON["charge_C"], masses[0], weight, temps[0] * pkg.EVTOK, nel * 16)
+    pl.set_species(1, pkg.ARGON_ION["charge_C"], masses[1], weight, temps[1] * pkg.EVTOK, nel * 16)
+    pl.load_maxwellian(0, 8, temps[0], 5)
+    pl.load_maxwellian(1, 8, temps[1], 6)
+    pl.set_run_params(20, 77, 0)
+    norm = dom.normalisation()
+    y_lo, y_hi = 0.0, float(dom.ymax)
+    src = {}
+    for s in (0, 1):
+        xm = masses[s] / pkg.ELECTRON_MASS
+        tmpj = temps[s] * pkg.EVTOK / (dom.T_ref_eV * pkg.EVTOK)
+        vt = np.sqrt(tmpj / xm)
+        drift = (0.1 * vt * norm["v_ref"] * (1 if s == 0 else 20), 0.02 * norm["v_ref"], -0.01 * norm["v_ref"])
+        vd = [d / norm["v_ref"] for d in drift]
+        c = orc.inject_consts(vd[0], vt, tmpj, xm, 0.0, dom.dt)
+        dy = y_hi - y_lo
+        per_dens = abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * np.pi * (dy * dy) * \
+            (norm["L_ref"] * norm["L_ref"]) / weight
+        dens = (3000.0 if s == 0 else 500.0) / per_dens
+        pl.set_injection(s, dens, temps[s], drift, r_start=0.0, y_lo=y_lo, y_hi=y_hi)
+        expect = (dens / dom.n_ref) * dom.n_ref * abs(c[0]) * norm["v_ref"] * dom.dt * norm["time_ref"] * np.pi * \
+            (dy * dy) * (norm["L_ref"] * norm["L_ref"]) / weight
+        src[s] = (c, vd, expect)
+    before = [pl.counts(s)["nslots"] for s in (0, 1)]
+    th_before = [pl.azimuth(s) for s in (0, 1)]
+    added = pl.InjectBeams_2D(5, 1)
+    say(f"  added {added[:2]}")
+    assert added[0] > 2000 and added[1] > 300
+    for s in (0, 1):
+        c, vd, expect = src[s]
+        tg = orc.tag(s, orc.RNG_INJECT)
+        assert added[s] == orc.inject_count(expect, 77, 5, tg)
+        x, y, vx, vy, vz, cell, th = orc.inject(bits, added[s], 1 if s == 0 else 0, c, vd, y_lo, y_hi, 77, 5, tg,
+                                                dom.xmin, dom.ymin, 1.0, 1.0, dom.nnx - 1, dom.nny - 1, delta=1)
+        p = pl.particles(s)
+        n = added[s]
+        assert pl.counts(s)["nslots"] == before[s] + n
+        for k, ref in (("x", x), ("y", y), ("vx", vx), ("vy", vy), ("vz", vz)):
+            assert np.array_equal(p[k][-n:], ref), (s, k)
+        assert np.array_equal(p["loc"][-n:] - 1, cell)
+        az = pl.azimuth(s)
+        assert np.array_equal(az[:-n], th_before[s]) and np.array_equal(az[-n:], th)
+        assert np.all(cell >= 0) and np.all(y < y_hi) and np.all((th >= 0) & (th < 2 * np.pi))
+    pl.initial_field()
+    pl.step(1)

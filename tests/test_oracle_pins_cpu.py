@@ -300,3 +300,56 @@ def test_patch_lists_next_to_an_object_hand_derived():
     # planted: interface elements taken as vacuum give the plain 3 x 3 patches instead
     getb, blockb, _, _ = _patches(8, [(4.0, 4.0, 1.5)], treat_interface_as_vacuum=True)
     assert getb(2, 4) == blockb(1, 3, 3, 5) != get(2, 4)
+
+
+# ------------------------------------------------------------------ axisymmetric wall injection
+def test_axisymmetric_injection_fills_the_disk_uniformly(orc):
+    """InjectBeams_2D on an axisymmetric domain (PosInit, ParticleOne.f90:95-98): the injected
+    particles are uniform over the annulus y_lo <= r <= y_hi in area, i.e. r^2 is uniform, and
+    the azimuth is uniform on [0, 2 pi); planted: r drawn uniform in r is told apart."""
+    c = orc.inject_consts(0.05, 0.5, 0.25, 1.0, 0.0, 0.05)
+    y_lo, y_hi, n = 0.5, 6.0, 20000
+    out = orc.inject(64, n, 1, c, [0.05, 0.0, 0.0], y_lo, y_hi, 9, 3, orc.tag(0, orc.RNG_INJECT), 0.0, 0.0, 1.0,
+                     1.0, 8, 8, delta=1)
+    r, th = out[1], out[6]
+    u = (r ** 2 - y_lo ** 2) / (y_hi ** 2 - y_lo ** 2)
+    assert stats.kstest(u, stats.uniform(0, 1).cdf).pvalue > 1e-3
+    assert stats.kstest(th, stats.uniform(0, 2 * np.pi).cdf).pvalue > 1e-3
+    assert r.min() >= y_lo and r.max() < y_hi
+    with orc.planted(5):
+        rb = orc.inject(64, n, 1, c, [0.05, 0.0, 0.0], y_lo, y_hi, 9, 3, orc.tag(0, orc.RNG_INJECT), 0.0, 0.0, 1.0,
+                        1.0, 8, 8, delta=1)[1]
+    ub = (rb ** 2 - y_lo ** 2) / (y_hi ** 2 - y_lo ** 2)
+    assert stats.kstest(ub, stats.uniform(0, 1).cdf).pvalue < 1e-6
+
+
+# ------------------------------------------------------------------ kappa and polytropic initial loads
+def kappa_load_cdf(s, kappa=2.0):
+    """CDF of the kappa load's Vx: density (1 + V^2 / ((2 kappa - 3) s^2))^-kappa on |V| <= 10 s
+    (numerical quadrature)."""
+    f = lambda V: (1.0 + V * V / ((2 * kappa - 3) * s * s)) ** (-kappa)
+    tot = integrate.quad(f, -10 * s, 10 * s, epsabs=0, epsrel=1e-12, limit=200)[0]
+    return np.vectorize(lambda V: integrate.quad(f, -10 * s, min(max(V, -10 * s), 10 * s), epsabs=0, epsrel=1e-12,
+                                                 limit=200)[0] / tot)
+
+
+def load_scale(T, M, kind):
+    """s = sqrt(Sigma / Sigma_e) of ParticleOne.f90:167-168 (kappa) and :216-217 (polytropic)."""
+    kB, Te, Me = 1.3807e-23, 11605.0, 9.1095e-31
+    return np.sqrt((2.0 * kB * T / M) / (2.0 * kB * Te / Me))
+
+
+def test_kappa_and_polytropic_loads(orc):
+    """VelKappaInit draws Vx from the kappa = 2 law (a scaled Student-t with 3 degrees of
+    freedom, cut at 10 s) and VelPolyInit (gamma = 3) uniform on +-sqrt(3) s, Vy = Vz = 0;
+    planted: the acceptance density with half its exponent is told apart."""
+    for T, M in ((23210.0, 9.1095e-31), (301.7, 6.69e-26)):
+        s = load_scale(T, M, 0)
+        v = orc.load_velocity(0, 5, 20000, T, M)
+        assert np.all(v[:, 1:] == 0.0)
+        assert stats.kstest(v[:, 0], kappa_load_cdf(s)).pvalue > 1e-3
+        p = orc.load_velocity(2, 5, 20000, T, M)[:, 0]
+        assert stats.kstest(p, stats.uniform(-np.sqrt(3) * s, 2 * np.sqrt(3) * s).cdf).pvalue > 1e-3
+    with orc.planted(6):
+        vb = orc.load_velocity(0, 5, 20000, 23210.0, 9.1095e-31)[:, 0]
+    assert stats.kstest(vb, kappa_load_cdf(load_scale(23210.0, 9.1095e-31, 0))).pvalue < 1e-6
