@@ -11,7 +11,6 @@
 // MCCIon / MCCChi drawing from the particle's own (slot, step, species) Philox stream.
 // Ionisation products are staged per selected particle and appended in selected order.
 #include "pic_device.h"
-#include <hipcub/hipcub.hpp>
 
 // A wave owns 2048 consecutive slots as 8 chunks of 256; in chunk j lane l tests slots
 // wave_base + 256 j + 4 l + (0..3) (one int4 of cell, one Philox call) and keeps the
@@ -504,9 +503,129 @@ int run_mcc() {
 }
 
 // ---------------------------------------------------------------- cell sort
-// Stable radix sort of (tile-major element key, slot) with dead slots keyed past the end,
-// then one gather of every SoA array into spare arrays that are swapped in (group-transposed
-// layout, pic_sorted_position); occupied slots shrink to the live count.
+// Stable LSD radix sort of (tile-major element key, slot) with dead slots keyed past the end,
+// 8-bit digits, one pass per digit the largest key needs; then one gather of every SoA array
+// into spare arrays that are swapped in (group-transposed layout, pic_sorted_position);
+// occupied slots shrink to the live count. A pass is three launches:
+//   k_rs_hist    each block of RS_BLOCK_ITEMS consecutive items counts its digits (LDS),
+//                written digit-major: hist[d * nblocks + b]
+//   scan         exclusive prefix of hist (k_rs_scan_local / k_rs_scan_top / k_rs_scan_add):
+//                the first output position of digit d from block b
+//   k_rs_scatter the block walks its items in rounds of 256 (one per thread); a wave finds the
+//                lanes with its digit by 8 ballots, ranks them by lane, and adds the counts of
+//                the waves before it and of the rounds before; so equal digits keep the order
+//                of their slots (stable), and the result does not depend on scheduling.
+#define RS_THREADS 256
+#define RS_ROUNDS 16
+#define RS_BLOCK_ITEMS (RS_THREADS * RS_ROUNDS)
+#define RS_SCAN_THREADS 1024
+#define RS_SCAN_ITEMS (RS_SCAN_THREADS * 4)
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int64_t n, const unsigned int* keys, int shift,
+                                                       unsigned int* hist, int nblocks) {
+    __shared__ unsigned int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_BLOCK_ITEMS;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// exclusive scan of a[0..len) in place: chunks of RS_SCAN_ITEMS, their totals, their offsets
+__device__ inline unsigned int rs_block_exclusive(unsigned int v, unsigned int* total) {
+    __shared__ unsigned int wsum[RS_SCAN_THREADS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned int incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        unsigned int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int acc = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) { unsigned int t = wsum[k]; wsum[k] = acc; acc += t; }
+        *total = acc;
+    }
+    __syncthreads();
+    const unsigned int r = wsum[w] + incl - v;
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(RS_SCAN_THREADS) void k_rs_scan_local(int64_t len, unsigned int* a, unsigned int* sums) {
+    __shared__ unsigned int tot;
+    const int64_t b0 = (int64_t)blockIdx.x * RS_SCAN_ITEMS + (int64_t)threadIdx.x * 4;
+    unsigned int v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { v[k] = (b0 + k < len) ? a[b0 + k] : 0u; s += v[k]; }
+    unsigned int ex = rs_block_exclusive(s, &tot);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { if (b0 + k < len) a[b0 + k] = ex; ex += v[k]; }
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ __launch_bounds__(RS_SCAN_THREADS) void k_rs_scan_top(int nsums, unsigned int* sums) {
+    __shared__ unsigned int tot;
+    unsigned int carry = 0;
+    for (int c0 = 0; c0 < nsums; c0 += RS_SCAN_THREADS) {
+        const int i = c0 + threadIdx.x;
+        const unsigned int v = i < nsums ? sums[i] : 0u;
+        const unsigned int ex = rs_block_exclusive(v, &tot);
+        if (i < nsums) sums[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(RS_SCAN_THREADS) void k_rs_scan_add(int64_t len, unsigned int* a, const unsigned int* sums) {
+    const int64_t b0 = (int64_t)blockIdx.x * RS_SCAN_ITEMS + (int64_t)threadIdx.x * 4;
+    const unsigned int off = sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) if (b0 + k < len) a[b0 + k] += off;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int64_t n, const unsigned int* kin, const unsigned int* vin,
+                                                          unsigned int* kout, unsigned int* vout, int shift,
+                                                          const unsigned int* hist, int nblocks) {
+    __shared__ unsigned int run[256];
+    __shared__ unsigned int wcount[RS_THREADS / 64][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    run[threadIdx.x] = hist[(size_t)threadIdx.x * nblocks + blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < RS_THREADS / 64; ++k) wcount[k][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_BLOCK_ITEMS;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + r * RS_THREADS + threadIdx.x;
+        const bool valid = i < n;
+        const unsigned int key = valid ? kin[i] : 0u;
+        const unsigned int val = valid ? vin[i] : 0u;
+        const unsigned int d = (key >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const unsigned long long bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const unsigned int lrank = (unsigned int)__popcll(peers & below);
+        if (valid && lrank == 0) wcount[w][d] = (unsigned int)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned int pos = run[d] + lrank;
+            for (int k = 0; k < w; ++k) pos += wcount[k][d];
+            kout[pos] = key; vout[pos] = val;
+        }
+        __syncthreads();
+        unsigned int add = 0;
+#pragma unroll
+        for (int k = 0; k < RS_THREADS / 64; ++k) { add += wcount[k][threadIdx.x]; wcount[k][threadIdx.x] = 0; }
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
 __global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsigned int* vals, unsigned int dead_key,
                             int ney, int nty) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -554,8 +673,11 @@ static int sort_species(int isp) {
     unsigned int* v1 = v0 + n;
     int blocks = (int)((n + 255) / 256);
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, g.stream, n, S.cell, k0, v0, dead_key, g.mesh.ney, nty);
-    size_t tb = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
+    // digit histograms (256 per block) and the scan's chunk totals, sized for the capacity
+    const int64_t cap = std::max<int64_t>(n, S.capacity);
+    const int nb_cap = (int)((cap + RS_BLOCK_ITEMS - 1) / RS_BLOCK_ITEMS);
+    const int64_t hist_len_cap = (int64_t)256 * nb_cap;
+    const size_t tb = (size_t)(hist_len_cap + hist_len_cap / RS_SCAN_ITEMS + 2) * sizeof(unsigned int);
     if (g.sort_tmp_bytes < tb) {
         if (g.d_sort_tmp) (void)hipFree(g.d_sort_tmp);
         g.d_sort_tmp = nullptr;
@@ -563,7 +685,22 @@ static int sort_species(int isp) {
         HIPCHK(hipMalloc(&g.d_sort_tmp, bytes));
         g.sort_tmp_bytes = bytes;
     }
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(g.d_sort_tmp, tb, k0, k1, v0, v1, (int)n, 0, end_bit, g.stream));
+    const int nb = (int)((n + RS_BLOCK_ITEMS - 1) / RS_BLOCK_ITEMS);
+    const int64_t hist_len = (int64_t)256 * nb;
+    const int nchunks = (int)((hist_len + RS_SCAN_ITEMS - 1) / RS_SCAN_ITEMS);
+    unsigned int* hist = (unsigned int*)g.d_sort_tmp;
+    unsigned int* sums = hist + hist_len;
+    unsigned int *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+    for (int shift = 0; shift < end_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, g.stream, n, kin, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_scan_local, dim3(nchunks), dim3(RS_SCAN_THREADS), 0, g.stream, hist_len, hist, sums);
+        hipLaunchKernelGGL(k_rs_scan_top, dim3(1), dim3(RS_SCAN_THREADS), 0, g.stream, nchunks, sums);
+        hipLaunchKernelGGL(k_rs_scan_add, dim3(nchunks), dim3(RS_SCAN_THREADS), 0, g.stream, hist_len, hist, sums);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, g.stream, n, kin, vin, kout, vout, shift, hist, nb);
+        std::swap(kin, kout); std::swap(vin, vout);
+    }
+    HIPCHK(hipGetLastError());
+    unsigned int* perm = vin;             // the sorted slot order after the last pass
     // live count; the live particles occupy the first `live` sorted positions
     int64_t live = 0;
     TRY_MCC(count_live_slots(n, S.cell, &live));
@@ -575,13 +712,13 @@ static int sort_species(int isp) {
     const int gblocks = (int)((live + 255) / 256);
     if (live > 0) {
         if (esz == 8)
-            hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
+            hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, perm,
                                (const double*)S.x, (const double*)S.y, (const double*)S.vx, (const double*)S.vy,
                                (const double*)S.vz, S.cell, rz ? (const double*)S.th : nullptr,
                                (double*)S.spare[0], (double*)S.spare[1], (double*)S.spare[2], (double*)S.spare[3],
                                (double*)S.spare[4], S.cell_next, rz ? (double*)S.spare[5] : nullptr);
         else
-            hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, v1,
+            hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, perm,
                                (const float*)S.x, (const float*)S.y, (const float*)S.vx, (const float*)S.vy,
                                (const float*)S.vz, S.cell, rz ? (const float*)S.th : nullptr,
                                (float*)S.spare[0], (float*)S.spare[1], (float*)S.spare[2], (float*)S.spare[3],

@@ -676,3 +676,58 @@ def test_injection_axisymmetric_matches_oracle(gpu_lib, pkg, orc, bits):
         assert np.all(cell >= 0) and np.all(y < y_hi) and np.all((th >= 0) & (th < 2 * np.pi))
     pl.initial_field()
     pl.step(1)
+
+
+# ------------------------------------------------------------------ cell sort
+def _sort_keys(loc0, nex, ney, tile=16):
+    """The tile-major element key of the cell sort: 16 x 16-element tiles visited column by
+    column in serpentine order, elements column-major inside a tile."""
+    ci, cj = loc0 // ney, loc0 % ney
+    nty = (ney + tile - 1) // tile
+    ti, tj = ci // tile, cj // tile
+    tpos = np.where(ti & 1, nty - 1 - tj, tj)
+    return (ti * nty + tpos) * tile * tile + (ci % tile) * tile + cj % tile
+
+
+def _transposed(n, group=256):
+    """Slot d holds the particle at sorted position p(d): inside a full group of 256 slots the
+    sorted position 4 l + k sits at slot 64 k + l; a partial last group keeps the order."""
+    d = np.arange(n)
+    nfull = n // group * group
+    g0 = d & ~(group - 1)
+    r = d - g0
+    return np.where(d < nfull, g0 + (r & 63) * (group // 64) + (r >> 6), d)
+
+
+@pytest.mark.parametrize("case", ["small32", "rz32", "cfg3s64"])
+def test_cell_sort_is_the_stable_tile_order(gpu_lib, pkg, case):
+    """The hand-written radix sort: after steps that delete particles and append MCC products,
+    pic_sort_particles leaves exactly the live particles, in the stable order of their tile-major
+    element keys (ties in slot order), laid out group-transposed; every array (and the azimuth)
+    moves with its particle; a permutation: nothing lost or duplicated."""
+    pl = gpu_lib
+    say(f"case: cell sort {case}")
+    w = pkg.build_workload(pl, case, seed=51, ppc=12, mcc=True, gas_density=3.2e24)
+    dom = w["domain"]
+    pl.set_run_params(0, 51, 1)
+    pl.set_step(0)
+    pl.initial_field()
+    pl.step(5)
+    pres = [pl.particles(s) for s in (0, 1)]
+    ths = [pl.azimuth(s) if dom.delta == 1 else None for s in (0, 1)]
+    c0 = pl.counts(0)
+    assert c0["nslots"] > c0["nlive"]                # dead slots to drop
+    pl.sort_particles()
+    for s in (0, 1):
+        pre, th = pres[s], ths[s]
+        keys = _sort_keys(pre["loc"] - 1, dom.nnx - 1, dom.nny - 1)
+        order = np.argsort(keys, kind="stable")
+        pos = order[_transposed(len(order))]
+        post = pl.particles(s)
+        assert pl.counts(s)["nslots"] == len(pre["x"]) == pl.counts(s)["nlive"]
+        for k in ("x", "y", "vx", "vy", "vz", "loc"):
+            assert np.array_equal(post[k], pre[k][pos]), (s, k)
+        if th is not None:
+            assert np.array_equal(pl.azimuth(s), th[pos])
+        # keys ascending in sorted order
+        assert np.all(np.diff(keys[order]) >= 0)
