@@ -15,7 +15,9 @@ The timed steps use the learned multigrid preconditioner (FCG inner solver, plai
 fallback), trained on the GPU on pairs the plain solver harvests during the warm-up; the
 same number of steps with the plain one-kernel CG is timed after them and reported under
 "plain", with a per-solve comparison on the same RHS and warm start. --no-learned times the
-plain solver only.
+plain solver only. --workload cfg5 (1024^2, 2 x 33.1M particles) also fine-tunes the network
+online inside the step every 500 steps (harvesting solve on the plain CG, then Adam steps on
+the GPU): one such step is timed apart and its cost amortised over the period.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3] [--no-cpu-baseline]
 """
@@ -46,6 +48,9 @@ WORKLOAD_TEXT = {
     "cfg2": "256x256 axisymmetric r-z, a conductor and a dielectric disk (eps 4, IFE interface elements), "
             "2 species x 3.9M particles, FP32 particles + mixed-precision field solve to 1e-8, MCC, sort/20",
     "cfg1": "64x64 Cartesian, one conductor, 2 species x 410k particles, FP64 particles and solve to 1e-10, MCC",
+    "cfg5": "1024x1024 Cartesian, the 3 conductors of the 512^2 layout scaled, 2 species x 33.1M particles, "
+            "left-wall injection, FP32 particles + learned-multigrid FCG to 1e-8 fine-tuned online every 500 "
+            "steps on plain-solver pairs, MCC, sort/20",
 }
 
 
@@ -197,7 +202,7 @@ def solve_compare(pl, reps: int) -> dict:
     phi0 = pl.field("PHI")
     pl.step(1)
     rho = pl.field("RHO")
-    res = {}
+    res, phis = {}, {}
     for name, mode in (("plain", 1), ("learned", 2)):
         pl.set_solver(mode)
         times, its = [], []
@@ -210,7 +215,10 @@ def solve_compare(pl, reps: int) -> dict:
             its.append(info["iterations"])
         res[name] = {"ms_per_solve": 1e3 * float(np.median(times)), "iterations": int(np.median(its)),
                      "relres": info["relres"], "outer": info["outer"]}
+        phis[name] = phi
     pl.set_solver(1)
+    res["phi_rel_l2_learned_vs_plain"] = float(np.linalg.norm(phis["learned"] - phis["plain"]) /
+                                               np.linalg.norm(phis["plain"]))
     return res
 
 
@@ -218,6 +226,7 @@ def timed_steps(pl, steps: int) -> dict:
     """K explicit steps with every input resident in HBM; device phase times and kernel
     statistics from HIP events on the library's stream."""
     n0 = pl.counts(0)["nlive"] + pl.counts(1)["nlive"]
+    step0 = pl.get_step()
     t0 = time.perf_counter()
     pl.step(steps)
     elapsed = time.perf_counter() - t0
@@ -225,7 +234,7 @@ def timed_steps(pl, steps: int) -> dict:
     phase = pl.phase_times()
     ks = pl.kernel_stats_full()
     solve = pl.last_solve()
-    return {"n0": n0, "n1": n1, "elapsed": elapsed, "value": 0.5 * (n0 + n1) * steps / elapsed,
+    return {"n0": n0, "n1": n1, "step0": step0, "elapsed": elapsed, "value": 0.5 * (n0 + n1) * steps / elapsed,
             "ms_per_step": 1e3 * elapsed / steps, "phase": phase, "ks": ks, "solve": solve}
 
 
@@ -372,6 +381,9 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--solve-reps", type=int, default=5)
     ap.add_argument("--no-fp64-error", action="store_true", help="skip the FP64 comparison step (it ends the run)")
+    ap.add_argument("--finetune-every", type=int, default=None,
+                    help="online fine-tuning period in steps (default: 500 for cfg5, off otherwise)")
+    ap.add_argument("--finetune-train-steps", type=int, default=50)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -422,9 +434,31 @@ def main() -> int:
         pl.set_solver(2)
         if n_learned_warm:
             pl.step(n_learned_warm)      # the last warm-up steps run the learned solver
+    ft_every = args.finetune_every if args.finetune_every is not None else (500 if args.workload == "cfg5" else 0)
+    ft_every = ft_every if learned else 0
+    if ft_every:
+        # fine-tuning runs inside pic_step; with a period longer than the timed window the window
+        # holds ordinary steps, and one fine-tune step is timed apart below and amortised
+        pl.learn_finetune(ft_every, args.finetune_train_steps, 4, 1e-3, 21)
     log(f"bench: timing {args.steps} steps ({'learned' if learned else 'plain'} solver)")
     main_run = timed_steps(pl, args.steps)
     fallbacks = pl.learn_info()["fallbacks"] if learned else 0
+    ft = None
+    if ft_every:
+        fires = any((main_run["step0"] + k) % ft_every == 0 and main_run["step0"] + k > 0 for k in range(args.steps))
+        pl.learn_finetune(1, args.finetune_train_steps, 4, 1e-3, 21)
+        one = timed_steps(pl, 1)
+        pl.learn_finetune(ft_every, args.finetune_train_steps, 4, 1e-3, 21)
+        fi = pl.learn_finetune_info()
+        ft = {"every": ft_every, "train_steps_per_finetune": args.finetune_train_steps, "finetunes": fi["count"],
+              "finetune_in_timed_window": fires, "finetune_step_ms": one["ms_per_step"],
+              "train_ms_total": fi["train_ms"], "last_loss": fi["last_loss"],
+              "harvest_solve_iterations": one["solve"]["iterations"],
+              "amortised_ms_per_step": main_run["ms_per_step"] + (one["ms_per_step"] - main_run["ms_per_step"]) / ft_every,
+              "fallbacks_after": pl.learn_info()["fallbacks"]}
+        ft["amortised_value"] = main_run["value"] * main_run["ms_per_step"] / ft["amortised_ms_per_step"]
+        log(f"bench: fine-tune step {ft}")
+        pl.learn_finetune(0)
     plain_run = main_run
     if learned:
         pl.set_solver(1)
@@ -464,6 +498,8 @@ def main() -> int:
                           "iteration_ratio": cmp_["plain"]["iterations"] / max(cmp_["learned"]["iterations"], 1),
                           "speedup_per_solve": cmp_["plain"]["ms_per_solve"] / cmp_["learned"]["ms_per_solve"],
                           "step_speedup": plain_run["ms_per_step"] / main_run["ms_per_step"]}
+        if ft:
+            out["learned"]["online_finetune"] = ft
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle, 1 core)")
         try:

@@ -419,6 +419,13 @@ int  pic_learn_eval(int32_t npairs, double* mean_loss);
 /* Pairs in the ring, FCG iterations of the last learned solve, fallbacks to the plain CG so
  * far, last training loss. */
 int  pic_learn_info(int32_t* dataset, int32_t* last_fcg_iterations, int64_t* fallbacks, double* last_loss);
+/* Online fine-tuning inside pic_step (solver_mode 2): every `every` steps (0 = off) that step's
+ * solve runs the plain inner CG, harvesting (r, e) pairs into the ring, and after the step's
+ * MCC the network takes train_steps Adam steps (minibatch `batch`, learning rate lr) on the
+ * ring; the plain CG stays the fallback on stagnation. Info: fine-tunes done, milliseconds spent
+ * training (host clock), loss of the last minibatch. */
+int  pic_learn_finetune(int32_t every, int32_t train_steps, int32_t batch, double lr, uint64_t seed);
+int  pic_learn_finetune_info(int64_t* count, double* train_ms, double* last_loss);
 /* z = M r on host arrays of nnodes floats (node numbering). */
 int  pic_learn_apply(const float* r, float* z, int64_t n);
 int  pic_learn_get_params(float* dst, int64_t n, int64_t* nparams);

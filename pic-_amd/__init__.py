@@ -149,6 +149,8 @@ _PROTOS = {
     "pic_learn_train": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, _dp, _dp]),
     "pic_learn_eval": (C.c_int, [C.c_int32, _dp]),
     "pic_learn_info": (C.c_int, [_ip, _ip, _i64p, _dp]),
+    "pic_learn_finetune": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64]),
+    "pic_learn_finetune_info": (C.c_int, [_i64p, _dp, _dp]),
     "pic_learn_apply": (C.c_int, [_fp, _fp, C.c_int64]),
     "pic_learn_get_params": (C.c_int, [_fp, C.c_int64, _i64p]),
     "pic_learn_set_params": (C.c_int, [_fp, C.c_int64]),
@@ -596,6 +598,16 @@ class PicLibrary:
         ds, it, fb, loss = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
         check(lib().pic_learn_info(C.byref(ds), C.byref(it), C.byref(fb), C.byref(loss)))
         return {"dataset": ds.value, "last_fcg_iterations": it.value, "fallbacks": fb.value, "last_loss": loss.value}
+
+    def learn_finetune(self, every: int, train_steps: int = 100, batch: int = 4, lr: float = 1e-3, seed: int = 1):
+        """Online fine-tuning inside step(): every `every` steps (0 = off) the solve harvests
+        plain-CG pairs and the network trains train_steps steps on them."""
+        check(lib().pic_learn_finetune(every, train_steps, batch, lr, seed))
+
+    def learn_finetune_info(self) -> dict:
+        n, ms, loss = C.c_int64(), C.c_double(), C.c_double()
+        check(lib().pic_learn_finetune_info(C.byref(n), C.byref(ms), C.byref(loss)))
+        return {"count": n.value, "train_ms": ms.value, "last_loss": loss.value}
 
     def learn_apply(self, r) -> np.ndarray:
         r = np.ascontiguousarray(r, dtype=np.float32)

@@ -2,6 +2,7 @@
 // the per-step drop-ins and the stand-alone drop-ins.
 #include "pic_device.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -271,6 +272,7 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     g.nspecies = 0;
     g.step = 0;
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.moved[s] = false;
+    g.ft_every = 0; g.ft_count = 0; g.ft_ms = 0.0; g.ft_last_loss = 0.0;
     g.mcc_ready = false;
     g.have_domain = false;
     TRY(setup_build_domain(cfg));
@@ -993,13 +995,38 @@ int pic_step(int nsteps) {
         HIPCHK(hipEventRecord(g.ev[2], g.stream));
         TRY(launch_rhs());
         HIPCHK(hipEventRecord(g.ev[3], g.stream));
-        TRY(g.solver_mode == 0 ? run_pcg_fp64(true) : run_pcg_mixed());
+        const bool ft = g.ft_every > 0 && g.solver_mode == 2 && learn_ready() && g.step > 0 &&
+                        g.step % g.ft_every == 0;
+        if (ft) {
+            // online fine-tuning: this solve runs the plain inner CG and harvests pairs
+            g.solver_mode = 1;
+            int rc = learn_set_harvest(1, 0, 1);
+            if (!rc) rc = run_pcg_mixed();
+            g.solver_mode = 2;
+            int rc2 = learn_set_harvest(0, 0, 1);
+            TRY(rc); TRY(rc2);
+        } else {
+            TRY(g.solver_mode == 0 ? run_pcg_fp64(true) : run_pcg_mixed());
+        }
         HIPCHK(hipEventRecord(g.ev[4], g.stream));
         TRY(launch_ppr());
         if (injection_on()) TRY(run_injection((int)g.step, nullptr));     // Main_IFE_Test_2.f90:344
         HIPCHK(hipEventRecord(g.ev[5], g.stream));
         if (g.mcc_enabled && g.mcc_ready) TRY(run_mcc());
         HIPCHK(hipEventRecord(g.ev[6], g.stream));
+        if (ft) {
+            // the training of the fine-tune, timed apart (host clock around the synchronous calls)
+            HIPCHK(hipStreamSynchronize(g.stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            double l0 = 0.0, l1 = 0.0;
+            int32_t nset = 0, last_it = 0; int64_t nfb = 0; double lloss = 0.0;
+            TRY(learn_info(&nset, &last_it, &nfb, &lloss));
+            if (nset > 0)        // a solve that converged before its first snapshot harvests nothing
+                TRY(learn_train(g.ft_train, g.ft_batch, g.ft_lr, g.ft_seed + (uint64_t)g.ft_count, &l0, &l1));
+            g.ft_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            g.ft_count++;
+            g.ft_last_loss = l1;
+        }
         advance_step();
         if (g.sort_every > 0 && g.step % g.sort_every == 0) TRY(run_sort_all());
         HIPCHK(hipEventRecord(g.ev[7], g.stream));
@@ -1254,6 +1281,7 @@ int pic_learn_setup(int32_t max_levels, const int32_t* channels, int32_t nchanne
 }
 int pic_learn_harvest(int32_t enable, int32_t capacity, int32_t every) {
     REQUIRE_DOMAIN();
+    if (capacity < 1) return pic_fail(PIC_ERR_ARG, "capacity must be positive");
     return learn_set_harvest(enable, capacity, every);
 }
 int pic_learn_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
@@ -1264,6 +1292,19 @@ int pic_learn_eval(int32_t npairs, double* mean_loss) {
     REQUIRE_DOMAIN();
     if (!mean_loss) return pic_fail(PIC_ERR_ARG, "mean_loss is NULL");
     return learn_eval(npairs, mean_loss);
+}
+int pic_learn_finetune(int32_t every, int32_t train_steps, int32_t batch, double lr, uint64_t seed) {
+    REQUIRE_DOMAIN();
+    if (every < 0 || train_steps < 0 || batch < 1 || batch > 64 || !(lr > 0)) return pic_fail(PIC_ERR_ARG, "bad fine-tune parameters");
+    if (every > 0 && !learn_ready()) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
+    g.ft_every = every; g.ft_train = train_steps; g.ft_batch = batch; g.ft_lr = lr; g.ft_seed = seed;
+    return PIC_OK;
+}
+int pic_learn_finetune_info(int64_t* count, double* train_ms, double* last_loss) {
+    if (count) *count = g.ft_count;
+    if (train_ms) *train_ms = g.ft_ms;
+    if (last_loss) *last_loss = g.ft_last_loss;
+    return PIC_OK;
 }
 int pic_learn_info(int32_t* dataset, int32_t* last_fcg_iterations, int64_t* fallbacks, double* last_loss) {
     return learn_info(dataset, last_fcg_iterations, fallbacks, last_loss);

@@ -1704,6 +1704,7 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
 // --------------------------------------------------------------- data set
 int learn_set_harvest(int enable, int capacity, int every) {
     if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
+    if (capacity == 0) capacity = ln.ds_cap > 0 ? ln.ds_cap : 64;      // keep the current ring
     if (capacity < 1 || every < 1) return pic_fail(PIC_ERR_ARG, "capacity and every must be positive");
     const size_t N = (size_t)ln_npix0();
     if (capacity != ln.ds_cap) {

@@ -203,6 +203,13 @@ struct PicState {
     unsigned int* d_wall_count = nullptr;
     int64_t wall_cap = 0;
     int learn_maxit = 400;             // FCG iterations per inner solve before handing over
+    // online fine-tuning inside pic_step: every ft_every steps the step's solve runs the plain
+    // inner CG harvesting pairs, then ft_train Adam steps train the network on the ring
+    int ft_every = 0, ft_train = 0, ft_batch = 4;
+    double ft_lr = 1e-3;
+    uint64_t ft_seed = 1;
+    int64_t ft_count = 0;
+    double ft_ms = 0.0, ft_last_loss = 0.0;
     // learned refinement: passes and FCG iterations per pass of the last learned solve, the
     // launch budget of the next one (all its passes go out before one host synchronisation)
     int lp_passes = 2;

@@ -328,6 +328,40 @@ def test_cfg3_product_configuration_properties(gpu_lib, pkg):
     assert n[0]["nlive"] > 16_000_000 and n[1]["nlive"] > 16_000_000
 
 
+def test_online_finetune_inside_step(gpu_lib, pkg):
+    """Online fine-tuning on the 128^2 cfg3 geometry: every 3rd step's solve runs the plain CG
+    and harvests pairs, the network then trains 30 Adam steps; every step still meets the solve
+    tolerance, no fallback, and the count, time and loss are reported."""
+    pl = gpu_lib
+    say("case: online fine-tune, cfg3s128, every 3 steps")
+    w = pkg.build_workload(pl, "cfg3s128", seed=2024)
+    pl.set_step(0)
+    pl.initial_field()
+    pl.learn_setup(12, (16, 16), 1)
+    pl.learn_harvest(True, 256, 1)
+    pl.step(2)
+    pl.learn_train(60, 4, 1e-3, 1)
+    pl.learn_harvest(False)
+    pl.set_solver(2)
+    fb0 = pl.learn_info()["fallbacks"]
+    pl.learn_finetune(3, 30, 4, 1e-3, 7)
+    its = []
+    for k in range(7):                       # counter 2..8 before each step: fine-tunes at 3 and 6
+        pl.step(1)
+        info = pl.last_solve()
+        assert info["relres"] <= w["tol"], (k, info)
+        its.append(info["iterations"])
+    ft = pl.learn_finetune_info()
+    pl.learn_finetune(0)
+    pl.set_solver(1)
+    say(f"  iterations {its}, fine-tune {ft}")
+    assert ft["count"] == 2
+    assert ft["train_ms"] > 0 and np.isfinite(ft["last_loss"]) and ft["last_loss"] > 0
+    assert pl.learn_info()["fallbacks"] == fb0
+    # the two fine-tune solves ran the plain inner CG: many more iterations than the learned ones
+    assert its[1] > 3 * its[0] and its[4] > 3 * its[3], its
+
+
 def _sigma_workload(E, electron):
     """The workloads' synthetic argon cross sections (m^2, E in eV; thresholds 11.55 and 15.76 eV),
     the gas the configurations define: electron elastic, excitation, ionisation; ion isotropic
