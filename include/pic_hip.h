@@ -396,17 +396,17 @@ int  pic_locate(int64_t n, const double* x, const double* y, int32_t* n_element)
 int  pic_generate_Elementmap(int32_t* Element_Map, int repeat_refinement);
 
 /* ------------------------------------------- learned preconditioner (no reference counterpart)
- * A multigrid-structured convolutional encoder-decoder on the node grid: a symmetric
- * V(1,1)-cycle over Galerkin levels (bilinear interpolation, full-weighting restriction,
- * exact coarsest solve) whose smoothers on the finest levels are learned convolutions
- * K = D^-1/2 In^T (Q Q^T) In D^-1/2 (BF16 MFMA, FP32 accumulation; damped Jacobi on the
- * coarser levels). It is the preconditioner of a flexible CG on the Jacobi-scaled system
+ * A multigrid V-cycle on the node grid: a symmetric V(1,1)-cycle over Galerkin levels
+ * (bilinear interpolation, full-weighting restriction, exact coarsest solve) whose smoothers
+ * on the finest levels are learned convolutions K = D^-1/2 In^T (Q Q^T) In D^-1/2 (In a 3x3
+ * convolution from one channel to C; applied as the collapsed 81 tap-pair weights in FP32;
+ * damped Jacobi on the coarser levels). It is the preconditioner of a flexible CG on the Jacobi-scaled system
  * (solver_mode 2), trained on the GPU by the library's own forward and backward kernels on
  * residual/error pairs (r = A' e) that the plain inner CG harvests: the solve's RHS with its
  * solution correction, and the iterates every `every` chunks of 32 iterations. */
 /* Build the network for the current domain: levels down to ~3x3 nodes (at most max_levels,
  * 2..12); the nchannels finest levels (0..4) get learned smoothers with channels[l] in
- * {16, 32, 64}; initialised to damped Jacobi (the untrained network is the classical
+ * 1..64; initialised to damped Jacobi (the untrained network is the classical
  * V-cycle). */
 int  pic_learn_setup(int32_t max_levels, const int32_t* channels, int32_t nchannels, uint64_t seed);
 /* Harvest pairs from the plain inner solves into a device ring of `capacity` pairs. */
@@ -431,7 +431,7 @@ int  pic_learn_apply(const float* r, float* z, int64_t n);
 int  pic_learn_get_params(float* dst, int64_t n, int64_t* nparams);
 int  pic_learn_set_params(const float* src, int64_t n);
 /* levels[0] = levels, levels[1] = learned smoother levels Ls; H, W per level (12 entries);
- * C per learned level; parameter offsets (2*Ls+1 entries: In_0..In_{Ls-1} [9][C][16],
+ * C per learned level; parameter offsets (2*Ls+1 entries: In_0..In_{Ls-1} [9][C],
  * Q_0..Q_{Ls-1} [C][C], total). */
 int  pic_learn_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* C, int64_t* offsets);
 /* Replace the ring by npairs host pairs (r, e), each nnodes floats. */

@@ -1,5 +1,4 @@
-"""CPU restatement of the learned multigrid preconditioner (numpy/scipy, FP64 with optional
-BF16 emulation).
+"""CPU restatement of the learned multigrid preconditioner (numpy/scipy, FP64).
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/ as the checker of the GPU kernels in
 pic-_amd/csrc/k_learn.hip, never by the library. The network has no counterpart in the
@@ -16,12 +15,16 @@ Symmetric V(1,1)-cycle y = V_l(r):
     x2 = x + P_l ec
     y  = x2 + K_l (r - A_l x2)
 Smoothers: learned on levels l < Ls,  K_l = Dh In_l^T S_l In_l Dh  (Dh = diag(A_l)^-1/2,
-In_l a 3x3 conv 1 -> C_l channels, S_l = Q_l Q_l^T), damped Jacobi omega/diag elsewhere.
+In_l a 3x3 conv 1 -> C_l channels with weights W_in[t][c], S_l = Q_l Q_l^T), damped Jacobi
+omega/diag elsewhere. Parameters (flat, the library's layout): per learned level W_in [9][C],
+then per learned level Q [C][C].
 Loss over pairs (r, e = A'^-1 r):  sum (y - e)^T A' (y - e) / (e^T A' e).
 For symmetric K and M, with g = dL/dy, the gradient of a learned smoother is
     dL/dK = s2(g) (x) r_l(r) + r_l(g) (x) s2(r)
 where r_l(.) and s2(.) are the inputs of its pre- and post-smoothing in the V-cycle run on
-r or on g (see tests/test_learned_cpu.py for the central-difference check).
+r or on g (see tests/test_learned_cpu.py for the central-difference check). This file keeps
+the factored form (lifts U = In Dh u, V = In Dh v, correlations over pixels); the library
+contracts the same gradient to a tap-pair Gram matrix, so agreement checks that algebra.
 """
 from __future__ import annotations
 
@@ -30,13 +33,6 @@ import scipy.sparse as sp
 
 TAPS = [(t // 3 - 1, t % 3 - 1) for t in range(9)]
 OMEGA = 0.8
-
-
-def bf16(a):
-    """Round to bfloat16 (nearest even), returned as float64."""
-    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
-    return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 def level_sizes(H0: int, W0: int, max_levels: int = 12):
@@ -55,7 +51,7 @@ def layout(H0: int, W0: int, channels, learned_levels: int = 2, max_levels: int 
     off, off_in, off_q = 0, [], []
     for l in range(Ls):
         off_in.append(off)
-        off += 9 * C[l] * 16
+        off += 9 * C[l]
     for l in range(Ls):
         off_q.append(off)
         off += C[l] * C[l]
@@ -64,7 +60,7 @@ def layout(H0: int, W0: int, channels, learned_levels: int = 2, max_levels: int 
 
 def unpack(params, lay):
     p = np.asarray(params, dtype=np.float64)
-    w_in = [p[o:o + 9 * c * 16].reshape(9, c, 16) for o, c in zip(lay["off_in"], lay["C"])]
+    w_in = [p[o:o + 9 * c].reshape(9, c, 1) for o, c in zip(lay["off_in"], lay["C"])]
     q = [p[o:o + c * c].reshape(c, c) for o, c in zip(lay["off_q"], lay["C"])]
     return w_in, q
 
@@ -132,7 +128,7 @@ def corr_same(o, x):
 
 
 def _as_chan(v, H, W):
-    x = np.zeros((H, W, 16))
+    x = np.zeros((H, W, 1))
     x[:, :, 0] = v.reshape(H, W)
     return x
 
@@ -140,16 +136,14 @@ def _as_chan(v, H, W):
 # ------------------------------------------------------------------ network
 class _Net:
     """The V-cycle in FP64 (the library applies the smoother as the collapsed FP32 stencil,
-    which equals the factored form up to rounding). BF16 emulation, when asked for, applies
-    to the factored lifts of the gradient, where the library uses BF16 MFMA."""
-    def __init__(self, params, lay, levels, emulate_bf16=False):
+    which equals the factored form up to rounding)."""
+    def __init__(self, params, lay, levels):
         self.lay, self.levels = lay, levels
-        self.rnd = bf16 if emulate_bf16 else (lambda a: a)
         w_in, q = unpack(params, lay)
         self.w_in, self.q = w_in, q
         self.S = [qq @ qq.T for qq in q]
-        self.SW = [np.einsum("oc,tci->toi", S, w) for S, w in zip(self.S, w_in)]   # S In, [9][C][16]
-        self.WT = [np.transpose(w, (0, 2, 1)) for w in w_in]                          # In^T, [9][16][C]
+        self.SW = [np.einsum("oc,tci->toi", S, w) for S, w in zip(self.S, w_in)]   # S In, [9][C][1]
+        self.WT = [np.transpose(w, (0, 2, 1)) for w in w_in]                          # In^T, [9][1][C]
 
     def smooth(self, l, v):
         lev = self.levels[l]
@@ -226,9 +220,9 @@ def forward(params, lay, levels, r):
     return y, sites
 
 
-def loss_and_grad(params, lay, levels, pairs, emulate_bf16: bool = False):
+def loss_and_grad(params, lay, levels, pairs):
     """Summed loss over pairs [(r, e)] and its gradient (flat, library layout)."""
-    net = _Net(params, lay, levels, emulate_bf16)
+    net = _Net(params, lay, levels)
     A = levels[0]["A"]
     m = levels[0]["mask"]
     grad = np.zeros(lay["nparams"])
@@ -251,12 +245,12 @@ def loss_and_grad(params, lay, levels, pairs, emulate_bf16: bool = False):
             dS = np.zeros_like(net.S[l])
             dW = np.zeros_like(net.w_in[l])
             for u, v in ((sg[l][1], sr[l][0]), (sg[l][0], sr[l][1])):
-                uu = _as_chan(net.rnd(dh * u), H, W)
-                vv = _as_chan(net.rnd(dh * v), H, W)
-                U = net.rnd(conv_same(uu, net.rnd(net.w_in[l])))
-                V = net.rnd(conv_same(vv, net.rnd(net.w_in[l])))
-                SU = net.rnd(conv_same(uu, net.rnd(net.SW[l])))
-                SV = net.rnd(conv_same(vv, net.rnd(net.SW[l])))
+                uu = _as_chan(dh * u, H, W)
+                vv = _as_chan(dh * v, H, W)
+                U = conv_same(uu, net.w_in[l])
+                V = conv_same(vv, net.w_in[l])
+                SU = conv_same(uu, net.SW[l])
+                SV = conv_same(vv, net.SW[l])
                 dS += np.einsum("pqo,pqi->oi", U, V)
                 dW += corr_same(SV, uu) + corr_same(SU, vv)
             dQ = (dS + dS.T) @ net.q[l]
@@ -282,10 +276,10 @@ def initial_params(lay, seed: int = 1, small: float = 0.01):
     p = np.zeros(lay["nparams"])
     for l in range(lay["learned"]):
         C = lay["C"][l]
-        w = p[lay["off_in"][l]: lay["off_in"][l] + 9 * C * 16].reshape(9, C, 16)
-        w[:, :, 0] = small * rng.uniform(-1, 1, (9, C))
-        w[:, 0, 0] = 0.0
-        w[4, 0, 0] = 1.0
+        w = p[lay["off_in"][l]: lay["off_in"][l] + 9 * C].reshape(9, C)
+        w[:, :] = small * rng.uniform(-1, 1, (9, C))
+        w[:, 0] = 0.0
+        w[4, 0] = 1.0
         q = p[lay["off_q"][l]: lay["off_q"][l] + C * C].reshape(C, C)
         q[:] = small * rng.uniform(-1, 1, (C, C))
         q[0, 0] = np.sqrt(OMEGA)

@@ -1,8 +1,7 @@
-// Learned preconditioner for the field solve: a multigrid-structured convolutional
-// encoder-decoder on the node grid whose smoothers are small convolutional networks (BF16
-// MFMA, FP32 accumulation). It is trained on the GPU with its own forward and backward
-// kernels on residual/error pairs harvested from the plain solver, and applied as the
-// preconditioner of a flexible CG (FCG(1)) on the Jacobi-scaled system.
+// Learned preconditioner for the field solve: a multigrid V-cycle on the node grid whose
+// smoothers on the finest levels are learned convolutions. It is trained on the GPU with its
+// own forward and backward kernels on residual/error pairs harvested from the plain solver,
+// and applied as the preconditioner of a flexible CG (FCG(1)) on the Jacobi-scaled system.
 //
 // Hierarchy (set up once per domain, FP64 on the host, stored FP32): level 0 is the
 // Jacobi-scaled node operator A' (unit diagonal on unknowns); level l+1 holds the nodes
@@ -15,25 +14,24 @@
 //   x  = K_l r;  rc = R_l (r - A_l x);  ec = V_{l+1}(rc);  x2 = x + P_l ec;
 //   y  = x2 + K_l (r - A_l x2)
 // Smoothers: on the finest levels l < Ls a learned convolution
-//   K_l = Dh In_l^T S_l In_l Dh    (Dh = diag(A_l)^-1/2, In_l: 3x3 conv 1 -> C_l channels,
-//                                   S_l = Q_l Q_l^T, so K_l is symmetric positive semi-definite)
+//   K_l = Dh In_l^T S_l In_l Dh    (Dh = diag(A_l)^-1/2, In_l: 3x3 conv 1 -> C_l channels with
+//                                   weights W_l[t][c], S_l = Q_l Q_l^T, so K_l is symmetric
+//                                   positive semi-definite)
 // initialised to damped Jacobi (In = identity tap on channel 0, S = omega e0 e0^T); damped
 // Jacobi omega/diag on the coarser levels. M is symmetric; the untrained M is the classical
 // Jacobi V-cycle.
+//
+// The smoother is linear with one input channel, so In^T S In collapses to 81 tap-pair
+// weights (a symmetric 5x5 stencil away from the grid edge), recomputed from the FP32 master
+// weights after every update; the V-cycle (solve and training forward) applies it in FP32.
 // Training minimises sum (M r - e)^T A' (M r - e) / (e^T A' e) over pairs r = A' e. With
 // g = dL/dy, the gradient of a learned smoother is dL/dK = s2(g) (x) r_l(r) + r_l(g) (x) s2(r)
-// (inputs of its pre-/post-smoothing in the V-cycles run on r and on g), which gives, with
-// U = In Dh u, V = In Dh v:
-//   dS += sum_p U V^T,  dIn += corr(S V, Dh u) + corr(S U, Dh v),  dQ = (dS + dS^T) Q.
-// corr(o, x)[t][co][ci] = sum_p o[co](p) x[ci](p + d_t).
-//
-// The smoother is linear, so In^T S In collapses to 81 tap-pair weights (a symmetric 5x5
-// stencil away from the grid edge), recomputed from the FP32 master weights after every
-// update; the V-cycle (solve and training forward) applies it in FP32. The factored network runs on MFMA where the factors matter, in the
-// gradient: the lifts U, V, S U, S V are NHWC bf16 (pixel-major, C contiguous, C a multiple
-// of 16, one pixel per node) and the weight correlations are MFMA reductions over pixels.
-// MFMA 16x16x16 bf16: a wave computes 16 pixels x 16 channels per instruction with K = 16
-// input channels of one tap (lifts) or K = 16 pixels (correlations).
+// (inputs of its pre-/post-smoothing in the V-cycles run on r and on g). In the factored
+// form, with U = In Dh u and V = In Dh v, dS = sum_p U V^T and dIn = corr(S V, Dh u) +
+// corr(S U, Dh v); both are linear in the 3x3 neighbourhoods of Dh u and Dh v, so they
+// contract exactly to the tap-pair Gram matrix of the two scalar fields (k_tapgram below):
+// the gradient needs one pass over four scalar fields per level, no lifted C-channel tensors
+// and no matrix-core work (there is no GEMM-shaped operand: the input has one channel).
 #include "pic_device.h"
 #include <algorithm>
 #include <cmath>
@@ -42,124 +40,12 @@
 #include <cstring>
 #include <vector>
 
-typedef uint16_t bf16;
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
 #define LN_MAX_LEVELS 12
 #define LN_MAX_LEARNED 4
 #define LN_KW 128                                    // smoother weights per level: k81 then k25
 #define LN_K25 96
 #define LN_OMEGA 0.8f
 #define LN_MAX_COARSE 2048
-
-__host__ __device__ inline bf16 f2bf(float f) {
-    // round to nearest even (finite inputs)
-#ifdef __HIP_DEVICE_COMPILE__
-    uint32_t u = __float_as_uint(f);
-#else
-    uint32_t u; std::memcpy(&u, &f, 4);
-#endif
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
-}
-
-enum { CM_SAME = 0, CM_SAME_T = 1, CM_NONE = 4, CM_CENTER = 5 };
-
-// source pixel of tap (di, dj) for output pixel (pi, pj); false if it does not exist
-__device__ inline bool conv_src(int mode, int pi, int pj, int di, int dj, int Hin, int Win, int& si, int& sj) {
-    if (mode == CM_SAME_T) { si = pi - di; sj = pj - dj; }
-    else if (mode == CM_SAME) { si = pi + di; sj = pj + dj; }
-    else { si = pi; sj = pj; }
-    return si >= 0 && si < Hin && sj >= 0 && sj < Win;
-}
-
-// ---------------------------------------------------------------- convolution (MFMA)
-struct ConvArgs {
-    const bf16* in; const float* in_f32; int Cin, Hin, Win; int64_t in_bs;
-    const float* in_scale;                           // in_f32 only: multiplied by in_scale[src]
-    const bf16* w;                                   // [9][Cout][Cin]
-    const bf16* in2; int Cin2; int64_t in2_bs; const bf16* w2;   // 1x1 term at the output pixel: [Cout][Cin2]
-    bf16* out; float* out_f32; int Cout, Hout, Wout; int64_t out_bs;
-    const float* out_scale;                          // out_f32 only: multiplied by out_scale[p]
-    const float* add_f32;                            // out_f32 only: + add_f32[b][p] (after scaling)
-    int mode;
-    const double* done;                              // skip the launch when *done != 0
-};
-
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv(ConvArgs a) {
-    if (a.done && *a.done != 0.0) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int b = blockIdx.y;
-    const int npix = a.Hout * a.Wout;
-    const int tile = (blockIdx.x * 4 + wv) * 16;
-    if (tile >= npix) return;                        // no barrier in this kernel
-    const int row = lane & 15, kq = lane >> 4;
-    const int pa = tile + row;
-    const bool va = pa < npix;
-    const int pi = va ? pa / a.Wout : 0, pj = va ? pa - (pa / a.Wout) * a.Wout : 0;
-    f32x4 acc[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16x4 zero4 = {0, 0, 0, 0};
-    if (a.mode != CM_NONE) {
-        const bf16* inb = a.in ? a.in + (int64_t)b * a.in_bs : nullptr;
-        const float* inf = a.in_f32 ? a.in_f32 + (int64_t)b * a.in_bs : nullptr;
-        for (int t = 0; t < 9; ++t) {
-            const int di = t / 3 - 1, dj = t % 3 - 1;
-            int si = 0, sj = 0;
-            bool ok = va && conv_src(a.mode, pi, pj, di, dj, a.Hin, a.Win, si, sj);
-            const int64_t sp = (int64_t)si * a.Win + sj;
-            for (int cb = 0; cb < a.Cin; cb += 16) {
-                bf16x4 av;
-                if (inf) {                             // one FP32 input channel (channel 0)
-                    float v = 0.0f;
-                    if (ok) { v = inf[sp]; if (a.in_scale) v = v * a.in_scale[sp]; }
-                    av = zero4;
-                    if (kq == 0 && cb == 0) av[0] = (short)f2bf(v);
-                } else {
-                    av = ok ? *(const bf16x4*)(inb + sp * a.Cin + cb + 4 * kq) : zero4;
-                }
-                const bf16* wt = a.w + (int64_t)t * a.Cout * a.Cin + cb + 4 * kq;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    bf16x4 bv = *(const bf16x4*)(wt + (int64_t)(nt * 16 + row) * a.Cin);
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nt], 0, 0, 0);
-                }
-            }
-        }
-    }
-    if (a.in2) {
-        const bf16* in2b = a.in2 + (int64_t)b * a.in2_bs;
-        for (int cb = 0; cb < a.Cin2; cb += 16) {
-            bf16x4 av = va ? *(const bf16x4*)(in2b + (int64_t)pa * a.Cin2 + cb + 4 * kq) : zero4;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                bf16x4 bv = *(const bf16x4*)(a.w2 + (int64_t)(nt * 16 + row) * a.Cin2 + cb + 4 * kq);
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[nt], 0, 0, 0);
-            }
-        }
-    }
-    // lane holds output rows (pixels) tile + 4*kq + r, column (channel) nt*16 + row
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        int p = tile + 4 * kq + r;
-        if (p >= npix) continue;
-        if (a.out_f32) {
-            if (row == 0) {
-                float v = acc[0][r];
-                if (a.out_scale) v = v * a.out_scale[p];
-                if (a.add_f32) v = a.add_f32[(int64_t)b * a.out_bs + p] + v;
-                a.out_f32[(int64_t)b * a.out_bs + p] = v;
-            }
-        } else {
-            bf16* o = a.out + (int64_t)b * a.out_bs + (int64_t)p * a.Cout + row;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) o[nt * 16] = f2bf(acc[nt][r]);
-        }
-    }
-}
 
 // ---------------------------------------------------------------- multigrid transfer kernels
 // ---------------------------------------------------------------- FCG(1) on the scaled system
@@ -814,117 +700,158 @@ __global__ __launch_bounds__(256) void k_smooth_k81(int H, int W, const float* k
     ob[p] = add ? o0 + s : s;
 }
 
-// ---------------------------------------------------------------- weight-gradient correlation (MFMA)
-// part[blk][t][co][ci] = sum_{b, p in block} og[b][p][co] * in[b][src(p, t)][ci]
-struct CorrArgs {
-    const bf16* og; int Cout, Hout, Wout; int64_t og_bs;
-    const bf16* in; const float* in_f32; int Cin, Hin, Win; int64_t in_bs;
-    const float* in_scale;                           // in_f32 only: multiplied by in_scale[src]
-    int mode;                                        // CM_SAME, CM_DOWN or CM_CENTER (1 tap)
-    int batch, ppb;                                  // batch size, pixels per block (multiple of 16)
-    float* part;
+// ---------------------------------------------------------------- weight gradient: tap-pair Gram matrix
+// G[t][t2] = sum_{b, p on the grid} (u0~_b(p + d_t) v0~_b(p + d_t2) + u1~_b(p + d_t) v1~_b(p + d_t2))
+// with x~ = Dh x (zero off the grid), over the two pairings (u0, v0), (u1, v1) of one level;
+// d_t = (t / 3 - 1, t % 3 - 1) in (row, column). With Gs = G + G^T and S = Q Q^T the factored
+// gradient contracts exactly to
+//   dIn[t][c] = (Gs W S)[t][c],   dS + dS^T = W^T Gs W,   dQ = (W^T Gs W) Q
+// (corr(S V, u~) + corr(S U, v~) and sum_p U V^T with U = In u~, V = In v~, summed over the
+// pairings; U and V are linear in the 3 x 3 neighbourhoods of u~ and v~).
+// A block owns TG_I x TG_J nodes of one batch entry (blockIdx.z). It stages the four scaled
+// fields on the tile with a halo of 1 in LDS (each field read once), then each thread walks a
+// column of TG_R nodes with the 3-row window of every field in registers (12 LDS reads per
+// node for 162 FMAs). The 81 sums are reduced over the block (wave shuffles, then the four
+// waves in a fixed order) into one FP64 row of partials per block.
+#define TG_J 64
+#define TG_R 8
+#define TG_I (4 * TG_R)
+struct TapGramArgs {
+    int H, W;
+    const float* dh;
+    const float* f[4];                               // u0, v0, u1, v1
+    int64_t bs[4];                                   // batch strides (floats)
+    double* part;                                    // [block][81]
 };
 
-template <int MT, int NT>
-__global__ __launch_bounds__(256) void k_corr(CorrArgs a) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int row = lane & 15, kq = lane >> 4;
-    const int ntaps = a.mode == CM_CENTER ? 1 : 9;
-    const int npix = a.Hout * a.Wout;
-    const int p0 = blockIdx.x * a.ppb, p1 = min(npix, p0 + a.ppb);
-    const int64_t tsize = (int64_t)a.Cout * a.Cin;
-    float* part = a.part + (int64_t)blockIdx.x * ntaps * tsize;
-    for (int t = wv; t < ntaps; t += 4) {
-        const int di = a.mode == CM_CENTER ? 0 : t / 3 - 1, dj = a.mode == CM_CENTER ? 0 : t % 3 - 1;
-        const int smode = a.mode == CM_CENTER ? CM_SAME : a.mode;
-        f32x4 acc[MT][NT];
+__global__ __launch_bounds__(256) void k_tapgram(TapGramArgs a) {
+    constexpr int SH = TG_I + 2, SW = TG_J + 2;
+    __shared__ float sf[4][SH][SW];
+    __shared__ float red[4][81];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.y * TG_I, j0 = blockIdx.x * TG_J, b = blockIdx.z;
+    const float* fb[4];
 #pragma unroll
-        for (int i = 0; i < MT; ++i)
+    for (int q = 0; q < 4; ++q) fb[q] = a.f[q] + (int64_t)b * a.bs[q];
+    for (int k = tid; k < SH * SW; k += 256) {
+        const int li = k / SW, lj = k - li * SW;     // constant divisor
+        const int gi = i0 + li - 1, gj = j0 + lj - 1;
+        float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (gi >= 0 && gi < a.H && gj >= 0 && gj < a.W) {
+            const int64_t n = (int64_t)gi * a.W + gj;
+            const float d = a.dh[n];
 #pragma unroll
-            for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int b = 0; b < a.batch; ++b) {
-            const bf16* ogb = a.og + (int64_t)b * a.og_bs;
-            const bf16* inb = a.in ? a.in + (int64_t)b * a.in_bs : nullptr;
-            const float* inf = a.in_f32 ? a.in_f32 + (int64_t)b * a.in_bs : nullptr;
-            for (int pc = p0; pc < p1; pc += 16) {
-                // K index k = 4*kq + j  <->  pixel pc + k
-                int64_t sp[4]; bool okp[4]; bool vp[4];
+            for (int q = 0; q < 4; ++q) v[q] = fb[q][n] * d;
+        }
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    int p = pc + 4 * kq + j;
-                    vp[j] = p < p1;
-                    int pi = vp[j] ? p / a.Wout : 0, pj = vp[j] ? p - (p / a.Wout) * a.Wout : 0;
-                    int si = 0, sj = 0;
-                    okp[j] = vp[j] && conv_src(smode, pi, pj, di, dj, a.Hin, a.Win, si, sj);
-                    sp[j] = (int64_t)si * a.Win + sj;
-                }
-                bf16x4 av[MT], bv[NT];
+        for (int q = 0; q < 4; ++q) sf[q][li][lj] = v[q];
+    }
+    __syncthreads();
+    const int tx = tid & 63, ty = tid >> 6;
+    const int r0 = ty * TG_R;                        // first LDS row of the window (node row r0 + 1)
+    const bool colok = j0 + tx < a.W;
+    float G[81];
 #pragma unroll
-                for (int i = 0; i < MT; ++i) {
-                    int co = i * 16 + row;
+    for (int k = 0; k < 81; ++k) G[k] = 0.0f;
+    float w[4][3][3];                                // [field][window row][column tx .. tx + 2]
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        int p = pc + 4 * kq + j;
-                        av[i][j] = vp[j] ? (short)ogb[(int64_t)p * a.Cout + co] : (short)0;
-                    }
-                }
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int i = 0; i < NT; ++i) {
-                    int ci = i * 16 + row;
+        for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        short v = 0;
-                        if (okp[j]) {
-                            if (inf) {
-                                float xv = inf[sp[j]];
-                                if (a.in_scale) xv = xv * a.in_scale[sp[j]];
-                                v = (ci == 0) ? (short)f2bf(xv) : (short)0;
-                            }
-                            else v = (short)inb[sp[j] * a.Cin + ci];
-                        }
-                        bv[i][j] = v;
-                    }
-                }
+            for (int c = 0; c < 3; ++c) w[q][rr + 1][c] = sf[q][r0 + rr][tx + c];
 #pragma unroll
-                for (int i = 0; i < MT; ++i)
+    for (int r = 0; r < TG_R; ++r) {
 #pragma unroll
-                    for (int j = 0; j < NT; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                w[q][0][c] = w[q][1][c];
+                w[q][1][c] = w[q][2][c];
+                w[q][2][c] = sf[q][r0 + r + 2][tx + c];
+            }
+        if (colok && i0 + r0 + r < a.H) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const float u0 = w[0][t / 3][t % 3], u1 = w[2][t / 3][t % 3];
+#pragma unroll
+                for (int t2 = 0; t2 < 9; ++t2)
+                    G[9 * t + t2] = fmaf(u0, w[1][t2 / 3][t2 % 3], fmaf(u1, w[3][t2 / 3][t2 % 3], G[9 * t + t2]));
             }
         }
-        // D rows = co (4*kq + r + 16 i), columns = ci (row + 16 j)
-        float* pt = part + (int64_t)t * tsize;
+    }
+    const int lane = tid & 63;
 #pragma unroll
-        for (int i = 0; i < MT; ++i)
+    for (int k = 0; k < 81; ++k) {
+        float v = G[k];
 #pragma unroll
-            for (int j = 0; j < NT; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    int co = i * 16 + 4 * kq + r, ci = j * 16 + row;
-                    pt[(int64_t)co * a.Cin + ci] = acc[i][j][r];
-                }
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[ty][k] = v;
+    }
+    __syncthreads();
+    if (tid < 81) {
+        const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        a.part[blk * 81 + tid] = (((double)red[0][tid] + (double)red[1][tid]) + (double)red[2][tid]) + (double)red[3][tid];
     }
 }
 
-// grad[k] = sum over the regions, over their blocks, of part[off + blk*size + k] (fixed order)
-struct GradRegions { int64_t off[4]; int nblk[4]; int n; };
-__global__ void k_reduce_grad(int64_t size, const float* part, GradRegions rg, float* grad) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= size) return;
-    float s = 0.0f;
-    for (int q = 0; q < rg.n; ++q)
-        for (int b = 0; b < rg.nblk[q]; ++b) s += part[rg.off[q] + (int64_t)b * size + k];
-    grad[k] = s;
+// G[k] = sum over blocks of part[blk][k] (one workgroup per entry, fixed order)
+__global__ __launch_bounds__(256) void k_tapgram_sum(int nblk, const double* part, double* G) {
+    __shared__ double sh[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    double s = 0.0;
+    for (int b = tid; b < nblk; b += 256) s += part[(int64_t)b * 81 + k];
+    sh[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) G[k] = sh[0];
 }
 
-// dQ = (dS + dS^T) Q   (C x C)
-__global__ void k_qgrad(int C, const float* dS, const float* Q, float* dQ) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= C * C) return;
-    int i = k / C, j = k - i * C;
-    float s = 0.0f;
-    for (int m = 0; m < C; ++m) s += (dS[i * C + m] + dS[m * C + i]) * Q[m * C + j];
-    dQ[k] = s;
+// One learned level's gradient from its Gram matrix (one workgroup, C <= 64):
+// T = Gs W (9 x C), dIn = T S, dQ = (W^T T) Q.
+__global__ __launch_bounds__(256) void k_tapgrad(int C, const double* G, const float* Wt, const float* Q, float* dW,
+                                                 float* dQ) {
+    __shared__ float gs[81], w[9 * 64], T[9 * 64], S[64 * 64], M[64 * 64];
+    const int tid = threadIdx.x;
+    const int CC = C * C;
+    if (tid < 81) gs[tid] = (float)(G[tid] + G[(tid % 9) * 9 + tid / 9]);
+    for (int k = tid; k < 9 * C; k += 256) w[k] = Wt[k];
+    for (int k = tid; k < CC; k += 256) {
+        const int i = k / C, j = k - i * C;
+        float acc = 0.0f;
+        for (int m = 0; m < C; ++m) acc = fmaf(Q[i * C + m], Q[j * C + m], acc);
+        S[k] = acc;
+    }
+    __syncthreads();
+    for (int k = tid; k < 9 * C; k += 256) {
+        const int t = k / C, c = k - t * C;
+        float acc = 0.0f;
+        for (int t2 = 0; t2 < 9; ++t2) acc = fmaf(gs[9 * t + t2], w[t2 * C + c], acc);
+        T[k] = acc;
+    }
+    __syncthreads();
+    for (int k = tid; k < 9 * C; k += 256) {
+        const int t = k / C, c = k - t * C;
+        float acc = 0.0f;
+        for (int m = 0; m < C; ++m) acc = fmaf(T[t * C + m], S[m * C + c], acc);
+        dW[k] = acc;
+    }
+    for (int k = tid; k < CC; k += 256) {
+        const int i = k / C, j = k - i * C;
+        float acc = 0.0f;
+        for (int t = 0; t < 9; ++t) acc = fmaf(w[t * C + i], T[t * C + j], acc);
+        M[k] = acc;
+    }
+    __syncthreads();
+    for (int k = tid; k < CC; k += 256) {
+        const int i = k / C, j = k - i * C;
+        float acc = 0.0f;
+        for (int m = 0; m < C; ++m) acc = fmaf(M[i * C + m], Q[m * C + j], acc);
+        dQ[k] = acc;
+    }
 }
 
 __global__ void k_adam(int64_t n, float* p, const float* gr, float* m, float* v, float lr, float b1, float b2,
@@ -945,20 +872,27 @@ __global__ void k_adam(int64_t n, float* p, const float* gr, float* m, float* v,
 // p - d_t is on the grid (all but the outer ring of nodes) this is the 5 x 5 stencil
 //   k25[delta] = sum_{d_t2 - d_t = delta} k81[t][t2]           (summed in FP64, rounded once)
 // One block per level writes both: kw[0..80] = k81, kw[LN_K25 .. LN_K25+24] = k25.
-__global__ void k_pack_k81(int C, const float* W, const float* Q, float* kw) {
-    __shared__ float k81s[81];
+__global__ __launch_bounds__(128) void k_pack_k81(int C, const float* W, const float* Q, float* kw) {
+    __shared__ float S[64 * 64], WS[9 * 64], k81s[81];
     const int d = threadIdx.x;
+    for (int k = d; k < C * C; k += 128) {           // S = Q Q^T
+        const int i = k / C, j = k - i * C;
+        float acc = 0.0f;
+        for (int m = 0; m < C; ++m) acc += Q[i * C + m] * Q[j * C + m];
+        S[k] = acc;
+    }
+    __syncthreads();
+    for (int k = d; k < 9 * C; k += 128) {           // W S (9 x C)
+        const int t = k / C, c = k - t * C;
+        float acc = 0.0f;
+        for (int m = 0; m < C; ++m) acc += W[t * C + m] * S[m * C + c];
+        WS[k] = acc;
+    }
+    __syncthreads();
     if (d < 81) {
         const int t = d / 9, t2 = d - t * 9;
         float acc = 0.0f;
-        for (int c = 0; c < C; ++c) {
-            float wc = W[((int64_t)t * C + c) * 16];
-            for (int c2 = 0; c2 < C; ++c2) {
-                float sc = 0.0f;                         // S[c][c2]
-                for (int m = 0; m < C; ++m) sc += Q[c * C + m] * Q[c2 * C + m];
-                acc += wc * sc * W[((int64_t)t2 * C + c2) * 16];
-            }
-        }
+        for (int c = 0; c < C; ++c) acc += WS[t * C + c] * W[t2 * C + c];
         kw[d] = acc;
         k81s[d] = acc;
     }
@@ -971,22 +905,6 @@ __global__ void k_pack_k81(int C, const float* W, const float* Q, float* kw) {
                 if (t2 / 3 - t / 3 == di && t2 % 3 - t % 3 == dj) acc += (double)k81s[9 * t + t2];
         kw[LN_K25 + d] = (float)acc;
     }
-}
-
-// Packed bf16 weights of one learned smoother (the lifts of the gradient) from W_in
-// [9][C][16] (input channel 0) and Q [C][C]: pin = W_in, pSin = (Q Q^T) W_in per tap.
-__global__ void k_pack_smoother(int C, const float* W, const float* Q, bf16* pin, bf16* pSin) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 9 * C * 16) return;
-    int t = k / (C * 16), r = k - t * C * 16, co = r / 16, ci = r - co * 16;
-    pin[k] = f2bf(W[k]);
-    float s = 0.0f;
-    for (int c = 0; c < C; ++c) {
-        float sc = 0.0f;                                 // S[co][c] = sum_m Q[co][m] Q[c][m]
-        for (int m = 0; m < C; ++m) sc += Q[co * C + m] * Q[c * C + m];
-        s += sc * W[((int64_t)t * C + c) * 16 + ci];
-    }
-    pSin[k] = f2bf(s);
 }
 
 // The scaled operator A' (unit diagonal on unknown rows, eight scaled off-diagonals, Dia8)
@@ -1252,20 +1170,17 @@ struct LearnNet {
     float* d_cops = nullptr; int cops_n = 0;         // coarse workgroup operators in its LDS order
     int c_ooff[LN_MAX_LEVELS] = {0}, c_voff[LN_MAX_LEVELS] = {0}, c_ioff = -1, c_vtot = 0;
     void* d_ops = nullptr;                           // one allocation: dia, dinv, dh of all levels
-    // parameters (FP32, flat): per learned level W_in [9][C][16] then Q [C][C]
+    // parameters (FP32, flat): per learned level W_in [9][C] then Q [C][C]
     int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
-    float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr, *d_dS = nullptr;
-    bf16* d_pk = nullptr;
-    bf16 *pin[LN_MAX_LEARNED] = {nullptr}, *pSin[LN_MAX_LEARNED] = {nullptr};
+    float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;
+    double* d_G = nullptr;                           // tap-pair Gram matrices [Ls][81]
     float* d_k81 = nullptr;                          // collapsed smoother weights [Ls][LN_KW] (k81, k25)
     int adam_t = 0;
     // activations for a batch of bcap samples
     int bcap = 0;
     void* d_act = nullptr;
-    bf16 *U[LN_MAX_LEARNED] = {nullptr}, *V[LN_MAX_LEARNED] = {nullptr}, *SU[LN_MAX_LEARNED] = {nullptr},
-         *SV[LN_MAX_LEARNED] = {nullptr};
     float *y0 = nullptr, *gl = nullptr, *rb = nullptr, *eb = nullptr;   // [bcap][N]
-    float* d_part = nullptr; int64_t part_cap = 0;
+    double* d_gpart = nullptr;                       // Gram partials [blocks][81] (in d_act)
     double* d_lpart = nullptr; double* d_lsum = nullptr;
     int* d_idx = nullptr;
     double* d_eab = nullptr;                         // e^T A' e of the batch entries
@@ -1292,34 +1207,10 @@ static LearnNet ln;
 static int64_t ln_npix0() { return ln.lv[0].n; }
 
 static void free_learn() {
-    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_cops, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_dS, ln.d_pk, ln.d_act, ln.d_part,
+    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_cops, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_G, ln.d_act,
                     ln.d_lpart, ln.d_lsum, ln.d_idx, ln.d_eab, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     ln = LearnNet();
-}
-
-static int launch_conv(const ConvArgs& a, int batch) {
-    int npix = a.Hout * a.Wout;
-    dim3 grid((npix + 63) / 64, batch);
-    switch (a.Cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, grid, dim3(256), 0, g.stream, a); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, grid, dim3(256), 0, g.stream, a); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, grid, dim3(256), 0, g.stream, a); break;
-    default: return pic_fail(PIC_ERR_ARG, "learned network: channel count must be 16, 32 or 64");
-    }
-    HIPCHK(hipGetLastError());
-    return PIC_OK;
-}
-
-// lift: out (bf16, C channels) = conv SAME of Dh u with weights w ([9][C][16])
-static int conv_lift(int l, int batch, const float* u, int64_t u_bs, const bf16* w, bf16* out, const double* done) {
-    const MgLevel& lv = ln.lv[l];
-    ConvArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.in_f32 = u; a.Cin = 16; a.Hin = lv.H; a.Win = lv.W; a.in_bs = u_bs; a.in_scale = lv.dh;
-    a.w = w; a.out = out; a.Cout = ln.C[l]; a.Hout = lv.H; a.Wout = lv.W; a.out_bs = lv.n * ln.C[l];
-    a.mode = CM_SAME; a.done = done;
-    return launch_conv(a, batch);
 }
 
 static MgArgs mg_args(int l, int run, const float* r, int64_t r_bs, float* y, int64_t y_bs, const double* done) {
@@ -1404,14 +1295,17 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
 
 static int pack_weights() {
     for (int l = 0; l < ln.Ls; ++l) {
-        int n = 9 * ln.C[l] * 16;
-        hipLaunchKernelGGL(k_pack_smoother, dim3((n + 255) / 256), dim3(256), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l],
-                           ln.d_p + ln.off_q[l], ln.pin[l], ln.pSin[l]);
         hipLaunchKernelGGL(k_pack_k81, dim3(1), dim3(128), 0, g.stream, ln.C[l], ln.d_p + ln.off_in[l], ln.d_p + ln.off_q[l],
                            ln.d_k81 + LN_KW * l);
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
+}
+
+// blocks of k_tapgram on level l for a batch (level 0 is the largest)
+static int64_t tapgram_blocks(int l, int batch) {
+    const MgLevel& v = ln.lv[l];
+    return (int64_t)((v.W + TG_J - 1) / TG_J) * ((v.H + TG_I - 1) / TG_I) * batch;
 }
 
 static int ensure_batch(int batch) {
@@ -1420,9 +1314,9 @@ static int ensure_batch(int batch) {
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     size_t bytes = 0;
     for (int l = 0; l < ln.L; ++l) bytes += 8 * al((size_t)ln.lv[l].n * sizeof(float) * batch);
-    for (int l = 0; l < ln.Ls; ++l) bytes += 4 * al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch);
     const size_t N = (size_t)ln_npix0();
     bytes += 4 * al(N * sizeof(float) * batch);
+    bytes += al((size_t)tapgram_blocks(0, batch) * 81 * sizeof(double));
     HIPCHK(hipMalloc(&ln.d_act, bytes));
     HIPCHK(hipMemset(ln.d_act, 0, bytes));
     char* p = (char*)ln.d_act;
@@ -1431,12 +1325,9 @@ static int ensure_batch(int batch) {
         float** vs[8] = {&v.r[0], &v.r[1], &v.x[0], &v.x[1], &v.y[0], &v.y[1], &v.s2[0], &v.s2[1]};
         for (int k = 0; k < 8; ++k) { *vs[k] = (float*)p; p += al((size_t)v.n * sizeof(float) * batch); }
     }
-    for (int l = 0; l < ln.Ls; ++l) {
-        bf16** bs[4] = {&ln.U[l], &ln.V[l], &ln.SU[l], &ln.SV[l]};
-        for (int k = 0; k < 4; ++k) { *bs[k] = (bf16*)p; p += al((size_t)ln.lv[l].n * ln.C[l] * sizeof(bf16) * batch); }
-    }
     float** fs[4] = {&ln.y0, &ln.gl, &ln.rb, &ln.eb};
     for (int s = 0; s < 4; ++s) { *fs[s] = (float*)p; p += al(N * sizeof(float) * batch); }
+    ln.d_gpart = (double*)p;
     ln.bcap = batch;
     return PIC_OK;
 }
@@ -1518,7 +1409,7 @@ static void init_params(std::vector<float>& P, uint64_t seed) {
         const int C = ln.C[l];
         for (int t = 0; t < 9; ++t)
             for (int co = 0; co < C; ++co)
-                P[ln.off_in[l] + ((int64_t)t * C + co) * 16] = (co == 0) ? (t == 4 ? 1.0f : 0.0f) : small * rnd();
+                P[ln.off_in[l] + (int64_t)t * C + co] = (co == 0) ? (t == 4 ? 1.0f : 0.0f) : small * rnd();
         for (int i = 0; i < C; ++i)
             for (int j = 0; j < C; ++j)
                 P[ln.off_q[l] + i * C + j] = (i == 0 && j == 0) ? std::sqrt(LN_OMEGA) : small * rnd();
@@ -1545,7 +1436,7 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     while (ln.lc > std::max(ln.Ls, 1) && (int64_t)H[ln.lc - 1] * W[ln.lc - 1] <= MG_CB_MAXN) ln.lc--;
     for (int l = 0; l < ln.Ls; ++l) {
         int c = channels[l];
-        if (c != 16 && c != 32 && c != 64) return pic_fail(PIC_ERR_ARG, "channels must be 16, 32 or 64");
+        if (c < 1 || c > 64) return pic_fail(PIC_ERR_ARG, "channels must be 1..64");
         ln.C[l] = c;
     }
     // level-0 operator: A' = D^-1/2 A D^-1/2 on unknowns, nine node-space diagonals
@@ -1654,7 +1545,7 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     }
     // parameters
     int64_t o = 0;
-    for (int l = 0; l < ln.Ls; ++l) { ln.off_in[l] = o; o += 9LL * ln.C[l] * 16; }
+    for (int l = 0; l < ln.Ls; ++l) { ln.off_in[l] = o; o += 9LL * ln.C[l]; }
     for (int l = 0; l < ln.Ls; ++l) { ln.off_q[l] = o; o += (int64_t)ln.C[l] * ln.C[l]; }
     ln.nparams = o;
     std::vector<float> P(std::max<int64_t>(o, 1), 0.0f);
@@ -1662,24 +1553,11 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     const int64_t no = std::max<int64_t>(o, 1);
     HIPCHK(hipMalloc(&ln.d_p, no * sizeof(float))); HIPCHK(hipMalloc(&ln.d_m, no * sizeof(float)));
     HIPCHK(hipMalloc(&ln.d_v, no * sizeof(float))); HIPCHK(hipMalloc(&ln.d_g, no * sizeof(float)));
-    HIPCHK(hipMalloc(&ln.d_dS, no * sizeof(float)));
+    HIPCHK(hipMalloc(&ln.d_G, 81 * LN_MAX_LEARNED * sizeof(double)));
     HIPCHK(hipMemcpy(ln.d_p, P.data(), no * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(ln.d_m, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_v, 0, no * sizeof(float)));
-    HIPCHK(hipMemset(ln.d_g, 0, no * sizeof(float))); HIPCHK(hipMemset(ln.d_dS, 0, no * sizeof(float)));
-    size_t nb = 1;
-    for (int l = 0; l < ln.Ls; ++l) nb += 2 * 9 * (size_t)ln.C[l] * 16;
-    HIPCHK(hipMalloc(&ln.d_pk, nb * sizeof(bf16)));
+    HIPCHK(hipMemset(ln.d_g, 0, no * sizeof(float)));
     HIPCHK(hipMalloc(&ln.d_k81, LN_KW * LN_MAX_LEARNED * sizeof(float)));
-    bf16* q = ln.d_pk;
-    for (int l = 0; l < ln.Ls; ++l) {
-        size_t s = 9 * (size_t)ln.C[l] * 16;
-        ln.pin[l] = q; q += s; ln.pSin[l] = q; q += s;
-    }
-    // gradient partials: per learned level 4 SAME correlations + 2 CENTER, <= 256 blocks each
-    int64_t pc = 64;
-    for (int l = 0; l < ln.Ls; ++l) pc += 256LL * (4 * 9 * ln.C[l] * 16 + 2 * ln.C[l] * ln.C[l]);
-    ln.part_cap = pc;
-    HIPCHK(hipMalloc(&ln.d_part, pc * sizeof(float)));
     HIPCHK(hipMalloc(&ln.d_lpart, ((N + 255) / 256 + 64) * sizeof(double)));
     HIPCHK(hipMalloc(&ln.d_lsum, 64 * sizeof(double)));
     HIPCHK(hipMalloc(&ln.d_idx, 1024 * sizeof(int)));
@@ -1766,51 +1644,9 @@ int learn_harvest_finish(const float* xfinal) {
 }
 
 // --------------------------------------------------------------- training
-struct CorrPlan { int64_t off; int nblk; };
-
-static int launch_corr(CorrArgs a, int64_t& part_off, CorrPlan& plan) {
-    const int npix = a.Hout * a.Wout;
-    int nblk = std::min(256, std::max(1, (npix + 63) / 64));
-    int ppb = ((npix + nblk - 1) / nblk + 15) / 16 * 16;
-    nblk = (npix + ppb - 1) / ppb;
-    const int ntaps = a.mode == CM_CENTER ? 1 : 9;
-    const int64_t size = (int64_t)ntaps * a.Cout * a.Cin;
-    if (part_off + (int64_t)nblk * size > ln.part_cap) return pic_fail(PIC_ERR_STATE, "gradient partial buffer too small");
-    a.ppb = ppb;
-    a.part = ln.d_part + part_off;
-    plan.off = part_off; plan.nblk = nblk;
-    part_off += (int64_t)nblk * size;
-    const int mt = a.Cout / 16, nt = a.Cin / 16;
-    dim3 grid(nblk), blk(256);
-#define CORR_CASE(M_, N_) if (mt == M_ && nt == N_) { hipLaunchKernelGGL((k_corr<M_, N_>), grid, blk, 0, g.stream, a); HIPCHK(hipGetLastError()); return PIC_OK; }
-    CORR_CASE(1, 1) CORR_CASE(2, 1) CORR_CASE(4, 1) CORR_CASE(2, 2) CORR_CASE(4, 4)
-#undef CORR_CASE
-    return pic_fail(PIC_ERR_ARG, "learned network: unsupported channel pair");
-}
-
-// og (C channels at level l) against the FP32 input u (scaled by Dh), 9 taps
-static CorrArgs corr_in(int l, const bf16* og, const float* u, int64_t u_bs, int batch) {
-    CorrArgs a;
-    std::memset(&a, 0, sizeof(a));
-    const MgLevel& v = ln.lv[l];
-    a.og = og; a.Cout = ln.C[l]; a.Hout = v.H; a.Wout = v.W; a.og_bs = v.n * ln.C[l];
-    a.in_f32 = u; a.in_scale = v.dh; a.Cin = 16; a.Hin = v.H; a.Win = v.W; a.in_bs = u_bs;
-    a.mode = CM_SAME; a.batch = batch;
-    return a;
-}
-// og against in (both C channels at level l), centre tap
-static CorrArgs corr_center(int l, const bf16* og, const bf16* in, int batch) {
-    CorrArgs a;
-    std::memset(&a, 0, sizeof(a));
-    const MgLevel& v = ln.lv[l];
-    a.og = og; a.Cout = ln.C[l]; a.Hout = v.H; a.Wout = v.W; a.og_bs = v.n * ln.C[l];
-    a.in = in; a.Cin = ln.C[l]; a.Hin = v.H; a.Win = v.W; a.in_bs = v.n * ln.C[l];
-    a.mode = CM_CENTER; a.batch = batch;
-    return a;
-}
-
 // Loss and gradient for `batch` pairs already in ln.rb / ln.eb (indices in d_idx), gradient
-// summed over the batch into ln.d_g. Returns the summed loss in *loss.
+// summed over the batch into ln.d_g. Returns the summed loss in *loss (nullptr: no read-back,
+// no host synchronisation).
 static int loss_and_gradient(int batch, double* loss) {
     const int n = (int)ln_npix0();
     const int64_t N = n;
@@ -1822,42 +1658,28 @@ static int loss_and_gradient(int batch, double* loss) {
     HIPCHK(hipGetLastError());
     // the same V-cycle on g (its level-0 output is not needed; it lands in ln.y0's twin below)
     rc = net_apply(batch, ln.gl, ln.lv[0].y[1], N, 1, nullptr); if (rc) return rc;
-    int64_t po = 0;
     for (int l = 0; l < ln.Ls; ++l) {
         const MgLevel& v = ln.lv[l];
         const float* r_r = l == 0 ? ln.rb : v.r[0];
         const float* r_g = l == 0 ? ln.gl : v.r[1];
         const int64_t rbs = l == 0 ? N : v.n;
         // pairings (u, v): (s2(g), r(r)) and (r(g), s2(r))
-        const float* us[2] = {v.s2[1], r_g};
-        const int64_t ubs[2] = {v.n, rbs};
-        const float* vs[2] = {r_r, v.s2[0]};
-        const int64_t vbs[2] = {rbs, v.n};
-        CorrPlan pw[4], ps[2];
-        for (int k = 0; k < 2; ++k) {
-            rc = conv_lift(l, batch, us[k], ubs[k], ln.pin[l], ln.U[l], nullptr); if (rc) return rc;
-            rc = conv_lift(l, batch, vs[k], vbs[k], ln.pin[l], ln.V[l], nullptr); if (rc) return rc;
-            rc = conv_lift(l, batch, us[k], ubs[k], ln.pSin[l], ln.SU[l], nullptr); if (rc) return rc;
-            rc = conv_lift(l, batch, vs[k], vbs[k], ln.pSin[l], ln.SV[l], nullptr); if (rc) return rc;
-            rc = launch_corr(corr_center(l, ln.U[l], ln.V[l], batch), po, ps[k]); if (rc) return rc;
-            rc = launch_corr(corr_in(l, ln.SV[l], us[k], ubs[k], batch), po, pw[2 * k]); if (rc) return rc;
-            rc = launch_corr(corr_in(l, ln.SU[l], vs[k], vbs[k], batch), po, pw[2 * k + 1]); if (rc) return rc;
-        }
-        GradRegions rw, rs;
-        rw.n = 4; rs.n = 2;
-        for (int k = 0; k < 4; ++k) { rw.off[k] = pw[k].off; rw.nblk[k] = pw[k].nblk; }
-        for (int k = 0; k < 2; ++k) { rs.off[k] = ps[k].off; rs.nblk[k] = ps[k].nblk; }
-        const int64_t nw = 9LL * ln.C[l] * 16, ns = (int64_t)ln.C[l] * ln.C[l];
-        hipLaunchKernelGGL(k_reduce_grad, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, g.stream, nw, ln.d_part, rw,
-                           ln.d_g + ln.off_in[l]);
-        hipLaunchKernelGGL(k_reduce_grad, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, g.stream, ns, ln.d_part, rs,
-                           ln.d_dS + ln.off_q[l]);
-        hipLaunchKernelGGL(k_qgrad, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, g.stream, ln.C[l],
-                           ln.d_dS + ln.off_q[l], ln.d_p + ln.off_q[l], ln.d_g + ln.off_q[l]);
+        TapGramArgs ta;
+        ta.H = v.H; ta.W = v.W; ta.dh = v.dh; ta.part = ln.d_gpart;
+        ta.f[0] = v.s2[1]; ta.bs[0] = v.n; ta.f[1] = r_r; ta.bs[1] = rbs;
+        ta.f[2] = r_g; ta.bs[2] = rbs; ta.f[3] = v.s2[0]; ta.bs[3] = v.n;
+        dim3 grid((unsigned)((v.W + TG_J - 1) / TG_J), (unsigned)((v.H + TG_I - 1) / TG_I), batch);
+        hipLaunchKernelGGL(k_tapgram, grid, dim3(256), 0, g.stream, ta);
+        hipLaunchKernelGGL(k_tapgram_sum, dim3(81), dim3(256), 0, g.stream, (int)tapgram_blocks(l, batch), ln.d_gpart,
+                           ln.d_G + 81 * l);
+        hipLaunchKernelGGL(k_tapgrad, dim3(1), dim3(256), 0, g.stream, ln.C[l], ln.d_G + 81 * l, ln.d_p + ln.off_in[l],
+                           ln.d_p + ln.off_q[l], ln.d_g + ln.off_in[l], ln.d_g + ln.off_q[l]);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(loss, ln.d_lsum, sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
+    if (loss) {
+        HIPCHK(hipMemcpyAsync(loss, ln.d_lsum, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
     return PIC_OK;
 }
 
@@ -1908,12 +1730,16 @@ int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_fir
             idx[b] = (int)(st % (uint64_t)ln.ds_n);
         }
         rc = gather_batch(batch, nh, idx.data()); if (rc) return rc;
+        // the loss is read back on the first and last step only (no host wait in between)
+        const bool want = s == 0 || s == steps - 1;
         double loss = 0.0;
-        rc = loss_and_gradient(batch, &loss); if (rc) return rc;
-        loss /= batch;
-        if (s == 0 && loss_first) *loss_first = loss;
-        if (loss_last) *loss_last = loss;
-        ln.last_loss = loss;
+        rc = loss_and_gradient(batch, want ? &loss : nullptr); if (rc) return rc;
+        if (want) {
+            loss /= batch;
+            if (s == 0 && loss_first) *loss_first = loss;
+            if (loss_last) *loss_last = loss;
+            ln.last_loss = loss;
+        }
         ln.adam_t++;
         float c1 = 1.0f - std::pow(b1, (float)ln.adam_t), c2 = 1.0f - std::pow(b2, (float)ln.adam_t);
         hipLaunchKernelGGL(k_adam, dim3((unsigned)((ln.nparams + 255) / 256)), dim3(256), 0, g.stream, ln.nparams, ln.d_p,

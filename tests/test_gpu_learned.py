@@ -1,7 +1,8 @@
 """GPU checks of the learned multigrid preconditioner: Galerkin hierarchy and network output
 (FP32 V-cycle with the collapsed 5x5 smoother) against the FP64 numpy oracle (stated bound
-1e-4 relative), the training gradient (BF16 MFMA lifts and correlations) against the oracle
-with BF16 emulated (5e-2 relative per tensor), training on pairs harvested from the plain
+1e-4 relative), the training gradient (FP32 V-cycles, tap-pair Gram matrices accumulated in
+FP32 per thread and FP64 across blocks) against the oracle's factored FP64 gradient (2e-3
+relative per tensor), training on pairs harvested from the plain
 solver, the learned FCG against the plain CG at the same 1e-8 residual, and the fallback to
 the plain CG when the network is unusable."""
 import numpy as np
@@ -47,9 +48,6 @@ def net(gpu_lib, pkg):
 def perturbed_params(pl, lay, rng, scale):
     p = pl.learn_params().astype(np.float64)
     p += scale * rng.standard_normal(p.size)
-    for l in range(lay["learned"]):
-        C = lay["C"][l]
-        p[lay["off_in"][l]: lay["off_in"][l] + 9 * C * 16].reshape(9, C, 16)[:, :, 1:] = 0.0
     return p.astype(np.float32)
 
 
@@ -104,16 +102,16 @@ def test_gradient_matches_oracle(gpu_lib, net):
     pl.learn_set_pairs(np.stack(rs), np.stack(es32))
     loss_gpu, g_gpu = pl.learn_gradient([0, 1], lay["nparams"])
     pairs = [(r.astype(np.float64), e.astype(np.float64)) for r, e in zip(rs, es32)]
-    loss_or, g_or = lrn.loss_and_grad(p.astype(np.float64), lay, net["levels"], pairs, emulate_bf16=True)
+    loss_or, g_or = lrn.loss_and_grad(p.astype(np.float64), lay, net["levels"], pairs)
     say(f"  loss gpu {loss_gpu:.6e} oracle {loss_or:.6e}")
-    assert abs(loss_gpu - loss_or) <= 1e-2 * abs(loss_or)
-    segs = [(f"In{l}", lay["off_in"][l], 9 * lay["C"][l] * 16) for l in range(lay["learned"])]
+    assert abs(loss_gpu - loss_or) <= 1e-3 * abs(loss_or)
+    segs = [(f"In{l}", lay["off_in"][l], 9 * lay["C"][l]) for l in range(lay["learned"])]
     segs += [(f"Q{l}", lay["off_q"][l], lay["C"][l] ** 2) for l in range(lay["learned"])]
     for name, o, n in segs:
         a, b = g_gpu[o:o + n].astype(np.float64), g_or[o:o + n]
         err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         say(f"  grad {name}: |g| {np.linalg.norm(b):.3e} rel err {err:.2e}")
-        assert err < 5e-2, name
+        assert err < 2e-3, name
 
 
 def test_training_and_learned_solve(gpu_lib, net):

@@ -126,7 +126,7 @@ def test_gradient_matches_central_differences():
     coords = []
     for l in range(lay["learned"]):
         C = lay["C"][l]
-        coords += [lay["off_in"][l] + (4 * C + 0) * 16, lay["off_in"][l] + (2 * C + 3) * 16,
+        coords += [lay["off_in"][l] + 4 * C + 0, lay["off_in"][l] + 2 * C + 3,
                    lay["off_q"][l], lay["off_q"][l] + C + 2]
     for c in coords:
         d = np.zeros_like(p); d[c] = 1.0
@@ -134,9 +134,6 @@ def test_gradient_matches_central_differences():
         assert abs(fd - grad[c]) <= 1e-5 * max(1e-3, abs(fd)), (c, fd, grad[c])
     for _ in range(3):
         d = rng.standard_normal(p.size)
-        for l in range(lay["learned"]):
-            C = lay["C"][l]
-            d[lay["off_in"][l]: lay["off_in"][l] + 9 * C * 16].reshape(9, C, 16)[:, :, 1:] = 0.0
         fd = _fd(p, lay, lev, pairs, d)
         assert abs(fd - grad @ d) <= 1e-6 * max(1.0, abs(fd))
 
@@ -169,6 +166,50 @@ def test_planted_mistake_in_gradient_is_caught():
     fd = _fd(p, lay, lev, pairs, d)
     assert abs(fd - grad[c]) <= 1e-5 * max(1e-3, abs(fd))
     assert abs(fd - wrong[0]) > 1e-2 * abs(fd)
+
+
+def _tap_gram(u, v, H, W):
+    """G[t][t2] = sum_p u(p + d_t) v(p + d_t2) over the grid, zero outside."""
+    up = np.zeros((H + 2, W + 2)); up[1:-1, 1:-1] = u.reshape(H, W)
+    vp = np.zeros((H + 2, W + 2)); vp[1:-1, 1:-1] = v.reshape(H, W)
+    G = np.zeros((9, 9))
+    for t, (di, dj) in enumerate(lrn.TAPS):
+        for t2, (ei, ej) in enumerate(lrn.TAPS):
+            G[t, t2] = np.sum(up[1 + di:1 + di + H, 1 + dj:1 + dj + W] * vp[1 + ei:1 + ei + H, 1 + ej:1 + ej + W])
+    return G
+
+
+def test_gradient_contracts_to_tap_pair_gram_matrix():
+    """The library's gradient of a learned level, from the tap-pair Gram matrix G summed over
+    both pairings (Gs = G + G^T): dIn = Gs W S and dQ = (W^T Gs W) Q, equals the oracle's
+    factored gradient (lifts and correlations over pixels) on a rectangular grid; with G in
+    place of Gs (a dropped transpose) it does not."""
+    rng = np.random.default_rng(7)
+    n = 17
+    A, mask = q1_operator(n)
+    lev = lrn.build_hierarchy(A, n, n)
+    lay = lrn.layout(n, n, [16], learned_levels=1)
+    p = lrn.initial_params(lay, small=0.3)
+    e = rng.standard_normal(n * n) * mask
+    pairs = [(A @ e, e)]
+    _, grad = lrn.loss_and_grad(p, lay, lev, pairs)
+    net = lrn._Net(p, lay, lev)
+    sr, sg = {}, {}
+    y = net.vcycle(0, pairs[0][0], sr)
+    g = 2 * mask * (A @ (y - e)) / (e @ (A @ e))
+    net.vcycle(0, g, sg)
+    dh = lev[0]["dh"]
+    G = _tap_gram(dh * sg[0][1], dh * sr[0][0], n, n) + _tap_gram(dh * sg[0][0], dh * sr[0][1], n, n)
+    w_in, q = lrn.unpack(p, lay)
+    Wm, Q = w_in[0][:, :, 0], q[0]
+    S = Q @ Q.T
+    C = lay["C"][0]
+    g_in = grad[lay["off_in"][0]: lay["off_in"][0] + 9 * C].reshape(9, C)
+    g_q = grad[lay["off_q"][0]: lay["off_q"][0] + C * C].reshape(C, C)
+    Gs = G + G.T
+    assert np.allclose(Gs @ Wm @ S, g_in, rtol=1e-10, atol=1e-12 * np.abs(g_in).max())
+    assert np.allclose(Wm.T @ Gs @ Wm @ Q, g_q, rtol=1e-10, atol=1e-12 * np.abs(g_q).max())
+    assert np.abs(2 * G @ Wm @ S - g_in).max() > 1e-3 * np.abs(g_in).max()
 
 
 def test_collapsed_smoother_equals_factored_smoother():
