@@ -110,8 +110,9 @@ int  pic_device_count(int* n);
 /* Precision of the particle state (32 or 64) and of the field solve
  * (0 = FP64 Jacobi-PCG, the reference semantics; 1 = FP32 inner PCG inside FP64
  * iterative refinement; 2 = as 1 with the learned flexible CG as the inner solver once
- * pic_learn_setup has run, falling back to the plain inner CG on stagnation). Must precede
- * pic_setup_domain / pic_set_species. */
+ * pic_learn_setup has run, falling back to the plain inner CG on stagnation; 3 = FP64 PCG
+ * with a multicolour IC(0) preconditioner, a comparison solver with no counterpart in the
+ * reference, to the same relative residual). Must precede pic_setup_domain / pic_set_species. */
 int  pic_set_precision(int particle_bits, int solver_mode);
 
 /* ------------------------------------------------------------- setup (host FP64)
@@ -259,6 +260,9 @@ int  pic_IFE_Solve_2D(const double* Rho, double* Phi, int nnx, int nny, int delt
 /* Iterations and final relative residual ||b-Ax||_2/||b||_2 (recomputed in FP64) of the
  * last solve; outer refinement steps for solver_mode 1. */
 int  pic_last_solve_info(int32_t* iterations, double* relres, int32_t* outer);
+/* Device time (HIP events on the library's stream) of the solver in the last
+ * pic_IFE_Solve_2D, RHS assembly excluded. No reference counterpart. */
+int  pic_last_solve_time(double* ms);
 /* PCG_Tol and PCG_MaxIT (ife.inp lines 6-7, IFE_MAIN_PARAM). */
 int  pic_set_pcg_params(double tol, int32_t maxit);
 
@@ -414,6 +418,9 @@ int  pic_learn_harvest(int32_t enable, int32_t capacity, int32_t every);
 /* Adam on the energy-norm loss, minibatches drawn from the ring; loss of the first and
  * the last minibatch (mean per pair). */
 int  pic_learn_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+/* Share (0..1, default 0.25) of harvested pairs in every training minibatch; the rest are
+ * synthetic pairs e = uniform noise after 0..8 damped-Jacobi sweeps, r = A' e. */
+int  pic_learn_data_mix(double harvested_share);
 /* Mean loss over pairs [0, npairs) of the ring, no update. */
 int  pic_learn_eval(int32_t npairs, double* mean_loss);
 /* Pairs in the ring, FCG iterations of the last learned solve, fallbacks to the plain CG so

@@ -119,6 +119,7 @@ _PROTOS = {
     "pic_GetParChg_2D": (C.c_int, [C.c_int]),
     "pic_IFE_Solve_2D": (C.c_int, [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double]),
     "pic_last_solve_info": (C.c_int, [_ip, _dp, _ip]),
+    "pic_last_solve_time": (C.c_int, [_dp]),
     "pic_set_pcg_params": (C.c_int, [C.c_double, C.c_int32]),
     "pic_GetEField_SIDG_PPR": (C.c_int, []),
     "pic_MCC": (C.c_int, []),
@@ -150,6 +151,7 @@ _PROTOS = {
     "pic_learn_eval": (C.c_int, [C.c_int32, _dp]),
     "pic_learn_info": (C.c_int, [_ip, _ip, _i64p, _dp]),
     "pic_learn_finetune": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_uint64]),
+    "pic_learn_data_mix": (C.c_int, [C.c_double]),
     "pic_learn_finetune_info": (C.c_int, [_i64p, _dp, _dp]),
     "pic_learn_apply": (C.c_int, [_fp, _fp, C.c_int64]),
     "pic_learn_get_params": (C.c_int, [_fp, C.c_int64, _i64p]),
@@ -339,7 +341,8 @@ class PicLibrary:
         self.bits = bits
 
     def set_solver(self, solver: int) -> None:
-        """0 FP64 Jacobi-PCG, 1 mixed precision, 2 mixed precision with the learned FCG."""
+        """0 FP64 Jacobi-PCG, 1 mixed precision, 2 mixed precision with the learned FCG,
+        3 FP64 multicolour IC(0)-PCG (comparison)."""
         check(lib().pic_set_precision(self.bits, solver))
 
     def sizes(self):
@@ -477,6 +480,12 @@ class PicLibrary:
         check(lib().pic_last_solve_info(C.byref(it), C.byref(rr), C.byref(ou)))
         return dict(iterations=it.value, relres=rr.value, outer=ou.value)
 
+    def last_solve_ms(self) -> float:
+        """Device time of the solver in the last IFE_Solve_2D (RHS assembly excluded)."""
+        ms = C.c_double()
+        check(lib().pic_last_solve_time(C.byref(ms)))
+        return ms.value
+
     def GetEField_SIDG_PPR(self):
         check(lib().pic_GetEField_SIDG_PPR())
 
@@ -598,6 +607,10 @@ class PicLibrary:
         ds, it, fb, loss = C.c_int32(), C.c_int32(), C.c_int64(), C.c_double()
         check(lib().pic_learn_info(C.byref(ds), C.byref(it), C.byref(fb), C.byref(loss)))
         return {"dataset": ds.value, "last_fcg_iterations": it.value, "fallbacks": fb.value, "last_loss": loss.value}
+
+    def learn_data_mix(self, harvested_share: float) -> None:
+        """Share of harvested pairs in each training minibatch (the rest synthetic)."""
+        check(lib().pic_learn_data_mix(harvested_share))
 
     def learn_finetune(self, every: int, train_steps: int = 100, batch: int = 4, lr: float = 1e-3, seed: int = 1):
         """Online fine-tuning inside step(): every `every` steps (0 = off) the solve harvests

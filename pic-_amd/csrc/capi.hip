@@ -248,7 +248,7 @@ int pic_finalize(void) {
 
 int pic_set_precision(int particle_bits, int solver_mode) {
     if (particle_bits != 32 && particle_bits != 64) return pic_fail(PIC_ERR_ARG, "particle_bits must be 32 or 64");
-    if (solver_mode < 0 || solver_mode > 2) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0, 1 or 2");
+    if (solver_mode < 0 || solver_mode > 3) return pic_fail(PIC_ERR_ARG, "solver_mode must be 0, 1, 2 or 3");
     if (particle_bits != g.particle_bits) {
         // particle storage changes width: existing species are discarded
         if (g.inited) HIPCHK(hipStreamSynchronize(g.stream));
@@ -273,6 +273,8 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     g.step = 0;
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) g.moved[s] = false;
     g.ft_every = 0; g.ft_count = 0; g.ft_ms = 0.0; g.ft_last_loss = 0.0;
+    g.domain_epoch++;                                // a cached IC(0) factor belongs to the old domain
+    if (g.d_ic) { (void)hipFree(g.d_ic); g.d_ic = nullptr; }
     g.mcc_ready = false;
     g.have_domain = false;
     TRY(setup_build_domain(cfg));
@@ -781,10 +783,22 @@ int pic_GetParChg_2D(int delta) {
 }
 
 
+static int run_solver() {
+    if (g.solver_mode == 0) return run_pcg_fp64(true);
+    if (g.solver_mode == 3) return run_pcg_ic();
+    return run_pcg_mixed();
+}
+
 static int solve_now() {
     TRY(launch_rhs());
-    if (g.solver_mode == 0) return run_pcg_fp64(true);
-    return run_pcg_mixed();
+    HIPCHK(hipEventRecord(g.ev[14], g.stream));
+    const int rc = run_solver();
+    HIPCHK(hipEventRecord(g.ev[15], g.stream));
+    HIPCHK(hipEventSynchronize(g.ev[15]));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, g.ev[14], g.ev[15]));
+    g.last_solve_ms = ms;
+    return rc;
 }
 
 int pic_IFE_Solve_2D(const double* Rho, double* Phi, int nnx, int nny, int delta, double time) {
@@ -809,6 +823,12 @@ int pic_last_solve_info(int32_t* iterations, double* relres, int32_t* outer) {
     if (iterations) *iterations = g.last_iter;
     if (relres) *relres = g.last_relres;
     if (outer) *outer = g.last_outer;
+    return PIC_OK;
+}
+
+int pic_last_solve_time(double* ms) {
+    if (!ms) return pic_fail(PIC_ERR_ARG, "ms is NULL");
+    *ms = g.last_solve_ms;
     return PIC_OK;
 }
 
@@ -1006,7 +1026,7 @@ int pic_step(int nsteps) {
             int rc2 = learn_set_harvest(0, 0, 1);
             TRY(rc); TRY(rc2);
         } else {
-            TRY(g.solver_mode == 0 ? run_pcg_fp64(true) : run_pcg_mixed());
+            TRY(run_solver());
         }
         HIPCHK(hipEventRecord(g.ev[4], g.stream));
         TRY(launch_ppr());
@@ -1292,6 +1312,10 @@ int pic_learn_eval(int32_t npairs, double* mean_loss) {
     REQUIRE_DOMAIN();
     if (!mean_loss) return pic_fail(PIC_ERR_ARG, "mean_loss is NULL");
     return learn_eval(npairs, mean_loss);
+}
+int pic_learn_data_mix(double harvested_share) {
+    REQUIRE_DOMAIN();
+    return learn_set_mix(harvested_share);
 }
 int pic_learn_finetune(int32_t every, int32_t train_steps, int32_t batch, double lr, uint64_t seed) {
     REQUIRE_DOMAIN();

@@ -749,3 +749,232 @@ int pcg_sparse_aos_fp64(int n, const pic_sparse* dmat, const double* diag, const
     SparseAosOp A; A.m = dmat;
     return pcg_run<double>(n, A, diag, b, x, r, z, p, p2, q, tol, maxit, partials, pstride, iters, ierr, scal);
 }
+
+// ------------------------------------------------------------------ IC(0)-PCG (solver_mode 3)
+// A comparison solver: FP64 PCG with a multicolour incomplete Cholesky preconditioner on the
+// node-space stiffness, to the same relative residual as MY_PCG. Colours by node parity,
+// c = (i & 1) + 2 (j & 1) for node i * nny + j: nodes of one colour never share a nine-point
+// stencil, so in the colour-major order each triangular solve is four parallel sweeps. The
+// factor keeps the stencil's sparsity (IC(0), fill within the pattern updated): built once
+// per domain on the host in FP64.
+struct IcArgs {
+    const double* L[8]; const double* U[8]; int off[8];
+    const double* dinv;
+    int nnx, nny;
+};
+
+__device__ inline int ic_node(const IcArgs& a, int color, int t, int* ok) {
+    const int ci = color & 1, cj = color >> 1;
+    const int wc = (a.nny - cj + 1) / 2, hc = (a.nnx - ci + 1) / 2;
+    const int r = t / wc, c = t - r * wc;
+    *ok = r < hc;
+    return (ci + 2 * r) * a.nny + cj + 2 * c;
+}
+
+// forward (L y = r, colours ascending) or backward (L^T x = y, colours descending) sweep
+// over one colour; reads only entries of other colours, written by earlier sweeps
+__global__ __launch_bounds__(256) void k_ic_sweep(IcArgs a, int color, int backward, const double* in, double* out,
+                                                  const double* scal) {
+    if (scal[S_DONE] != 0.0) return;
+    int ok = 0;
+    const int i = ic_node(a, color, blockIdx.x * blockDim.x + threadIdx.x, &ok);
+    if (!ok) return;
+    const double d = a.dinv[i];
+    double s = in[i];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const double c = backward ? a.U[q][i] : a.L[q][i];
+        s -= c * out[i + a.off[q]];
+    }
+    out[i] = d != 0.0 ? s * d : 0.0;
+}
+
+// r = b - A x; partials of b.b (P_BB)
+template <class Op>
+__global__ __launch_bounds__(PCG_BLOCK) void k_ic_init(int n, Op A, const double* b, const double* x, double* r,
+                                                       double* scal, double* partials, int pstride) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < n) {
+        double ri = b[i] - A.row(i, [&](int j) { return x[j]; });
+        r[i] = ri;
+        v[0] = b[i] * b[i];
+    }
+    const int regs[1] = {P_BB};
+    block_partials<1>(v, partials, pstride, regs);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_DONE] = 0.0; scal[S_ITER] = 0.0; scal[S_IERR] = 0.0; }
+}
+
+// partials of r.z (P_RZ)
+__global__ __launch_bounds__(PCG_BLOCK) void k_ic_rz(int n, const double* r, const double* z, const double* scal,
+                                                     double* partials, int pstride) {
+    if (scal[S_DONE] != 0.0) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {i < n ? r[i] * z[i] : 0.0};
+    const int regs[1] = {P_RZ};
+    block_partials<1>(v, partials, pstride, regs);
+}
+
+// alpha = rho_k / p.q, x += alpha p, r -= alpha q; partials of r.r (P_RR)
+__global__ __launch_bounds__(PCG_BLOCK) void k_ic_update(int n, int k, int nb, const double* p, const double* q,
+                                                         double* x, double* r, double* scal, double* partials,
+                                                         int pstride) {
+    if (scal[S_DONE] != 0.0) return;
+    double pq = reduce_region(partials, pstride, P_PQ, nb);
+    if (pq == 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { scal[S_DONE] = 1.0; scal[S_IERR] = 3.0; scal[S_ITER] = (double)k; }
+        return;
+    }
+    const double alpha = scal[k & 1 ? S_RHO1 : S_RHO0] / pq;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < n) {
+        const double ri = r[i] - alpha * q[i];
+        x[i] = x[i] + alpha * p[i];
+        r[i] = ri;
+        v[0] = ri * ri;
+    }
+    const int regs[1] = {P_RR};
+    block_partials<1>(v, partials, pstride, regs);
+}
+
+// The factor, on the host: rows in colour order; for row i (unknown), each lower neighbour j
+// (an unknown of a lower colour coupled to i) in ascending colour,
+//   l_ij = (a_ij - sum_k l_ik l_jk) / l_jj    over k lower neighbours of both, below j
+//   l_ii = sqrt(a_ii - sum_j l_ij^2)
+// A pivot that is not positive (possible only off the M-matrix case) is replaced by a_ii.
+static int ic_build() {
+    const MeshParams& m = g.mesh;
+    const int n = m.nnodes, nny = m.nny;
+    std::vector<std::vector<double>> dia(9, std::vector<double>(n));
+    for (int q = 0; q < 9; ++q)
+        HIPCHK(hipMemcpy(dia[q].data(), g.d_dia64[q], n * sizeof(double), hipMemcpyDeviceToHost));
+    auto color = [&](int i) { return (i / nny) % 2 + 2 * ((i % nny) % 2); };
+    const int* off = g.dia_off;                        // off[4] = 0; off[8 - q] = -off[q]
+    std::vector<double> L(8 * (size_t)n, 0.0), U(8 * (size_t)n, 0.0), lii(n, 0.0);
+    auto lq = [](int q) { return q < 4 ? q : q - 1; };   // stencil direction -> factor slot
+    // direction between two nodes (-1 if not stencil neighbours)
+    auto dir = [&](int from, int to) {
+        const int di = to / nny - from / nny, dj = to % nny - from % nny;
+        if (di < -1 || di > 1 || dj < -1 || dj > 1) return -1;
+        return (di + 1) * 3 + (dj + 1);
+    };
+    int shifted = 0;
+    for (int c = 0; c < 4; ++c)
+        for (int i = 0; i < n; ++i) {
+            if (color(i) != c || !(dia[4][i] > 0.0)) continue;
+            double sq = 0.0;
+            for (int cj = 0; cj < c; ++cj)
+                for (int q = 0; q < 9; ++q) {
+                    if (q == 4 || dia[q][i] == 0.0) continue;
+                    const int j = i + off[q];
+                    if (j < 0 || j >= n || color(j) != cj || !(dia[4][j] > 0.0)) continue;
+                    double a = dia[q][i];
+                    // common lower neighbours k of i and j with colour below cj
+                    for (int q2 = 0; q2 < 9; ++q2) {
+                        if (q2 == 4) continue;
+                        const int k = i + off[q2];
+                        if (k < 0 || k >= n || color(k) >= cj) continue;
+                        const double lik = L[(size_t)lq(q2) * n + i];
+                        if (lik == 0.0) continue;
+                        const int qj = dir(j, k);
+                        if (qj < 0 || qj == 4) continue;
+                        a -= lik * L[(size_t)lq(qj) * n + j];
+                    }
+                    const double l = a / lii[j];
+                    L[(size_t)lq(q) * n + i] = l;
+                    sq += l * l;
+                }
+            double p = dia[4][i] - sq;
+            if (!(p > 1e-12 * dia[4][i])) { p = dia[4][i]; shifted++; }
+            lii[i] = std::sqrt(p);
+        }
+    // U[q][i] = l_{j i} for j = i + off[q] of a higher colour (stored at L[8 - q][j])
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < 9; ++q) {
+            if (q == 4) continue;
+            const int j = i + off[q];
+            if (j < 0 || j >= n) continue;
+            U[(size_t)lq(q) * n + i] = L[(size_t)lq(8 - q) * n + j];
+        }
+    std::vector<double> dinv(n, 0.0);
+    for (int i = 0; i < n; ++i) dinv[i] = lii[i] > 0.0 ? 1.0 / lii[i] : 0.0;
+    if (g.d_ic) { (void)hipFree(g.d_ic); g.d_ic = nullptr; }
+    const size_t pad = ((size_t)nny + 2 + 63) / 64 * 64, len = (size_t)n + 2 * pad;
+    const size_t bytes = (17 * (size_t)n + len) * sizeof(double);
+    HIPCHK(hipMalloc(&g.d_ic, bytes));
+    double* d = (double*)g.d_ic;
+    HIPCHK(hipMemcpy(d, L.data(), 8 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + 8 * (size_t)n, U.data(), 8 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d + 16 * (size_t)n, dinv.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(d + 17 * (size_t)n, 0, len * sizeof(double)));
+    for (int q = 0; q < 8; ++q) { g.ic_L[q] = d + (size_t)q * n; g.ic_U[q] = d + (size_t)(8 + q) * n; }
+    g.ic_dinv = d + 16 * (size_t)n;
+    g.ic_y = d + 17 * (size_t)n + pad;
+    g.ic_shifted = shifted;
+    g.ic_epoch = g.domain_epoch;
+    return PIC_OK;
+}
+
+static void ic_apply(const IcArgs& a, const double* r, double* z) {
+    const int n = g.mesh.nnodes;
+    for (int c = 0; c < 4; ++c) {
+        const int nc = ((a.nnx - (c & 1) + 1) / 2) * ((a.nny - (c >> 1) + 1) / 2);
+        hipLaunchKernelGGL(k_ic_sweep, dim3((nc + 255) / 256), dim3(256), 0, g.stream, a, c, 0, r, g.ic_y, g.d_scal);
+    }
+    for (int c = 3; c >= 0; --c) {
+        const int nc = ((a.nnx - (c & 1) + 1) / 2) * ((a.nny - (c >> 1) + 1) / 2);
+        hipLaunchKernelGGL(k_ic_sweep, dim3((nc + 255) / 256), dim3(256), 0, g.stream, a, c, 1, g.ic_y, z, g.d_scal);
+    }
+    (void)n;
+}
+
+int run_pcg_ic() {
+    if (g.ic_epoch != g.domain_epoch) { int rc = ic_build(); if (rc) return rc; }
+    const int n = g.mesh.nnodes, nb = pcg_blocks(n), nb256 = (n + 255) / 256;
+    if (nb > g.pcg_pstride) return pic_fail(PIC_ERR_STATE, "PCG partial buffer too small");
+    IcArgs a;
+    for (int q = 0; q < 8; ++q) { a.L[q] = g.ic_L[q]; a.U[q] = g.ic_U[q]; a.off[q] = g.dia_off[q < 4 ? q : q + 1]; }
+    a.dinv = g.ic_dinv; a.nnx = g.mesh.nnx; a.nny = g.mesh.nny;
+    hipLaunchKernelGGL(k_phi_to_xn, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_phi, g.d_xn);
+    DiaOp<double> A = dia_op<double>(1);
+    double* scal = g.d_scal;
+    double h2[2] = {g.pcg_tol * g.pcg_tol, (double)g.pcg_maxit};
+    HIPCHK(hipMemcpyAsync(scal + S_TOL2, h2, 2 * sizeof(double), hipMemcpyHostToDevice, g.stream));
+    hipLaunchKernelGGL((k_ic_init<DiaOp<double>>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, A, g.d_bn, g.d_xn, g.d_rn,
+                       scal, g.d_partials, g.pcg_pstride);
+    ic_apply(a, g.d_rn, g.d_zn);
+    hipLaunchKernelGGL(k_ic_rz, dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, g.d_rn, g.d_zn, scal, g.d_partials,
+                       g.pcg_pstride);
+    HIPCHK(hipGetLastError());
+    double* pa = g.d_pn; double* pb = g.d_p2n;
+    const int CHUNK = 16;
+    int k = 0;
+    for (;;) {
+        for (int c = 0; c < CHUNK; ++c, ++k) {
+            hipLaunchKernelGGL((k_pcg_spmv<double, DiaOp<double>>), dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, k, nb, A,
+                               g.d_zn, pa, pb, g.d_qn, scal, g.d_partials, g.pcg_pstride);
+            hipLaunchKernelGGL(k_ic_update, dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, k, nb, pb, g.d_qn, g.d_xn, g.d_rn,
+                               scal, g.d_partials, g.pcg_pstride);
+            ic_apply(a, g.d_rn, g.d_zn);
+            hipLaunchKernelGGL(k_ic_rz, dim3(nb), dim3(PCG_BLOCK), 0, g.stream, n, g.d_rn, g.d_zn, scal, g.d_partials,
+                               g.pcg_pstride);
+            double* t = pa; pa = pb; pb = t;
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        if (g.h_scal_pinned[S_DONE] != 0.0) break;
+        if (k > g.pcg_maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
+    }
+    const int iters = (int)g.h_scal_pinned[S_ITER], ierr = (int)g.h_scal_pinned[S_IERR];
+    double rr, bb;
+    { int rc = resid64(n, A, g.d_bn, g.d_xn, nullptr, &rr, &bb); if (rc) return rc; }
+    hipLaunchKernelGGL(k_xn_to_phi, dim3(nb256), dim3(256), 0, g.stream, n, g.d_node_type2, g.d_xn, g.d_phi);
+    HIPCHK(hipGetLastError());
+    g.last_iter = iters; g.last_outer = 1;
+    g.last_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    if (ierr == 1) return pic_fail(PIC_ERR_PCG_MAXITER, "IC(0)-PCG reached PCG_MaxIT");
+    if (ierr == 2 || ierr == 3) return pic_fail(PIC_ERR_PCG_BREAKDOWN, "IC(0)-PCG breakdown");
+    return PIC_OK;
+}

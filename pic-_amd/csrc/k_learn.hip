@@ -1199,6 +1199,7 @@ struct LearnNet {
     int last_iters = 0; long long fallbacks = 0;
     double last_loss = 0.0;
     bool trace = false;
+    double harvest_share = 0.25;                     // share of harvested pairs in a training minibatch
     double inner_tol = 1e-3;                         // FCG tolerance per refinement pass (the FP64 outer
                                                      // residual decides convergence)
 };
@@ -1716,13 +1717,15 @@ static int gather_batch(int batch, int nh, const int* idx) {
 
 int learn_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
     if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
-    if (ln.ds_n < 1) return pic_fail(PIC_ERR_STATE, "learned network: empty data set");
     if (ln.nparams == 0) return pic_fail(PIC_ERR_STATE, "learned network: no learned smoother levels");
     if (batch < 1 || batch > 64 || steps < 0) return pic_fail(PIC_ERR_ARG, "batch must be 1..64");
+    // harvested pairs in a minibatch (rounded, at least one for a positive share); the rest are
+    // synthetic pairs of every smoothness
+    const int nh = ln.harvest_share > 0.0 ? std::max(1, (int)std::lround(batch * ln.harvest_share)) : 0;
+    if (nh > 0 && ln.ds_n < 1) return pic_fail(PIC_ERR_STATE, "learned network: empty data set");
     int rc = ensure_batch(batch); if (rc) return rc;
     uint64_t st = seed * 0x9E3779B97F4A7C15ull + 7;
     std::vector<int> idx(batch);
-    const int nh = batch - batch / 2;                // half of every minibatch is synthetic rough pairs
     const float b1 = 0.9f, b2 = 0.999f;
     for (int s = 0; s < steps; ++s) {
         for (int b = 0; b < nh; ++b) {
@@ -1811,6 +1814,13 @@ bool learn_trace_on() { return ln.trace; }
 void learn_note_iterations(int it) { ln.last_iters = it; }
 
 bool learn_ready() { return ln.ready; }
+
+int learn_set_mix(double harvested_share) {
+    if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
+    if (!(harvested_share >= 0.0 && harvested_share <= 1.0)) return pic_fail(PIC_ERR_ARG, "share must be in [0, 1]");
+    ln.harvest_share = harvested_share;
+    return PIC_OK;
+}
 double learn_inner_tol() { return ln.inner_tol; }
 
 // --------------------------------------------------------------- test/diagnostic access
@@ -1834,7 +1844,12 @@ int learn_params(float* dst, const float* src, int64_t n, int64_t* nparams) {
     }
     if (src) {
         if (n != ln.nparams) return pic_fail(PIC_ERR_ARG, "parameter count mismatch");
+        // new parameters start a new optimisation: the Adam moments are cleared
+        HIPCHK(hipStreamSynchronize(g.stream));
         HIPCHK(hipMemcpy(ln.d_p, src, n * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(ln.d_m, 0, n * sizeof(float)));
+        HIPCHK(hipMemset(ln.d_v, 0, n * sizeof(float)));
+        ln.adam_t = 0;
         int rc = pack_weights(); if (rc) return rc;
         HIPCHK(hipStreamSynchronize(g.stream));
     }

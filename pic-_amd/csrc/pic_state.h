@@ -199,6 +199,15 @@ struct PicState {
     float *d_cg_r[2] = {nullptr, nullptr}, *d_cg_w[2] = {nullptr, nullptr}, *d_cg_s[2] = {nullptr, nullptr};
     float *d_cg_p = nullptr, *d_cg_x = nullptr;
     int cg_rpt = 1;                    // rows per thread of the CG kernels
+    // multicolour IC(0) factor of the node-space stiffness (solver_mode 3), built on the first
+    // such solve of a domain: strictly lower (L) and upper (U = L^T read at the row) factor
+    // entries per stencil direction, 1 / l_ii, and a padded FP64 work vector
+    void* d_ic = nullptr;
+    const double* ic_L[8] = {nullptr}; const double* ic_U[8] = {nullptr};
+    const double* ic_dinv = nullptr; double* ic_y = nullptr;
+    int64_t ic_epoch = -1, domain_epoch = 0;
+    int ic_shifted = 0;                // pivots replaced by the diagonal (IC(0) breakdown guard)
+    double last_solve_ms = 0.0;        // device time of the last IFE_Solve_2D solve
     int* d_wall_list = nullptr;        // particles the push hands to the wall kernel
     unsigned int* d_wall_count = nullptr;
     int64_t wall_cap = 0;
@@ -287,6 +296,7 @@ int launch_rhs();
 int launch_ppr();
 int run_pcg_fp64(bool reference_semantics);
 int run_pcg_mixed();
+int run_pcg_ic();
 int run_mcc();
 int run_sort_all();
 int launch_zero_acc(int isp);
@@ -312,6 +322,7 @@ int learn_fcg_pass(int n, const double* b64, double inner_tol, int K, double oto
 bool learn_trace_on();
 void learn_note_iterations(int it);
 bool learn_ready();
+int learn_set_mix(double harvested_share);
 double learn_inner_tol();
 int learn_apply_host(const float* r, float* z, int64_t n);
 int learn_params(float* dst, const float* src, int64_t n, int64_t* nparams);
