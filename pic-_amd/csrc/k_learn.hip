@@ -58,6 +58,10 @@ enum { F_S_NORMRHS = 4, F_S_NORMR = 5, F_S_DONE = 6, F_S_ITER = 7, F_S_IERR = 8,
 // skipped (the outer residual met its tolerance before the pass, or an earlier pass failed).
 // FAIL is set by a pass that ends in a NaN or a breakdown.
 #define FCG_BLOCK 256
+// tightest relative inner tolerance, squared: (3.2e-4)^2. An FP32 pass cannot cut the FP64
+// outer residual by much more (the FP32 operator's coefficient rounding times the condition
+// number), so iterations past it would be lost
+#define FCG_INNER_FLOOR2 1e-7
 
 __device__ inline void fcg_block_partials(int nv, double* v, double* part, int pstride) {
     __shared__ double sh[4][FCG_BLOCK / 64];
@@ -1056,7 +1060,12 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_init(int n, const double* b64
         const bool skip = failed || obb == 0.0 || orr <= otol2 * obb;
         if (first) scal[F_S_FAIL] = 0.0;
         scal[F_S_DONE] = skip ? 1.0 : 0.0; scal[F_S_ITER] = 0.0; scal[F_S_IERR] = 0.0;
-        scal[F_S_TOL2] = tol2; scal[F_S_MAXIT] = maxit;
+        // the pass aims at the reduction the outer residual still needs (half of it in norm, the
+        // inner residual being in the scaled system), at least a halving and no tighter than
+        // what the FP32 iteration reaches reliably; tol2 when there is no outer residual yet
+        double t2 = tol2;
+        if (orr > 0.0) t2 = fmin(0.25, fmax(FCG_INNER_FLOOR2, 0.25 * otol2 * obb / orr));
+        scal[F_S_TOL2] = t2; scal[F_S_MAXIT] = maxit;
         scal[F_S_NORMR] = 0.0; scal[F_S_NORMRHS] = 0.0;
         scal[FCG_REC_BASE + FCG_RECW * slot + 4] = skip ? 1.0 : 0.0;
     }

@@ -55,7 +55,6 @@ template <typename R> struct PushArgs {
     PicErrInfo* errinfo;
     int species;
     int* wall_list; unsigned int* wall_count;   // particles whose first pass reaches a wall
-    int all_slots;            // k_push_walls over every slot (axisymmetric mover) instead of the list
 };
 
 __device__ inline long long shfl_xor_ll(long long v, int d) {
@@ -424,16 +423,70 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& 
     return false;
 }
 
+// The axisymmetric first pass (MoveOne.f90:94-104 gather weights, :221-236 the radial field
+// turned by the azimuth, :246-263 the move in Cartesian coordinates and back to (r, th)) for
+// the common case, with the same arithmetic as move_one so that both paths give the same
+// bits; the conditions for handing over are those of push_hot (r < ymin is the reference's
+// wrong-crossing error, r > ymax an absorption: both go to the wall kernel).
+template <typename R>
+__device__ inline bool push_hot_rz(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, R& VZ, R& TH,
+                                   int* e_new, int* ci_new, int* cj_new) {
+    const MeshParams& m = a.m;
+    const R T = a.dt;
+    int ci, cj;
+    pic_elem_ij(e_old, m, &ci, &cj);
+    const unsigned int nbr = pic_ld(m.elem_nbr, e_old);
+    R x1 = pic_ld(a.xsR, ci), x2 = pic_ld(a.xsR, ci + 1), y1 = pic_ld(a.ysR, cj), y2 = pic_ld(a.ysR, cj + 1);
+    const int n1 = (int)__umul24((unsigned)ci, (unsigned)m.nny) + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+    const R e1x = pic_ld(a.efx, n1), e2x = pic_ld(a.efx, n2), e3x = pic_ld(a.efx, n3), e4x = pic_ld(a.efx, n4);
+    const R e1y = pic_ld(a.efy, n1), e2y = pic_ld(a.efy, n2), e3y = pic_ld(a.efy, n3), e4y = pic_ld(a.efy, n4);
+    R hxp = x2 - x1, hyp = y2 - y1;
+    R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
+    R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
+    (void)dy;
+    R xm = (R)1 - dx;
+    const R R1 = (R)m.ymin + y1, R2 = (R)m.ymin + y2, Rp = Y;
+    const R den = R2 * R2 - R1 * R1;
+    const R W1 = xm * (R2 * R2 - Rp * Rp) / den;
+    const R W2 = dx * (R2 * R2 - Rp * Rp) / den;
+    const R W3 = dx * (Rp * Rp - R1 * R1) / den;
+    const R W4 = xm * (Rp * Rp - R1 * R1) / den;
+    const R Ex = W1 * e1x + W2 * e2x + W3 * e3x + W4 * e4x;
+    const R Ey = W1 * e1y + W2 * e2y + W3 * e3y + W4 * e4y;
+    double sn, cs;
+    pic_sincos((double)TH, &sn, &cs);
+    const R xe = Ey * (R)cs - (R)0 * (R)sn;
+    const R ye = Ey * (R)sn + (R)0 * (R)cs;
+    const R VX1 = VX + a.qm * Ex * T;
+    const R VY1 = VY + a.qm * xe * T;
+    const R VZ1 = VZ + a.qm * ye * T;
+    double xc = (double)Y * cs, yc = (double)Y * sn;
+    xc = xc + (double)T * (double)VY1;
+    if (xc == 0.0) xc = m.hy * (double)1.0e-5f;
+    yc = yc + (double)T * (double)VZ1;
+    const R X1 = X + T * VX1;
+    const double rr = sqrt(xc * xc + yc * yc);
+    double tt = pic_atan(yc / xc);
+    if (xc <= 0.0) tt = tt + 3.14159265358979;
+    const R Y1 = (R)rr;
+    if (!(nbr & 16u) || X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
+    int bad = 0;
+    int en = pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
+    const int di = *ci_new - ci + 1, dj = *cj_new - cj + 1;
+    if (bad || (unsigned int)di > 2u || (unsigned int)dj > 2u || !((nbr >> (3 * di + dj)) & 1u)) return true;
+    X = X1; Y = Y1; VX = VX1; VY = VY1; VZ = VZ1; TH = (R)tt;
+    *e_new = en;
+    return false;
+}
+
 // The particles the push kernel handed over: the whole MoveOne/AdjustParticleBoundary loop,
 // lookup, write-back, and the deposit merged per wave (segmented by element, the list is
 // in slot order within each wave's append, so neighbours mostly share an element) before
-// the global atomics. Grid-stride over the list, every lane of a wave in every round. With
-// all_slots the kernel runs over every slot instead (the axisymmetric mover, whose every
-// particle takes this path; dead slots are skipped).
+// the global atomics. Grid-stride over the list, every lane of a wave in every round.
 template <typename R>
 __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
     const MeshParams& m = a.m;
-    const unsigned int nw = a.all_slots ? (unsigned int)a.n : *a.wall_count;
+    const unsigned int nw = *a.wall_count;
     const DepWindow none = {0, 0, 0, 0};
     const unsigned int lane = threadIdx.x & 63;
     const unsigned int wave0 = (blockIdx.x * blockDim.x + threadIdx.x) - lane;
@@ -445,9 +498,8 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
         long long v[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) v[q] = 0;
-        const int64_t p = i < nw ? (a.all_slots ? (int64_t)i : (int64_t)a.wall_list[i]) : 0;
+        const int64_t p = i < nw ? (int64_t)a.wall_list[i] : 0;
         const int e_old = i < nw ? a.cell[p] : -1;
-        if (i < nw && e_old < 0) a.cell_next[p] = -1;       // dead slot (all_slots only)
         if (e_old >= 0) {
             R X = a.x[p], Y = a.y[p], VX = a.vx[p], VY = a.vy[p], VZ = a.vz[p];
             R TH = a.th ? a.th[p] : (R)0;
@@ -500,7 +552,7 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 #else
 #define PUSH_ATTR
 #endif
-template <typename R, int DO_PUSH>
+template <typename R, int DO_PUSH, int RZ>
 __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<R> a) {
     __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
     __shared__ int s_center;
@@ -548,7 +600,8 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     const int* cellb = a.cell + base;
     int* nextb = a.cell_next + base;
     R* xb = a.x + base; R* yb = a.y + base; R* vxb = a.vx + base; R* vyb = a.vy + base;
-    const R* vzb = a.vz + base;
+    R* vzb = a.vz + base;
+    R* thb = RZ ? a.th + base : nullptr;
     int rkey = -1;
     long long acc[8];
 #pragma unroll
@@ -556,14 +609,16 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     // all PUSH_PPT slots' state is loaded up front (one wait); the loop is unrolled so the
     // per-slot registers are indexed statically
     int cel[PUSH_PPT];
-    R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT];
+    R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT], ths_[PUSH_PPT];
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
         const unsigned int o = o0 + 64 * k;
-        cel[k] = -1; xs_[k] = (R)0; ys_[k] = (R)0; vxs_[k] = (R)0; vys_[k] = (R)0; vzs_[k] = (R)0;
+        cel[k] = -1; xs_[k] = (R)0; ys_[k] = (R)0; vxs_[k] = (R)0; vys_[k] = (R)0; vzs_[k] = (R)0; ths_[k] = (R)0;
         if (o < nb) {
             cel[k] = pic_ld(cellb, o); xs_[k] = pic_ld((const R*)xb, o); ys_[k] = pic_ld((const R*)yb, o);
-            vxs_[k] = pic_ld((const R*)vxb, o); vys_[k] = pic_ld((const R*)vyb, o); vzs_[k] = pic_ld(vzb, o);
+            vxs_[k] = pic_ld((const R*)vxb, o); vys_[k] = pic_ld((const R*)vyb, o);
+            vzs_[k] = pic_ld((const R*)vzb, o);
+            if (RZ && DO_PUSH) ths_[k] = pic_ld((const R*)thb, o);
         }
     }
     __syncthreads();     // window zeroed before any deposit
@@ -572,13 +627,16 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         const unsigned int o = o0 + 64 * k;
         const bool in = o < nb;
         const int e_old = cel[k];
-        R X = xs_[k], Y = ys_[k], VX = vxs_[k], VY = vys_[k];
-        const R VZ = vzs_[k];
+        R X = xs_[k], Y = ys_[k], VX = vxs_[k], VY = vys_[k], VZ = vzs_[k], TH = ths_[k];
         int next = -1, nci = 0, ncj = 0;
         bool dep = false;
         if (DO_PUSH) {
             bool defer = false;
-            if (e_old >= 0) { defer = push_hot<R>(a, e_old, X, Y, VX, VY, &next, &nci, &ncj); dep = !defer; }
+            if (e_old >= 0) {
+                defer = RZ ? push_hot_rz<R>(a, e_old, X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
+                           : push_hot<R>(a, e_old, X, Y, VX, VY, &next, &nci, &ncj);
+                dep = !defer;
+            }
             const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
             if (wm) {
                 unsigned int wb = 0;
@@ -617,7 +675,10 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         // also writes its cell_next)
         if (in) {
             pic_st(nextb, o, next);
-            if (DO_PUSH && dep) { pic_st(xb, o, X); pic_st(yb, o, Y); pic_st(vxb, o, VX); pic_st(vyb, o, VY); }
+            if (DO_PUSH && dep) {
+                pic_st(xb, o, X); pic_st(yb, o, Y); pic_st(vxb, o, VX); pic_st(vyb, o, VY);
+                if (RZ) { pic_st(vzb, o, VZ); pic_st(thb, o, TH); }
+            }
         }
     }
     wave_deposit(rkey, m, acc, a.acc_w, a.acc_e, w, winW, winE);
@@ -640,7 +701,6 @@ static PushArgs<R> make_push_args(int isp) {
     PushArgs<R> a;
     a.x = (R*)S.x; a.y = (R*)S.y; a.vx = (R*)S.vx; a.vy = (R*)S.vy; a.vz = (R*)S.vz;
     a.th = g.mesh.delta == 1 ? (R*)S.th : nullptr;
-    a.all_slots = 0;
     a.cell = S.cell; a.cell_next = S.cell_next; a.n = S.nslots;
     a.m = g.mesh;
     a.m.xs = g.d_xs; a.m.ys = g.d_ys; a.m.xsf = g.d_xsf; a.m.ysf = g.d_ysf;
@@ -709,41 +769,25 @@ int launch_push_deposit(int isp, bool do_push) {
         HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
     }
     const int wall_blocks = 512;
-    if (do_push && g.mesh.delta == 1) {
-        // axisymmetric: every particle takes the full MoveOne / AdjustParticleBoundary loop
-        if (!S.th) return pic_fail(PIC_ERR_STATE, "axisymmetric species without azimuth storage");
-        const int ab = (int)std::min<int64_t>((S.nslots + 255) / 256, 8192);
-        if (g.particle_bits == 64) {
-            PushArgs<double> a = make_push_args<double>(isp);
-            a.all_slots = 1;
-            hipLaunchKernelGGL((k_push_walls<double>), dim3(ab), dim3(256), 0, g.stream, a);
-        } else {
-            PushArgs<float> a = make_push_args<float>(isp);
-            a.all_slots = 1;
-            hipLaunchKernelGGL((k_push_walls<float>), dim3(ab), dim3(256), 0, g.stream, a);
-        }
-    } else if (g.particle_bits == 64) {
-        PushArgs<double> a = make_push_args<double>(isp);
-        if (do_push) {
-            hipLaunchKernelGGL((k_push_deposit<double, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
-            hipLaunchKernelGGL((k_push_walls<double>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
-        } else {
-            hipLaunchKernelGGL((k_push_deposit<double, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-        }
-    } else {
-        PushArgs<float> a = make_push_args<float>(isp);
-        if (do_push) {
-            hipLaunchKernelGGL((k_push_deposit<float, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
-            hipLaunchKernelGGL((k_push_walls<float>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
-        } else {
-            hipLaunchKernelGGL((k_push_deposit<float, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);
-        }
+    const bool rz = g.mesh.delta == 1;
+    if (do_push && rz && !S.th) return pic_fail(PIC_ERR_STATE, "axisymmetric species without azimuth storage");
+    // the fused first pass, then the full MoveOne / AdjustParticleBoundary loop for the
+    // particles it handed over (axisymmetric: the r-z pass of MoveOne)
+#define PIC_PUSH_LAUNCH(R_)                                                                                    \
+    {                                                                                                          \
+        PushArgs<R_> a = make_push_args<R_>(isp);                                                              \
+        if (do_push) {                                                                                         \
+            if (rz) hipLaunchKernelGGL((k_push_deposit<R_, 1, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a); \
+            else hipLaunchKernelGGL((k_push_deposit<R_, 1, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);   \
+            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));                                               \
+            hipLaunchKernelGGL((k_push_walls<R_>), dim3(wall_blocks), dim3(256), 0, g.stream, a);              \
+        } else {                                                                                               \
+            hipLaunchKernelGGL((k_push_deposit<R_, 0, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);   \
+        }                                                                                                      \
     }
+    if (g.particle_bits == 64) PIC_PUSH_LAUNCH(double) else PIC_PUSH_LAUNCH(float)
+#undef PIC_PUSH_LAUNCH
     HIPCHK(hipGetLastError());
-    // axisymmetric: no fused push kernel, the interval covers the all-slot mover
-    if (do_push && g.mesh.delta == 1) HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
     if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][2], g.stream)); g.ev_push_pending[isp] = true; }
     S.acc_dirty = true;
     return PIC_OK;
