@@ -238,20 +238,25 @@ def timed_steps(pl, steps: int) -> dict:
             "ms_per_step": 1e3 * elapsed / steps, "phase": phase, "ks": ks, "solve": solve}
 
 
+def push_bytes(delta: int) -> float:
+    """Algorithmic HBM bytes per particle of the fused push (FP32 SoA state read and written once)."""
+    return 56.0 if delta == 1 else 44.0
+
+
 def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0) -> tuple:
     """Rooflines of the push kernel and of the plain CG iteration kernel (the two single
     kernels that can dominate a step). Algorithmic bytes per launch:
-      push: 44 B per particle (read x, y, vx, vy, vz, cell; write x, y, vx, vy, cell_next, FP32)
+      push: 44 B per particle (read x, y, vx, vy, vz, cell; write x, y, vx, vy, cell_next, FP32);
+            axisymmetric 56 B (the azimuth read and written, vz written too)
       CG iteration: 72 B per node (read r, w, s, p, x and 8 scaled off-diagonals; write r, w, s, p, x)"""
     ks, phase = run["ks"], run["phase"]
     out = []
     if ks["push_launches"]:
         ms = ks["push_ms"] / ks["push_launches"]
-        b = 44.0 * (0.5 * (run["n0"] + run["n1"])) / 2
+        b = push_bytes(delta) * (0.5 * (run["n0"] + run["n1"])) / 2
         out.append((ks["push_ms"], {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "traffic": traffic.get("k_push_deposit"),
-                                    # r-z: every slot moves through the general mover
-                                    "kernel": "k_push_walls<float>" if delta == 1 else "k_push_deposit<float,1>",
+                                    "kernel": "k_push_deposit<float,1,%d>" % delta,
                                     "bytes_per_launch": b, "ms_per_launch": ms,
                                     "launches": ks["push_launches"], "share_of_step": ks["push_ms"] / phase[7]}))
     if ks["cg_launches"]:
@@ -270,7 +275,8 @@ def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0)
 PHASES = ["push_lookup_deposit", "deposit_finalize", "rhs", "pcg", "ppr", "mcc", "sort", "total"]
 
 # Algorithmic HBM bytes of one step, by phase (the figures behind "full_step"):
-#   push + gather + lookup + deposit (fused)  44 B per particle (FP32 SoA state read and written once)
+#   push + gather + lookup + deposit (fused)  44 B per particle (FP32 SoA state read and written once;
+#                                             axisymmetric 56 B with the azimuth)
 #   deposit finalisation                      per node: 3 int64 sums read and cleared per species, rho_s,
 #                                             rho_p, Ek_tot, Ek_s written per species, rho and the volume
 #   RHS (B rho - c)                           per unknown: 9 B entries (12 B) + rho + c + two rhs writes
@@ -289,7 +295,8 @@ SORT_B_PART = 52.0
 CG_B_NODE, FCG_B_NODE, RES64_B_NODE = 72.0, 240.0, 96.0
 
 
-def full_step_roofline(run: dict, steps: int, nnodes: int, nunk: int, learned: bool, sort_every: int) -> dict:
+def full_step_roofline(run: dict, steps: int, nnodes: int, nunk: int, learned: bool, sort_every: int,
+                       delta: int = 0) -> dict:
     """Summed algorithmic bytes per step / measured time per step, against the 8 TB/s HBM peak."""
     npart = 0.5 * (run["n0"] + run["n1"])
     ks, solve = run["ks"], run["solve"]
@@ -299,7 +306,7 @@ def full_step_roofline(run: dict, steps: int, nnodes: int, nunk: int, learned: b
     else:
         iters = ks["cg_launches"] / steps
         solve_b = iters * CG_B_NODE * nnodes + (solve["outer"] + 1) * RES64_B_NODE * nnodes
-    by = {"push_lookup_deposit": 44.0 * npart, "deposit_finalize": FIN_B_NODE * nnodes, "rhs": RHS_B_NODE * nunk,
+    by = {"push_lookup_deposit": push_bytes(delta) * npart, "deposit_finalize": FIN_B_NODE * nnodes, "rhs": RHS_B_NODE * nunk,
           "pcg": solve_b, "ppr": PPR_B_NODE * nnodes, "mcc": 4.125 * npart,
           "sort": (SORT_B_PART * npart / sort_every) if sort_every else 0.0}
     total = sum(by.values())
@@ -592,7 +599,8 @@ def main() -> int:
         "phase_ms_per_step": {k: ph[i] / args.steps for i, k in enumerate(PHASES)},
         "roofline": roofs[0],
         "roofline_other": other[0] if other else None,
-        "full_step": full_step_roofline(main_run, args.steps, nnodes, pl.sizes()["nunknowns"], learned, 20),
+        "full_step": full_step_roofline(main_run, args.steps, nnodes, pl.sizes()["nunknowns"], learned, 20,
+                                        dom.delta),
     }
     if learned:
         cmp_ = solve_compare(pl, args.solve_reps)
