@@ -44,6 +44,7 @@ template <typename R> struct PushArgs {
     R qm, dt;
     R xmin, xmax, ymin, ymax, emit_eps;   // domain and the 10E-6 offset of the reference
     R hxR, hxiR, hyR, hyiR;               // mesh spacing and reciprocal in R (exact if pow2)
+    R x0R, y0R;                           // xmin, ymin in R (node lines, node_x / node_y)
     double vref_inv;          // 1/v_ref  (VelRes(1.0/PB%VFactor))
     double kappa_c;           // (Kappa-1.5)/beta, beta = m/(2 kB T)   (SI)
     double ek_c;              // 0.5 Mass v_ref^2 / JtoeV: kinetic energy (eV) = |v|^2 ek_c
@@ -158,8 +159,10 @@ __device__ inline void fold_y(R& Y, R& VY, R ymin, R ymax) {
 
 // Deposition weights of GetParChg_2D.f90:788-821 on element `e`, FP64. With a
 // power-of-two spacing the division is the (bit-identical) multiplication by 1/h.
+// The node lines are xmin + hx i, ymin + hy j (the set-up's formula, same roundings).
 __device__ inline void deposit_weights_ij(double X, double Y, int ci, int cj, const MeshParams& m, double W[4]) {
-    double x1 = pic_ld(m.xs, ci), x2 = pic_ld(m.xs, ci + 1), y1 = pic_ld(m.ys, cj), y2 = pic_ld(m.ys, cj + 1);
+    const double x1 = m.xmin + m.hx * (double)ci, x2 = m.xmin + m.hx * (double)(ci + 1);
+    const double y1 = m.ymin + m.hy * (double)cj, y2 = m.ymin + m.hy * (double)(cj + 1);
     double hxp = x2 - x1, hyp = y2 - y1;
     double dx = (m.pow2_map && hxp == m.hx) ? (X - x1) * m.hxi : (X - x1) / hxp;
     double dy = (m.pow2_map && hyp == m.hy) ? (Y - y1) * m.hyi : (Y - y1) / hyp;
@@ -183,6 +186,38 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
 }
 
 struct PushEvents { int emit, right, object, reflect, deleted; };
+
+// Node coordinate lines in R. The FP64 lines are xmin + hx i by construction (the set-up's
+// formula, recomputed with the same two roundings); the FP32 lines are their FP32 roundings,
+// recomputed in FP32 when the set-up found that evaluation exact for every node (nodes_f32),
+// else loaded.
+template <typename R>
+__device__ inline R node_x(const PushArgs<R>& a, int i) {
+    if (sizeof(R) == 8) return (R)(a.m.xmin + a.m.hx * (double)i);
+    return a.m.nodes_f32 ? a.x0R + a.hxR * (R)i : pic_ld(a.xsR, i);
+}
+template <typename R>
+__device__ inline R node_y(const PushArgs<R>& a, int j) {
+    if (sizeof(R) == 8) return (R)(a.m.ymin + a.m.hy * (double)j);
+    return a.m.nodes_f32 ? a.y0R + a.hyR * (R)j : pic_ld(a.ysR, j);
+}
+
+// The loads of a particle's first pass that depend only on its element: the neighbourhood
+// flags and the four corner fields. Issued for every slot of a thread before any slot is
+// computed, so a thread waits for its gathers once.
+template <typename R> struct HotIn { unsigned int nbr; R ex[4], ey[4]; };
+template <typename R>
+__device__ inline void hot_gather(const PushArgs<R>& a, int e, HotIn<R>& h) {
+    const MeshParams& m = a.m;
+    int ci, cj;
+    pic_elem_ij(e, m, &ci, &cj);
+    h.nbr = pic_ld(m.elem_nbr, e);
+    // a 24-bit product (mesh sides < 2^24): the generic 32-bit multiply-add is a 64-bit
+    // v_mad_u64_u32 whose addend pair can take the register of a load still in flight
+    const int n1 = (int)__umul24((unsigned)ci, (unsigned)m.nny) + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
+    h.ex[0] = pic_ld(a.efx, n1); h.ex[1] = pic_ld(a.efx, n2); h.ex[2] = pic_ld(a.efx, n3); h.ex[3] = pic_ld(a.efx, n4);
+    h.ey[0] = pic_ld(a.efy, n1); h.ey[1] = pic_ld(a.efy, n2); h.ey[2] = pic_ld(a.efy, n3); h.ey[3] = pic_ld(a.efy, n4);
+}
 
 // The whole-eV parts of a particle's energy quanta (pic_fix8 returned fx != 0): global
 // atomics on the element's corner nodes (rare: W E >= 2^11 eV), and the range flag.
@@ -382,22 +417,16 @@ __device__ inline bool move_one(const PushArgs<R>& a, int64_t p, int e_old, R& X
 // whole loop from the original state with the same arithmetic. Otherwise the reference's
 // loop ends after this pass with no boundary or object event, and *e_new is the element.
 template <typename R>
-__device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, int* e_new, int* ci_new,
-                                int* cj_new) {
+__device__ inline bool push_hot(const PushArgs<R>& a, int e_old, const HotIn<R>& h, R& X, R& Y, R& VX, R& VY,
+                                int* e_new, int* ci_new, int* cj_new) {
     const MeshParams& m = a.m;
     const R T = a.dt;
     int ci, cj;
     pic_elem_ij(e_old, m, &ci, &cj);
-    // every load of the pass is issued before the first test (all addresses are valid for
-    // any element), so one wait covers them
-    const unsigned int nbr = pic_ld(m.elem_nbr, e_old);     // flags of the 3x3 neighbourhood
-    R x1 = pic_ld(a.xsR, ci), x2 = pic_ld(a.xsR, ci + 1), y1 = pic_ld(a.ysR, cj), y2 = pic_ld(a.ysR, cj + 1);
-    // a 24-bit product (mesh sides < 2^24): the generic 32-bit multiply-add is a 64-bit
-    // v_mad_u64_u32 whose addend pair can take the register of the load above still in flight,
-    // which makes the field gathers wait for it
-    const int n1 = (int)__umul24((unsigned)ci, (unsigned)m.nny) + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-    const R e1x = pic_ld(a.efx, n1), e2x = pic_ld(a.efx, n2), e3x = pic_ld(a.efx, n3), e4x = pic_ld(a.efx, n4);
-    const R e1y = pic_ld(a.efy, n1), e2y = pic_ld(a.efy, n2), e3y = pic_ld(a.efy, n3), e4y = pic_ld(a.efy, n4);
+    const unsigned int nbr = h.nbr;                           // flags of the 3x3 neighbourhood
+    const R x1 = node_x(a, ci), x2 = node_x(a, ci + 1), y1 = node_y(a, cj), y2 = node_y(a, cj + 1);
+    const R e1x = h.ex[0], e2x = h.ex[1], e3x = h.ex[2], e4x = h.ex[3];
+    const R e1y = h.ey[0], e2y = h.ey[1], e3y = h.ey[2], e4y = h.ey[3];
     R hxp = x2 - x1, hyp = y2 - y1;
     R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
     R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
@@ -429,17 +458,16 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, R& X, R& Y, R& 
 // bits; the conditions for handing over are those of push_hot (r < ymin is the reference's
 // wrong-crossing error, r > ymax an absorption: both go to the wall kernel).
 template <typename R>
-__device__ inline bool push_hot_rz(const PushArgs<R>& a, int e_old, R& X, R& Y, R& VX, R& VY, R& VZ, R& TH,
-                                   int* e_new, int* ci_new, int* cj_new) {
+__device__ inline bool push_hot_rz(const PushArgs<R>& a, int e_old, const HotIn<R>& h, R& X, R& Y, R& VX, R& VY,
+                                   R& VZ, R& TH, int* e_new, int* ci_new, int* cj_new) {
     const MeshParams& m = a.m;
     const R T = a.dt;
     int ci, cj;
     pic_elem_ij(e_old, m, &ci, &cj);
-    const unsigned int nbr = pic_ld(m.elem_nbr, e_old);
-    R x1 = pic_ld(a.xsR, ci), x2 = pic_ld(a.xsR, ci + 1), y1 = pic_ld(a.ysR, cj), y2 = pic_ld(a.ysR, cj + 1);
-    const int n1 = (int)__umul24((unsigned)ci, (unsigned)m.nny) + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
-    const R e1x = pic_ld(a.efx, n1), e2x = pic_ld(a.efx, n2), e3x = pic_ld(a.efx, n3), e4x = pic_ld(a.efx, n4);
-    const R e1y = pic_ld(a.efy, n1), e2y = pic_ld(a.efy, n2), e3y = pic_ld(a.efy, n3), e4y = pic_ld(a.efy, n4);
+    const unsigned int nbr = h.nbr;
+    const R x1 = node_x(a, ci), x2 = node_x(a, ci + 1), y1 = node_y(a, cj), y2 = node_y(a, cj + 1);
+    const R e1x = h.ex[0], e2x = h.ex[1], e3x = h.ex[2], e4x = h.ex[3];
+    const R e1y = h.ey[0], e2y = h.ey[1], e3y = h.ey[2], e4y = h.ey[3];
     R hxp = x2 - x1, hyp = y2 - y1;
     R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
     R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
@@ -621,6 +649,13 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
             if (RZ && DO_PUSH) ths_[k] = pic_ld((const R*)thb, o);
         }
     }
+    // the element gathers of every slot, before any slot is computed (a dead slot reads
+    // element 0's, unused)
+    HotIn<R> hin[PUSH_PPT];
+    if (DO_PUSH) {
+#pragma unroll
+        for (int k = 0; k < PUSH_PPT; ++k) hot_gather<R>(a, cel[k] >= 0 ? cel[k] : 0, hin[k]);
+    }
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
@@ -633,8 +668,8 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         if (DO_PUSH) {
             bool defer = false;
             if (e_old >= 0) {
-                defer = RZ ? push_hot_rz<R>(a, e_old, X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
-                           : push_hot<R>(a, e_old, X, Y, VX, VY, &next, &nci, &ncj);
+                defer = RZ ? push_hot_rz<R>(a, e_old, hin[k], X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
+                           : push_hot<R>(a, e_old, hin[k], X, Y, VX, VY, &next, &nci, &ncj);
                 dep = !defer;
             }
             const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
@@ -719,6 +754,7 @@ static PushArgs<R> make_push_args(int isp) {
     a.xmin = (R)g.mesh.xmin; a.xmax = (R)g.mesh.xmax; a.ymin = (R)g.mesh.ymin; a.ymax = (R)g.mesh.ymax;
     a.emit_eps = (R)(double)10.0e-6f;
     a.hxR = (R)g.mesh.hx; a.hxiR = (R)g.mesh.hxi; a.hyR = (R)g.mesh.hy; a.hyiR = (R)g.mesh.hyi;
+    a.x0R = (R)g.mesh.xmin; a.y0R = (R)g.mesh.ymin;
     a.vref_inv = 1.0 / g.v_ref;
     double beta = S.p.mass_kg / (2.0 * PIC_KB * S.p.init_temperature_K);
     a.kappa_c = (1.0 * 2.0 - 1.5) / beta;

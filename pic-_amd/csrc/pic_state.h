@@ -59,6 +59,8 @@ struct MeshParams {
     double mapx, mapy;                  // hx / 2^r
     double imapx, imapy;                // 1 / mapx, 1 / mapy
     int pow2_map;                       // 1 if mapx, mapy are powers of two (exact division)
+    int nodes_f32;                      // 1 if (float)xs[i] == (float)xmin + (float)hx * (float)i for
+                                        // every node line (and likewise y): FP32 lines recomputed
     float ney_rcp;                      // 1 / ney rounded to FP32 (element -> column, pic_elem_ij)
     int ij_exact_div;                   // 1 when nelem >= 2^24: pic_elem_ij divides
     const double* xs; const double* ys; // node coordinate lines

@@ -251,6 +251,12 @@ int setup_build_domain(const pic_domain_config* c) {
     m.imapx = 1.0 / hx; m.imapy = 1.0 / hy;
     int ex_;
     m.pow2_map = (std::frexp(hx, &ex_) == 0.5 && std::frexp(hy, &ex_) == 0.5) ? 1 : 0;
+    m.nodes_f32 = 1;
+    {
+        const float x0 = (float)c->xmin, y0 = (float)c->ymin, hxf = (float)hx, hyf = (float)hy;
+        for (int i = 0; i < nnx; ++i) if ((float)g.h_xs[i] != x0 + hxf * (float)i) m.nodes_f32 = 0;
+        for (int j = 0; j < nny; ++j) if ((float)g.h_ys[j] != y0 + hyf * (float)j) m.nodes_f32 = 0;
+    }
     m.ney_rcp = 1.0f / (float)ney;
     m.ij_exact_div = ((int64_t)nelem >= ((int64_t)1 << 24)) ? 1 : 0;
     m.n_objects = c->n_objects;
