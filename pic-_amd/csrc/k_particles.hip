@@ -442,7 +442,8 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, const HotIn<R>&
     // so no branch separates the flag load from the field loads
     if (!(nbr & 16u) || X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
     int bad = 0;
-    int en = pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
+    int en = (sizeof(R) == 4 && m.locate_f32) ? pic_locate_inside_f32((float)X1, (float)Y1, m, &bad, ci_new, cj_new)
+                                               : pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
     // the new element's flag from the neighbourhood mask (no second load); a particle that
     // moved further than one element is handed over like a flagged one
     const int di = *ci_new - ci + 1, dj = *cj_new - cj + 1;
@@ -499,7 +500,8 @@ __device__ inline bool push_hot_rz(const PushArgs<R>& a, int e_old, const HotIn<
     const R Y1 = (R)rr;
     if (!(nbr & 16u) || X1 <= a.xmin || X1 > a.xmax || Y1 < a.ymin || Y1 > a.ymax) return true;
     int bad = 0;
-    int en = pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
+    int en = (sizeof(R) == 4 && m.locate_f32) ? pic_locate_inside_f32((float)X1, (float)Y1, m, &bad, ci_new, cj_new)
+                                               : pic_locate_ij((double)X1, (double)Y1, m, &bad, ci_new, cj_new);
     const int di = *ci_new - ci + 1, dj = *cj_new - cj + 1;
     if (bad || (unsigned int)di > 2u || (unsigned int)dj > 2u || !((nbr >> (3 * di + dj)) & 1u)) return true;
     X = X1; Y = Y1; VX = VX1; VY = VY1; VZ = VZ1; TH = (R)tt;
@@ -651,11 +653,6 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     }
     // the element gathers of every slot, before any slot is computed (a dead slot reads
     // element 0's, unused)
-    HotIn<R> hin[PUSH_PPT];
-    if (DO_PUSH) {
-#pragma unroll
-        for (int k = 0; k < PUSH_PPT; ++k) hot_gather<R>(a, cel[k] >= 0 ? cel[k] : 0, hin[k]);
-    }
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
@@ -668,8 +665,10 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         if (DO_PUSH) {
             bool defer = false;
             if (e_old >= 0) {
-                defer = RZ ? push_hot_rz<R>(a, e_old, hin[k], X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
-                           : push_hot<R>(a, e_old, hin[k], X, Y, VX, VY, &next, &nci, &ncj);
+                HotIn<R> hin;
+                hot_gather<R>(a, e_old, hin);
+                defer = RZ ? push_hot_rz<R>(a, e_old, hin, X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
+                           : push_hot<R>(a, e_old, hin, X, Y, VX, VY, &next, &nci, &ncj);
                 dep = !defer;
             }
             const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
@@ -716,7 +715,9 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
             }
         }
     }
-    wave_deposit(rkey, m, acc, a.acc_w, a.acc_e, w, winW, winE);
+    // each thread's last run straight to the window (LDS atomics; lanes of one element add
+    // to the same eight words, which the LDS serialises)
+    if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
 
     // ---------------- flush the window
     __syncthreads();

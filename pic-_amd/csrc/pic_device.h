@@ -45,6 +45,26 @@ PIC_HD int pic_locate_ij(double X, double Y, const MeshParams& m, int* bad, int*
     return (j - 1) + (i - 1) * m.ney;
 }
 
+// pic_locate_ij for an FP32 coordinate pair inside the domain (xmin <= X, ymin <= Y) on a mesh
+// where it reduces exactly (locate_f32: power-of-two spacing and map, origin a multiple of the
+// spacing): X - Vert_o and X - xmin are then exact in FP64 and the scaling by 1/h is exact, so
+// INT((X - xmin) / mapx) + 1 equals INT(rxp) and the grid-line test only chooses between equal
+// values. Same element, same clamps and range test, without the grid-line arithmetic.
+PIC_HD int pic_locate_inside_f32(float Xf, float Yf, const MeshParams& m, int* bad, int* ci, int* cj) {
+    const double X = (double)Xf, Y = (double)Yf;
+    int i = (int)((X - m.vox) * m.hxi);
+    int j = (int)((Y - m.voy) * m.hyi);
+    if (j == m.nny) j = j - 1;
+    if (i == m.nnx) i = i - 1;
+    if (i < 1 || i > m.nex || j < 1 || j > m.ney || !(X == X) || !(Y == Y)) {
+        *bad = 1;
+        i = i < 1 ? 1 : (i > m.nex ? m.nex : i);
+        j = j < 1 ? 1 : (j > m.ney ? m.ney : j);
+    }
+    *ci = i - 1; *cj = j - 1;
+    return (j - 1) + (i - 1) * m.ney;
+}
+
 PIC_HD int pic_locate_elem(double X, double Y, const MeshParams& m, int* bad) {
     int ci, cj;
     return pic_locate_ij(X, Y, m, bad, &ci, &cj);

@@ -59,6 +59,8 @@ struct MeshParams {
     double mapx, mapy;                  // hx / 2^r
     double imapx, imapy;                // 1 / mapx, 1 / mapy
     int pow2_map;                       // 1 if mapx, mapy are powers of two (exact division)
+    int locate_f32;                     // 1 if pic_locate_inside_f32 equals pic_locate_ij (pow2 map
+                                        // equal to the spacing, xmin / hx and ymin / hy integers)
     int nodes_f32;                      // 1 if (float)xs[i] == (float)xmin + (float)hx * (float)i for
                                         // every node line (and likewise y): FP32 lines recomputed
     float ney_rcp;                      // 1 / ney rounded to FP32 (element -> column, pic_elem_ij)

@@ -251,6 +251,11 @@ int setup_build_domain(const pic_domain_config* c) {
     m.imapx = 1.0 / hx; m.imapy = 1.0 / hy;
     int ex_;
     m.pow2_map = (std::frexp(hx, &ex_) == 0.5 && std::frexp(hy, &ex_) == 0.5) ? 1 : 0;
+    {
+        const double kx = c->xmin / hx, ky = c->ymin / hy;
+        m.locate_f32 = (m.pow2_map && m.mapx == hx && m.mapy == hy && std::floor(kx) == kx && std::floor(ky) == ky &&
+                        std::fabs(kx) < 1048576.0 && std::fabs(ky) < 1048576.0) ? 1 : 0;
+    }
     m.nodes_f32 = 1;
     {
         const float x0 = (float)c->xmin, y0 = (float)c->ymin, hxf = (float)hx, hyf = (float)hy;

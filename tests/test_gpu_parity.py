@@ -254,6 +254,59 @@ def test_moveall_bit_exact(gpu_lib, pkg, orc, bits):
         say(f"  species {isp}: {keep.sum()} live, emitted {cnt[0]}, absorbed {cnt[1] + cnt[2]}, reflected {cnt[3]}")
 
 
+def test_lookup_on_grid_lines_fp32(gpu_lib, pkg, orc):
+    """FP32 particles at rest exactly on element edges, one float step either side of them,
+    mid-cell and on the top and right walls (the top-edge clamps of the lookup): after the
+    fused pass (which locates in-domain FP32 coordinates without the grid-line arithmetic on
+    this mesh) every element equals the oracle's restatement of the reference's lookup, and
+    the particles are where the oracle's MoveAll leaves them."""
+    pl = gpu_lib
+    say("case: lookup on grid lines, FP32, small24")
+    dom = pkg.workload("small24")["domain"]
+    pl.setup(dom, bits=32, solver=0, tol=1e-12)
+    xs, ys, ei, ni = mesh_of(pl, dom)
+    f = np.float32
+    def cand(lines):
+        out = [lines[-1]]
+        for v in lines[1:-1]:
+            v32 = f(v)
+            out += [v32, np.nextafter(v32, f(-np.inf)), np.nextafter(v32, f(np.inf)), v32 + f(0.5)]
+        return np.unique(np.array(out, dtype=f))
+    cx, cy = cand(xs), cand(ys)
+    X, Y = np.meshgrid(cx, cy, indexing="ij")
+    x = X.ravel().astype(np.float64)
+    y = Y.ravel().astype(np.float64)
+    n = len(x)
+    weight = pkg.species_weight(dom, 16)
+    pl.set_species(0, pkg.ELECTRON["charge_C"], pkg.ELECTRON["mass_kg"], weight, 2 * pkg.EVTOK, n + 64)
+    zero = np.zeros(n)
+    pl.load_particles(0, x, y, zero, zero, zero)
+    pl.set_run_params(0, 77, 0)
+    pl.set_step(1)
+    nn = dom.nnx * dom.nny
+    pl.set_field("EFX", np.zeros(nn))
+    pl.set_field("EFY", np.zeros(nn))
+    pl.set_field("COEF_PPR", np.zeros(6 * nn))
+    p0 = pl.particles(0)
+    norm = dom.normalisation()
+    qm = (pkg.ELECTRON["charge_C"] / pkg.ELECTRON_CHARGE) / (pkg.ELECTRON["mass_kg"] / pkg.ELECTRON_MASS)
+    z32 = np.zeros(nn, dtype=f)
+    res = orc.move_all(32, xs, ys, ei, ni, dom.object_array(), z32, z32, np.zeros(5 * nn, dtype=f), qm, dom.dt,
+                       norm["v_ref"], pkg.ELECTRON["mass_kg"], 2 * pkg.EVTOK, 77, 1, orc.tag(0, orc.RNG_EMIT),
+                       p0["x"], p0["y"], p0["vx"], p0["vy"], p0["vz"], p0["loc"] - 1)
+    xo, yo, vxo, vyo, vzo, alive, cnt, err = res
+    assert err == 0
+    pl.MoveAll(0)
+    pl.GetParChg_2D()
+    p1 = pl.particles(0)
+    keep = alive.astype(bool)
+    assert len(p1["x"]) == keep.sum()
+    assert np.array_equal(p1["x"], xo[keep].astype(np.float64)) and np.array_equal(p1["y"], yo[keep].astype(np.float64))
+    loc, bad = orc.locate(xs, ys, xo[keep].astype(np.float64), yo[keep].astype(np.float64))
+    assert bad == 0 and np.array_equal(p1["loc"] - 1, loc)
+    say(f"  {n} particles, {keep.sum()} live")
+
+
 @pytest.mark.parametrize("bits", [64, 32])
 def test_moveall_axisymmetric_bit_exact(gpu_lib, pkg, orc, bits):
     """The r-z mover (MoveOne.f90:94-104, :221-236, :246-263; x axial, y radial, velocities
