@@ -10,6 +10,7 @@
 // Kernels of an iteration after convergence return at once, so the host checks the flag
 // only every few iterations.
 #include "pic_device.h"
+#include <chrono>
 
 // ---------------------------------------------------------------- RHS
 // rhs_u = (B rho)_u - c_u for every unknown u, stored at its node (node-space PCG) and
@@ -625,6 +626,7 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
     bool first = true;
     int done_passes = 0;
     for (;;) {
+        const auto th0 = std::chrono::steady_clock::now();
         HIPCHK(hipEventRecord(g.ev[12], g.stream));
         resid64_launch(n, A64, g.d_bn, g.d_xn, g.d_rn);
         int K[FCG_MAX_PASS];
@@ -638,6 +640,7 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(g.ev[13], g.stream));
+        const double host_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - th0).count();
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, g.d_scal, 64 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
         const double* h = g.h_scal_pinned;
@@ -665,6 +668,8 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, g.ev[12], g.ev[13]));
         g.kstat[4] += ms; g.kstat[5] += its;
+        if (learn_trace_on())
+            fprintf(stderr, "learned batch: %d passes, host enqueue %.1f us, device %.1f us\n", batch, host_us, 1e3 * ms);
         learn_note_iterations(its);
         if (*fallback) return PIC_OK;
         if (*bb == 0.0 || *rr <= tol2 * *bb) { g.lp_passes = std::max(1, done_passes); return PIC_OK; }
