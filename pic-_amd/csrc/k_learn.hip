@@ -456,6 +456,146 @@ __global__ __launch_bounds__(256) void k_mg_up(MgArgs a) {
     }
 }
 
+// Up on a learned level with the post-smoothing fused in: y = x2 + Dh K Dh s2, s2 = r - A x2,
+// x2 = x + P ec, with the arithmetic of k_mg_up followed by k_smooth_k81 (add), so the result is
+// the same bits. A block owns MGU_FI x MGU_FJ fine nodes; it stages ec on the covering coarse
+// nodes and x on the fine halo 3 (x2 there), forms s2 on the halo 2 (the smoother's footprint:
+// every operator row of those nodes is loaded up front) and writes y and s2 (the training
+// gradient's site) on its own nodes. Replaces two launches by one at 2.1x the operator rows.
+#define MGS_HI (MGU_FI + 6)                          // x2: fine rows fi0-3 .. fi0+FI+2
+#define MGS_HJ (MGU_FJ + 6)
+#define MGS_NX (MGS_HI * MGS_HJ)
+#define MGS_XIT ((MGS_NX + 255) / 256)
+#define MGS_SI (MGU_FI + 4)                          // s2: halo 2
+#define MGS_SJ (MGU_FJ + 4)
+#define MGS_NS (MGS_SI * MGS_SJ)
+#define MGS_SIT ((MGS_NS + 255) / 256)
+#define MGS_CJ (MGU_FJ / 2 + 6)                       // coarse columns staged (fj0/2-2 .. fj0/2+FJ/2+3)
+#define MGS_NE ((MGU_FI / 2 + 4) * MGS_CJ)
+__global__ __launch_bounds__(256) void k_mg_up_smooth(MgArgs a) {
+    static_assert(MGU_FI * MGU_FJ == 256 && MGS_NE <= 256, "k_mg_up_smooth tile");
+    __shared__ float es[MGS_NE];
+    __shared__ float x2s[MGS_NX];
+    __shared__ float us[MGS_NS];
+    __shared__ float s2s[MGS_NS];
+    __shared__ float kw[LN_KW];
+    const int b = blockIdx.z;
+    const int fi0 = blockIdx.y * MGU_FI, fj0 = blockIdx.x * MGU_FJ;
+    const int H = a.Hf, W = a.Wf;
+    const int nf = H * W;
+    const float* r = a.r + (int64_t)b * a.r_bs;
+    const float* x = a.x + (int64_t)b * a.fs;
+    const float* ec = a.ec + (int64_t)b * a.cs;
+    float* y = a.y + (int64_t)b * a.y_bs;
+    constexpr int ECI = MGU_FI / 2 + 4;                 // coarse rows staged (fi0/2-2 .. fi0/2+FI/2+1)
+    const int ci0 = (fi0 - 3) >> 1, cj0 = (fj0 - 3) >> 1;    // floor
+    // ---- every global load first
+    const float kv = a.k81[threadIdx.x < LN_KW ? threadIdx.x : 0];
+    float ev;
+    {
+        const int k = threadIdx.x;
+        const int u = k / MGS_CJ, v = k - u * MGS_CJ;
+        const int I = ci0 + u, J = cj0 + v;
+        const bool in = k < MGS_NE && u < ECI && I >= 0 && I < a.Hc && J >= 0 && J < a.Wc;
+        ev = ec[in ? I * a.Wc + J : 0];
+        if (!in) ev = 0.0f;
+    }
+    float hx[MGS_XIT], hd[MGS_XIT];
+#pragma unroll
+    for (int it = 0; it < MGS_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        const int u = k / MGS_HJ, v = k - u * MGS_HJ;
+        const int gi = fi0 - 3 + u, gj = fj0 - 3 + v;
+        const bool in = k < MGS_NX && gi >= 0 && gi < H && gj >= 0 && gj < W;
+        const int g = in ? gi * W + gj : 0;
+        hd[it] = a.dia[4 * (int64_t)nf + g];
+        hx[it] = x[g];
+        if (!in) hd[it] = 0.0f;
+    }
+    float sr[MGS_SIT], sdh[MGS_SIT], sdv[MGS_SIT][9];
+#pragma unroll
+    for (int it = 0; it < MGS_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        const int u = k / MGS_SJ, v = k - u * MGS_SJ;
+        const int gi = fi0 - 2 + u, gj = fj0 - 2 + v;
+        const bool in = k < MGS_NS && gi >= 0 && gi < H && gj >= 0 && gj < W;
+        const int g = in ? gi * W + gj : 0;
+        sr[it] = r[g];
+        sdh[it] = a.dh[g];
+#pragma unroll
+        for (int t = 0; t < 9; ++t) sdv[it][t] = a.dia[(int64_t)t * nf + g];
+        if (!in) { sdh[it] = 0.0f; sdv[it][4] = 0.0f; }
+    }
+    if (a.done && *a.done != 0.0) return;
+    if (threadIdx.x < MGS_NE) es[threadIdx.x] = ev;
+    if (threadIdx.x < LN_KW) kw[threadIdx.x] = kv;
+    __syncthreads();
+    // ---- x2 = x + P ec on the halo 3 (0 on Dirichlet nodes and off the grid)
+#pragma unroll
+    for (int it = 0; it < MGS_XIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGS_NX) {
+            const int u = k / MGS_HJ, v = k - u * MGS_HJ;
+            const int gi = fi0 - 3 + u, gj = fj0 - 3 + v;
+            float val = 0.0f;
+            if (hd[it] != 0.0f) {
+                const int i0 = gi >> 1, j0 = gj >> 1;
+                const bool oi = gi & 1, oj = gj & 1;
+                float pe = 0.0f;
+                for (int p = 0; p < (oi ? 2 : 1); ++p)
+                    for (int q = 0; q < (oj ? 2 : 1); ++q)
+                        pe = pe + (oi ? 0.5f : 1.0f) * (oj ? 0.5f : 1.0f) * es[(i0 + p - ci0) * MGS_CJ + (j0 + q - cj0)];
+                val = hx[it] + pe;
+            }
+            x2s[k] = val;
+        }
+    }
+    __syncthreads();
+    // ---- s2 = r - A x2 on the halo 2 (0 on Dirichlet nodes and off the grid), and Dh s2
+#pragma unroll
+    for (int it = 0; it < MGS_SIT; ++it) {
+        const int k = threadIdx.x + it * 256;
+        if (k < MGS_NS) {
+            const int u = k / MGS_SJ, v = k - u * MGS_SJ;
+            float sres = 0.0f;
+            if (sdv[it][4] != 0.0f) {
+                sres = sr[it];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) sres = sres - sdv[it][t] * x2s[(u + t / 3) * MGS_HJ + (v + t % 3)];
+            }
+            s2s[k] = sres;
+            us[k] = sdh[it] * sres;
+        }
+    }
+    __syncthreads();
+    const int ti = threadIdx.x / MGU_FJ, tj = threadIdx.x - (threadIdx.x / MGU_FJ) * MGU_FJ;
+    const int i = fi0 + ti, j = fj0 + tj;
+    if (i >= H || j >= W) return;
+    const int f = i * W + j;
+    const float dp = a.dh[f];
+    // y before the smoothing: x2 (0 on a Dirichlet node, where x2 is 0)
+    const float o0 = x2s[(ti + 3) * MGS_HJ + (tj + 3)];
+    float s = 0.0f;
+    if (dp != 0.0f) {
+        if (i >= 1 && i <= H - 2 && j >= 1 && j <= W - 2) {
+#pragma unroll
+            for (int d = 0; d < 25; ++d) s = s + kw[LN_K25 + d] * us[(ti + d / 5) * MGS_SJ + (tj + d % 5)];
+        } else {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const int qi = i - (t / 3 - 1), qj = j - (t % 3 - 1);
+                if (qi < 0 || qi >= H || qj < 0 || qj >= W) continue;
+#pragma unroll
+                for (int t2 = 0; t2 < 9; ++t2)
+                    s = s + kw[9 * t + t2] * us[(ti + 2 - (t / 3 - 1) + (t2 / 3 - 1)) * MGS_SJ + (tj + 2 - (t % 3 - 1) + (t2 % 3 - 1))];
+            }
+        }
+        s = dp * s;
+    }
+    y[f] = o0 + s;
+    a.s2[(int64_t)b * a.fs + f] = s2s[(ti + 2) * MGS_SJ + (tj + 2)];
+}
+
 // Coarsest level: y = inv r (dense, n <= LN_MAX_COARSE), one block per batch entry.
 __global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, const float* r, float* y, int64_t bs,
                                                    const double* done) {
@@ -1296,8 +1436,13 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
         MgArgs a = mg_args(l, run, r, r_bs, y, y_bs, done);
         const MgLevel& f = ln.lv[l];
         dim3 grid((unsigned)((f.W + MGU_FJ - 1) / MGU_FJ), (unsigned)((f.H + MGU_FI - 1) / MGU_FI), batch);
-        hipLaunchKernelGGL(k_mg_up, grid, dim3(256), 0, g.stream, a);
-        if (l < ln.Ls) smooth5(l, batch, ln.lv[l].s2[run], ln.lv[l].n, y, y_bs, true, done, none);
+        if (l < ln.Ls) {
+            // learned level: interpolation, residual and post-smoothing in one launch
+            a.dh = ln.lv[l].dh; a.k81 = ln.d_k81 + LN_KW * l;
+            hipLaunchKernelGGL(k_mg_up_smooth, grid, dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL(k_mg_up, grid, dim3(256), 0, g.stream, a);
+        }
     }
     HIPCHK(hipGetLastError());
     return PIC_OK;
