@@ -216,7 +216,10 @@ int pic_init(int device) {
     // A blocking stream: the synchronous hipMemcpy / hipMemset calls of the API (null stream)
     // are then ordered with the library's kernels on both sides.
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamDefault));
+    HIPCHK(hipStreamCreateWithFlags(&g.stream2, hipStreamDefault));
     HIPCHK(hipHostMalloc((void**)&g.h_scal_pinned, 64 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&g.h_side_pinned, 16 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_side, hipEventDisableTiming));
     HIPCHK(hipMalloc((void**)&g.d_scal, 64 * sizeof(double)));
     HIPCHK(hipMemset(g.d_scal, 0, 64 * sizeof(double)));
     HIPCHK(hipMalloc((void**)&g.d_sync, 64 * sizeof(unsigned int)));
@@ -998,6 +1001,22 @@ static bool injection_on() {
     return false;
 }
 
+// pic_step's wall injection and MCC on the side stream, after the deposit of the step (the
+// event recorded after the finalisation) and in the reference's order (InjectBeams_2D, then
+// MCC); neither reads the potential or the fields, so the results are those of the serial
+// order. Run from overlap_point with solver work queued, so both overlap the solve.
+static int side_inject_mcc() {
+    int rc = PIC_OK;
+    if (hipStreamWaitEvent(g.stream2, g.ev[2], 0) != hipSuccess) rc = pic_fail(PIC_ERR_HIP, "side stream wait");
+    hipStream_t main = g.stream;
+    g.stream = g.stream2;
+    if (!rc && injection_on()) rc = run_injection((int)g.step, nullptr);    // Main_IFE_Test_2.f90:344
+    if (!rc && g.mcc_enabled && g.mcc_ready) rc = run_mcc();
+    g.stream = main;
+    if (hipEventRecord(g.ev_side, g.stream2) != hipSuccess && !rc) rc = pic_fail(PIC_ERR_HIP, "side stream event");
+    return rc;
+}
+
 int pic_step(int nsteps) {
     REQUIRE_DOMAIN();
     double acc[8] = {0};
@@ -1017,23 +1036,30 @@ int pic_step(int nsteps) {
         HIPCHK(hipEventRecord(g.ev[3], g.stream));
         const bool ft = g.ft_every > 0 && g.solver_mode == 2 && learn_ready() && g.step > 0 &&
                         g.step % g.ft_every == 0;
+        // injection and MCC go to the side stream at the solver's first host wait
+        g.overlap_rc = PIC_OK;
+        g.overlap_hook = side_inject_mcc;
+        int rc_solve = PIC_OK;
         if (ft) {
             // online fine-tuning: this solve runs the plain inner CG and harvests pairs
             g.solver_mode = 1;
-            int rc = learn_set_harvest(1, 0, 1);
-            if (!rc) rc = run_pcg_mixed();
+            rc_solve = learn_set_harvest(1, 0, 1);
+            if (!rc_solve) rc_solve = run_pcg_mixed();
             g.solver_mode = 2;
             int rc2 = learn_set_harvest(0, 0, 1);
-            TRY(rc); TRY(rc2);
+            if (!rc_solve) rc_solve = rc2;
         } else {
-            TRY(run_solver());
+            rc_solve = run_solver();
         }
+        overlap_point();                         // a solve that never waited: the side work now
+        if (rc_solve) { (void)hipStreamWaitEvent(g.stream, g.ev_side, 0); return rc_solve; }
         HIPCHK(hipEventRecord(g.ev[4], g.stream));
         TRY(launch_ppr());
-        if (injection_on()) TRY(run_injection((int)g.step, nullptr));     // Main_IFE_Test_2.f90:344
         HIPCHK(hipEventRecord(g.ev[5], g.stream));
-        if (g.mcc_enabled && g.mcc_ready) TRY(run_mcc());
+        // the next step's push and the sort need the injected and collided particles
+        HIPCHK(hipStreamWaitEvent(g.stream, g.ev_side, 0));
         HIPCHK(hipEventRecord(g.ev[6], g.stream));
+        TRY(g.overlap_rc);
         if (ft) {
             // the training of the fine-tune, timed apart (host clock around the synchronous calls)
             HIPCHK(hipStreamSynchronize(g.stream));

@@ -493,6 +493,7 @@ static int cg1_run_t(int n, const Dia8& A, const double* b64, double tol, int ma
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(g.ev[11], g.stream));
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        overlap_point();
         HIPCHK(hipStreamSynchronize(g.stream));
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, g.ev[10], g.ev[11]));
@@ -520,6 +521,13 @@ static int cg1_run(int n, const double* b64, double tol, int maxit, int* iters, 
 
 static inline int pcg_blocks(int n) { return (n + PCG_BLOCK - 1) / PCG_BLOCK; }
 
+void overlap_point() {
+    if (!g.overlap_hook) return;
+    int (*h)() = g.overlap_hook;
+    g.overlap_hook = nullptr;
+    g.overlap_rc = h();
+}
+
 // One Jacobi-PCG solve on device vectors (storage T, operator Op). The host launches the
 // iterations in chunks and reads the done flag between chunks.
 template <typename T, class Op>
@@ -546,6 +554,7 @@ static int pcg_run(int n, const Op& A, const T* diag, const T* b, T* x, T* r, T*
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        overlap_point();
         HIPCHK(hipStreamSynchronize(g.stream));
         if (g.h_scal_pinned[S_DONE] != 0.0) break;
         if (k > maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }
@@ -642,6 +651,7 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         HIPCHK(hipEventRecord(g.ev[13], g.stream));
         const double host_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - th0).count();
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, g.d_scal, 64 * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        overlap_point();
         HIPCHK(hipStreamSynchronize(g.stream));
         const double* h = g.h_scal_pinned;
         *rr = h[S_NSCAL]; *bb = h[S_NSCAL + 1];
@@ -968,6 +978,7 @@ int run_pcg_ic() {
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(g.h_scal_pinned, scal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        overlap_point();
         HIPCHK(hipStreamSynchronize(g.stream));
         if (g.h_scal_pinned[S_DONE] != 0.0) break;
         if (k > g.pcg_maxit + CHUNK) { g.h_scal_pinned[S_IERR] = 1.0; g.h_scal_pinned[S_ITER] = k; break; }

@@ -445,7 +445,7 @@ static int run_mcc_t() {
     if (nb_e) launch_select(e, n_e, nb_e, mask_e, bc_e, info + 0);
     if (nb_i) launch_select(ion, n_i, nb_i, mask_i, bc_i, info + 1);
     HIPCHK(hipGetLastError());
-    int* hinfo = (int*)g.h_scal_pinned;
+    int* hinfo = g.h_side_pinned;                   // MCC may run on the side stream during a solve
     HIPCHK(hipMemcpyAsync(hinfo, info, 2 * sizeof(int), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     int nsel_e = hinfo[0], nsel_i = hinfo[1];

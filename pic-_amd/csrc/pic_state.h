@@ -134,6 +134,13 @@ struct PicState {
     bool inited = false;
     int device = 0;
     hipStream_t stream = nullptr;
+    // side stream of pic_step: wall injection and MCC run there while the field solve runs on
+    // `stream` (neither reads the fields); its own pinned read-back words
+    hipStream_t stream2 = nullptr;
+    int* h_side_pinned = nullptr;
+    hipEvent_t ev_side = nullptr;            // end of the side stream's work of a step
+    int (*overlap_hook)() = nullptr;         // run once at a solver's first host wait (overlap_point)
+    int overlap_rc = 0;
     int particle_bits = 64;
     int solver_mode = 0;
     std::string err;
@@ -301,6 +308,9 @@ int launch_ppr();
 int run_pcg_fp64(bool reference_semantics);
 int run_pcg_mixed();
 int run_pcg_ic();
+// Called by the solvers right before their first host wait, with a chunk of solver work queued:
+// runs the pending side-stream work of pic_step (if any) so that it overlaps the solve.
+void overlap_point();
 int run_mcc();
 int run_sort_all();
 int launch_zero_acc(int isp);
