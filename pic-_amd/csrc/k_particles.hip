@@ -80,6 +80,14 @@ __device__ inline void wave_add(unsigned long long* ctr, int c) {
 // LDS deposition window of a block: node columns x0..x0+nx-1, rows y0..y0+ny-1.
 struct DepWindow { int x0, y0, nx, ny; };
 
+// 64-bit add to an LDS word through an LDS-typed pointer: the compiler cannot merge it with
+// the global atomic of the other branch into one flat atomic (which would count on both the
+// vector-memory and the LDS wait counters).
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+__device__ inline void lds_add(long long* p, long long v) {
+    __hip_atomic_fetch_add((lds_u64*)p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Add the eight moments of element `key` to the window (when its four nodes lie in it)
 // or to the global node accumulators.
 __device__ inline void add_element(int key, const long long v[8], const MeshParams& m, long long* acc_w,
@@ -89,14 +97,8 @@ __device__ inline void add_element(int key, const long long v[8], const MeshPara
     int li = ci - w.x0, lj = cj - w.y0;
     if (li >= 0 && lj >= 0 && li + 1 < w.nx && lj + 1 < w.ny) {
         int l1 = li * w.ny + lj, l2 = l1 + w.ny, l3 = l2 + 1, l4 = l1 + 1;
-        atomicAdd((unsigned long long*)&winW[l1], (unsigned long long)v[0]);
-        atomicAdd((unsigned long long*)&winW[l2], (unsigned long long)v[1]);
-        atomicAdd((unsigned long long*)&winW[l3], (unsigned long long)v[2]);
-        atomicAdd((unsigned long long*)&winW[l4], (unsigned long long)v[3]);
-        atomicAdd((unsigned long long*)&winE[l1], (unsigned long long)v[4]);
-        atomicAdd((unsigned long long*)&winE[l2], (unsigned long long)v[5]);
-        atomicAdd((unsigned long long*)&winE[l3], (unsigned long long)v[6]);
-        atomicAdd((unsigned long long*)&winE[l4], (unsigned long long)v[7]);
+        lds_add(&winW[l1], v[0]); lds_add(&winW[l2], v[1]); lds_add(&winW[l3], v[2]); lds_add(&winW[l4], v[3]);
+        lds_add(&winE[l1], v[4]); lds_add(&winE[l2], v[5]); lds_add(&winE[l3], v[6]); lds_add(&winE[l4], v[7]);
     } else {
         int n1 = ci * m.nny + cj, n2 = n1 + m.nny, n3 = n2 + 1, n4 = n1 + 1;
         atomicAdd((unsigned long long*)&acc_w[n1], (unsigned long long)v[0]);
@@ -636,23 +638,29 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
-    // all PUSH_PPT slots' state is loaded up front (one wait); the loop is unrolled so the
+    // all PUSH_PPT slots' state is loaded up front, with no branch around the loads (a slot
+    // past the end reads the block's first slot, nb >= 1, and is marked dead), so the loads of
+    // every slot are in flight together and one wait covers them; the loop is unrolled so the
     // per-slot registers are indexed statically
     int cel[PUSH_PPT];
     R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT], ths_[PUSH_PPT];
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
         const unsigned int o = o0 + 64 * k;
-        cel[k] = -1; xs_[k] = (R)0; ys_[k] = (R)0; vxs_[k] = (R)0; vys_[k] = (R)0; vzs_[k] = (R)0; ths_[k] = (R)0;
-        if (o < nb) {
-            cel[k] = pic_ld(cellb, o); xs_[k] = pic_ld((const R*)xb, o); ys_[k] = pic_ld((const R*)yb, o);
-            vxs_[k] = pic_ld((const R*)vxb, o); vys_[k] = pic_ld((const R*)vyb, o);
-            vzs_[k] = pic_ld((const R*)vzb, o);
-            if (RZ && DO_PUSH) ths_[k] = pic_ld((const R*)thb, o);
-        }
+        const unsigned int oc = o < nb ? o : 0u;
+        const int c = pic_ld(cellb, oc);
+        cel[k] = o < nb ? c : -1;
+        xs_[k] = pic_ld((const R*)xb, oc); ys_[k] = pic_ld((const R*)yb, oc);
+        vxs_[k] = pic_ld((const R*)vxb, oc); vys_[k] = pic_ld((const R*)vyb, oc);
+        vzs_[k] = pic_ld((const R*)vzb, oc);
+        ths_[k] = (RZ && DO_PUSH) ? pic_ld((const R*)thb, oc) : (R)0;
     }
-    // the element gathers of every slot, before any slot is computed (a dead slot reads
-    // element 0's, unused)
+    // one wait for all of them on every path (vmcnt(0)): the compiler sinks the first slot's
+    // gathers into a branch, and on the path around it would otherwise treat these loads as
+    // outstanding up to their uses in later slots, where its wait also drains those slots'
+    // stores
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    unsigned int defer_bits = 0;     // slot k handed to the wall kernel: bit k
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
@@ -664,20 +672,15 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         bool dep = false;
         if (DO_PUSH) {
             bool defer = false;
+            // the gathers outside the branch (a dead slot reads element 0's, unused)
+            HotIn<R> hin;
+            hot_gather<R>(a, e_old >= 0 ? e_old : 0, hin);
             if (e_old >= 0) {
-                HotIn<R> hin;
-                hot_gather<R>(a, e_old, hin);
                 defer = RZ ? push_hot_rz<R>(a, e_old, hin, X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
                            : push_hot<R>(a, e_old, hin, X, Y, VX, VY, &next, &nci, &ncj);
                 dep = !defer;
             }
-            const unsigned long long wm = __ballot(defer);   // wave-aggregated append to the wall list
-            if (wm) {
-                unsigned int wb = 0;
-                if (lane == 0) wb = atomicAdd(a.wall_count, (unsigned int)__popcll(wm));
-                wb = __shfl(wb, 0, 64);
-                if (defer) a.wall_list[wb + __popcll(wm & ((1ull << lane) - 1ull))] = (int)(base + o);
-            }
+            defer_bits |= (defer ? 1u : 0u) << k;
         } else if (e_old >= 0) {
             int bad = 0;
             next = pic_locate_ij((double)X, (double)Y, m, &bad, &nci, &ncj);
@@ -718,6 +721,29 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     // each thread's last run straight to the window (LDS atomics; lanes of one element add
     // to the same eight words, which the LDS serialises)
     if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
+    // the handed-over slots of all PUSH_PPT iterations appended to the wall list with one
+    // wave-aggregated atomic (its returned base is the only wait; the wall kernel treats the
+    // list as an unordered set)
+    if (DO_PUSH) {
+        unsigned long long wm[PUSH_PPT];
+        unsigned int tot = 0;
+#pragma unroll
+        for (int k = 0; k < PUSH_PPT; ++k) {
+            wm[k] = __ballot((defer_bits >> k) & 1u);
+            tot += (unsigned int)__popcll(wm[k]);
+        }
+        if (tot) {
+            unsigned int wb = 0;
+            if (lane == 0) wb = atomicAdd(a.wall_count, tot);
+            wb = __shfl(wb, 0, 64);
+            const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+            for (int k = 0; k < PUSH_PPT; ++k) {
+                if ((defer_bits >> k) & 1u) a.wall_list[wb + __popcll(wm[k] & below)] = (int)(base + o0 + 64 * k);
+                wb += (unsigned int)__popcll(wm[k]);
+            }
+        }
+    }
 
     // ---------------- flush the window
     __syncthreads();
