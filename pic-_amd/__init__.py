@@ -316,6 +316,14 @@ def workload(name: str) -> dict:
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n), delta=1,
                      objects=[Circle(0.5 * n, 0.5 * n, 0.15 * n, -1.0)], wall_dirichlet=(1, 1, 0, 1))
         return dict(domain=dom, ppc=8, bits=64, solver=0, tol=1e-12, Te=2.0, Ti=0.026)
+    if name.startswith("skew"):   # parity cases skew<N>: origin (0.3, -0.1), spacings 0.7 x 0.9 (neither a
+        # power of two nor exact in FP32), so the lookup, the deposition weights and the node lines take
+        # their general paths (FP64 location, divisions by the spacing, FP32 node lines rounded from FP64)
+        n = int(name[4:] or 24)
+        x0, y0, hx, hy = 0.3, -0.1, 0.7, 0.9
+        dom = Domain(n + 1, n + 1, x0, x0 + hx * n, y0, y0 + hy * n,
+                     objects=[Circle(x0 + 0.5 * hx * n, y0 + 0.5 * hy * n, 0.15 * hx * n, -1.0)])
+        return dict(domain=dom, ppc=8, bits=64, solver=0, tol=1e-12, Te=2.0, Ti=0.026)
     if name.startswith("small"):  # small parity cases: small<N>
         n = int(name[5:] or 16)
         dom = Domain(n + 1, n + 1, 0.0, float(n), 0.0, float(n),

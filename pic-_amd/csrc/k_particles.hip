@@ -189,19 +189,20 @@ __device__ inline void deposit_weights(double X, double Y, int e, const MeshPara
 
 struct PushEvents { int emit, right, object, reflect, deleted; };
 
-// Node coordinate lines in R. The FP64 lines are xmin + hx i by construction (the set-up's
-// formula, recomputed with the same two roundings); the FP32 lines are their FP32 roundings,
-// recomputed in FP32 when the set-up found that evaluation exact for every node (nodes_f32),
-// else loaded.
+// Node coordinate lines in R, recomputed (no loads in the hot loop). The FP64 lines are
+// xmin + hx i by construction (the set-up's formula, evaluated with the same two roundings);
+// the FP32 lines are their FP32 roundings, evaluated in FP32 when the set-up found that exact
+// for every node (nodes_f32), else as the FP64 formula rounded once to FP32 (the value the
+// set-up stores in its FP32 node table).
 template <typename R>
 __device__ inline R node_x(const PushArgs<R>& a, int i) {
-    if (sizeof(R) == 8) return (R)(a.m.xmin + a.m.hx * (double)i);
-    return a.m.nodes_f32 ? a.x0R + a.hxR * (R)i : pic_ld(a.xsR, i);
+    if (sizeof(R) == 8 || !a.m.nodes_f32) return (R)(a.m.xmin + a.m.hx * (double)i);
+    return a.x0R + a.hxR * (R)i;
 }
 template <typename R>
 __device__ inline R node_y(const PushArgs<R>& a, int j) {
-    if (sizeof(R) == 8) return (R)(a.m.ymin + a.m.hy * (double)j);
-    return a.m.nodes_f32 ? a.y0R + a.hyR * (R)j : pic_ld(a.ysR, j);
+    if (sizeof(R) == 8 || !a.m.nodes_f32) return (R)(a.m.ymin + a.m.hy * (double)j);
+    return a.y0R + a.hyR * (R)j;
 }
 
 // The loads of a particle's first pass that depend only on its element: the neighbourhood
@@ -574,18 +575,14 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 }
 
 // Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
-// at 8 slots per thread the register budget of 4 waves per SIMD is asked for explicitly
-// (the unconstrained allocation lands a few registers above it, at 3 waves)
-#if !defined(PUSH_WAVES_PER_EU) && PIC_SORT_GROUP == 512
-#define PUSH_WAVES_PER_EU 4
-#endif
-#ifdef PUSH_WAVES_PER_EU
-#define PUSH_ATTR __attribute__((amdgpu_waves_per_eu(PUSH_WAVES_PER_EU, 8)))
-#else
-#define PUSH_ATTR
-#endif
+// Register budget per SIMD: the Cartesian FP32 pass at 4 slots per thread is held to 5 waves
+// (96 VGPRs; left alone the allocation lands at 98, i.e. 4 waves, and the fifth wave is worth
+// more than the two registers spilled around the slot loop); the axisymmetric and FP64 passes
+// and 8 slots per thread keep 4 (at 5 they spill 9 to 24 registers for no gain).
+#define PUSH_WAVES(R, RZ) ((sizeof(R) == 4 && !(RZ) && PUSH_PPT == 4) ? 5 : 4)
 template <typename R, int DO_PUSH, int RZ>
-__global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<R> a) {
+__global__ __launch_bounds__(PUSH_BLOCK) __attribute__((amdgpu_waves_per_eu(PUSH_WAVES(R, RZ), 8)))
+void k_push_deposit(PushArgs<R> a) {
     __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
     __shared__ int s_center;
     const MeshParams& m = a.m;
@@ -638,28 +635,33 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
-    // all PUSH_PPT slots' state is loaded up front, with no branch around the loads (a slot
-    // past the end reads the block's first slot, nb >= 1, and is marked dead), so the loads of
-    // every slot are in flight together and one wait covers them; the loop is unrolled so the
-    // per-slot registers are indexed statically
+    // Software pipeline over the PUSH_PPT slots: the cells of every slot are loaded up front
+    // (one wait), then the state and the element gathers of slot k + 1 are issued before slot
+    // k is computed, so each slot's loads are in flight during the previous slot's arithmetic
+    // and a slot waits only for its own loads. No branch surrounds a load: a slot past the end
+    // reads the block's first slot (nb >= 1) and is marked dead, a dead slot's gathers read
+    // element 0's; the loop is unrolled so the per-slot registers are indexed statically.
     int cel[PUSH_PPT];
-    R xs_[PUSH_PPT], ys_[PUSH_PPT], vxs_[PUSH_PPT], vys_[PUSH_PPT], vzs_[PUSH_PPT], ths_[PUSH_PPT];
 #pragma unroll
     for (int k = 0; k < PUSH_PPT; ++k) {
         const unsigned int o = o0 + 64 * k;
-        const unsigned int oc = o < nb ? o : 0u;
-        const int c = pic_ld(cellb, oc);
+        const int c = pic_ld(cellb, o < nb ? o : 0u);
         cel[k] = o < nb ? c : -1;
-        xs_[k] = pic_ld((const R*)xb, oc); ys_[k] = pic_ld((const R*)yb, oc);
-        vxs_[k] = pic_ld((const R*)vxb, oc); vys_[k] = pic_ld((const R*)vyb, oc);
-        vzs_[k] = pic_ld((const R*)vzb, oc);
-        ths_[k] = (RZ && DO_PUSH) ? pic_ld((const R*)thb, oc) : (R)0;
     }
-    // one wait for all of them on every path (vmcnt(0)): the compiler sinks the first slot's
-    // gathers into a branch, and on the path around it would otherwise treat these loads as
-    // outstanding up to their uses in later slots, where its wait also drains those slots'
-    // stores
+    // vmcnt(0) on every path, so the compiler tracks no cell load as outstanding past here
     __builtin_amdgcn_s_waitcnt(0x0F70);
+    R nx, ny, nvx, nvy, nvz, nth;    // state of the slot issued next
+    HotIn<R> nh{};                   // its element gathers (DO_PUSH)
+    auto issue = [&](int k) {
+        const unsigned int o = o0 + 64 * k;
+        const unsigned int oc = o < nb ? o : 0u;
+        nx = pic_ld((const R*)xb, oc); ny = pic_ld((const R*)yb, oc);
+        nvx = pic_ld((const R*)vxb, oc); nvy = pic_ld((const R*)vyb, oc);
+        nvz = pic_ld((const R*)vzb, oc);
+        nth = (RZ && DO_PUSH) ? pic_ld((const R*)thb, oc) : (R)0;
+        if (DO_PUSH) hot_gather<R>(a, cel[k] >= 0 ? cel[k] : 0, nh);
+    };
+    issue(0);
     unsigned int defer_bits = 0;     // slot k handed to the wall kernel: bit k
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
@@ -667,14 +669,13 @@ __global__ __launch_bounds__(PUSH_BLOCK) PUSH_ATTR void k_push_deposit(PushArgs<
         const unsigned int o = o0 + 64 * k;
         const bool in = o < nb;
         const int e_old = cel[k];
-        R X = xs_[k], Y = ys_[k], VX = vxs_[k], VY = vys_[k], VZ = vzs_[k], TH = ths_[k];
+        R X = nx, Y = ny, VX = nvx, VY = nvy, VZ = nvz, TH = nth;
+        const HotIn<R> hin = nh;
+        if (k + 1 < PUSH_PPT) issue(k + 1);
         int next = -1, nci = 0, ncj = 0;
         bool dep = false;
         if (DO_PUSH) {
             bool defer = false;
-            // the gathers outside the branch (a dead slot reads element 0's, unused)
-            HotIn<R> hin;
-            hot_gather<R>(a, e_old >= 0 ? e_old : 0, hin);
             if (e_old >= 0) {
                 defer = RZ ? push_hot_rz<R>(a, e_old, hin, X, Y, VX, VY, VZ, TH, &next, &nci, &ncj)
                            : push_hot<R>(a, e_old, hin, X, Y, VX, VY, &next, &nci, &ncj);

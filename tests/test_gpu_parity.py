@@ -202,11 +202,13 @@ def test_axisymmetric_load_follows_volume(gpu_lib, pkg):
 
 
 # ------------------------------------------------------------------ mover
-@pytest.mark.parametrize("bits", [64, 32])
-def test_moveall_bit_exact(gpu_lib, pkg, orc, bits):
+@pytest.mark.parametrize("bits,case", [(64, "small24"), (32, "small24"), (64, "skew24"), (32, "skew24")])
+def test_moveall_bit_exact(gpu_lib, pkg, orc, bits, case):
+    """skew24: origin (0.3, -0.1) and spacings 0.7 x 0.9, so the fused pass locates in FP64,
+    divides by the spacing in the weights and rounds its FP32 node lines from the FP64 ones."""
     pl = gpu_lib
-    say(f"case: MoveAll bit-exact bits={bits}")
-    w = pkg.workload("small24")
+    say(f"case: MoveAll bit-exact bits={bits} {case}")
+    w = pkg.workload(case)
     dom = w["domain"]
     pl.setup(dom, bits=bits, solver=0, tol=1e-12)
     weight = pkg.species_weight(dom, 16)
@@ -370,7 +372,8 @@ def test_moveall_axisymmetric_bit_exact(gpu_lib, pkg, orc, bits):
 
 
 # ------------------------------------------------------------------ deposition
-@pytest.mark.parametrize("bits,case", [(64, "small24"), (32, "small24"), (64, "rz24"), (32, "rz24")])
+@pytest.mark.parametrize("bits,case", [(64, "small24"), (32, "small24"), (64, "rz24"), (32, "rz24"),
+                                       (64, "skew24"), (32, "skew24")])
 def test_getparchg_moments(gpu_lib, pkg, orc, bits, case):
     pl = gpu_lib
     say(f"case: GetParChg moments bits={bits} {case}")
@@ -580,7 +583,7 @@ def test_positioning_dropins(gpu_lib, pkg, orc):
 
 
 # ------------------------------------------------------------------ full step determinism
-@pytest.mark.parametrize("case", ["small32", "rz32", "diel32", "dielrz32"])
+@pytest.mark.parametrize("case", ["small32", "rz32", "diel32", "dielrz32", "skew32"])
 def test_full_step_deterministic_and_conserving(gpu_lib, pkg, case):
     """Seven full steps (mover, deposit, solve, PPR, MCC with ionisation products, a cell sort
     every 3 steps) twice from the same state: identical bits, the live count consistent; on
