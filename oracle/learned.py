@@ -7,7 +7,9 @@ reference; this file restates the build's own definition.
 Hierarchy: level 0 is the Jacobi-scaled node-space operator A' (unit diagonal on unknown
 nodes). Level l+1 has the nodes (I, J) <-> fine (2I, 2J); P_l is bilinear interpolation
 with the rows of fine Dirichlet nodes (zero diagonal) removed, R_l = P_l^T, and
-A_{l+1} = P_l^T A_l P_l (Galerkin). The coarsest level is solved exactly.
+A_{l+1} = P_l^T A_l P_l (Galerkin). Coarsening stops at the first level with at most
+DENSE_NODES nodes once every learned level has a coarser one (or at about 3 x 3, or at
+max_levels); that coarsest level is solved exactly with a dense inverse.
 
 Symmetric V(1,1)-cycle y = V_l(r):
     x  = K_l r
@@ -35,16 +37,25 @@ TAPS = [(t // 3 - 1, t % 3 - 1) for t in range(9)]
 OMEGA = 0.8
 
 
-def level_sizes(H0: int, W0: int, max_levels: int = 12):
+DENSE_NODES = 1089            # 33 x 33: the largest coarsest level solved by its dense inverse
+
+
+def level_sizes(H0: int, W0: int, max_levels: int = 12, learned_levels: int = 2, dense_nodes: int = DENSE_NODES):
+    """Node-grid sizes of the hierarchy: halve (vertex-centred) while the coarsest level has
+    more than dense_nodes nodes or fewer than max(2, learned_levels + 1) levels exist, down to
+    about 3 x 3 and at most max_levels levels (dense_nodes = 0: always down to 3 x 3)."""
     H, W = [H0], [W0]
-    while len(H) < max_levels and min(H[-1], W[-1]) > 3:
+    min_levels = max(2, learned_levels + 1)
+    while (len(H) < max_levels and min(H[-1], W[-1]) > 3 and
+           (H[-1] * W[-1] > dense_nodes or len(H) < min_levels)):
         H.append((H[-1] + 1) // 2)
         W.append((W[-1] + 1) // 2)
     return H, W
 
 
-def layout(H0: int, W0: int, channels, learned_levels: int = 2, max_levels: int = 12) -> dict:
-    H, W = level_sizes(H0, W0, max_levels)
+def layout(H0: int, W0: int, channels, learned_levels: int = 2, max_levels: int = 12,
+           dense_nodes: int = DENSE_NODES) -> dict:
+    H, W = level_sizes(H0, W0, max_levels, learned_levels, dense_nodes)
     L = len(H)
     Ls = min(learned_levels, L - 1)
     C = [channels[min(l, len(channels) - 1)] for l in range(Ls)]
@@ -82,8 +93,8 @@ def prolongation(Hf, Wf, Hc, Wc, maskf):
     return sp.csr_matrix((vals, (rows, cols)), shape=(Hf * Wf, Hc * Wc))
 
 
-def build_hierarchy(A0, H0, W0, max_levels: int = 12):
-    H, W = level_sizes(H0, W0, max_levels)
+def build_hierarchy(A0, H0, W0, max_levels: int = 12, learned_levels: int = 2, dense_nodes: int = DENSE_NODES):
+    H, W = level_sizes(H0, W0, max_levels, learned_levels, dense_nodes)
     levels = []
     A = sp.csr_matrix(A0)
     for l in range(len(H)):

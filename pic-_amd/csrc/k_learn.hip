@@ -7,7 +7,8 @@
 // Jacobi-scaled node operator A' (unit diagonal on unknowns); level l+1 holds the nodes
 // (I, J) <-> fine (2I, 2J); P_l is bilinear interpolation with the rows of fine Dirichlet
 // nodes removed, R_l = P_l^T, A_{l+1} = P_l^T A_l P_l (nine diagonals per level); the
-// coarsest level (<= 3 nodes across) is solved exactly with a dense inverse.
+// coarsening stops at the first level with at most 33 x 33 nodes (once every learned level
+// has a coarser one), and that coarsest level is solved exactly with a dense inverse.
 //
 // Network = symmetric V(1,1)-cycle (encoder: smooth, residual, restrict; decoder:
 // interpolate, residual, smooth):
@@ -46,6 +47,7 @@
 #define LN_K25 96
 #define LN_OMEGA 0.8f
 #define LN_MAX_COARSE 2048
+#define LN_DENSE_NODES 1089                          // 33 x 33: coarsest level of the hierarchy
 
 // ---------------------------------------------------------------- multigrid transfer kernels
 // ---------------------------------------------------------------- FCG(1) on the scaled system
@@ -596,186 +598,45 @@ __global__ __launch_bounds__(256) void k_mg_up_smooth(MgArgs a) {
     a.s2[(int64_t)b * a.fs + f] = s2s[(ti + 2) * MGS_SJ + (tj + 2)];
 }
 
-// Coarsest level: y = inv r (dense, n <= LN_MAX_COARSE), one block per batch entry.
-__global__ __launch_bounds__(256) void k_mg_coarse(int n, const float* inv, const float* r, float* y, int64_t bs,
-                                                   const double* done) {
-    if (done && *done != 0.0) return;
+// Coarsest level: y = inv r with the dense inverse of the level's operator over its unknown
+// nodes (rows and columns of Dirichlet nodes are zero; n <= LN_MAX_COARSE nodes). One wave per
+// row: the lanes read the row with unit stride (the inverse, 4.7 MB at 33 x 33, stays in the
+// last-level cache between applications) against r staged in LDS, then a wave reduction in a
+// fixed order. MGX_ROWS rows per block, so the rows spread over the whole GPU; the batch entry
+// is blockIdx.y. Replaces the Jacobi levels below 33 x 33: one launch instead of a chain of
+// small phases, and an exact coarse correction (fewer FCG iterations).
+#define MGX_ROWS 4
+#define MGX_RIT (LN_MAX_COARSE / 64)
+#define MGX_SIT (LN_MAX_COARSE / (64 * MGX_ROWS))
+__global__ __launch_bounds__(64 * MGX_ROWS) void k_mg_dense(int n, const float* inv, const float* r, int64_t bs, float* y,
+                                                            const double* done) {
+    __shared__ float rs[LN_MAX_COARSE];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = blockIdx.x * MGX_ROWS + w;
     const float* rb = r + (int64_t)blockIdx.y * bs;
-    float* yb = y + (int64_t)blockIdx.y * bs;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        float s = 0.0f;
-        for (int j = 0; j < n; ++j) s = s + inv[(int64_t)i * n + j] * rb[j];
-        yb[i] = s;
-    }
-}
-
-// Levels lc .. L-1 (Jacobi smoothers, all <= 33 x 33 nodes) in ONE workgroup per batch entry:
-// the down sweeps, the exact coarsest solve and the up sweeps of the V-cycle, phase by phase
-// with block barriers. The operators of these levels (nine diagonals and 1/diag, node order,
-// and the dense coarsest inverse when it fits) are packed on the host in the LDS order and
-// copied in with a few float4 loads per thread at the start, so after that first wait every
-// phase works from LDS. The vectors r, x, y, t of each level are stored with a one-node zero
-// halo ((H + 2) x (W + 2), zeroed once), so the stencils, restriction and interpolation read
-// without bounds tests; a node's row is (f + 1/2) * RN(1/W) truncated (exact for these sizes)
-// instead of an integer division. One CU's issue rate bounds these phases, so every
-// instruction per node counts; the arithmetic uses FMA (the solve is not bit-compared).
-// Replaces ~2 launches per level whose run time is pure latency at these sizes.
-#define MG_CB_THREADS 1024
-#define MG_CB_MAXN 1089
-#define MG_CB_OPS_IT 6                               // float4 loads per thread for the operators
-#define MG_CB_OPS_CAP (MG_CB_OPS_IT * MG_CB_THREADS * 4)
-#define MG_CB_RIT ((MG_CB_MAXN + MG_CB_THREADS - 1) / MG_CB_THREADS)
-static_assert(MG_CB_RIT * MG_CB_THREADS >= LN_MAX_COARSE, "coarse workgroup stages the coarsest level too");
-struct MgCoarseArgs {
-    int lc, L;
-    int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
-    float rw[LN_MAX_LEVELS];                     // RN(1 / W)
-    int voff[LN_MAX_LEVELS];                     // LDS offset (floats) of each level's padded vectors r, x, y, t
-    int ooff[LN_MAX_LEVELS];                     // LDS offset of each level's dia [9][n] then dinv [n]
-    int vtot;                                    // floats of all padded vectors (zeroed at the start)
-    int ioff;                                    // LDS offset of the coarsest inverse, -1: read from inv
-    int nops4;                                   // packed operator float4s (<= MG_CB_OPS_IT * threads)
-    const float4* ops;
-    const float* rin; int64_t rin_bs;            // residual of level lc (global)
-    float* yout; int64_t yout_bs;                // correction of level lc (global)
-    const float* inv; float omega;
-    const double* done;
-};
-
-__device__ inline int mgcb_row(int f, float rw) { return (int)(((float)f + 0.5f) * rw); }
-
-__global__ __launch_bounds__(MG_CB_THREADS) void k_mg_coarse_block(MgCoarseArgs a) {
-    extern __shared__ float lds[];
-    const int b = blockIdx.y;
-    float4 ov[MG_CB_OPS_IT];
+    const float* ir = inv + (int64_t)(row < n ? row : 0) * n;
+    // every global load first: the row, then the block's share of r
+    float iv[MGX_RIT];
 #pragma unroll
-    for (int q = 0; q < MG_CB_OPS_IT; ++q) {
-        const int k = threadIdx.x + q * MG_CB_THREADS;
-        ov[q] = a.ops[k < a.nops4 ? k : 0];
+    for (int c = 0; c < MGX_RIT; ++c) {
+        const int j = lane + 64 * c;
+        iv[c] = j < n ? ir[j] : 0.0f;
     }
-    const int H0 = a.H[a.lc], W0 = a.W[a.lc];
-    const int n0 = H0 * W0;
-    float rv[MG_CB_RIT];
+    float rv[MGX_SIT];
 #pragma unroll
-    for (int q = 0; q < MG_CB_RIT; ++q) {
-        const int k = threadIdx.x + q * MG_CB_THREADS;
-        rv[q] = a.rin[(int64_t)b * a.rin_bs + (k < n0 ? k : 0)];
+    for (int q = 0; q < MGX_SIT; ++q) {
+        const int k = threadIdx.x + 64 * MGX_ROWS * q;
+        rv[q] = k < n ? rb[k] : 0.0f;
     }
-    if (a.done && *a.done != 0.0) return;
+    if (done && *done != 0.0) return;
 #pragma unroll
-    for (int q = 0; q < MG_CB_OPS_IT; ++q) {
-        const int k = threadIdx.x + q * MG_CB_THREADS;
-        if (k < a.nops4) reinterpret_cast<float4*>(lds)[k] = ov[q];
-    }
-    float* vbase = lds + a.voff[a.lc];
-    for (int k = threadIdx.x; k < a.vtot; k += MG_CB_THREADS) vbase[k] = 0.0f;
+    for (int q = 0; q < MGX_SIT; ++q) rs[threadIdx.x + 64 * MGX_ROWS * q] = rv[q];
     __syncthreads();
-    {
-        float* r = lds + a.voff[a.lc];
-        const int Wp = W0 + 2;
+    float acc = 0.0f;
 #pragma unroll
-        for (int q = 0; q < MG_CB_RIT; ++q) {
-            const int k = threadIdx.x + q * MG_CB_THREADS;
-            if (k < n0) {
-                const int fi = mgcb_row(k, a.rw[a.lc]), fj = k - fi * W0;
-                r[(fi + 1) * Wp + fj + 1] = rv[q];
-            }
-        }
-    }
-    __syncthreads();
-    for (int l = a.lc; l < a.L - 1; ++l) {
-        const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
-        const int nf = Hf * Wf, nc = Hc * Wc;
-        const int Wp = Wf + 2, Pf = (Hf + 2) * Wp, Wpc = Wc + 2;
-        const float rwf = a.rw[l], rwc = a.rw[l + 1];
-        float* r = lds + a.voff[l];
-        float* x = r + Pf;
-        float* s = r + 3 * Pf;
-        float* rc = lds + a.voff[l + 1];
-        const float* dia = lds + a.ooff[l];
-        const float* dinv = dia + 9 * nf;
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
-            const int p = (fi + 1) * Wp + fj + 1;
-            x[p] = a.omega * dinv[f] * r[p];
-        }
-        __syncthreads();
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
-            const int p = (fi + 1) * Wp + fj + 1;
-            float v = r[p];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) v = fmaf(-dia[t * nf + f], x[p + (t / 3 - 1) * Wp + (t % 3 - 1)], v);
-            s[p] = dia[4 * nf + f] != 0.0f ? v : 0.0f;
-        }
-        __syncthreads();
-        for (int c = threadIdx.x; c < nc; c += MG_CB_THREADS) {
-            const int I = mgcb_row(c, rwc), J = c - I * Wc;
-            const float* s0 = s + (2 * I) * Wp + 2 * J;          // fine node (2I - 1, 2J - 1), padded
-            const float acc = 0.25f * (s0[0] + s0[2] + s0[2 * Wp] + s0[2 * Wp + 2]) +
-                              0.5f * (s0[1] + s0[Wp] + s0[Wp + 2] + s0[2 * Wp + 1]) + s0[Wp + 1];
-            rc[(I + 1) * Wpc + J + 1] = acc;
-        }
-        __syncthreads();
-    }
-    {
-        const int l = a.L - 1;
-        const int Wl = a.W[l], n = a.H[l] * Wl, Wp = Wl + 2, P = (a.H[l] + 2) * Wp;
-        const float* r = lds + a.voff[l];
-        float* y = lds + a.voff[l] + 2 * P;
-        const float* inv = a.ioff >= 0 ? lds + a.ioff : a.inv;
-        for (int i = threadIdx.x; i < n; i += MG_CB_THREADS) {
-            float sacc = 0.0f;
-            for (int j = 0; j < n; ++j) {
-                const int jr = mgcb_row(j, a.rw[l]);
-                sacc = fmaf(inv[i * n + j], r[(jr + 1) * Wp + (j - jr * Wl) + 1], sacc);
-            }
-            const int ir = mgcb_row(i, a.rw[l]);
-            y[(ir + 1) * Wp + (i - ir * Wl) + 1] = sacc;
-        }
-        __syncthreads();
-    }
-    for (int l = a.L - 2; l >= a.lc; --l) {
-        const int Hf = a.H[l], Wf = a.W[l], Hc = a.H[l + 1], Wc = a.W[l + 1];
-        const int nf = Hf * Wf;
-        const int Wp = Wf + 2, Pf = (Hf + 2) * Wp, Wpc = Wc + 2, Pc = (Hc + 2) * Wpc;
-        const float rwf = a.rw[l];
-        const float* r = lds + a.voff[l];
-        const float* x = r + Pf;
-        float* y = lds + a.voff[l] + 2 * Pf;
-        float* x2 = lds + a.voff[l] + 3 * Pf;
-        const float* yc = lds + a.voff[l + 1] + 2 * Pc;
-        const float* dia = lds + a.ooff[l];
-        const float* dinv = dia + 9 * nf;
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int gi = mgcb_row(f, rwf), gj = f - gi * Wf;
-            const int p = (gi + 1) * Wp + gj + 1;
-            // bilinear interpolation: weights (1, 0) on an even index, (1/2, 1/2) on an odd one
-            const int i0 = gi >> 1, j0 = gj >> 1;
-            const float ai = (gi & 1) ? 0.5f : 1.0f, bi = (gi & 1) ? 0.5f : 0.0f;
-            const float aj = (gj & 1) ? 0.5f : 1.0f, bj = (gj & 1) ? 0.5f : 0.0f;
-            const float* e0 = yc + (i0 + 1) * Wpc + j0 + 1;
-            const float pe = ai * fmaf(aj, e0[0], bj * e0[1]) + bi * fmaf(aj, e0[Wpc], bj * e0[Wpc + 1]);
-            x2[p] = dia[4 * nf + f] != 0.0f ? x[p] + pe : 0.0f;
-        }
-        __syncthreads();
-        for (int f = threadIdx.x; f < nf; f += MG_CB_THREADS) {
-            const int fi = mgcb_row(f, rwf), fj = f - fi * Wf;
-            const int p = (fi + 1) * Wp + fj + 1;
-            float sres = r[p];
-#pragma unroll
-            for (int t = 0; t < 9; ++t) sres = fmaf(-dia[t * nf + f], x2[p + (t / 3 - 1) * Wp + (t % 3 - 1)], sres);
-            y[p] = dia[4 * nf + f] != 0.0f ? fmaf(a.omega * dinv[f], sres, x2[p]) : 0.0f;
-        }
-        __syncthreads();
-    }
-    {
-        const float* y = lds + a.voff[a.lc] + 2 * (H0 + 2) * (W0 + 2);
-        for (int f = threadIdx.x; f < n0; f += MG_CB_THREADS) {
-            const int fi = mgcb_row(f, a.rw[a.lc]), fj = f - fi * W0;
-            a.yout[(int64_t)b * a.yout_bs + f] = y[(fi + 1) * (W0 + 2) + fj + 1];
-        }
-    }
+    for (int c = 0; c < MGX_RIT; ++c) acc = fmaf(iv[c], rs[lane + 64 * c], acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0 && row < n) y[(int64_t)blockIdx.y * bs + row] = acc;
 }
 
 // Learned smoother, collapsed: out = [out +] Dh (In^T S In) (Dh v) on one level with the
@@ -1311,13 +1172,9 @@ struct MgLevel {
 struct LearnNet {
     bool ready = false;
     int L = 0, Ls = 0;                               // levels, learned smoother levels
-    int lc = 0;                                      // first level of the single-workgroup coarse part
-    size_t lc_lds = 0;                               // its LDS bytes
     MgLevel lv[LN_MAX_LEVELS];
     int C[LN_MAX_LEARNED] = {0};
-    float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator
-    float* d_cops = nullptr; int cops_n = 0;         // coarse workgroup operators in its LDS order
-    int c_ooff[LN_MAX_LEVELS] = {0}, c_voff[LN_MAX_LEVELS] = {0}, c_ioff = -1, c_vtot = 0;
+    float* d_inv = nullptr; int ncoarse = 0;         // dense inverse of the coarsest operator [n][n]
     void* d_ops = nullptr;                           // one allocation: dia, dinv, dh of all levels
     // parameters (FP32, flat): per learned level W_in [9][C] then Q [C][C]
     int64_t off_in[LN_MAX_LEARNED] = {0}, off_q[LN_MAX_LEARNED] = {0}, nparams = 0;
@@ -1357,7 +1214,7 @@ static LearnNet ln;
 static int64_t ln_npix0() { return ln.lv[0].n; }
 
 static void free_learn() {
-    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_cops, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_G, ln.d_act,
+    void* ptrs[] = {ln.d_k81, ln.d_inv, ln.d_ops, ln.d_p, ln.d_m, ln.d_v, ln.d_g, ln.d_G, ln.d_act,
                     ln.d_lpart, ln.d_lsum, ln.d_idx, ln.d_eab, ln.ds_r, ln.ds_e, ln.ds_ea, ln.d_snap, ln.d_hpart, ln.d_fvec};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     ln = LearnNet();
@@ -1388,9 +1245,9 @@ static void smooth5(int l, int batch, const float* v, int64_t v_bs, float* out, 
 // chk (FCG solve only): the convergence test, run by the first kernel of the V-cycle.
 static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run, const double* done,
                      const FcgCheck* chk = nullptr) {
-    const int L = ln.L, lc = ln.lc;
+    const int L = ln.L;
     const FcgCheck none = {nullptr, 0, 0, 0, nullptr};
-    for (int l = 0; l < lc; ++l) {
+    for (int l = 0; l < L - 1; ++l) {
         const FcgCheck& c0 = (l == 0 && chk) ? *chk : none;
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
@@ -1412,23 +1269,13 @@ static int net_apply(int batch, const float* r0, float* y0, int64_t bs, int run,
             hipLaunchKernelGGL(k_mg_down, grid, dim3(256), 0, g.stream, a);
         }
     }
-    if (lc == 0) return pic_fail(PIC_ERR_STATE, "learned network: coarse part without a fine level");
     {
-        MgCoarseArgs c;
-        std::memset(&c, 0, sizeof(c));
-        c.lc = lc; c.L = L;
-        for (int l = lc; l < L; ++l) {
-            const MgLevel& v = ln.lv[l];
-            c.H[l] = v.H; c.W[l] = v.W; c.voff[l] = ln.c_voff[l]; c.ooff[l] = ln.c_ooff[l];
-            c.rw[l] = 1.0f / (float)v.W;
-        }
-        c.ioff = ln.c_ioff; c.nops4 = ln.cops_n / 4; c.ops = (const float4*)ln.d_cops; c.vtot = ln.c_vtot;
-        c.rin = ln.lv[lc].r[run]; c.rin_bs = ln.lv[lc].n;
-        c.yout = ln.lv[lc].y[run]; c.yout_bs = ln.lv[lc].n;
-        c.inv = ln.d_inv; c.omega = LN_OMEGA; c.done = done;
-        hipLaunchKernelGGL(k_mg_coarse_block, dim3(1, batch), dim3(MG_CB_THREADS), ln.lc_lds, g.stream, c);
+        const MgLevel& v = ln.lv[L - 1];
+        const int n = (int)v.n;
+        hipLaunchKernelGGL(k_mg_dense, dim3((unsigned)((n + MGX_ROWS - 1) / MGX_ROWS), batch), dim3(64 * MGX_ROWS), 0,
+                           g.stream, n, ln.d_inv, v.r[run], v.n, v.y[run], done);
     }
-    for (int l = lc - 1; l >= 0; --l) {
+    for (int l = L - 2; l >= 0; --l) {
         const float* r = l == 0 ? r0 : ln.lv[l].r[run];
         int64_t r_bs = l == 0 ? bs : ln.lv[l].n;
         float* y = l == 0 ? y0 : ln.lv[l].y[run];
@@ -1581,14 +1428,18 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     // level sizes: coarse node I <-> fine node 2I, down to about 3 x 3
     int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
     H[0] = g.mesh.nnx; W[0] = g.mesh.nny;
+    // coarsen until the coarsest level has at most LN_DENSE_NODES nodes and every learned
+    // level has a coarser one (at most max_levels levels, down to about 3 x 3 nodes)
     int L = 1;
-    while (L < max_levels && std::min(H[L - 1], W[L - 1]) > 3) { H[L] = (H[L - 1] + 1) / 2; W[L] = (W[L - 1] + 1) / 2; L++; }
+    const int min_levels = std::max(2, nch + 1);
+    while (L < max_levels && std::min(H[L - 1], W[L - 1]) > 3 &&
+           ((int64_t)H[L - 1] * W[L - 1] > LN_DENSE_NODES || L < min_levels)) {
+        H[L] = (H[L - 1] + 1) / 2; W[L] = (W[L - 1] + 1) / 2; L++;
+    }
     if (L < 2) return pic_fail(PIC_ERR_UNSUPPORTED, "mesh too small for a multigrid hierarchy");
     if ((int64_t)H[L - 1] * W[L - 1] > LN_MAX_COARSE) return pic_fail(PIC_ERR_UNSUPPORTED, "coarsest level too large");
     ln.L = L;
     ln.Ls = std::min(nch, L - 1);
-    ln.lc = L - 1;                                   // levels of <= 33 x 33 nodes go to one workgroup
-    while (ln.lc > std::max(ln.Ls, 1) && (int64_t)H[ln.lc - 1] * W[ln.lc - 1] <= MG_CB_MAXN) ln.lc--;
     for (int l = 0; l < ln.Ls; ++l) {
         int c = channels[l];
         if (c < 1 || c > 64) return pic_fail(PIC_ERR_ARG, "channels must be 1..64");
@@ -1664,40 +1515,6 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     HIPCHK(hipMalloc(&ln.d_inv, invf.size() * sizeof(float)));
     HIPCHK(hipMemcpy(ln.d_inv, invf.data(), invf.size() * sizeof(float), hipMemcpyHostToDevice));
     ln.ncoarse = nc;
-    // the coarse workgroup's LDS: [dia 9n | dinv n] of levels lc..L-1, the coarsest inverse
-    // when it fits, then the vectors r, x, y, t of each level
-    {
-        std::vector<float> cp;
-        for (int l = ln.lc; l < L; ++l) {
-            const MgLevel& v = ln.lv[l];
-            std::vector<float> lvl(10 * v.n);
-            HIPCHK(hipMemcpy(lvl.data(), v.dia, 10 * v.n * sizeof(float), hipMemcpyDeviceToHost));
-            ln.c_ooff[l] = (int)cp.size();
-            cp.insert(cp.end(), lvl.begin(), lvl.end());
-        }
-        ln.c_ioff = -1;
-        if (cp.size() + invf.size() <= (size_t)MG_CB_OPS_CAP) {
-            ln.c_ioff = (int)cp.size();
-            cp.insert(cp.end(), invf.begin(), invf.end());
-        }
-        while (cp.size() % 4) cp.push_back(0.0f);
-        if (cp.size() > (size_t)MG_CB_OPS_CAP) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse operators exceed the workgroup staging");
-        ln.cops_n = (int)cp.size();
-        size_t off = cp.size();
-        for (int l = ln.lc; l < L; ++l) {
-            ln.c_voff[l] = (int)off;
-            off += 4 * (size_t)(ln.lv[l].H + 2) * (size_t)(ln.lv[l].W + 2);   // padded r, x, y, t
-        }
-        ln.c_vtot = (int)(off - cp.size());
-        ln.lc_lds = off * sizeof(float);
-        int maxlds = 0;
-        HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, g.device));
-        if (ln.lc_lds > (size_t)maxlds) return pic_fail(PIC_ERR_UNSUPPORTED, "coarse levels exceed the workgroup LDS");
-        HIPCHK(hipFuncSetAttribute((const void*)k_mg_coarse_block, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)ln.lc_lds));
-        HIPCHK(hipMalloc((void**)&ln.d_cops, cp.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(ln.d_cops, cp.data(), cp.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
     // parameters
     int64_t o = 0;
     for (int l = 0; l < ln.Ls; ++l) { ln.off_in[l] = o; o += 9LL * ln.C[l]; }

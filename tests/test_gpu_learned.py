@@ -176,17 +176,18 @@ def test_fallback_to_plain_cg(gpu_lib, net):
 
 
 def test_forward_one_learned_level_matches_oracle(gpu_lib, pkg):
-    """129 x 129 nodes, one learned level: two Jacobi levels as separate launches (65, 33 is
-    the first level of the coarse workgroup), against the FP64 oracle at the same 1e-4 bound."""
+    """129 x 129 nodes, one learned level: the Jacobi level 65 x 65 by separate launches, 33 x 33
+    the dense coarsest level, against the FP64 oracle at the same 1e-4 bound."""
     say("case: learned forward small128, one learned level")
     dom = pkg.workload("small128")["domain"]
     pl = gpu_lib
     pl.setup(dom, bits=32, solver=1, tol=1e-8)
     pl.learn_setup(12, [16], seed=5)
     lay = pl.learn_layout()
+    assert lay["H"] == [129, 65, 33]
     A, sinv = lrn.scaled_operator(pl.setup_array("A_ROWPTR"), pl.setup_array("A_COL"), pl.setup_array("A_VAL"),
                                   pl.setup_array("A_DIAG"), pl.setup_array("UNKNOWN_NODE"), dom.nnx * dom.nny)
-    levels = lrn.build_hierarchy(A, dom.nnx, dom.nny)
+    levels = lrn.build_hierarchy(A, dom.nnx, dom.nny, learned_levels=1)
     mask = (sinv > 0).astype(np.float64)
     rng = np.random.default_rng(17)
     p = perturbed_params(pl, lay, rng, 0.05)

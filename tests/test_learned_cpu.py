@@ -53,10 +53,17 @@ def test_transposed_convolution_is_adjoint():
 
 
 def test_level_sizes_and_galerkin_hierarchy():
-    H, W = lrn.level_sizes(513, 513)
-    assert H == [513, 257, 129, 65, 33, 17, 9, 5, 3]
+    # coarsening stops at the first level of <= 33 x 33 nodes ...
+    assert lrn.level_sizes(513, 513)[0] == [513, 257, 129, 65, 33]
+    assert lrn.level_sizes(1025, 1025)[0] == [1025, 513, 257, 129, 65, 33]
+    assert lrn.level_sizes(513, 129) == ([513, 257, 129, 65, 33], [129, 65, 33, 17, 9])
+    # ... once every learned level has a coarser one, and never below about 3 x 3
+    assert lrn.level_sizes(17, 17, learned_levels=2)[0] == [17, 9, 5]
+    assert lrn.level_sizes(65, 65, learned_levels=4)[0] == [65, 33, 17, 9, 5]
+    assert lrn.level_sizes(9, 9, learned_levels=4)[0] == [9, 5, 3]
+    assert lrn.level_sizes(513, 513, dense_nodes=0)[0] == [513, 257, 129, 65, 33, 17, 9, 5, 3]
     A, mask = q1_operator(17)
-    lev = lrn.build_hierarchy(A, 17, 17)
+    lev = lrn.build_hierarchy(A, 17, 17, dense_nodes=0)
     assert [l["H"] for l in lev] == [17, 9, 5, 3]
     for l in range(len(lev) - 1):
         Ac = lev[l + 1]["A"].toarray()
