@@ -447,6 +447,42 @@ int  pic_learn_get_pair(int32_t k, float* r, float* e, double* ea);
 /* Summed loss and gradient (no update) over the ring pairs idx[0..npairs). */
 int  pic_learn_gradient(int32_t npairs, const int32_t* idx, double* loss, float* grad, int64_t n);
 
+/* ------------------------------------------- learned warm start (no reference counterpart)
+ * A convolutional encoder-decoder on the node grid (levels l = 0..L-1 of nodes (I, J) <->
+ * (2I, 2J), C channels, 3x3 convolutions, skips between the encoder and decoder levels, no
+ * biases and no nonlinearity, so it is linear in its input) maps the Jacobi-scaled residual
+ * of the solve's starting potential to a correction of it: for a cold start that residual is
+ * the right-hand side of the deposited charge, inside the time loop the change of the charge
+ * since the last solve. With solver_mode 2 and the warm start enabled, every IFE_Solve_2D
+ * adds the A-norm-optimal multiple of that correction to its starting potential before the
+ * learned flexible CG runs (never worse than no warm start in the A-norm). The C -> C
+ * convolutions and their weight gradients run on BF16 matrix cores with FP32 accumulation.
+ * It trains on the harvest ring's pairs that hold a solve's initial residual (the first pass
+ * of a plain solve, its first snapshot), so pic_learn_setup and pic_learn_harvest come first. */
+/* Build the network for the current domain: at most `levels` levels (2..8, down to about
+ * 5 x 5 nodes), channels 16 or 32, weights from the seed. Disabled until pic_warm_enable. */
+int  pic_warm_setup(int32_t levels, int32_t channels, uint64_t seed);
+/* Adam on the energy-norm error (y - e)^T A' (y - e) / (e^T A' e) over minibatches of
+ * initial-residual pairs; loss of the first and last minibatch (mean per pair). */
+int  pic_warm_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+int  pic_warm_enable(int32_t enable);
+/* Mean loss over the first npairs initial-residual pairs, of the raw output and of its
+ * A-norm-optimal multiple (the correction the solve applies). */
+int  pic_warm_eval(int32_t npairs, double* mean_loss, double* mean_loss_scaled);
+/* y = N(r) on host arrays of nnodes floats (node numbering). */
+int  pic_warm_apply(const float* r, float* y, int64_t n);
+int  pic_warm_get_params(float* dst, int64_t n, int64_t* nparams);
+int  pic_warm_set_params(const float* src, int64_t n);
+/* levels, H and W per level (8 entries), channels. Parameters (FP32, flat): lift [9][C],
+ * out [9][C], encoder E_1..E_{L-1} [9][C][C], decoder D_0..D_{L-2} [9][C][C] ([tap][in][out],
+ * tap t <-> node offset (t/3 - 1, t%3 - 1)). */
+int  pic_warm_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* channels);
+/* Summed loss and gradient (no update) over the ring pairs idx[0..npairs). */
+int  pic_warm_gradient(int32_t npairs, const int32_t* idx, double* loss, float* grad, int64_t n);
+/* Initial-residual pairs in the ring, whether the solve applies the warm start, alpha of the
+ * last applied correction, last training loss, corrections applied so far. */
+int  pic_warm_info(int32_t* initial_pairs, int32_t* enabled, double* last_alpha, double* last_loss, int64_t* applied);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,6 +160,16 @@ _PROTOS = {
     "pic_learn_set_pairs": (C.c_int, [C.c_int32, _fp, _fp]),
     "pic_learn_get_pair": (C.c_int, [C.c_int32, _fp, _fp, _dp]),
     "pic_learn_gradient": (C.c_int, [C.c_int32, _ip, _dp, _fp, C.c_int64]),
+    "pic_warm_setup": (C.c_int, [C.c_int32, C.c_int32, C.c_uint64]),
+    "pic_warm_train": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, _dp, _dp]),
+    "pic_warm_enable": (C.c_int, [C.c_int32]),
+    "pic_warm_eval": (C.c_int, [C.c_int32, _dp, _dp]),
+    "pic_warm_apply": (C.c_int, [_fp, _fp, C.c_int64]),
+    "pic_warm_get_params": (C.c_int, [_fp, C.c_int64, _i64p]),
+    "pic_warm_set_params": (C.c_int, [_fp, C.c_int64]),
+    "pic_warm_layout": (C.c_int, [_ip, _ip, _ip, _ip]),
+    "pic_warm_gradient": (C.c_int, [C.c_int32, _ip, _dp, _fp, C.c_int64]),
+    "pic_warm_info": (C.c_int, [_ip, _ip, _dp, _dp, _i64p]),
 }
 
 
@@ -673,6 +683,60 @@ class PicLibrary:
         loss = C.c_double()
         check(lib().pic_learn_gradient(len(idx), _i(idx), C.byref(loss), _f(g), nparams))
         return loss.value, g
+
+    # ------------------------------------------------------- learned warm start
+    def warm_setup(self, levels: int = 6, channels: int = 16, seed: int = 1) -> None:
+        """Encoder-decoder warm start of the solve (after learn_setup: it trains on that ring)."""
+        check(lib().pic_warm_setup(levels, channels, seed))
+
+    def warm_train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 1) -> tuple:
+        a, b = C.c_double(), C.c_double()
+        check(lib().pic_warm_train(steps, batch, lr, seed, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def warm_enable(self, enable: bool = True) -> None:
+        check(lib().pic_warm_enable(int(enable)))
+
+    def warm_eval(self, npairs: int) -> tuple:
+        a, b = C.c_double(), C.c_double()
+        check(lib().pic_warm_eval(npairs, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def warm_apply(self, r) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        y = np.zeros_like(r)
+        check(lib().pic_warm_apply(_f(r), _f(y), len(r)))
+        return y
+
+    def warm_params(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().pic_warm_get_params(None, 0, C.byref(n)))
+        p = np.zeros(n.value, dtype=np.float32)
+        check(lib().pic_warm_get_params(_f(p), len(p), C.byref(n)))
+        return p
+
+    def warm_set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().pic_warm_set_params(_f(p), len(p)))
+
+    def warm_layout(self) -> dict:
+        L = np.zeros(1, np.int32); H = np.zeros(8, np.int32); W = np.zeros(8, np.int32); Cc = np.zeros(1, np.int32)
+        check(lib().pic_warm_layout(_i(L), _i(H), _i(W), _i(Cc)))
+        n = int(L[0])
+        return {"levels": n, "H": H[:n].tolist(), "W": W[:n].tolist(), "C": int(Cc[0])}
+
+    def warm_gradient(self, idx, nparams: int) -> tuple:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        g = np.zeros(nparams, np.float32)
+        loss = C.c_double()
+        check(lib().pic_warm_gradient(len(idx), _i(idx), C.byref(loss), _f(g), nparams))
+        return loss.value, g
+
+    def warm_info(self) -> dict:
+        n, en, a, loss, ap = C.c_int32(), C.c_int32(), C.c_double(), C.c_double(), C.c_int64()
+        check(lib().pic_warm_info(C.byref(n), C.byref(en), C.byref(a), C.byref(loss), C.byref(ap)))
+        return {"initial_pairs": n.value, "enabled": bool(en.value), "last_alpha": a.value, "last_loss": loss.value,
+                "applied": ap.value}
 
     def kernel_stats_full(self) -> dict:
         a = np.zeros(14)

@@ -242,6 +242,7 @@ int pic_finalize(void) {
     if (!g.inited) return PIC_OK;
     hipDeviceSynchronize();
     learn_free();
+    warm_free();
     hipDeviceReset();
     // the reset unloads the code object: the scratch array gets a new address next time
     if (g.device >= 0 && g.device < PIC_MAX_DEVICES) g_scratch_dev[g.device] = nullptr;
@@ -271,6 +272,7 @@ int pic_setup_domain(const pic_domain_config* cfg) {
     // species, MCC tables, the learned network and the step counter belong to a domain's run
     HIPCHK(hipStreamSynchronize(g.stream));
     learn_free();
+    warm_free();
     for (int s = 0; s < PIC_MAX_SPECIES; ++s) free_species(s);
     g.nspecies = 0;
     g.step = 0;
@@ -1391,6 +1393,51 @@ int pic_learn_gradient(int32_t npairs, const int32_t* idx, double* loss, float* 
     REQUIRE_DOMAIN();
     if (!idx || !loss || !grad) return pic_fail(PIC_ERR_ARG, "NULL buffer");
     return learn_gradient(npairs, idx, loss, grad, n);
+}
+
+int pic_warm_setup(int32_t levels, int32_t channels, uint64_t seed) {
+    REQUIRE_DOMAIN();
+    return warm_setup(levels, channels, seed);
+}
+int pic_warm_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+    REQUIRE_DOMAIN();
+    return warm_train(steps, batch, lr, seed, loss_first, loss_last);
+}
+int pic_warm_enable(int32_t enable) {
+    REQUIRE_DOMAIN();
+    return warm_enable(enable);
+}
+int pic_warm_eval(int32_t npairs, double* mean_loss, double* mean_loss_scaled) {
+    REQUIRE_DOMAIN();
+    if (!mean_loss) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return warm_eval(npairs, mean_loss, mean_loss_scaled);
+}
+int pic_warm_apply(const float* r, float* y, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!r || !y) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return warm_apply_host(r, y, n);
+}
+int pic_warm_get_params(float* dst, int64_t n, int64_t* nparams) {
+    REQUIRE_DOMAIN();
+    return warm_params(dst, nullptr, n, nparams);
+}
+int pic_warm_set_params(const float* src, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!src) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return warm_params(nullptr, src, n, nullptr);
+}
+int pic_warm_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* channels) {
+    REQUIRE_DOMAIN();
+    if (!levels || !H || !W || !channels) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return warm_layout(levels, H, W, channels);
+}
+int pic_warm_gradient(int32_t npairs, const int32_t* idx, double* loss, float* grad, int64_t n) {
+    REQUIRE_DOMAIN();
+    if (!idx || !loss || !grad) return pic_fail(PIC_ERR_ARG, "NULL buffer");
+    return warm_gradient(npairs, idx, loss, grad, n);
+}
+int pic_warm_info(int32_t* initial_pairs, int32_t* enabled, double* last_alpha, double* last_loss, int64_t* applied) {
+    return warm_info(initial_pairs, enabled, last_alpha, last_loss, applied);
 }
 
 }  // extern "C"

@@ -637,6 +637,11 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
     for (;;) {
         const auto th0 = std::chrono::steady_clock::now();
         HIPCHK(hipEventRecord(g.ev[12], g.stream));
+        if (first && warm_active()) {
+            // warm start: xn += alpha D^-1/2 N(D^-1/2 (b - A xn)), then the passes start from it
+            resid64_launch(n, A64, g.d_bn, g.d_xn, g.d_rn);
+            int rc = warm_solve_start(n, g.d_rn, g.d_xn); if (rc) return rc;
+        }
         resid64_launch(n, A64, g.d_bn, g.d_xn, g.d_rn);
         int K[FCG_MAX_PASS];
         for (int p = 0; p < batch; ++p) {
@@ -727,6 +732,7 @@ int run_pcg_mixed() {
             return pic_fail(PIC_ERR_PCG_MAXITER, "mixed-precision PCG did not reach the tolerance");
         }
         int it = 0, ierr = 0;
+        learn_set_solve_pass(outer);                 // the harvest tags a solve's initial residual
         rc = cg1_run(n, g.d_rn, inner_tol, g.pcg_maxit - total, &it, &ierr);
         if (rc) return rc;
         hipLaunchKernelGGL(k_cg_apply, dim3(nb256), dim3(256), 0, g.stream, n, g.d_sinv, g.d_cg_x, g.d_xn);

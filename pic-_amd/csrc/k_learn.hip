@@ -1198,6 +1198,11 @@ struct LearnNet {
     int harvest_every = 1;                           // snapshot every k-th CG chunk
     float* d_snap = nullptr; int snap_cap = 0, nsnap = 0;
     size_t snap_len = 0, snap_pad = 0;
+    // per ring slot: 1 when the pair holds the initial residual of a solve (first refinement
+    // pass, chunk 0: e is the whole correction of the solve); per snapshot its chunk
+    std::vector<int> ds_initial;
+    int snap_chunk[64] = {0};
+    int solve_pass = 0;                              // refinement pass of the running inner CG
     double* d_hpart = nullptr;
     // FCG work vectors (padded node space)
     void* d_fvec = nullptr; float *z = nullptr, *w = nullptr, *q = nullptr;
@@ -1566,6 +1571,7 @@ int learn_set_harvest(int enable, int capacity, int every) {
         HIPCHK(hipMalloc(&ln.ds_e, N * capacity * sizeof(float)));
         HIPCHK(hipMalloc(&ln.ds_ea, capacity * sizeof(double)));
         ln.ds_cap = capacity; ln.ds_n = 0; ln.ds_head = 0;
+        ln.ds_initial.assign(capacity, 0);
     }
     if (!ln.d_snap) {
         ln.snap_cap = 64;
@@ -1586,6 +1592,7 @@ int learn_snapshot(const float* x, int chunk) {
     if (chunk % ln.harvest_every != 0 || ln.nsnap >= ln.snap_cap) return PIC_OK;
     float* dst = ln.d_snap + ln.nsnap * ln.snap_len;
     HIPCHK(hipMemcpyAsync(dst, x - ln.snap_pad, ln.snap_len * sizeof(float), hipMemcpyDeviceToDevice, g.stream));
+    ln.snap_chunk[ln.nsnap] = chunk;
     ln.nsnap++;
     return PIC_OK;
 }
@@ -1608,6 +1615,7 @@ int learn_harvest_finish(const float* xfinal) {
         HIPCHK(hipMemcpyAsync(&ea, ln.ds_ea + slot, sizeof(double), hipMemcpyDeviceToHost, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
         if (!(ea > 0.0)) continue;                      // converged snapshot: no information
+        ln.ds_initial[slot] = (ln.solve_pass == 0 && ln.snap_chunk[s] == 0) ? 1 : 0;
         ln.ds_head = (ln.ds_head + 1) % ln.ds_cap;
         ln.ds_n = std::min(ln.ds_n + 1, ln.ds_cap);
     }
@@ -1854,6 +1862,7 @@ int learn_set_pairs(int npairs, const float* r, const float* e) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(g.stream));
     ln.ds_n = npairs; ln.ds_head = npairs % ln.ds_cap;
+    for (int k = 0; k < ln.ds_cap; ++k) ln.ds_initial[k] = k < npairs ? 1 : 0;   // given pairs: solves' initial residuals
     return PIC_OK;
 }
 
@@ -1890,3 +1899,66 @@ int learn_info(int32_t* dataset, int32_t* last_iters, int64_t* fallbacks, double
 }
 void learn_count_fallback() { ln.fallbacks++; }
 void learn_free() { free_learn(); }
+
+// --------------------------------------------------------------- access for the warm start
+// (k_warm.hip trains on the same ring and minibatch buffers)
+void learn_set_solve_pass(int pass) { ln.solve_pass = pass; }
+
+void learn_ring_initial(std::vector<int>& idx) {
+    idx.clear();
+    for (int k = 0; k < ln.ds_n; ++k) if (ln.ds_initial[k]) idx.push_back(k);
+}
+
+Dia8 learn_scaled_op() { return scaled_op(); }
+
+// pairs idx[0..batch) of the ring into the minibatch buffers
+int learn_batch_pairs(int batch, const int* idx, LearnBatch* out) {
+    if (!ln.ready) return pic_fail(PIC_ERR_STATE, "pic_learn_setup not called");
+    for (int k = 0; k < batch; ++k) if (idx[k] < 0 || idx[k] >= ln.ds_n) return pic_fail(PIC_ERR_ARG, "pair index out of range");
+    int rc = ensure_batch(batch); if (rc) return rc;
+    rc = gather_batch(batch, batch, idx); if (rc) return rc;
+    out->r = ln.rb; out->e = ln.eb; out->y = ln.y0; out->gy = ln.gl;
+    return PIC_OK;
+}
+
+// energy-norm loss of the minibatch outputs y against e and its gradient into gy; the summed
+// loss is read back when loss != nullptr
+int learn_batch_loss(int batch, double* loss) {
+    const int n = (int)ln_npix0();
+    const int nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_loss_grad, dim3(nb), dim3(256), 0, g.stream, n, batch, scaled_op(), g.d_sinv, ln.y0, ln.eb,
+                       ln.d_eab, ln.gl, ln.d_lpart);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.stream, nb, ln.d_lpart, ln.d_lsum);
+    HIPCHK(hipGetLastError());
+    if (loss) {
+        HIPCHK(hipMemcpyAsync(loss, ln.d_lsum, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
+    return PIC_OK;
+}
+
+// loss of minibatch entry 0 (output y) and the loss of its A'-optimal multiple alpha y:
+// with L e'Ae = y'Ay - 2 y'r + e'r, the scaled loss is 1 - (y'r)^2 / (y'Ay e'r)
+int learn_batch_loss_scaled(const float* y, double* loss, double* scaled) {
+    const int n = (int)ln_npix0();
+    const int nb = (n + 255) / 256;
+    int rc = learn_batch_loss(1, loss); if (rc) return rc;
+    hipLaunchKernelGGL(k_dot_f, dim3(nb), dim3(256), 0, g.stream, n, y, ln.rb, ln.d_hpart);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, g.stream, nb, ln.d_hpart, ln.d_lsum + 1);
+    HIPCHK(hipGetLastError());
+    double h[2] = {0.0, 0.0};
+    HIPCHK(hipMemcpyAsync(h, ln.d_lsum + 1, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(h + 1, ln.d_eab, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    const double yr = h[0], ea = h[1];
+    const double yay = *loss * ea + 2.0 * yr - ea;
+    *scaled = (yay > 0.0 && ea > 0.0) ? 1.0 - yr * yr / (yay * ea) : *loss;
+    return PIC_OK;
+}
+
+void learn_adam_step(int64_t n, float* p, const float* gr, float* m, float* v, float lr, int t, float gscale) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)t), c2 = 1.0f - std::pow(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g.stream, n, p, gr, m, v, lr, b1, b2, c1, c2,
+                       1e-8f, gscale);
+}

@@ -347,6 +347,29 @@ int learn_gradient(int np, const int32_t* idx, double* loss, float* grad, int64_
 int learn_info(int32_t* dataset, int32_t* last_iters, int64_t* fallbacks, double* last_loss);
 void learn_count_fallback();
 void learn_free();
+// access for the warm-start network (k_warm.hip)
+struct LearnBatch { const float* r; const float* e; float* y; float* gy; };
+void learn_set_solve_pass(int pass);
+void learn_ring_initial(std::vector<int>& idx);
+struct Dia8;
+Dia8 learn_scaled_op();
+int learn_batch_pairs(int batch, const int* idx, LearnBatch* out);
+int learn_batch_loss(int batch, double* loss);
+int learn_batch_loss_scaled(const float* y, double* loss, double* scaled);
+void learn_adam_step(int64_t n, float* p, const float* gr, float* m, float* v, float lr, int t, float gscale);
+// warm-start network (k_warm.hip)
+int warm_setup(int levels, int channels, uint64_t seed);
+bool warm_active();
+int warm_enable(int enable);
+int warm_solve_start(int n, const double* rn, double* xn);
+int warm_train(int steps, int batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+int warm_eval(int npairs, double* mean_loss, double* mean_scaled);
+int warm_apply_host(const float* r, float* y, int64_t n);
+int warm_params(float* dst, const float* src, int64_t n, int64_t* nparams);
+int warm_layout(int32_t* levels, int32_t* H, int32_t* W, int32_t* channels);
+int warm_gradient(int np, const int32_t* idx, double* loss, float* grad, int64_t n);
+int warm_info(int32_t* initial_pairs, int32_t* enabled, double* last_alpha, double* last_loss, int64_t* applied);
+void warm_free();
 int pcg_generic_fp64(int n, const int* rowptr, const int* col, const double* val, const double* diag,
                      const double* b, double* x, double tol, int maxit, int* iters, int* ierr,
                      double* r_out);
