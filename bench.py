@@ -193,29 +193,40 @@ def cpu_baseline(pkg, pl, w, sample_particles: int, phi_prev) -> dict:
                       f"not a target: the GPU's roofline fractions measure the kernels"}
 
 
-def solve_compare(pl, reps: int) -> dict:
+def solve_compare(pl, reps: int, warm: bool = False) -> dict:
     """Plain (mode 1) vs learned (mode 2) inner solver on the same RHS and warm start: the
     charge density deposited in one step and the potential of the step before it (what the
-    solve inside a step sees)."""
+    solve inside a step sees). warm: also the learned solver with the learned warm start
+    ("learned_warm"); ms_per_solve is the host wall time of the call, device_ms the solve's
+    HIP-event time."""
     import time as _t
     pl.set_solver(1)
     phi0 = pl.field("PHI")
     pl.step(1)
     rho = pl.field("RHO")
     res, phis = {}, {}
-    for name, mode in (("plain", 1), ("learned", 2)):
+    was_on = pl.warm_info()["enabled"] if warm else False
+    legs = [("plain", 1, False), ("learned", 2, False)] + ([("learned_warm", 2, True)] if warm else [])
+    for name, mode, wm_on in legs:
         pl.set_solver(mode)
-        times, its = [], []
+        if warm:
+            pl.warm_enable(wm_on)
+        times, its, dev = [], [], []
         for _ in range(reps):
             phi = phi0.copy()
             t0 = _t.perf_counter()
             pl.IFE_Solve_2D(Rho=rho, Phi=phi)
             times.append(_t.perf_counter() - t0)
+            dev.append(pl.last_solve_ms())
             info = pl.last_solve()
             its.append(info["iterations"])
-        res[name] = {"ms_per_solve": 1e3 * float(np.median(times)), "iterations": int(np.median(its)),
-                     "relres": info["relres"], "outer": info["outer"]}
+        res[name] = {"ms_per_solve": 1e3 * float(np.median(times)), "device_ms": float(np.median(dev)),
+                     "iterations": int(np.median(its)), "relres": info["relres"], "outer": info["outer"]}
         phis[name] = phi
+    if warm:
+        pl.warm_enable(was_on)
+        res["phi_rel_l2_learned_warm_vs_plain"] = float(np.linalg.norm(phis["learned_warm"] - phis["plain"]) /
+                                                        np.linalg.norm(phis["plain"]))
     pl.set_solver(1)
     res["phi_rel_l2_learned_vs_plain"] = float(np.linalg.norm(phis["learned"] - phis["plain"]) /
                                                np.linalg.norm(phis["plain"]))
@@ -466,7 +477,53 @@ def cfg4_protocol(pkg, pl, args) -> dict:
                        "ms_ratio_untrained_over_trained": un["ms_per_solve_median"] / tr["ms_per_solve_median"]})
             variants[f"L{Ls}_share{share}"] = tr
             log(f"cfg4: L{Ls} share {share}: {tr}")
+            if Ls == 2 and share == 0.25:
+                p_product = pl.learn_params().copy()   # the product's V-cycle configuration
         variants[f"L{Ls}_fallbacks"] = pl.learn_info()["fallbacks"] - fb0
+    # the learned warm start on top of the product's V-cycle (2 learned levels, share 1/4): trained
+    # on the initial residuals of plain cold solves of the charge of a run with the product seed
+    nw = args.cfg4_warm_pairs
+    log(f"cfg4: warm start: {nw} plain cold solves for training pairs")
+    w = pkg.build_workload(pl, "cfg3", seed=2024)
+    pl.sort_particles()
+    pl.initial_field()
+    pl.learn_setup(12, (16, 16), seed=7)
+    cap = 4 * nw
+    t0 = _t.perf_counter()
+    for _ in range(nw):
+        pl.learn_harvest(False, cap, 1000)
+        pl.step(1)
+        rho = pl.field("RHO")
+        pl.learn_harvest(True, cap, 1000)
+        phi = phi0.copy()
+        pl.IFE_Solve_2D(Rho=rho, Phi=phi)
+    pl.learn_harvest(False, cap, 1000)
+    t_harvest = _t.perf_counter() - t0
+    pl.learn_set_params(p_product)
+    pl.warm_setup(6, 16, seed=3)
+    ninit = pl.warm_info()["initial_pairs"]
+    l0, l0s = pl.warm_eval(min(ninit, 256))
+    t0 = _t.perf_counter()
+    lf, ll = pl.warm_train(args.cfg4_warm_train, batch=8, lr=2e-3, seed=5)
+    t_train = _t.perf_counter() - t0
+    l1, l1s = pl.warm_eval(min(ninit, 256))
+    base, _, it_base = solve_all(2, ref)
+    pl.warm_enable(True)
+    wr, _, it_w = solve_all(2, ref)
+    pl.warm_enable(False)
+    p_w = pl.warm_params()
+    wr.update({"training_pairs": ninit, "harvest_s": t_harvest, "train_steps": args.cfg4_warm_train, "train_batch": 8,
+               "train_s": t_train, "parameters": int(p_w.size), "levels": pl.warm_layout()["H"], "channels": 16,
+               "loss_train_pairs_before": l0, "loss_train_pairs_after": l1,
+               "loss_after_line_search_before": l0s, "loss_after_line_search_after": l1s,
+               "loss_first_batch": lf, "loss_last_batch": ll,
+               "fields_fewer_iterations_than_without": int((it_w < it_base).sum()),
+               "fields_more_iterations_than_without": int((it_w > it_base).sum()),
+               "iteration_ratio_without_over_with": float(it_base.mean() / it_w.mean()),
+               "ms_ratio_without_over_with": base["ms_per_solve_median"] / wr["ms_per_solve_median"]})
+    variants["L2_share0.25_rerun"] = base
+    variants["L2_share0.25_warm_start"] = wr
+    log(f"cfg4: warm start: {wr}")
     best = min((k for k in variants if "share" in k), key=lambda k: variants[k]["ms_per_solve_mean"])
     return {"workload": "512x512 three-conductor mesh, stand-alone Poisson, cold start",
             "heldout_fields": len(held), "heldout_source": "every step of an independent run (seed 77)",
@@ -484,6 +541,10 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--solve-reps", type=int, default=5)
+    ap.add_argument("--no-warm-net", action="store_true", help="do not train the learned warm start")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="apply the learned warm start inside the timed steps (default: compared on one RHS only)")
+    ap.add_argument("--warm-train-steps", type=int, default=300)
     ap.add_argument("--no-fp64-error", action="store_true", help="skip the FP64 comparison step (it ends the run)")
     ap.add_argument("--finetune-every", type=int, default=None,
                     help="online fine-tuning period in steps (default: 500 for cfg5, off otherwise)")
@@ -492,6 +553,8 @@ def main() -> int:
     ap.add_argument("--cfg4-pairs", type=int, default=20000, help="cfg4: harvested training pairs")
     ap.add_argument("--cfg4-max-steps", type=int, default=2000, help="cfg4: at most this many harvest steps")
     ap.add_argument("--cfg4-train", type=int, default=4000, help="cfg4: training steps")
+    ap.add_argument("--cfg4-warm-pairs", type=int, default=400, help="cfg4: plain cold solves harvested for the warm start")
+    ap.add_argument("--cfg4-warm-train", type=int, default=2000, help="cfg4: warm-start training steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy 5 times (scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -548,6 +611,17 @@ def main() -> int:
         t0 = time.perf_counter()
         l0, l1 = pl.learn_train(args.train_steps, batch=8, lr=5e-3, seed=11)
         t_train = time.perf_counter() - t0
+        warm = None
+        if not args.no_warm_net:
+            # the learned warm start trains on the initial residuals of the warm-up solves
+            pl.warm_setup(6, 16, seed=3)
+            nwi = pl.warm_info()["initial_pairs"]
+            t0 = time.perf_counter()
+            wl0, wl1 = pl.warm_train(args.warm_train_steps, batch=8, lr=2e-3, seed=5)
+            warm = {"initial_pairs": nwi, "train_steps": args.warm_train_steps, "train_s": time.perf_counter() - t0,
+                    "loss_first_batch": wl0, "loss_last_batch": wl1, "in_time_loop": bool(args.warm_start)}
+            pl.warm_enable(args.warm_start)
+            log(f"bench: warm start trained: {warm}")
         pl.set_solver(2)
         if n_learned_warm:
             pl.step(n_learned_warm)      # the last warm-up steps run the learned solver
@@ -603,7 +677,7 @@ def main() -> int:
                                         dom.delta),
     }
     if learned:
-        cmp_ = solve_compare(pl, args.solve_reps)
+        cmp_ = solve_compare(pl, args.solve_reps, warm=warm is not None)
         log(f"bench: solve comparison {cmp_}")
         out["plain"] = {"value": plain_run["value"], "ms_per_step": plain_run["ms_per_step"],
                         "pcg_ms_per_solve": plain_run["phase"][3] / args.steps,
@@ -618,6 +692,13 @@ def main() -> int:
                           "step_speedup": plain_run["ms_per_step"] / main_run["ms_per_step"]}
         if ft:
             out["learned"]["online_finetune"] = ft
+        if warm is not None:
+            w_ = dict(warm)
+            lw, ln_ = cmp_["learned_warm"], cmp_["learned"]
+            w_.update({"iterations_with": lw["iterations"], "iterations_without": ln_["iterations"],
+                       "device_ms_with": lw["device_ms"], "device_ms_without": ln_["device_ms"],
+                       "applied": pl.warm_info()["applied"]})
+            out["learned"]["warm_start"] = w_
     if not args.no_cpu_baseline:
         log("bench: CPU baseline (oracle, 1 core)")
         try:

@@ -465,6 +465,8 @@ int  pic_warm_setup(int32_t levels, int32_t channels, uint64_t seed);
 /* Adam on the energy-norm error (y - e)^T A' (y - e) / (e^T A' e) over minibatches of
  * initial-residual pairs; loss of the first and last minibatch (mean per pair). */
 int  pic_warm_train(int32_t steps, int32_t batch, double lr, uint64_t seed, double* loss_first, double* loss_last);
+/* 0 off; 1 on, the solve replays the application as one captured HIP graph (captured again
+ * when its buffers move); 2 on, the same kernels launched one by one. */
 int  pic_warm_enable(int32_t enable);
 /* Mean loss over the first npairs initial-residual pairs, of the raw output and of its
  * A-norm-optimal multiple (the correction the solve applies). */

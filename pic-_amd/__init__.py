@@ -694,7 +694,8 @@ class PicLibrary:
         check(lib().pic_warm_train(steps, batch, lr, seed, C.byref(a), C.byref(b)))
         return a.value, b.value
 
-    def warm_enable(self, enable: bool = True) -> None:
+    def warm_enable(self, enable=True) -> None:
+        """False/0 off; True/1 on (one captured HIP graph per application); 2 on, eager launches."""
         check(lib().pic_warm_enable(int(enable)))
 
     def warm_eval(self, npairs: int) -> tuple:

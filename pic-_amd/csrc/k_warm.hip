@@ -136,9 +136,9 @@ __device__ inline void ws_input8(const WsIn& in, int b, int i, int j, int gi, fl
 
 // Stage the input on the nodes [i0 - 1, i0 + TI] x [j0 - 1, j0 + TJ] (zero outside the grid)
 // into xs[node][group] (16 B per entry).
-template <int C, int MODE, int TJ>
+template <int C, int MODE, int TI, int TJ>
 __device__ inline void ws_stage(const WsIn& in, int b, int i0, int j0, ws_u32x4* xs) {
-    constexpr int G = C / 8, HI = WS_TI + 2, HJ = TJ + 2;
+    constexpr int G = C / 8, HI = TI + 2, HJ = TJ + 2;
     for (int k = threadIdx.x; k < HI * HJ * G; k += 256) {
         const int node = k / G, gi = k - node * G;
         const int u = node / HJ, v = node - u * HJ;
@@ -160,26 +160,32 @@ __device__ inline void ws_stage(const WsIn& in, int b, int i0, int j0, ws_u32x4*
 // out = mask . conv(x; W) on the (H, W) grid, C -> C channels, batch entry blockIdx.z.
 // wp: packed BF16 weights [MT][S][64 lanes] of 8 values: lane l of tile mt, step s holds
 // W[tap][ci0 .. ci0 + 7][16 mt + l % 16] with 32 s + 8 (l / 16) = C tap + ci0 (zero for tap 9).
+// A workgroup owns WS_CTI x WS_CTJ = 4 x 16 nodes, one 16-node row segment per wave: every
+// level is a latency-bound pass (its input was written by the previous launch), so the tile
+// is small enough that the input (6 x 18 nodes) is staged in one round of loads per thread
+// and the grid has many workgroups even on the coarse levels.
+#define WS_CTI 4
+#define WS_CTJ 16
 template <int C, int MODE>
 __global__ __launch_bounds__(256) void k_ws_conv(WsIn in, const ws_u32x4* __restrict__ wp,
                                                  const float* __restrict__ mask, uint16_t* __restrict__ out,
                                                  int64_t out_bs) {
     using Sh = WsShape<C>;
-    constexpr int TJ = Sh::TJ, G = Sh::G, S = Sh::S, MT = Sh::MT, HJ = TJ + 2;
-    __shared__ ws_u32x4 xs[(WS_TI + 2) * HJ * G];
+    constexpr int TI = WS_CTI, TJ = WS_CTJ, G = Sh::G, S = Sh::S, MT = Sh::MT, HJ = TJ + 2;
+    __shared__ ws_u32x4 xs[(TI + 2) * HJ * G];
     const int b = blockIdx.z;
-    const int i0 = blockIdx.y * WS_TI, j0 = blockIdx.x * TJ;
+    const int i0 = blockIdx.y * TI, j0 = blockIdx.x * TJ;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     ws_u32x4 wr[MT][S];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int s = 0; s < S; ++s) wr[mt][s] = wp[(mt * S + s) * 64 + lane];
-    ws_stage<C, MODE, TJ>(in, b, i0, j0, xs);
+    ws_stage<C, MODE, TI, TJ>(in, b, i0, j0, xs);
     __syncthreads();
     const int q = lane >> 4, pc = lane & 15;
     constexpr int SEGR = TJ / 16;                       // 16-node segments per tile row
-    for (int seg = wid; seg < WS_TI * SEGR; seg += 4) {
+    for (int seg = wid; seg < TI * SEGR; seg += 4) {
         const int ri = seg / SEGR, cj = (seg - ri * SEGR) * 16 + pc;
         ws_f32x4 acc[MT];
 #pragma unroll
@@ -239,7 +245,7 @@ __global__ __launch_bounds__(256) void k_ws_wgrad(WsIn in, const uint16_t* __res
         const int b = tile / (ntj * nti), rem = tile - b * ntj * nti;
         const int i0 = (rem / ntj) * WS_TI, j0 = (rem - (rem / ntj) * ntj) * TJ;
         __syncthreads();                                 // the previous tile's reads are done
-        ws_stage<C, MODE, TJ>(in, b, i0, j0, xs);
+        ws_stage<C, MODE, WS_TI, TJ>(in, b, i0, j0, xs);
         for (int k = threadIdx.x; k < WS_TI * TJ * G; k += 256) {
             const int node = k / G, gi = k - node * G;
             const int u = node / TJ, v = node - u * TJ;
@@ -285,50 +291,62 @@ __global__ __launch_bounds__(256) void k_ws_wgrad(WsIn in, const uint16_t* __res
                 for (int r = 0; r < 4; ++r) pw[t * C * C + (16 * mt + 4 * q + r) * C + 16 * nt + ch] = acc[t][mt][nt][r];
 }
 
-// out[k] = sum over nparts of part[p][k] (FP64, p in order; 4 part-lanes combined in order)
-__global__ __launch_bounds__(256) void k_ws_reduce(int nparts, int K, const float* __restrict__ part,
-                                                   float* __restrict__ out) {
+// Fixed-order sums of partials: out[y][k] = sum of part[p][k] over the parts p of chunk y
+// (chunk y: parts [y * chunk, min((y + 1) * chunk, nparts)), 4 part-lanes of a block summed in
+// FP64 and combined in order). One pass with one chunk gives the final sums (FP32); a first
+// pass over many chunks writes FP64 chunk sums for the second.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_ws_reduce(int nparts, int chunk, int K, const TI* __restrict__ part,
+                                                   TO* __restrict__ out) {
     __shared__ double sh[4][64];
     const int kk = blockIdx.x * 64 + (threadIdx.x & 63), pl = threadIdx.x >> 6;
+    const int p0 = blockIdx.y * chunk, p1 = min(nparts, p0 + chunk);
     double s = 0.0;
     if (kk < K)
-        for (int p = pl; p < nparts; p += 4) s += (double)part[(int64_t)p * K + kk];
+        for (int p = p0 + pl; p < p1; p += 4) s += (double)part[(int64_t)p * K + kk];
     sh[pl][threadIdx.x & 63] = s;
     __syncthreads();
-    if (pl == 0 && kk < K) out[kk] = (float)(((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x]);
+    if (pl == 0 && kk < K)
+        out[(int64_t)blockIdx.y * K + kk] =
+            (TO)(((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x]);
 }
 
 // 1 -> C lift: out(p)[c] = mask(p) sum_tap w[tap'][c] s(p + d_tap), tap' = flip ? 8 - tap : tap
-// (flip: the adjoint of the C -> 1 output convolution). One thread per node and 8 channels.
+// (flip: the adjoint of the C -> 1 output convolution). One thread per node and all C
+// channels: the weight reads are the same for every lane (scalar loads).
 template <int C>
 __global__ __launch_bounds__(256) void k_ws_lift(int H, int W, const float* __restrict__ s, int64_t s_bs,
                                                  const float* __restrict__ w, int flip, const float* __restrict__ mask,
                                                  uint16_t* __restrict__ out, int64_t o_bs) {
     constexpr int G = C / 8;
     const int b = blockIdx.y;
-    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t n = (int64_t)H * W;
-    if (k >= n * G) return;
-    const int64_t p = k / G;
-    const int gi = (int)(k - p * G);
+    if (p >= n) return;
     const int i = (int)(p / W), j = (int)(p - (int64_t)(p / W) * W);
     const float* sb = s + (int64_t)b * s_bs;
-    float f[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) f[c] = 0.0f;
+    float sv[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
         const int ii = i + t / 3 - 1, jj = j + t % 3 - 1;
-        if (ii < 0 || ii >= H || jj < 0 || jj >= W) continue;
-        const float sv = sb[(int64_t)ii * W + jj];
-        const int tt = flip ? 8 - t : t;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) f[c] = f[c] + w[tt * C + 8 * gi + c] * sv;
+        sv[t] = (ii < 0 || ii >= H || jj < 0 || jj >= W) ? 0.0f : sb[(int64_t)ii * W + jj];
     }
     const float m = mask[p];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) f[c] = f[c] * m;
-    *(ws_u32x4*)(out + (int64_t)b * o_bs + p * C + 8 * gi) = ws_pack8(f);
+    for (int gi = 0; gi < G; ++gi) {
+        float f[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) f[c] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int tt = flip ? 8 - t : t;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) f[c] = f[c] + w[tt * C + 8 * gi + c] * sv[t];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) f[c] = f[c] * m;
+        *(ws_u32x4*)(out + (int64_t)b * o_bs + p * C + 8 * gi) = ws_pack8(f);
+    }
 }
 
 // C -> 1 output: y(p) = mask(p) sum_tap sum_c w[tap][c] v(p + d_tap)[c], FP32
@@ -360,7 +378,9 @@ __global__ __launch_bounds__(256) void k_ws_out(int H, int W, const uint16_t* __
 
 // Weight gradients of the 1-channel convolutions, summed over the batch:
 //   LIFT: dW[tap][c] = sum s(p + d_tap) v(p)[c]     OUT: dW[tap][c] = sum v(p + d_tap)[c] s(p)
-// Threads: c = tid % C, node lane tid / C; per block 9 C partial sums (fixed order).
+// A block owns WS_CORR_NODES consecutive nodes of the batch; threads: c = tid % C, node lane
+// tid / C; per block 9 C partial sums (fixed order).
+#define WS_CORR_NODES 1024
 template <int C, bool LIFT>
 __global__ __launch_bounds__(256) void k_ws_corr(int H, int W, int batch, const float* __restrict__ s, int64_t s_bs,
                                                  const uint16_t* __restrict__ v, int64_t v_bs, float* __restrict__ part) {
@@ -368,10 +388,11 @@ __global__ __launch_bounds__(256) void k_ws_corr(int H, int W, int batch, const 
     constexpr int NL = 256 / C;
     const int c = threadIdx.x % C, nl = threadIdx.x / C;
     const int64_t n = (int64_t)H * W, tot = n * batch;
+    const int64_t P1 = min(tot, (int64_t)(blockIdx.x + 1) * WS_CORR_NODES);
     float acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = 0.0f;
-    for (int64_t P = (int64_t)blockIdx.x * NL + nl; P < tot; P += (int64_t)gridDim.x * NL) {
+    for (int64_t P = (int64_t)blockIdx.x * WS_CORR_NODES + nl; P < P1; P += NL) {
         const int b = (int)(P / n);
         const int64_t p = P - (int64_t)b * n;
         const int i = (int)(p / W), j = (int)(p - (int64_t)(p / W) * W);
@@ -526,16 +547,29 @@ struct WarmNet {
     uint16_t *gb[WS_MAX_LEVELS] = {nullptr}, *ub[WS_MAX_LEVELS] = {nullptr}, *hb[WS_MAX_LEVELS] = {nullptr},
              *tb[WS_MAX_LEVELS] = {nullptr};
     float* d_part = nullptr; int64_t part_cap = 0;      // floats
+    double* d_part2 = nullptr;                          // FP64 chunk sums of the partials
     float *d_in = nullptr, *d_y = nullptr;              // solve: r' and y (one node vector each)
     double *d_dots = nullptr, *d_alpha = nullptr;
     double last_alpha = 0.0, last_loss = 0.0;
     int64_t applied = 0;
+    // the solve's application as one HIP graph (scale, network, line search), captured for the
+    // buffers it was recorded with and captured again when any of them moves
+    hipGraph_t graph = nullptr; hipGraphExec_t graph_exec = nullptr;
+    const double* graph_rn = nullptr; double* graph_xn = nullptr; void* graph_act = nullptr; int graph_n = 0;
+    int mode = 0;                                       // 0 off, 1 graph, 2 eager launches
 };
 static WarmNet wn;
 
+static void ws_drop_graph() {
+    if (wn.graph_exec) (void)hipGraphExecDestroy(wn.graph_exec);
+    if (wn.graph) (void)hipGraphDestroy(wn.graph);
+    wn.graph_exec = nullptr; wn.graph = nullptr;
+}
+
 static void ws_free() {
-    void* ptrs[] = {wn.d_mask, wn.d_p, wn.d_m, wn.d_v, wn.d_g, wn.d_wp, wn.d_act, wn.d_part, wn.d_in, wn.d_y, wn.d_dots,
-                    wn.d_alpha};
+    ws_drop_graph();
+    void* ptrs[] = {wn.d_mask, wn.d_p, wn.d_m, wn.d_v, wn.d_g, wn.d_wp, wn.d_act, wn.d_part, wn.d_part2, wn.d_in, wn.d_y,
+                    wn.d_dots, wn.d_alpha};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     wn = WarmNet();
 }
@@ -559,8 +593,7 @@ static int ws_pack() { return wn.C == 16 ? ws_pack_t<16>() : ws_pack_t<32>(); }
 
 template <int C, int MODE>
 static void ws_conv_launch(const WsIn& in, int batch, const ws_u32x4* w, const float* mask, uint16_t* out, int64_t obs) {
-    constexpr int TJ = WsShape<C>::TJ;
-    dim3 grid((unsigned)((in.W + TJ - 1) / TJ), (unsigned)((in.H + WS_TI - 1) / WS_TI), batch);
+    dim3 grid((unsigned)((in.W + WS_CTJ - 1) / WS_CTJ), (unsigned)((in.H + WS_CTI - 1) / WS_CTI), batch);
     hipLaunchKernelGGL((k_ws_conv<C, MODE>), grid, dim3(256), 0, g.stream, in, w, mask, out, obs);
 }
 
@@ -579,7 +612,7 @@ template <int C>
 static int ws_forward_t(int batch, const float* r, int64_t rbs, float* y, int64_t ybs) {
     const int L = wn.L;
     const int64_t n0 = wn.n[0];
-    hipLaunchKernelGGL(k_ws_lift<C>, dim3((unsigned)((n0 * (C / 8) + 255) / 256), batch), dim3(256), 0, g.stream,
+    hipLaunchKernelGGL(k_ws_lift<C>, dim3((unsigned)((n0 + 255) / 256), batch), dim3(256), 0, g.stream,
                        wn.H[0], wn.W[0], r, rbs, wn.d_p + wn.off_lift, 0, wn.mask[0], wn.h[0], act_bs(0));
     for (int l = 1; l < L; ++l)
         ws_conv_launch<C, WS_RESTRICT>(ws_in(l, l - 1, wn.h[l - 1], act_bs(l - 1), nullptr, 0, 1.0f), batch,
@@ -595,6 +628,17 @@ static int ws_forward_t(int batch, const float* r, int64_t rbs, float* y, int64_
 
 static int ws_blocks_for(int64_t work) { return (int)std::max<int64_t>(1, std::min<int64_t>(work, 512)); }
 
+// grad[k] = sum of the nparts partial sums part[p][k] (two fixed-order passes: chunks of 64
+// parts to FP64 chunk sums, then the chunk sums)
+static void ws_reduce_launch(int nparts, int K, float* grad) {
+    const int chunk = 64, nch = (nparts + chunk - 1) / chunk;
+    const unsigned kb = (unsigned)((K + 63) / 64);
+    hipLaunchKernelGGL((k_ws_reduce<float, double>), dim3(kb, nch), dim3(256), 0, g.stream, nparts, chunk, K,
+                       (const float*)wn.d_part, wn.d_part2);
+    hipLaunchKernelGGL((k_ws_reduce<double, float>), dim3(kb, 1), dim3(256), 0, g.stream, nch, nch, K,
+                       (const double*)wn.d_part2, grad);
+}
+
 template <int C, int MODE>
 static int ws_wgrad_launch(const WsIn& in, int batch, const uint16_t* z, int64_t zbs, float* grad) {
     constexpr int TJ = WsShape<C>::TJ;
@@ -603,18 +647,18 @@ static int ws_wgrad_launch(const WsIn& in, int batch, const uint16_t* z, int64_t
     const int K = 9 * C * C;
     if ((int64_t)nblk * 4 * K > wn.part_cap) return pic_fail(PIC_ERR_STATE, "warm-start partial buffer too small");
     hipLaunchKernelGGL((k_ws_wgrad<C, MODE>), dim3(nblk), dim3(256), 0, g.stream, in, z, zbs, ntj, nti, batch, wn.d_part);
-    hipLaunchKernelGGL(k_ws_reduce, dim3((unsigned)((K + 63) / 64)), dim3(256), 0, g.stream, nblk * 4, K, wn.d_part, grad);
+    ws_reduce_launch(nblk * 4, K, grad);
     return PIC_OK;
 }
 
 template <int C, bool LIFT>
 static int ws_corr_launch(int batch, const float* s, int64_t sbs, const uint16_t* v, int64_t vbs, float* grad) {
     const int64_t tot = wn.n[0] * batch;
-    const int nblk = ws_blocks_for((tot + (256 / C) - 1) / (256 / C));
-    if ((int64_t)nblk * 9 * C > wn.part_cap) return pic_fail(PIC_ERR_STATE, "warm-start partial buffer too small");
-    hipLaunchKernelGGL((k_ws_corr<C, LIFT>), dim3(nblk), dim3(256), 0, g.stream, wn.H[0], wn.W[0], batch, s, sbs, v, vbs,
-                       wn.d_part);
-    hipLaunchKernelGGL(k_ws_reduce, dim3((unsigned)((9 * C + 63) / 64)), dim3(256), 0, g.stream, nblk, 9 * C, wn.d_part, grad);
+    const int64_t nblk = (tot + WS_CORR_NODES - 1) / WS_CORR_NODES;
+    if (nblk * 9 * C > wn.part_cap) return pic_fail(PIC_ERR_STATE, "warm-start partial buffer too small");
+    hipLaunchKernelGGL((k_ws_corr<C, LIFT>), dim3((unsigned)nblk), dim3(256), 0, g.stream, wn.H[0], wn.W[0], batch, s, sbs,
+                       v, vbs, wn.d_part);
+    ws_reduce_launch((int)nblk, 9 * C, grad);
     return PIC_OK;
 }
 
@@ -624,7 +668,7 @@ static int ws_backward_t(int batch, const float* r, int64_t rbs, const float* gy
     const int L = wn.L;
     const int64_t n0 = wn.n[0];
     int rc = ws_corr_launch<C, false>(batch, gy, gbs, wn.g[0], act_bs(0), wn.d_g + wn.off_out); if (rc) return rc;
-    hipLaunchKernelGGL(k_ws_lift<C>, dim3((unsigned)((n0 * (C / 8) + 255) / 256), batch), dim3(256), 0, g.stream,
+    hipLaunchKernelGGL(k_ws_lift<C>, dim3((unsigned)((n0 + 255) / 256), batch), dim3(256), 0, g.stream,
                        wn.H[0], wn.W[0], gy, gbs, wn.d_p + wn.off_out, 1, wn.mask[0], wn.gb[0], act_bs(0));
     // decoder, finest first: z_l = gb[l] (masked by its producer)
     for (int l = 0; l + 1 < L; ++l) {
@@ -677,9 +721,14 @@ static int ws_ensure_batch(int batch) {
         uint16_t** vs[6] = {&wn.h[l], &wn.g[l], &wn.gb[l], &wn.ub[l], &wn.hb[l], &wn.tb[l]};
         for (int k = 0; k < 6; ++k) { *vs[k] = (uint16_t*)p; p += al((size_t)act_bs(l) * sizeof(uint16_t) * batch); }
     }
-    // partial sums: the largest weight-gradient launch (512 blocks x 4 waves x 9 C C) or 1-channel one
-    wn.part_cap = std::max<int64_t>(512LL * 4 * 9 * wn.C * wn.C, 512LL * 9 * wn.C);
+    // partial sums: the largest weight-gradient launch (512 blocks x 4 waves x 9 C C) or 1-channel
+    // one (a block per WS_CORR_NODES nodes of the batch); chunk sums of 64 parts each in FP64
+    const int64_t corr_blocks = (wn.n[0] * batch + WS_CORR_NODES - 1) / WS_CORR_NODES;
+    wn.part_cap = std::max<int64_t>(512LL * 4 * 9 * wn.C * wn.C, corr_blocks * 9 * wn.C);
     HIPCHK(hipMalloc(&wn.d_part, wn.part_cap * sizeof(float)));
+    if (wn.d_part2) { (void)hipFree(wn.d_part2); wn.d_part2 = nullptr; }
+    const int64_t nch_max = std::max<int64_t>((512LL * 4 + 63) / 64, (corr_blocks + 63) / 64);
+    HIPCHK(hipMalloc(&wn.d_part2, nch_max * 9 * wn.C * wn.C * sizeof(double)));
     wn.bcap = batch;
     return PIC_OK;
 }
@@ -770,21 +819,46 @@ int warm_setup(int levels, int channels, uint64_t seed) {
 bool warm_active() { return wn.ready && wn.enabled && learn_ready(); }
 
 int warm_enable(int enable) {
+    if (enable < 0 || enable > 2) return pic_fail(PIC_ERR_ARG, "enable must be 0, 1 or 2");
     if (enable && !wn.ready) return pic_fail(PIC_ERR_STATE, "pic_warm_setup not called");
     wn.enabled = enable != 0;
+    wn.mode = enable;
     return PIC_OK;
 }
 
-// In the solve: xn += alpha D^-1/2 N(D^-1/2 rn), rn the FP64 residual of xn (no host wait)
-int warm_solve_start(int n, const double* rn, double* xn) {
-    if (!warm_active()) return PIC_OK;
-    if ((int64_t)n != wn.n[0]) return pic_fail(PIC_ERR_STATE, "warm-start network built for another mesh");
+static int ws_enqueue_solve_start(int n, const double* rn, double* xn) {
     const int nb = (n + 255) / 256;
     hipLaunchKernelGGL(k_ws_scale_in, dim3(nb), dim3(256), 0, g.stream, n, g.d_sinv, rn, wn.d_in);
     int rc = ws_forward(1, wn.d_in, n, wn.d_y, n); if (rc) return rc;
     hipLaunchKernelGGL(k_ws_dots, dim3(nb), dim3(256), 0, g.stream, n, learn_scaled_op(), g.d_sinv, rn, wn.d_y, wn.d_dots, nb);
     hipLaunchKernelGGL(k_ws_update, dim3(nb), dim3(256), 0, g.stream, n, nb, wn.d_dots, g.d_sinv, wn.d_y, xn, wn.d_alpha);
     HIPCHK(hipGetLastError());
+    return PIC_OK;
+}
+
+// In the solve: xn += alpha D^-1/2 N(D^-1/2 rn), rn the FP64 residual of xn (no host wait).
+// Mode 1 replays a captured graph of the same launches (no memset nodes: every buffer is
+// written by a kernel), mode 2 launches them one by one.
+int warm_solve_start(int n, const double* rn, double* xn) {
+    if (!warm_active()) return PIC_OK;
+    if ((int64_t)n != wn.n[0]) return pic_fail(PIC_ERR_STATE, "warm-start network built for another mesh");
+    if (wn.mode == 2) {
+        int rc = ws_enqueue_solve_start(n, rn, xn); if (rc) return rc;
+    } else {
+        if (!wn.graph_exec || wn.graph_rn != rn || wn.graph_xn != xn || wn.graph_act != wn.d_act || wn.graph_n != n) {
+            ws_drop_graph();
+            HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            int rc = ws_enqueue_solve_start(n, rn, xn);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(g.stream, &gr);
+            if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+            if (e != hipSuccess) return pic_fail(PIC_ERR_HIP, std::string("warm-start graph capture: ") + hipGetErrorString(e));
+            wn.graph = gr;
+            HIPCHK(hipGraphInstantiate(&wn.graph_exec, wn.graph, nullptr, nullptr, 0));
+            wn.graph_rn = rn; wn.graph_xn = xn; wn.graph_act = wn.d_act; wn.graph_n = n;
+        }
+        HIPCHK(hipGraphLaunch(wn.graph_exec, g.stream));
+    }
     wn.applied++;
     return PIC_OK;
 }
