@@ -36,6 +36,10 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 HBM_PEAK_GBS = 8000.0
 METRIC = "particle-pushes/s + PCG ms/solve @1e-8 rel residual, learned vs plain, 1 GPU"
+# the product's learned V-cycle: learned smoothers on the finest levels, minibatch share of
+# harvested pairs
+PRODUCT_LEARNED_LEVELS = 2
+PRODUCT_SHARE = 0.25
 
 
 def log(*a):
@@ -439,7 +443,16 @@ def cfg4_protocol(pkg, pl, args) -> dict:
     methods["ic0_pcg_fp64"], _, _ = solve_all(3, ref)
     log(f"cfg4: {methods}")
     variants = {}
-    for Ls in (2, 4):
+    levels_list = [int(v) for v in args.cfg4_levels.split(",") if v.strip()]
+    if 0 in levels_list:
+        # the classical V-cycle: damped Jacobi on every level (no learned smoother to train)
+        w = pkg.build_workload(pl, "cfg3", seed=2024)
+        pl.sort_particles()
+        pl.initial_field()
+        pl.learn_setup(12, (), seed=7)
+        variants["L0_jacobi_vcycle"], _, _ = solve_all(2, ref)
+        log(f"cfg4: Jacobi V-cycle {variants['L0_jacobi_vcycle']}")
+    for Ls in [v for v in levels_list if v > 0]:
         w = pkg.build_workload(pl, "cfg3", seed=2024)
         pl.sort_particles()
         pl.initial_field()
@@ -459,7 +472,7 @@ def cfg4_protocol(pkg, pl, args) -> dict:
         variants[f"L{Ls}_untrained"] = un
         log(f"cfg4: {Ls} learned levels, {ds} pairs from {steps} steps ({t_harvest:.1f} s); untrained {un}")
         neval = min(ds, 256)
-        for share in (0.5, 0.25, 0.0):
+        for share in (0.5, 0.25, 0.0) if Ls > 1 else (0.25, 0.0):
             pl.learn_set_params(p_untrained)
             pl.learn_data_mix(share)
             l_before = pl.learn_eval(neval)
@@ -477,7 +490,7 @@ def cfg4_protocol(pkg, pl, args) -> dict:
                        "ms_ratio_untrained_over_trained": un["ms_per_solve_median"] / tr["ms_per_solve_median"]})
             variants[f"L{Ls}_share{share}"] = tr
             log(f"cfg4: L{Ls} share {share}: {tr}")
-            if Ls == 2 and share == 0.25:
+            if Ls == PRODUCT_LEARNED_LEVELS and share == PRODUCT_SHARE:
                 p_product = pl.learn_params().copy()   # the product's V-cycle configuration
         variants[f"L{Ls}_fallbacks"] = pl.learn_info()["fallbacks"] - fb0
     # the learned warm start on top of the product's V-cycle (2 learned levels, share 1/4): trained
@@ -487,7 +500,7 @@ def cfg4_protocol(pkg, pl, args) -> dict:
     w = pkg.build_workload(pl, "cfg3", seed=2024)
     pl.sort_particles()
     pl.initial_field()
-    pl.learn_setup(12, (16, 16), seed=7)
+    pl.learn_setup(12, (16,) * PRODUCT_LEARNED_LEVELS, seed=7)
     cap = 4 * nw
     t0 = _t.perf_counter()
     for _ in range(nw):
@@ -521,8 +534,8 @@ def cfg4_protocol(pkg, pl, args) -> dict:
                "fields_more_iterations_than_without": int((it_w > it_base).sum()),
                "iteration_ratio_without_over_with": float(it_base.mean() / it_w.mean()),
                "ms_ratio_without_over_with": base["ms_per_solve_median"] / wr["ms_per_solve_median"]})
-    variants["L2_share0.25_rerun"] = base
-    variants["L2_share0.25_warm_start"] = wr
+    variants[f"L{PRODUCT_LEARNED_LEVELS}_share{PRODUCT_SHARE}_rerun"] = base
+    variants[f"L{PRODUCT_LEARNED_LEVELS}_share{PRODUCT_SHARE}_warm_start"] = wr
     log(f"cfg4: warm start: {wr}")
     best = min((k for k in variants if "share" in k), key=lambda k: variants[k]["ms_per_solve_mean"])
     return {"workload": "512x512 three-conductor mesh, stand-alone Poisson, cold start",
@@ -541,6 +554,10 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--train-steps", type=int, default=1000)
     ap.add_argument("--solve-reps", type=int, default=5)
+    ap.add_argument("--learn-levels", type=int, default=PRODUCT_LEARNED_LEVELS,
+                    help="learned smoother levels of the V-cycle (0..4)")
+    ap.add_argument("--learn-share", type=float, default=PRODUCT_SHARE,
+                    help="share of harvested pairs in a training minibatch")
     ap.add_argument("--no-warm-net", action="store_true", help="do not train the learned warm start")
     ap.add_argument("--warm-start", action="store_true",
                     help="apply the learned warm start inside the timed steps (default: compared on one RHS only)")
@@ -553,6 +570,7 @@ def main() -> int:
     ap.add_argument("--cfg4-pairs", type=int, default=20000, help="cfg4: harvested training pairs")
     ap.add_argument("--cfg4-max-steps", type=int, default=2000, help="cfg4: at most this many harvest steps")
     ap.add_argument("--cfg4-train", type=int, default=4000, help="cfg4: training steps")
+    ap.add_argument("--cfg4-levels", default="0,1,2,4", help="cfg4: learned smoother levels of the variants (0: Jacobi V-cycle)")
     ap.add_argument("--cfg4-warm-pairs", type=int, default=400, help="cfg4: plain cold solves harvested for the warm start")
     ap.add_argument("--cfg4-warm-train", type=int, default=2000, help="cfg4: warm-start training steps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -599,7 +617,7 @@ def main() -> int:
     log(f"bench: set-up {time.perf_counter() - t0:.1f} s; warmup {args.warmup} steps (plain solver)")
     if learned:
         # the network is trained on pairs the plain solver harvests during the untimed warm-up
-        pl.learn_setup(12, (16, 16), seed=7)
+        pl.learn_setup(12, (16,) * args.learn_levels, seed=7)
         pl.learn_harvest(True, 512, 1)
     n_learned_warm = min(2, max(args.warmup - 1, 0)) if learned else 0
     pl.step(args.warmup - n_learned_warm)
@@ -609,7 +627,9 @@ def main() -> int:
         info = pl.learn_info()
         log(f"bench: {info['dataset']} harvested pairs; training {args.train_steps} steps on the GPU")
         t0 = time.perf_counter()
-        l0, l1 = pl.learn_train(args.train_steps, batch=8, lr=5e-3, seed=11)
+        pl.learn_data_mix(args.learn_share)
+        l0, l1 = (pl.learn_train(args.train_steps, batch=8, lr=5e-3, seed=11) if args.learn_levels > 0
+                  else (0.0, 0.0))
         t_train = time.perf_counter() - t0
         warm = None
         if not args.no_warm_net:

@@ -1100,18 +1100,23 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_finish(int n, int nb, const d
 // w = A' z; partials (z, r), (z, w), (z, q_prev) in regions 1, 2, 3
 __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_aw(int n, Dia8 A, const float* z, const float* r, const float* q,
                                                       float* w, const double* scal, double* part, int pstride) {
+    // every global load first (the pass-done flag with them), so the block waits for memory once
+    const int i = blockIdx.x * FCG_BLOCK + threadIdx.x;
+    const int ii = i < n ? i : 0;
+    const float zi = z[ii], ri = r[ii], qi = q[ii];
+    float cz[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cz[k] = A.c[k][ii] * z[ii + A.off[k]];
     if (scal[F_S_DONE] != 0.0) return;
-    int i = blockIdx.x * FCG_BLOCK + threadIdx.x;
     double v[3] = {0.0, 0.0, 0.0};
     if (i < n) {
-        float zi = z[i];
         float wi = zi;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) wi = wi + A.c[k][i] * z[i + A.off[k]];
+        for (int k = 0; k < 8; ++k) wi = wi + cz[k];
         w[i] = wi;
-        v[0] = (double)zi * (double)r[i];
+        v[0] = (double)zi * (double)ri;
         v[1] = (double)zi * (double)wi;
-        v[2] = (double)zi * (double)q[i];
+        v[2] = (double)zi * (double)qi;
     }
     fcg_block_partials(3, v, part + (size_t)pstride, pstride);
 }
@@ -1123,6 +1128,10 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, const f
                                                           float* q, float* x, float* r, double* scal, double* part,
                                                           int pstride, int kk) {
     __shared__ double s_done;
+    // the vectors are loaded before the partials are reduced, so both memory round trips overlap
+    const int i0 = blockIdx.x * FCG_BLOCK + threadIdx.x;
+    const int ic = i0 < n ? i0 : 0;
+    const float z0 = z[ic], w0 = w[ic], p0 = p[ic], q0 = q[ic], x0 = x[ic], r0 = r[ic];
     if (threadIdx.x == 0) s_done = scal[F_S_DONE];
     double d[3];
     fcg_reduce(3, part + (size_t)pstride, pstride, nb, d);     // contains a barrier after s_done
@@ -1145,13 +1154,13 @@ __global__ __launch_bounds__(FCG_BLOCK) void k_fcg_update(int n, int nb, const f
         return;
     }
     const float fb = (float)beta, fa = (float)alpha;
-    int i = blockIdx.x * FCG_BLOCK + threadIdx.x;
+    const int i = i0;
     double v[1] = {0.0};
     if (i < n) {
-        float pi = (k == 0.0) ? z[i] : z[i] + fb * p[i];
-        float qi = (k == 0.0) ? w[i] : w[i] + fb * q[i];
-        float xi = x[i] + fa * pi;
-        float ri = r[i] - fa * qi;
+        float pi = (k == 0.0) ? z0 : z0 + fb * p0;
+        float qi = (k == 0.0) ? w0 : w0 + fb * q0;
+        float xi = x0 + fa * pi;
+        float ri = r0 - fa * qi;
         p[i] = pi; q[i] = qi; x[i] = xi; r[i] = ri;
         v[0] = (double)ri * (double)ri;
     }

@@ -320,10 +320,9 @@ __global__ __launch_bounds__(256) void k_ws_lift(int H, int W, const float* __re
                                                  uint16_t* __restrict__ out, int64_t o_bs) {
     constexpr int G = C / 8;
     const int b = blockIdx.y;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t n = (int64_t)H * W;
-    if (p >= n) return;
-    const int i = (int)(p / W), j = (int)(p - (int64_t)(p / W) * W);
+    const int p = blockIdx.x * 256 + threadIdx.x;           // node index (a level has < 2^31 nodes)
+    if (p >= H * W) return;
+    const int i = p / W, j = p - (p / W) * W;
     const float* sb = s + (int64_t)b * s_bs;
     float sv[9];
 #pragma unroll
@@ -345,7 +344,7 @@ __global__ __launch_bounds__(256) void k_ws_lift(int H, int W, const float* __re
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c) f[c] = f[c] * m;
-        *(ws_u32x4*)(out + (int64_t)b * o_bs + p * C + 8 * gi) = ws_pack8(f);
+        *(ws_u32x4*)(out + (int64_t)b * o_bs + (int64_t)p * C + 8 * gi) = ws_pack8(f);
     }
 }
 
@@ -356,9 +355,9 @@ __global__ __launch_bounds__(256) void k_ws_out(int H, int W, const uint16_t* __
                                                 float* __restrict__ y, int64_t y_bs) {
     constexpr int G = C / 8;
     const int b = blockIdx.y;
-    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (p >= (int64_t)H * W) return;
-    const int i = (int)(p / W), j = (int)(p - (int64_t)(p / W) * W);
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= H * W) return;
+    const int i = p / W, j = p - (p / W) * W;
     const uint16_t* vb = v + (int64_t)b * v_bs;
     float acc = 0.0f;
 #pragma unroll
@@ -378,50 +377,51 @@ __global__ __launch_bounds__(256) void k_ws_out(int H, int W, const uint16_t* __
 
 // Weight gradients of the 1-channel convolutions, summed over the batch:
 //   LIFT: dW[tap][c] = sum s(p + d_tap) v(p)[c]     OUT: dW[tap][c] = sum v(p + d_tap)[c] s(p)
-// A block owns WS_CORR_NODES consecutive nodes of the batch; threads: c = tid % C, node lane
-// tid / C; per block 9 C partial sums (fixed order).
-#define WS_CORR_NODES 1024
+// A block owns WS_CORR_ROWS rows of one batch entry (blockIdx.y); threads: c = tid % C, node
+// lane tid / C stepping along the row; per block 9 C partial sums (fixed order).
+#define WS_CORR_ROWS 2
 template <int C, bool LIFT>
-__global__ __launch_bounds__(256) void k_ws_corr(int H, int W, int batch, const float* __restrict__ s, int64_t s_bs,
+__global__ __launch_bounds__(256) void k_ws_corr(int H, int W, const float* __restrict__ s, int64_t s_bs,
                                                  const uint16_t* __restrict__ v, int64_t v_bs, float* __restrict__ part) {
     __shared__ float red[256 * 9];
     constexpr int NL = 256 / C;
     const int c = threadIdx.x % C, nl = threadIdx.x / C;
-    const int64_t n = (int64_t)H * W, tot = n * batch;
-    const int64_t P1 = min(tot, (int64_t)(blockIdx.x + 1) * WS_CORR_NODES);
+    const int b = blockIdx.y;
+    const int i0 = blockIdx.x * WS_CORR_ROWS, i1 = min(H, i0 + WS_CORR_ROWS);
+    const float* sb = s + (int64_t)b * s_bs;
+    const uint16_t* vb = v + (int64_t)b * v_bs;
     float acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = 0.0f;
-    for (int64_t P = (int64_t)blockIdx.x * WS_CORR_NODES + nl; P < P1; P += NL) {
-        const int b = (int)(P / n);
-        const int64_t p = P - (int64_t)b * n;
-        const int i = (int)(p / W), j = (int)(p - (int64_t)(p / W) * W);
-        const float* sb = s + (int64_t)b * s_bs;
-        const uint16_t* vb = v + (int64_t)b * v_bs;
-        if (LIFT) {
-            const float vv = ws_bf2f(vb[p * C + c]);
+    for (int i = i0; i < i1; ++i) {
+        for (int j = nl; j < W; j += NL) {
+            const int p = i * W + j;
+            if (LIFT) {
+                const float vv = ws_bf2f(vb[(int64_t)p * C + c]);
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int ii = i + t / 3 - 1, jj = j + t % 3 - 1;
-                if (ii >= 0 && ii < H && jj >= 0 && jj < W) acc[t] = acc[t] + sb[(int64_t)ii * W + jj] * vv;
-            }
-        } else {
-            const float sv = sb[p];
+                for (int t = 0; t < 9; ++t) {
+                    const int ii = i + t / 3 - 1, jj = j + t % 3 - 1;
+                    if (ii >= 0 && ii < H && jj >= 0 && jj < W) acc[t] = acc[t] + sb[ii * W + jj] * vv;
+                }
+            } else {
+                const float sv = sb[p];
 #pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                const int ii = i + t / 3 - 1, jj = j + t % 3 - 1;
-                if (ii >= 0 && ii < H && jj >= 0 && jj < W) acc[t] = acc[t] + ws_bf2f(vb[((int64_t)ii * W + jj) * C + c]) * sv;
+                for (int t = 0; t < 9; ++t) {
+                    const int ii = i + t / 3 - 1, jj = j + t % 3 - 1;
+                    if (ii >= 0 && ii < H && jj >= 0 && jj < W) acc[t] = acc[t] + ws_bf2f(vb[(int64_t)(ii * W + jj) * C + c]) * sv;
+                }
             }
         }
     }
 #pragma unroll
     for (int t = 0; t < 9; ++t) red[threadIdx.x * 9 + t] = acc[t];
     __syncthreads();
+    const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
     for (int k = threadIdx.x; k < 9 * C; k += 256) {
         const int t = k / C, cc = k - t * C;
         float sum = 0.0f;
         for (int l = 0; l < NL; ++l) sum += red[(l * C + cc) * 9 + t];
-        part[(int64_t)blockIdx.x * 9 * C + t * C + cc] = sum;
+        part[blk * 9 * C + t * C + cc] = sum;
     }
 }
 
@@ -432,18 +432,17 @@ __global__ __launch_bounds__(256) void k_ws_transfer(WsIn in, const float* __res
                                                      int64_t o_bs) {
     constexpr int G = C / 8;
     const int b = blockIdx.y;
-    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t n = (int64_t)in.H * in.W;
-    if (k >= n * G) return;
-    const int64_t p = k / G;
-    const int gi = (int)(k - p * G);
-    const int i = (int)(p / in.W), j = (int)(p - (int64_t)(p / in.W) * in.W);
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= in.H * in.W * G) return;
+    const int p = k / G;
+    const int gi = k - p * G;
+    const int i = p / in.W, j = p - (p / in.W) * in.W;
     float f[8];
     ws_input8<C, MODE>(in, b, i, j, gi, f);
     const float m = mask ? mask[p] : 1.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) f[c] = f[c] * m;
-    *(ws_u32x4*)(out + (int64_t)b * o_bs + p * C + 8 * gi) = ws_pack8(f);
+    *(ws_u32x4*)(out + (int64_t)b * o_bs + (int64_t)p * C + 8 * gi) = ws_pack8(f);
 }
 
 // Packs the FP32 master weights of every C -> C convolution into the MFMA A-operand order,
@@ -651,12 +650,14 @@ static int ws_wgrad_launch(const WsIn& in, int batch, const uint16_t* z, int64_t
     return PIC_OK;
 }
 
+static int64_t ws_corr_blocks(int batch) { return (int64_t)((wn.H[0] + WS_CORR_ROWS - 1) / WS_CORR_ROWS) * batch; }
+
 template <int C, bool LIFT>
 static int ws_corr_launch(int batch, const float* s, int64_t sbs, const uint16_t* v, int64_t vbs, float* grad) {
-    const int64_t tot = wn.n[0] * batch;
-    const int64_t nblk = (tot + WS_CORR_NODES - 1) / WS_CORR_NODES;
+    const int nbx = (wn.H[0] + WS_CORR_ROWS - 1) / WS_CORR_ROWS;
+    const int64_t nblk = ws_corr_blocks(batch);
     if (nblk * 9 * C > wn.part_cap) return pic_fail(PIC_ERR_STATE, "warm-start partial buffer too small");
-    hipLaunchKernelGGL((k_ws_corr<C, LIFT>), dim3((unsigned)nblk), dim3(256), 0, g.stream, wn.H[0], wn.W[0], batch, s, sbs,
+    hipLaunchKernelGGL((k_ws_corr<C, LIFT>), dim3((unsigned)nbx, batch), dim3(256), 0, g.stream, wn.H[0], wn.W[0], s, sbs,
                        v, vbs, wn.d_part);
     ws_reduce_launch((int)nblk, 9 * C, grad);
     return PIC_OK;
@@ -722,8 +723,8 @@ static int ws_ensure_batch(int batch) {
         for (int k = 0; k < 6; ++k) { *vs[k] = (uint16_t*)p; p += al((size_t)act_bs(l) * sizeof(uint16_t) * batch); }
     }
     // partial sums: the largest weight-gradient launch (512 blocks x 4 waves x 9 C C) or 1-channel
-    // one (a block per WS_CORR_NODES nodes of the batch); chunk sums of 64 parts each in FP64
-    const int64_t corr_blocks = (wn.n[0] * batch + WS_CORR_NODES - 1) / WS_CORR_NODES;
+    // one (a block per WS_CORR_ROWS rows of a batch entry); chunk sums of 64 parts each in FP64
+    const int64_t corr_blocks = ws_corr_blocks(batch);
     wn.part_cap = std::max<int64_t>(512LL * 4 * 9 * wn.C * wn.C, corr_blocks * 9 * wn.C);
     HIPCHK(hipMalloc(&wn.d_part, wn.part_cap * sizeof(float)));
     if (wn.d_part2) { (void)hipFree(wn.d_part2); wn.d_part2 = nullptr; }
