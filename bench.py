@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-pushes/s + PCG ms/solve @1e-8 rel residual, learned vs plain, 1 GPU"
 # the product's learned V-cycle: learned smoothers on the finest levels, minibatch share of
 # harvested pairs
-PRODUCT_LEARNED_LEVELS = 2
+PRODUCT_LEARNED_LEVELS = 1
 PRODUCT_SHARE = 0.25
 
 
@@ -686,7 +686,9 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: " + WORKLOAD_TEXT.get(args.workload, "see pic-_amd workload()"),
                    "mesh": [dom.nnx - 1, dom.nny - 1], "particles_per_species": int(main_run["n0"] // 2),
-                   "solver": "learned multigrid FCG (plain CG fallback)" if learned else "plain one-kernel CG",
+                   "solver": (f"learned multigrid FCG ({args.learn_levels} learned smoother level"
+                              f"{'s' if args.learn_levels != 1 else ''}, plain CG fallback)") if learned
+                             else "plain one-kernel CG",
                    "parallelism": "single GPU"},
         "pcg_ms_per_solve": ph[3] / args.steps, "pcg_iterations": main_run["solve"]["iterations"],
         "pcg_outer": main_run["solve"]["outer"], "pcg_relres": main_run["solve"]["relres"],

@@ -274,7 +274,7 @@ def _product_run(pl, pkg, name, steps, train_steps):
     dom = w["domain"]
     pl.set_step(0)
     pl.initial_field()
-    pl.learn_setup(12, (16, 16), 1)
+    pl.learn_setup(12, (16,), 1)          # the product's V-cycle: one learned level
     pl.learn_harvest(True, 256, 1)
     pl.step(4)                               # plain solver: harvest pairs
     pl.learn_train(train_steps, 4, 1e-3, 1)
@@ -337,7 +337,7 @@ def test_online_finetune_inside_step(gpu_lib, pkg):
     w = pkg.build_workload(pl, "cfg3s128", seed=2024)
     pl.set_step(0)
     pl.initial_field()
-    pl.learn_setup(12, (16, 16), 1)
+    pl.learn_setup(12, (16,), 1)          # the product's V-cycle: one learned level
     pl.learn_harvest(True, 256, 1)
     pl.step(2)
     pl.learn_train(60, 4, 1e-3, 1)
