@@ -163,6 +163,15 @@ __device__ inline void fold_y(R& Y, R& VY, R ymin, R ymax) {
 // power-of-two spacing the division is the (bit-identical) multiplication by 1/h.
 // The node lines are xmin + hx i, ymin + hy j (the set-up's formula, same roundings).
 __device__ inline void deposit_weights_ij(double X, double Y, int ci, int cj, const MeshParams& m, double W[4]) {
+    if (m.unif_r64 && m.delta == 0) {
+        // every element's spacing is hx (hy) exactly: the same arithmetic as below without the
+        // far node lines
+        const double x1 = m.xmin + m.hx * (double)ci, y1 = m.ymin + m.hy * (double)cj;
+        const double dx = (X - x1) * m.hxi, dy = (Y - y1) * m.hyi;
+        const double xm = 1.0 - dx, ym = 1.0 - dy;
+        W[0] = xm * ym; W[1] = dx * ym; W[2] = dx * dy; W[3] = xm * dy;
+        return;
+    }
     const double x1 = m.xmin + m.hx * (double)ci, x2 = m.xmin + m.hx * (double)(ci + 1);
     const double y1 = m.ymin + m.hy * (double)cj, y2 = m.ymin + m.hy * (double)(cj + 1);
     double hxp = x2 - x1, hyp = y2 - y1;
@@ -427,12 +436,21 @@ __device__ inline bool push_hot(const PushArgs<R>& a, int e_old, const HotIn<R>&
     int ci, cj;
     pic_elem_ij(e_old, m, &ci, &cj);
     const unsigned int nbr = h.nbr;                           // flags of the 3x3 neighbourhood
-    const R x1 = node_x(a, ci), x2 = node_x(a, ci + 1), y1 = node_y(a, cj), y2 = node_y(a, cj + 1);
+    const R x1 = node_x(a, ci), y1 = node_y(a, cj);
     const R e1x = h.ex[0], e2x = h.ex[1], e3x = h.ex[2], e4x = h.ex[3];
     const R e1y = h.ey[0], e2y = h.ey[1], e3y = h.ey[2], e4y = h.ey[3];
-    R hxp = x2 - x1, hyp = y2 - y1;
-    R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
-    R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
+    R dx, dy;
+    if (sizeof(R) == 4 ? m.unif_r32 : m.unif_r64) {
+        // every element's spacing is hxR (hyR) exactly: the multiplication by the reciprocal
+        // below, without the far node lines
+        dx = (X - x1) * a.hxiR;
+        dy = (Y - y1) * a.hyiR;
+    } else {
+        const R x2 = node_x(a, ci + 1), y2 = node_y(a, cj + 1);
+        R hxp = x2 - x1, hyp = y2 - y1;
+        dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
+        dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
+    }
     R xm = (R)1 - dx, ym = (R)1 - dy;
     R W1 = xm * ym, W2 = dx * ym, W3 = dx * dy, W4 = xm * dy;
     R Ex = W1 * e1x + W2 * e2x + W3 * e3x + W4 * e4x;
@@ -469,13 +487,16 @@ __device__ inline bool push_hot_rz(const PushArgs<R>& a, int e_old, const HotIn<
     int ci, cj;
     pic_elem_ij(e_old, m, &ci, &cj);
     const unsigned int nbr = h.nbr;
-    const R x1 = node_x(a, ci), x2 = node_x(a, ci + 1), y1 = node_y(a, cj), y2 = node_y(a, cj + 1);
+    const R x1 = node_x(a, ci), y1 = node_y(a, cj), y2 = node_y(a, cj + 1);
     const R e1x = h.ex[0], e2x = h.ex[1], e3x = h.ex[2], e4x = h.ex[3];
     const R e1y = h.ey[0], e2y = h.ey[1], e3y = h.ey[2], e4y = h.ey[3];
-    R hxp = x2 - x1, hyp = y2 - y1;
-    R dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
-    R dy = (m.pow2_map && hyp == a.hyR) ? (Y - y1) * a.hyiR : (Y - y1) / hyp;
-    (void)dy;
+    R dx;
+    if (sizeof(R) == 4 ? m.unif_r32 : m.unif_r64) {
+        dx = (X - x1) * a.hxiR;
+    } else {
+        const R hxp = node_x(a, ci + 1) - x1;
+        dx = (m.pow2_map && hxp == a.hxR) ? (X - x1) * a.hxiR : (X - x1) / hxp;
+    }
     R xm = (R)1 - dx;
     const R R1 = (R)m.ymin + y1, R2 = (R)m.ymin + y2, Rp = Y;
     const R den = R2 * R2 - R1 * R1;

@@ -63,6 +63,9 @@ struct MeshParams {
                                         // equal to the spacing, xmin / hx and ymin / hy integers)
     int nodes_f32;                      // 1 if (float)xs[i] == (float)xmin + (float)hx * (float)i for
                                         // every node line (and likewise y): FP32 lines recomputed
+    int unif_r32, unif_r64;             // 1 if pow2_map and every element's spacing, from the node
+                                        // lines the push recomputes in FP32 / FP64, equals hx (hy)
+                                        // exactly: the lookup weights skip the far node line
     float ney_rcp;                      // 1 / ney rounded to FP32 (element -> column, pic_elem_ij)
     int ij_exact_div;                   // 1 when nelem >= 2^24: pic_elem_ij divides
     const double* xs; const double* ys; // node coordinate lines

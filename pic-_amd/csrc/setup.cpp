@@ -262,6 +262,22 @@ int setup_build_domain(const pic_domain_config* c) {
         for (int i = 0; i < nnx; ++i) if ((float)g.h_xs[i] != x0 + hxf * (float)i) m.nodes_f32 = 0;
         for (int j = 0; j < nny; ++j) if ((float)g.h_ys[j] != y0 + hyf * (float)j) m.nodes_f32 = 0;
     }
+    // the spacing of every element from the node lines as the push recomputes them (FP32:
+    // x0 + hx i in FP32 when nodes_f32, else the FP64 line rounded; FP64: xmin + hx i)
+    m.unif_r32 = m.pow2_map; m.unif_r64 = m.pow2_map;
+    {
+        const float x0 = (float)c->xmin, y0 = (float)c->ymin, hxf = (float)hx, hyf = (float)hy;
+        auto nx32 = [&](int i) { return m.nodes_f32 ? x0 + hxf * (float)i : (float)(c->xmin + hx * (double)i); };
+        auto ny32 = [&](int j) { return m.nodes_f32 ? y0 + hyf * (float)j : (float)(c->ymin + hy * (double)j); };
+        for (int i = 0; i + 1 < nnx; ++i) {
+            if (nx32(i + 1) - nx32(i) != hxf) m.unif_r32 = 0;
+            if ((c->xmin + hx * (double)(i + 1)) - (c->xmin + hx * (double)i) != hx) m.unif_r64 = 0;
+        }
+        for (int j = 0; j + 1 < nny; ++j) {
+            if (ny32(j + 1) - ny32(j) != hyf) m.unif_r32 = 0;
+            if ((c->ymin + hy * (double)(j + 1)) - (c->ymin + hy * (double)j) != hy) m.unif_r64 = 0;
+        }
+    }
     m.ney_rcp = 1.0f / (float)ney;
     m.ij_exact_div = ((int64_t)nelem >= ((int64_t)1 << 24)) ? 1 : 0;
     m.n_objects = c->n_objects;
