@@ -325,6 +325,12 @@ int  pic_initial_field(void);
 /* Cell sort / compaction period in steps (0 = never) and the RNG seed of the run. */
 int  pic_set_run_params(int32_t sort_every, uint64_t seed, int32_t mcc_enabled);
 int  pic_sort_particles(void);
+/* Slots per group of the cell sort's layout for the current mesh and precision (256, or 512
+ * for FP32 particles on a Cartesian mesh with power-of-two spacing and exact FP32 node lines):
+ * inside a full group of 64 K slots the sorted position K l + k sits at slot 64 k + l. No
+ * reference counterpart (the reference keeps particles unsorted); for callers that read the
+ * slot order. */
+int  pic_sort_group(int32_t* slots);
 /* Current step counter (the RNG counter word) and its setter (restart). The counter
  * advances after MCC (pic_MCC, or inside pic_step), and also when a species is moved a
  * second time at the same counter value (a caller that runs no MCC). */

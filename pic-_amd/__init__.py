@@ -133,6 +133,7 @@ _PROTOS = {
     "pic_initial_field": (C.c_int, []),
     "pic_set_run_params": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32]),
     "pic_sort_particles": (C.c_int, []),
+    "pic_sort_group": (C.c_int, [_ip]),
     "pic_get_step": (C.c_int, [_i64p]),
     "pic_set_step": (C.c_int, [C.c_int64]),
     "pic_get_field": (C.c_int, [C.c_int, _dp, C.c_int64]),
@@ -538,6 +539,12 @@ class PicLibrary:
 
     def sort_particles(self):
         check(lib().pic_sort_particles())
+
+    def sort_group(self) -> int:
+        """Slots per group of the sort's transposed layout (256 or 512)."""
+        n = C.c_int32()
+        check(lib().pic_sort_group(C.byref(n)))
+        return n.value
 
     def initial_field(self):
         check(lib().pic_initial_field())

@@ -932,6 +932,13 @@ int pic_sort_particles(void) {
     return run_sort_all();
 }
 
+int pic_sort_group(int32_t* slots) {
+    REQUIRE_DOMAIN();
+    if (!slots) return pic_fail(PIC_ERR_ARG, "null slots");
+    *slots = 64 * pic_push_ppt();
+    return PIC_OK;
+}
+
 int pic_get_step(int64_t* step) { *step = g.step; return PIC_OK; }
 int pic_set_step(int64_t step) {
     if (step < 0) return pic_fail(PIC_ERR_ARG, "step < 0");

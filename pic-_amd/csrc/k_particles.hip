@@ -4,8 +4,8 @@
 // normalisation) and the batched drop-ins (point lookup, PPR point gather).
 //
 // Particles are structure-of-arrays slots kept in tile-major element order (periodic
-// cell sort). A wave owns a group of PIC_SORT_GROUP slots and a thread PUSH_PPT of them,
-// 64 slots apart, so every access is lane-contiguous; the sort places consecutive sorted
+// cell sort). A wave owns a group of 64 PPT slots and a thread PPT of them (4, or 8 for FP32
+// particles on a Cartesian mesh of the common layout), 64 slots apart, so every access is lane-contiguous; the sort places consecutive sorted
 // particles on a thread's slots. Its
 // deposits are summed in registers while consecutive particles share an element; the
 // per-thread runs are merged by one segmented wave reduction keyed by element, and run
@@ -16,11 +16,9 @@
 #include "pic_trig.h"
 
 #define PUSH_BLOCK 256
-#define PUSH_PPT (PIC_SORT_GROUP / 64)
-#define PUSH_SLOTS (PUSH_BLOCK * PUSH_PPT)
 // LDS deposition window of a push block: element columns ci - M .. ci + 1 + M around the
 // element of the block's middle particle, and the rows of its sort tile +- M, with M =
-// PUSH_WIN_MARGIN cells of drift since the last sort. A block's PUSH_SLOTS sorted particles
+// PUSH_WIN_MARGIN cells of drift since the last sort. A block's PUSH_BLOCK x PPT sorted particles
 // cover one or two element columns of one tile; deposits outside the window go to global
 // atomics.
 #ifndef PUSH_WIN_MARGIN
@@ -600,18 +598,31 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 // (96 VGPRs; left alone the allocation lands at 98, i.e. 4 waves, and the fifth wave is worth
 // more than the two registers spilled around the slot loop); the axisymmetric and FP64 passes
 // and 8 slots per thread keep 4 (at 5 they spill 9 to 24 registers for no gain).
-#define PUSH_WAVES(R, RZ) ((sizeof(R) == 4 && !(RZ) && PUSH_PPT == 4) ? 5 : 4)
-template <typename R, int DO_PUSH, int RZ>
-__global__ __launch_bounds__(PUSH_BLOCK) __attribute__((amdgpu_waves_per_eu(PUSH_WAVES(R, RZ), 8)))
-void k_push_deposit(PushArgs<R> a) {
+#define PUSH_WAVES(R, RZ, PPT) ((sizeof(R) == 4 && !(RZ) && (PPT) == 4) ? 5 : 4)
+// FAST: the mesh flags of the common layout (power-of-two spacing and map, FP32 lookup and
+// node lines exact, every element's spacing exact, fewer than 2^24 elements) are compile-time
+// constants, so the general branches and the scalars only they read drop out of the kernel
+// (the host launches FAST only where set-up found all of these true; same arithmetic).
+// PPT: slots per thread, matching the cell sort's groups of 64 PPT slots (pic_push_ppt). With
+// the general branches gone the FP32 Cartesian kernel holds 8 slots at 4 waves per SIMD without
+// spills, which halves the per-block costs (window zeroing and flush, the centre element,
+// the first loads' latency) per particle.
+template <typename R, int DO_PUSH, int RZ, int FAST, int PPT>
+__global__ __launch_bounds__(PUSH_BLOCK) __attribute__((amdgpu_waves_per_eu(PUSH_WAVES(R, RZ, PPT), 8)))
+void k_push_deposit(PushArgs<R> a_in) {
     __shared__ long long winW[WIN_NODES], winE[WIN_NODES];
     __shared__ int s_center;
+    PushArgs<R> a = a_in;
+    if (FAST) {
+        a.m.pow2_map = 1; a.m.locate_f32 = 1; a.m.nodes_f32 = 1; a.m.unif_r32 = 1; a.m.unif_r64 = 1;
+        a.m.ij_exact_div = 0;
+    }
     const MeshParams& m = a.m;
-    const int64_t base = (int64_t)blockIdx.x * PUSH_SLOTS;
+    const int64_t base = (int64_t)blockIdx.x * (PUSH_BLOCK * PPT);
     // window around the element of the first live particle at or after the block's middle
     // slot (WIN_NX x WIN_NY nodes)
     if (threadIdx.x < 64) {
-        int64_t mid = base + PUSH_SLOTS / 2;
+        int64_t mid = base + (PUSH_BLOCK * PPT) / 2;
         if (mid >= a.n) mid = base;
         int64_t q = mid + threadIdx.x;
         int c = (q < a.n) ? a.cell[q] : -1;
@@ -638,15 +649,15 @@ void k_push_deposit(PushArgs<R> a) {
     for (int k = threadIdx.x; k < wn; k += PUSH_BLOCK) { winW[k] = 0; winE[k] = 0; }
 
     // ---------------- move, locate, deposit (run accumulation in registers)
-    // Wave w of the block owns the PIC_SORT_GROUP-slot group at wbase; iteration k handles
+    // Wave w of the block owns the 64 PPT-slot group at wbase; iteration k handles
     // slot wbase + 64 k + lane, so every load and store is lane-contiguous. The cell sort lays
-    // a full group out transposed (sorted position PUSH_PPT lane + k at slot 64 k + lane), so
+    // a full group out transposed (sorted position PPT lane + k at slot 64 k + lane), so
     // a thread's iterations walk consecutive sorted particles, which mostly share an element.
     // Slot arrays are addressed from the block's first slot with 32-bit offsets.
     const int lane = threadIdx.x & 63;
-    const unsigned int o0 = (threadIdx.x >> 6) * PIC_SORT_GROUP + lane;
+    const unsigned int o0 = (threadIdx.x >> 6) * (64 * PPT) + lane;
     const int64_t nrem = a.n - base;
-    const unsigned int nb = nrem < (int64_t)PUSH_SLOTS ? (unsigned int)nrem : (unsigned int)PUSH_SLOTS;
+    const unsigned int nb = nrem < (int64_t)(PUSH_BLOCK * PPT) ? (unsigned int)nrem : (unsigned int)(PUSH_BLOCK * PPT);
     const int* cellb = a.cell + base;
     int* nextb = a.cell_next + base;
     R* xb = a.x + base; R* yb = a.y + base; R* vxb = a.vx + base; R* vyb = a.vy + base;
@@ -656,15 +667,15 @@ void k_push_deposit(PushArgs<R> a) {
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
-    // Software pipeline over the PUSH_PPT slots: the cells of every slot are loaded up front
+    // Software pipeline over the PPT slots: the cells of every slot are loaded up front
     // (one wait), then the state and the element gathers of slot k + 1 are issued before slot
     // k is computed, so each slot's loads are in flight during the previous slot's arithmetic
     // and a slot waits only for its own loads. No branch surrounds a load: a slot past the end
     // reads the block's first slot (nb >= 1) and is marked dead, a dead slot's gathers read
     // element 0's; the loop is unrolled so the per-slot registers are indexed statically.
-    int cel[PUSH_PPT];
+    int cel[PPT];
 #pragma unroll
-    for (int k = 0; k < PUSH_PPT; ++k) {
+    for (int k = 0; k < PPT; ++k) {
         const unsigned int o = o0 + 64 * k;
         const int c = pic_ld(cellb, o < nb ? o : 0u);
         cel[k] = o < nb ? c : -1;
@@ -686,13 +697,13 @@ void k_push_deposit(PushArgs<R> a) {
     unsigned int defer_bits = 0;     // slot k handed to the wall kernel: bit k
     __syncthreads();     // window zeroed before any deposit
 #pragma unroll
-    for (int k = 0; k < PUSH_PPT; ++k) {
+    for (int k = 0; k < PPT; ++k) {
         const unsigned int o = o0 + 64 * k;
         const bool in = o < nb;
         const int e_old = cel[k];
         R X = nx, Y = ny, VX = nvx, VY = nvy, VZ = nvz, TH = nth;
         const HotIn<R> hin = nh;
-        if (k + 1 < PUSH_PPT) issue(k + 1);
+        if (k + 1 < PPT) issue(k + 1);
         int next = -1, nci = 0, ncj = 0;
         bool dep = false;
         if (DO_PUSH) {
@@ -743,14 +754,14 @@ void k_push_deposit(PushArgs<R> a) {
     // each thread's last run straight to the window (LDS atomics; lanes of one element add
     // to the same eight words, which the LDS serialises)
     if (rkey >= 0) add_element(rkey, acc, m, a.acc_w, a.acc_e, w, winW, winE);
-    // the handed-over slots of all PUSH_PPT iterations appended to the wall list with one
+    // the handed-over slots of all PPT iterations appended to the wall list with one
     // wave-aggregated atomic (its returned base is the only wait; the wall kernel treats the
     // list as an unordered set)
     if (DO_PUSH) {
-        unsigned long long wm[PUSH_PPT];
+        unsigned long long wm[PPT];
         unsigned int tot = 0;
 #pragma unroll
-        for (int k = 0; k < PUSH_PPT; ++k) {
+        for (int k = 0; k < PPT; ++k) {
             wm[k] = __ballot((defer_bits >> k) & 1u);
             tot += (unsigned int)__popcll(wm[k]);
         }
@@ -760,7 +771,7 @@ void k_push_deposit(PushArgs<R> a) {
             wb = __shfl(wb, 0, 64);
             const unsigned long long below = (1ull << lane) - 1ull;
 #pragma unroll
-            for (int k = 0; k < PUSH_PPT; ++k) {
+            for (int k = 0; k < PPT; ++k) {
                 if ((defer_bits >> k) & 1u) a.wall_list[wb + __popcll(wm[k] & below)] = (int)(base + o0 + 64 * k);
                 wb += (unsigned int)__popcll(wm[k]);
             }
@@ -836,11 +847,25 @@ int launch_zero_acc(int isp) {
     return PIC_OK;
 }
 
+// The mesh flags of the common layout hold (the FAST kernels' compile-time constants).
+static bool push_fast_layout() {
+    const MeshParams& mm = g.mesh;
+    return mm.pow2_map && !mm.ij_exact_div && mm.unif_r64 &&
+           (g.particle_bits == 64 || (mm.locate_f32 && mm.nodes_f32 && mm.unif_r32));
+}
+
+int pic_push_ppt() {
+    return (g.particle_bits == 32 && g.mesh.delta == 0 && push_fast_layout()) ? PIC_PPT_WIDE : PIC_PPT_NARROW;
+}
+
 int launch_push_deposit(int isp, bool do_push) {
     SpeciesHost& S = g.sp[isp];
     if (S.acc_dirty) { int rc = launch_zero_acc(isp); if (rc) return rc; }
     if (S.nslots == 0) { S.acc_dirty = true; return PIC_OK; }
-    int blocks = (int)((S.nslots + PUSH_SLOTS - 1) / PUSH_SLOTS);
+    const bool rz = g.mesh.delta == 1;
+    const bool fast = push_fast_layout() && !rz;
+    const int ppt = do_push ? pic_push_ppt() : PIC_PPT_NARROW;
+    const int blocks = (int)((S.nslots + PUSH_BLOCK * ppt - 1) / (PUSH_BLOCK * ppt));
     if (do_push) {
         // the wall list holds at most every slot once
         if (g.wall_cap < S.nslots) {
@@ -854,24 +879,33 @@ int launch_push_deposit(int isp, bool do_push) {
         HIPCHK(hipEventRecord(g.ev_push[isp][0], g.stream));
     }
     const int wall_blocks = 512;
-    const bool rz = g.mesh.delta == 1;
     if (do_push && rz && !S.th) return pic_fail(PIC_ERR_STATE, "axisymmetric species without azimuth storage");
     // the fused first pass, then the full MoveOne / AdjustParticleBoundary loop for the
     // particles it handed over (axisymmetric: the r-z pass of MoveOne)
-#define PIC_PUSH_LAUNCH(R_)                                                                                    \
-    {                                                                                                          \
-        PushArgs<R_> a = make_push_args<R_>(isp);                                                              \
-        if (do_push) {                                                                                         \
-            if (rz) hipLaunchKernelGGL((k_push_deposit<R_, 1, 1>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a); \
-            else hipLaunchKernelGGL((k_push_deposit<R_, 1, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);   \
-            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));                                               \
-            hipLaunchKernelGGL((k_push_walls<R_>), dim3(wall_blocks), dim3(256), 0, g.stream, a);              \
-        } else {                                                                                               \
-            hipLaunchKernelGGL((k_push_deposit<R_, 0, 0>), dim3(blocks), dim3(PUSH_BLOCK), 0, g.stream, a);   \
-        }                                                                                                      \
+    const dim3 grid(blocks), block(PUSH_BLOCK);
+    if (g.particle_bits == 64) {
+        PushArgs<double> a = make_push_args<double>(isp);
+        if (do_push) {
+            if (rz) hipLaunchKernelGGL((k_push_deposit<double, 1, 1, 0, 4>), grid, block, 0, g.stream, a);
+            else if (fast) hipLaunchKernelGGL((k_push_deposit<double, 1, 0, 1, 4>), grid, block, 0, g.stream, a);
+            else hipLaunchKernelGGL((k_push_deposit<double, 1, 0, 0, 4>), grid, block, 0, g.stream, a);
+            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
+            hipLaunchKernelGGL((k_push_walls<double>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL((k_push_deposit<double, 0, 0, 0, 4>), grid, block, 0, g.stream, a);
+        }
+    } else {
+        PushArgs<float> a = make_push_args<float>(isp);
+        if (do_push) {
+            if (rz) hipLaunchKernelGGL((k_push_deposit<float, 1, 1, 0, 4>), grid, block, 0, g.stream, a);
+            else if (fast) hipLaunchKernelGGL((k_push_deposit<float, 1, 0, 1, 8>), grid, block, 0, g.stream, a);
+            else hipLaunchKernelGGL((k_push_deposit<float, 1, 0, 0, 4>), grid, block, 0, g.stream, a);
+            HIPCHK(hipEventRecord(g.ev_push[isp][1], g.stream));
+            hipLaunchKernelGGL((k_push_walls<float>), dim3(wall_blocks), dim3(256), 0, g.stream, a);
+        } else {
+            hipLaunchKernelGGL((k_push_deposit<float, 0, 0, 0, 4>), grid, block, 0, g.stream, a);
+        }
     }
-    if (g.particle_bits == 64) PIC_PUSH_LAUNCH(double) else PIC_PUSH_LAUNCH(float)
-#undef PIC_PUSH_LAUNCH
     HIPCHK(hipGetLastError());
     if (do_push) { HIPCHK(hipEventRecord(g.ev_push[isp][2], g.stream)); g.ev_push_pending[isp] = true; }
     S.acc_dirty = true;

@@ -99,18 +99,19 @@ PIC_HD void pic_st(T* base, unsigned int i, T v) {
     *(T*)((char*)base + (size_t)(i * (unsigned int)sizeof(T))) = v;
 }
 
-// Slot layout after a cell sort: within every full group of PIC_SORT_GROUP = 64 K slots the
-// sorted position K l + k (l = 0..63, k = 0..K-1) sits at slot 64 k + l, so lane l of a wave
-// that walks the group 64 slots at a time meets K consecutive sorted particles (a partial
-// last group keeps the sorted order). K = 4 keeps the push kernel at 5 waves per SIMD.
-#ifndef PIC_SORT_GROUP
-#define PIC_SORT_GROUP 256
-#endif
-PIC_HD int64_t pic_sorted_position(int64_t slot, int64_t nfull) {
+// Slot layout after a cell sort: within every full group of 64 K slots the sorted position
+// K l + k (l = 0..63, k = 0..K-1) sits at slot 64 k + l, so lane l of a wave that walks the
+// group 64 slots at a time meets K consecutive sorted particles (a partial last group keeps
+// the sorted order). K is the push kernel's slots per thread (pic_push_ppt): 8 for FP32
+// particles on a Cartesian mesh of the common layout, else 4. Any order is valid input for
+// the push; the layout only sets how often a thread's consecutive slots share an element.
+#define PIC_PPT_NARROW 4
+#define PIC_PPT_WIDE 8
+PIC_HD int64_t pic_sorted_position(int64_t slot, int64_t nfull, int ppt) {
     if (slot >= nfull) return slot;
-    const int64_t g = slot & ~(int64_t)(PIC_SORT_GROUP - 1);
+    const int64_t g = slot & ~(int64_t)(64 * ppt - 1);
     const int r = (int)(slot - g);
-    return g + (int64_t)((r & 63) * (PIC_SORT_GROUP / 64) + (r >> 6));
+    return g + (int64_t)((r & 63) * ppt + (r >> 6));
 }
 
 // Sort tiles: SORT_TILE x SORT_TILE elements, tiles visited column by column with the

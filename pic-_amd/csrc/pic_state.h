@@ -305,6 +305,9 @@ void setup_maxwellian(int isp, int ppc, double temp_eV, uint64_t seed,
 
 // device launchers
 int launch_push_deposit(int isp, bool do_push);
+// slots per thread of the moving push kernel for the current mesh and precision (4 or 8); the
+// cell sort lays its groups out for it
+int pic_push_ppt();
 int launch_finalize_deposit();
 int launch_rhs();
 int launch_ppr();

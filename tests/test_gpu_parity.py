@@ -745,9 +745,9 @@ def _sort_keys(loc0, nex, ney, tile=16):
     return (ti * nty + tpos) * tile * tile + (ci % tile) * tile + cj % tile
 
 
-def _transposed(n, group=256):
-    """Slot d holds the particle at sorted position p(d): inside a full group of 256 slots the
-    sorted position 4 l + k sits at slot 64 k + l; a partial last group keeps the order."""
+def _transposed(n, group):
+    """Slot d holds the particle at sorted position p(d): inside a full group of 64 K slots the
+    sorted position K l + k sits at slot 64 k + l; a partial last group keeps the order."""
     d = np.arange(n)
     nfull = n // group * group
     g0 = d & ~(group - 1)
@@ -773,12 +773,15 @@ def test_cell_sort_is_the_stable_tile_order(gpu_lib, pkg, case):
     ths = [pl.azimuth(s) if dom.delta == 1 else None for s in (0, 1)]
     c0 = pl.counts(0)
     assert c0["nslots"] > c0["nlive"]                # dead slots to drop
+    # 8 slots per thread for the FP32 Cartesian case (h = 1 from the origin), else 4
+    group = pl.sort_group()
+    assert group == (512 if (w["bits"] == 32 and dom.delta == 0) else 256)
     pl.sort_particles()
     for s in (0, 1):
         pre, th = pres[s], ths[s]
         keys = _sort_keys(pre["loc"] - 1, dom.nnx - 1, dom.nny - 1)
         order = np.argsort(keys, kind="stable")
-        pos = order[_transposed(len(order))]
+        pos = order[_transposed(len(order), group)]
         post = pl.particles(s)
         assert pl.counts(s)["nslots"] == len(pre["x"]) == pl.counts(s)["nlive"]
         for k in ("x", "y", "vx", "vy", "vz", "loc"):
