@@ -258,7 +258,7 @@ def push_bytes(delta: int) -> float:
     return 56.0 if delta == 1 else 44.0
 
 
-def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0) -> tuple:
+def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0, push_kernel: str = "") -> tuple:
     """Rooflines of the push kernel and of the plain CG iteration kernel (the two single
     kernels that can dominate a step). Algorithmic bytes per launch:
       push: 44 B per particle (read x, y, vx, vy, vz, cell; write x, y, vx, vy, cell_next, FP32);
@@ -271,7 +271,7 @@ def rooflines(run: dict, steps: int, nnodes: int, traffic: dict, delta: int = 0)
         b = push_bytes(delta) * (0.5 * (run["n0"] + run["n1"])) / 2
         out.append((ks["push_ms"], {"bound": "hbm", "achieved": b / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "traffic": traffic.get("k_push_deposit"),
-                                    "kernel": "k_push_deposit<float,1,%d>" % delta,
+                                    "kernel": push_kernel or "k_push_deposit<float,1,%d>" % delta,
                                     "bytes_per_launch": b, "ms_per_launch": ms,
                                     "launches": ks["push_launches"], "share_of_step": ks["push_ms"] / phase[7]}))
     if ks["cg_launches"]:
@@ -676,8 +676,15 @@ def main() -> int:
         pl.step(1)
         log(f"bench: timing {args.steps} steps (plain solver)")
         plain_run = timed_steps(pl, args.steps)
-    roofs = rooflines(main_run, args.steps, nnodes, traffic, dom.delta)
-    roofs_plain = rooflines(plain_run, args.steps, nnodes, traffic, dom.delta)
+    # the moving push instantiation <R, DO_PUSH, RZ, FAST, slots per thread>: FP32 Cartesian
+    # meshes of the common layout run the specialised kernel with 8 slots per thread (the sort
+    # group is then 512 slots)
+    if dom.delta == 1:
+        push_kernel = "k_push_deposit<float,1,1,0,4>"
+    else:
+        push_kernel = "k_push_deposit<float,1,0,1,8>" if pl.sort_group() == 512 else "k_push_deposit<float,1,0,0,4>"
+    roofs = rooflines(main_run, args.steps, nnodes, traffic, dom.delta, push_kernel)
+    roofs_plain = rooflines(plain_run, args.steps, nnodes, traffic, dom.delta, push_kernel)
     other = [r for r in roofs_plain if r["kernel"] != roofs[0]["kernel"]]
     ph = main_run["phase"]
     out = {

@@ -636,8 +636,10 @@ __global__ void k_sort_keys(int64_t n, const int* cell, unsigned int* keys, unsi
 }
 // Gather of every particle array in one pass: slot d takes the particle at sorted position
 // pic_sorted_position(d) (the group-transposed layout the push kernel walks, ppt slots per thread).
+// A block is one group of 64 ppt slots: its waves read the same source lines (sorted positions
+// ppt apart per lane), so they share one CU's caches instead of two XCDs' L2s.
 template <typename R>
-__global__ __launch_bounds__(256) void k_sort_gather(int64_t n, int64_t nfull, int ppt, const unsigned int* perm,
+__global__ __launch_bounds__(64 * PIC_PPT_WIDE) void k_sort_gather(int64_t n, int64_t nfull, int ppt, const unsigned int* perm,
                                                      const R* x, const R* y, const R* vx, const R* vy, const R* vz,
                                                      const int* cell, const R* th, R* ox, R* oy, R* ovx, R* ovy,
                                                      R* ovz, int* ocell, R* oth) {
@@ -710,16 +712,17 @@ static int sort_species(int isp) {
         if (!S.spare[a]) HIPCHK(hipMalloc(&S.spare[a], (size_t)S.capacity * esz));
     const int ppt = pic_push_ppt();
     const int64_t nfull = live / (64 * ppt) * (64 * ppt);
-    const int gblocks = (int)((live + 255) / 256);
+    const int gthreads = 64 * ppt;
+    const int gblocks = (int)((live + gthreads - 1) / gthreads);
     if (live > 0) {
         if (esz == 8)
-            hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, ppt, perm,
+            hipLaunchKernelGGL(k_sort_gather<double>, dim3(gblocks), dim3(gthreads), 0, g.stream, live, nfull, ppt, perm,
                                (const double*)S.x, (const double*)S.y, (const double*)S.vx, (const double*)S.vy,
                                (const double*)S.vz, S.cell, rz ? (const double*)S.th : nullptr,
                                (double*)S.spare[0], (double*)S.spare[1], (double*)S.spare[2], (double*)S.spare[3],
                                (double*)S.spare[4], S.cell_next, rz ? (double*)S.spare[5] : nullptr);
         else
-            hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(256), 0, g.stream, live, nfull, ppt, perm,
+            hipLaunchKernelGGL(k_sort_gather<float>, dim3(gblocks), dim3(gthreads), 0, g.stream, live, nfull, ppt, perm,
                                (const float*)S.x, (const float*)S.y, (const float*)S.vx, (const float*)S.vy,
                                (const float*)S.vz, S.cell, rz ? (const float*)S.th : nullptr,
                                (float*)S.spare[0], (float*)S.spare[1], (float*)S.spare[2], (float*)S.spare[3],
