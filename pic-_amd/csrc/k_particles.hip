@@ -604,7 +604,7 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 // constants, so the general branches and the scalars only they read drop out of the kernel
 // (the host launches FAST only where set-up found all of these true; same arithmetic).
 // PPT: slots per thread, matching the cell sort's groups of 64 PPT slots (pic_push_ppt). With
-// the general branches gone the FP32 Cartesian kernel holds 8 slots at 4 waves per SIMD without
+// the general branches gone the FP32 Cartesian kernel holds 8 slots at 5 waves per SIMD without
 // spills, which halves the per-block costs (window zeroing and flush, the centre element,
 // the first loads' latency) per particle.
 template <typename R, int DO_PUSH, int RZ, int FAST, int PPT>
@@ -667,19 +667,22 @@ void k_push_deposit(PushArgs<R> a_in) {
     long long acc[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[q] = 0;
-    // Software pipeline over the PPT slots: the cells of every slot are loaded up front
-    // (one wait), then the state and the element gathers of slot k + 1 are issued before slot
-    // k is computed, so each slot's loads are in flight during the previous slot's arithmetic
-    // and a slot waits only for its own loads. No branch surrounds a load: a slot past the end
-    // reads the block's first slot (nb >= 1) and is marked dead, a dead slot's gathers read
-    // element 0's; the loop is unrolled so the per-slot registers are indexed statically.
+    // Software pipeline over the PPT slots: the cells of the first two slots are loaded up
+    // front (one wait); while slot k is computed, the cell of slot k + 2 and then the state and
+    // the element gathers of slot k + 1 are in flight, so a slot waits only for its own loads
+    // (the cell load comes first, so waiting for it never waits for the state loads issued
+    // after it) and at most three cells are live in registers (8 slots per thread then fit in
+    // 93 registers, 5 waves per SIMD). No branch surrounds a load: a slot past the end reads
+    // the block's first slot (nb >= 1) and is marked dead, a dead slot's gathers read element
+    // 0's; the loop is unrolled so the per-slot registers are indexed statically.
     int cel[PPT];
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) {
+    auto load_cell = [&](int k) {
         const unsigned int o = o0 + 64 * k;
         const int c = pic_ld(cellb, o < nb ? o : 0u);
         cel[k] = o < nb ? c : -1;
-    }
+    };
+    load_cell(0);
+    if (PPT > 1) load_cell(1);
     // vmcnt(0) on every path, so the compiler tracks no cell load as outstanding past here
     __builtin_amdgcn_s_waitcnt(0x0F70);
     R nx, ny, nvx, nvy, nvz, nth;    // state of the slot issued next
@@ -703,6 +706,7 @@ void k_push_deposit(PushArgs<R> a_in) {
         const int e_old = cel[k];
         R X = nx, Y = ny, VX = nvx, VY = nvy, VZ = nvz, TH = nth;
         const HotIn<R> hin = nh;
+        if (k + 2 < PPT) load_cell(k + 2);
         if (k + 1 < PPT) issue(k + 1);
         int next = -1, nci = 0, ncj = 0;
         bool dep = false;
