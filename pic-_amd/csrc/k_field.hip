@@ -619,11 +619,12 @@ static void resid64_launch(int n, const Op& A, const double* b, const double* x,
 }
 
 // Iterative refinement with the learned FCG as the inner solver. A batch of passes (as many
-// as the last learned solve needed, each with the FCG iterations its counterpart needed) is
-// launched with no host synchronisation in between: every pass tests the FP64 residual
-// before it on the device and does nothing once rr <= tol^2 bb (or after a failed pass).
-// One synchronisation per batch reads the records; a pass that ran out of iterations gets
-// two more next time, and further batches of one pass follow until the tolerance is met.
+// as the last learned solve needed, each with the FCG iterations its counterpart needed, the
+// last one with a spare while recent solves needed it) is launched with no host
+// synchronisation in between: every pass tests the FP64 residual before it on the device and
+// does nothing once rr <= tol^2 bb (or after a failed pass). One synchronisation per batch
+// reads the records; a pass that ran out of iterations gets two more next time, and further
+// batches of one pass follow until the tolerance is met.
 // *fallback: the network failed (NaN, breakdown) or stagnated; the caller finishes with CG.
 static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, int* outer, double* rr, double* bb,
                                   bool* fallback) {
@@ -646,7 +647,7 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         int K[FCG_MAX_PASS];
         for (int p = 0; p < batch; ++p) {
             const int idx = std::min(done_passes + p, FCG_MAX_PASS - 1);
-            K[p] = std::min(cap, std::max(1, g.lp_iters[idx]) + (p == batch - 1 ? 1 : 0));
+            K[p] = std::min(cap, std::max(1, g.lp_iters[idx]) + (p == batch - 1 ? g.lp_spare : 0));
             int rc = learn_fcg_pass(n, g.d_rn, inner, K[p], tol2, g.d_scal + S_NSCAL, first && p == 0 ? 1 : 0, p,
                                     g.d_xn);
             if (rc) return rc;
@@ -661,11 +662,13 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         const double* h = g.h_scal_pinned;
         *rr = h[S_NSCAL]; *bb = h[S_NSCAL + 1];
         int its = 0;
+        bool ran_out = false, spare_used = false;
         for (int p = 0; p < batch; ++p) {
             const double* rec = h + FCG_REC_BASE + FCG_RECW * p;
             if (rec[4] != 0.0) break;                       // skipped: converged before (or failed earlier)
             const int it = (int)rec[0];
             const int idx = std::min(done_passes, FCG_MAX_PASS - 1);
+            if (p == batch - 1 && g.lp_spare && it >= K[p]) spare_used = true;
             its += it; *total += it; (*outer)++; done_passes++;
             g.last_learned_iters += it;
             if (learn_trace_on())
@@ -676,6 +679,7 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
                 // out of iterations: stagnation if the residual norm fell less than 10x
                 if (K[p] >= 4 && !(rec[3] < 1e-2)) { *fallback = true; break; }
                 g.lp_iters[idx] = K[p] + 2;
+                ran_out = true;
             } else {
                 g.lp_iters[idx] = std::max(1, it);
             }
@@ -686,6 +690,12 @@ static int run_learned_refinement(int n, const DiaOp<double>& A64, int* total, i
         if (learn_trace_on())
             fprintf(stderr, "learned batch: %d passes, host enqueue %.1f us, device %.1f us\n", batch, host_us, 1e3 * ms);
         learn_note_iterations(its);
+        if (ran_out) {
+            g.lp_spare = 1; g.lp_calm = 0;
+        } else if (g.lp_spare) {
+            if (spare_used) g.lp_calm = 0;
+            else if (++g.lp_calm >= 8) { g.lp_spare = 0; g.lp_calm = 0; }
+        }
         if (*fallback) return PIC_OK;
         if (*bb == 0.0 || *rr <= tol2 * *bb) { g.lp_passes = std::max(1, done_passes); return PIC_OK; }
         if (*outer >= 30 || *total >= g.pcg_maxit) {

@@ -1439,6 +1439,7 @@ int learn_setup(int max_levels, const int32_t* channels, int nch, uint64_t seed)
     free_learn();
     g.lp_passes = 2;                                 // launch budget of the first learned solve
     for (int k = 0; k < FCG_MAX_PASS; ++k) g.lp_iters[k] = 8;
+    g.lp_spare = 1; g.lp_calm = 0;
     // level sizes: coarse node I <-> fine node 2I, down to about 3 x 3
     int H[LN_MAX_LEVELS], W[LN_MAX_LEVELS];
     H[0] = g.mesh.nnx; W[0] = g.mesh.nny;

@@ -237,6 +237,11 @@ struct PicState {
     // launch budget of the next one (all its passes go out before one host synchronisation)
     int lp_passes = 2;
     int lp_iters[6] = {8, 8, 8, 8, 8, 8};
+    // one spare FCG iteration in the batch's last pass, so that a solve needing one iteration
+    // more than the last one finishes without another batch; dropped after 8 batches in a row
+    // that did not use it (in a steady time loop it only costs launches that exit at once),
+    // restored when a pass runs out of iterations
+    int lp_spare = 1, lp_calm = 0;
     int last_learned_iters = 0, last_fallback = 0;
     // kernel statistics of the last pic_step: [0] ms in CG iteration kernels (chunk events),
     // [1] their launches, [2] ms in push kernels (events around each launch), [3] launches,
