@@ -594,10 +594,12 @@ __global__ __launch_bounds__(256) void k_push_walls(PushArgs<R> a) {
 }
 
 // Fused mover + lookup + deposition (DO_PUSH) or lookup + deposition only.
-// Register budget per SIMD: the Cartesian FP32 pass at 4 slots per thread is held to 5 waves
-// (96 VGPRs; left alone the allocation lands at 98, i.e. 4 waves, and the fifth wave is worth
-// more than the two registers spilled around the slot loop); the axisymmetric and FP64 passes
-// and 8 slots per thread keep 4 (at 5 they spill 9 to 24 registers for no gain).
+// Register budget per SIMD: the general Cartesian FP32 pass at 4 slots per thread is held to 5
+// waves (96 VGPRs; left alone the allocation lands at 98, i.e. 4 waves, and the fifth wave is
+// worth more than the two registers spilled around the slot loop); the axisymmetric and FP64
+// passes keep 4 (at 5 they spill 9 to 24 registers for no gain). The 8-slot specialisation is
+// asked for 4 and allocates 93 VGPRs, so it runs 5 waves without spills (asked for 5 with all
+// eight cells held up front, it spilled 36 B per lane).
 #define PUSH_WAVES(R, RZ, PPT) ((sizeof(R) == 4 && !(RZ) && (PPT) == 4) ? 5 : 4)
 // FAST: the mesh flags of the common layout (power-of-two spacing and map, FP32 lookup and
 // node lines exact, every element's spacing exact, fewer than 2^24 elements) are compile-time
