@@ -732,3 +732,16 @@ def test_axisymmetric_outer_radius_absorbs(orc):
     X, Y, VX, VY, VZ, alive, cnt, err, TH = out
     assert err == 0 and alive.tolist() == [0, 0, 1]
     assert cnt[7] == 2 and cnt[1] == 0 and cnt[3] == 0
+
+
+def test_sort_group_needs_a_domain(pkg):
+    """pic_sort_group before pic_init / pic_setup_domain is a call-order error (host-side
+    check only; run in a fresh process so no earlier test's library state is visible)."""
+    import subprocess
+    import sys
+    code = ("import ctypes; lib = ctypes.CDLL(%r); n = ctypes.c_int32(-1); "
+            "rc = lib.pic_sort_group(ctypes.byref(n)); print(rc, n.value)" % str(pkg.LIB_PATH))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rc, n = map(int, out.stdout.split())
+    assert rc == 2 and n == -1        # PIC_ERR_STATE, output untouched
